@@ -1,0 +1,275 @@
+"""CPU parity oracle for the SciWasm.Sparse Level-II hot path.
+
+TEST INFRASTRUCTURE ONLY: this package may be imported by ``tests/``, by
+``__graft_entry__.smoke()`` and by ``bench.py``'s ``cpu_baseline`` leg, and
+there only as the checker (or as the timed CPU baseline).  The product
+package ``sciwasm.sparse_amd`` never imports it.
+
+It wraps ``liboracle.so`` (built from ``sable_oracle.c`` by ``make -C
+oracle``), a C restatement of the reference routines in their own operation
+order; each C function cites the reference file:line it follows.  The
+wrappers below take and return numpy arrays.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_F32P = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+
+
+def build():
+    """Compile liboracle.so in place (gcc, a few seconds)."""
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        for suf, FP in (("f32", _F32P), ("f64", _F64P)):
+            getattr(L, "oracle_coo_csr_" + suf).argtypes = [_I, _I, _I32P, _I32P, FP, _I32P, _I32P, FP]
+            getattr(L, "oracle_csr_ell_" + suf).argtypes = [_I, _I, _I32P, _I32P, FP, _I32P, FP]
+            getattr(L, "oracle_csr_dia_" + suf).argtypes = [_I, _I, _I, _I32P, _I32P, FP, _I32P, FP]
+            getattr(L, "oracle_count_transpose_cols_ell_" + suf).argtypes = [_I, _I, FP, _I32P]
+            getattr(L, "oracle_count_transpose_cols_ell_" + suf).restype = _I
+            getattr(L, "oracle_transpose_ell_" + suf).argtypes = [_I, _I, _I, FP, _I32P, FP, _I32P]
+            getattr(L, "oracle_level_reorder_" + suf).argtypes = [_I, _I32P, _I32P, FP, _I32P, _I32P, _I32P, _I32P, FP]
+            getattr(L, "oracle_spmv_coo_" + suf).argtypes = [_I, _I32P, _I32P, FP, FP, FP, _I]
+            getattr(L, "oracle_spmv_csr_" + suf).argtypes = [_I, _I32P, _I32P, FP, FP, FP, _I]
+            getattr(L, "oracle_spmv_dia_" + suf).argtypes = [_I, _I, _I, _I32P, FP, FP, FP, _I]
+            getattr(L, "oracle_spmv_ell_" + suf).argtypes = [_I, _I, _I32P, FP, FP, FP, _I]
+            getattr(L, "oracle_spts_level_" + suf).argtypes = [_I, _I32P, _I32P, FP, FP, FP]
+            getattr(L, "oracle_spts_syncfree_" + suf).argtypes = [_I, _I32P, _I32P, FP, FP, FP]
+            getattr(L, "oracle_fletcher_" + suf).argtypes = [_I, FP]
+            getattr(L, "oracle_fletcher_" + suf).restype = ctypes.c_double
+        L.oracle_num_cols.argtypes = [_I, _I32P]
+        L.oracle_num_cols.restype = _I
+        L.oracle_num_diags.argtypes = [_I, _I32P, _I32P, ctypes.POINTER(_I)]
+        L.oracle_num_diags.restype = _I
+        L.oracle_transpose_csr.argtypes = [_I, _I, _I, _I32P, _I32P, _VP, _I32P, _I32P, _VP]
+        L.oracle_transpose_dia.argtypes = [_I, _I, _I, _I, _I32P, _VP, _I32P, _VP]
+        L.oracle_level_sets.argtypes = [_I, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P]
+        L.oracle_level_sets.restype = _I
+        L.oracle_spmv_csr_ld.argtypes = [_I, _I32P, _I32P, _F64P, _F64P, _VP, _F64P, _VP]
+        L.oracle_spts_ld.argtypes = [_I, _I32P, _I32P, _F64P, _F64P, _F64P]
+        L.oracle_rel_residual.argtypes = [_I, _I32P, _I32P, _F64P, _F64P, _F64P]
+        L.oracle_rel_residual.restype = ctypes.c_double
+        L.oracle_time_spmv_csr_f64.argtypes = [_I, _I32P, _I32P, _F64P, _F64P, _F64P, _I, _I]
+        L.oracle_time_spmv_csr_f64.restype = ctypes.c_double
+        _LIB = L
+    return _LIB
+
+
+def _suf(a):
+    if a.dtype == np.float32:
+        return "f32"
+    if a.dtype == np.float64:
+        return "f64"
+    raise TypeError("values must be float32 or float64, got %s" % a.dtype)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ---------------------------------------------------------------- conversions
+def coo_csr(N, row, col, val):
+    """seq/src/sparse32.js:1674-1720 -> (rowptr, col, val)."""
+    row, col, val = _i32(row), _i32(col), np.ascontiguousarray(val)
+    nnz = len(row)
+    rp = np.zeros(N + 1, np.int32)
+    oc = np.zeros(nnz, np.int32)
+    ov = np.zeros(nnz, val.dtype)
+    getattr(lib(), "oracle_coo_csr_" + _suf(val))(N, nnz, row, col, val, rp, oc, ov)
+    return rp, oc, ov
+
+
+def num_cols(N, rowptr):
+    return lib().oracle_num_cols(N, _i32(rowptr))
+
+
+def num_diags(N, rowptr, col):
+    """Returns (nd, min_abs_offset)."""
+    mn = _I(0)
+    nd = lib().oracle_num_diags(N, _i32(rowptr), _i32(col), ctypes.byref(mn))
+    return nd, mn.value
+
+
+def csr_ell(N, rowptr, col, val, nc=None):
+    rowptr, col, val = _i32(rowptr), _i32(col), np.ascontiguousarray(val)
+    if nc is None:
+        nc = num_cols(N, rowptr)
+    idx = np.zeros(nc * N, np.int32)
+    data = np.zeros(nc * N, val.dtype)
+    getattr(lib(), "oracle_csr_ell_" + _suf(val))(N, nc, rowptr, col, val, idx, data)
+    return nc, idx, data
+
+
+def csr_dia(N, rowptr, col, val, stride=None):
+    """Returns (nd, stride, offset, data); stride defaults to N."""
+    rowptr, col, val = _i32(rowptr), _i32(col), np.ascontiguousarray(val)
+    nd, _ = num_diags(N, rowptr, col)
+    if stride is None:
+        stride = N
+    off = np.zeros(max(nd, 1), np.int32)
+    data = np.zeros(max(nd * stride, 1), val.dtype)
+    getattr(lib(), "oracle_csr_dia_" + _suf(val))(N, nd, stride, rowptr, col, val, off, data)
+    return nd, stride, off[:nd], data[:nd * stride]
+
+
+def transpose_csr(N, rowptr, col, val):
+    rowptr, col, val = _i32(rowptr), _i32(col), np.ascontiguousarray(val)
+    nnz = len(col)
+    orp = np.zeros(N + 1, np.int32)
+    oc = np.zeros(max(nnz, 1), np.int32)
+    ov = np.zeros(max(nnz, 1), val.dtype)
+    lib().oracle_transpose_csr(N, nnz, val.itemsize, rowptr, col, val.ctypes.data,
+                               orp, oc, ov.ctypes.data)
+    return orp, oc[:nnz], ov[:nnz]
+
+
+def transpose_dia(N, stride, offset, data):
+    offset, data = _i32(offset), np.ascontiguousarray(data)
+    nd = len(offset)
+    oo = np.zeros(max(nd, 1), np.int32)
+    od = np.zeros(max(nd * stride, 1), data.dtype)
+    lib().oracle_transpose_dia(N, nd, stride, data.itemsize, offset, data.ctypes.data,
+                               oo, od.ctypes.data)
+    return oo[:nd], od[:nd * stride]
+
+
+def transpose_ell(N, nc, indices, data):
+    """Returns (nc_out, indices_out, data_out)."""
+    indices, data = _i32(indices), np.ascontiguousarray(data)
+    s = _suf(data)
+    nco = getattr(lib(), "oracle_count_transpose_cols_ell_" + s)(N, nc, data, indices)
+    oi = np.zeros(max(nco * N, 1), np.int32)
+    od = np.zeros(max(nco * N, 1), data.dtype)
+    getattr(lib(), "oracle_transpose_ell_" + s)(N, nc, nco, data, indices, od, oi)
+    return nco, oi[:nco * N], od[:nco * N]
+
+
+# ------------------------------------------------------------------ level sets
+def level_sets(N, rowptr, col):
+    """par/src/sparse32.js:2942-3181 -> (nlevels, level_ptr, permutation, order, level_of_row)."""
+    rowptr, col = _i32(rowptr), _i32(col)
+    lvl = np.zeros(max(N, 1), np.int32)
+    lp = np.zeros(N + 2, np.int32)
+    perm = np.zeros(max(N, 1), np.int32)
+    order = np.zeros(max(N, 1), np.int32)
+    nl = lib().oracle_level_sets(N, rowptr, col, lvl, lp, perm, order)
+    return nl, lp[:nl + 1].copy(), perm[:N], order[:N], lvl[:N]
+
+
+def level_reorder(N, rowptr, col, val, permutation, order):
+    rowptr, col, val = _i32(rowptr), _i32(col), np.ascontiguousarray(val)
+    nrp = np.zeros(N + 1, np.int32)
+    nc = np.zeros(max(len(col), 1), np.int32)
+    nv = np.zeros(max(len(col), 1), val.dtype)
+    getattr(lib(), "oracle_level_reorder_" + _suf(val))(N, rowptr, col, val, _i32(permutation),
+                                                        _i32(order), nrp, nc, nv)
+    return nrp, nc[:len(col)], nv[:len(col)]
+
+
+# ------------------------------------------------------------------------ spmv
+def spmv_coo(row, col, val, x, y, inner_max=1):
+    y = np.array(y, dtype=val.dtype, copy=True)
+    getattr(lib(), "oracle_spmv_coo_" + _suf(val))(len(row), _i32(row), _i32(col),
+                                                  np.ascontiguousarray(val),
+                                                  np.ascontiguousarray(x, val.dtype), y, inner_max)
+    return y
+
+
+def spmv_csr(N, rowptr, col, val, x, y, inner_max=1):
+    y = np.array(y, dtype=val.dtype, copy=True)
+    getattr(lib(), "oracle_spmv_csr_" + _suf(val))(N, _i32(rowptr), _i32(col),
+                                                  np.ascontiguousarray(val),
+                                                  np.ascontiguousarray(x, val.dtype), y, inner_max)
+    return y
+
+
+def spmv_dia(N, stride, offset, data, x, y, inner_max=1):
+    y = np.array(y, dtype=data.dtype, copy=True)
+    getattr(lib(), "oracle_spmv_dia_" + _suf(data))(N, len(offset), stride, _i32(offset),
+                                                   np.ascontiguousarray(data),
+                                                   np.ascontiguousarray(x, data.dtype), y, inner_max)
+    return y
+
+
+def spmv_ell(N, nc, indices, data, x, y, inner_max=1):
+    y = np.array(y, dtype=data.dtype, copy=True)
+    getattr(lib(), "oracle_spmv_ell_" + _suf(data))(N, nc, _i32(indices),
+                                                   np.ascontiguousarray(data),
+                                                   np.ascontiguousarray(x, data.dtype), y, inner_max)
+    return y
+
+
+def spmv_csr_fp64(N, rowptr, col, val, x, y0=None):
+    """FP64 oracle: returns (y, sum_j |a_ij x_j|) computed in long double."""
+    val = np.ascontiguousarray(val, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros(N, np.float64)
+    ab = np.zeros(N, np.float64)
+    y0p = None
+    if y0 is not None:
+        y0 = np.ascontiguousarray(y0, np.float64)
+        y0p = y0.ctypes.data
+    lib().oracle_spmv_csr_ld(N, _i32(rowptr), _i32(col), val, x, y0p, out, ab.ctypes.data)
+    return out, ab
+
+
+# ------------------------------------------------------------------------ spts
+def spts_level(N, rowptr, col, val, b):
+    """Level-set order on a (reordered) lower CSR with b in the same order."""
+    y = np.zeros(N, val.dtype)
+    getattr(lib(), "oracle_spts_level_" + _suf(val))(N, _i32(rowptr), _i32(col),
+                                                    np.ascontiguousarray(val),
+                                                    np.ascontiguousarray(b, val.dtype), y)
+    return y
+
+
+def spts_syncfree(N, rowptr, col, val, b):
+    y = np.zeros(N, val.dtype)
+    getattr(lib(), "oracle_spts_syncfree_" + _suf(val))(N, _i32(rowptr), _i32(col),
+                                                       np.ascontiguousarray(val),
+                                                       np.ascontiguousarray(b, val.dtype), y)
+    return y
+
+
+def spts_fp64(N, rowptr, col, val, b):
+    y = np.zeros(N, np.float64)
+    lib().oracle_spts_ld(N, _i32(rowptr), _i32(col), np.ascontiguousarray(val, np.float64),
+                         np.ascontiguousarray(b, np.float64), y)
+    return y
+
+
+def rel_residual(N, rowptr, col, val, x, b):
+    return lib().oracle_rel_residual(N, _i32(rowptr), _i32(col),
+                                     np.ascontiguousarray(val, np.float64),
+                                     np.ascontiguousarray(x, np.float64),
+                                     np.ascontiguousarray(b, np.float64))
+
+
+def fletcher_sum(y):
+    y = np.ascontiguousarray(y)
+    return getattr(lib(), "oracle_fletcher_" + _suf(y))(len(y), y)
+
+
+def time_spmv_csr_f64(nrows, rowptr, col, val, x, warm=10, reps=30):
+    """Mean seconds per serial CSR SpMV over rows [0, nrows) (C timer)."""
+    y = np.zeros(nrows, np.float64)
+    t = lib().oracle_time_spmv_csr_f64(nrows, _i32(rowptr), _i32(col), np.ascontiguousarray(val, np.float64),
+                                       np.ascontiguousarray(x, np.float64), y, warm, reps)
+    return t, y

@@ -1,0 +1,529 @@
+/*
+ * sable_oracle.c -- CPU restatement of the SciWasm.Sparse Level-II hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: it is compiled
+ * into oracle/liboracle.so and may be loaded only by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and there only
+ * as the checker (or the timed CPU baseline).  The shipped product
+ * (libsablesparse_hip.so) never links or calls it.
+ *
+ * Each function restates one reference routine in the reference's own
+ * operation order (built with -ffp-contract=off so a*b+c is a rounded
+ * multiply followed by a rounded add, as in WebAssembly f32/f64 arithmetic).
+ * The *_ld functions are the FP64 oracle: long-double accumulation used to
+ * report the error of every floating-point result.
+ *
+ * Pinning (how each restatement is checked, see tests/test_oracle.py):
+ *   coo_csr, csr_ell, csr_dia, transposes, level sets: brute-force dense
+ *     reconstruction on random small matrices, closed forms (5-point and
+ *     7-point Laplacians: DIA offsets, ELL width, level of grid point
+ *     (i,j,k) = i+j+k and trinomial level sizes), invariants (monotone
+ *     row_ptr, ascending columns, stable ties, bijective permutations,
+ *     transpose(transpose(A)) = A).
+ *   spmv_*: exact integer arithmetic cases (values and x small integers, so
+ *     every rounding is exact) against a dense product; Laplacian closed
+ *     forms; order-sensitive cases (1e16, 1, -1e16) that a reordered sum
+ *     fails.
+ *   spts_*: manufactured solutions with exactly representable integer data,
+ *     summation-order-sensitive cases, and residual <= 1e-12 of the FP64
+ *     oracle.
+ *   fletcher_sum: hand-computed checksums.
+ * The reference ships no golden vectors or fixtures for this path, so these
+ * checks are the pin.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <time.h>
+
+#define EXPORT __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------ */
+/* coo_csr: seq/src/sparse32.js:1674-1720 (+ sort :1612-1627).          */
+/* Row histogram, exclusive scan, scatter in input order, shift back,   */
+/* then a per-row bubble sort by column that swaps only on '>' (so ties  */
+/* keep input order) and has no early exit.                             */
+/* ------------------------------------------------------------------ */
+#define DEF_COO_CSR(T, SUF)                                                       \
+EXPORT void oracle_coo_csr_##SUF(int N, int nnz, const int *row, const int *col,  \
+                                 const T *val, int *csr_row, int *csr_col,       \
+                                 T *csr_val)                                     \
+{                                                                                 \
+    memset(csr_row, 0, sizeof(int) * (size_t)(N + 1));                           \
+    memset(csr_col, 0, sizeof(int) * (size_t)nnz);                               \
+    memset(csr_val, 0, sizeof(T) * (size_t)nnz);                                 \
+    for (int i = 0; i < nnz; i++) csr_row[row[i]]++;                             \
+    int j = 0, j0;                                                                \
+    for (int i = 0; i < N; i++) { j0 = csr_row[i]; csr_row[i] = j; j += j0; }     \
+    for (int i = 0; i < nnz; i++) {                                               \
+        j = csr_row[row[i]];                                                      \
+        csr_col[j] = col[i];                                                      \
+        csr_val[j] = val[i];                                                      \
+        csr_row[row[i]]++;                                                        \
+    }                                                                             \
+    for (int i = N - 1; i > 0; i--) csr_row[i] = csr_row[i - 1];                 \
+    csr_row[0] = 0;                                                               \
+    csr_row[N] = nnz;                                                             \
+    for (int r = 0; r < N; r++) {                                                 \
+        int start = csr_row[r], end = csr_row[r + 1];                            \
+        for (int p = 0; p < end - start - 1; p++)                                 \
+            for (int q = start; q < end - p - 1; q++)                             \
+                if (csr_col[q] > csr_col[q + 1]) {                                \
+                    int tc = csr_col[q]; csr_col[q] = csr_col[q + 1];             \
+                    csr_col[q + 1] = tc;                                          \
+                    T tv = csr_val[q]; csr_val[q] = csr_val[q + 1];               \
+                    csr_val[q + 1] = tv;                                          \
+                }                                                                 \
+    }                                                                             \
+}
+DEF_COO_CSR(float, f32)
+DEF_COO_CSR(double, f64)
+
+/* num_cols: seq/src/sparse32.js:1436-1447 -- the longest row. */
+EXPORT int oracle_num_cols(int N, const int *rowptr)
+{
+    int max = 0;
+    for (int i = 0; i < N; i++) {
+        int t = rowptr[i + 1] - rowptr[i];
+        if (max < t) max = t;
+    }
+    return max;
+}
+
+/* csr_ell: seq/src/sparse32.js:1449-1484.  Column-major slots,
+ * data[k*N+i] / indices[k*N+i], padding value 0 and index 0.
+ * The guard (nc*N < 2^27 && nc*N/nnz <= 5) lives in the caller. */
+#define DEF_CSR_ELL(T, SUF)                                                       \
+EXPORT void oracle_csr_ell_##SUF(int N, int nc, const int *rowptr,               \
+                                 const int *col, const T *val, int *indices,     \
+                                 T *data)                                        \
+{                                                                                 \
+    memset(indices, 0, sizeof(int) * (size_t)nc * N);                            \
+    memset(data, 0, sizeof(T) * (size_t)nc * N);                                 \
+    for (int i = 0; i < N; i++) {                                                 \
+        int k = 0;                                                                \
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++) {                         \
+            data[(size_t)k * N + i] = val[j];                                     \
+            indices[(size_t)k * N + i] = col[j];                                  \
+            k++;                                                                  \
+        }                                                                         \
+    }                                                                             \
+}
+DEF_CSR_ELL(float, f32)
+DEF_CSR_ELL(double, f64)
+
+/* num_diags: seq/src/sparse32.js:1486-1512 (stride N) and
+ * seq/src/sparse64.js:1066-1092 (stride N - min|offset|).
+ * Returns the number of occupied diagonals; *min_abs receives the
+ * smallest |offset| among them (N-1 when none are occupied). */
+EXPORT int oracle_num_diags(int N, const int *rowptr, const int *col, int *min_abs)
+{
+    int *ind = (int *)calloc((size_t)(2 * N > 1 ? 2 * N - 1 : 1), sizeof(int));
+    int num_diag = 0;
+    for (int i = 0; i < N; i++)
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++)
+            if (!ind[N + col[j] - i - 1]++) num_diag++;
+    int diag_no = -((2 * N - 1) / 2);
+    int mn = abs(diag_no);
+    for (int i = 0; i < 2 * N - 1; i++) {
+        if (ind[i] && mn > abs(diag_no)) mn = abs(diag_no);
+        diag_no++;
+    }
+    free(ind);
+    if (min_abs) *min_abs = mn;
+    return num_diag;
+}
+
+/* csr_dia: seq/src/sparse32.js:1515-1574 (and sparse64.js:1095-1157).
+ * offset[] ascending over occupied diagonals; entry (i, i+off_k) goes to
+ * data[k*stride + i - move], move = N - stride for negative offsets.
+ * A later duplicate (i, c) overwrites an earlier one.  Padding is zero
+ * (sparse32.js zero-fills; sparse64.js leaves malloc contents, which this
+ * restatement defines as zero). */
+#define DEF_CSR_DIA(T, SUF)                                                       \
+EXPORT void oracle_csr_dia_##SUF(int N, int nd, int stride, const int *rowptr,   \
+                                 const int *col, const T *val, int *offset,      \
+                                 T *data)                                        \
+{                                                                                 \
+    memset(data, 0, sizeof(T) * (size_t)nd * stride);                            \
+    int *ind = (int *)calloc((size_t)(2 * N > 1 ? 2 * N - 1 : 1), sizeof(int));  \
+    for (int i = 0; i < N; i++)                                                   \
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++) ind[N + col[j] - i - 1]++;\
+    int diag_no = -((2 * N - 1) / 2), index = 0;                                 \
+    for (int i = 0; i < 2 * N - 1; i++) {                                         \
+        if (ind[i]) offset[index++] = diag_no;                                    \
+        diag_no++;                                                                \
+    }                                                                             \
+    free(ind);                                                                    \
+    for (int i = 0; i < N; i++)                                                   \
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++) {                         \
+            int c = col[j];                                                       \
+            for (int k = 0; k < nd; k++) {                                        \
+                int move = 0;                                                     \
+                if (c - i == offset[k]) {                                         \
+                    if (offset[k] < 0) move = N - stride;                         \
+                    data[(size_t)k * stride + i - move] = val[j];                 \
+                    break;                                                        \
+                }                                                                 \
+            }                                                                     \
+        }                                                                         \
+}
+DEF_CSR_DIA(float, f32)
+DEF_CSR_DIA(double, f64)
+
+/* transpose_csr: seq/src/sparse_opt_32.wat:6348-6477.  Column histogram
+ * into out_rowptr, exclusive scan, scatter in row order (so the new
+ * columns ascend), shift back.  Values are moved as bit patterns
+ * (esize bytes each). */
+EXPORT void oracle_transpose_csr(int N, int nnz, int esize, const int *rowptr,
+                                 const int *col, const void *val, int *out_rowptr,
+                                 int *out_col, void *out_val)
+{
+    memset(out_rowptr, 0, sizeof(int) * (size_t)(N + 1));
+    for (int i = 0; i < nnz; i++) out_rowptr[col[i]]++;
+    int sum = 0;
+    for (int i = 0; i < N; i++) { int c = out_rowptr[i]; out_rowptr[i] = sum; sum += c; }
+    out_rowptr[N] = nnz;
+    for (int i = 0; i < N; i++)
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++) {
+            int addr = out_rowptr[col[j]];
+            out_col[addr] = i;
+            memcpy((char *)out_val + (size_t)addr * esize,
+                   (const char *)val + (size_t)j * esize, (size_t)esize);
+            out_rowptr[col[j]] = addr + 1;
+        }
+    int prev = 0;
+    for (int i = 0; i <= N; i++) { int t = out_rowptr[i]; out_rowptr[i] = prev; prev = t; }
+}
+
+/* transpose_dia: seq/src/sparse_opt_32.wat:6479-6572.  out_offset[k] =
+ * -offset[k] (so the order is descending); each diagonal's stored run is
+ * shifted so that it is indexed by the new row.  Slots outside the run
+ * are zero here (the reference leaves them as allocated). */
+EXPORT void oracle_transpose_dia(int N, int nd, int stride, int esize,
+                                 const int *offset, const void *data,
+                                 int *out_offset, void *out_data)
+{
+    memset(out_data, 0, (size_t)esize * nd * stride);
+    int exp1 = N - stride;
+    for (int i = 0; i < nd; i++) {
+        int k = offset[i];
+        int base = i * stride - (k < 0 ? exp1 : 0);
+        int in_n = k < 0 ? -k : 0;
+        int in_end = (N < N - k) ? N : N - k;
+        out_offset[i] = -k;
+        int ko = -k;
+        int obase = i * stride - (ko < 0 || k < 0 ? exp1 : 0);
+        int out_n = ko < 0 ? -ko : 0;
+        for (int t = 0; in_n + t < in_end; t++)
+            memcpy((char *)out_data + (size_t)(obase + out_n + t) * esize,
+                   (const char *)data + (size_t)(base + in_n + t) * esize, (size_t)esize);
+    }
+}
+
+/* count_transpose_cols_ell + transpose_ell: seq/src/sparse_opt_32.wat
+ * :6574-6701.  A slot counts when its value compares > 0 or < 0 (so zeros,
+ * -0 and NaN are dropped); nnz_row[col] counts per new row; the new width
+ * is the largest count.  Scatter walks input rows j ascending and their
+ * slots ascending; padding of the output is zero here. */
+#define DEF_TELL(T, SUF)                                                          \
+EXPORT int oracle_count_transpose_cols_ell_##SUF(int N, int nc, const T *data,   \
+                                                 const int *indices)             \
+{                                                                                 \
+    if (!(nc > 0 && N > 0)) return 0;                                             \
+    int *cnt = (int *)calloc((size_t)N, sizeof(int));                             \
+    int mx = 0;                                                                   \
+    for (size_t e = 0; e < (size_t)nc * N; e++)                                   \
+        if (data[e] > 0 || data[e] < 0) {                                         \
+            int c = ++cnt[indices[e]];                                            \
+            if (c > mx) mx = c;                                                   \
+        }                                                                         \
+    free(cnt);                                                                    \
+    return mx;                                                                    \
+}                                                                                 \
+EXPORT void oracle_transpose_ell_##SUF(int N, int nc, int nc_out, const T *data, \
+                                       const int *indices, T *out_data,          \
+                                       int *out_indices)                         \
+{                                                                                 \
+    memset(out_data, 0, sizeof(T) * (size_t)nc_out * N);                          \
+    memset(out_indices, 0, sizeof(int) * (size_t)nc_out * N);                     \
+    if (!(nc > 0 && N > 0)) return;                                               \
+    int *cnt = (int *)calloc((size_t)N, sizeof(int));                             \
+    for (int j = 0; j < N; j++)                                                   \
+        for (int s = 0; s < nc; s++) {                                            \
+            size_t e = (size_t)s * N + j;                                         \
+            if (data[e] > 0 || data[e] < 0) {                                     \
+                int c = indices[e];                                               \
+                size_t o = (size_t)cnt[c] * N + c;                                \
+                out_data[o] = data[e];                                            \
+                out_indices[o] = j;                                               \
+                cnt[c]++;                                                         \
+            }                                                                     \
+        }                                                                         \
+    free(cnt);                                                                    \
+}
+DEF_TELL(float, f32)
+DEF_TELL(double, f64)
+
+/* ------------------------------------------------------------------ */
+/* Level sets: par/src/sparse32.js:2942-3008 and :3010-3181.            */
+/* level[i] = 0 for a row with exactly one entry, else 1 + max(0, levels  */
+/* of the columns of all entries but the last).  level_ptr is the prefix */
+/* sum of level sizes; permutation[new] = old by a stable counting sort; */
+/* order[old] = new.  Returns the number of levels.                     */
+/* ------------------------------------------------------------------ */
+EXPORT int oracle_level_sets(int N, const int *rowptr, const int *col,
+                             int *level_of_row, int *level_ptr_out /* N+1 cap */,
+                             int *permutation, int *order)
+{
+    int max_level = 0;
+    for (int i = 0; i < N; i++) {
+        if (rowptr[i + 1] - rowptr[i] == 1) {
+            level_of_row[i] = 0;
+        } else {
+            int m = 0;
+            for (int j = rowptr[i]; j < rowptr[i + 1] - 1; j++)
+                if (level_of_row[col[j]] > m) m = level_of_row[col[j]];
+            level_of_row[i] = 1 + m;
+            if (level_of_row[i] > max_level) max_level = level_of_row[i];
+        }
+    }
+    int tot = max_level + 1;
+    int *freq = (int *)calloc((size_t)tot, sizeof(int));
+    int *start = (int *)calloc((size_t)tot, sizeof(int));
+    for (int i = 0; i < N; i++) freq[level_of_row[i]]++;
+    level_ptr_out[0] = 0;
+    start[0] = 0;
+    for (int l = 1; l < tot; l++) start[l] = level_ptr_out[l] = level_ptr_out[l - 1] + freq[l - 1];
+    level_ptr_out[tot] = level_ptr_out[tot - 1] + freq[tot - 1];
+    for (int i = 0; i < N; i++) {
+        int l = level_of_row[i];
+        permutation[start[l]] = i;
+        if (order) order[i] = start[l];
+        start[l]++;
+    }
+    free(freq);
+    free(start);
+    return tot;
+}
+
+/* Reordered CSR of CSR_create_level_sets_with_reorder
+ * (par/src/sparse32.js:3136-3150): new row i is old row permutation[i],
+ * its columns renamed order[col], column order within the row unchanged. */
+#define DEF_REORDER(T, SUF)                                                       \
+EXPORT void oracle_level_reorder_##SUF(int N, const int *rowptr, const int *col, \
+                                       const T *val, const int *permutation,     \
+                                       const int *order, int *new_rowptr,        \
+                                       int *new_col, T *new_val)                 \
+{                                                                                 \
+    int j = 0;                                                                    \
+    new_rowptr[0] = 0;                                                            \
+    for (int i = 0; i < N; i++) {                                                 \
+        int k = rowptr[permutation[i]];                                           \
+        int len = rowptr[permutation[i] + 1] - k;                                 \
+        new_rowptr[i + 1] = new_rowptr[i] + len;                                  \
+        while (len--) { new_col[j] = order[col[k]]; new_val[j++] = val[k++]; }   \
+    }                                                                             \
+}
+DEF_REORDER(float, f32)
+DEF_REORDER(double, f64)
+
+/* ------------------------------------------------------------------ */
+/* SpMV in reference order.  Semantics are y += A.x, inner_max times    */
+/* (the *_wrapper exports).                                             */
+/* ------------------------------------------------------------------ */
+/* spmv_coo: seq/src/sparse_opt_32.wat:175-202: y[r] = v*x[c] + y[r], k ascending. */
+#define DEF_SPMV(T, SUF)                                                          \
+EXPORT void oracle_spmv_coo_##SUF(int nnz, const int *row, const int *col,       \
+                                  const T *val, const T *x, T *y, int inner_max) \
+{                                                                                 \
+    for (int it = 0; it < inner_max; it++)                                        \
+        for (int k = 0; k < nnz; k++) {                                           \
+            T p = val[k] * x[col[k]];                                             \
+            y[row[k]] = p + y[row[k]];                                            \
+        }                                                                         \
+}                                                                                 \
+/* spmv_csr: seq/src/sparse_opt_32.wat:227-271: t = y[i]; t = v*x + t. */         \
+EXPORT void oracle_spmv_csr_##SUF(int N, const int *rowptr, const int *col,      \
+                                  const T *val, const T *x, T *y, int inner_max) \
+{                                                                                 \
+    for (int it = 0; it < inner_max; it++)                                        \
+        for (int i = 0; i < N; i++) {                                             \
+            int j = rowptr[i], e = rowptr[i + 1];                                 \
+            if (j < e) {                                                          \
+                T t = y[i];                                                       \
+                for (; j < e; j++) { T p = val[j] * x[col[j]]; t = p + t; }       \
+                y[i] = t;                                                         \
+            }                                                                     \
+        }                                                                         \
+}                                                                                 \
+/* spmv_dia: seq/src/sparse_opt_32.wat:290-407: diagonals in stored order,       \
+ * rows n in [max(0,-off), min(N,N-off)), y[n] = d*x[n+off] + y[n]. */           \
+EXPORT void oracle_spmv_dia_##SUF(int N, int nd, int stride, const int *offset,  \
+                                  const T *data, const T *x, T *y, int inner_max)\
+{                                                                                 \
+    for (int it = 0; it < inner_max; it++)                                        \
+        for (int k = 0; k < nd; k++) {                                            \
+            int off = offset[k];                                                  \
+            long base = (long)k * stride - (off < 0 ? N - stride : 0);            \
+            int n0 = off < 0 ? -off : 0;                                          \
+            int n1 = (N < N - off) ? N : N - off;                                 \
+            for (int n = n0; n < n1; n++) {                                       \
+                T p = data[base + n] * x[n + off];                                \
+                y[n] = p + y[n];                                                  \
+            }                                                                     \
+        }                                                                         \
+}                                                                                 \
+/* spmv_ell: seq/src/sparse_opt_32.wat:427-479: slot-major sweep,                \
+ * y[i] = d*x[idx] + y[i] (padding adds 0*x[0]). */                              \
+EXPORT void oracle_spmv_ell_##SUF(int N, int nc, const int *indices,             \
+                                  const T *data, const T *x, T *y, int inner_max)\
+{                                                                                 \
+    if (!(nc > 0 && N > 0)) return;                                               \
+    for (int it = 0; it < inner_max; it++)                                        \
+        for (int j = 0; j < nc; j++)                                              \
+            for (int i = 0; i < N; i++) {                                         \
+                size_t e = (size_t)j * N + i;                                     \
+                T p = data[e] * x[indices[e]];                                    \
+                y[i] = p + y[i];                                                  \
+            }                                                                     \
+}
+DEF_SPMV(float, f32)
+DEF_SPMV(double, f64)
+
+/* FP64 oracle for CSR SpMV: y_ld[i] = y0[i] + sum_j v*x in long double. */
+EXPORT void oracle_spmv_csr_ld(int N, const int *rowptr, const int *col,
+                               const double *val, const double *x, const double *y0,
+                               double *y_out, double *absrow /* sum |v||x| (nullable) */)
+{
+    for (int i = 0; i < N; i++) {
+        long double t = y0 ? (long double)y0[i] : 0.0L, a = 0.0L;
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++) {
+            t += (long double)val[j] * (long double)x[col[j]];
+            a += fabsl((long double)val[j] * (long double)x[col[j]]);
+        }
+        y_out[i] = (double)t;
+        if (absrow) absrow[i] = (double)a;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Lower triangular solve.  The diagonal is the last entry of each row. */
+/* ------------------------------------------------------------------ */
+/* Level-set order, spts_csr par/src/sparse_opt_32.wat:11679-11757 driven
+ * by spts_level_csr :11759-11984: t = 0; t = t + v*y[c] over the
+ * off-diagonals in stored order; y[i] = (b[i] - t)/diag.  Rows are processed
+ * level by level (in row order within a level, which gives the same result
+ * because rows of one level are independent). */
+#define DEF_SPTS(T, SUF)                                                          \
+EXPORT void oracle_spts_level_##SUF(int N, const int *rowptr, const int *col,    \
+                                    const T *val, const T *b, T *y)              \
+{                                                                                 \
+    for (int i = 0; i < N; i++) {                                                 \
+        int j = rowptr[i], end = rowptr[i + 1] - 1;                               \
+        if (end > j) {                                                            \
+            T t = 0;                                                              \
+            for (; j < end; j++) { T p = val[j] * y[col[j]]; t = t + p; }         \
+            y[i] = (b[i] - t) / val[end];                                         \
+        } else {                                                                  \
+            y[i] = b[i] / val[j];                                                 \
+        }                                                                         \
+    }                                                                             \
+}                                                                                 \
+/* Sync-free order, spts_csr_sync_free par/src/spts_opt_32.wat:8-117:            \
+ * t = b[i]; t = t - v*y[c]; y[i] = t/diag. */                                    \
+EXPORT void oracle_spts_syncfree_##SUF(int N, const int *rowptr, const int *col, \
+                                       const T *val, const T *b, T *y)           \
+{                                                                                 \
+    for (int i = 0; i < N; i++) {                                                 \
+        int j = rowptr[i], end = rowptr[i + 1] - 1;                               \
+        T t = b[i];                                                               \
+        for (; j < end; j++) { T p = val[j] * y[col[j]]; t = t - p; }             \
+        y[i] = t / val[end];                                                      \
+    }                                                                             \
+}
+DEF_SPTS(float, f32)
+DEF_SPTS(double, f64)
+
+/* FP64 oracle forward substitution in long double. */
+EXPORT void oracle_spts_ld(int N, const int *rowptr, const int *col,
+                           const double *val, const double *b, double *y)
+{
+    long double *t = (long double *)malloc(sizeof(long double) * (size_t)(N > 0 ? N : 1));
+    for (int i = 0; i < N; i++) {
+        int end = rowptr[i + 1] - 1;
+        long double s = (long double)b[i];
+        for (int j = rowptr[i]; j < end; j++) s -= (long double)val[j] * t[col[j]];
+        t[i] = s / (long double)val[end];
+        y[i] = (double)t[i];
+    }
+    free(t);
+}
+
+/* Relative residual ||b - L x||_inf / (||L||_inf ||x||_inf + ||b||_inf) in
+ * long double, for any square CSR (the lower-solve acceptance metric). */
+EXPORT double oracle_rel_residual(int N, const int *rowptr, const int *col,
+                                  const double *val, const double *x, const double *b)
+{
+    long double rmax = 0, lnorm = 0, xmax = 0, bmax = 0;
+    for (int i = 0; i < N; i++) {
+        long double s = (long double)b[i], rs = 0;
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++) {
+            s -= (long double)val[j] * (long double)x[col[j]];
+            rs += fabsl((long double)val[j]);
+        }
+        if (fabsl(s) > rmax) rmax = fabsl(s);
+        if (rs > lnorm) lnorm = rs;
+        if (fabsl((long double)x[i]) > xmax) xmax = fabsl((long double)x[i]);
+        if (fabsl((long double)b[i]) > bmax) bmax = fabsl((long double)b[i]);
+    }
+    long double den = lnorm * xmax + bmax;
+    return den > 0 ? (double)(rmax / den) : (double)rmax;
+}
+
+/* fletcher_sum_y: seq/src/sparse32.js:1291-1311.  JS numbers are doubles,
+ * matlab_modulo(x,255) = x - floor(x/255)*255, parseInt truncates. */
+static double mmod(double x, double y) { return x - floor(x / y) * y; }
+EXPORT double oracle_fletcher_f32(int n, const float *y)
+{
+    double s1 = 0, s2 = 0;
+    for (int i = 0; i < n; i++) { s1 = mmod(s1 + (double)y[i], 255); s2 = mmod(s2 + s1, 255); }
+    double r = s2 * 256 + s1;
+    return isnan(r) ? r : trunc(r);
+}
+EXPORT double oracle_fletcher_f64(int n, const double *y)
+{
+    double s1 = 0, s2 = 0;
+    for (int i = 0; i < n; i++) { s1 = mmod(s1 + y[i], 255); s2 = mmod(s2 + s1, 255); }
+    double r = s2 * 256 + s1;
+    return isnan(r) ? r : trunc(r);
+}
+
+/* CPU baseline timing of the serial restatement (seq/tests/main32.js:122-134
+ * method: warm-up calls, then timed calls; y is cleared before each call).
+ * Returns the mean seconds per spmv_csr call over `reps` timed calls. */
+#include <time.h>
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+EXPORT double oracle_time_spmv_csr_f64(int nrows, const int *rowptr, const int *col,
+                                       const double *val, const double *x, double *y,
+                                       int warm, int reps)
+{
+    for (int w = 0; w < warm; w++) {
+        memset(y, 0, sizeof(double) * (size_t)nrows);
+        oracle_spmv_csr_f64(nrows, rowptr, col, val, x, y, 1);
+    }
+    double t = 0;
+    for (int r = 0; r < reps; r++) {
+        memset(y, 0, sizeof(double) * (size_t)nrows);
+        double t0 = now_s();
+        oracle_spmv_csr_f64(nrows, rowptr, col, val, x, y, 1);
+        t += now_s() - t0;
+    }
+    return reps > 0 ? t / reps : 0.0;
+}

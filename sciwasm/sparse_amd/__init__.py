@@ -1,0 +1,21 @@
+"""sciwasm.sparse_amd -- MI355X (gfx950) HIP implementation of the
+SciWasm.Sparse Level-II hot path, with a host API that mirrors the
+reference's sparse32.js / sparse64.js call surface.
+
+All compute runs in libsablesparse_hip.so (HIP kernels); this package is
+the host-side object layer and binding.  Importing it does not touch the
+GPU; the first operation creates the default device context.
+"""
+from ._lib import SableError, load as _load_library  # noqa: F401
+from .core import (Context, DeviceArray, sparse_COO_t, sparse_CSR_t, sparse_DIA_t, sparse_ELL_t,  # noqa: F401
+                   sparse_vec_t, coo_from_host, csr_from_host, dia_from_host, ell_from_host, create_vec,
+                   vec_from_host, clear_vec, init_x, fletcher_sum, fletcher_sum_y)
+from .spmv import spmv, CsrPlan  # noqa: F401
+
+__all__ = [
+    "Context", "DeviceArray", "SableError",
+    "sparse_COO_t", "sparse_CSR_t", "sparse_DIA_t", "sparse_ELL_t", "sparse_vec_t",
+    "coo_from_host", "csr_from_host", "dia_from_host", "ell_from_host",
+    "create_vec", "vec_from_host", "clear_vec", "init_x", "fletcher_sum", "fletcher_sum_y",
+    "spmv", "CsrPlan",
+]

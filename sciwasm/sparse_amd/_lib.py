@@ -1,0 +1,108 @@
+"""ctypes binding of libsablesparse_hip.so (the C ABI in include/sable_sparse.h).
+
+The library is loaded from this package directory.  There is no fallback:
+if the shared object is missing or fails to load, every call raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsablesparse_hip.so")
+
+c_i32 = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_int = ctypes.c_int
+c_dbl = ctypes.c_double
+c_flt = ctypes.c_float
+c_vp = ctypes.c_void_p
+c_sz = ctypes.c_size_t
+c_vpp = ctypes.POINTER(ctypes.c_void_p)
+
+F32 = 0
+F64 = 1
+
+ERRORS = {
+    -1: "SABLE_EINVAL", -2: "SABLE_EREFUSED", -3: "SABLE_EMALFORMED", -4: "SABLE_ENOMEM",
+    -5: "SABLE_EHIP", -6: "SABLE_EBUDGET", -7: "SABLE_ETIMEOUT", -8: "SABLE_ESINGULAR",
+}
+
+
+class SableError(RuntimeError):
+    def __init__(self, code, fn, msg):
+        super().__init__("%s from %s: %s" % (ERRORS.get(code, code), fn, msg))
+        self.code = code
+        self.name = ERRORS.get(code, str(code))
+
+
+class CsrTune(ctypes.Structure):
+    _fields_ = [("block_weight", c_int), ("row_weight", c_int)]
+
+
+# name -> argtypes; every function returns int unless listed in _RESTYPES.
+SIGNATURES = {
+    "sable_ctx_create": [c_int, c_vp, c_vpp],
+    "sable_ctx_destroy": [c_vp],
+    "sable_ctx_sync": [c_vp],
+    "sable_ctx_stream": [c_vp],
+    "sable_last_error": [],
+    "sable_version": [],
+    "sable_device_alloc": [c_vp, c_sz, c_vpp],
+    "sable_device_free": [c_vp, c_vp],
+    "sable_memcpy_h2d": [c_vp, c_vp, c_vp, c_sz],
+    "sable_memcpy_d2h": [c_vp, c_vp, c_vp, c_sz],
+    "sable_memcpy_d2d": [c_vp, c_vp, c_vp, c_sz],
+    "sable_fill_bytes": [c_vp, c_vp, c_int, c_sz],
+    "sable_fill_value": [c_vp, c_int, c_vp, c_i64, c_dbl],
+    "sable_event_create": [c_vpp],
+    "sable_event_destroy": [c_vp],
+    "sable_event_record": [c_vp, c_vp],
+    "sable_event_elapsed_ms": [c_vp, c_vp, ctypes.POINTER(c_flt)],
+    "sable_csr_plan_create": [c_vp, c_i32, c_vp, ctypes.POINTER(CsrTune), c_vpp],
+    "sable_csr_plan_destroy": [c_vp],
+    "sable_csr_plan_info": [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64), ctypes.POINTER(c_i32)],
+    "sable_spmv_csr_plan": [c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
+    "sable_spmv_csr": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
+    "sable_spmv_coo": [c_vp, c_int, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
+    "sable_spmv_dia": [c_vp, c_int, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
+    "sable_spmv_ell": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
+}
+
+_RESTYPES = {
+    "sable_ctx_destroy": None,
+    "sable_csr_plan_destroy": None,
+    "sable_ctx_stream": c_vp,
+    "sable_last_error": ctypes.c_char_p,
+    "sable_version": ctypes.c_char_p,
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libsablesparse_hip.so is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, c_int)
+    _lib = L
+    return L
+
+
+def call(name, *args):
+    """Call an int-returning entry point and raise SableError on failure."""
+    L = load()
+    rc = getattr(L, name)(*args)
+    if rc != 0:
+        msg = L.sable_last_error()
+        raise SableError(rc, name, msg.decode() if msg else "")
+    return rc
+
+
+def exported_symbols():
+    return sorted(SIGNATURES)

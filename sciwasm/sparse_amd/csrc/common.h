@@ -1,0 +1,87 @@
+// common.h -- shared internals of libsablesparse_hip (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+#include <vector>
+#include <mutex>
+
+#include "../../../include/sable_sparse.h"
+
+namespace sable {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char* fmt, ...);
+int fail(int code, const char* fmt, ...);
+
+#define SABLE_HIP(call)                                                          \
+    do {                                                                         \
+        hipError_t e_ = (call);                                                  \
+        if (e_ != hipSuccess)                                                    \
+            return ::sable::fail(SABLE_EHIP, "%s failed: %s (%s:%d)", #call,    \
+                                 hipGetErrorString(e_), __FILE__, __LINE__);     \
+    } while (0)
+
+#define SABLE_CHECK(cond, code, ...)                                             \
+    do {                                                                         \
+        if (!(cond)) return ::sable::fail((code), __VA_ARGS__);                  \
+    } while (0)
+
+#define SABLE_LAUNCH_CHECK() SABLE_HIP(hipGetLastError())
+
+}  // namespace sable
+
+// ---------------------------------------------------------------- context
+struct sable_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int num_cu = 256;
+    // grow-only device scratch for transient plans and flags
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // small pinned host buffer for reading device flags / counters
+    void* pinned = nullptr;
+    std::mutex mu;
+};
+
+namespace sable {
+int ctx_scratch(sable_ctx_t* ctx, size_t bytes, void** out);
+int read_device_i64(sable_ctx_t* ctx, const void* dsrc, size_t count, int64_t* host_out);
+
+constexpr int WAVE = 64;
+
+inline unsigned grid_for(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 0x7fffffff) g = 0x7fffffff;
+    return (unsigned)g;
+}
+
+// Rounded multiply and add without contraction: the reference's WebAssembly
+// arithmetic is a separately rounded mul followed by add.
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ float sub_rn(float a, float b) { return __fsub_rn(a, b); }
+__device__ __forceinline__ double sub_rn(double a, double b) { return __dsub_rn(a, b); }
+__device__ __forceinline__ float div_rn(float a, float b) { return __fdiv_rn(a, b); }
+__device__ __forceinline__ double div_rn(double a, double b) { return __ddiv_rn(a, b); }
+
+// Sum over aligned groups of L lanes (L a power of two <= 64).
+template <typename T>
+__device__ __forceinline__ T group_sum(T v, int L) {
+    for (int off = L >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+}  // namespace sable

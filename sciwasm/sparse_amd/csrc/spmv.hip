@@ -1,0 +1,527 @@
+// spmv.hip -- SpMV over CSR, COO, DIA and ELL for gfx950.
+//
+// CSR uses a row-block schedule computed on the device: rows are cut into
+// blocks of roughly equal weight  nnz + row_weight*rows  (a monotone key, so
+// block boundaries are found in parallel, one thread per row).  A block whose
+// nonzeros fit the LDS buffer is "streamed": the workgroup reads val/col
+// fully coalesced, gathers x, stores the products in LDS, then reduces each
+// row with L lanes (L = largest power of two with L*rows <= 256) and a
+// shuffle tree.  A block that does not fit (it holds a long row) reduces
+// each of its rows with the whole workgroup.  Workgroup ids are remapped so
+// that each XCD processes a contiguous range of blocks (its L2 then sees a
+// contiguous band of rows and, for banded matrices, of x).
+#include "common.h"
+
+#include <cstdlib>
+#include <cstring>
+
+struct sable_csr_plan_s {
+    sable_ctx_t* ctx = nullptr;
+    int32_t N = 0;
+    int64_t nnz = 0;
+    const int32_t* row_ptr = nullptr;
+    int32_t* blk = nullptr;  // nblocks + 1 row boundaries
+    int64_t nblocks = 0;
+    int block_weight = 0;
+    int row_weight = 0;
+    int lds_cap = 0;  // elements staged per block
+    bool owns_blk = false;
+};
+
+namespace sable {
+
+constexpr int BS = 256;
+constexpr int DEFAULT_BLOCK_WEIGHT = 1024;
+constexpr int DEFAULT_ROW_WEIGHT = 2;
+constexpr int NUM_XCD = 8;
+
+// Physical workgroup p runs on XCD p % 8; give XCD x a contiguous range of
+// logical blocks.
+__device__ __forceinline__ int64_t xcd_remap(int64_t p, int64_t nb) {
+    int64_t q = nb / NUM_XCD, r = nb % NUM_XCD;
+    int64_t x = p % NUM_XCD, idx = p / NUM_XCD;
+    return x * q + (x < r ? x : r) + idx;
+}
+
+// One thread per row i in [0, N]: write blk[b] = first row whose key reaches
+// b*T.  key(i) = (row_ptr[i] - row_ptr[0]) + C*i.  Flags a non-monotone
+// row_ptr in *err.
+__global__ void k_plan_fill(int32_t N, const int32_t* __restrict__ rp, int64_t C, int64_t T,
+                            int64_t B, int32_t* __restrict__ blk, int* __restrict__ err) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > N) return;
+    int64_t base = rp[0];
+    int64_t kprev = -1;
+    if (i > 0) kprev = ((int64_t)rp[i - 1] - base) + C * (i - 1);
+    int64_t blo = kprev < 0 ? 0 : kprev / T + 1;
+    int64_t bhi;
+    if (i < N) {
+        int64_t a = rp[i], e = rp[i + 1];
+        if (e < a) atomicOr(err, 1);
+        bhi = ((a - base) + C * i) / T;
+        if (bhi > B) bhi = B;
+    } else {
+        bhi = B;
+    }
+    for (int64_t b = blo; b <= bhi; b++) blk[b] = (int32_t)i;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* red) {
+    v = wave_sum(v);
+    int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    T s = 0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < BS / WAVE; k++) s += red[k];
+    }
+    return s;
+}
+
+template <typename T>
+__device__ __forceinline__ void write_y(T* y, int64_t r, T s, T beta, bool nonempty) {
+    if (beta == T(0)) {
+        y[r] = s;
+    } else if (beta == T(1)) {
+        if (nonempty) y[r] = y[r] + s;  // the reference does not store empty rows
+    } else {
+        y[r] = beta * y[r] + s;
+    }
+}
+
+template <typename T, int U>
+__global__ __launch_bounds__(BS) void k_csr_block(const int32_t* __restrict__ blk, int64_t nblocks,
+                                                  const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ col,
+                                                  const T* __restrict__ val, const T* __restrict__ x,
+                                                  T* __restrict__ y, T beta, int lds_cap) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* prod = reinterpret_cast<T*>(smem_raw);
+    __shared__ T red[BS / WAVE];
+    const int64_t b = xcd_remap(blockIdx.x, nblocks);
+    const int r0 = blk[b], r1 = blk[b + 1];
+    if (r0 >= r1) return;
+    const int tid = threadIdx.x;
+    const int64_t nz0 = rp[r0];
+    const int nnzb = (int)((int64_t)rp[r1] - nz0);
+    const int R = r1 - r0;
+    if (nnzb <= lds_cap) {
+        const int32_t* cb = col + nz0;
+        const T* vb = val + nz0;
+        int k = tid;
+        for (; k + (U - 1) * BS < nnzb; k += U * BS) {
+            int c[U];
+            T v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) { c[u] = cb[k + u * BS]; v[u] = vb[k + u * BS]; }
+            T xv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < U; u++) prod[k + u * BS] = v[u] * xv[u];
+        }
+        for (; k < nnzb; k += BS) prod[k] = vb[k] * x[cb[k]];
+        __syncthreads();
+        int L = WAVE;
+        while (L > 1 && L * R > BS) L >>= 1;
+        const int groups = BS / L, g = tid / L, l = tid & (L - 1);
+        for (int rb = 0; rb < R; rb += groups) {
+            const int r = r0 + rb + g;
+            T s = 0;
+            int a = 0, e = 0;
+            if (r < r1) {
+                a = (int)((int64_t)rp[r] - nz0);
+                e = (int)((int64_t)rp[r + 1] - nz0);
+                for (int kk = a + l; kk < e; kk += L) s += prod[kk];
+            }
+            s = group_sum(s, L);
+            if (l == 0 && r < r1) write_y(y, r, s, beta, e > a);
+        }
+    } else {
+        for (int r = r0; r < r1; r++) {
+            const int64_t a = rp[r], e = rp[r + 1];
+            T s = 0;
+            int64_t k = a + tid;
+            for (; k + (U - 1) * BS < e; k += U * BS) {
+                int c[U];
+                T v[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) { c[u] = col[k + u * BS]; v[u] = val[k + u * BS]; }
+#pragma unroll
+                for (int u = 0; u < U; u++) s += v[u] * x[c[u]];
+            }
+            for (; k < e; k += BS) s += val[k] * x[col[k]];
+            s = block_sum(s, red);
+            if (tid == 0) write_y(y, r, s, beta, e > a);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ DIA
+// One thread per row; diagonals in stored order; rounded mul then add.
+template <typename T>
+__global__ __launch_bounds__(BS) void k_dia(int32_t N, int32_t nd, int32_t stride,
+                                            const int32_t* __restrict__ offset,
+                                            const T* __restrict__ data, const T* __restrict__ x,
+                                            T* __restrict__ y, T beta) {
+    __shared__ int32_t offs[1024];
+    const int ndl = nd < 1024 ? nd : 1024;
+    for (int k = threadIdx.x; k < ndl; k += blockDim.x) offs[k] = offset[k];
+    __syncthreads();
+    const int64_t shiftneg = (int64_t)N - stride;
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        T t = beta == T(1) ? y[n] : (beta == T(0) ? T(0) : beta * y[n]);
+        for (int k = 0; k < nd; k++) {
+            const int off = k < 1024 ? offs[k] : offset[k];
+            const int64_t c = n + off;
+            if (c >= 0 && c < N) {
+                const T d = data[(int64_t)k * stride + n - (off < 0 ? shiftneg : 0)];
+                t = add_rn(mul_rn(d, x[c]), t);
+            }
+        }
+        y[n] = t;
+    }
+}
+
+// ------------------------------------------------------------------ ELL
+// One thread per row; slots in order; rounded mul then add (padding slots
+// add 0*x[0], as in the reference).
+template <typename T>
+__global__ __launch_bounds__(BS) void k_ell(int32_t N, int32_t nc, const int32_t* __restrict__ idx,
+                                            const T* __restrict__ data, const T* __restrict__ x,
+                                            T* __restrict__ y, T beta) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        T t = beta == T(1) ? y[i] : (beta == T(0) ? T(0) : beta * y[i]);
+        int j = 0;
+        for (; j + 3 < nc; j += 4) {
+            int c[4];
+            T d[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int64_t e = (int64_t)(j + u) * N + i;
+                c[u] = idx[e];
+                d[u] = data[e];
+            }
+            T xv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < 4; u++) t = add_rn(mul_rn(d[u], xv[u]), t);
+        }
+        for (; j < nc; j++) {
+            const int64_t e = (int64_t)j * N + i;
+            t = add_rn(mul_rn(data[e], x[idx[e]]), t);
+        }
+        y[i] = t;
+    }
+}
+
+// ------------------------------------------------------------------ COO
+// Flags rows out of order (bit 1) or out of [0, N) (bit 2).
+__global__ void k_coo_check(int64_t nnz, int32_t N, const int32_t* __restrict__ row, int* __restrict__ err) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int r = row[k];
+        if (r < 0 || r >= N) atomicOr(err, 2);
+        if (k + 1 < nnz && row[k + 1] < r) atomicOr(err, 1);
+    }
+}
+
+// Row heads of a row-sorted COO become a row pointer: thread k in [0, nnz]
+// writes rp[r] = k for every row r in (row[k-1], row[k]].
+__global__ void k_coo_heads(int64_t nnz, int32_t N, const int32_t* __restrict__ row, int32_t* __restrict__ rp) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nnz) return;
+    const int lo = k == 0 ? 0 : row[k - 1] + 1;
+    const int hi = k == nnz ? N : row[k];
+    for (int r = lo; r <= hi; r++) rp[r] = (int32_t)k;
+}
+
+template <typename T>
+__global__ void k_scale(int32_t N, T* __restrict__ y, T beta) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = beta == T(0) ? T(0) : beta * y[i];
+}
+
+template <typename T>
+__global__ void k_coo_atomic(int64_t nnz, const int32_t* __restrict__ row, const int32_t* __restrict__ col,
+                             const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+         k += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&y[row[k]], val[k] * x[col[k]]);
+}
+
+// ---------------------------------------------------------------- host side
+static void tune_defaults(const sable_csr_tune_t* t, int* T, int* C) {
+    *T = (t && t->block_weight > 0) ? t->block_weight : DEFAULT_BLOCK_WEIGHT;
+    *C = (t && t->row_weight > 0) ? t->row_weight : DEFAULT_ROW_WEIGHT;
+}
+
+static int64_t nblocks_for(int64_t nnz, int32_t N, int T, int C) {
+    int64_t W = nnz + (int64_t)C * N;
+    return W <= 0 ? 0 : (W + T - 1) / T;
+}
+
+static int lds_cap_for(int T) {
+    int cap = 2 * T;
+    if (cap < 512) cap = 512;
+    if (cap > 8192) cap = 8192;
+    return cap;
+}
+
+// Reads row_ptr[0] and row_ptr[N] (synchronises).
+static int read_nnz(sable_ctx_t* ctx, int32_t N, const int32_t* rp, int64_t* nnz) {
+    int32_t ends[2];
+    SABLE_HIP(hipMemcpyAsync(ctx->pinned, rp, 4, hipMemcpyDeviceToHost, ctx->stream));
+    SABLE_HIP(hipMemcpyAsync((char*)ctx->pinned + 4, rp + N, 4, hipMemcpyDeviceToHost, ctx->stream));
+    SABLE_HIP(hipStreamSynchronize(ctx->stream));
+    memcpy(ends, ctx->pinned, 8);
+    SABLE_CHECK(ends[1] >= ends[0] && ends[0] >= 0, SABLE_EMALFORMED,
+                "row_ptr[N]=%d < row_ptr[0]=%d", ends[1], ends[0]);
+    *nnz = (int64_t)ends[1] - ends[0];
+    return SABLE_OK;
+}
+
+// Fill blk (device, nblocks+1) and check monotonicity; err is a device int.
+static int plan_fill(sable_ctx_t* ctx, int32_t N, const int32_t* rp, int T, int C, int64_t B,
+                     int32_t* blk, int* err, bool check) {
+    SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_plan_fill, dim3(grid_for((int64_t)N + 1, BS)), dim3(BS), 0, ctx->stream, N, rp,
+                       (int64_t)C, (int64_t)T, B, blk, err);
+    SABLE_LAUNCH_CHECK();
+    if (check) {
+        int64_t e = 0;
+        SABLE_HIP(hipMemcpyAsync(ctx->pinned, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        SABLE_HIP(hipStreamSynchronize(ctx->stream));
+        e = *(int*)ctx->pinned;
+        SABLE_CHECK(e == 0, SABLE_EMALFORMED, "row_ptr is not monotone");
+    }
+    return SABLE_OK;
+}
+
+template <typename T>
+static int launch_csr(sable_ctx_t* ctx, const int32_t* blk, int64_t nb, const int32_t* rp, const int32_t* col,
+                      const void* val, const void* x, void* y, double beta, int inner_max, int lds_cap) {
+    if (nb <= 0) return SABLE_OK;
+    SABLE_CHECK(nb < 0x7fffffffLL, SABLE_EINVAL, "too many row blocks");
+    size_t shm = (size_t)lds_cap * sizeof(T);
+    for (int it = 0; it < inner_max; it++) {
+        hipLaunchKernelGGL((k_csr_block<T, 4>), dim3((unsigned)nb), dim3(BS), shm, ctx->stream, blk, nb, rp, col,
+                           (const T*)val, (const T*)x, (T*)y, (T)beta, lds_cap);
+        SABLE_LAUNCH_CHECK();
+    }
+    return SABLE_OK;
+}
+
+static int dispatch_csr(sable_ctx_t* ctx, sable_dtype_t dt, const int32_t* blk, int64_t nb, const int32_t* rp,
+                        const int32_t* col, const void* val, const void* x, void* y, double beta,
+                        int inner_max, int lds_cap) {
+    if (dt == SABLE_F32) return launch_csr<float>(ctx, blk, nb, rp, col, val, x, y, beta, inner_max, lds_cap);
+    if (dt == SABLE_F64) return launch_csr<double>(ctx, blk, nb, rp, col, val, x, y, beta, inner_max, lds_cap);
+    return fail(SABLE_EINVAL, "bad dtype %d", (int)dt);
+}
+
+}  // namespace sable
+
+using namespace sable;
+
+extern "C" {
+
+SABLE_API int sable_csr_plan_create(sable_ctx_t* ctx, int32_t N, const int32_t* row_ptr,
+                                    const sable_csr_tune_t* tune, sable_csr_plan_t** out) {
+    SABLE_CHECK(ctx && out && row_ptr, SABLE_EINVAL, "sable_csr_plan_create: NULL argument");
+    SABLE_CHECK(N >= 0, SABLE_EINVAL, "negative N");
+    *out = nullptr;
+    int T, C;
+    tune_defaults(tune, &T, &C);
+    int64_t nnz = 0;
+    int rc = read_nnz(ctx, N, row_ptr, &nnz);
+    if (rc) return rc;
+    int64_t B = nblocks_for(nnz, N, T, C);
+    auto* p = new sable_csr_plan_s();
+    p->ctx = ctx;
+    p->N = N;
+    p->nnz = nnz;
+    p->row_ptr = row_ptr;
+    p->nblocks = B;
+    p->block_weight = T;
+    p->row_weight = C;
+    p->lds_cap = lds_cap_for(T);
+    if (hipMalloc(&p->blk, sizeof(int32_t) * (size_t)(B + 1) + 256) != hipSuccess) {
+        (void)hipGetLastError();
+        delete p;
+        return fail(SABLE_ENOMEM, "plan allocation failed");
+    }
+    p->owns_blk = true;
+    int* err = (int*)((char*)p->blk + sizeof(int32_t) * (size_t)(B + 1));
+    err = (int*)(((uintptr_t)err + 3) & ~(uintptr_t)3);
+    if (N > 0) {
+        rc = plan_fill(ctx, N, row_ptr, T, C, B, p->blk, err, true);
+        if (rc) {
+            (void)hipFree(p->blk);
+            delete p;
+            return rc;
+        }
+    }
+    *out = p;
+    return SABLE_OK;
+}
+
+SABLE_API void sable_csr_plan_destroy(sable_csr_plan_t* p) {
+    if (!p) return;
+    if (p->owns_blk && p->blk) {
+        (void)hipStreamSynchronize(p->ctx->stream);
+        (void)hipFree(p->blk);
+    }
+    delete p;
+}
+
+SABLE_API int sable_csr_plan_info(const sable_csr_plan_t* p, int64_t* nblocks, int64_t* nnz, int32_t* lds_cap) {
+    SABLE_CHECK(p, SABLE_EINVAL, "NULL plan");
+    if (nblocks) *nblocks = p->nblocks;
+    if (nnz) *nnz = p->nnz;
+    if (lds_cap) *lds_cap = p->lds_cap;
+    return SABLE_OK;
+}
+
+SABLE_API int sable_spmv_csr_plan(sable_ctx_t* ctx, const sable_csr_plan_t* p, sable_dtype_t dtype,
+                                  const int32_t* col, const void* val, const void* x, void* y, double beta,
+                                  int inner_max) {
+    SABLE_CHECK(ctx && p, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(inner_max >= 0, SABLE_EINVAL, "negative inner_max");
+    if (p->N == 0 || inner_max == 0) return SABLE_OK;
+    return dispatch_csr(ctx, dtype, p->blk, p->nblocks, p->row_ptr, col, val, x, y, beta, inner_max, p->lds_cap);
+}
+
+SABLE_API int sable_spmv_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                             const int32_t* col, const void* val, const void* x, void* y, double beta,
+                             int inner_max) {
+    SABLE_CHECK(ctx && row_ptr, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(N >= 0 && inner_max >= 0, SABLE_EINVAL, "negative size");
+    if (N == 0 || inner_max == 0) return SABLE_OK;
+    int T, C;
+    tune_defaults(nullptr, &T, &C);
+    int64_t nnz;
+    int rc = read_nnz(ctx, N, row_ptr, &nnz);
+    if (rc) return rc;
+    int64_t B = nblocks_for(nnz, N, T, C);
+    void* s;
+    rc = ctx_scratch(ctx, sizeof(int32_t) * (size_t)(B + 1) + 256, &s);
+    if (rc) return rc;
+    int32_t* blk = (int32_t*)s;
+    int* err = (int*)((char*)s + ((sizeof(int32_t) * (size_t)(B + 1) + 3) & ~(size_t)3));
+    rc = plan_fill(ctx, N, row_ptr, T, C, B, blk, err, false);
+    if (rc) return rc;
+    return dispatch_csr(ctx, dtype, blk, B, row_ptr, col, val, x, y, beta, inner_max, lds_cap_for(T));
+}
+
+SABLE_API int sable_spmv_coo(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int64_t nnz, const int32_t* row,
+                             const int32_t* col, const void* val, const void* x, void* y, double beta,
+                             int inner_max) {
+    SABLE_CHECK(ctx, SABLE_EINVAL, "NULL ctx");
+    SABLE_CHECK(N >= 0 && nnz >= 0 && inner_max >= 0, SABLE_EINVAL, "negative size");
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    if (N == 0 || inner_max == 0) return SABLE_OK;
+    int T, C;
+    tune_defaults(nullptr, &T, &C);
+    int64_t B = nblocks_for(nnz, N, T, C);
+    size_t rp_bytes = (sizeof(int32_t) * ((size_t)N + 1) + 255) & ~(size_t)255;
+    size_t blk_bytes = (sizeof(int32_t) * ((size_t)B + 1) + 255) & ~(size_t)255;
+    void* s;
+    int rc = ctx_scratch(ctx, rp_bytes + blk_bytes + 256, &s);
+    if (rc) return rc;
+    int32_t* rp = (int32_t*)s;
+    int32_t* blk = (int32_t*)((char*)s + rp_bytes);
+    int* err = (int*)((char*)s + rp_bytes + blk_bytes);
+    SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), ctx->stream));
+    if (nnz > 0) {
+        unsigned g = grid_for(nnz, BS);
+        if (g > 16384) g = 16384;
+        hipLaunchKernelGGL(k_coo_check, dim3(g), dim3(BS), 0, ctx->stream, nnz, N, row, err);
+        SABLE_LAUNCH_CHECK();
+    }
+    int64_t e;
+    SABLE_HIP(hipMemcpyAsync(ctx->pinned, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    SABLE_HIP(hipStreamSynchronize(ctx->stream));
+    e = *(int*)ctx->pinned;
+    SABLE_CHECK(!(e & 2), SABLE_EMALFORMED, "COO row index out of range");
+    if (e & 1) {
+        // Unsorted: atomics (order of accumulation is not deterministic).
+        unsigned g = grid_for(N, BS);
+        if (g > 16384) g = 16384;
+        unsigned gz = grid_for(nnz, BS);
+        if (gz > 65536) gz = 65536;
+        for (int it = 0; it < inner_max; it++) {
+            if (beta != 1.0) {
+                if (dtype == SABLE_F32)
+                    hipLaunchKernelGGL(k_scale<float>, dim3(g), dim3(BS), 0, ctx->stream, N, (float*)y, (float)beta);
+                else
+                    hipLaunchKernelGGL(k_scale<double>, dim3(g), dim3(BS), 0, ctx->stream, N, (double*)y, beta);
+            }
+            if (dtype == SABLE_F32)
+                hipLaunchKernelGGL(k_coo_atomic<float>, dim3(gz), dim3(BS), 0, ctx->stream, nnz, row, col,
+                                   (const float*)val, (const float*)x, (float*)y);
+            else
+                hipLaunchKernelGGL(k_coo_atomic<double>, dim3(gz), dim3(BS), 0, ctx->stream, nnz, row, col,
+                                   (const double*)val, (const double*)x, (double*)y);
+            SABLE_LAUNCH_CHECK();
+        }
+        return SABLE_OK;
+    }
+    hipLaunchKernelGGL(k_coo_heads, dim3(grid_for(nnz + 1, BS)), dim3(BS), 0, ctx->stream, nnz, N, row, rp);
+    SABLE_LAUNCH_CHECK();
+    rc = plan_fill(ctx, N, rp, T, C, B, blk, err, false);
+    if (rc) return rc;
+    return dispatch_csr(ctx, dtype, blk, B, rp, col, val, x, y, beta, inner_max, lds_cap_for(T));
+}
+
+SABLE_API int sable_spmv_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
+                             const int32_t* offset, const void* data, const void* x, void* y, double beta,
+                             int inner_max) {
+    SABLE_CHECK(ctx, SABLE_EINVAL, "NULL ctx");
+    SABLE_CHECK(N >= 0 && nd >= 0 && inner_max >= 0, SABLE_EINVAL, "negative size");
+    SABLE_CHECK(stride >= 0 && stride <= N, SABLE_EINVAL, "stride %d outside [0, N=%d]", stride, N);
+    if (N == 0 || inner_max == 0) return SABLE_OK;
+    unsigned g = grid_for(N, BS);
+    for (int it = 0; it < inner_max; it++) {
+        if (dtype == SABLE_F32)
+            hipLaunchKernelGGL(k_dia<float>, dim3(g), dim3(BS), 0, ctx->stream, N, nd, stride, offset,
+                               (const float*)data, (const float*)x, (float*)y, (float)beta);
+        else if (dtype == SABLE_F64)
+            hipLaunchKernelGGL(k_dia<double>, dim3(g), dim3(BS), 0, ctx->stream, N, nd, stride, offset,
+                               (const double*)data, (const double*)x, (double*)y, beta);
+        else
+            return fail(SABLE_EINVAL, "bad dtype");
+        SABLE_LAUNCH_CHECK();
+    }
+    return SABLE_OK;
+}
+
+SABLE_API int sable_spmv_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc, const int32_t* indices,
+                             const void* data, const void* x, void* y, double beta, int inner_max) {
+    SABLE_CHECK(ctx, SABLE_EINVAL, "NULL ctx");
+    SABLE_CHECK(N >= 0 && nc >= 0 && inner_max >= 0, SABLE_EINVAL, "negative size");
+    // The reference returns without touching y when nc or N is not positive.
+    if (N == 0 || nc == 0 || inner_max == 0) return SABLE_OK;
+    unsigned g = grid_for(N, BS);
+    for (int it = 0; it < inner_max; it++) {
+        if (dtype == SABLE_F32)
+            hipLaunchKernelGGL(k_ell<float>, dim3(g), dim3(BS), 0, ctx->stream, N, nc, indices,
+                               (const float*)data, (const float*)x, (float*)y, (float)beta);
+        else if (dtype == SABLE_F64)
+            hipLaunchKernelGGL(k_ell<double>, dim3(g), dim3(BS), 0, ctx->stream, N, nc, indices,
+                               (const double*)data, (const double*)x, (double*)y, beta);
+        else
+            return fail(SABLE_EINVAL, "bad dtype");
+        SABLE_LAUNCH_CHECK();
+    }
+    return SABLE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+"""SpMV dispatch, mirroring spmv() of seq/src/sparse32.js:1232-1283.
+
+``spmv(A, x, y, inner_max)`` computes y += A.x, inner_max times, on the GPU
+for every format; ``spmv(A, x, y, beta=0)`` is the plain product.
+"""
+import ctypes
+
+from . import _lib
+from ._lib import call, c_vp, c_i64, c_i32
+from .core import (sparse_COO_t, sparse_CSR_t, sparse_DIA_t, sparse_ELL_t, dtype_code)
+
+
+class CsrPlan:
+    """Device row-block schedule for a CSR matrix (sable_csr_plan_*)."""
+
+    def __init__(self, A, tune=None):
+        self.ctx = A.row.ctx
+        self.A = A
+        h = c_vp()
+        t = None
+        if tune is not None:
+            t = ctypes.byref(_lib.CsrTune(int(tune.get("block_weight", 0)), int(tune.get("row_weight", 0))))
+        call("sable_csr_plan_create", self.ctx.handle, A.N, A.row.ptr, t, ctypes.byref(h))
+        self.handle = h
+
+    def info(self):
+        nb, nnz, cap = c_i64(), c_i64(), c_i32()
+        call("sable_csr_plan_info", self.handle, ctypes.byref(nb), ctypes.byref(nnz), ctypes.byref(cap))
+        return {"nblocks": nb.value, "nnz": nnz.value, "lds_cap": cap.value}
+
+    def run(self, x, y, beta=1.0, inner_max=1):
+        A = self.A
+        call("sable_spmv_csr_plan", self.ctx.handle, self.handle, dtype_code(A.dtype), A.col.ptr, A.val.ptr,
+             x.ptr, y.ptr, float(beta), int(inner_max))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.load().sable_csr_plan_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def _check_vec(A, x, y):
+    if x.vec.dtype != A.dtype or y.vec.dtype != A.dtype:
+        raise TypeError("x, y must have the matrix dtype %s" % A.dtype)
+    if x.vec_nelem < A.N or y.vec_nelem < A.N:
+        raise ValueError("x and y need at least N=%d elements" % A.N)
+
+
+def spmv(A, x_view, y_view, inner_max=1, beta=1.0, use_plan=True):
+    """y = A.x + beta*y, inner_max times (beta=1: the reference's y += A.x)."""
+    if A is None:
+        print("spmv input : undefined")
+        return
+    _check_vec(A, x_view, y_view)
+    ctx = x_view.vec.ctx.handle
+    dt = dtype_code(A.dtype)
+    x, y = x_view.vec.ptr, y_view.vec.ptr
+    if isinstance(A, sparse_COO_t):
+        call("sable_spmv_coo", ctx, dt, A.N, A.nnz, A.row.ptr, A.col.ptr, A.val.ptr, x, y, float(beta),
+             int(inner_max))
+    elif isinstance(A, sparse_CSR_t):
+        if use_plan:
+            A.plan().run(x_view.vec, y_view.vec, beta, inner_max)
+        else:
+            call("sable_spmv_csr", ctx, dt, A.N, A.row.ptr, A.col.ptr, A.val.ptr, x, y, float(beta),
+                 int(inner_max))
+    elif isinstance(A, sparse_DIA_t):
+        call("sable_spmv_dia", ctx, dt, A.N, A.ndiags, A.stride, A.offset.ptr, A.data.ptr, x, y, float(beta),
+             int(inner_max))
+    elif isinstance(A, sparse_ELL_t):
+        call("sable_spmv_ell", ctx, dt, A.N, A.ncols, A.indices.ptr, A.data.ptr, x, y, float(beta),
+             int(inner_max))
+    else:
+        raise TypeError("unsupported matrix type %r" % type(A))

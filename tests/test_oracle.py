@@ -1,0 +1,305 @@
+"""Pins the CPU oracle (oracle/sable_oracle.c) against brute force, closed
+forms and order-sensitive known answers.  The reference ships no golden
+vectors for this path (its only checksums are for SuiteSparse matrices that
+are not in the repository), so these checks are what pin each restatement.
+CPU only."""
+import numpy as np
+import pytest
+
+import oracle
+from sciwasm.sparse_amd import gen
+
+
+def dense_from_csr(N, rp, col, val):
+    A = np.zeros((N, N), val.dtype)
+    for i in range(N):
+        for j in range(rp[i], rp[i + 1]):
+            A[i, col[j]] = val[j]          # a later duplicate overwrites
+    return A
+
+
+def dense_sum_from_csr(N, rp, col, val):
+    A = np.zeros((N, N), np.float64)
+    for i in range(N):
+        for j in range(rp[i], rp[i + 1]):
+            A[i, col[j]] += val[j]
+    return A
+
+
+def random_coo(N, nnz, seed, dtype=np.float64, dup=True):
+    rng = np.random.default_rng(seed)
+    row = rng.integers(0, N, nnz).astype(np.int32)
+    col = rng.integers(0, N, nnz).astype(np.int32)
+    if not dup:
+        key = np.unique(row.astype(np.int64) * N + col)
+        rng.shuffle(key)
+        row, col = (key // N).astype(np.int32), (key % N).astype(np.int32)
+    val = rng.integers(-9, 10, len(row)).astype(dtype)
+    return row, col, val
+
+
+# ------------------------------------------------------------------ coo_csr
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_coo_csr_is_stable_row_col_sort(dtype, seed):
+    N = 40
+    row, col, val = random_coo(N, 300, seed, dtype)
+    # tie-breaker: make duplicates distinguishable by value
+    val = np.arange(len(row)).astype(dtype)
+    rp, oc, ov = oracle.coo_csr(N, row, col, val)
+    order = np.lexsort((np.arange(len(row)), col, row))  # row, then col, then input position
+    assert np.array_equal(oc, col[order])
+    assert np.array_equal(ov, val[order])
+    assert rp[0] == 0 and rp[-1] == len(row)
+    assert np.array_equal(np.diff(rp), np.bincount(row, minlength=N))
+
+
+def test_coo_csr_empty_rows_and_single():
+    rp, oc, ov = oracle.coo_csr(5, np.array([3], np.int32), np.array([1], np.int32), np.array([2.5]))
+    assert rp.tolist() == [0, 0, 0, 0, 1, 1]
+    assert oc.tolist() == [1] and ov.tolist() == [2.5]
+
+
+# --------------------------------------------------------------- csr_ell/dia
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_csr_ell_dense_and_layout(dtype):
+    N = 30
+    rp, c, v = gen.random_csr(N, 4, seed=3, dtype=dtype)
+    nc, idx, data = oracle.csr_ell(N, rp, c, v)
+    assert nc == np.diff(rp).max()
+    D = np.zeros((N, N), dtype)
+    for k in range(nc):
+        for i in range(N):
+            D[i, idx[k * N + i]] += data[k * N + i]
+    assert np.array_equal(D, dense_from_csr(N, rp, c, v))
+    # slot k of row i holds the k-th stored entry; padding is (0, 0)
+    for i in range(N):
+        L = rp[i + 1] - rp[i]
+        assert np.array_equal(idx[np.arange(L) * N + i], c[rp[i]:rp[i + 1]])
+        assert np.all(idx[np.arange(L, nc) * N + i] == 0)
+        assert np.all(data[np.arange(L, nc) * N + i] == 0)
+
+
+def test_csr_dia_laplacian_closed_form():
+    n = 16
+    rp, c, v = gen.laplacian_2d(n)
+    N = n * n
+    nd, mn = oracle.num_diags(N, rp, c)
+    assert (nd, mn) == (5, 0)
+    nd, stride, off, data = oracle.csr_dia(N, rp, c, v)
+    assert off.tolist() == [-n, -1, 0, 1, n]
+    assert stride == N
+    D = data.reshape(nd, N)
+    assert np.all(D[2] == 4.0)
+    # row i of the offset -1 diagonal holds a(i, i-1): -1 unless i is at a grid row start
+    assert np.array_equal(D[1], np.where(np.arange(N) % n == 0, 0.0, -1.0))
+    assert np.array_equal(D[0], np.where(np.arange(N) < n, 0.0, -1.0))
+
+
+@pytest.mark.parametrize("stride_mode", ["N", "N-min"])
+def test_csr_dia_dense_roundtrip(stride_mode):
+    N = 50
+    rp, c, v = gen.band_csr(N, [-7, -3, 2, 5], seed=4)
+    nd, mn = oracle.num_diags(N, rp, c)
+    assert (nd, mn) == (4, 2)
+    stride = N if stride_mode == "N" else N - mn
+    nd, stride, off, data = oracle.csr_dia(N, rp, c, v, stride=stride)
+    D = np.zeros((N, N))
+    for k, o in enumerate(off):
+        for i in range(max(0, -o), min(N, N - o)):
+            D[i, i + o] = data[k * stride + i - (N - stride if o < 0 else 0)]
+    assert np.array_equal(D, dense_from_csr(N, rp, c, v))
+
+
+# ---------------------------------------------------------------- transposes
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_transpose_csr(dtype):
+    N = 37
+    rp, c, v = gen.random_csr(N, 5, seed=5, dtype=dtype)
+    trp, tc, tv = oracle.transpose_csr(N, rp, c, v)
+    assert np.array_equal(dense_from_csr(N, trp, tc, tv), dense_from_csr(N, rp, c, v).T)
+    for i in range(N):
+        assert np.all(np.diff(tc[trp[i]:trp[i + 1]]) > 0)
+    rrp, rc, rv = oracle.transpose_csr(N, trp, tc, tv)
+    assert np.array_equal(rrp, rp) and np.array_equal(rc, c) and np.array_equal(rv, v)
+
+
+def test_transpose_csr_duplicates_keep_row_order():
+    N = 4
+    rp = np.array([0, 2, 3, 3, 4], np.int32)
+    c = np.array([1, 1, 1, 1], np.int32)
+    v = np.array([10.0, 11.0, 20.0, 40.0])
+    trp, tc, tv = oracle.transpose_csr(N, rp, c, v)
+    assert trp.tolist() == [0, 0, 4, 4, 4]
+    assert tc.tolist() == [0, 0, 1, 3] and tv.tolist() == [10.0, 11.0, 20.0, 40.0]
+
+
+def test_transpose_dia():
+    N = 40
+    rp, c, v = gen.band_csr(N, [-5, 0, 1, 9], seed=6)
+    nd, stride, off, data = oracle.csr_dia(N, rp, c, v)
+    toff, tdata = oracle.transpose_dia(N, stride, off, data)
+    assert toff.tolist() == [-o for o in off]          # descending
+    D = np.zeros((N, N))
+    for k, o in enumerate(toff):
+        for i in range(max(0, -o), min(N, N - o)):
+            D[i, i + o] = tdata[k * stride + i]
+    assert np.array_equal(D, dense_from_csr(N, rp, c, v).T)
+
+
+def test_transpose_ell_drops_zeros():
+    N = 25
+    rp, c, v = gen.random_csr(N, 3, seed=7)
+    v[::4] = 0.0
+    nc, idx, data = oracle.csr_ell(N, rp, c, v)
+    nco, tidx, tdata = oracle.transpose_ell(N, nc, idx, data)
+    A = dense_from_csr(N, rp, c, v)
+    assert nco == (A != 0).sum(0).max()
+    D = np.zeros((N, N))
+    for k in range(nco):
+        for i in range(N):
+            if tdata[k * N + i] != 0:
+                D[i, tidx[k * N + i]] = tdata[k * N + i]
+    assert np.array_equal(D, A.T)
+    for i in range(N):   # new row entries ascend by column
+        cols = [tidx[k * N + i] for k in range(nco) if tdata[k * N + i] != 0]
+        assert cols == sorted(cols)
+
+
+# ---------------------------------------------------------------- level sets
+def test_level_sets_laplacian_3d_closed_form():
+    n = 6
+    rp, c, v = gen.laplacian_3d_lower(n)
+    N = n ** 3
+    nl, lp, perm, order, lvl = oracle.level_sets(N, rp, c)
+    i = np.arange(N)
+    gx, gy, gz = i % n, (i // n) % n, i // (n * n)
+    assert nl == 3 * (n - 1) + 1
+    assert np.array_equal(lvl, gx + gy + gz)
+    sizes = np.bincount(gx + gy + gz)
+    assert np.array_equal(np.diff(lp), sizes)
+    # trinomial count of (i,j,k) in [0,n)^3 with i+j+k = l
+    for l in range(nl):
+        cnt = sum(1 for a in range(n) for b in range(n) if 0 <= l - a - b < n)
+        assert sizes[l] == cnt
+    assert np.array_equal(perm, np.lexsort((i, lvl)))
+    assert np.array_equal(order[perm], np.arange(N))
+
+
+def test_level_sets_random_lower_invariants():
+    N = 300
+    rp, c, v = gen.random_lower(N, 8, seed=8)
+    nl, lp, perm, order, lvl = oracle.level_sets(N, rp, c)
+    for i in range(N):
+        deps = c[rp[i]:rp[i + 1] - 1]
+        if len(deps) == 0:
+            assert lvl[i] == 0
+        else:
+            assert lvl[i] == 1 + lvl[deps].max()
+    assert sorted(perm.tolist()) == list(range(N))
+    nrp, nc_, nv = oracle.level_reorder(N, rp, c, v, perm, order)
+    # reordered matrix is the permuted one: P A P^T
+    A = dense_from_csr(N, rp, c, v)
+    B = dense_from_csr(N, nrp, nc_, nv)
+    assert np.array_equal(B, A[np.ix_(perm, perm)])
+
+
+def test_level_sets_empty_row_is_level_one():
+    rp = np.array([0, 1, 1, 3], np.int32)
+    c = np.array([0, 0, 2], np.int32)
+    nl, lp, perm, order, lvl = oracle.level_sets(3, rp, c)
+    assert lvl.tolist() == [0, 1, 1] and nl == 2 and lp.tolist() == [0, 1, 3]
+
+
+# ---------------------------------------------------------------------- spmv
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_spmv_exact_integer_all_formats(dtype):
+    N = 60
+    rp, c, v = gen.random_csr(N, 6, seed=9, dtype=dtype)
+    v = np.round(v * 4).astype(dtype)
+    x = np.random.default_rng(1).integers(-5, 6, N).astype(dtype)
+    y0 = np.random.default_rng(2).integers(-5, 6, N).astype(dtype)
+    exact = dense_sum_from_csr(N, rp, c, v) @ x.astype(np.float64) + y0
+    assert np.array_equal(oracle.spmv_csr(N, rp, c, v, x, y0), exact.astype(dtype))
+    row = gen.csr_to_coo_rows(rp)
+    assert np.array_equal(oracle.spmv_coo(row, c, v, x, y0), exact.astype(dtype))
+    nc, idx, data = oracle.csr_ell(N, rp, c, v)
+    assert np.array_equal(oracle.spmv_ell(N, nc, idx, data, x, y0), exact.astype(dtype))
+    nd, stride, off, dd = oracle.csr_dia(N, rp, c, v)
+    assert np.array_equal(oracle.spmv_dia(N, stride, off, dd, x, y0), exact.astype(dtype))
+    # inner_max repeats y += A.x
+    y3 = oracle.spmv_csr(N, rp, c, v, x, np.zeros(N, dtype), inner_max=3)
+    assert np.array_equal(y3, (3 * (exact - y0)).astype(dtype))
+
+
+@pytest.mark.parametrize("dtype,big", [(np.float64, 1e16), (np.float32, 1e8)])
+def test_spmv_sequential_order(dtype, big):
+    # y = ((0 + big) + 1) + (-big): sequential order loses the 1; exact is 1.
+    rp = np.array([0, 3], np.int32)
+    c = np.array([0, 0, 0], np.int32)
+    v = np.array([big, 1.0, -big], dtype)
+    x = np.ones(1, dtype)
+    assert oracle.spmv_csr(1, rp, c, v, x, np.zeros(1, dtype))[0] == 0.0
+    assert oracle.spmv_coo(np.zeros(3, np.int32), c, v, x, np.zeros(1, dtype))[0] == 0.0
+    v2 = np.array([1.0, big, -big], dtype)       # (1 + big) - big = 0 too, but
+    v3 = np.array([big, -big, 1.0], dtype)       # (big - big) + 1 = 1
+    assert oracle.spmv_csr(1, rp, c, v2, x, np.zeros(1, dtype))[0] == 0.0
+    assert oracle.spmv_csr(1, rp, c, v3, x, np.zeros(1, dtype))[0] == 1.0
+
+
+def test_spmv_laplacian_closed_forms():
+    n = 32
+    rp, c, v = gen.laplacian_2d(n)
+    N = n * n
+    y = oracle.spmv_csr(N, rp, c, v, np.ones(N), np.zeros(N))
+    i = np.arange(N)
+    nb = (i % n > 0).astype(int) + (i % n < n - 1) + (i // n > 0) + (i // n < n - 1)
+    assert np.array_equal(y, 4.0 - nb)
+    y = oracle.spmv_csr(N, rp, c, v, i.astype(np.float64), np.zeros(N))
+    interior = (i % n > 0) & (i % n < n - 1) & (i // n > 0) & (i // n < n - 1)
+    assert np.all(y[interior] == 0)
+
+
+def test_spmv_fp64_oracle():
+    N = 50
+    rp, c, v = gen.random_csr(N, 5, seed=10)
+    x = np.random.default_rng(3).uniform(-1, 1, N)
+    y, ab = oracle.spmv_csr_fp64(N, rp, c, v, x)
+    assert np.allclose(y, dense_sum_from_csr(N, rp, c, v) @ x, rtol=1e-14, atol=1e-14)
+    assert np.all(ab >= np.abs(y) - 1e-15)
+
+
+# ---------------------------------------------------------------------- spts
+def test_spts_manufactured_exact():
+    N = 200
+    rng = np.random.default_rng(11)
+    rp, c, v = gen.random_lower(N, 6, seed=11)
+    v = np.where(c == np.repeat(np.arange(N), np.diff(rp)), 1.0, rng.integers(-2, 3, len(c))).astype(np.float64)
+    xs = rng.integers(-3, 4, N).astype(np.float64)
+    b = dense_sum_from_csr(N, rp, c, v) @ xs
+    assert np.array_equal(oracle.spts_syncfree(N, rp, c, v, b), xs)
+    assert np.array_equal(oracle.spts_fp64(N, rp, c, v, b), xs)
+    nl, lp, perm, order, lvl = oracle.level_sets(N, rp, c)
+    nrp, ncc, nv = oracle.level_reorder(N, rp, c, v, perm, order)
+    yl = oracle.spts_level(N, nrp, ncc, nv, b[perm])
+    assert np.array_equal(yl, xs[perm])
+    assert oracle.rel_residual(N, rp, c, v, xs, b) == 0.0
+
+
+def test_spts_summation_orders_differ_as_specified():
+    # row 2: (b - (0 + 1e16*y0 + 1*y1)) / 1  vs  ((b - 1e16*y0) - 1*y1) / 1
+    rp = np.array([0, 1, 2, 5], np.int32)
+    c = np.array([0, 1, 0, 1, 2], np.int32)
+    v = np.array([1.0, 1.0, 1e16, 1.0, 1.0])
+    b = np.array([1.0, 1.0, 1e16])
+    ys = oracle.spts_syncfree(3, rp, c, v, b)
+    assert ys.tolist() == [1.0, 1.0, -1.0]           # (1e16 - 1e16) - 1
+    yl = oracle.spts_level(3, rp, c, v, b)
+    assert yl.tolist() == [1.0, 1.0, 0.0]            # 1e16 - (1e16 + 1) rounds to 0
+
+
+def test_fletcher_known_answers():
+    assert oracle.fletcher_sum(np.array([1, 2, 3], np.float32)) == 2566.0
+    y = np.array([300.0, -1.5], np.float64)
+    # s1 = 45, s2 = 45; s1 = 43.5, s2 = 88.5 -> 88.5*256 + 43.5 = 22699.5 -> 22699
+    assert oracle.fletcher_sum(y) == 22699.0
