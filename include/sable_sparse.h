@@ -132,6 +132,140 @@ SABLE_API int sable_spmv_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, i
                              const int32_t* indices, const void* data, const void* x, void* y,
                              double beta, int inner_max);
 
+
+/* ------------------------------------------------------------ conversions */
+/* All conversions are bit-identical to the reference algorithms.  The
+ * reference's storage guards (nc*N < 2^27 and nc*N/nnz <= 5 for ELL,
+ * nd*stride < 2^27 and nd*stride/nnz <= 5 for DIA) live in the caller, as
+ * in the reference's JS; these entry points convert whatever they are given. */
+
+/* COO -> CSR: rows and, within a row, columns ascending; equal (row, col)
+ * pairs keep their input order (a stable sort; duplicates are not summed).
+ * out_row_ptr has N+1 entries.  Replaces coo_csr (seq/src/sparse32.js:
+ * 1674-1720, seq/src/sparse64.js:1255; par/src/sparse32.js:2096-2145). */
+SABLE_API int sable_coo_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int64_t nnz, const int32_t* row,
+                            const int32_t* col, const void* val, int32_t* out_row_ptr, int32_t* out_col,
+                            void* out_val);
+
+/* Longest row (synchronises).  Replaces num_cols (seq/src/sparse32.js:1436-1447). */
+SABLE_API int sable_csr_num_cols(sable_ctx_t* ctx, int32_t N, const int32_t* row_ptr, int32_t* nc);
+
+/* CSR -> column-major ELL of width nc (padding value 0, index 0).
+ * Replaces csr_ell (seq/src/sparse32.js:1449-1484) and csr_ell_col
+ * (par/src/sparse32.js:1796-1836). */
+SABLE_API int sable_csr_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc, const int32_t* row_ptr,
+                            const int32_t* col, const void* val, int32_t* out_indices, void* out_data);
+
+/* Number of occupied diagonals and the smallest |offset| among them
+ * (synchronises).  Replaces num_diags (seq/src/sparse32.js:1486-1512;
+ * seq/src/sparse64.js:1066-1092 uses stride = N - min|offset|). */
+SABLE_API int sable_csr_num_diags(sable_ctx_t* ctx, int32_t N, const int32_t* row_ptr, const int32_t* col,
+                                  int32_t* nd, int32_t* min_abs);
+
+/* CSR -> DIA: out_offset ascending (nd entries), out_data[k*stride + i -
+ * (offset<0 ? N-stride : 0)] = a(i, i+offset[k]), zero padding; a later
+ * duplicate (i, c) overwrites an earlier one.  Replaces csr_dia
+ * (seq/src/sparse32.js:1515-1574, seq/src/sparse64.js:1095-1157) and
+ * csr_dia_col (par/src/sparse32.js:1976-2040). */
+SABLE_API int sable_csr_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
+                            const int32_t* row_ptr, const int32_t* col, const void* val, int32_t* out_offset,
+                            void* out_data);
+
+/* CSR transpose (counting sort by column, rows ascending within each new
+ * row; values moved bitwise).  Replaces transpose_csr
+ * (seq/src/sparse_opt_32.wat:6348-6477, caller seq/src/sparse32.js:1116-1120). */
+SABLE_API int sable_transpose_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                                  const int32_t* col, const void* val, int32_t* out_row_ptr, int32_t* out_col,
+                                  void* out_val);
+
+/* DIA transpose, stride == N: out_offset[k] = -offset[k] (descending),
+ * each stored run re-indexed by its new row; other slots zero.  Replaces
+ * transpose_dia (seq/src/sparse_opt_32.wat:6479-6572). */
+SABLE_API int sable_transpose_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
+                                  const int32_t* offset, const void* data, int32_t* out_offset, void* out_data);
+
+/* Width of the transposed ELL: the largest per-column count of slots whose
+ * value compares > 0 or < 0 (zeros, -0 and NaN are dropped) (synchronises).
+ * Replaces count_transpose_cols_ell (seq/src/sparse_opt_32.wat:6574-6632). */
+SABLE_API int sable_count_transpose_cols_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc,
+                                             const void* data, const int32_t* indices, int32_t* nc_out);
+
+/* ELL transpose into width nc_out: new row c lists input rows j ascending
+ * (then slots ascending); padding zero.  Replaces transpose_ell
+ * (seq/src/sparse_opt_32.wat:6634-6701). */
+SABLE_API int sable_transpose_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc, int32_t nc_out,
+                                  const void* data, const int32_t* indices, void* out_data, int32_t* out_indices);
+
+/* ------------------------------------------------------------ triangular solve */
+/* L.x = b for a lower-triangular CSR whose rows each end with the diagonal
+ * (the reference's layout; par/src/spts32.js:14-136 builds L = tril(A,-1)+I
+ * that way).  lanes = lanes per row (1, 4, 8, 16 or 64; 0 = chosen from the
+ * mean row length).  With lanes = 1 both solves keep the reference's per-row
+ * operation order and are bit-identical to it. */
+typedef struct sable_spts_s sable_spts_t;
+
+/* Level-set analysis: checks the structure (SABLE_EMALFORMED) and the
+ * diagonal (SABLE_ESINGULAR), computes level[i], level_ptr, the stable
+ * level permutation and order, and the level-ordered CSR (all bit-identical
+ * to CSR_create_level_sets_with_reorder, par/src/sparse32.js:2942-3181,
+ * par/src/spts32.js:158-357).  The caller's arrays must outlive the
+ * analysis (the sync-free solve and the refinement read them). */
+SABLE_API int sable_spts_analyze(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                                 const int32_t* col, const void* val, int lanes, sable_spts_t** out);
+SABLE_API void sable_spts_destroy(sable_spts_t* a);
+SABLE_API int sable_spts_info(const sable_spts_t* a, int32_t* nlevels, int32_t* max_width, int32_t* lanes);
+SABLE_API int sable_spts_set_lanes(sable_spts_t* a, int lanes);
+/* Copies of the schedule (any pointer may be NULL): h_level_ptr is a host
+ * array of nlevels+1 entries, the others are device arrays. */
+SABLE_API int sable_spts_get_schedule(sable_ctx_t* ctx, const sable_spts_t* a, int32_t* h_level_ptr,
+                                      int32_t* d_permutation, int32_t* d_order, int32_t* d_row_ptr, int32_t* d_col,
+                                      void* d_val);
+
+/* Level-set solve with b and x in natural order: one launch per level,
+ * captured once in a hipGraph and replayed.  Replaces spts_level_csr
+ * (par/src/sparse_opt_32.wat:11759-11984 with spts_csr :11679-11757) and
+ * the level + sync-free combined solve (:12846-13353). */
+SABLE_API int sable_spts_solve_level(sable_ctx_t* ctx, sable_spts_t* a, const void* b, void* x);
+
+/* Sync-free solve, natural order, one launch.  Replaces
+ * spts_csr_sync_free(_wrapper) (par/src/spts_opt_32.wat:8-260). */
+SABLE_API int sable_spts_solve_syncfree(sable_ctx_t* ctx, sable_spts_t* a, const void* b, void* x);
+
+/* Synchronises; SABLE_ETIMEOUT if a sync-free solve of this analysis hit
+ * its spin limit since the last call. */
+SABLE_API int sable_spts_status(sable_ctx_t* ctx, sable_spts_t* a);
+
+/* FP32 solve plus `iters` steps of FP64 iterative refinement on an FP64
+ * analysis: r = b - L.x in FP64, L32.d = r in FP32 (inner 0 = level-set,
+ * 1 = sync-free), x += d in FP64.  No reference counterpart (the
+ * reference has no mixed-precision solve). */
+SABLE_API int sable_spts_solve_refined(sable_ctx_t* ctx, sable_spts_t* a, const double* b, double* x, int iters,
+                                       int inner);
+
+/* Reference-shaped level-set solve on an already level-ordered matrix with
+ * level-ordered b and y; h_level_ptr on the host.  One launch per level.
+ * Replaces spts_level_csr (par/src/sparse_opt_32.wat:11759-11984). */
+SABLE_API int sable_spts_level_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nlevels,
+                                   const int32_t* h_level_ptr, const int32_t* row_ptr, const int32_t* col,
+                                   const void* val, const void* b, void* y, int lanes);
+
+/* Reference-shaped sync-free solve on a natural-order matrix; checks the
+ * structure first; every wait is bounded by spin_limit polls without
+ * progress (0 = default); synchronises and returns SABLE_ETIMEOUT when the
+ * bound was hit.  Replaces spts_csr_sync_free_wrapper
+ * (par/src/spts_opt_32.wat:119-260). */
+SABLE_API int sable_spts_syncfree_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                                      const int32_t* col, const void* val, const void* b, void* x, int lanes,
+                                      long long spin_limit);
+
+/* ------------------------------------------------------------ building blocks */
+/* No reference counterpart: device-wide primitives the conversions are
+ * built from, exported for callers that assemble their own formats. */
+/* out[0..n) = exclusive prefix sum of in[0..n); out[n] = total (out has n+1). */
+SABLE_API int sable_scan_exclusive_i32(sable_ctx_t* ctx, const int32_t* in, int32_t* out, int64_t n);
+/* Stable sort of (keys, vals) by key bits [0, end_bit), in place. */
+SABLE_API int sable_sort_pairs_u64(sable_ctx_t* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int end_bit);
+
 #ifdef __cplusplus
 }
 #endif

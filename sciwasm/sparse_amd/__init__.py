@@ -11,6 +11,9 @@ from .core import (Context, DeviceArray, sparse_COO_t, sparse_CSR_t, sparse_DIA_
                    sparse_vec_t, coo_from_host, csr_from_host, dia_from_host, ell_from_host, create_vec,
                    vec_from_host, clear_vec, init_x, fletcher_sum, fletcher_sum_y)
 from .spmv import spmv, CsrPlan  # noqa: F401
+from .convert import coo_csr, num_cols, csr_ell, num_diags, csr_dia, transpose  # noqa: F401
+from .spts import (SptsAnalysis, CSR_create_level_sets, CSR_create_level_sets_with_reorder,  # noqa: F401
+                   spts_level_csr, spts_syncfree)
 
 __all__ = [
     "Context", "DeviceArray", "SableError",
@@ -18,4 +21,7 @@ __all__ = [
     "coo_from_host", "csr_from_host", "dia_from_host", "ell_from_host",
     "create_vec", "vec_from_host", "clear_vec", "init_x", "fletcher_sum", "fletcher_sum_y",
     "spmv", "CsrPlan",
+    "coo_csr", "num_cols", "csr_ell", "num_diags", "csr_dia", "transpose",
+    "SptsAnalysis", "CSR_create_level_sets", "CSR_create_level_sets_with_reorder", "spts_level_csr",
+    "spts_syncfree",
 ]

@@ -65,11 +65,34 @@ SIGNATURES = {
     "sable_spmv_coo": [c_vp, c_int, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_spmv_dia": [c_vp, c_int, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_spmv_ell": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
+    "sable_coo_csr": [c_vp, c_int, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "sable_csr_num_cols": [c_vp, c_i32, c_vp, ctypes.POINTER(c_i32)],
+    "sable_csr_ell": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "sable_csr_num_diags": [c_vp, c_i32, c_vp, c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)],
+    "sable_csr_dia": [c_vp, c_int, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "sable_transpose_csr": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "sable_transpose_dia": [c_vp, c_int, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp],
+    "sable_count_transpose_cols_ell": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, ctypes.POINTER(c_i32)],
+    "sable_transpose_ell": [c_vp, c_int, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp],
+    "sable_spts_analyze": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_int, c_vpp],
+    "sable_spts_destroy": [c_vp],
+    "sable_spts_info": [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)],
+    "sable_spts_set_lanes": [c_vp, c_int],
+    "sable_spts_get_schedule": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "sable_spts_solve_level": [c_vp, c_vp, c_vp, c_vp],
+    "sable_spts_solve_syncfree": [c_vp, c_vp, c_vp, c_vp],
+    "sable_spts_status": [c_vp, c_vp],
+    "sable_spts_solve_refined": [c_vp, c_vp, c_vp, c_vp, c_int, c_int],
+    "sable_spts_level_csr": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int],
+    "sable_spts_syncfree_csr": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, ctypes.c_longlong],
+    "sable_scan_exclusive_i32": [c_vp, c_vp, c_vp, c_i64],
+    "sable_sort_pairs_u64": [c_vp, c_vp, c_vp, c_i64, c_int],
 }
 
 _RESTYPES = {
     "sable_ctx_destroy": None,
     "sable_csr_plan_destroy": None,
+    "sable_spts_destroy": None,
     "sable_ctx_stream": c_vp,
     "sable_last_error": ctypes.c_char_p,
     "sable_version": ctypes.c_char_p,

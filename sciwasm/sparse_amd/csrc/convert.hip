@@ -1,0 +1,652 @@
+// convert.hip -- coo_csr, csr_ell, csr_dia and the transposes on the GPU.
+//
+// Results are bit-identical to the reference algorithms:
+//  * coo_csr (seq/src/sparse32.js:1674-1720): the reference's counting
+//    scatter followed by a stable per-row bubble sort by column is a stable
+//    sort of the entries by (row, col) with ties in input order; here a
+//    stable radix sort of the key row<<colbits|col over (input position).
+//  * transpose_csr (seq/src/sparse_opt_32.wat:6348-6477): the counting
+//    scatter in row order is a stable sort of the row-major entries by
+//    column; here a stable radix sort by column.
+//  * csr_ell / csr_dia: one thread per row writes the row's slots in order
+//    (a later duplicate (i,c) overwrites an earlier one in DIA, as in the
+//    reference's sequential loop).
+//  * transpose_ell: nonzero slots enumerated row-major (row j, then slot),
+//    compacted with a scan, then stably sorted by column.
+#include "common.h"
+#include "prims.h"
+
+#include <cstring>
+
+namespace sable {
+
+constexpr int CB = 256;
+
+__global__ void k_coo_keys(int64_t nnz, int32_t N, int colbits, const int32_t* __restrict__ row,
+                           const int32_t* __restrict__ col, uint64_t* __restrict__ keys, uint32_t* __restrict__ idx,
+                           int* __restrict__ err) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
+        const int r = row[k], c = col[k];
+        if (r < 0 || r >= N || c < 0 || c >= N) atomicOr(err, 1);
+        keys[k] = ((uint64_t)(uint32_t)r << colbits) | (uint32_t)c;
+        idx[k] = (uint32_t)k;
+    }
+}
+
+// rp[r] = first sorted position whose row (key >> shift) is >= r.
+__global__ void k_heads_from_keys(int64_t n, int32_t N, int shift, const uint64_t* __restrict__ keys,
+                                  int32_t* __restrict__ rp) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n) return;
+    const int64_t lo = k == 0 ? 0 : (int64_t)(keys[k - 1] >> shift) + 1;
+    const int64_t hi = k == n ? N : (int64_t)(keys[k] >> shift);
+    for (int64_t r = lo; r <= hi; r++) rp[r] = (int32_t)k;
+}
+
+template <typename T>
+__global__ void k_gather_coo(int64_t nnz, uint64_t mask, const uint64_t* __restrict__ keys,
+                             const uint32_t* __restrict__ idx, const T* __restrict__ val, int32_t* __restrict__ ocol,
+                             T* __restrict__ oval) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+        ocol[p] = (int32_t)(keys[p] & mask);
+        oval[p] = val[idx[p]];
+    }
+}
+
+// Row of every stored entry (one thread per row).
+__global__ void k_expand_rows(int32_t N, const int32_t* __restrict__ rp, int32_t* __restrict__ rows) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t base = rp[0];
+        for (int64_t j = rp[i]; j < rp[i + 1]; j++) rows[j - base] = (int32_t)i;
+    }
+}
+
+__global__ void k_col_keys(int64_t nnz, int32_t N, const int32_t* __restrict__ col, uint64_t* __restrict__ keys,
+                           uint32_t* __restrict__ idx, int* __restrict__ err) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
+        const int c = col[k];
+        if (c < 0 || c >= N) atomicOr(err, 1);
+        keys[k] = (uint32_t)c;
+        idx[k] = (uint32_t)k;
+    }
+}
+
+template <typename T>
+__global__ void k_gather_t(int64_t nnz, const uint32_t* __restrict__ idx, const int32_t* __restrict__ rows,
+                           const T* __restrict__ val, int32_t* __restrict__ ocol, T* __restrict__ oval) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t s = idx[p];
+        ocol[p] = rows[s];
+        oval[p] = val[s];
+    }
+}
+
+__global__ void k_max_row_len(int32_t N, const int32_t* __restrict__ rp, int* __restrict__ out) {
+    int m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const int len = rp[i + 1] - rp[i];
+        if (len > m) m = len;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        int o = __shfl_xor(m, off, WAVE);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(out, m);
+}
+
+template <typename T>
+__global__ void k_csr_ell(int32_t N, int32_t nc, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                          const T* __restrict__ val, int32_t* __restrict__ oidx, T* __restrict__ odata,
+                          int* __restrict__ err) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t a = rp[i];
+        const int len = rp[i + 1] - (int)a;
+        if (len > nc) atomicOr(err, 1);
+        for (int k = 0; k < nc; k++) {
+            const int64_t e = (int64_t)k * N + i;
+            if (k < len) {
+                oidx[e] = col[a + k];
+                odata[e] = val[a + k];
+            } else {
+                oidx[e] = 0;
+                odata[e] = T(0);
+            }
+        }
+    }
+}
+
+// Diagonal histogram: ind[N-1 + c - i] counts entries of diagonal c - i.
+__global__ void k_diag_hist(int32_t N, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                            int32_t* __restrict__ ind, int* __restrict__ err) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        for (int64_t j = rp[i]; j < rp[i + 1]; j++) {
+            const int c = col[j];
+            if (c < 0 || c >= N) {
+                atomicOr(err, 1);
+                continue;
+            }
+            atomicAdd(&ind[(int64_t)N - 1 + c - i], 1);
+        }
+    }
+}
+
+// flags[d] = ind[d] != 0; nd and min |offset| by atomics.
+__global__ void k_diag_flags(int64_t nd2, int32_t N, const int32_t* __restrict__ ind, int32_t* __restrict__ flags,
+                             int* __restrict__ stats /* [0]=count, [1]=min_abs */) {
+    int cnt = 0, mn = 0x7fffffff;
+    for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < nd2; d += (int64_t)gridDim.x * blockDim.x) {
+        const int f = ind[d] != 0;
+        if (flags) flags[d] = f;
+        if (f) {
+            cnt++;
+            int a = (int)(d - (N - 1));
+            a = a < 0 ? -a : a;
+            mn = a < mn ? a : mn;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        cnt += __shfl_xor(cnt, off, WAVE);
+        int o = __shfl_xor(mn, off, WAVE);
+        mn = o < mn ? o : mn;
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        if (cnt) atomicAdd(&stats[0], cnt);
+        atomicMin(&stats[1], mn);
+    }
+}
+
+__global__ void k_diag_offsets(int64_t nd2, int32_t N, const int32_t* __restrict__ flags,
+                               const int32_t* __restrict__ pos, int32_t* __restrict__ offset) {
+    for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < nd2; d += (int64_t)gridDim.x * blockDim.x)
+        if (flags[d]) offset[pos[d]] = (int32_t)(d - (N - 1));
+}
+
+template <typename T>
+__global__ void k_csr_dia(int32_t N, int32_t stride, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                          const T* __restrict__ val, const int32_t* __restrict__ pos, T* __restrict__ data) {
+    const int64_t shiftneg = (int64_t)N - stride;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        for (int64_t j = rp[i]; j < rp[i + 1]; j++) {
+            const int c = col[j];
+            if (c < 0 || c >= N) continue;
+            const int off = c - (int)i;
+            const int64_t k = pos[(int64_t)N - 1 + off];
+            const int64_t slot = i - (off < 0 ? shiftneg : 0);
+            if (slot >= 0 && slot < stride) data[k * stride + slot] = val[j];
+        }
+    }
+}
+
+__global__ void k_set4(int* p, int a, int b, int c, int d) {
+    p[0] = a;
+    p[1] = b;
+    p[2] = c;
+    p[3] = d;
+}
+
+template <typename T>
+__global__ void k_fill_t(T* __restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = T(0);
+}
+
+// transpose_dia with stride == N: out_offset[k] = -offset[k]; the run of
+// diagonal k moves from slot n to slot n + offset[k].
+template <typename T>
+__global__ void k_transpose_dia(int32_t N, int32_t nd, const int32_t* __restrict__ offset, const T* __restrict__ in,
+                                int32_t* __restrict__ out_offset, T* __restrict__ out) {
+    const int64_t total = (int64_t)nd * N;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(e / N);
+        const int64_t n = e - (int64_t)k * N;
+        const int o = offset[k];
+        if (n == 0) out_offset[k] = -o;
+        const int64_t m = n + o;
+        if (m >= 0 && m < N) out[(int64_t)k * N + m] = in[e];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ bool nonzero(T v) {
+    return v > T(0) || v < T(0);
+}
+
+template <typename T>
+__global__ void k_ell_col_counts(int32_t N, int32_t nc, const T* __restrict__ data, const int32_t* __restrict__ idx,
+                                 int32_t* __restrict__ cnt, int* __restrict__ err) {
+    const int64_t total = (int64_t)nc * N;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        if (nonzero(data[e])) {
+            const int c = idx[e];
+            if (c < 0 || c >= N) {
+                atomicOr(err, 1);
+                continue;
+            }
+            atomicAdd(&cnt[c], 1);
+        }
+    }
+}
+
+__global__ void k_max_i32(int64_t n, const int32_t* __restrict__ a, int* __restrict__ out) {
+    int m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = a[i] > m ? a[i] : m;
+    for (int off = 32; off > 0; off >>= 1) {
+        int o = __shfl_xor(m, off, WAVE);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(out, m);
+}
+
+// flags in row-major (row j, slot s) order: f[j*nc + s].
+template <typename T>
+__global__ void k_ell_nz_flags(int32_t N, int32_t nc, const T* __restrict__ data, int32_t* __restrict__ f) {
+    const int64_t total = (int64_t)nc * N;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = q / nc, s = q - j * nc;
+        f[q] = nonzero(data[s * N + j]) ? 1 : 0;
+    }
+}
+
+template <typename T>
+__global__ void k_ell_compact(int32_t N, int32_t nc, const T* __restrict__ data, const int32_t* __restrict__ idx,
+                              const int32_t* __restrict__ pos, uint64_t* __restrict__ keys,
+                              uint32_t* __restrict__ src) {
+    const int64_t total = (int64_t)nc * N;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = q / nc, s = q - j * nc;
+        const int64_t e = s * N + j;
+        if (nonzero(data[e])) {
+            const int32_t p = pos[q];
+            keys[p] = (uint32_t)idx[e];
+            src[p] = (uint32_t)e;
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_ell_place(int64_t M, int32_t N, int32_t nco, const uint64_t* __restrict__ keys,
+                            const uint32_t* __restrict__ src, const int32_t* __restrict__ starts,
+                            const T* __restrict__ data, T* __restrict__ odata, int32_t* __restrict__ oidx) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < M; p += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)keys[p];
+        const int64_t r = p - starts[c];
+        if (r < nco) {
+            const uint32_t e = src[p];
+            odata[r * N + c] = data[e];
+            oidx[r * N + c] = (int32_t)(e % (uint32_t)N);
+        }
+    }
+}
+
+static unsigned gs(int64_t n) {
+    unsigned g = grid_for(n, CB);
+    return g > 32768 ? 32768 : g;
+}
+
+static int read_err(sable_ctx_t* ctx, int* err_dev, int* out) {
+    SABLE_HIP(hipMemcpyAsync(ctx->pinned, err_dev, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    SABLE_HIP(hipStreamSynchronize(ctx->stream));
+    *out = *(int*)ctx->pinned;
+    return SABLE_OK;
+}
+
+static int read_i32(sable_ctx_t* ctx, const int32_t* dev, int count, int32_t* out) {
+    SABLE_HIP(hipMemcpyAsync(ctx->pinned, dev, sizeof(int32_t) * count, hipMemcpyDeviceToHost, ctx->stream));
+    SABLE_HIP(hipStreamSynchronize(ctx->stream));
+    memcpy(out, ctx->pinned, sizeof(int32_t) * count);
+    return SABLE_OK;
+}
+
+template <typename T>
+static int coo_csr_t(sable_ctx_t* ctx, int32_t N, int64_t nnz, const int32_t* row, const int32_t* col, const void* val,
+                     int32_t* orp, int32_t* ocol, void* oval) {
+    hipStream_t st = ctx->stream;
+    if (nnz == 0) {
+        hipLaunchKernelGGL(k_fill_t<int32_t>, dim3(gs(N + 1)), dim3(CB), 0, st, orp, (int64_t)N + 1);
+        SABLE_LAUNCH_CHECK();
+        return SABLE_OK;
+    }
+    SABLE_CHECK(nnz < 0xffffffffLL, SABLE_EINVAL, "nnz too large");
+    const int colbits = bits_for(N > 0 ? N - 1 : 0);
+    const int rowbits = bits_for(N > 0 ? N - 1 : 0);
+    size_t need = Carver::round(8 * nnz) + Carver::round(4 * nnz) + 256 + sort_tmp_bytes(nnz);
+    void* s;
+    int rc = ctx_scratch(ctx, need, &s);
+    if (rc) return rc;
+    Carver cv(s);
+    uint64_t* keys = cv.take<uint64_t>(nnz);
+    uint32_t* idx = cv.take<uint32_t>(nnz);
+    int* err = cv.take<int>(1);
+    void* stmp = cv.p + cv.used;
+    SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_coo_keys, dim3(gs(nnz)), dim3(CB), 0, st, nnz, N, colbits, row, col, keys, idx, err);
+    SABLE_LAUNCH_CHECK();
+    int e = 0;
+    rc = read_err(ctx, err, &e);
+    if (rc) return rc;
+    SABLE_CHECK(!e, SABLE_EMALFORMED, "COO index outside [0, N)");
+    rc = sort_pairs_u64(st, keys, idx, nnz, colbits + rowbits, stmp);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_heads_from_keys, dim3(grid_for(nnz + 1, CB)), dim3(CB), 0, st, nnz, N, colbits, keys, orp);
+    SABLE_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gather_coo<T>, dim3(gs(nnz)), dim3(CB), 0, st, nnz, ((uint64_t)1 << colbits) - 1, keys, idx,
+                       (const T*)val, ocol, (T*)oval);
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+template <typename T>
+static int transpose_csr_t(sable_ctx_t* ctx, int32_t N, const int32_t* rp, const int32_t* col, const void* val,
+                           int32_t* orp, int32_t* ocol, void* oval) {
+    hipStream_t st = ctx->stream;
+    int32_t ends[2];
+    int rc = read_i32(ctx, rp, 1, &ends[0]);
+    if (rc) return rc;
+    rc = read_i32(ctx, rp + N, 1, &ends[1]);
+    if (rc) return rc;
+    const int64_t nnz = (int64_t)ends[1] - ends[0];
+    SABLE_CHECK(nnz >= 0, SABLE_EMALFORMED, "row_ptr[N] < row_ptr[0]");
+    if (nnz == 0) {
+        hipLaunchKernelGGL(k_fill_t<int32_t>, dim3(gs(N + 1)), dim3(CB), 0, st, orp, (int64_t)N + 1);
+        SABLE_LAUNCH_CHECK();
+        return SABLE_OK;
+    }
+    const int colbits = bits_for(N > 0 ? N - 1 : 0);
+    size_t need = Carver::round(8 * nnz) + Carver::round(4 * nnz) * 2 + 256 + sort_tmp_bytes(nnz);
+    void* s;
+    rc = ctx_scratch(ctx, need, &s);
+    if (rc) return rc;
+    Carver cv(s);
+    uint64_t* keys = cv.take<uint64_t>(nnz);
+    uint32_t* idx = cv.take<uint32_t>(nnz);
+    int32_t* rows = cv.take<int32_t>(nnz);
+    int* err = cv.take<int>(1);
+    void* stmp = cv.p + cv.used;
+    SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
+    const int32_t* colb = col + ends[0];
+    const T* valb = (const T*)val + ends[0];
+    hipLaunchKernelGGL(k_expand_rows, dim3(gs(N)), dim3(CB), 0, st, N, rp, rows);
+    hipLaunchKernelGGL(k_col_keys, dim3(gs(nnz)), dim3(CB), 0, st, nnz, N, colb, keys, idx, err);
+    SABLE_LAUNCH_CHECK();
+    int e = 0;
+    rc = read_err(ctx, err, &e);
+    if (rc) return rc;
+    SABLE_CHECK(!e, SABLE_EMALFORMED, "column index outside [0, N)");
+    rc = sort_pairs_u64(st, keys, idx, nnz, colbits, stmp);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_heads_from_keys, dim3(grid_for(nnz + 1, CB)), dim3(CB), 0, st, nnz, N, 0, keys, orp);
+    hipLaunchKernelGGL(k_gather_t<T>, dim3(gs(nnz)), dim3(CB), 0, st, nnz, idx, rows, valb, ocol, (T*)oval);
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+}  // namespace sable
+
+using namespace sable;
+
+#define DT_DISPATCH(dtype, FN, ...)                                                   \
+    ((dtype) == SABLE_F32 ? FN<float>(__VA_ARGS__)                                   \
+                          : (dtype) == SABLE_F64 ? FN<double>(__VA_ARGS__)           \
+                                                 : fail(SABLE_EINVAL, "bad dtype %d", (int)(dtype)))
+
+extern "C" {
+
+SABLE_API int sable_coo_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int64_t nnz, const int32_t* row,
+                            const int32_t* col, const void* val, int32_t* out_row_ptr, int32_t* out_col,
+                            void* out_val) {
+    SABLE_CHECK(ctx && out_row_ptr, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(N >= 0 && nnz >= 0, SABLE_EINVAL, "negative size");
+    SABLE_CHECK(nnz <= 0x7fffffffLL, SABLE_EINVAL, "nnz exceeds the int32 row_ptr range");
+    return DT_DISPATCH(dtype, coo_csr_t, ctx, N, nnz, row, col, val, out_row_ptr, out_col, out_val);
+}
+
+SABLE_API int sable_transpose_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                                  const int32_t* col, const void* val, int32_t* out_row_ptr, int32_t* out_col,
+                                  void* out_val) {
+    SABLE_CHECK(ctx && row_ptr && out_row_ptr, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(N >= 0, SABLE_EINVAL, "negative size");
+    if (N == 0) return SABLE_OK;
+    return DT_DISPATCH(dtype, transpose_csr_t, ctx, N, row_ptr, col, val, out_row_ptr, out_col, out_val);
+}
+
+SABLE_API int sable_csr_num_cols(sable_ctx_t* ctx, int32_t N, const int32_t* row_ptr, int32_t* nc) {
+    SABLE_CHECK(ctx && row_ptr && nc, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(N >= 0, SABLE_EINVAL, "negative size");
+    *nc = 0;
+    if (N == 0) return SABLE_OK;
+    void* s;
+    int rc = ctx_scratch(ctx, 256, &s);
+    if (rc) return rc;
+    int* out = (int*)s;
+    SABLE_HIP(hipMemsetAsync(out, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_max_row_len, dim3(gs(N)), dim3(CB), 0, ctx->stream, N, row_ptr, out);
+    SABLE_LAUNCH_CHECK();
+    return read_i32(ctx, out, 1, nc);
+}
+
+SABLE_API int sable_csr_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc, const int32_t* row_ptr,
+                            const int32_t* col, const void* val, int32_t* out_indices, void* out_data) {
+    SABLE_CHECK(ctx && row_ptr, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(N >= 0 && nc >= 0, SABLE_EINVAL, "negative size");
+    if (N == 0 || nc == 0) return SABLE_OK;
+    void* s;
+    int rc = ctx_scratch(ctx, 256, &s);
+    if (rc) return rc;
+    int* err = (int*)s;
+    SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), ctx->stream));
+    if (dtype == SABLE_F32)
+        hipLaunchKernelGGL(k_csr_ell<float>, dim3(gs(N)), dim3(CB), 0, ctx->stream, N, nc, row_ptr, col,
+                           (const float*)val, out_indices, (float*)out_data, err);
+    else if (dtype == SABLE_F64)
+        hipLaunchKernelGGL(k_csr_ell<double>, dim3(gs(N)), dim3(CB), 0, ctx->stream, N, nc, row_ptr, col,
+                           (const double*)val, out_indices, (double*)out_data, err);
+    else
+        return fail(SABLE_EINVAL, "bad dtype");
+    SABLE_LAUNCH_CHECK();
+    int e = 0;
+    rc = read_err(ctx, err, &e);
+    if (rc) return rc;
+    SABLE_CHECK(!e, SABLE_EINVAL, "a row is longer than nc=%d", nc);
+    return SABLE_OK;
+}
+
+static int diag_hist(sable_ctx_t* ctx, int32_t N, const int32_t* rp, const int32_t* col, int32_t** ind_out,
+                     int32_t** flags_out, int32_t** pos_out, int** stats_out, size_t extra) {
+    const int64_t nd2 = 2 * (int64_t)N - 1;
+    size_t need = Carver::round(4 * (nd2 + 1)) * 3 + 512 + scan_tmp_bytes(nd2 + 1) + extra;
+    void* s;
+    int rc = ctx_scratch(ctx, need, &s);
+    if (rc) return rc;
+    Carver cv(s);
+    int32_t* ind = cv.take<int32_t>(nd2 + 1);
+    int32_t* flags = cv.take<int32_t>(nd2 + 1);
+    int32_t* pos = cv.take<int32_t>(nd2 + 1);
+    int* stats = cv.take<int>(4);
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(k_fill_t<int32_t>, dim3(gs(nd2 + 1)), dim3(CB), 0, st, ind, nd2 + 1);
+    hipLaunchKernelGGL(k_set4, dim3(1), dim3(1), 0, st, stats, 0, 0x7fffffff, 0, 0);
+    hipLaunchKernelGGL(k_diag_hist, dim3(gs(N)), dim3(CB), 0, st, N, rp, col, ind, stats + 2);
+    hipLaunchKernelGGL(k_diag_flags, dim3(gs(nd2)), dim3(CB), 0, st, nd2, N, ind, flags, stats);
+    SABLE_LAUNCH_CHECK();
+    *ind_out = ind;
+    *flags_out = flags;
+    *pos_out = pos;
+    *stats_out = stats;
+    // scan needs its own scratch after the carved buffers
+    return SABLE_OK;
+}
+
+SABLE_API int sable_csr_num_diags(sable_ctx_t* ctx, int32_t N, const int32_t* row_ptr, const int32_t* col,
+                                  int32_t* nd, int32_t* min_abs) {
+    SABLE_CHECK(ctx && row_ptr && nd, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(N >= 0, SABLE_EINVAL, "negative size");
+    *nd = 0;
+    if (min_abs) *min_abs = N > 0 ? N - 1 : 0;
+    if (N == 0) return SABLE_OK;
+    int32_t *ind, *flags, *pos;
+    int* stats;
+    int rc = diag_hist(ctx, N, row_ptr, col, &ind, &flags, &pos, &stats, 0);
+    if (rc) return rc;
+    int32_t h[4];
+    rc = read_i32(ctx, stats, 4, h);
+    if (rc) return rc;
+    SABLE_CHECK(!h[2], SABLE_EMALFORMED, "column index outside [0, N)");
+    *nd = h[0];
+    if (min_abs) *min_abs = h[0] ? h[1] : N - 1;
+    return SABLE_OK;
+}
+
+SABLE_API int sable_csr_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
+                            const int32_t* row_ptr, const int32_t* col, const void* val, int32_t* out_offset,
+                            void* out_data) {
+    SABLE_CHECK(ctx && row_ptr, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(N >= 0 && nd >= 0 && stride >= 0 && stride <= N, SABLE_EINVAL, "bad sizes");
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    if (N == 0) return SABLE_OK;
+    int32_t *ind, *flags, *pos;
+    int* stats;
+    const int64_t nd2 = 2 * (int64_t)N - 1;
+    int rc = diag_hist(ctx, N, row_ptr, col, &ind, &flags, &pos, &stats, 0);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    void* stmp = (char*)stats + 256;
+    rc = scan_exclusive_i32(st, flags, pos, nd2, nullptr, stmp);
+    if (rc) return rc;
+    int32_t h[4];
+    rc = read_i32(ctx, stats, 4, h);
+    if (rc) return rc;
+    SABLE_CHECK(!h[2], SABLE_EMALFORMED, "column index outside [0, N)");
+    SABLE_CHECK(h[0] == nd, SABLE_EINVAL, "nd=%d but the matrix occupies %d diagonals", nd, h[0]);
+    if (nd > 0 && stride < N) SABLE_CHECK(N - stride <= h[1], SABLE_EINVAL, "stride %d < N - min|offset|", stride);
+    hipLaunchKernelGGL(k_diag_offsets, dim3(gs(nd2)), dim3(CB), 0, st, nd2, N, flags, pos, out_offset);
+    const int64_t total = (int64_t)nd * stride;
+    if (dtype == SABLE_F32) {
+        hipLaunchKernelGGL(k_fill_t<float>, dim3(gs(total)), dim3(CB), 0, st, (float*)out_data, total);
+        hipLaunchKernelGGL(k_csr_dia<float>, dim3(gs(N)), dim3(CB), 0, st, N, stride, row_ptr, col, (const float*)val,
+                           pos, (float*)out_data);
+    } else {
+        hipLaunchKernelGGL(k_fill_t<double>, dim3(gs(total)), dim3(CB), 0, st, (double*)out_data, total);
+        hipLaunchKernelGGL(k_csr_dia<double>, dim3(gs(N)), dim3(CB), 0, st, N, stride, row_ptr, col,
+                           (const double*)val, pos, (double*)out_data);
+    }
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+SABLE_API int sable_transpose_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
+                                  const int32_t* offset, const void* data, int32_t* out_offset, void* out_data) {
+    SABLE_CHECK(ctx, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(N >= 0 && nd >= 0, SABLE_EINVAL, "negative size");
+    SABLE_CHECK(stride == N, SABLE_EINVAL, "transpose_dia needs stride == N (got %d, N=%d)", stride, N);
+    if (N == 0 || nd == 0) return SABLE_OK;
+    const int64_t total = (int64_t)nd * N;
+    hipStream_t st = ctx->stream;
+    if (dtype == SABLE_F32) {
+        hipLaunchKernelGGL(k_fill_t<float>, dim3(gs(total)), dim3(CB), 0, st, (float*)out_data, total);
+        hipLaunchKernelGGL(k_transpose_dia<float>, dim3(gs(total)), dim3(CB), 0, st, N, nd, offset,
+                           (const float*)data, out_offset, (float*)out_data);
+    } else if (dtype == SABLE_F64) {
+        hipLaunchKernelGGL(k_fill_t<double>, dim3(gs(total)), dim3(CB), 0, st, (double*)out_data, total);
+        hipLaunchKernelGGL(k_transpose_dia<double>, dim3(gs(total)), dim3(CB), 0, st, N, nd, offset,
+                           (const double*)data, out_offset, (double*)out_data);
+    } else {
+        return fail(SABLE_EINVAL, "bad dtype");
+    }
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+SABLE_API int sable_count_transpose_cols_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc,
+                                             const void* data, const int32_t* indices, int32_t* nc_out) {
+    SABLE_CHECK(ctx && nc_out, SABLE_EINVAL, "NULL argument");
+    *nc_out = 0;
+    if (!(nc > 0 && N > 0)) return SABLE_OK;
+    void* s;
+    int rc = ctx_scratch(ctx, Carver::round(4 * (size_t)N) + 512, &s);
+    if (rc) return rc;
+    Carver cv(s);
+    int32_t* cnt = cv.take<int32_t>(N);
+    int* st4 = cv.take<int>(2);
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(k_fill_t<int32_t>, dim3(gs(N)), dim3(CB), 0, st, cnt, (int64_t)N);
+    hipLaunchKernelGGL(k_fill_t<int32_t>, dim3(1), dim3(CB), 0, st, (int32_t*)st4, (int64_t)2);
+    if (dtype == SABLE_F32)
+        hipLaunchKernelGGL(k_ell_col_counts<float>, dim3(gs((int64_t)nc * N)), dim3(CB), 0, st, N, nc,
+                           (const float*)data, indices, cnt, st4 + 1);
+    else if (dtype == SABLE_F64)
+        hipLaunchKernelGGL(k_ell_col_counts<double>, dim3(gs((int64_t)nc * N)), dim3(CB), 0, st, N, nc,
+                           (const double*)data, indices, cnt, st4 + 1);
+    else
+        return fail(SABLE_EINVAL, "bad dtype");
+    hipLaunchKernelGGL(k_max_i32, dim3(gs(N)), dim3(CB), 0, st, (int64_t)N, cnt, st4);
+    SABLE_LAUNCH_CHECK();
+    int32_t h[2];
+    rc = read_i32(ctx, (int32_t*)st4, 2, h);
+    if (rc) return rc;
+    SABLE_CHECK(!h[1], SABLE_EMALFORMED, "ELL index outside [0, N)");
+    *nc_out = h[0];
+    return SABLE_OK;
+}
+
+}  // extern "C"
+
+namespace sable {
+template <typename T>
+static int transpose_ell_t(sable_ctx_t* ctx, int32_t N, int32_t nc, int32_t nco, const void* data,
+                           const int32_t* indices, void* out_data, int32_t* out_indices) {
+    hipStream_t st = ctx->stream;
+    const int64_t total = (int64_t)nc * N;
+    SABLE_CHECK(total < 0x7fffffffLL, SABLE_EINVAL, "ELL too large for int32 positions");
+    const int64_t outn = (int64_t)nco * N;
+    hipLaunchKernelGGL(k_fill_t<T>, dim3(gs(outn)), dim3(CB), 0, st, (T*)out_data, outn);
+    hipLaunchKernelGGL(k_fill_t<int32_t>, dim3(gs(outn)), dim3(CB), 0, st, out_indices, outn);
+    SABLE_LAUNCH_CHECK();
+    if (total == 0 || nco == 0) return SABLE_OK;
+    size_t need = Carver::round(4 * (size_t)(total + 1)) * 2 + Carver::round(8 * (size_t)total) +
+                  Carver::round(4 * (size_t)total) + Carver::round(4 * ((size_t)N + 1)) + 512 +
+                  scan_tmp_bytes(total + 1) + sort_tmp_bytes(total);
+    void* s;
+    int rc = ctx_scratch(ctx, need, &s);
+    if (rc) return rc;
+    Carver cv(s);
+    int32_t* f = cv.take<int32_t>(total + 1);
+    int32_t* pos = cv.take<int32_t>(total + 1);
+    uint64_t* keys = cv.take<uint64_t>(total);
+    uint32_t* src = cv.take<uint32_t>(total);
+    int32_t* starts = cv.take<int32_t>((size_t)N + 1);
+    int32_t* Mdev = cv.take<int32_t>(1);
+    void* tmp = cv.p + cv.used;
+    hipLaunchKernelGGL(k_ell_nz_flags<T>, dim3(gs(total)), dim3(CB), 0, st, N, nc, (const T*)data, f);
+    SABLE_LAUNCH_CHECK();
+    rc = scan_exclusive_i32(st, f, pos, total, Mdev, tmp);
+    if (rc) return rc;
+    int32_t M;
+    rc = read_i32(ctx, Mdev, 1, &M);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_ell_compact<T>, dim3(gs(total)), dim3(CB), 0, st, N, nc, (const T*)data, indices, pos, keys,
+                       src);
+    SABLE_LAUNCH_CHECK();
+    rc = sort_pairs_u64(st, keys, src, M, bits_for(N - 1), tmp);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_heads_from_keys, dim3(grid_for((int64_t)M + 1, CB)), dim3(CB), 0, st, (int64_t)M, N, 0, keys,
+                       starts);
+    hipLaunchKernelGGL(k_ell_place<T>, dim3(gs(M)), dim3(CB), 0, st, (int64_t)M, N, nco, keys, src, starts,
+                       (const T*)data, (T*)out_data, out_indices);
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+}  // namespace sable
+
+extern "C" {
+
+SABLE_API int sable_transpose_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc, int32_t nc_out,
+                                  const void* data, const int32_t* indices, void* out_data, int32_t* out_indices) {
+    SABLE_CHECK(ctx, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(N >= 0 && nc >= 0 && nc_out >= 0, SABLE_EINVAL, "negative size");
+    if (N == 0) return SABLE_OK;
+    return DT_DISPATCH(dtype, transpose_ell_t, ctx, N, nc, nc_out, data, indices, out_data, out_indices);
+}
+
+}  // extern "C"

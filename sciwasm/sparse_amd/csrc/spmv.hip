@@ -178,8 +178,10 @@ __global__ __launch_bounds__(BS) void k_dia(int32_t N, int32_t nd, int32_t strid
         for (int k = 0; k < nd; k++) {
             const int off = k < 1024 ? offs[k] : offset[k];
             const int64_t c = n + off;
-            if (c >= 0 && c < N) {
-                const T d = data[(int64_t)k * stride + n - (off < 0 ? shiftneg : 0)];
+            const int64_t slot = n - (off < 0 ? shiftneg : 0);
+            // entries outside the stored run of this diagonal are not read
+            if (c >= 0 && c < N && slot >= 0 && slot < stride) {
+                const T d = data[(int64_t)k * stride + slot];
                 t = add_rn(mul_rn(d, x[c]), t);
             }
         }

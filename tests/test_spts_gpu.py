@@ -1,0 +1,178 @@
+"""GPU lower triangular solve against the oracle.
+
+* Level schedule (level_ptr, permutation, order, reordered CSR): bit-exact.
+* lanes = 1 keeps the reference's per-row operation order, so the level-set
+  solve is bit-identical to spts_csr/spts_level_csr and the sync-free solve
+  to spts_csr_sync_free (both restated in the oracle).
+* lanes > 1 reassociates each row's dot product: the result must satisfy
+  the acceptance bound of the FP64 path, relative residual
+  ||b - L x||_inf / (||L||_inf ||x||_inf + ||b||_inf) <= 1e-12 against the
+  long-double oracle (FP32 solves: <= 1e-5).
+* FP32 solve + FP64 refinement must reach the same 1e-12 bound."""
+import numpy as np
+import pytest
+
+import oracle
+import sciwasm.sparse_amd as sp
+from sciwasm.sparse_amd import gen, SableError
+
+pytestmark = pytest.mark.gpu
+
+MATS = {
+    "lap3d_16": lambda: gen.laplacian_3d_lower(16),
+    "random_lower_20k": lambda: gen.random_lower(20000, 32, seed=3),
+    "random_lower_sparse": lambda: gen.random_lower(5000, 2, seed=4),
+}
+
+
+def host_level_solution(N, rp, c, v, b):
+    nl, lp, perm, order, lvl = oracle.level_sets(N, rp, c)
+    nrp, nc_, nv = oracle.level_reorder(N, rp, c, v, perm, order)
+    yl = oracle.spts_level(N, nrp, nc_, nv, b[perm])
+    x = np.empty_like(yl)
+    x[perm] = yl
+    return x
+
+
+@pytest.mark.parametrize("name", list(MATS))
+def test_schedule_bit_exact(ctx, name):
+    rp, c, v = MATS[name]()
+    N = len(rp) - 1
+    an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v))
+    lp, perm, order, nrp, ncol, nval = an.schedule()
+    enl, elp, eperm, eorder, _ = oracle.level_sets(N, rp, c)
+    assert an.nlevels == enl
+    assert np.array_equal(lp, elp) and np.array_equal(perm, eperm) and np.array_equal(order, eorder)
+    erp, ec, ev = oracle.level_reorder(N, rp, c, v, eperm, eorder)
+    assert np.array_equal(nrp, erp) and np.array_equal(ncol, ec) and np.array_equal(nval, ev)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("name", list(MATS))
+def test_lanes1_bit_exact(ctx, dtype, name):
+    rp, c, v = MATS[name]()
+    v = v.astype(dtype)
+    N = len(rp) - 1
+    b = np.random.default_rng(1).uniform(-1, 1, N).astype(dtype)
+    L = sp.csr_from_host(N, rp, c, v)
+    an = sp.SptsAnalysis(L, lanes=1)
+    bv = sp.vec_from_host(b)
+    xl = sp.create_vec(N, 0.0, dtype)
+    an.solve_level(bv, xl)
+    assert np.array_equal(xl.array(), host_level_solution(N, rp, c, v, b))
+    xs = sp.create_vec(N, 0.0, dtype)
+    an.solve_syncfree(bv, xs)
+    an.status()
+    assert np.array_equal(xs.array(), oracle.spts_syncfree(N, rp, c, v, b))
+    xr = sp.create_vec(N, 0.0, dtype)
+    sp.spts_syncfree(L, bv, xr, lanes=1)
+    assert np.array_equal(xr.array(), xs.array())
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16, 64])
+@pytest.mark.parametrize("name", list(MATS))
+def test_multilane_residual(ctx, lanes, name):
+    rp, c, v = MATS[name]()
+    N = len(rp) - 1
+    b = np.random.default_rng(2).uniform(-1, 1, N)
+    an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v), lanes=lanes)
+    bv = sp.vec_from_host(b)
+    for method in ("level", "syncfree"):
+        x = sp.create_vec(N, 0.0, np.float64)
+        getattr(an, "solve_" + method)(bv, x)
+        an.status()
+        r = oracle.rel_residual(N, rp, c, v, x.array(), b)
+        assert r <= 1e-12, (method, r)
+
+
+def test_graph_replays_match_eager(ctx):
+    rp, c, v = gen.random_lower(30000, 16, seed=5)
+    N = len(rp) - 1
+    an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v), lanes=1)
+    rng = np.random.default_rng(6)
+    for k in range(4):
+        b = rng.uniform(-1, 1, N)
+        x = sp.create_vec(N, 0.0, np.float64)
+        an.solve_level(sp.vec_from_host(b), x)
+        # eager work between the replays
+        sp.spmv(sp.csr_from_host(N, rp, c, v), sp.vec_from_host(b), sp.create_vec(N, 1.0, np.float64))
+        assert np.array_equal(x.array(), host_level_solution(N, rp, c, v, b)), k
+
+
+def test_reference_shaped_level_solve(ctx):
+    rp, c, v = gen.laplacian_3d_lower(12)
+    N = len(rp) - 1
+    A = sp.CSR_create_level_sets_with_reorder(sp.csr_from_host(N, rp, c, v), lanes=1)
+    enl, elp, eperm, eorder, _ = oracle.level_sets(N, rp, c)
+    assert A.nlevels == enl and np.array_equal(A.level, elp)
+    b = np.random.default_rng(7).uniform(-1, 1, N)
+    bl = sp.vec_from_host(b[eperm])
+    yl = sp.create_vec(N, 0.0, np.float64)
+    sp.spts_level_csr(A, bl, yl, lanes=1)
+    erp, ec, ev = oracle.level_reorder(N, rp, c, v, eperm, eorder)
+    assert np.array_equal(yl.array(), oracle.spts_level(N, erp, ec, ev, b[eperm]))
+
+
+@pytest.mark.parametrize("inner", ["syncfree", "level"])
+@pytest.mark.parametrize("name", ["lap3d_16", "random_lower_20k"])
+def test_fp32_refinement_reaches_1e12(ctx, inner, name):
+    rp, c, v = MATS[name]()
+    N = len(rp) - 1
+    b = np.random.default_rng(8).uniform(-1, 1, N)
+    an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v))
+    x = sp.create_vec(N, 0.0, np.float64)
+    an.solve_refined(sp.vec_from_host(b), x, iters=0, inner=inner)
+    r0 = oracle.rel_residual(N, rp, c, v, x.array(), b)
+    assert r0 < 1e-5
+    an.solve_refined(sp.vec_from_host(b), x, iters=2, inner=inner)
+    an.status()
+    r2 = oracle.rel_residual(N, rp, c, v, x.array(), b)
+    assert r2 <= 1e-12, (r0, r2)
+
+
+def test_fp32_solve_residual(ctx):
+    rp, c, v = gen.random_lower(20000, 32, seed=9, dtype=np.float32)
+    N = len(rp) - 1
+    b = np.random.default_rng(10).uniform(-1, 1, N).astype(np.float32)
+    an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v), lanes=8)
+    x = sp.create_vec(N, 0.0, np.float32)
+    an.solve_syncfree(sp.vec_from_host(b), x)
+    assert oracle.rel_residual(N, rp, c, v.astype(np.float64), x.array().astype(np.float64),
+                               b.astype(np.float64)) <= 1e-5
+
+
+def test_malformed_and_singular(ctx):
+    rp = np.array([0, 1, 3, 4], np.int32)
+    c = np.array([0, 1, 0, 2], np.int32)        # row 1: diagonal not last
+    L = sp.csr_from_host(3, rp, c, np.ones(4))
+    with pytest.raises(SableError) as e:
+        sp.SptsAnalysis(L)
+    assert e.value.name == "SABLE_EMALFORMED"
+    with pytest.raises(SableError) as e:
+        sp.spts_syncfree(L, sp.create_vec(3, 1.0, np.float64), sp.create_vec(3, 0.0, np.float64))
+    assert e.value.name == "SABLE_EMALFORMED"
+    rp = np.array([0, 1, 3], np.int32)
+    c = np.array([0, 0, 1], np.int32)
+    with pytest.raises(SableError) as e:
+        sp.SptsAnalysis(sp.csr_from_host(2, rp, c, np.array([1.0, 2.0, 0.0])))
+    assert e.value.name == "SABLE_ESINGULAR"
+
+
+def test_forced_spin_limit_reports_timeout(ctx):
+    # a 4096-long dependency chain: every row waits on the previous one, so a
+    # spin limit of one poll without progress must trip; the kernel exits.
+    N = 4096
+    rp = np.concatenate([[0, 1], 1 + 2 * np.arange(1, N)]).astype(np.int32)
+    c = np.empty(2 * N - 1, np.int32)
+    c[0] = 0
+    c[1::2] = np.arange(N - 1)
+    c[2::2] = np.arange(1, N)
+    v = np.ones(2 * N - 1)
+    L = sp.csr_from_host(N, rp, c, v)
+    b = sp.create_vec(N, 1.0, np.float64)
+    with pytest.raises(SableError) as e:
+        sp.spts_syncfree(L, b, sp.create_vec(N, 0.0, np.float64), lanes=1, spin_limit=1)
+    assert e.value.name == "SABLE_ETIMEOUT"
+    x = sp.create_vec(N, 0.0, np.float64)
+    sp.spts_syncfree(L, b, x, lanes=1)          # the default bound solves it
+    assert np.array_equal(x.array(), oracle.spts_syncfree(N, rp, c, v, np.ones(N)))
