@@ -258,6 +258,117 @@ SABLE_API int sable_spts_syncfree_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int
                                       const int32_t* col, const void* val, const void* b, void* x, int lanes,
                                       long long spin_limit);
 
+/* ------------------------------------------------------------ batches */
+/* No reference counterpart (the reference runs one matrix per call): many
+ * independent matrices in one persistent launch.  Item arrays live in
+ * DEVICE memory; items are ordered by ascending blk_begin / row_begin. */
+
+/* Device row-block boundaries of a CSR plan (nblocks+1 entries) for
+ * building batch items. */
+SABLE_API int sable_csr_plan_blocks(const sable_csr_plan_t* plan, const int32_t** d_blk, int64_t* nblocks);
+
+typedef struct {
+    const int32_t* row_ptr;
+    const int32_t* col;
+    const void* val;
+    const int32_t* blk;  /* the matrix's plan blocks (sable_csr_plan_blocks) */
+    const void* x;       /* [N][nrhs] row-major */
+    void* y;             /* [N][nrhs] row-major */
+    int64_t blk_begin;   /* first global block of this item */
+    double beta;
+} sable_batch_spmv_item_t;
+
+/* y_k = A_k.x_k + beta_k*y_k for every item, nrhs in {1, 2, 4, 8} right-hand
+ * sides, over total_blocks = sum of the items' block counts; workgroups = 0
+ * picks a persistent grid of 4 per CU. */
+SABLE_API int sable_batch_spmv(sable_ctx_t* ctx, sable_dtype_t dtype, int nrhs, const sable_batch_spmv_item_t* d_items,
+                               int nitems, int64_t total_blocks, int workgroups);
+
+typedef struct {
+    const int32_t* row_ptr;  /* lower CSR, diagonal last in each row */
+    const int32_t* col;
+    const void* val;
+    const void* b;
+    void* x;
+    int64_t row_begin;  /* first global row of this item */
+    int32_t N;
+    int32_t pad;
+} sable_batch_spts_item_t;
+
+/* Sync-free solve of every item in one launch (synchronises; SABLE_ETIMEOUT
+ * when a wait exceeded its bound). */
+SABLE_API int sable_batch_spts(sable_ctx_t* ctx, sable_dtype_t dtype, int lanes, const sable_batch_spts_item_t* d_items,
+                               int nitems, int64_t total_rows);
+
+typedef struct {
+    const int32_t* row_ptr;
+    const int32_t* col;
+    const double* val64;
+    const float* val32;
+    const double* b64;
+    double* x64;
+    float* r32;  /* N floats of work space */
+    float* d32;  /* N floats of work space */
+    int64_t row_begin;
+    int32_t N;
+    int32_t pad;
+} sable_batch_refine_item_t;
+
+/* FP32 sync-free solve + `iters` FP64 refinement steps for every item
+ * (synchronises). */
+SABLE_API int sable_batch_spts_refined(sable_ctx_t* ctx, int lanes, const sable_batch_refine_item_t* d_items,
+                                       int nitems, int64_t total_rows, int iters);
+
+/* ------------------------------------------------------------ serving */
+/* No reference counterpart: a request server over resident matrices.
+ * Matrices and their SpMV plans (and FP32 copies) stay in HBM and are
+ * charged against hbm_budget; requests queue on the host and are batched
+ * continuously into `slots` in-flight batches (see mode). */
+typedef struct sable_server_s sable_server_t;
+typedef struct {
+    int64_t hbm_budget;   /* bytes for matrices + batch arenas (0 = unlimited) */
+    int mode;             /* 0 plain FP64, one launch per request; 1 batched FP64;
+                             2 batched, FP32 SpMV and FP32 solve + FP64 refinement */
+    int max_batch;        /* requests per batch (0 = 256) */
+    int slots;            /* batches in flight (0 = 3) */
+    int64_t arena_bytes;  /* device vector arena per slot (0 = 256 MiB) */
+    int refine_iters;     /* refinement steps in mode 2 */
+    int spts_lanes;       /* lanes per row of the solves (0 = 4) */
+} sable_server_config_t;
+
+SABLE_API int sable_server_create(sable_ctx_t* ctx, const sable_server_config_t* cfg, sable_server_t** out);
+SABLE_API void sable_server_destroy(sable_server_t* s);
+/* Host CSR arrays (row_ptr[0] = 0); kind 0 = general (SpMV), 1 = lower
+ * triangular with the diagonal last (SpMV and solves).  SABLE_EBUDGET when
+ * the matrix does not fit the remaining HBM budget. */
+SABLE_API int sable_server_add_matrix(sable_server_t* s, int32_t N, const int32_t* row_ptr, const int32_t* col,
+                                      const double* val, int kind, int32_t* matrix_id);
+/* op 0: out = A.in; op 1: solve A.out = in.  Host vectors of N doubles;
+ * `in` must stay valid until the request completes. */
+SABLE_API int sable_server_submit(sable_server_t* s, int32_t matrix_id, int op, const double* in, double* out,
+                                  int64_t* request_id);
+/* Blocks until the request is done; returns its status (SABLE_EBUDGET when
+ * it can never fit a batch arena). */
+SABLE_API int sable_server_wait(sable_server_t* s, int64_t request_id, double* latency_us);
+
+typedef struct {
+    double rate;           /* requests per second (Poisson arrivals) */
+    double duration_s;
+    double spts_fraction;  /* share of lower solves (the rest are SpMV) */
+    uint64_t seed;
+    int64_t max_requests;  /* 0 = no cap */
+} sable_load_t;
+typedef struct {
+    int64_t submitted, completed, failed, rejected, batches;
+    double elapsed_s, solves_per_s, p50_us, p99_us, mean_us, max_us, mean_batch;
+} sable_load_stats_t;
+/* Open-loop load: requests arrive at exponential gaps, choose a matrix
+ * uniformly (solves: among the lower ones), inputs U(-1,1) generated on the
+ * device from a per-request seed; latency runs from the scheduled arrival
+ * to the completion of the request's batch. */
+SABLE_API int sable_server_run_load(sable_server_t* s, const sable_load_t* load, sable_load_stats_t* stats);
+SABLE_API int sable_server_info(sable_server_t* s, int64_t* hbm_used, int32_t* nmatrices);
+
 /* ------------------------------------------------------------ building blocks */
 /* No reference counterpart: device-wide primitives the conversions are
  * built from, exported for callers that assemble their own formats. */

@@ -14,6 +14,7 @@ from .spmv import spmv, CsrPlan  # noqa: F401
 from .convert import coo_csr, num_cols, csr_ell, num_diags, csr_dia, transpose  # noqa: F401
 from .spts import (SptsAnalysis, CSR_create_level_sets, CSR_create_level_sets_with_reorder,  # noqa: F401
                    spts_level_csr, spts_syncfree)
+from .serve import Server, batch_spmv, batch_spts, batch_spts_refined, lower_dominant  # noqa: F401
 
 __all__ = [
     "Context", "DeviceArray", "SableError",
@@ -23,5 +24,5 @@ __all__ = [
     "spmv", "CsrPlan",
     "coo_csr", "num_cols", "csr_ell", "num_diags", "csr_dia", "transpose",
     "SptsAnalysis", "CSR_create_level_sets", "CSR_create_level_sets_with_reorder", "spts_level_csr",
-    "spts_syncfree",
+    "spts_syncfree", "Server", "batch_spmv", "batch_spts", "batch_spts_refined", "lower_dominant",
 ]

@@ -34,6 +34,22 @@ class SableError(RuntimeError):
         self.name = ERRORS.get(code, str(code))
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", c_i64), ("mode", c_int), ("max_batch", c_int), ("slots", c_int),
+                ("arena_bytes", c_i64), ("refine_iters", c_int), ("spts_lanes", c_int)]
+
+
+class LoadSpec(ctypes.Structure):
+    _fields_ = [("rate", c_dbl), ("duration_s", c_dbl), ("spts_fraction", c_dbl), ("seed", ctypes.c_uint64),
+                ("max_requests", c_i64)]
+
+
+class LoadStats(ctypes.Structure):
+    _fields_ = [("submitted", c_i64), ("completed", c_i64), ("failed", c_i64), ("rejected", c_i64),
+                ("batches", c_i64), ("elapsed_s", c_dbl), ("solves_per_s", c_dbl), ("p50_us", c_dbl),
+                ("p99_us", c_dbl), ("mean_us", c_dbl), ("max_us", c_dbl), ("mean_batch", c_dbl)]
+
+
 class CsrTune(ctypes.Structure):
     _fields_ = [("block_weight", c_int), ("row_weight", c_int)]
 
@@ -85,6 +101,17 @@ SIGNATURES = {
     "sable_spts_solve_refined": [c_vp, c_vp, c_vp, c_vp, c_int, c_int],
     "sable_spts_level_csr": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int],
     "sable_spts_syncfree_csr": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, ctypes.c_longlong],
+    "sable_csr_plan_blocks": [c_vp, c_vpp, ctypes.POINTER(c_i64)],
+    "sable_batch_spmv": [c_vp, c_int, c_int, c_vp, c_int, c_i64, c_int],
+    "sable_batch_spts": [c_vp, c_int, c_int, c_vp, c_int, c_i64],
+    "sable_batch_spts_refined": [c_vp, c_int, c_vp, c_int, c_i64, c_int],
+    "sable_server_create": [c_vp, c_vp, c_vpp],
+    "sable_server_destroy": [c_vp],
+    "sable_server_add_matrix": [c_vp, c_i32, c_vp, c_vp, c_vp, c_int, ctypes.POINTER(c_i32)],
+    "sable_server_submit": [c_vp, c_i32, c_int, c_vp, c_vp, ctypes.POINTER(c_i64)],
+    "sable_server_wait": [c_vp, c_i64, ctypes.POINTER(c_dbl)],
+    "sable_server_run_load": [c_vp, c_vp, c_vp],
+    "sable_server_info": [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i32)],
     "sable_scan_exclusive_i32": [c_vp, c_vp, c_vp, c_i64],
     "sable_sort_pairs_u64": [c_vp, c_vp, c_vp, c_i64, c_int],
 }
@@ -93,6 +120,7 @@ _RESTYPES = {
     "sable_ctx_destroy": None,
     "sable_csr_plan_destroy": None,
     "sable_spts_destroy": None,
+    "sable_server_destroy": None,
     "sable_ctx_stream": c_vp,
     "sable_last_error": ctypes.c_char_p,
     "sable_version": ctypes.c_char_p,

@@ -1,0 +1,481 @@
+// batch.hip -- many independent matrices in one persistent launch.
+//
+// SpMV: every matrix brings its own row-block schedule (sable_csr_plan_t).
+// The batch's blocks form one global work list (item k owns global blocks
+// [blk_begin_k, blk_begin_k + nblocks_k)); a persistent grid of workgroups
+// pops block indices from a device counter, finds the owning item by binary
+// search and runs the same block kernel as single-matrix CSR SpMV.  Items
+// are ordered by the caller (the server puts blocks of long rows first).
+// Up to 8 right-hand sides share each matrix read: x and y are row-major
+// [N][nrhs], so one gather of x[col] fetches all right-hand sides.
+//
+// SpTS: the batch's rows form one global row space (item k owns rows
+// [row_begin_k, row_begin_k + N_k)); waves claim consecutive global rows
+// from a ticket, so the sync-free forward-progress argument of the single
+// solve holds for the whole batch.  The refined variant adds batched FP64
+// residual and update kernels.
+#include "common.h"
+#include "batch.h"
+
+namespace sable {
+
+constexpr int BB = 256;
+
+__device__ __forceinline__ int find_item(const int64_t* __restrict__ begin, int n, int64_t g) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// LDS staging per workgroup: 2048 nonzeros x nrhs products, at most 8192.
+template <int NR>
+struct Lds {
+    static constexpr int ELEMS = (2048 * NR < 8192) ? 2048 * NR : 8192;
+};
+
+template <typename T, int NR>
+__device__ __forceinline__ void block_spmv(const sable_batch_spmv_item_t& it, int64_t b, T* prod, T* red) {
+    constexpr int LDS_ELEMS = Lds<NR>::ELEMS;
+    const int32_t* __restrict__ rp = it.row_ptr;
+    const int32_t* __restrict__ col = it.col;
+    const T* __restrict__ val = (const T*)it.val;
+    const T* __restrict__ x = (const T*)it.x;
+    T* __restrict__ y = (T*)it.y;
+    const T beta = (T)it.beta;
+    const int r0 = it.blk[b], r1 = it.blk[b + 1];
+    if (r0 >= r1) return;
+    const int tid = threadIdx.x;
+    const int64_t nz0 = rp[r0];
+    const int nnzb = (int)((int64_t)rp[r1] - nz0);
+    const int R = r1 - r0;
+    if (nnzb * NR <= LDS_ELEMS) {
+        for (int k = tid; k < nnzb; k += BB) {
+            const int c = col[nz0 + k];
+            const T v = val[nz0 + k];
+#pragma unroll
+            for (int r = 0; r < NR; r++) prod[k * NR + r] = v * x[(int64_t)c * NR + r];
+        }
+        __syncthreads();
+        int L = WAVE;
+        while (L > 1 && L * R > BB) L >>= 1;
+        const int groups = BB / L, g = tid / L, l = tid & (L - 1);
+        for (int rb = 0; rb < R; rb += groups) {
+            const int row = r0 + rb + g;
+            T s[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) s[r] = T(0);
+            int a = 0, e = 0;
+            if (row < r1) {
+                a = (int)((int64_t)rp[row] - nz0);
+                e = (int)((int64_t)rp[row + 1] - nz0);
+                for (int kk = a + l; kk < e; kk += L)
+#pragma unroll
+                    for (int r = 0; r < NR; r++) s[r] += prod[kk * NR + r];
+            }
+#pragma unroll
+            for (int r = 0; r < NR; r++) s[r] = group_sum(s[r], L);
+            if (l == 0 && row < r1) {
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    T* yp = &y[(int64_t)row * NR + r];
+                    if (beta == T(0)) *yp = s[r];
+                    else if (beta == T(1)) { if (e > a) *yp = *yp + s[r]; }
+                    else *yp = beta * *yp + s[r];
+                }
+            }
+        }
+        __syncthreads();  // prod is reused by the next block of this workgroup
+    } else {
+        for (int row = r0; row < r1; row++) {
+            const int64_t a = rp[row], e = rp[row + 1];
+            T s[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) s[r] = T(0);
+            for (int64_t k = a + tid; k < e; k += BB) {
+                const int c = col[k];
+                const T v = val[k];
+#pragma unroll
+                for (int r = 0; r < NR; r++) s[r] += v * x[(int64_t)c * NR + r];
+            }
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                T v = wave_sum(s[r]);
+                __syncthreads();
+                if ((tid & (WAVE - 1)) == 0) red[tid / WAVE] = v;
+                __syncthreads();
+                if (tid == 0) {
+                    T t = 0;
+                    for (int w = 0; w < BB / WAVE; w++) t += red[w];
+                    T* yp = &y[(int64_t)row * NR + r];
+                    if (beta == T(0)) *yp = t;
+                    else if (beta == T(1)) { if (e > a) *yp = *yp + t; }
+                    else *yp = beta * *yp + t;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int NR>
+__global__ __launch_bounds__(BB) void k_batch_spmv(const sable_batch_spmv_item_t* __restrict__ items, int nitems,
+                                                   int64_t total, unsigned long long* __restrict__ counter) {
+    __shared__ __align__(16) T prod[Lds<NR>::ELEMS];
+    __shared__ T red[BB / WAVE];
+    __shared__ long long next;
+    __shared__ int64_t begins[64];
+    // the first 64 block begins are kept in LDS for the common small batch
+    const int nb = nitems < 64 ? nitems : 64;
+    for (int k = threadIdx.x; k < nb; k += BB) begins[k] = items[k].blk_begin;
+    __syncthreads();
+    while (true) {
+        if (threadIdx.x == 0) next = (long long)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const int64_t g = next;
+        __syncthreads();
+        if (g >= total) break;
+        int k;
+        if (nitems <= 64) {
+            k = 0;
+            int lo = 0, hi = nitems - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (begins[mid] <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            k = lo;
+        } else {
+            // items[].blk_begin is ascending: binary search in global memory
+            int lo = 0, hi = nitems - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (items[mid].blk_begin <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            k = lo;
+        }
+        const sable_batch_spmv_item_t it = items[k];
+        block_spmv<T, NR>(it, g - it.blk_begin, prod, red);
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+// ------------------------------------------------------------- batched SpTS
+constexpr uint64_t BSENT64 = 0x7FF4DEADBEEF0001ull;
+constexpr uint32_t BSENT32 = 0x7FA0BEEFu;
+
+__device__ __forceinline__ double bsent(double) { return __longlong_as_double((long long)BSENT64); }
+__device__ __forceinline__ float bsent(float) { return __uint_as_float(BSENT32); }
+__device__ __forceinline__ bool is_bsent(double v) { return (uint64_t)__double_as_longlong(v) == BSENT64; }
+__device__ __forceinline__ bool is_bsent(float v) { return __float_as_uint(v) == BSENT32; }
+
+template <typename T>
+__global__ void k_batch_sentinel(const sable_batch_spts_item_t* __restrict__ items, int nitems, int64_t total,
+                                 unsigned* __restrict__ ticket) {
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nitems - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (items[mid].row_begin <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        ((T*)items[lo].x)[g - items[lo].row_begin] = bsent(T(0));
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ticket = 0u;
+}
+
+template <typename T, int G>
+__global__ __launch_bounds__(BB) void k_batch_syncfree(const sable_batch_spts_item_t* __restrict__ items, int nitems,
+                                                       int64_t total, unsigned* __restrict__ ticket,
+                                                       int* __restrict__ err, long long spin_limit) {
+    constexpr int R = WAVE / G;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int l = lane & (G - 1);
+    const int grp = lane / G;
+    const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << (grp * G));
+    while (true) {
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(ticket, (unsigned)R);
+        base = __shfl(base, 0, WAVE);
+        if ((int64_t)base >= total) break;
+        const int64_t g = (int64_t)base + grp;
+        bool published = g >= total;
+        const int32_t* rp = nullptr;
+        const int32_t* col = nullptr;
+        const T* val = nullptr;
+        T* x = nullptr;
+        int32_t i = 0, j = 0, e = 0;
+        T s = T(0), bi = T(0);
+        if (!published) {
+            int lo = 0, hi = nitems - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (items[mid].row_begin <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            rp = items[lo].row_ptr;
+            col = items[lo].col;
+            val = (const T*)items[lo].val;
+            x = (T*)items[lo].x;
+            i = (int32_t)(g - items[lo].row_begin);
+            j = rp[i] + l;
+            e = rp[i + 1] - 1;
+            bi = ((const T*)items[lo].b)[i];
+            if (G == 1) s = bi;
+        }
+        long long spins = 0;
+        while (__ballot(!published) != 0ull) {
+            bool progressed = false;
+            if (!published) {
+                while (j < e) {
+                    const T v = __hip_atomic_load(&x[col[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (is_bsent(v)) break;
+                    if (G == 1) s = sub_rn(s, mul_rn(val[j], v));
+                    else s = add_rn(s, mul_rn(val[j], v));
+                    j += G;
+                    progressed = true;
+                }
+            }
+            const bool ldone = published || j >= e;
+            const uint64_t m = __ballot(ldone);
+            const bool gdone = (m & gmask) == gmask;
+            T tot = s;
+            if (G > 1) tot = group_sum(s, G);
+            if (!published && gdone) {
+                if (l == 0) {
+                    const T yv = (G == 1) ? div_rn(tot, val[e]) : div_rn(sub_rn(bi, tot), val[e]);
+                    __hip_atomic_store(&x[i], yv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                published = true;
+                progressed = true;
+            }
+            if (__ballot(progressed) == 0ull) {
+                if (++spins > spin_limit) {
+                    if (lane == 0) atomicOr(err, 4);
+                    if (!published && l == 0)
+                        __hip_atomic_store(&x[i], T(__builtin_nan("")), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    published = true;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+    }
+}
+
+// r32 = b64 - L64 x64 for every item (G lanes per row); first pass (x = 0)
+// just converts b.
+template <int G>
+__global__ __launch_bounds__(BB) void k_batch_residual(const sable_batch_refine_item_t* __restrict__ items,
+                                                       int nitems, int64_t total, int first) {
+    const int64_t gid = ((int64_t)blockIdx.x * BB + threadIdx.x) / G;
+    const int l = threadIdx.x & (G - 1);
+    const bool valid = gid < total;
+    int lo = 0;
+    if (valid) {
+        int hi = nitems - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (items[mid].row_begin <= gid) lo = mid;
+            else hi = mid - 1;
+        }
+    }
+    double s = 0.0;
+    int32_t i = 0;
+    if (valid) {
+        const sable_batch_refine_item_t& it = items[lo];
+        i = (int32_t)(gid - it.row_begin);
+        if (!first)
+            for (int32_t j = it.row_ptr[i] + l; j < it.row_ptr[i + 1]; j += G) s = fma(it.val64[j], it.x64[it.col[j]], s);
+    }
+    if (G > 1) s = group_sum(s, G);
+    if (valid && l == 0) {
+        const sable_batch_refine_item_t& it = items[lo];
+        it.r32[i] = (float)(it.b64[i] - s);
+    }
+}
+
+__global__ void k_batch_update(const sable_batch_refine_item_t* __restrict__ items, int nitems, int64_t total,
+                               int first) {
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nitems - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (items[mid].row_begin <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        const sable_batch_refine_item_t& it = items[lo];
+        const int64_t i = g - it.row_begin;
+        it.x64[i] = first ? (double)it.d32[i] : it.x64[i] + (double)it.d32[i];
+    }
+}
+
+// Sync-free item view of a refine item: solve L32 d = r.
+__global__ void k_refine_to_spts(const sable_batch_refine_item_t* __restrict__ in, sable_batch_spts_item_t* __restrict__ out,
+                                 int nitems) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nitems; k += gridDim.x * blockDim.x) {
+        sable_batch_spts_item_t o;
+        o.row_ptr = in[k].row_ptr;
+        o.col = in[k].col;
+        o.val = in[k].val32;
+        o.b = in[k].r32;
+        o.x = in[k].d32;
+        o.row_begin = in[k].row_begin;
+        o.N = in[k].N;
+        o.pad = 0;
+        out[k] = o;
+    }
+}
+
+static unsigned gsz(int64_t n) {
+    unsigned g = grid_for(n, BB);
+    return g > 16384 ? 16384 : g;
+}
+
+int batch_spts_launch(hipStream_t st, int num_cu, sable_dtype_t dtype, int lanes, const sable_batch_spts_item_t* items,
+                      int nitems, int64_t total_rows, unsigned* ticket, int* err, long long spin) {
+    if (total_rows <= 0) return SABLE_OK;
+    const int64_t waves = (total_rows * lanes + WAVE - 1) / WAVE;
+    int64_t blocks = (waves + (BB / WAVE) - 1) / (BB / WAVE);
+    const int64_t cap = (int64_t)num_cu * 8;
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+#define SF_LAUNCH(T, G)                                                                                          \
+    hipLaunchKernelGGL((k_batch_syncfree<T, G>), dim3((unsigned)blocks), dim3(BB), 0, st, items, nitems, total_rows, \
+                       ticket, err, spin)
+#define SF_SWITCH(T)                                   \
+    hipLaunchKernelGGL(k_batch_sentinel<T>, dim3(gsz(total_rows)), dim3(BB), 0, st, items, nitems, total_rows, ticket); \
+    switch (lanes) {                                   \
+        case 1: SF_LAUNCH(T, 1); break;                \
+        case 4: SF_LAUNCH(T, 4); break;                \
+        case 8: SF_LAUNCH(T, 8); break;                \
+        case 16: SF_LAUNCH(T, 16); break;              \
+        default: SF_LAUNCH(T, 64); break;              \
+    }
+    if (dtype == SABLE_F64) {
+        SF_SWITCH(double)
+    } else {
+        SF_SWITCH(float)
+    }
+#undef SF_SWITCH
+#undef SF_LAUNCH
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+int batch_spmv_launch(hipStream_t st, int num_cu, sable_dtype_t dtype, int nrhs, const sable_batch_spmv_item_t* items,
+                      int nitems, int64_t total_blocks, unsigned long long* counter, int wgs) {
+    if (total_blocks <= 0 || nitems <= 0) return SABLE_OK;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, counter);
+    int64_t grid = wgs > 0 ? wgs : (int64_t)num_cu * 4;
+    if (grid > total_blocks) grid = total_blocks;
+#define BS_LAUNCH(T, NR) \
+    hipLaunchKernelGGL((k_batch_spmv<T, NR>), dim3((unsigned)grid), dim3(BB), 0, st, items, nitems, total_blocks, counter)
+#define BS_SWITCH(T)                        \
+    switch (nrhs) {                         \
+        case 1: BS_LAUNCH(T, 1); break;     \
+        case 2: BS_LAUNCH(T, 2); break;     \
+        case 4: BS_LAUNCH(T, 4); break;     \
+        default: BS_LAUNCH(T, 8); break;    \
+    }
+    if (dtype == SABLE_F64) {
+        BS_SWITCH(double)
+    } else {
+        BS_SWITCH(float)
+    }
+#undef BS_SWITCH
+#undef BS_LAUNCH
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+int batch_refine_launch(hipStream_t st, int num_cu, int lanes, const sable_batch_refine_item_t* items,
+                        sable_batch_spts_item_t* spts_view, int nitems, int64_t total_rows, int iters,
+                        unsigned* ticket, int* err, long long spin) {
+    if (total_rows <= 0) return SABLE_OK;
+    hipLaunchKernelGGL(k_refine_to_spts, dim3(1), dim3(BB), 0, st, items, spts_view, nitems);
+    const unsigned rgrid = (unsigned)((total_rows * lanes + BB - 1) / BB);
+    for (int it = 0; it <= iters; it++) {
+        const int first = it == 0;
+        switch (lanes) {
+            case 1: hipLaunchKernelGGL((k_batch_residual<1>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
+            case 4: hipLaunchKernelGGL((k_batch_residual<4>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
+            case 8: hipLaunchKernelGGL((k_batch_residual<8>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
+            case 16: hipLaunchKernelGGL((k_batch_residual<16>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
+            default: hipLaunchKernelGGL((k_batch_residual<64>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
+        }
+        int rc = batch_spts_launch(st, num_cu, SABLE_F32, lanes, spts_view, nitems, total_rows, ticket, err, spin);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_batch_update, dim3(gsz(total_rows)), dim3(BB), 0, st, items, nitems, total_rows, first);
+    }
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+}  // namespace sable
+
+using namespace sable;
+
+extern "C" {
+
+SABLE_API int sable_batch_spmv(sable_ctx_t* ctx, sable_dtype_t dtype, int nrhs, const sable_batch_spmv_item_t* d_items,
+                               int nitems, int64_t total_blocks, int workgroups) {
+    SABLE_CHECK(ctx && (d_items || nitems == 0), SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(nrhs == 1 || nrhs == 2 || nrhs == 4 || nrhs == 8, SABLE_EINVAL, "nrhs must be 1, 2, 4 or 8");
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    if (nitems == 0 || total_blocks == 0) return SABLE_OK;
+    void* s;
+    int rc = ctx_scratch(ctx, 256, &s);
+    if (rc) return rc;
+    return batch_spmv_launch(ctx->stream, ctx->num_cu, dtype, nrhs, d_items, nitems, total_blocks,
+                             (unsigned long long*)s, workgroups);
+}
+
+SABLE_API int sable_batch_spts(sable_ctx_t* ctx, sable_dtype_t dtype, int lanes, const sable_batch_spts_item_t* d_items,
+                               int nitems, int64_t total_rows) {
+    SABLE_CHECK(ctx && (d_items || nitems == 0), SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(lanes == 1 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 64, SABLE_EINVAL, "bad lanes");
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    if (nitems == 0 || total_rows == 0) return SABLE_OK;
+    void* s;
+    int rc = ctx_scratch(ctx, 512, &s);
+    if (rc) return rc;
+    int* err = (int*)((char*)s + 256);
+    SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), ctx->stream));
+    rc = batch_spts_launch(ctx->stream, ctx->num_cu, dtype, lanes, d_items, nitems, total_rows, (unsigned*)s, err,
+                           1ll << 24);
+    if (rc) return rc;
+    int flag;
+    SABLE_HIP(hipMemcpyAsync(ctx->pinned, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    SABLE_HIP(hipStreamSynchronize(ctx->stream));
+    flag = *(int*)ctx->pinned;
+    SABLE_CHECK(!flag, SABLE_ETIMEOUT, "batched sync-free solve exceeded its spin limit");
+    return SABLE_OK;
+}
+
+SABLE_API int sable_batch_spts_refined(sable_ctx_t* ctx, int lanes, const sable_batch_refine_item_t* d_items,
+                                       int nitems, int64_t total_rows, int iters) {
+    SABLE_CHECK(ctx && (d_items || nitems == 0), SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(lanes == 1 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 64, SABLE_EINVAL, "bad lanes");
+    SABLE_CHECK(iters >= 0, SABLE_EINVAL, "negative iters");
+    if (nitems == 0 || total_rows == 0) return SABLE_OK;
+    void* s;
+    int rc = ctx_scratch(ctx, 1024 + sizeof(sable_batch_spts_item_t) * (size_t)nitems, &s);
+    if (rc) return rc;
+    int* err = (int*)((char*)s + 256);
+    auto* view = (sable_batch_spts_item_t*)((char*)s + 1024);
+    SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), ctx->stream));
+    rc = batch_refine_launch(ctx->stream, ctx->num_cu, lanes, d_items, view, nitems, total_rows, iters, (unsigned*)s,
+                             err, 1ll << 24);
+    if (rc) return rc;
+    SABLE_HIP(hipMemcpyAsync(ctx->pinned, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    SABLE_HIP(hipStreamSynchronize(ctx->stream));
+    SABLE_CHECK(!*(int*)ctx->pinned, SABLE_ETIMEOUT, "batched sync-free solve exceeded its spin limit");
+    return SABLE_OK;
+}
+
+}  // extern "C"

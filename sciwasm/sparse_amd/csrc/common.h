@@ -47,6 +47,13 @@ struct sable_ctx_s {
 };
 
 namespace sable {
+// Stream-explicit launchers shared by the batch server (spmv.hip, spts.hip).
+int csr_launch(hipStream_t st, sable_dtype_t dt, const int32_t* blk, int64_t nb, const int32_t* rp,
+               const int32_t* col, const void* val, const void* x, void* y, double beta, int lds_cap);
+int csr_plan_view(const sable_csr_plan_t* p, const int32_t** blk, int64_t* nb, int* lds_cap);
+int syncfree_launch(hipStream_t st, int num_cu, sable_dtype_t dt, int lanes, int32_t N, const int32_t* rp,
+                    const int32_t* col, const void* val, const void* b, void* x, unsigned* ticket, int* err,
+                    long long spin);
 int ctx_scratch(sable_ctx_t* ctx, size_t bytes, void** out);
 int read_device_i64(sable_ctx_t* ctx, const void* dsrc, size_t count, int64_t* host_out);
 
@@ -60,15 +67,35 @@ inline unsigned grid_for(int64_t n, int block) {
 }
 
 // Rounded multiply and add without contraction: the reference's WebAssembly
-// arithmetic is a separately rounded mul followed by add.
-__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
-__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
-__device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
-__device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
-__device__ __forceinline__ float sub_rn(float a, float b) { return __fsub_rn(a, b); }
-__device__ __forceinline__ double sub_rn(double a, double b) { return __dsub_rn(a, b); }
+// arithmetic is a separately rounded mul followed by add.  HIP's __fmul_rn
+// family are plain operators that the default -ffp-contract=fast would fuse
+// into an FMA, so contraction is switched off inside each helper.
+__device__ __forceinline__ float mul_rn(float a, float b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ double mul_rn(double a, double b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ float add_rn(float a, float b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ double add_rn(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ float sub_rn(float a, float b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+__device__ __forceinline__ double sub_rn(double a, double b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
 __device__ __forceinline__ float div_rn(float a, float b) { return __fdiv_rn(a, b); }
-__device__ __forceinline__ double div_rn(double a, double b) { return __ddiv_rn(a, b); }
+__device__ __forceinline__ double div_rn(double a, double b) { return a / b; }
 
 // Sum over aligned groups of L lanes (L a power of two <= 64).
 template <typename T>

@@ -1,0 +1,683 @@
+// serve.hip -- a request server for SpMV and lower SpTS over resident matrices.
+//
+// Matrices (and their SpMV row-block plans and FP32 copies) are uploaded
+// once and stay in HBM; they are charged against the server's HBM budget.
+// Requests (SpMV y = A.x, or lower solve L.x = b) queue on the host.  A
+// scheduler thread forms batches continuously: whenever one of `slots`
+// in-flight batch slots is free it admits queued requests in arrival order
+// until the batch is full or the slot's device arena (inputs, outputs and
+// refinement work space) is exhausted, stages the inputs, and launches
+//   mode 0 (plain):   one request per batch, FP64, one SpMV launch or one
+//                     sync-free solve launch per request;
+//   mode 1 (batched): FP64, one persistent SpMV launch and one sync-free
+//                     solve launch for the whole batch;
+//   mode 2 (mixed):   batched, SpMV with FP32 values/vectors and FP32
+//                     accumulation; solves in FP32 with FP64 residual and
+//                     update for `refine_iters` refinement steps.
+// A completion thread waits on each slot's event in order, copies results
+// out and records latencies.  Several slots in flight let a new batch be
+// staged and launched while earlier ones run on the GPU.
+#include "common.h"
+#include "batch.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <random>
+#include <thread>
+#include <unordered_map>
+
+namespace sable {
+
+static double now_us() {
+    using namespace std::chrono;
+    return duration<double, std::micro>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct SMatrix {
+    int32_t N = 0;
+    int64_t nnz = 0;
+    int kind = 0;  // 0 general, 1 lower triangular (diagonal last)
+    int32_t* rp = nullptr;
+    int32_t* col = nullptr;
+    double* val64 = nullptr;
+    float* val32 = nullptr;
+    sable_csr_plan_t* plan = nullptr;
+    const int32_t* blk = nullptr;
+    int64_t nblocks = 0;
+    int lds_cap = 0;
+    size_t bytes = 0;
+};
+
+struct SRequest {
+    int64_t id = 0;
+    int32_t mat = 0;
+    int op = 0;  // 0 SpMV, 1 lower solve
+    const double* in = nullptr;
+    double* out = nullptr;
+    uint64_t seed = 0;
+    bool seeded = false;
+    double t_arrival = 0, t_done = 0;
+    int status = 0;
+    bool done = false;
+    size_t in_off = 0, out_off = 0, work_off = 0;
+};
+
+struct FillItem {
+    void* ptr;
+    int64_t begin;  // first global element
+    int32_t n;
+    int32_t is_f32;
+    uint64_t seed;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    char* arena = nullptr;
+    char* stage_in = nullptr;
+    char* stage_out = nullptr;
+    char* h_items = nullptr;  // pinned staging for item arrays
+    char* d_items = nullptr;  // device item arrays
+    unsigned long long* d_counter = nullptr;
+    unsigned* d_ticket = nullptr;
+    int* d_err = nullptr;
+    int* h_err = nullptr;
+    size_t out_begin = 0, out_end = 0;
+    std::vector<SRequest*> reqs;
+    bool busy = false;
+};
+
+constexpr size_t ITEM_BYTES = 1 << 20;  // per slot, per item kind
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// U(-1,1) inputs from (seed, index) for seeded requests.
+__global__ void k_fill_uniform(const FillItem* __restrict__ items, int nitems, int64_t total) {
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nitems - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (items[mid].begin <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        const FillItem& it = items[lo];
+        const int64_t i = g - it.begin;
+        const double u = (double)(mix64(it.seed * 0x100000001B3ull + (uint64_t)i) >> 11) * (1.0 / 9007199254740992.0);
+        const double v = 2.0 * u - 1.0;
+        if (it.is_f32) ((float*)it.ptr)[i] = (float)v;
+        else ((double*)it.ptr)[i] = v;
+    }
+}
+
+__global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ o) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        o[i] = (float)a[i];
+}
+
+static size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace sable
+
+using namespace sable;
+
+struct sable_server_s {
+    sable_ctx_t* ctx = nullptr;
+    sable_server_config_t cfg{};
+    int64_t used = 0;
+    std::vector<SMatrix> mats;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_pending, cv_slot, cv_done, cv_complete;
+    std::deque<SRequest*> pending;
+    std::deque<int> inflight;  // slot indices in launch order
+    std::unordered_map<int64_t, std::unique_ptr<SRequest>> reqs;
+    int64_t next_id = 1;
+    bool stop = false;
+    std::thread sched, compl_;
+    // statistics
+    std::vector<double> latencies;
+    int64_t completed = 0, rejected = 0, batches = 0, batched_reqs = 0;
+    double t_last_done = 0;
+    int error = 0;
+    char errmsg[256] = "";
+
+    size_t vbytes_spmv() const { return cfg.mode == 2 ? 4 : 8; }
+    void scheduler();
+    void completer();
+    int launch_batch(Slot& s, std::vector<SRequest*>& batch);
+    size_t need_bytes(const SRequest* r) const {
+        const size_t n = (size_t)mats[r->mat].N;
+        if (r->op == 0) return 2 * al(n * vbytes_spmv());
+        return 2 * al(n * 8) + (cfg.mode == 2 ? 2 * al(n * 4) : 0);
+    }
+};
+
+void sable_server_s::scheduler() {
+    (void)hipSetDevice(ctx->device);
+    size_t next_slot = 0;
+    while (true) {
+        Slot* sl;
+        std::vector<SRequest*> batch;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_pending.wait(lk, [&] { return stop || !pending.empty(); });
+            if (stop && pending.empty()) return;
+            cv_slot.wait(lk, [&] { return stop || !slots[next_slot].busy; });
+            if (stop && pending.empty()) return;
+            sl = &slots[next_slot];
+            const size_t cap = (size_t)cfg.arena_bytes;
+            size_t used_b = 0;
+            const int maxb = cfg.mode == 0 ? 1 : cfg.max_batch;
+            while (!pending.empty() && (int)batch.size() < maxb) {
+                SRequest* r = pending.front();
+                const size_t nb = need_bytes(r);
+                if (nb > cap) {  // can never fit: reject
+                    pending.pop_front();
+                    r->status = SABLE_EBUDGET;
+                    r->done = true;
+                    r->t_done = now_us();
+                    rejected++;
+                    cv_done.notify_all();
+                    continue;
+                }
+                if (used_b + nb > cap) break;
+                used_b += nb;
+                batch.push_back(r);
+                pending.pop_front();
+            }
+            if (batch.empty()) continue;
+            sl->busy = true;
+        }
+        int rc = launch_batch(*sl, batch);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (rc) {
+                error = rc;
+                snprintf(errmsg, sizeof errmsg, "%s", sable_last_error());
+            }
+            inflight.push_back((int)next_slot);
+            batches++;
+            batched_reqs += (int64_t)batch.size();
+        }
+        cv_complete.notify_all();
+        next_slot = (next_slot + 1) % slots.size();
+    }
+}
+
+int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
+    s.reqs = batch;
+    const bool mixed = cfg.mode == 2;
+    const size_t vs = vbytes_spmv();
+    // arena layout: inputs, then outputs, then refinement work
+    size_t off = 0;
+    for (auto* r : batch) {
+        const size_t n = (size_t)mats[r->mat].N;
+        r->in_off = off;
+        off += al(n * (r->op == 0 ? vs : 8));
+    }
+    const size_t in_end = off;
+    s.out_begin = off;
+    for (auto* r : batch) {
+        const size_t n = (size_t)mats[r->mat].N;
+        r->out_off = off;
+        off += al(n * (r->op == 0 ? vs : 8));
+    }
+    s.out_end = off;
+    for (auto* r : batch) {
+        if (r->op == 1 && mixed) {
+            r->work_off = off;
+            off += 2 * al((size_t)mats[r->mat].N * 4);
+        }
+    }
+    hipStream_t st = s.stream;
+    // host inputs -> pinned staging -> device
+    bool any_host = false;
+    std::vector<FillItem> fills;
+    int64_t fill_total = 0;
+    for (auto* r : batch) {
+        const int32_t n = mats[r->mat].N;
+        const bool f32 = r->op == 0 && mixed;
+        if (r->seeded) {
+            fills.push_back(FillItem{s.arena + r->in_off, fill_total, n, f32 ? 1 : 0, r->seed});
+            fill_total += n;
+        } else {
+            any_host = true;
+            if (f32) {
+                float* d = (float*)(s.stage_in + r->in_off);
+                for (int32_t i = 0; i < n; i++) d[i] = (float)r->in[i];
+            } else {
+                memcpy(s.stage_in + r->in_off, r->in, 8 * (size_t)n);
+            }
+        }
+    }
+    if (any_host) SABLE_HIP(hipMemcpyAsync(s.arena, s.stage_in, in_end, hipMemcpyHostToDevice, st));
+    // item arrays: [fill | spmv | spts or refine | spts view]
+    char* hi = s.h_items;
+    size_t fill_off = 0, spmv_off = ITEM_BYTES, spts_off = 2 * ITEM_BYTES, view_off = 3 * ITEM_BYTES;
+    SABLE_CHECK(fills.size() * sizeof(FillItem) <= ITEM_BYTES, SABLE_EINVAL, "too many requests in a batch");
+    memcpy(hi + fill_off, fills.data(), fills.size() * sizeof(FillItem));
+    std::vector<sable_batch_spmv_item_t> sv;
+    std::vector<sable_batch_spts_item_t> tv;
+    std::vector<sable_batch_refine_item_t> rv;
+    int64_t blk_total = 0, row_total = 0;
+    for (auto* r : batch) {
+        const SMatrix& m = mats[r->mat];
+        if (r->op == 0) {
+            sable_batch_spmv_item_t it;
+            it.row_ptr = m.rp;
+            it.col = m.col;
+            it.val = mixed ? (const void*)m.val32 : (const void*)m.val64;
+            it.blk = m.blk;
+            it.x = s.arena + r->in_off;
+            it.y = s.arena + r->out_off;
+            it.blk_begin = blk_total;
+            it.beta = 0.0;
+            blk_total += m.nblocks;
+            sv.push_back(it);
+        } else if (!mixed) {
+            sable_batch_spts_item_t it;
+            it.row_ptr = m.rp;
+            it.col = m.col;
+            it.val = m.val64;
+            it.b = s.arena + r->in_off;
+            it.x = s.arena + r->out_off;
+            it.row_begin = row_total;
+            it.N = m.N;
+            it.pad = 0;
+            row_total += m.N;
+            tv.push_back(it);
+        } else {
+            sable_batch_refine_item_t it;
+            it.row_ptr = m.rp;
+            it.col = m.col;
+            it.val64 = m.val64;
+            it.val32 = m.val32;
+            it.b64 = (const double*)(s.arena + r->in_off);
+            it.x64 = (double*)(s.arena + r->out_off);
+            it.r32 = (float*)(s.arena + r->work_off);
+            it.d32 = (float*)(s.arena + r->work_off + al((size_t)m.N * 4));
+            it.row_begin = row_total;
+            it.N = m.N;
+            it.pad = 0;
+            row_total += m.N;
+            rv.push_back(it);
+        }
+    }
+    SABLE_CHECK(sv.size() * sizeof(sv[0]) <= ITEM_BYTES && tv.size() * sizeof(tv[0]) <= ITEM_BYTES &&
+                    rv.size() * sizeof(rv[0]) <= ITEM_BYTES,
+                SABLE_EINVAL, "too many requests in a batch");
+    memcpy(hi + spmv_off, sv.data(), sv.size() * sizeof(sv[0]));
+    if (mixed) memcpy(hi + spts_off, rv.data(), rv.size() * sizeof(rv[0]));
+    else memcpy(hi + spts_off, tv.data(), tv.size() * sizeof(tv[0]));
+    SABLE_HIP(hipMemcpyAsync(s.d_items, s.h_items, 3 * ITEM_BYTES, hipMemcpyHostToDevice, st));
+    if (!fills.empty()) {
+        unsigned g = grid_for(fill_total, 256);
+        if (g > 16384) g = 16384;
+        hipLaunchKernelGGL(k_fill_uniform, dim3(g), dim3(256), 0, st, (const FillItem*)(s.d_items + fill_off),
+                           (int)fills.size(), fill_total);
+        SABLE_LAUNCH_CHECK();
+    }
+    const int lanes = cfg.spts_lanes > 0 ? cfg.spts_lanes : 4;
+    const long long spin = 1ll << 24;
+    SABLE_HIP(hipMemsetAsync(s.d_err, 0, sizeof(int), st));
+    int rc = SABLE_OK;
+    if (cfg.mode == 0) {
+        // plain: exactly one request, its own single-matrix launch
+        SRequest* r = batch[0];
+        const SMatrix& m = mats[r->mat];
+        if (r->op == 0)
+            rc = csr_launch(st, SABLE_F64, m.blk, m.nblocks, m.rp, m.col, m.val64, s.arena + r->in_off,
+                            s.arena + r->out_off, 0.0, m.lds_cap);
+        else
+            rc = syncfree_launch(st, ctx->num_cu, SABLE_F64, lanes, m.N, m.rp, m.col, m.val64, s.arena + r->in_off,
+                                 s.arena + r->out_off, s.d_ticket, s.d_err, spin);
+    } else {
+        if (!sv.empty())
+            rc = batch_spmv_launch(st, ctx->num_cu, mixed ? SABLE_F32 : SABLE_F64, 1,
+                                   (const sable_batch_spmv_item_t*)(s.d_items + spmv_off), (int)sv.size(), blk_total,
+                                   s.d_counter, 0);
+        if (!rc && !tv.empty())
+            rc = batch_spts_launch(st, ctx->num_cu, SABLE_F64, lanes,
+                                   (const sable_batch_spts_item_t*)(s.d_items + spts_off), (int)tv.size(), row_total,
+                                   s.d_ticket, s.d_err, spin);
+        if (!rc && !rv.empty())
+            rc = batch_refine_launch(st, ctx->num_cu, lanes, (const sable_batch_refine_item_t*)(s.d_items + spts_off),
+                                     (sable_batch_spts_item_t*)(s.d_items + view_off), (int)rv.size(), row_total,
+                                     cfg.refine_iters, s.d_ticket, s.d_err, spin);
+    }
+    SABLE_HIP(hipMemcpyAsync(s.stage_out + s.out_begin, s.arena + s.out_begin, s.out_end - s.out_begin,
+                             hipMemcpyDeviceToHost, st));
+    SABLE_HIP(hipMemcpyAsync(s.h_err, s.d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+    SABLE_HIP(hipEventRecord(s.ev, st));
+    return rc;
+}
+
+void sable_server_s::completer() {
+    (void)hipSetDevice(ctx->device);
+    while (true) {
+        int si;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_complete.wait(lk, [&] { return !inflight.empty() || (stop && pending.empty()); });
+            if (inflight.empty()) return;
+            si = inflight.front();
+        }
+        Slot& s = slots[si];
+        hipError_t e = hipEventSynchronize(s.ev);
+        const double t = now_us();
+        const bool mixed = cfg.mode == 2;
+        const int errflag = *s.h_err;
+        for (auto* r : s.reqs) {
+            if (r->out) {
+                const int32_t n = mats[r->mat].N;
+                if (r->op == 0 && mixed) {
+                    const float* src = (const float*)(s.stage_out + r->out_off);
+                    for (int32_t i = 0; i < n; i++) r->out[i] = (double)src[i];
+                } else {
+                    memcpy(r->out, s.stage_out + r->out_off, 8 * (size_t)n);
+                }
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (auto* r : s.reqs) {
+                r->t_done = t;
+                r->status = e != hipSuccess ? SABLE_EHIP : (errflag ? SABLE_ETIMEOUT : SABLE_OK);
+                r->done = true;
+                latencies.push_back(t - r->t_arrival);
+                completed++;
+            }
+            t_last_done = t;
+            s.reqs.clear();
+            s.busy = false;
+            inflight.pop_front();
+        }
+        cv_slot.notify_all();
+        cv_done.notify_all();
+    }
+}
+
+static void free_matrix(SMatrix& m) {
+    if (m.plan) sable_csr_plan_destroy(m.plan);
+    void* ps[] = {m.rp, m.col, m.val64, m.val32};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    m = SMatrix();
+}
+
+static void free_server(sable_server_s* s) {
+    for (auto& m : s->mats) free_matrix(m);
+    for (auto& sl : s->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.arena) (void)hipFree(sl.arena);
+        if (sl.d_items) (void)hipFree(sl.d_items);
+        if (sl.d_counter) (void)hipFree(sl.d_counter);
+        if (sl.stage_in) (void)hipHostFree(sl.stage_in);
+        if (sl.stage_out) (void)hipHostFree(sl.stage_out);
+        if (sl.h_items) (void)hipHostFree(sl.h_items);
+        if (sl.h_err) (void)hipHostFree(sl.h_err);
+        if (sl.ev) (void)hipEventDestroy(sl.ev);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    }
+    delete s;
+}
+
+extern "C" {
+
+SABLE_API int sable_server_create(sable_ctx_t* ctx, const sable_server_config_t* cfg, sable_server_t** out) {
+    SABLE_CHECK(ctx && cfg && out, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(cfg->mode >= 0 && cfg->mode <= 2, SABLE_EINVAL, "mode must be 0, 1 or 2");
+    *out = nullptr;
+    auto* s = new sable_server_s();
+    s->ctx = ctx;
+    s->cfg = *cfg;
+    if (s->cfg.max_batch <= 0) s->cfg.max_batch = 256;
+    if (s->cfg.slots <= 0) s->cfg.slots = 3;
+    if (s->cfg.arena_bytes <= 0) s->cfg.arena_bytes = 256ll << 20;
+    if (s->cfg.refine_iters < 0) s->cfg.refine_iters = 2;
+    if (s->cfg.max_batch > 4096) s->cfg.max_batch = 4096;
+    const int64_t slot_bytes = s->cfg.arena_bytes + 4 * (int64_t)ITEM_BYTES + 4096;
+    if (cfg->hbm_budget > 0 && slot_bytes * s->cfg.slots > cfg->hbm_budget) {
+        delete s;
+        return fail(SABLE_EBUDGET, "batch arenas (%lld bytes) exceed the HBM budget", (long long)(slot_bytes * cfg->slots));
+    }
+    (void)hipSetDevice(ctx->device);
+    s->slots.resize(s->cfg.slots);
+    for (auto& sl : s->slots) {
+        bool ok = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) == hipSuccess &&
+                  hipMalloc(&sl.arena, (size_t)s->cfg.arena_bytes) == hipSuccess &&
+                  hipMalloc(&sl.d_items, 4 * ITEM_BYTES) == hipSuccess &&
+                  hipMalloc(&sl.d_counter, 4096) == hipSuccess &&
+                  hipHostMalloc(&sl.stage_in, (size_t)s->cfg.arena_bytes, hipHostMallocDefault) == hipSuccess &&
+                  hipHostMalloc(&sl.stage_out, (size_t)s->cfg.arena_bytes, hipHostMallocDefault) == hipSuccess &&
+                  hipHostMalloc(&sl.h_items, 4 * ITEM_BYTES, hipHostMallocDefault) == hipSuccess &&
+                  hipHostMalloc(&sl.h_err, 256, hipHostMallocDefault) == hipSuccess;
+        if (!ok) {
+            (void)hipGetLastError();
+            free_server(s);
+            return fail(SABLE_ENOMEM, "server slot allocation failed");
+        }
+        sl.d_ticket = (unsigned*)((char*)sl.d_counter + 256);
+        sl.d_err = (int*)((char*)sl.d_counter + 512);
+        s->used += slot_bytes;
+    }
+    s->sched = std::thread([s] { s->scheduler(); });
+    s->compl_ = std::thread([s] { s->completer(); });
+    *out = s;
+    return SABLE_OK;
+}
+
+SABLE_API void sable_server_destroy(sable_server_t* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_pending.notify_all();
+    s->cv_slot.notify_all();
+    s->cv_complete.notify_all();
+    if (s->sched.joinable()) s->sched.join();
+    s->cv_complete.notify_all();
+    if (s->compl_.joinable()) s->compl_.join();
+    free_server(s);
+}
+
+SABLE_API int sable_server_add_matrix(sable_server_t* s, int32_t N, const int32_t* row_ptr, const int32_t* col,
+                                      const double* val, int kind, int32_t* matrix_id) {
+    SABLE_CHECK(s && row_ptr && matrix_id && N > 0, SABLE_EINVAL, "bad argument");
+    SABLE_CHECK(kind == 0 || kind == 1, SABLE_EINVAL, "kind must be 0 (general) or 1 (lower)");
+    const int64_t nnz = (int64_t)row_ptr[N] - row_ptr[0];
+    SABLE_CHECK(row_ptr[0] == 0 && nnz >= 0, SABLE_EMALFORMED, "row_ptr must start at 0 and be monotone");
+    if (kind == 1) {
+        for (int32_t i = 0; i < N; i++) {
+            SABLE_CHECK(row_ptr[i + 1] > row_ptr[i] && col[row_ptr[i + 1] - 1] == i && val[row_ptr[i + 1] - 1] != 0.0,
+                        SABLE_EMALFORMED, "row %d: the last entry must be a nonzero diagonal", i);
+            for (int32_t j = row_ptr[i]; j < row_ptr[i + 1] - 1; j++)
+                SABLE_CHECK(col[j] >= 0 && col[j] < i, SABLE_EMALFORMED, "row %d is not lower triangular", i);
+        }
+    }
+    SMatrix m;
+    m.N = N;
+    m.nnz = nnz;
+    m.kind = kind;
+    const size_t bytes = al(4 * (size_t)(N + 1)) + al(4 * (size_t)nnz) + al(8 * (size_t)nnz) + al(4 * (size_t)nnz) +
+                         al(4 * ((size_t)(nnz + N) / 512 + 16));
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->cfg.hbm_budget > 0 && s->used + (int64_t)bytes > s->cfg.hbm_budget)
+            return fail(SABLE_EBUDGET, "matrix of %zu bytes exceeds the HBM budget (%lld of %lld used)", bytes,
+                        (long long)s->used, (long long)s->cfg.hbm_budget);
+        s->used += (int64_t)bytes;
+    }
+    m.bytes = bytes;
+    hipStream_t st = s->ctx->stream;
+    bool ok = hipMalloc(&m.rp, 4 * (size_t)(N + 1)) == hipSuccess &&
+              hipMalloc(&m.col, 4 * (size_t)(nnz ? nnz : 1)) == hipSuccess &&
+              hipMalloc(&m.val64, 8 * (size_t)(nnz ? nnz : 1)) == hipSuccess &&
+              hipMalloc(&m.val32, 4 * (size_t)(nnz ? nnz : 1)) == hipSuccess;
+    int rc = SABLE_OK;
+    if (!ok) {
+        (void)hipGetLastError();
+        rc = fail(SABLE_ENOMEM, "matrix allocation failed");
+    } else if (hipMemcpyAsync(m.rp, row_ptr, 4 * (size_t)(N + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
+               hipMemcpyAsync(m.col, col, 4 * (size_t)nnz, hipMemcpyHostToDevice, st) != hipSuccess ||
+               hipMemcpyAsync(m.val64, val, 8 * (size_t)nnz, hipMemcpyHostToDevice, st) != hipSuccess) {
+        rc = fail(SABLE_EHIP, "matrix upload failed");
+    } else {
+        unsigned g = grid_for(nnz, 256);
+        if (g > 16384) g = 16384;
+        if (nnz) hipLaunchKernelGGL(k_f64_to_f32, dim3(g), dim3(256), 0, st, nnz, m.val64, m.val32);
+        rc = sable_csr_plan_create(s->ctx, N, m.rp, nullptr, &m.plan);
+        if (!rc) csr_plan_view(m.plan, &m.blk, &m.nblocks, &m.lds_cap);
+    }
+    if (rc) {
+        free_matrix(m);
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->used -= (int64_t)bytes;
+        return rc;
+    }
+    SABLE_HIP(hipStreamSynchronize(st));
+    std::lock_guard<std::mutex> lk(s->mu);
+    *matrix_id = (int32_t)s->mats.size();
+    s->mats.push_back(m);
+    return SABLE_OK;
+}
+
+static int submit_req(sable_server_s* s, int32_t mid, int op, const double* in, double* out, bool seeded,
+                      uint64_t seed, double t_arrival, int64_t* rid) {
+    SABLE_CHECK(mid >= 0 && mid < (int32_t)s->mats.size(), SABLE_EINVAL, "unknown matrix %d", mid);
+    SABLE_CHECK(op == 0 || op == 1, SABLE_EINVAL, "op must be 0 (SpMV) or 1 (lower solve)");
+    SABLE_CHECK(op == 0 || s->mats[mid].kind == 1, SABLE_EINVAL, "matrix %d is not lower triangular", mid);
+    SABLE_CHECK(seeded || in, SABLE_EINVAL, "NULL input");
+    auto r = std::make_unique<SRequest>();
+    r->mat = mid;
+    r->op = op;
+    r->in = in;
+    r->out = out;
+    r->seeded = seeded;
+    r->seed = seed;
+    r->t_arrival = t_arrival > 0 ? t_arrival : now_us();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        SABLE_CHECK(!s->stop, SABLE_EINVAL, "server is stopping");
+        r->id = s->next_id++;
+        if (rid) *rid = r->id;
+        s->pending.push_back(r.get());
+        s->reqs.emplace(r->id, std::move(r));
+    }
+    s->cv_pending.notify_one();
+    return SABLE_OK;
+}
+
+SABLE_API int sable_server_submit(sable_server_t* s, int32_t matrix_id, int op, const double* in, double* out,
+                                  int64_t* request_id) {
+    SABLE_CHECK(s, SABLE_EINVAL, "NULL server");
+    return submit_req(s, matrix_id, op, in, out, false, 0, 0, request_id);
+}
+
+SABLE_API int sable_server_wait(sable_server_t* s, int64_t request_id, double* latency_us) {
+    SABLE_CHECK(s, SABLE_EINVAL, "NULL server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto itr = s->reqs.find(request_id);
+    SABLE_CHECK(itr != s->reqs.end(), SABLE_EINVAL, "unknown request %lld", (long long)request_id);
+    SRequest* r = itr->second.get();
+    s->cv_done.wait(lk, [&] { return r->done; });
+    if (latency_us) *latency_us = r->t_done - r->t_arrival;
+    const int st = r->status;
+    s->reqs.erase(itr);
+    if (st) return fail(st, "request %lld failed", (long long)request_id);
+    return SABLE_OK;
+}
+
+SABLE_API int sable_server_run_load(sable_server_t* s, const sable_load_t* load, sable_load_stats_t* out) {
+    SABLE_CHECK(s && load && out, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(load->rate > 0 && load->duration_s > 0, SABLE_EINVAL, "rate and duration must be positive");
+    std::vector<int32_t> general, lower;
+    for (int32_t k = 0; k < (int32_t)s->mats.size(); k++) {
+        general.push_back(k);
+        if (s->mats[k].kind == 1) lower.push_back(k);
+    }
+    SABLE_CHECK(!general.empty(), SABLE_EINVAL, "no matrices");
+    SABLE_CHECK(load->spts_fraction <= 0 || !lower.empty(), SABLE_EINVAL, "no lower-triangular matrices");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->latencies.clear();
+        s->completed = s->rejected = s->batches = s->batched_reqs = 0;
+    }
+    std::mt19937_64 rng(load->seed ? load->seed : 1);
+    std::exponential_distribution<double> gap(load->rate);
+    std::uniform_real_distribution<double> u01(0.0, 1.0);
+    const double t0 = now_us();
+    double t = t0;
+    int64_t submitted = 0;
+    std::vector<int64_t> ids;
+    while (true) {
+        t += gap(rng) * 1e6;
+        if (t - t0 > load->duration_s * 1e6) break;
+        if (load->max_requests > 0 && submitted >= load->max_requests) break;
+        while (now_us() < t) {
+            const double d = t - now_us();
+            if (d > 200) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(d - 100)));
+        }
+        const bool spts = u01(rng) < load->spts_fraction;
+        const int32_t m = spts ? lower[rng() % lower.size()] : general[rng() % general.size()];
+        int64_t id;
+        int rc = submit_req(s, m, spts ? 1 : 0, nullptr, nullptr, true, rng(), t, &id);
+        if (rc) return rc;
+        ids.push_back(id);
+        submitted++;
+    }
+    int64_t fails = 0;
+    for (int64_t id : ids) {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto itr = s->reqs.find(id);
+        if (itr == s->reqs.end()) continue;
+        SRequest* r = itr->second.get();
+        s->cv_done.wait(lk, [&] { return r->done; });
+        if (r->status) fails++;
+        s->reqs.erase(itr);
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::vector<double> lat = s->latencies;
+    std::sort(lat.begin(), lat.end());
+    memset(out, 0, sizeof *out);
+    out->submitted = submitted;
+    out->completed = s->completed;
+    out->failed = fails;
+    out->rejected = s->rejected;
+    out->elapsed_s = (s->t_last_done - t0) * 1e-6;
+    out->solves_per_s = out->elapsed_s > 0 ? (double)s->completed / out->elapsed_s : 0;
+    if (!lat.empty()) {
+        out->p50_us = lat[lat.size() / 2];
+        out->p99_us = lat[std::min(lat.size() - 1, (size_t)(0.99 * (double)lat.size()))];
+        out->max_us = lat.back();
+        double sm = 0;
+        for (double v : lat) sm += v;
+        out->mean_us = sm / (double)lat.size();
+    }
+    out->batches = s->batches;
+    out->mean_batch = s->batches ? (double)s->batched_reqs / (double)s->batches : 0;
+    if (s->error) return fail(s->error, "server error: %s", s->errmsg);
+    return SABLE_OK;
+}
+
+SABLE_API int sable_server_info(sable_server_t* s, int64_t* hbm_used, int32_t* nmatrices) {
+    SABLE_CHECK(s, SABLE_EINVAL, "NULL server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (hbm_used) *hbm_used = s->used;
+    if (nmatrices) *nmatrices = (int32_t)s->mats.size();
+    return SABLE_OK;
+}
+
+}  // extern "C"

@@ -321,6 +321,29 @@ static int launch_csr(sable_ctx_t* ctx, const int32_t* blk, int64_t nb, const in
     return SABLE_OK;
 }
 
+int csr_launch(hipStream_t st, sable_dtype_t dt, const int32_t* blk, int64_t nb, const int32_t* rp,
+               const int32_t* col, const void* val, const void* x, void* y, double beta, int lds_cap) {
+    if (nb <= 0) return SABLE_OK;
+    SABLE_CHECK(nb < 0x7fffffffLL, SABLE_EINVAL, "too many row blocks");
+    if (dt == SABLE_F32)
+        hipLaunchKernelGGL((k_csr_block<float, 4>), dim3((unsigned)nb), dim3(BS), (size_t)lds_cap * 4, st, blk, nb, rp,
+                           col, (const float*)val, (const float*)x, (float*)y, (float)beta, lds_cap);
+    else if (dt == SABLE_F64)
+        hipLaunchKernelGGL((k_csr_block<double, 4>), dim3((unsigned)nb), dim3(BS), (size_t)lds_cap * 8, st, blk, nb, rp,
+                           col, (const double*)val, (const double*)x, (double*)y, beta, lds_cap);
+    else
+        return fail(SABLE_EINVAL, "bad dtype");
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+int csr_plan_view(const sable_csr_plan_t* p, const int32_t** blk, int64_t* nb, int* lds_cap) {
+    *blk = p->blk;
+    *nb = p->nblocks;
+    *lds_cap = p->lds_cap;
+    return SABLE_OK;
+}
+
 static int dispatch_csr(sable_ctx_t* ctx, sable_dtype_t dt, const int32_t* blk, int64_t nb, const int32_t* rp,
                         const int32_t* col, const void* val, const void* x, void* y, double beta,
                         int inner_max, int lds_cap) {
@@ -389,6 +412,13 @@ SABLE_API int sable_csr_plan_info(const sable_csr_plan_t* p, int64_t* nblocks, i
     if (nblocks) *nblocks = p->nblocks;
     if (nnz) *nnz = p->nnz;
     if (lds_cap) *lds_cap = p->lds_cap;
+    return SABLE_OK;
+}
+
+SABLE_API int sable_csr_plan_blocks(const sable_csr_plan_t* p, const int32_t** d_blk, int64_t* nblocks) {
+    SABLE_CHECK(p && d_blk && nblocks, SABLE_EINVAL, "NULL argument");
+    *d_blk = p->blk;
+    *nblocks = p->nblocks;
     return SABLE_OK;
 }
 

@@ -334,6 +334,20 @@ static void launch_syncfree(hipStream_t st, int G, int num_cu, int32_t N, const 
     }
 }
 
+int syncfree_launch(hipStream_t st, int num_cu, sable_dtype_t dt, int lanes, int32_t N, const int32_t* rp,
+                    const int32_t* col, const void* val, const void* b, void* x, unsigned* ticket, int* err,
+                    long long spin) {
+    if (N <= 0) return SABLE_OK;
+    if (dt == SABLE_F64)
+        launch_syncfree<double>(st, lanes, num_cu, N, rp, col, (const double*)val, (const double*)b, (double*)x, ticket,
+                                err, spin);
+    else
+        launch_syncfree<float>(st, lanes, num_cu, N, rp, col, (const float*)val, (const float*)b, (float*)x, ticket,
+                               err, spin);
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
 static int read_flag(sable_ctx_t* ctx, int* dflag, int* out) {
     SABLE_HIP(hipMemcpyAsync(ctx->pinned, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     SABLE_HIP(hipStreamSynchronize(ctx->stream));

@@ -1,0 +1,168 @@
+"""Batched launches and the request server (host side).
+
+``batch_spmv`` / ``batch_spts`` / ``batch_spts_refined`` run many matrices
+in one persistent launch; ``Server`` wraps the native request server
+(continuous batching over resident matrices, HBM-budget admission,
+p50/p99 latency).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, c_vp, c_i32, c_i64, c_dbl
+from .core import Context, DeviceArray, dtype_code
+
+
+class BatchSpmvItem(ctypes.Structure):
+    _fields_ = [("row_ptr", c_vp), ("col", c_vp), ("val", c_vp), ("blk", c_vp), ("x", c_vp), ("y", c_vp),
+                ("blk_begin", c_i64), ("beta", c_dbl)]
+
+
+class BatchSptsItem(ctypes.Structure):
+    _fields_ = [("row_ptr", c_vp), ("col", c_vp), ("val", c_vp), ("b", c_vp), ("x", c_vp),
+                ("row_begin", c_i64), ("N", c_i32), ("pad", c_i32)]
+
+
+class BatchRefineItem(ctypes.Structure):
+    _fields_ = [("row_ptr", c_vp), ("col", c_vp), ("val64", c_vp), ("val32", c_vp), ("b64", c_vp), ("x64", c_vp),
+                ("r32", c_vp), ("d32", c_vp), ("row_begin", c_i64), ("N", c_i32), ("pad", c_i32)]
+
+
+def _upload_items(items, ctx):
+    raw = bytes(memoryview((type(items[0]) * len(items))(*items)).cast("B"))
+    return DeviceArray.from_host(np.frombuffer(raw, np.uint8), ctx=ctx)
+
+
+def batch_spmv(mats, xs, ys, nrhs=1, beta=0.0, workgroups=0):
+    """ys[k] = mats[k].x[k] (+ beta*ys[k]) for every k in one launch.  x and y
+    are DeviceArrays of N*nrhs elements, row-major [N][nrhs]."""
+    ctx = mats[0].row.ctx
+    items, total = [], 0
+    for A, x, y in zip(mats, xs, ys):
+        blk, nb = c_vp(), c_i64()
+        call("sable_csr_plan_blocks", A.plan().handle, ctypes.byref(blk), ctypes.byref(nb))
+        items.append(BatchSpmvItem(A.row.ptr, A.col.ptr, A.val.ptr, blk.value, x.ptr, y.ptr, total, float(beta)))
+        total += nb.value
+    d = _upload_items(items, ctx)
+    call("sable_batch_spmv", ctx.handle, dtype_code(mats[0].dtype), int(nrhs), d.ptr, len(items), total,
+         int(workgroups))
+    ctx.sync()
+
+
+def batch_spts(mats, bs, xs, lanes=4):
+    ctx = mats[0].row.ctx
+    items, total = [], 0
+    for L, b, x in zip(mats, bs, xs):
+        items.append(BatchSptsItem(L.row.ptr, L.col.ptr, L.val.ptr, b.ptr, x.ptr, total, L.N, 0))
+        total += L.N
+    d = _upload_items(items, ctx)
+    call("sable_batch_spts", ctx.handle, dtype_code(mats[0].dtype), int(lanes), d.ptr, len(items), total)
+
+
+def batch_spts_refined(mats, bs, xs, lanes=4, iters=2):
+    """FP64 matrices; FP32 solves with FP64 refinement, all in one batch."""
+    ctx = mats[0].row.ctx
+    items, keep, total = [], [], 0
+    for L, b, x in zip(mats, bs, xs):
+        v32 = DeviceArray.from_host(L.val.numpy().astype(np.float32), ctx=ctx)
+        r32 = DeviceArray(L.N, np.float32, ctx)
+        d32 = DeviceArray(L.N, np.float32, ctx)
+        keep += [v32, r32, d32]
+        items.append(BatchRefineItem(L.row.ptr, L.col.ptr, L.val.ptr, v32.ptr, b.ptr, x.ptr, r32.ptr, d32.ptr, total,
+                                     L.N, 0))
+        total += L.N
+    d = _upload_items(items, ctx)
+    call("sable_batch_spts_refined", ctx.handle, int(lanes), d.ptr, len(items), total, int(iters))
+
+
+MODES = {"plain": 0, "batched": 1, "mixed": 2}
+
+
+class Server:
+    """The native request server (sable_server_*)."""
+
+    def __init__(self, mode="mixed", hbm_budget=0, max_batch=256, slots=3, arena_bytes=256 << 20, refine_iters=2,
+                 spts_lanes=4, ctx=None):
+        self.ctx = ctx if ctx is not None else Context.default()
+        cfg = _lib.ServerConfig(int(hbm_budget), MODES[mode], int(max_batch), int(slots), int(arena_bytes),
+                                int(refine_iters), int(spts_lanes))
+        h = c_vp()
+        call("sable_server_create", self.ctx.handle, ctypes.byref(cfg), ctypes.byref(h))
+        self.handle = h
+        self.mode = mode
+        self.sizes = []
+        self._inflight = {}
+
+    def add_matrix(self, row_ptr, col, val, lower=False):
+        row_ptr = np.ascontiguousarray(row_ptr, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        mid = c_i32()
+        call("sable_server_add_matrix", self.handle, len(row_ptr) - 1, row_ptr.ctypes.data, col.ctypes.data,
+             val.ctypes.data, 1 if lower else 0, ctypes.byref(mid))
+        self.sizes.append(len(row_ptr) - 1)
+        return mid.value
+
+    def submit(self, mid, op, vec):
+        vec = np.ascontiguousarray(vec, np.float64)
+        out = np.zeros(self.sizes[mid], np.float64)
+        rid = c_i64()
+        call("sable_server_submit", self.handle, int(mid), 0 if op == "spmv" else 1, vec.ctypes.data,
+             out.ctypes.data, ctypes.byref(rid))
+        self._inflight[rid.value] = (vec, out)
+        return rid.value
+
+    def wait(self, rid):
+        lat = c_dbl()
+        try:
+            call("sable_server_wait", self.handle, int(rid), ctypes.byref(lat))
+        finally:
+            vec, out = self._inflight.pop(rid, (None, None))
+        return out, lat.value
+
+    def run_load(self, rate, duration_s, spts_fraction=0.3, seed=1, max_requests=0):
+        spec = _lib.LoadSpec(float(rate), float(duration_s), float(spts_fraction), int(seed), int(max_requests))
+        st = _lib.LoadStats()
+        call("sable_server_run_load", self.handle, ctypes.byref(spec), ctypes.byref(st))
+        return {f: getattr(st, f) for f, _ in _lib.LoadStats._fields_}
+
+    def info(self):
+        used, n = c_i64(), c_i32()
+        call("sable_server_info", self.handle, ctypes.byref(used), ctypes.byref(n))
+        return {"hbm_used": used.value, "matrices": n.value}
+
+    def close(self):
+        if self.handle:
+            _lib.load().sable_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def lower_dominant(row_ptr, col, val):
+    """Lower triangle with the diagonal last and diagonal = 1 + sum |off|
+    (the diagonally dominant lower triangle of a square CSR)."""
+    N = len(row_ptr) - 1
+    rows = np.repeat(np.arange(N), np.diff(row_ptr))
+    keep = col < rows
+    r, c, v = rows[keep], col[keep], val[keep]
+    cnt = np.bincount(r, minlength=N) + 1
+    rp = np.zeros(N + 1, np.int64)
+    rp[1:] = np.cumsum(cnt)
+    out_c = np.empty(rp[-1], np.int32)
+    out_v = np.empty(rp[-1], np.float64)
+    # rows are contiguous in r (sorted), so entries keep their order
+    starts = np.zeros(N + 1, np.int64)
+    starts[1:] = np.cumsum(cnt - 1)
+    idx = rp[r] + (np.arange(len(r)) - starts[r])
+    out_c[idx] = c
+    out_v[idx] = v
+    absum = np.bincount(r, weights=np.abs(v), minlength=N)
+    out_c[rp[1:] - 1] = np.arange(N)
+    out_v[rp[1:] - 1] = 1.0 + absum
+    return rp.astype(np.int32), out_c, out_v

@@ -1,0 +1,110 @@
+"""Batched persistent launches against per-matrix launches and the oracle.
+
+* batch_spmv with one right-hand side runs the single-matrix block kernel's
+  arithmetic on the same row-block plan, so it must equal the per-matrix
+  launch bit for bit; with nrhs > 1 each right-hand side must meet the CSR
+  SpMV bound of test_spmv_gpu.py against the oracle.
+* batch_spts with lanes = 1 is bit-identical to the reference-order
+  sync-free solve; lanes > 1 and the FP32 + FP64-refinement variant must
+  reach relative residual <= 1e-12 against the long-double oracle."""
+import numpy as np
+import pytest
+
+import oracle
+import sciwasm.sparse_amd as sp
+from sciwasm.sparse_amd import gen, DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mats():
+    return gen.batch_csr(40, rows_lo=512, rows_hi=6000, seed=11)
+
+
+def bound(rp, col, val, x, dtype):
+    u = np.finfo(dtype).eps / 2
+    k = np.diff(rp).astype(np.float64) + 1
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    s = np.bincount(rows, weights=np.abs(val.astype(np.float64) * x.astype(np.float64)[col]), minlength=len(rp) - 1)
+    return 2 * k * u / (1 - k * u) * s + np.finfo(dtype).tiny
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batch_spmv_equals_single(ctx, mats, dtype):
+    As, xs, ys, ref = [], [], [], []
+    rng = np.random.default_rng(1)
+    for n, rp, c, v in mats:
+        A = sp.csr_from_host(n, rp, c, v.astype(dtype))
+        x = rng.uniform(-1, 1, n).astype(dtype)
+        As.append(A)
+        xs.append(DeviceArray.from_host(x))
+        ys.append(DeviceArray.zeros(n, dtype))
+        yv = sp.create_vec(n, 0.0, dtype)
+        sp.spmv(A, sp.vec_from_host(x), yv, beta=0.0)
+        ref.append(yv.array())
+    sp.batch_spmv(As, xs, ys, nrhs=1)
+    for k in range(len(As)):
+        assert np.array_equal(ys[k].numpy(), ref[k]), k
+
+
+@pytest.mark.parametrize("nrhs", [2, 4, 8])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batch_spmv_multi_rhs(ctx, mats, nrhs, dtype):
+    rng = np.random.default_rng(2)
+    As, xs, ys, X = [], [], [], []
+    for n, rp, c, v in mats[:12]:
+        As.append(sp.csr_from_host(n, rp, c, v.astype(dtype)))
+        x = rng.uniform(-1, 1, (n, nrhs)).astype(dtype)
+        X.append(x)
+        xs.append(DeviceArray.from_host(x.ravel()))
+        ys.append(DeviceArray.zeros(n * nrhs, dtype))
+    sp.batch_spmv(As, xs, ys, nrhs=nrhs)
+    for k, (n, rp, c, v) in enumerate(mats[:12]):
+        Y = ys[k].numpy().reshape(n, nrhs)
+        for r in range(nrhs):
+            yref = oracle.spmv_csr(n, rp, c, v.astype(dtype), X[k][:, r].copy(), np.zeros(n, dtype))
+            d = np.abs(Y[:, r].astype(np.float64) - yref)
+            assert np.all(d <= bound(rp, c, v.astype(dtype), X[k][:, r], dtype)), (k, r)
+
+
+def lower_set(mats, dtype=np.float64):
+    out = []
+    for n, rp, c, v in mats:
+        lrp, lc, lv = sp.lower_dominant(rp, c, v)
+        out.append((n, lrp, lc, lv.astype(dtype)))
+    return out
+
+
+@pytest.mark.parametrize("lanes", [1, 4, 16])
+def test_batch_spts(ctx, mats, lanes):
+    Ls = lower_set(mats)
+    rng = np.random.default_rng(3)
+    dL, bs, xs, bh = [], [], [], []
+    for n, rp, c, v in Ls:
+        b = rng.uniform(-1, 1, n)
+        bh.append(b)
+        dL.append(sp.csr_from_host(n, rp, c, v))
+        bs.append(DeviceArray.from_host(b))
+        xs.append(DeviceArray.zeros(n, np.float64))
+    sp.batch_spts(dL, bs, xs, lanes=lanes)
+    for k, (n, rp, c, v) in enumerate(Ls):
+        x = xs[k].numpy()
+        if lanes == 1:
+            assert np.array_equal(x, oracle.spts_syncfree(n, rp, c, v, bh[k])), k
+        assert oracle.rel_residual(n, rp, c, v, x, bh[k]) <= 1e-12, k
+
+
+def test_batch_spts_refined(ctx, mats):
+    Ls = lower_set(mats)
+    rng = np.random.default_rng(4)
+    dL, bs, xs, bh = [], [], [], []
+    for n, rp, c, v in Ls:
+        b = rng.uniform(-1, 1, n)
+        bh.append(b)
+        dL.append(sp.csr_from_host(n, rp, c, v))
+        bs.append(DeviceArray.from_host(b))
+        xs.append(DeviceArray.zeros(n, np.float64))
+    sp.batch_spts_refined(dL, bs, xs, lanes=4, iters=2)
+    for k, (n, rp, c, v) in enumerate(Ls):
+        assert oracle.rel_residual(n, rp, c, v, xs[k].numpy(), bh[k]) <= 1e-12, k

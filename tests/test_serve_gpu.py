@@ -1,0 +1,124 @@
+"""The request server: results alone and in a concurrent mix, accuracy
+against the oracle, HBM-budget admission, and a short open-loop run.
+
+Alone vs in the mix: every mode computes each request with an arithmetic
+that does not depend on its batch (the batched kernels keep each item's
+per-row order; the solves' per-row order is fixed by the lane layout), so
+the two results must be identical; the stated tolerance against the oracle
+is the CSR SpMV bound for SpMV (FP32 unit roundoff in mixed mode) and
+relative residual <= 1e-12 for the solves in every mode."""
+import numpy as np
+import pytest
+
+import oracle
+import sciwasm.sparse_amd as sp
+from sciwasm.sparse_amd import gen, SableError
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pool():
+    mats = gen.batch_csr(24, rows_lo=512, rows_hi=8000, seed=21)
+    lows = [(n,) + sp.lower_dominant(rp, c, v) for n, rp, c, v in mats]
+    return mats, lows
+
+
+def make_server(mode, pool, **kw):
+    mats, lows = pool
+    S = sp.Server(mode=mode, arena_bytes=64 << 20, **kw)
+    gids = [S.add_matrix(rp, c, v) for n, rp, c, v in mats]
+    lids = [S.add_matrix(rp, c, v, lower=True) for n, rp, c, v in lows]
+    return S, gids, lids
+
+
+def spmv_bound(rp, col, val, x, dtype):
+    u = np.finfo(dtype).eps / 2
+    k = np.diff(rp).astype(np.float64) + 1
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    s = np.bincount(rows, weights=np.abs(val * x[col]), minlength=len(rp) - 1)
+    # mixed mode also rounds x and the values to FP32: 2u more per product
+    return (2 * k + 4) * u / (1 - (k + 2) * u) * s + 1e-300
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched", "mixed"])
+def test_alone_equals_mix_and_accuracy(ctx, pool, mode):
+    mats, lows = pool
+    S, gids, lids = make_server(mode, pool)
+    rng = np.random.default_rng(5)
+    probes = []
+    for k in range(6):
+        n, rp, c, v = mats[k]
+        probes.append(("spmv", gids[k], rng.uniform(-1, 1, n), (n, rp, c, v)))
+        n, rp, c, v = lows[k]
+        probes.append(("spts", lids[k], rng.uniform(-1, 1, n), (n, rp, c, v)))
+    alone = []
+    for op, mid, vec, _ in probes:
+        out, lat = S.wait(S.submit(mid, op, vec))
+        alone.append(out.copy())
+    # the same requests interleaved with 200 others submitted at once
+    ids = []
+    for j in range(200):
+        k = rng.integers(len(gids))
+        if j % 3 == 0:
+            ids.append(S.submit(lids[k], "spts", rng.uniform(-1, 1, lows[k][0])))
+        else:
+            ids.append(S.submit(gids[k], "spmv", rng.uniform(-1, 1, mats[k][0])))
+    probe_ids = {}
+    for p, (op, mid, vec, _) in enumerate(probes):
+        probe_ids[p] = S.submit(mid, op, vec)
+        for j in range(10):
+            k = rng.integers(len(gids))
+            ids.append(S.submit(gids[k], "spmv", rng.uniform(-1, 1, mats[k][0])))
+    for p, (op, mid, vec, (n, rp, c, v)) in enumerate(probes):
+        out, lat = S.wait(probe_ids[p])
+        assert np.array_equal(out, alone[p]), (mode, p, op)
+        if op == "spmv":
+            yref, _ = oracle.spmv_csr_fp64(n, rp, c, v, vec)
+            dt = np.float32 if mode == "mixed" else np.float64
+            assert np.all(np.abs(out - yref) <= spmv_bound(rp, c, v, vec, dt)), (mode, p)
+        else:
+            assert oracle.rel_residual(n, rp, c, v, out, vec) <= 1e-12, (mode, p)
+    for i in ids:
+        S.wait(i)
+    S.close()
+
+
+def test_hbm_budget_admission(ctx, pool):
+    mats, lows = pool
+    S = sp.Server(mode="batched", hbm_budget=3 * ((16 << 20) + (4 << 20) + 4096) + (4 << 20),
+                  arena_bytes=16 << 20, slots=3)
+    n, rp, c, v = mats[0]
+    S.add_matrix(rp, c, v)
+    big = gen.band_csr(200000, [-2, -1, 0, 1, 2], seed=1)
+    with pytest.raises(SableError) as e:
+        S.add_matrix(*big)
+    assert e.value.name == "SABLE_EBUDGET"
+    S.close()
+    # a request that can never fit a batch arena is rejected, not hung
+    S = sp.Server(mode="batched", arena_bytes=1 << 20)
+    mid = S.add_matrix(*big)
+    with pytest.raises(SableError) as e:
+        S.wait(S.submit(mid, "spmv", np.ones(200000)))
+    assert e.value.name == "SABLE_EBUDGET"
+    S.close()
+
+
+def test_malformed_lower_rejected(ctx):
+    S = sp.Server(mode="batched", arena_bytes=1 << 20)
+    with pytest.raises(SableError) as e:
+        S.add_matrix(np.array([0, 2, 3], np.int32), np.array([1, 0, 1], np.int32), np.ones(3), lower=True)
+    assert e.value.name == "SABLE_EMALFORMED"
+    S.close()
+
+
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+def test_open_loop_load(ctx, pool, mode):
+    S, gids, lids = make_server(mode, pool)
+    st = S.run_load(rate=2000, duration_s=0.5, spts_fraction=0.3, seed=7)
+    assert st["failed"] == 0 and st["rejected"] == 0
+    assert st["completed"] == st["submitted"] > 0
+    assert 0 < st["p50_us"] <= st["p99_us"] <= st["max_us"]
+    if mode == "mixed":
+        assert st["mean_batch"] >= 1.0
+    S.close()
