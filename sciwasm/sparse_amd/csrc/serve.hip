@@ -90,6 +90,7 @@ struct Slot {
     size_t out_begin = 0, out_end = 0;
     std::vector<SRequest*> reqs;
     bool busy = false;
+    int launch_rc = 0;  // a failed launch fails the batch's requests
 };
 
 constexpr size_t ITEM_BYTES = 1 << 20;  // per slot, per item kind
@@ -134,7 +135,7 @@ struct sable_server_s {
     sable_ctx_t* ctx = nullptr;
     sable_server_config_t cfg{};
     int64_t used = 0;
-    std::vector<SMatrix> mats;
+    std::deque<SMatrix> mats;
     std::vector<Slot> slots;
     std::mutex mu;
     std::condition_variable cv_pending, cv_slot, cv_done, cv_complete;
@@ -143,6 +144,7 @@ struct sable_server_s {
     std::unordered_map<int64_t, std::unique_ptr<SRequest>> reqs;
     int64_t next_id = 1;
     bool stop = false;
+    bool sched_done = false;
     std::thread sched, compl_;
     // statistics
     std::vector<double> latencies;
@@ -171,9 +173,20 @@ void sable_server_s::scheduler() {
         {
             std::unique_lock<std::mutex> lk(mu);
             cv_pending.wait(lk, [&] { return stop || !pending.empty(); });
-            if (stop && pending.empty()) return;
-            cv_slot.wait(lk, [&] { return stop || !slots[next_slot].busy; });
-            if (stop && pending.empty()) return;
+            if (!stop) cv_slot.wait(lk, [&] { return stop || !slots[next_slot].busy; });
+            if (stop) {
+                // requests still queued at shutdown fail; in-flight batches finish
+                for (SRequest* r : pending) {
+                    r->status = SABLE_EINVAL;
+                    r->done = true;
+                    r->t_done = now_us();
+                }
+                pending.clear();
+                sched_done = true;
+                cv_done.notify_all();
+                cv_complete.notify_all();
+                return;
+            }
             sl = &slots[next_slot];
             const size_t cap = (size_t)cfg.arena_bytes;
             size_t used_b = 0;
@@ -199,8 +212,10 @@ void sable_server_s::scheduler() {
             sl->busy = true;
         }
         int rc = launch_batch(*sl, batch);
+        if (rc) (void)hipEventRecord(sl->ev, sl->stream);  // keep completion order
         {
             std::lock_guard<std::mutex> lk(mu);
+            sl->launch_rc = rc;
             if (rc) {
                 error = rc;
                 snprintf(errmsg, sizeof errmsg, "%s", sable_last_error());
@@ -319,7 +334,11 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
     memcpy(hi + spmv_off, sv.data(), sv.size() * sizeof(sv[0]));
     if (mixed) memcpy(hi + spts_off, rv.data(), rv.size() * sizeof(rv[0]));
     else memcpy(hi + spts_off, tv.data(), tv.size() * sizeof(tv[0]));
-    SABLE_HIP(hipMemcpyAsync(s.d_items, s.h_items, 3 * ITEM_BYTES, hipMemcpyHostToDevice, st));
+    const size_t fb = fills.size() * sizeof(FillItem), sb = sv.size() * sizeof(sv[0]);
+    const size_t tb = mixed ? rv.size() * sizeof(rv[0]) : tv.size() * sizeof(tv[0]);
+    if (fb) SABLE_HIP(hipMemcpyAsync(s.d_items + fill_off, hi + fill_off, fb, hipMemcpyHostToDevice, st));
+    if (sb) SABLE_HIP(hipMemcpyAsync(s.d_items + spmv_off, hi + spmv_off, sb, hipMemcpyHostToDevice, st));
+    if (tb) SABLE_HIP(hipMemcpyAsync(s.d_items + spts_off, hi + spts_off, tb, hipMemcpyHostToDevice, st));
     if (!fills.empty()) {
         unsigned g = grid_for(fill_total, 256);
         if (g > 16384) g = 16384;
@@ -368,7 +387,7 @@ void sable_server_s::completer() {
         int si;
         {
             std::unique_lock<std::mutex> lk(mu);
-            cv_complete.wait(lk, [&] { return !inflight.empty() || (stop && pending.empty()); });
+            cv_complete.wait(lk, [&] { return !inflight.empty() || sched_done; });
             if (inflight.empty()) return;
             si = inflight.front();
         }
@@ -392,7 +411,8 @@ void sable_server_s::completer() {
             std::lock_guard<std::mutex> lk(mu);
             for (auto* r : s.reqs) {
                 r->t_done = t;
-                r->status = e != hipSuccess ? SABLE_EHIP : (errflag ? SABLE_ETIMEOUT : SABLE_OK);
+                r->status = s.launch_rc ? s.launch_rc
+                                        : e != hipSuccess ? SABLE_EHIP : (errflag ? SABLE_ETIMEOUT : SABLE_OK);
                 r->done = true;
                 latencies.push_back(t - r->t_arrival);
                 completed++;
