@@ -41,6 +41,16 @@ def main():
     ap.add_argument("--zipf-s", type=float, default=1.5)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--panel-cols", type=str, default="0",
+                    help="comma list of column-panel widths to time (0 = 2 MiB of x); the best is reported")
+    ap.add_argument("--no-rowblock", action="store_true", help="skip timing the unblocked row-block kernel")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first run one 1 GiB device-to-device copy (a plain copy kernel of known size, used to "
+                         "scale hardware byte counters)")
+    ap.add_argument("--csr-kernel", type=str, default="1",
+                    help="comma list of CSR block kernels to time (0 basic, 1 staged, 2 persistent)")
+    ap.add_argument("--block-weight", type=str, default="0",
+                    help="comma list of row-block weights (nnz + 2*rows) to time with each panel width (0 = 1024)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the hot path on a single MI355X; "
@@ -62,23 +72,62 @@ def main():
     A = sp.csr_from_host(N, rp, col, val, ctx)
     xv = sp.vec_from_host(x, ctx=ctx)
     yv = sp.create_vec(N, 0.0, np.float64, ctx=ctx)
-    plan = A.plan()
-    log("plan:", plan.info())
-
-    for _ in range(args.warmup):
-        plan.run(xv.vec, yv.vec, beta=0.0)
-    ctx.sync()
-    e0, e1 = ctx.event(), ctx.event()
-    ctx.record(e0)
-    for _ in range(args.steps):
-        plan.run(xv.vec, yv.vec, beta=0.0)
-    ctx.record(e1)
-    ctx.sync()
-    ms = ctx.elapsed_ms(e0, e1) / args.steps
     bytes_step = 4 * (N + 1) + 12 * nnz + 8 * N + 8 * N
-    gbs = bytes_step / (ms * 1e-3) / 1e9
-    y = yv.array()
-    log("csr fp64: %.4f ms/step, %.1f GB/s (%.1f%% of HBM peak)" % (ms, gbs, 100 * gbs / HBM_PEAK_GBS))
+
+    def timed(run):
+        for _ in range(args.warmup):
+            run()
+        ctx.sync()
+        e0, e1 = ctx.event(), ctx.event()
+        ctx.record(e0)
+        for _ in range(args.steps):
+            run()
+        ctx.record(e1)
+        ctx.sync()
+        return ctx.elapsed_ms(e0, e1) / args.steps
+
+    if args.calib_copy:
+        a = sp.DeviceArray(1 << 27, np.float64, ctx)
+        b = sp.DeviceArray(1 << 27, np.float64, ctx)
+        a.fill(1.0)
+        from sciwasm.sparse_amd._lib import call as _call
+        _call("sable_memcpy_d2d", ctx.handle, b.ptr, a.ptr, a.nbytes)
+        ctx.sync()
+        del a, b
+
+    from sciwasm.sparse_amd import _lib as L
+    variants = {}
+    results = {}
+    kernels = [int(v) for v in args.csr_kernel.split(",") if v.strip()]
+    for kern in kernels:
+        L.call("sable_csr_kernel_select", kern)
+        if not args.no_rowblock:
+            plan = A.plan()
+            ms = timed(lambda: plan.run(xv.vec, yv.vec, beta=0.0))
+            key = "rowblock_k%d" % kern
+            variants[key] = {"ms": ms, "gbs": bytes_step / (ms * 1e-3) / 1e9, "launches_per_step": 1,
+                             "csr_kernel": kern}
+            results[key] = yv.array()
+            log(key, plan.info(), "%.4f ms  %.1f GB/s" % (ms, variants[key]["gbs"]))
+            del plan
+            A._plan = None
+        for pc in [int(v) for v in args.panel_cols.split(",") if v.strip()]:
+            for bw in [int(v) for v in args.block_weight.split(",") if v.strip()]:
+                pan = sp.CsrPanels(A, panel_cols=pc, tune={"block_weight": bw} if bw else None)
+                info = pan.info()
+                ms = timed(lambda: pan.run(xv.vec, yv.vec, beta=0.0))
+                key = "panels_%d_bw%d_k%d" % (info["panel_cols"], bw or 1024, kern)
+                variants[key] = {"ms": ms, "gbs": bytes_step / (ms * 1e-3) / 1e9, "launches_per_step": info["npanels"],
+                                 "npanels": info["npanels"], "panel_cols": info["panel_cols"],
+                                 "block_weight": bw or 1024, "csr_kernel": kern}
+                results[key] = yv.array()
+                log(key, info, "%.4f ms  %.1f GB/s" % (ms, variants[key]["gbs"]))
+                del pan
+    best = min(variants, key=lambda k: variants[k]["ms"])
+    ms = variants[best]["ms"]
+    gbs = variants[best]["gbs"]
+    y = results[best]
+    log("best %s: %.4f ms/step, %.1f GB/s (%.1f%% of HBM peak)" % (best, ms, gbs, 100 * gbs / HBM_PEAK_GBS))
 
     cpu = None
     check = None
@@ -118,9 +167,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": "fp64 csr spmv y=A.x, zipf(s=%.2f) rows truncated [1,4096], N=%d, nnz=%d, "
                                "uniform columns, N(0,1) values, int32 indices" % (args.zipf_s, N, nnz),
-                   "N": N, "nnz": nnz, "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2)},
+                   "N": N, "nnz": nnz, "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
+                   "schedule": best,
+                   "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                                for k, v in variants.items()}},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_csr_block", "launches_per_step": variants[best]["launches_per_step"]},
         "cpu_baseline": cpu,
         "check": check,
     }

@@ -101,9 +101,31 @@ SABLE_API int sable_csr_plan_info(const sable_csr_plan_t* plan, int64_t* nblocks
 SABLE_API int sable_spmv_csr_plan(sable_ctx_t* ctx, const sable_csr_plan_t* plan, sable_dtype_t dtype,
                                   const int32_t* col, const void* val, const void* x, void* y,
                                   double beta, int inner_max);
+/* Selects the CSR block kernel for all later launches (process-wide): 0 =
+ * basic block kernel, one workgroup per row block (default); 1 = the kernel
+ * with staged row pointers and early y loads, one workgroup per row block;
+ * 2 = that kernel on a persistent grid of 8 workgroups per CU. */
+SABLE_API int sable_csr_kernel_select(int kernel);
 SABLE_API int sable_spmv_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
                              const int32_t* col, const void* val, const void* x, void* y,
                              double beta, int inner_max);
+
+/* Column-panel blocked CSR (no reference counterpart; a schedule of the same
+ * CSR product for matrices whose x does not fit one XCD's L2): the columns
+ * are cut into panels of panel_cols columns (0 = 2 MiB of x) and the matrix
+ * is copied panel-major; one launch per panel keeps every x gather of that
+ * launch inside an L2-resident slice of x.  The plan owns its copy of the
+ * matrix (the caller's arrays may be freed afterwards).  Same y = A.x +
+ * beta*y semantics and tolerance as sable_spmv_csr (the per-row sum is
+ * accumulated panel by panel). */
+typedef struct sable_csr_panels_s sable_csr_panels_t;
+SABLE_API int sable_csr_panels_create(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                                      const int32_t* col, const void* val, int32_t panel_cols,
+                                      const sable_csr_tune_t* tune, sable_csr_panels_t** out);
+SABLE_API void sable_csr_panels_destroy(sable_csr_panels_t* p);
+SABLE_API int sable_csr_panels_info(const sable_csr_panels_t* p, int32_t* npanels, int32_t* panel_cols, int64_t* bytes);
+SABLE_API int sable_spmv_csr_panels(sable_ctx_t* ctx, const sable_csr_panels_t* p, const void* x, void* y, double beta,
+                                    int inner_max);
 
 /* COO.  Row-sorted input runs a deterministic segmented reduction (row
  * heads found on the device, then the CSR block kernel); unsorted input is

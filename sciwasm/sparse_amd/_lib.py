@@ -77,7 +77,12 @@ SIGNATURES = {
     "sable_csr_plan_destroy": [c_vp],
     "sable_csr_plan_info": [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64), ctypes.POINTER(c_i32)],
     "sable_spmv_csr_plan": [c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
+    "sable_csr_kernel_select": [c_int],
     "sable_spmv_csr": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
+    "sable_csr_panels_create": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_i32, ctypes.POINTER(CsrTune), c_vpp],
+    "sable_csr_panels_destroy": [c_vp],
+    "sable_csr_panels_info": [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32), ctypes.POINTER(c_i64)],
+    "sable_spmv_csr_panels": [c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_spmv_coo": [c_vp, c_int, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_spmv_dia": [c_vp, c_int, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_spmv_ell": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
@@ -120,6 +125,7 @@ _RESTYPES = {
     "sable_ctx_destroy": None,
     "sable_csr_plan_destroy": None,
     "sable_spts_destroy": None,
+    "sable_csr_panels_destroy": None,
     "sable_server_destroy": None,
     "sable_ctx_stream": c_vp,
     "sable_last_error": ctypes.c_char_p,

@@ -111,19 +111,25 @@ __global__ __launch_bounds__(BS) void k_csr_block(const int32_t* __restrict__ bl
     if (nnzb <= lds_cap) {
         const int32_t* cb = col + nz0;
         const T* vb = val + nz0;
-        int k = tid;
-        for (; k + (U - 1) * BS < nnzb; k += U * BS) {
+        // All loads of an unrolled step are issued before any is used, with
+        // out-of-range slots predicated off, so a short block still keeps U
+        // gathers per thread in flight.
+        for (int k = tid; k < nnzb; k += U * BS) {
             int c[U];
             T v[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) { c[u] = cb[k + u * BS]; v[u] = vb[k + u * BS]; }
+            for (int u = 0; u < U; u++) {
+                const int kk = k + u * BS;
+                c[u] = kk < nnzb ? cb[kk] : 0;
+                v[u] = kk < nnzb ? vb[kk] : T(0);
+            }
             T xv[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) xv[u] = x[c[u]];
+            for (int u = 0; u < U; u++) xv[u] = (k + u * BS < nnzb) ? x[c[u]] : T(0);
 #pragma unroll
-            for (int u = 0; u < U; u++) prod[k + u * BS] = v[u] * xv[u];
+            for (int u = 0; u < U; u++)
+                if (k + u * BS < nnzb) prod[k + u * BS] = v[u] * xv[u];
         }
-        for (; k < nnzb; k += BS) prod[k] = vb[k] * x[cb[k]];
         __syncthreads();
         int L = WAVE;
         while (L > 1 && L * R > BS) L >>= 1;
@@ -156,6 +162,127 @@ __global__ __launch_bounds__(BS) void k_csr_block(const int32_t* __restrict__ bl
             for (; k < e; k += BS) s += val[k] * x[col[k]];
             s = block_sum(s, red);
             if (tid == 0) write_y(y, r, s, beta, e > a);
+        }
+    }
+}
+
+// Persistent variant of the block kernel.  A grid of about 8 workgroups
+// per CU walks the blocks (XCD x owns the contiguous x-th eighth of them;
+// its workgroups stride through it), so no workgroup is dispatched per
+// block.  Per block the dependent memory round trips are cut to three: the
+// next block's boundaries are prefetched while the current one runs, and
+// the block's row-pointer slice and the y values (beta != 0) are loaded in
+// the same step as col/val.
+constexpr int RPS_CAP = 520;
+
+template <typename T, int U>
+__global__ __launch_bounds__(BS) void k_csr_persist(const int32_t* __restrict__ blk, int64_t nblocks,
+                                                    const int32_t* __restrict__ rp,
+                                                    const int32_t* __restrict__ col,
+                                                    const T* __restrict__ val, const T* __restrict__ x,
+                                                    T* __restrict__ y, T beta, int lds_cap) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* prod = reinterpret_cast<T*>(smem_raw);
+    __shared__ T red[BS / WAVE];
+    __shared__ int32_t rps[RPS_CAP + 1];
+    const int tid = threadIdx.x;
+    const int64_t G = gridDim.x;
+    const int64_t xcd = blockIdx.x % NUM_XCD, idx = blockIdx.x / NUM_XCD;
+    const int64_t gx = G / NUM_XCD + (xcd < G % NUM_XCD ? 1 : 0);  // workgroups on this XCD
+    const int64_t b_lo = nblocks * xcd / NUM_XCD, b_hi = nblocks * (xcd + 1) / NUM_XCD;
+    int64_t b = b_lo + idx;
+    int nr0 = 0, nr1 = 0;
+    if (b < b_hi) {
+        nr0 = blk[b];
+        nr1 = blk[b + 1];
+    }
+    for (; b < b_hi; b += gx) {
+        const int r0 = nr0, r1 = nr1;
+        // prefetch the boundaries of this workgroup's next block
+        if (b + gx < b_hi) {
+            nr0 = blk[b + gx];
+            nr1 = blk[b + gx + 1];
+        }
+        if (r0 >= r1) continue;
+        const int64_t nz0 = rp[r0];
+        const int nnzb = (int)((int64_t)rp[r1] - nz0);
+        const int R = r1 - r0;
+        if (nnzb <= lds_cap) {
+            int L = WAVE;
+            while (L > 1 && L * R > BS) L >>= 1;
+            const int groups = BS / L, g = tid / L, l = tid & (L - 1);
+            const bool stage_rp = R < RPS_CAP;
+            if (stage_rp)
+                for (int k = tid; k <= R; k += BS) rps[k] = rp[r0 + k];
+            // y for the (single) row of each group, when every group has at most one row
+            T yold = T(0);
+            const bool pre_y = beta != T(0) && R <= groups && l == 0 && g < R;
+            if (pre_y) yold = y[r0 + g];
+            const int32_t* cb = col + nz0;
+            const T* vb = val + nz0;
+            for (int k = tid; k < nnzb; k += U * BS) {
+                int c[U];
+                T v[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int kk = k + u * BS;
+                    c[u] = kk < nnzb ? cb[kk] : 0;
+                    v[u] = kk < nnzb ? vb[kk] : T(0);
+                }
+                T xv[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) xv[u] = (k + u * BS < nnzb) ? x[c[u]] : T(0);
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (k + u * BS < nnzb) prod[k + u * BS] = v[u] * xv[u];
+            }
+            __syncthreads();
+            for (int rb = 0; rb < R; rb += groups) {
+                const int r = r0 + rb + g;
+                T s = 0;
+                int a = 0, e = 0;
+                if (r < r1) {
+                    if (stage_rp) {
+                        a = rps[rb + g] - (int)(rps[0]);
+                        e = rps[rb + g + 1] - (int)(rps[0]);
+                    } else {
+                        a = (int)((int64_t)rp[r] - nz0);
+                        e = (int)((int64_t)rp[r + 1] - nz0);
+                    }
+                    for (int kk = a + l; kk < e; kk += L) s += prod[kk];
+                }
+                s = group_sum(s, L);
+                if (l == 0 && r < r1) {
+                    if (beta == T(0)) {
+                        y[r] = s;
+                    } else if (e > a || beta != T(1)) {
+                        const T yo = pre_y ? yold : y[r];
+                        y[r] = (beta == T(1)) ? yo + s : beta * yo + s;
+                    }
+                }
+            }
+            __syncthreads();  // prod and rps are reused by the next block
+        } else {
+            for (int r = r0; r < r1; r++) {
+                const int64_t a = rp[r], e = rp[r + 1];
+                T s = 0;
+                for (int64_t k = a + tid; k < e; k += U * BS) {
+                    int c[U];
+                    T v[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int64_t kk = k + u * BS;
+                        c[u] = kk < e ? col[kk] : 0;
+                        v[u] = kk < e ? val[kk] : T(0);
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        if (k + u * BS < e) s += v[u] * x[c[u]];
+                }
+                s = block_sum(s, red);
+                if (tid == 0) write_y(y, r, s, beta, e > a);
+                __syncthreads();
+            }
         }
     }
 }
@@ -307,18 +434,36 @@ static int plan_fill(sable_ctx_t* ctx, int32_t N, const int32_t* rp, int T, int 
     return SABLE_OK;
 }
 
-template <typename T>
-static int launch_csr(sable_ctx_t* ctx, const int32_t* blk, int64_t nb, const int32_t* rp, const int32_t* col,
-                      const void* val, const void* x, void* y, double beta, int inner_max, int lds_cap) {
-    if (nb <= 0) return SABLE_OK;
-    SABLE_CHECK(nb < 0x7fffffffLL, SABLE_EINVAL, "too many row blocks");
-    size_t shm = (size_t)lds_cap * sizeof(T);
-    for (int it = 0; it < inner_max; it++) {
-        hipLaunchKernelGGL((k_csr_block<T, 4>), dim3((unsigned)nb), dim3(BS), shm, ctx->stream, blk, nb, rp, col,
-                           (const T*)val, (const T*)x, (T*)y, (T)beta, lds_cap);
-        SABLE_LAUNCH_CHECK();
-    }
+int64_t plan_nblocks(int64_t nnz, int32_t N, int T, int C) { return nblocks_for(nnz, N, T, C); }
+int plan_lds_cap(int T) { return lds_cap_for(T); }
+int plan_fill_stream(hipStream_t st, int32_t N, const int32_t* rp, int T, int C, int64_t B, int32_t* blk, int* err) {
+    if (N <= 0) return SABLE_OK;
+    hipLaunchKernelGGL(k_plan_fill, dim3(grid_for((int64_t)N + 1, BS)), dim3(BS), 0, st, N, rp, (int64_t)C, (int64_t)T,
+                       B, blk, err);
+    SABLE_LAUNCH_CHECK();
     return SABLE_OK;
+}
+
+int g_num_cu = 256;
+// 0: k_csr_block, one workgroup per block; 1: k_csr_persist with one
+// workgroup per block; 2: k_csr_persist with ~8 workgroups per CU.
+int g_csr_kernel = 0;
+
+template <typename T>
+static void csr_launch_t(hipStream_t st, const int32_t* blk, int64_t nb, const int32_t* rp, const int32_t* col,
+                         const void* val, const void* x, void* y, double beta, int lds_cap) {
+    const size_t shm = (size_t)lds_cap * sizeof(T);
+    if (g_csr_kernel >= 1) {
+        // at least one workgroup per XCD, so that every XCD's block range is covered
+        int64_t grid = g_csr_kernel == 2 ? (int64_t)g_num_cu * 8 : nb;
+        if (grid > nb) grid = nb;
+        if (grid < NUM_XCD) grid = NUM_XCD;
+        hipLaunchKernelGGL((k_csr_persist<T, 8>), dim3((unsigned)grid), dim3(BS), shm, st, blk, nb, rp, col,
+                           (const T*)val, (const T*)x, (T*)y, (T)beta, lds_cap);
+    } else {
+        hipLaunchKernelGGL((k_csr_block<T, 8>), dim3((unsigned)nb), dim3(BS), shm, st, blk, nb, rp, col,
+                           (const T*)val, (const T*)x, (T*)y, (T)beta, lds_cap);
+    }
 }
 
 int csr_launch(hipStream_t st, sable_dtype_t dt, const int32_t* blk, int64_t nb, const int32_t* rp,
@@ -326,11 +471,9 @@ int csr_launch(hipStream_t st, sable_dtype_t dt, const int32_t* blk, int64_t nb,
     if (nb <= 0) return SABLE_OK;
     SABLE_CHECK(nb < 0x7fffffffLL, SABLE_EINVAL, "too many row blocks");
     if (dt == SABLE_F32)
-        hipLaunchKernelGGL((k_csr_block<float, 4>), dim3((unsigned)nb), dim3(BS), (size_t)lds_cap * 4, st, blk, nb, rp,
-                           col, (const float*)val, (const float*)x, (float*)y, (float)beta, lds_cap);
+        csr_launch_t<float>(st, blk, nb, rp, col, val, x, y, beta, lds_cap);
     else if (dt == SABLE_F64)
-        hipLaunchKernelGGL((k_csr_block<double, 4>), dim3((unsigned)nb), dim3(BS), (size_t)lds_cap * 8, st, blk, nb, rp,
-                           col, (const double*)val, (const double*)x, (double*)y, beta, lds_cap);
+        csr_launch_t<double>(st, blk, nb, rp, col, val, x, y, beta, lds_cap);
     else
         return fail(SABLE_EINVAL, "bad dtype");
     SABLE_LAUNCH_CHECK();
@@ -347,9 +490,12 @@ int csr_plan_view(const sable_csr_plan_t* p, const int32_t** blk, int64_t* nb, i
 static int dispatch_csr(sable_ctx_t* ctx, sable_dtype_t dt, const int32_t* blk, int64_t nb, const int32_t* rp,
                         const int32_t* col, const void* val, const void* x, void* y, double beta,
                         int inner_max, int lds_cap) {
-    if (dt == SABLE_F32) return launch_csr<float>(ctx, blk, nb, rp, col, val, x, y, beta, inner_max, lds_cap);
-    if (dt == SABLE_F64) return launch_csr<double>(ctx, blk, nb, rp, col, val, x, y, beta, inner_max, lds_cap);
-    return fail(SABLE_EINVAL, "bad dtype %d", (int)dt);
+    g_num_cu = ctx->num_cu;
+    for (int it = 0; it < inner_max; it++) {
+        int rc = csr_launch(ctx->stream, dt, blk, nb, rp, col, val, x, y, beta, lds_cap);
+        if (rc) return rc;
+    }
+    return SABLE_OK;
 }
 
 }  // namespace sable
@@ -412,6 +558,12 @@ SABLE_API int sable_csr_plan_info(const sable_csr_plan_t* p, int64_t* nblocks, i
     if (nblocks) *nblocks = p->nblocks;
     if (nnz) *nnz = p->nnz;
     if (lds_cap) *lds_cap = p->lds_cap;
+    return SABLE_OK;
+}
+
+SABLE_API int sable_csr_kernel_select(int kernel) {
+    SABLE_CHECK(kernel >= 0 && kernel <= 2, SABLE_EINVAL, "kernel must be 0, 1 or 2");
+    g_csr_kernel = kernel;
     return SABLE_OK;
 }
 

@@ -42,6 +42,38 @@ class CsrPlan:
             pass
 
 
+class CsrPanels:
+    """Column-panel blocked CSR plan (sable_csr_panels_*): a panel-major copy
+    of the matrix, one launch per panel of panel_cols columns."""
+
+    def __init__(self, A, panel_cols=0, tune=None):
+        self.ctx = A.row.ctx
+        self.A = A
+        h = c_vp()
+        t = None
+        if tune is not None:
+            t = ctypes.byref(_lib.CsrTune(int(tune.get("block_weight", 0)), int(tune.get("row_weight", 0))))
+        call("sable_csr_panels_create", self.ctx.handle, dtype_code(A.dtype), A.N, A.row.ptr, A.col.ptr, A.val.ptr,
+             int(panel_cols), t, ctypes.byref(h))
+        self.handle = h
+
+    def info(self):
+        n, w, b = c_i32(), c_i32(), c_i64()
+        call("sable_csr_panels_info", self.handle, ctypes.byref(n), ctypes.byref(w), ctypes.byref(b))
+        return {"npanels": n.value, "panel_cols": w.value, "bytes": b.value}
+
+    def run(self, x, y, beta=1.0, inner_max=1):
+        call("sable_spmv_csr_panels", self.ctx.handle, self.handle, x.ptr, y.ptr, float(beta), int(inner_max))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.load().sable_csr_panels_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
 def _check_vec(A, x, y):
     if x.vec.dtype != A.dtype or y.vec.dtype != A.dtype:
         raise TypeError("x, y must have the matrix dtype %s" % A.dtype)

@@ -50,10 +50,18 @@ CASES = [
 ]
 
 
+@pytest.fixture(params=[0, 1, 2], ids=["basic", "staged", "persistent"])
+def csr_kernel(request):
+    from sciwasm.sparse_amd import _lib
+    _lib.call("sable_csr_kernel_select", request.param)
+    yield request.param
+    _lib.call("sable_csr_kernel_select", 0)
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("name,make", CASES)
 @pytest.mark.parametrize("inner_max", [1, 3])
-def test_csr_parity(ctx, dtype, name, make, inner_max):
+def test_csr_parity(ctx, csr_kernel, dtype, name, make, inner_max):
     rp, col, val = make(dtype)
     N = len(rp) - 1
     rng = np.random.default_rng(7)
@@ -76,7 +84,7 @@ def test_csr_parity(ctx, dtype, name, make, inner_max):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_csr_beta_zero_and_general(ctx, dtype):
+def test_csr_beta_zero_and_general(ctx, csr_kernel, dtype):
     rp, col, val = gen.random_csr(2000, 9, seed=3, dtype=dtype)
     N = len(rp) - 1
     rng = np.random.default_rng(8)
@@ -193,3 +201,28 @@ def test_ell_zero_width_leaves_y(ctx):
     yv = sp.vec_from_host(np.arange(5, dtype=np.float64))
     sp.spmv(A, sp.create_vec(5, 1.0, np.float64), yv)
     assert np.array_equal(yv.array(), np.arange(5))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("panel_cols", [1000, 4096, 0])
+@pytest.mark.parametrize("beta", [0.0, 1.0])
+def test_csr_panels_parity(ctx, csr_kernel, dtype, panel_cols, beta):
+    rp, col, val = gen.zipf_csr(1 << 16, 1.5, 4096, seed=31, dtype=dtype)
+    N = len(rp) - 1
+    rng = np.random.default_rng(12)
+    x = rng.uniform(-1, 1, N).astype(dtype)
+    y0 = rng.uniform(-1, 1, N).astype(dtype)
+    A = sp.csr_from_host(N, rp, col, val)
+    P = sp.CsrPanels(A, panel_cols=panel_cols)
+    info = P.info()
+    assert info["npanels"] == (1 if panel_cols == 0 else -(-N // panel_cols))
+    yv = sp.vec_from_host(y0)
+    P.run(sp.vec_from_host(x).vec, yv.vec, beta=beta, inner_max=2 if beta else 1)
+    y = yv.array()
+    if beta:
+        yref = oracle.spmv_csr(N, rp, col, val, x, y0, 2)
+        b = bound(rp, col, val, x, y0, dtype) * 2 + bound(rp, col, val, x, yref, dtype)
+    else:
+        yref = oracle.spmv_csr(N, rp, col, val, x, np.zeros(N, dtype))
+        b = bound(rp, col, val, x, np.zeros(N, dtype), dtype)
+    check_close(y, yref, b)
