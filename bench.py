@@ -1,24 +1,40 @@
 #!/usr/bin/env python3
-"""Headline benchmark: FP64 CSR SpMV on one MI355X.
+"""Benchmark of the Level-II hot path on one MI355X.
 
-Workload: random 4,194,304 x 4,194,304 matrix, row lengths from the Zipf
-law with s = 1.5 truncated to [1, 4096] (renormalised; mean about 49, about
-205M nonzeros), per-row columns uniform without replacement, values N(0,1),
-x ~ U(-1,1); int32 indices, float64 values.  One step is one y = A.x over
-the whole matrix with all inputs resident in HBM.
+Headline (the JSON line's `value`): FP64 CSR SpMV y = A.x on a random
+4,194,304 x 4,194,304 matrix whose row lengths follow the Zipf law with
+s = 1.5 truncated to [1, 4096] (renormalised; mean about 49, about 206M
+nonzeros), per-row columns uniform without replacement, values N(0,1),
+x ~ U(-1,1), int32 indices, float64 values; inputs resident in HBM.  One
+step is one whole product.  Bytes per step (algorithmic) = 4(N+1) +
+12 nnz + 8N (x once) + 8N (y).  The fastest of the measured schedules is
+reported (row-block CSR, or the column-panel blocked plan of the same CSR).
 
-Reported: whole-job algorithmic GB/s of the CSR kernel (bytes per step =
-4(N+1) + 12 nnz + 8N (x once) + 8N (y)), its fraction of the 8 TB/s HBM
-peak, and, as a reported baseline, the serial CPU restatement of the
-reference kernel timed on a quarter of the rows on this host.
+Also measured and reported under "extra" (each section bounded to seconds):
+  c1      FP32 5-point Laplacian 256x256 (65,536 rows): CSR/ELL/DIA/COO
+          SpMV in microseconds and GB/s, reference checksum protocol
+  band    FP64 7-diagonal band, 16,777,216 rows (offsets 0, +-1, +-64,
+          +-4096): csr_dia / csr_ell / coo_csr / transpose on the GPU and
+          DIA/ELL/CSR SpMV GB/s
+  convert coo_csr of the headline matrix (shuffled COO, 206M entries)
+  spts    lower solves, 3D 7-point Laplacian 128^3 and a random 1,048,576-row
+          lower matrix: analysis, level-set (hipGraph), sync-free, FP32 +
+          FP64 refinement, residuals
+  batch   2,048 matrices in one persistent launch vs one launch per matrix
+  serve   open-loop Poisson serving, plain FP64 one launch per request vs
+          batched FP32 + FP64 refinement: solves/s, p50/p99 latency
+The CPU baseline is the serial C restatement of the reference kernels on
+this host (the reference's own Wasm needs a WAT assembler and Node >= 16.4,
+which this image does not have).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--sections ...]
 """
 import argparse
 import json
 import os
 import sys
 import time
+import traceback
 
 import numpy as np
 
@@ -32,81 +48,64 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--rows", type=int, default=4194304)
-    ap.add_argument("--zipf-s", type=float, default=1.5)
-    ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--panel-cols", type=str, default="0",
-                    help="comma list of column-panel widths to time (0 = 2 MiB of x); the best is reported")
-    ap.add_argument("--no-rowblock", action="store_true", help="skip timing the unblocked row-block kernel")
-    ap.add_argument("--calib-copy", action="store_true",
-                    help="first run one 1 GiB device-to-device copy (a plain copy kernel of known size, used to "
-                         "scale hardware byte counters)")
-    ap.add_argument("--csr-kernel", type=str, default="1",
-                    help="comma list of CSR block kernels to time (0 basic, 1 staged, 2 persistent)")
-    ap.add_argument("--block-weight", type=str, default="0",
-                    help="comma list of row-block weights (nnz + 2*rows) to time with each panel width (0 = 1024)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs the hot path on a single MI355X; "
-              "multi-GPU sharding is a design note, not a measured configuration.", file=sys.stderr)
-        sys.exit(2)
+class Timer:
+    def __init__(self, ctx):
+        self.ctx = ctx
 
-    import sciwasm.sparse_amd as sp
-    from sciwasm.sparse_amd import gen
+    def __call__(self, run, steps, warmup):
+        for _ in range(warmup):
+            run()
+        self.ctx.sync()
+        e0, e1 = self.ctx.event(), self.ctx.event()
+        self.ctx.record(e0)
+        for _ in range(steps):
+            run()
+        self.ctx.record(e1)
+        self.ctx.sync()
+        return self.ctx.elapsed_ms(e0, e1) / steps
 
+    def once(self, run):
+        self.ctx.sync()
+        e0, e1 = self.ctx.event(), self.ctx.event()
+        self.ctx.record(e0)
+        r = run()
+        self.ctx.record(e1)
+        self.ctx.sync()
+        return self.ctx.elapsed_ms(e0, e1), r
+
+
+def r4(v):
+    return round(float(v), 4)
+
+
+# ------------------------------------------------------------------ headline
+def headline(args, sp, ctx, T):
+    from sciwasm.sparse_amd import gen, _lib as L
     N = args.rows
     t0 = time.time()
     rp, col, val = gen.zipf_csr(N, args.zipf_s, 4096, seed=args.seed, dtype=np.float64)
     nnz = int(rp[-1])
     x = np.random.default_rng(args.seed + 1).uniform(-1.0, 1.0, N)
-    log("generated N=%d nnz=%d (mean %.2f, max %d) in %.1fs" % (N, nnz, nnz / N, np.diff(rp).max(),
-                                                                 time.time() - t0))
-
-    ctx = sp.Context.default()
+    log("generated N=%d nnz=%d (mean %.2f, max %d) in %.1fs" % (N, nnz, nnz / N, np.diff(rp).max(), time.time() - t0))
     A = sp.csr_from_host(N, rp, col, val, ctx)
     xv = sp.vec_from_host(x, ctx=ctx)
     yv = sp.create_vec(N, 0.0, np.float64, ctx=ctx)
     bytes_step = 4 * (N + 1) + 12 * nnz + 8 * N + 8 * N
-
-    def timed(run):
-        for _ in range(args.warmup):
-            run()
-        ctx.sync()
-        e0, e1 = ctx.event(), ctx.event()
-        ctx.record(e0)
-        for _ in range(args.steps):
-            run()
-        ctx.record(e1)
-        ctx.sync()
-        return ctx.elapsed_ms(e0, e1) / args.steps
-
     if args.calib_copy:
         a = sp.DeviceArray(1 << 27, np.float64, ctx)
         b = sp.DeviceArray(1 << 27, np.float64, ctx)
         a.fill(1.0)
-        from sciwasm.sparse_amd._lib import call as _call
-        _call("sable_memcpy_d2d", ctx.handle, b.ptr, a.ptr, a.nbytes)
+        L.call("sable_memcpy_d2d", ctx.handle, b.ptr, a.ptr, a.nbytes)
         ctx.sync()
         del a, b
-
-    from sciwasm.sparse_amd import _lib as L
-    variants = {}
-    results = {}
-    kernels = [int(v) for v in args.csr_kernel.split(",") if v.strip()]
-    for kern in kernels:
+    variants, results = {}, {}
+    for kern in [int(v) for v in args.csr_kernel.split(",") if v.strip()]:
         L.call("sable_csr_kernel_select", kern)
         if not args.no_rowblock:
             plan = A.plan()
-            ms = timed(lambda: plan.run(xv.vec, yv.vec, beta=0.0))
+            ms = T(lambda: plan.run(xv.vec, yv.vec, beta=0.0), args.steps, args.warmup)
             key = "rowblock_k%d" % kern
-            variants[key] = {"ms": ms, "gbs": bytes_step / (ms * 1e-3) / 1e9, "launches_per_step": 1,
-                             "csr_kernel": kern}
+            variants[key] = {"ms": ms, "gbs": bytes_step / (ms * 1e-3) / 1e9, "launches_per_step": 1, "csr_kernel": kern}
             results[key] = yv.array()
             log(key, plan.info(), "%.4f ms  %.1f GB/s" % (ms, variants[key]["gbs"]))
             del plan
@@ -115,22 +114,19 @@ def main():
             for bw in [int(v) for v in args.block_weight.split(",") if v.strip()]:
                 pan = sp.CsrPanels(A, panel_cols=pc, tune={"block_weight": bw} if bw else None)
                 info = pan.info()
-                ms = timed(lambda: pan.run(xv.vec, yv.vec, beta=0.0))
+                ms = T(lambda: pan.run(xv.vec, yv.vec, beta=0.0), args.steps, args.warmup)
                 key = "panels_%d_bw%d_k%d" % (info["panel_cols"], bw or 1024, kern)
-                variants[key] = {"ms": ms, "gbs": bytes_step / (ms * 1e-3) / 1e9, "launches_per_step": info["npanels"],
-                                 "npanels": info["npanels"], "panel_cols": info["panel_cols"],
-                                 "block_weight": bw or 1024, "csr_kernel": kern}
+                variants[key] = {"ms": ms, "gbs": bytes_step / (ms * 1e-3) / 1e9,
+                                 "launches_per_step": info["npanels"], "npanels": info["npanels"],
+                                 "panel_cols": info["panel_cols"], "block_weight": bw or 1024, "csr_kernel": kern}
                 results[key] = yv.array()
                 log(key, info, "%.4f ms  %.1f GB/s" % (ms, variants[key]["gbs"]))
                 del pan
+    L.call("sable_csr_kernel_select", 0)
     best = min(variants, key=lambda k: variants[k]["ms"])
-    ms = variants[best]["ms"]
-    gbs = variants[best]["gbs"]
-    y = results[best]
+    ms, gbs, y = variants[best]["ms"], variants[best]["gbs"], results[best]
     log("best %s: %.4f ms/step, %.1f GB/s (%.1f%% of HBM peak)" % (best, ms, gbs, 100 * gbs / HBM_PEAK_GBS))
-
-    cpu = None
-    check = None
+    cpu, check = None, None
     if not args.no_cpu:
         import oracle
         R = N // 4
@@ -141,17 +137,352 @@ def main():
                "sample": "serial C restatement of seq/src/sparse_opt_64.wat spmv_csr on rows [0, N/4) "
                          "(%d rows, %d nnz) of the same matrix, 10 warm-up + 30 timed calls, %.4f s/call"
                          % (R, nnz_s, t)}
-        # the GPU result against the reference-order CPU result on the sample rows
         lens = np.diff(rp[:R + 1]).astype(np.float64) + 1
-        ab = np.add.reduceat(np.abs(val[:nnz_s] * x[col[:nnz_s]]), rp[:R]) if R else np.zeros(0)
+        ab = np.add.reduceat(np.abs(val[:nnz_s] * x[col[:nnz_s]]), rp[:R])
         ab[np.diff(rp[:R + 1]) == 0] = 0
         u = np.finfo(np.float64).eps / 2
-        rel = np.abs(y[:R] - ycpu) / np.maximum(ab, 1e-300)
-        check = {"rows_checked": R, "max_err_over_sum_abs": float(rel.max()),
-                 "bound_2gamma_k": float((2 * lens * u / (1 - lens * u)).max()),
-                 "ok": bool(np.all(np.abs(y[:R] - ycpu) <= 2 * lens * u / (1 - lens * u) * ab + 1e-300))}
+        bnd = 2 * lens * u / (1 - lens * u) * ab
+        check = {"rows_checked": R,
+                 "max_abs_err_over_bound": float(np.max(np.abs(y[:R] - ycpu) / np.maximum(bnd, 1e-300))),
+                 "ok": bool(np.all(np.abs(y[:R] - ycpu) <= bnd + 1e-300))}
         log("cpu baseline:", cpu, "check:", check)
+    head = {"N": N, "nnz": nnz, "ms": ms, "gbs": gbs, "best": best, "bytes_step": bytes_step,
+            "variants": {k: {kk: (r4(vv) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                         for k, v in variants.items()},
+            "cpu": cpu, "check": check}
+    host = (rp, col, val, x)
+    del A, xv, yv
+    return head, host
 
+
+# ------------------------------------------------------------------ c1
+def sec_c1(args, sp, ctx, T):
+    import oracle
+    from sciwasm.sparse_amd import gen
+    n = 256
+    rp, col, val = gen.laplacian_2d(n, np.float32)
+    N = n * n
+    nnz = len(col)
+    x = np.random.default_rng(1).uniform(-1, 1, N).astype(np.float32)
+    A = sp.csr_from_host(N, rp, col, val, ctx)
+    E = sp.csr_ell(A)
+    D = sp.csr_dia(A)
+    row = gen.csr_to_coo_rows(rp)
+    C = sp.coo_from_host(N, row, col, val, ctx)
+    xv, yv = sp.vec_from_host(x, ctx=ctx), sp.create_vec(N, 0.0, np.float32, ctx=ctx)
+    out = {"N": N, "nnz": nnz}
+    fmt_bytes = {"csr": 4 * (N + 1) + 8 * nnz + 8 * N, "ell": 8 * E.ncols * N + 8 * N,
+                 "dia": 4 * D.ndiags * N + 4 * D.ndiags + 8 * N, "coo": 12 * nnz + 8 * N}
+    for name, M in (("csr", A), ("ell", E), ("dia", D), ("coo", C)):
+        ms = T(lambda: sp.spmv(M, xv, yv, beta=0.0), 200, 20)
+        out[name] = {"us": r4(ms * 1e3), "gbs": r4(fmt_bytes[name] / (ms * 1e-3) / 1e9)}
+    # reference protocol (seq/tests/main32.js): x[i] = i, y = 0, y += A.x, checksum
+    sp.init_x(xv)
+    sp.clear_vec(yv)
+    sp.spmv(A, xv, yv)
+    ydev = yv.array()
+    yref = oracle.spmv_csr(N, rp, col, val, np.arange(N, dtype=np.float32), np.zeros(N, np.float32))
+    out["fletcher_gpu"] = oracle.fletcher_sum(ydev)
+    out["fletcher_ref_order"] = oracle.fletcher_sum(yref)
+    out["max_abs_diff_vs_ref_order"] = float(np.max(np.abs(ydev - yref)))
+    # CPU baseline of the same product: serial restatement, timed in C over many calls
+    t0 = time.perf_counter()
+    reps = 300
+    oracle.spmv_csr(N, rp, col, val, x, np.zeros(N, np.float32), reps)
+    t = (time.perf_counter() - t0) / reps
+    out["cpu_serial_us"] = r4(t * 1e6)
+    log("c1:", out)
+    return out
+
+
+# ------------------------------------------------------------------ band
+def sec_band(args, sp, ctx, T):
+    from sciwasm.sparse_amd import gen
+    N = 1 << 24
+    offs = [-4096, -64, -1, 0, 1, 64, 4096]
+    t0 = time.time()
+    rp, col, val = gen.band_csr(N, offs, seed=5)
+    nnz = len(col)
+    log("band generated nnz=%d in %.1fs" % (nnz, time.time() - t0))
+    x = np.random.default_rng(2).uniform(-1, 1, N)
+    A = sp.csr_from_host(N, rp, col, val, ctx)
+    xv, yv = sp.vec_from_host(x, ctx=ctx), sp.create_vec(N, 0.0, np.float64, ctx=ctx)
+    out = {"N": N, "nnz": nnz, "offsets": offs}
+    t_dia, D = T.once(lambda: sp.csr_dia(A))
+    t_ell, E = T.once(lambda: sp.csr_ell(A))
+    out["csr_dia_ms"] = r4(t_dia)
+    out["csr_ell_ms"] = r4(t_ell)
+    out["dia_guard"] = "accepted" if D is not None else "refused"
+    b_dia = 8 * D.ndiags * N + 4 * D.ndiags + 16 * N
+    b_ell = 12 * E.ncols * N + 16 * N
+    b_csr = 4 * (N + 1) + 12 * nnz + 16 * N
+    for name, M, b in (("dia", D, b_dia), ("ell", E, b_ell), ("csr", A, b_csr)):
+        ms = T(lambda: sp.spmv(M, xv, yv, beta=0.0), 20, 3)
+        out[name + "_spmv"] = {"ms": r4(ms), "gbs": r4(b / (ms * 1e-3) / 1e9),
+                               "pct_hbm": r4(100 * b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)}
+    t_tr, Tt = T.once(lambda: sp.transpose(A))
+    out["transpose_csr_ms"] = r4(t_tr)
+    del Tt, E
+    row = gen.csr_to_coo_rows(rp)
+    p = np.random.default_rng(3).permutation(nnz)
+    C = sp.coo_from_host(N, row[p], col[p], val[p], ctx)
+    del row, p
+    t_cc, B = T.once(lambda: sp.coo_csr(C))
+    out["coo_csr_ms"] = r4(t_cc)
+    out["coo_csr_matches_csr"] = bool(np.array_equal(B.col.numpy(), col) and np.array_equal(B.row.numpy(), rp))
+    log("band:", out)
+    return out
+
+
+# ------------------------------------------------------------------ convert
+def sec_convert(args, sp, ctx, T, host):
+    rp, col, val, x = host
+    N = len(rp) - 1
+    nnz = len(col)
+    from sciwasm.sparse_amd import gen
+    row = gen.csr_to_coo_rows(rp)
+    p = np.random.default_rng(4).permutation(nnz)
+    C = sp.coo_from_host(N, row[p], col[p], val[p], ctx)
+    del row, p
+    t_cc, B = T.once(lambda: sp.coo_csr(C))
+    ok = bool(np.array_equal(B.row.numpy(), rp))
+    del B, C
+    A = sp.csr_from_host(N, rp, col, val, ctx)
+    E = sp.csr_ell(A)
+    out = {"coo_csr_ms": r4(t_cc), "coo_csr_entries_per_s": r4(nnz / (t_cc * 1e-3)), "row_ptr_ok": ok,
+           "csr_ell": "refused by the reference guard" if E is None else "accepted"}
+    log("convert:", out)
+    return out
+
+
+# ------------------------------------------------------------------ spts
+def sec_spts(args, sp, ctx, T):
+    import oracle
+    from sciwasm.sparse_amd import gen
+    out = {}
+    for name, make in (("lap3d_128", lambda: gen.laplacian_3d_lower(128)),
+                       ("random_lower_1M", lambda: gen.random_lower(1 << 20, 32, seed=7))):
+        rp, col, val = make()
+        N, nnz = len(rp) - 1, len(col)
+        b = np.random.default_rng(8).uniform(-1, 1, N)
+        L = sp.csr_from_host(N, rp, col, val, ctx)
+        t0 = time.time()
+        an = sp.SptsAnalysis(L)
+        t_an = time.time() - t0
+        bv = sp.vec_from_host(b, ctx=ctx)
+        xv = sp.create_vec(N, 0.0, np.float64, ctx=ctx)
+        flops = 2 * nnz - N
+        res = {"N": N, "nnz": nnz, "nlevels": an.nlevels, "lanes": an.lanes, "analysis_s": r4(t_an)}
+        for meth, run in (("level_graph", lambda: an.solve_level(bv, xv)),
+                          ("syncfree", lambda: an.solve_syncfree(bv, xv)),
+                          ("fp32_refine2", lambda: an.solve_refined(bv, xv, iters=2))):
+            ms = T(run, 10, 2)
+            an.status()
+            rr = oracle.rel_residual(N, rp, col, val, xv.array(), b)
+            res[meth] = {"ms": r4(ms), "gflops": r4(flops / (ms * 1e-3) / 1e9), "rel_residual": rr}
+        an1 = sp.SptsAnalysis(L, lanes=1)
+        ms = T(lambda: an1.solve_syncfree(bv, xv), 10, 2)
+        res["syncfree_lanes1_ms"] = r4(ms)
+        t0 = time.perf_counter()
+        oracle.spts_syncfree(N, rp, col, val, b)
+        res["cpu_serial_ms"] = r4((time.perf_counter() - t0) * 1e3)
+        out[name] = res
+        log("spts", name, res)
+        del an, an1, L
+    return out
+
+
+# ------------------------------------------------------------------ batch
+def sec_batch(args, sp, ctx, T, pool):
+    import ctypes
+    from sciwasm.sparse_amd import _lib as L
+    from sciwasm.sparse_amd.serve import BatchSpmvItem, _upload_items
+    from sciwasm.sparse_amd._lib import c_vp, c_i64
+    ns, rs, rp, col, val = pool
+    count = len(ns)
+    mats, xs, ys = [], [], []
+    t0 = time.time()
+    for k in range(count):
+        a, b = rs[k], rs[k + 1]
+        lrp = (rp[a:b + 1] - rp[a]).astype(np.int32)
+        A = sp.csr_from_host(int(ns[k]), lrp, col[rp[a]:rp[b]], val[rp[a]:rp[b]], ctx)
+        A.plan()
+        mats.append(A)
+        xs.append(sp.DeviceArray(int(ns[k]) * 8, np.float64, ctx))
+        ys.append(sp.DeviceArray(int(ns[k]) * 8, np.float64, ctx))
+    for xx in xs:
+        xx.fill(0.5)
+    log("batch: %d matrices uploaded in %.1fs" % (count, time.time() - t0))
+    nnz = int(rp[-1])
+    rows = int(rs[-1])
+    out = {"matrices": count, "rows": rows, "nnz": nnz}
+    for nrhs in (1, 8):
+        items, total = [], 0
+        for A, xx, yy in zip(mats, xs, ys):
+            blk, nb = c_vp(), c_i64()
+            L.call("sable_csr_plan_blocks", A.plan().handle, ctypes.byref(blk), ctypes.byref(nb))
+            items.append(BatchSpmvItem(A.row.ptr, A.col.ptr, A.val.ptr, blk.value, xx.ptr, yy.ptr, total, 0.0))
+            total += nb.value
+        d = _upload_items(items, ctx)
+        ms_b = T(lambda: L.call("sable_batch_spmv", ctx.handle, 1, nrhs, d.ptr, count, total, 0), 10, 2)
+        bytes_ = 4 * (rows + count) + 12 * nnz + 16 * rows * nrhs
+        res = {"batched_ms": r4(ms_b), "batched_gbs": r4(bytes_ / (ms_b * 1e-3) / 1e9)}
+        if nrhs == 1:
+            def per_matrix():
+                for A, xx, yy in zip(mats, xs, ys):
+                    A.plan().run(xx, yy, beta=0.0)
+            ms_p = T(per_matrix, 3, 1)
+            res["one_launch_per_matrix_ms"] = r4(ms_p)
+            res["speedup"] = r4(ms_p / ms_b)
+        out["spmv_nrhs%d" % nrhs] = res
+        log("batch spmv nrhs=%d" % nrhs, res)
+    del mats, xs, ys
+    return out
+
+
+def make_lower_pool(pool, count):
+    """Diagonally dominant lower triangles of the first `count` matrices."""
+    from sciwasm.sparse_amd import lower_dominant
+    ns, rs, rp, col, val = pool
+    out = []
+    for k in range(count):
+        a, b = rs[k], rs[k + 1]
+        lrp = (rp[a:b + 1] - rp[a]).astype(np.int32)
+        out.append((int(ns[k]),) + lower_dominant(lrp, col[rp[a]:rp[b]], val[rp[a]:rp[b]]))
+    return out
+
+
+def sec_batch_spts(args, sp, ctx, T, lows):
+    Ls, bs, xs = [], [], []
+    for n, lrp, lc, lv in lows:
+        Ls.append(sp.csr_from_host(n, lrp, lc, lv, ctx))
+        bs.append(sp.DeviceArray.from_host(np.full(n, 0.5), ctx=ctx))
+        xs.append(sp.DeviceArray(n, np.float64, ctx))
+    rows = sum(lo[0] for lo in lows)
+    ms_b = T(lambda: sp.batch_spts(Ls, bs, xs, lanes=4), 5, 1)
+    ans = [sp.SptsAnalysis(Lm, lanes=4) for Lm in Ls[:64]]
+    bvs = [sp.sparse_vec_t(b, Lm.N) for b, Lm in zip(bs[:64], Ls[:64])]
+    xvs = [sp.sparse_vec_t(x, Lm.N) for x, Lm in zip(xs[:64], Ls[:64])]
+
+    def per():
+        for a, b, x in zip(ans, bvs, xvs):
+            a.solve_syncfree(b, x)
+    ms_p64 = T(per, 3, 1)
+    out = {"matrices": len(lows), "rows": rows, "batched_syncfree_ms": r4(ms_b),
+           "one_launch_per_matrix_ms_first64": r4(ms_p64),
+           "one_launch_per_matrix_ms_est_all": r4(ms_p64 * len(lows) / 64)}
+    out["speedup_est"] = r4(out["one_launch_per_matrix_ms_est_all"] / ms_b)
+    log("batch spts:", out)
+    return out
+
+
+# ------------------------------------------------------------------ serve
+def sec_serve(args, sp, ctx, pool, lows, big):
+    out = {}
+    rates = [float(r) for r in args.serve_rates.split(",")]
+    ns, rs, rp, col, val = pool
+    nsmall = min(512, len(ns))
+    for mode in ("plain", "mixed"):
+        S = sp.Server(mode=mode, arena_bytes=512 << 20, max_batch=512, slots=3, refine_iters=2, spts_lanes=4, ctx=ctx)
+        t0 = time.time()
+        for k in range(nsmall):
+            a, b = rs[k], rs[k + 1]
+            S.add_matrix((rp[a:b + 1] - rp[a]).astype(np.int32), col[rp[a]:rp[b]], val[rp[a]:rp[b]])
+        for n, lrp, lc, lv in lows[:nsmall]:
+            S.add_matrix(lrp, lc, lv, lower=True)
+        if big is not None:
+            for _ in range(args.serve_large):
+                S.add_matrix(*big)
+        out["pool"] = {"general": nsmall, "lower": min(nsmall, len(lows)),
+                       "large": args.serve_large if big is not None else 0,
+                       "hbm_used_gb": r4(S.info()["hbm_used"] / 1e9)}
+        log("serve %s: pool registered in %.1fs" % (mode, time.time() - t0), out["pool"])
+        res = []
+        for rate in rates:
+            st = S.run_load(rate=rate, duration_s=args.serve_seconds, spts_fraction=0.3, seed=int(rate))
+            row = {k: (r4(v) if isinstance(v, float) else v) for k, v in st.items()}
+            row["rate"] = rate
+            res.append(row)
+            log("serve %s rate %.0f:" % (mode, rate), row)
+        out[mode] = res
+        S.close()
+    best = {m: max(r["solves_per_s"] for r in out[m]) for m in ("plain", "mixed")}
+    out["max_solves_per_s"] = best
+    out["speedup_mixed_vs_plain"] = r4(best["mixed"] / max(best["plain"], 1e-9))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rows", type=int, default=4194304)
+    ap.add_argument("--zipf-s", type=float, default=1.5)
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--panel-cols", type=str, default="0,524288",
+                    help="comma list of column-panel widths to time (0 = 2 MiB of x); the best is reported")
+    ap.add_argument("--no-rowblock", action="store_true", help="skip timing the unblocked row-block kernel")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first run one 1 GiB device-to-device copy (a plain copy kernel of known size, used to "
+                         "scale hardware byte counters)")
+    ap.add_argument("--csr-kernel", type=str, default="0",
+                    help="comma list of CSR block kernels to time (0 basic, 1 staged, 2 persistent)")
+    ap.add_argument("--block-weight", type=str, default="0",
+                    help="comma list of row-block weights (nnz + 2*rows) to time with each panel width (0 = 1024)")
+    ap.add_argument("--sections", type=str, default="c1,band,convert,spts,batch,serve",
+                    help="comma list of extra sections (empty for the headline only)")
+    ap.add_argument("--batch-count", type=int, default=2048)
+    ap.add_argument("--serve-rates", type=str, default="2000,10000,30000,60000")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--serve-large", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs the hot path on a single MI355X; "
+              "multi-GPU sharding is a design note, not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+
+    import sciwasm.sparse_amd as sp
+    ctx = sp.Context.default()
+    T = Timer(ctx)
+    head, host = headline(args, sp, ctx, T)
+    sections = [s for s in args.sections.split(",") if s.strip()]
+    extra = {}
+
+    def run(name, fn):
+        t0 = time.time()
+        try:
+            extra[name] = fn()
+        except Exception as e:  # a failing section is reported; the headline line is still printed
+            extra[name] = {"error": "%s: %s" % (type(e).__name__, e)}
+            log(traceback.format_exc())
+        log("section %s done in %.1fs" % (name, time.time() - t0))
+
+    if "c1" in sections:
+        run("c1", lambda: sec_c1(args, sp, ctx, T))
+    if "band" in sections:
+        run("band", lambda: sec_band(args, sp, ctx, T))
+    if "convert" in sections:
+        run("convert", lambda: sec_convert(args, sp, ctx, T, host))
+    if "spts" in sections:
+        run("spts", lambda: sec_spts(args, sp, ctx, T))
+    pool = lows = None
+    if "batch" in sections or "serve" in sections:
+        from sciwasm.sparse_amd import gen
+        t0 = time.time()
+        pool = gen.batch_csr_concat(args.batch_count, seed=args.seed + 4)
+        log("pool generated: %d matrices, %d rows, %d nnz in %.1fs" % (len(pool[0]), pool[1][-1], pool[2][-1],
+                                                                      time.time() - t0))
+        lows = make_lower_pool(pool, min(512, args.batch_count))
+    if "batch" in sections:
+        run("batch", lambda: sec_batch(args, sp, ctx, T, pool))
+        run("batch_spts", lambda: sec_batch_spts(args, sp, ctx, T, lows))
+    if "serve" in sections:
+        big = (host[0], host[1], host[2]) if args.serve_large > 0 else None
+        run("serve", lambda: sec_serve(args, sp, ctx, pool, lows, big))
+
+    gbs, ms = head["gbs"], head["ms"]
     out = {
         "metric": "CSR SpMV GB/s (% HBM peak); served solves/s + p50/p99 latency vs plain",
         "value": round(gbs, 2),
@@ -166,17 +497,23 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "fp64 csr spmv y=A.x, zipf(s=%.2f) rows truncated [1,4096], N=%d, nnz=%d, "
-                               "uniform columns, N(0,1) values, int32 indices" % (args.zipf_s, N, nnz),
-                   "N": N, "nnz": nnz, "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
-                   "schedule": best,
-                   "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
-                                for k, v in variants.items()}},
+                               "uniform columns, N(0,1) values, int32 indices" % (args.zipf_s, head["N"], head["nnz"]),
+                   "N": head["N"], "nnz": head["nnz"], "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
+                   "schedule": head["best"], "variants": head["variants"]},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_csr_block", "launches_per_step": variants[best]["launches_per_step"]},
-        "cpu_baseline": cpu,
-        "check": check,
+                     "kernel": "k_csr_block",
+                     "launches_per_step": head["variants"][head["best"]]["launches_per_step"]},
+        "cpu_baseline": head["cpu"],
+        "check": head["check"],
     }
+    if "serve" in extra and "max_solves_per_s" in extra["serve"]:
+        sv = extra["serve"]
+        out["serving"] = {"max_solves_per_s": sv["max_solves_per_s"],
+                          "speedup_mixed_vs_plain": sv["speedup_mixed_vs_plain"],
+                          "p50_p99_us": {m: [(r["rate"], r["p50_us"], r["p99_us"]) for r in sv[m]]
+                                         for m in ("plain", "mixed")}}
+    out["extra"] = extra
     print(json.dumps(out), flush=True)
 
 

@@ -64,6 +64,10 @@ SABLE_API const char* sable_version(void);
  * (seq/libs/matmachjs-lib.js, used at seq/src/sparse32.js:2174-2395). */
 SABLE_API int sable_device_alloc(sable_ctx_t* ctx, size_t bytes, void** dptr);
 SABLE_API int sable_device_free(sable_ctx_t* ctx, void* dptr);
+/* Pinned (page-locked) host memory, e.g. for a host-side linear memory
+ * that is copied to and from the device. */
+SABLE_API int sable_host_alloc(size_t bytes, void** out);
+SABLE_API int sable_host_free(void* p);
 SABLE_API int sable_memcpy_h2d(sable_ctx_t* ctx, void* dst, const void* src, size_t bytes);
 SABLE_API int sable_memcpy_d2h(sable_ctx_t* ctx, void* dst, const void* src, size_t bytes);
 SABLE_API int sable_memcpy_d2d(sable_ctx_t* ctx, void* dst, const void* src, size_t bytes);

@@ -64,6 +64,8 @@ SIGNATURES = {
     "sable_version": [],
     "sable_device_alloc": [c_vp, c_sz, c_vpp],
     "sable_device_free": [c_vp, c_vp],
+    "sable_host_alloc": [c_sz, c_vpp],
+    "sable_host_free": [c_vp],
     "sable_memcpy_h2d": [c_vp, c_vp, c_vp, c_sz],
     "sable_memcpy_d2h": [c_vp, c_vp, c_vp, c_sz],
     "sable_memcpy_d2d": [c_vp, c_vp, c_vp, c_sz],

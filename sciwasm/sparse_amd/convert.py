@@ -3,11 +3,13 @@
 The storage guards are applied here, where the reference applies them (in
 its JS): csr_ell and csr_dia print the reference's message and return None
 when the padded storage would reach 2^27 slots or exceed 5x nnz
-(seq/src/sparse32.js:1457-1463, 1525-1531).  ``guard=False`` lifts the
+(seq/src/sparse32.js:1457-1463, 1525-1531); the message goes to stderr so
+that stdout stays free for the caller's data.  ``guard=False`` lifts the
 2^27 limit, which only exists because of the reference's 2 GiB wasm32
 memory; the 5x ratio check stays.
 """
 import ctypes
+import sys
 
 import numpy as np
 
@@ -46,7 +48,7 @@ def csr_ell(A, guard=True):
     """seq/src/sparse32.js:1449-1484; None when the guard refuses."""
     nc = num_cols(A)
     if not _allowed(nc * A.N, A.nnz, guard):
-        print("not efficient to store this matrix in ELL")
+        print("not efficient to store this matrix in ELL", file=sys.stderr)
         return None
     ctx = A.row.ctx
     idx = DeviceArray(nc * A.N, np.int32, ctx)
@@ -71,7 +73,7 @@ def csr_dia(A, guard=True, stride_mode="N"):
     """seq/src/sparse32.js:1515-1574; None when the guard refuses."""
     nd, stride = num_diags(A, stride_mode)
     if not _allowed(nd * stride, A.nnz, guard):
-        print("not efficient to store this matrix in DIA")
+        print("not efficient to store this matrix in DIA", file=sys.stderr)
         return None
     ctx = A.row.ctx
     off = DeviceArray(max(nd, 1), np.int32, ctx)

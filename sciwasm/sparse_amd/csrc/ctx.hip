@@ -152,6 +152,22 @@ SABLE_API int sable_device_free(sable_ctx_t* ctx, void* dptr) {
     return SABLE_OK;
 }
 
+SABLE_API int sable_host_alloc(size_t bytes, void** out) {
+    SABLE_CHECK(out, SABLE_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (hipHostMalloc(out, bytes ? bytes : 64, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        *out = nullptr;
+        return fail(SABLE_ENOMEM, "pinned host allocation of %zu bytes failed", bytes);
+    }
+    return SABLE_OK;
+}
+
+SABLE_API int sable_host_free(void* p) {
+    if (p) SABLE_HIP(hipHostFree(p));
+    return SABLE_OK;
+}
+
 SABLE_API int sable_memcpy_h2d(sable_ctx_t* ctx, void* dst, const void* src, size_t bytes) {
     SABLE_CHECK(ctx, SABLE_EINVAL, "ctx is NULL");
     if (!bytes) return SABLE_OK;

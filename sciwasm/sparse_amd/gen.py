@@ -79,7 +79,8 @@ def zipf_lengths(N, s, kmax, rng):
 
 
 def _sorted_distinct_cols(lens, N, rng):
-    """Per row, len distinct uniform columns in [0, N), ascending."""
+    """Per row, len distinct uniform columns in [0, N), ascending (N may be
+    a per-row array)."""
     m = lens.astype(np.int64)
     tot = int(m.sum())
     nrow = len(m)
@@ -93,7 +94,8 @@ def _sorted_distinct_cols(lens, N, rng):
     keep[grp_end] = False
     rid = np.repeat(np.arange(nrow), m)
     u = (cs[keep] - base[rid]) / total[rid]
-    cols = np.minimum((u * N).astype(np.int64), N - 1)
+    Nr = N[rid] if isinstance(N, np.ndarray) else N
+    cols = np.minimum((u * Nr).astype(np.int64), np.asarray(Nr) - 1)
     # repair the rare rows with a repeated column
     dup = np.zeros(tot, bool)
     if tot > 1:
@@ -103,7 +105,7 @@ def _sorted_distinct_cols(lens, N, rng):
         row_start[1:] = np.cumsum(m)
         for r in np.unique(rid[dup]):
             a, b = row_start[r], row_start[r + 1]
-            cols[a:b] = np.sort(rng.choice(N, b - a, replace=False))
+            cols[a:b] = np.sort(rng.choice(int(N[r]) if isinstance(N, np.ndarray) else N, b - a, replace=False))
     return cols.astype(np.int32)
 
 
@@ -221,3 +223,44 @@ def batch_csr(count, rows_lo=512, rows_hi=131072, lam=12, kmax=64, seed=0, dtype
 def csr_to_coo_rows(row_ptr):
     N = len(row_ptr) - 1
     return np.repeat(np.arange(N, dtype=np.int32), np.diff(row_ptr))
+
+
+def batch_csr_concat(count, rows_lo=512, rows_hi=131072, lam=12, kmax=64, seed=0, dtype=np.float64,
+                     chunk_rows=1 << 20):
+    """The batch_csr law for many matrices at once, concatenated: returns
+    (ns, row_start, row_ptr, col, val) where matrix k owns global rows
+    [row_start[k], row_start[k+1]), row_ptr is global (int64) and col holds
+    local column indices."""
+    ss = np.random.SeedSequence(seed)
+    rng = np.random.default_rng(ss.spawn(1)[0])
+    ns = np.exp(rng.uniform(np.log(rows_lo), np.log(rows_hi), count)).astype(np.int64)
+    row_start = np.zeros(count + 1, np.int64)
+    row_start[1:] = np.cumsum(ns)
+    R = int(row_start[-1])
+    Nrow = np.repeat(ns, ns)
+    lens = np.minimum(np.maximum(rng.poisson(lam, R), 1), np.minimum(kmax, Nrow))
+    row_ptr = np.zeros(R + 1, np.int64)
+    row_ptr[1:] = np.cumsum(lens)
+    nnz = int(row_ptr[-1])
+    col = np.empty(nnz, np.int32)
+    val = np.empty(nnz, dtype)
+    chunks = [(a, min(R, a + chunk_rows)) for a in range(0, R, chunk_rows)]
+    seeds = ss.spawn(len(chunks) + 1)[1:]
+
+    def work(args):
+        (a, b), sd = args
+        r = np.random.default_rng(sd)
+        lo, hi = row_ptr[a], row_ptr[b]
+        col[lo:hi] = _sorted_distinct_cols(lens[a:b], Nrow[a:b], r)
+        val[lo:hi] = r.standard_normal(hi - lo).astype(dtype)
+
+    with concurrent.futures.ThreadPoolExecutor(_workers()) as ex:
+        list(ex.map(work, zip(chunks, seeds)))
+    return ns, row_start, row_ptr, col, val
+
+
+def concat_matrix(k, ns, row_start, row_ptr, col, val):
+    """Matrix k of a batch_csr_concat result as (n, row_ptr, col, val)."""
+    a, b = row_start[k], row_start[k + 1]
+    rp = (row_ptr[a:b + 1] - row_ptr[a]).astype(np.int32)
+    return int(ns[k]), rp, col[row_ptr[a]:row_ptr[b]], val[row_ptr[a]:row_ptr[b]]

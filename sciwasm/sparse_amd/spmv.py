@@ -4,6 +4,7 @@
 for every format; ``spmv(A, x, y, beta=0)`` is the plain product.
 """
 import ctypes
+import sys
 
 from . import _lib
 from ._lib import call, c_vp, c_i64, c_i32
@@ -84,7 +85,7 @@ def _check_vec(A, x, y):
 def spmv(A, x_view, y_view, inner_max=1, beta=1.0, use_plan=True):
     """y = A.x + beta*y, inner_max times (beta=1: the reference's y += A.x)."""
     if A is None:
-        print("spmv input : undefined")
+        print("spmv input : undefined", file=sys.stderr)
         return
     _check_vec(A, x_view, y_view)
     ctx = x_view.vec.ctx.handle

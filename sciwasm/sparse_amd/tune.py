@@ -1,0 +1,107 @@
+"""Autotuner for the SpMV schedule, keyed on row-length statistics.
+
+For a CSR matrix it measures a small set of candidate schedules on the GPU
+(HIP-event timing on the context stream) and caches the winner under a key
+built from the matrix's row-length statistics (log2 buckets of N, of the
+mean and of the maximum row length, and a coefficient-of-variation bucket),
+so later matrices of the same class reuse the choice without measuring:
+
+* storage format: CSR, or ELL when the padded width is within the
+  reference's 5x guard and cheaper;
+* rows per wavefront: the block weight (nnz + row_weight * rows per
+  256-thread workgroup) and row weight, which set how many rows share a
+  wavefront in the reduction (a block of R rows gets 256/R lanes per row);
+* bin thresholds: the LDS capacity of a block (twice the block weight)
+  decides which rows take the whole-workgroup long-row path;
+* column panels for matrices whose x does not fit one XCD's L2.
+"""
+import json
+import math
+
+import numpy as np
+
+from .core import DeviceArray, sparse_vec_t, create_vec
+from .spmv import CsrPlan, CsrPanels
+from . import convert
+
+
+def row_stats_key(row_ptr):
+    lens = np.diff(np.asarray(row_ptr, np.int64))
+    N = len(lens)
+    mean = float(lens.mean()) if N else 0.0
+    mx = int(lens.max()) if N else 0
+    cv = float(lens.std() / mean) if mean > 0 else 0.0
+    b = lambda v: int(math.floor(math.log2(max(v, 1))))
+    return "N%d_mean%d_max%d_cv%d" % (b(N), b(mean), b(mx), min(int(cv * 2), 8))
+
+
+class Autotuner:
+    BLOCK_WEIGHTS = (256, 512, 1024, 2048)
+    ROW_WEIGHTS = (1, 2, 4)
+
+    def __init__(self, ctx, reps=20):
+        self.ctx = ctx
+        self.reps = reps
+        self.cache = {}
+
+    def _time(self, run):
+        run()
+        self.ctx.sync()
+        e0, e1 = self.ctx.event(), self.ctx.event()
+        self.ctx.record(e0)
+        for _ in range(self.reps):
+            run()
+        self.ctx.record(e1)
+        self.ctx.sync()
+        return self.ctx.elapsed_ms(e0, e1) / self.reps
+
+    def candidates(self, A, row_ptr):
+        N = A.N
+        out = []
+        for bw in self.BLOCK_WEIGHTS:
+            for rw in self.ROW_WEIGHTS:
+                out.append({"format": "csr", "block_weight": bw, "row_weight": rw})
+        if N * A.dtype.itemsize > (2 << 20):
+            out.append({"format": "panels", "block_weight": 1024, "row_weight": 2})
+        lens = np.diff(np.asarray(row_ptr, np.int64))
+        if len(lens) and lens.max() * N <= 5 * max(A.nnz, 1):
+            out.append({"format": "ell"})
+        return out
+
+    def tune(self, A, row_ptr=None):
+        """Best schedule for CSR matrix A (host row_ptr optional)."""
+        if row_ptr is None:
+            row_ptr = A.row.numpy()
+        key = row_stats_key(row_ptr) + "_" + A.dtype.name
+        if key in self.cache:
+            return key, self.cache[key]
+        x = DeviceArray(A.N, A.dtype, self.ctx)
+        x.fill(1.0)
+        y = DeviceArray(A.N, A.dtype, self.ctx)
+        best, best_ms, table = None, float("inf"), []
+        for cand in self.candidates(A, row_ptr):
+            if cand["format"] == "csr":
+                plan = CsrPlan(A, {"block_weight": cand["block_weight"], "row_weight": cand["row_weight"]})
+                ms = self._time(lambda: plan.run(x, y, beta=0.0))
+            elif cand["format"] == "panels":
+                pan = CsrPanels(A, 0, {"block_weight": cand["block_weight"], "row_weight": cand["row_weight"]})
+                ms = self._time(lambda: pan.run(x, y, beta=0.0))
+            else:
+                E = convert.csr_ell(A, guard=False)
+                from .spmv import spmv
+                xv, yv = sparse_vec_t(x, A.N), sparse_vec_t(y, A.N)
+                ms = self._time(lambda: spmv(E, xv, yv, beta=0.0))
+            table.append(dict(cand, ms=ms))
+            if ms < best_ms:
+                best, best_ms = cand, ms
+        result = dict(best, ms=best_ms, table=table)
+        self.cache[key] = result
+        return key, result
+
+    def save(self, path):
+        with open(path, "w") as f:
+            json.dump(self.cache, f, indent=1)
+
+    def load(self, path):
+        with open(path) as f:
+            self.cache.update(json.load(f))

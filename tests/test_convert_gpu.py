@@ -96,7 +96,7 @@ def test_csr_ell_guard_refuses_like_reference(ctx, capsys):
     rp, c, v = gen.zipf_csr(1 << 14, 1.5, 4096, seed=2)
     A = sp.csr_from_host(len(rp) - 1, rp, c, v)
     assert sp.csr_ell(A) is None
-    assert "not efficient to store this matrix in ELL" in capsys.readouterr().out
+    assert "not efficient to store this matrix in ELL" in capsys.readouterr().err
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
