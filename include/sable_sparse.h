@@ -75,6 +75,10 @@ SABLE_API int sable_memcpy_d2d(sable_ctx_t* ctx, void* dst, const void* src, siz
 SABLE_API int sable_fill_bytes(sable_ctx_t* ctx, void* dst, int byte, size_t bytes);
 /* Replaces clear_vec (seq/src/sparse32.js:1319-1322) and create_vec's fill. */
 SABLE_API int sable_fill_value(sable_ctx_t* ctx, sable_dtype_t dtype, void* dst, int64_t n, double value);
+/* y[i] = y[i] + w[i] for i in [0, n): the reduction of per-worker partial
+ * vectors after a parallel COO SpMV.  Replaces sum
+ * (par/src/sparse_opt_32.wat:1420-1481, driven from par/src/sparse32.js:2675-2691). */
+SABLE_API int sable_vec_add(sable_ctx_t* ctx, sable_dtype_t dtype, void* y, const void* w, int64_t n);
 
 /* Timing on the context stream (HIP events). */
 SABLE_API int sable_event_create(void** ev);

@@ -52,5 +52,31 @@ def build(verbose=False):
     return OUT
 
 
+NODE_DIR = os.path.join(HERE, "node")
+ADDON = os.path.join(NODE_DIR, "sable_addon.node")
+NODE_INCLUDE = "/usr/include/node"
+
+
+def build_addon(verbose=False):
+    """The Node-API addon (g++; host code only) linked against the HIP
+    library with an $ORIGIN rpath.  Skipped, with a message, when the
+    Node-API headers are absent."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        print("node_api.h not found: Node-API addon not built", file=sys.stderr)
+        return None
+    tmp = ADDON + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-DNAPI_VERSION=8",
+           "-I" + NODE_INCLUDE, os.path.join(NODE_DIR, "sable_addon.cc"), "-o", tmp,
+           "-L" + HERE, "-l:libsablesparse_hip.so", "-Wl,-rpath,$ORIGIN/..", "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("addon build failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-8000:]))
+    os.replace(tmp, ADDON)
+    if verbose:
+        print("built", ADDON, file=sys.stderr)
+    return ADDON
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_addon(verbose=True)

@@ -71,6 +71,7 @@ SIGNATURES = {
     "sable_memcpy_d2d": [c_vp, c_vp, c_vp, c_sz],
     "sable_fill_bytes": [c_vp, c_vp, c_int, c_sz],
     "sable_fill_value": [c_vp, c_int, c_vp, c_i64, c_dbl],
+    "sable_vec_add": [c_vp, c_int, c_vp, c_vp, c_i64],
     "sable_event_create": [c_vpp],
     "sable_event_destroy": [c_vp],
     "sable_event_record": [c_vp, c_vp],

@@ -72,6 +72,13 @@ __global__ void k_fill_value(T* __restrict__ p, T v, int64_t n) {
     for (; i < n; i += stride) p[i] = v;
 }
 
+template <typename T>
+__global__ void k_vec_add(T* __restrict__ y, const T* __restrict__ w, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) y[i] = add_rn(y[i], w[i]);
+}
+
 }  // namespace sable
 
 using namespace sable;
@@ -218,6 +225,22 @@ SABLE_API int sable_fill_value(sable_ctx_t* ctx, sable_dtype_t dtype, void* dst,
         hipLaunchKernelGGL(k_fill_value<float>, dim3(g), dim3(256), 0, ctx->stream, (float*)dst, (float)value, n);
     else if (dtype == SABLE_F64)
         hipLaunchKernelGGL(k_fill_value<double>, dim3(g), dim3(256), 0, ctx->stream, (double*)dst, value, n);
+    else
+        return fail(SABLE_EINVAL, "bad dtype %d", (int)dtype);
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+SABLE_API int sable_vec_add(sable_ctx_t* ctx, sable_dtype_t dtype, void* y, const void* w, int64_t n) {
+    SABLE_CHECK(ctx, SABLE_EINVAL, "ctx is NULL");
+    SABLE_CHECK(n >= 0, SABLE_EINVAL, "negative length");
+    if (!n) return SABLE_OK;
+    unsigned g = grid_for(n, 256);
+    if (g > 8192) g = 8192;
+    if (dtype == SABLE_F32)
+        hipLaunchKernelGGL(k_vec_add<float>, dim3(g), dim3(256), 0, ctx->stream, (float*)y, (const float*)w, n);
+    else if (dtype == SABLE_F64)
+        hipLaunchKernelGGL(k_vec_add<double>, dim3(g), dim3(256), 0, ctx->stream, (double*)y, (const double*)w, n);
     else
         return fail(SABLE_EINVAL, "bad dtype %d", (int)dtype);
     SABLE_LAUNCH_CHECK();

@@ -1,0 +1,904 @@
+// sable_addon.cc -- Node-API addon that stands in for the reference's
+// WebAssembly modules over libsablesparse_hip.
+//
+// The reference's JS layer (seq/src/sparse32.js, sparse64.js) talks to
+//   * a linear memory (Module['wasmMemory'].buffer) that it views through
+//     typed arrays at byte offsets,
+//   * malloc_instance.exports._malloc/_free returning such offsets, and
+//   * sparse_instance.exports.<kernel>(offsets..., sizes...).
+// This addon provides the same three things: `wasmMemory.buffer` is an
+// ArrayBuffer over host memory owned by the addon (pinned when possible),
+// `malloc_exports` allocates inside it, and `sparse_opt_32` / `sparse_opt_64`
+// carry every hot-path export with the reference's names and argument
+// orders.  Each kernel call mirrors the arrays it reads to HBM, runs the HIP
+// kernel and copies the outputs back into the linear memory.  Matrix arrays
+// are cached on the device keyed by (offset, length) and revalidated with a
+// 64-bit hash of their bytes, so repeated SpMV calls on an unchanged matrix
+// move only x and y.
+#include <node_api.h>
+
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../../include/sable_sparse.h"
+
+namespace {
+
+// ------------------------------------------------------------ linear memory
+struct Linear {
+    uint8_t* base = nullptr;
+    size_t size = 0;
+    bool pinned = false;
+    std::map<size_t, size_t> used;  // offset -> bytes
+    std::map<size_t, size_t> freeb;  // offset -> bytes
+};
+Linear g_mem;
+sable_ctx_t* g_ctx = nullptr;
+
+bool init_linear(size_t bytes) {
+    void* p = nullptr;
+    if (sable_host_alloc(bytes, &p) == SABLE_OK) {
+        g_mem.pinned = true;
+    } else {
+        p = aligned_alloc(4096, (bytes + 4095) & ~(size_t)4095);
+        g_mem.pinned = false;
+    }
+    if (!p) return false;
+    memset(p, 0, bytes);
+    g_mem.base = (uint8_t*)p;
+    g_mem.size = bytes;
+    // offset 0 stays reserved so that 0 can mean "no allocation"
+    g_mem.freeb[64] = bytes - 64;
+    return true;
+}
+
+size_t lm_malloc(size_t bytes) {
+    bytes = (bytes + 63) & ~(size_t)63;
+    if (!bytes) bytes = 64;
+    for (auto it = g_mem.freeb.begin(); it != g_mem.freeb.end(); ++it) {
+        if (it->second >= bytes) {
+            size_t off = it->first, rest = it->second - bytes;
+            g_mem.freeb.erase(it);
+            if (rest) g_mem.freeb[off + bytes] = rest;
+            g_mem.used[off] = bytes;
+            return off;
+        }
+    }
+    return 0;
+}
+
+void lm_free(size_t off) {
+    auto it = g_mem.used.find(off);
+    if (it == g_mem.used.end()) return;
+    size_t bytes = it->second;
+    g_mem.used.erase(it);
+    auto nx = g_mem.freeb.lower_bound(off);
+    if (nx != g_mem.freeb.end() && off + bytes == nx->first) {
+        bytes += nx->second;
+        g_mem.freeb.erase(nx);
+    }
+    auto pv = g_mem.freeb.lower_bound(off);
+    if (pv != g_mem.freeb.begin()) {
+        --pv;
+        if (pv->first + pv->second == off) {
+            pv->second += bytes;
+            return;
+        }
+    }
+    g_mem.freeb[off] = bytes;
+}
+
+// ------------------------------------------------------------ device mirrors
+uint64_t hash_bytes(const uint8_t* p, size_t n) {
+    uint64_t h = 0x9E3779B97F4A7C15ull ^ n;
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        memcpy(&w, p + i, 8);
+        h = (h ^ w) * 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 29;
+    }
+    for (; i < n; i++) h = (h ^ p[i]) * 0x100000001B3ull;
+    return h;
+}
+
+struct Mirror {
+    void* d = nullptr;
+    size_t bytes = 0;
+    uint64_t hash = 0;
+};
+std::map<std::pair<size_t, size_t>, Mirror> g_cache;  // read-only inputs
+std::vector<std::pair<void*, size_t>> g_scratch;      // per-call vectors
+
+struct Err {
+    std::string msg;
+};
+
+void check(int rc, const char* what) {
+    if (rc != SABLE_OK) throw Err{std::string(what) + ": " + sable_last_error()};
+}
+
+void ensure_ctx() {
+    if (!g_ctx) check(sable_ctx_create(0, nullptr, &g_ctx), "sable_ctx_create");
+}
+
+void check_range(size_t off, size_t bytes) {
+    if (off > g_mem.size || bytes > g_mem.size - off) throw Err{"offset outside the linear memory"};
+}
+
+// Device copy of a read-only region, reused while its bytes are unchanged.
+void* mirror(size_t off, size_t bytes) {
+    check_range(off, bytes);
+    const uint64_t h = hash_bytes(g_mem.base + off, bytes);
+    Mirror& m = g_cache[{off, bytes}];
+    if (m.d && m.hash == h) return m.d;
+    if (!m.d) check(sable_device_alloc(g_ctx, bytes ? bytes : 64, &m.d), "device alloc");
+    check(sable_memcpy_h2d(g_ctx, m.d, g_mem.base + off, bytes), "upload");
+    m.bytes = bytes;
+    m.hash = h;
+    return m.d;
+}
+
+// Fresh device copy of a region for this call (inputs that change per call,
+// or outputs); `upload` copies the current host bytes in.
+void* scratch(size_t off, size_t bytes, bool upload) {
+    check_range(off, bytes);
+    void* d = nullptr;
+    check(sable_device_alloc(g_ctx, bytes ? bytes : 64, &d), "device alloc");
+    g_scratch.push_back({d, bytes});
+    if (upload) check(sable_memcpy_h2d(g_ctx, d, g_mem.base + off, bytes), "upload");
+    return d;
+}
+
+void download(size_t off, const void* d, size_t bytes) {
+    check_range(off, bytes);
+    check(sable_memcpy_d2h(g_ctx, g_mem.base + off, d, bytes), "download");
+}
+
+void release_scratch() {
+    for (auto& s : g_scratch) sable_device_free(g_ctx, s.first);
+    g_scratch.clear();
+}
+
+const int32_t* host_i32(size_t off, size_t count) {
+    check_range(off, count * 4);
+    return (const int32_t*)(g_mem.base + off);
+}
+
+int32_t max_of(size_t off, size_t count) {
+    const int32_t* p = host_i32(off, count);
+    int32_t m = -1;
+    for (size_t i = 0; i < count; i++) m = p[i] > m ? p[i] : m;
+    return m;
+}
+
+void drop_mirrors_in(size_t off, size_t bytes) {
+    // outputs written into the linear memory invalidate overlapping mirrors
+    for (auto it = g_cache.begin(); it != g_cache.end();) {
+        const size_t a = it->first.first, b = a + it->first.second;
+        if (a < off + bytes && off < b) {
+            sable_device_free(g_ctx, it->second.d);
+            it = g_cache.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+// ------------------------------------------------------------ N-API helpers
+// Arguments of one call; `base` skips the parallel tree's leading worker id
+// so that one implementation serves both signatures.
+struct Args {
+    napi_env env;
+    std::vector<napi_value> v;
+    size_t base;
+    int64_t i(size_t k) const {
+        int64_t r = 0;
+        k += base;
+        if (k >= v.size() || napi_get_value_int64(env, v[k], &r) != napi_ok) throw Err{"expected a number argument"};
+        return r;
+    }
+    int64_t id() const { return base ? Args{env, v, 0}.i(0) : 0; }
+};
+
+// Module flavour, carried as the callback data of every export.
+struct Prec {
+    sable_dtype_t dt;
+    size_t vs;
+    bool stride_n_minus_min;  // seq/src/sparse64.js:1066-1092 num_diags
+    size_t lead;              // 1 for the parallel tree (leading worker id)
+};
+
+Prec* prec_of(napi_env env, napi_callback_info info) {
+    void* data = nullptr;
+    napi_get_cb_info(env, info, nullptr, nullptr, nullptr, &data);
+    return (Prec*)data;
+}
+
+Args get_args(napi_env env, napi_callback_info info, size_t n) {
+    Prec* P = prec_of(env, info);
+    const size_t lead = P ? P->lead : 0;
+    Args a{env, std::vector<napi_value>(n + lead), lead};
+    size_t argc = n + lead;
+    napi_get_cb_info(env, info, &argc, a.v.data(), nullptr, nullptr);
+    if (argc < n + lead) throw Err{"too few arguments"};
+    a.v.resize(argc);
+    return a;
+}
+
+napi_value undef(napi_env env) {
+    napi_value u;
+    napi_get_undefined(env, &u);
+    return u;
+}
+
+napi_value num(napi_env env, double d) {
+    napi_value r;
+    napi_create_double(env, d, &r);
+    return r;
+}
+
+// Every export runs under one process-wide lock: the parallel tree loads
+// the addon in several worker threads, which share the linear memory, the
+// device mirrors and the HIP context.
+std::mutex g_lock;
+
+template <typename F>
+napi_value guarded(napi_env env, F&& f) {
+    std::lock_guard<std::mutex> hold(g_lock);
+    try {
+        return f();
+    } catch (const Err& e) {
+        release_scratch();
+        napi_throw_error(env, nullptr, e.msg.c_str());
+        return nullptr;
+    }
+}
+
+// ------------------------------------------------------------ kernel exports
+Prec g_p32{SABLE_F32, 4, false, 0};
+Prec g_p64{SABLE_F64, 8, true, 0};
+Prec g_par32{SABLE_F32, 4, false, 1};
+Prec g_par64{SABLE_F64, 8, false, 1};
+
+// spmv_coo(row, col, val, x, y, len[, inner_max])
+napi_value spmv_coo_impl(napi_env env, napi_callback_info info, bool wrapper) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, wrapper ? 7 : 6);
+        ensure_ctx();
+        const size_t row = a.i(0), col = a.i(1), val = a.i(2), x = a.i(3), y = a.i(4);
+        const int64_t nnz = a.i(5);
+        const int im = wrapper ? (int)a.i(6) : 1;
+        if (nnz <= 0 || im <= 0) return undef(env);
+        const int32_t N = max_of(row, nnz) + 1, ncol = max_of(col, nnz) + 1;
+        void* dr = mirror(row, 4 * nnz);
+        void* dc = mirror(col, 4 * nnz);
+        void* dv = mirror(val, P->vs * nnz);
+        void* dx = scratch(x, P->vs * ncol, true);
+        void* dy = scratch(y, P->vs * N, true);
+        check(sable_spmv_coo(g_ctx, P->dt, N, nnz, (const int32_t*)dr, (const int32_t*)dc, dv, dx, dy, 1.0, im),
+              "spmv_coo");
+        download(y, dy, P->vs * N);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// spmv_csr(rowptr, col, val, x, y, N[, inner_max]); rowptr may be a slice
+napi_value spmv_csr_impl(napi_env env, napi_callback_info info, bool wrapper) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, wrapper ? 7 : 6);
+        ensure_ctx();
+        const size_t rp = a.i(0), col = a.i(1), val = a.i(2), x = a.i(3), y = a.i(4);
+        const int32_t N = (int32_t)a.i(5);
+        const int im = wrapper ? (int)a.i(6) : 1;
+        if (N <= 0 || im <= 0) return undef(env);
+        const int32_t* hrp = host_i32(rp, N + 1);
+        const int64_t j0 = hrp[0], j1 = hrp[N];
+        if (j1 < j0) throw Err{"row_ptr is not monotone"};
+        const int32_t ncol = j1 > j0 ? max_of(col + 4 * j0, j1 - j0) + 1 : 1;
+        void* drp = mirror(rp, 4 * (size_t)(N + 1));
+        // col/val mirrored from the first referenced entry; the base pointer is
+        // shifted back so that the absolute row_ptr values index it
+        char* dc = (char*)mirror(col + 4 * j0, 4 * (j1 - j0)) - 4 * j0;
+        char* dv = (char*)mirror(val + P->vs * j0, P->vs * (j1 - j0)) - P->vs * j0;
+        void* dx = scratch(x, P->vs * ncol, true);
+        void* dy = scratch(y, P->vs * N, true);
+        check(sable_spmv_csr(g_ctx, P->dt, N, (const int32_t*)drp, (const int32_t*)dc, dv, dx, dy, 1.0, im),
+              "spmv_csr");
+        download(y, dy, P->vs * N);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// spmv_dia(offset, data, N, nd, stride, x, y[, inner_max])
+napi_value spmv_dia_impl(napi_env env, napi_callback_info info, bool wrapper) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, wrapper ? 8 : 7);
+        ensure_ctx();
+        const size_t off = a.i(0), data = a.i(1), x = a.i(5), y = a.i(6);
+        const int32_t N = (int32_t)a.i(2), nd = (int32_t)a.i(3), stride = (int32_t)a.i(4);
+        const int im = wrapper ? (int)a.i(7) : 1;
+        if (nd <= 0 || N <= 0 || im <= 0) return undef(env);
+        void* doff = mirror(off, 4 * (size_t)nd);
+        void* dd = mirror(data, P->vs * (size_t)nd * stride);
+        void* dx = scratch(x, P->vs * N, true);
+        void* dy = scratch(y, P->vs * N, true);
+        check(sable_spmv_dia(g_ctx, P->dt, N, nd, stride, (const int32_t*)doff, dd, dx, dy, 1.0, im), "spmv_dia");
+        download(y, dy, P->vs * N);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// spmv_ell(indices, data, N, nc, x, y[, inner_max])
+napi_value spmv_ell_impl(napi_env env, napi_callback_info info, bool wrapper) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, wrapper ? 7 : 6);
+        ensure_ctx();
+        const size_t idx = a.i(0), data = a.i(1), x = a.i(4), y = a.i(5);
+        const int32_t N = (int32_t)a.i(2), nc = (int32_t)a.i(3);
+        const int im = wrapper ? (int)a.i(6) : 1;
+        if (nc <= 0 || N <= 0 || im <= 0) return undef(env);
+        const size_t slots = (size_t)nc * N;
+        const int32_t ncol = max_of(idx, slots) + 1;
+        void* di = mirror(idx, 4 * slots);
+        void* dd = mirror(data, P->vs * slots);
+        void* dx = scratch(x, P->vs * (ncol > N ? ncol : N), true);
+        void* dy = scratch(y, P->vs * N, true);
+        check(sable_spmv_ell(g_ctx, P->dt, N, nc, (const int32_t*)di, dd, dx, dy, 1.0, im), "spmv_ell");
+        download(y, dy, P->vs * N);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+#define WRAP2(NAME, IMPL)                                                                         \
+    napi_value NAME(napi_env env, napi_callback_info info) { return IMPL(env, info, false); }    \
+    napi_value NAME##_wrapper(napi_env env, napi_callback_info info) { return IMPL(env, info, true); }
+WRAP2(spmv_coo, spmv_coo_impl)
+WRAP2(spmv_csr, spmv_csr_impl)
+WRAP2(spmv_dia, spmv_dia_impl)
+WRAP2(spmv_ell, spmv_ell_impl)
+
+// transpose_csr(in_row_ptr, in_col, in_val, out_row_ptr, out_col, out_val, N, nnz)
+napi_value transpose_csr(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 8);
+        ensure_ctx();
+        const int32_t N = (int32_t)a.i(6);
+        const int64_t nnz = a.i(7);
+        if (N <= 0) return undef(env);
+        void* drp = mirror(a.i(0), 4 * (size_t)(N + 1));
+        void* dc = mirror(a.i(1), 4 * nnz);
+        void* dv = mirror(a.i(2), P->vs * nnz);
+        void* orp = scratch(a.i(3), 4 * (size_t)(N + 1), false);
+        void* oc = scratch(a.i(4), 4 * nnz, false);
+        void* ov = scratch(a.i(5), P->vs * nnz, false);
+        check(sable_transpose_csr(g_ctx, P->dt, N, (const int32_t*)drp, (const int32_t*)dc, dv, (int32_t*)orp,
+                                  (int32_t*)oc, ov),
+              "transpose_csr");
+        drop_mirrors_in(a.i(3), 4 * (size_t)(N + 1));
+        drop_mirrors_in(a.i(4), 4 * nnz);
+        drop_mirrors_in(a.i(5), P->vs * nnz);
+        download(a.i(3), orp, 4 * (size_t)(N + 1));
+        download(a.i(4), oc, 4 * nnz);
+        download(a.i(5), ov, P->vs * nnz);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// transpose_dia(in_offset, in_data, out_offset, out_data, ndiags, stride, N)
+napi_value transpose_dia(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 7);
+        ensure_ctx();
+        const int32_t nd = (int32_t)a.i(4), stride = (int32_t)a.i(5), N = (int32_t)a.i(6);
+        if (nd <= 0 || N <= 0) return undef(env);
+        const size_t db = P->vs * (size_t)nd * stride;
+        void* doff = mirror(a.i(0), 4 * (size_t)nd);
+        void* dd = mirror(a.i(1), db);
+        void* oo = scratch(a.i(2), 4 * (size_t)nd, false);
+        void* od = scratch(a.i(3), db, false);
+        check(sable_transpose_dia(g_ctx, P->dt, N, nd, stride, (const int32_t*)doff, dd, (int32_t*)oo, od),
+              "transpose_dia");
+        drop_mirrors_in(a.i(2), 4 * (size_t)nd);
+        drop_mirrors_in(a.i(3), db);
+        download(a.i(2), oo, 4 * (size_t)nd);
+        download(a.i(3), od, db);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// count_transpose_cols_ell(data, indices, nnz_row, ncols, N) -> i32
+napi_value count_transpose_cols_ell(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 5);
+        ensure_ctx();
+        const int32_t nc = (int32_t)a.i(3), N = (int32_t)a.i(4);
+        if (!(nc > 0 && N > 0)) return num(env, 0);
+        const size_t slots = (size_t)nc * N;
+        void* dd = mirror(a.i(0), P->vs * slots);
+        void* di = mirror(a.i(1), 4 * slots);
+        int32_t nco = 0;
+        check(sable_count_transpose_cols_ell(g_ctx, P->dt, N, nc, dd, (const int32_t*)di, &nco),
+              "count_transpose_cols_ell");
+        return num(env, nco);
+    });
+}
+
+// transpose_ell(in_data, in_indices, out_data, out_indices, nnz_row, ncols, N);
+// the output width is the count of the preceding count_transpose_cols_ell call
+napi_value transpose_ell(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 7);
+        ensure_ctx();
+        const int32_t nc = (int32_t)a.i(5), N = (int32_t)a.i(6);
+        if (!(nc > 0 && N > 0)) return undef(env);
+        const size_t slots = (size_t)nc * N;
+        void* dd = mirror(a.i(0), P->vs * slots);
+        void* di = mirror(a.i(1), 4 * slots);
+        int32_t nco = 0;
+        check(sable_count_transpose_cols_ell(g_ctx, P->dt, N, nc, dd, (const int32_t*)di, &nco), "count");
+        const size_t oslots = (size_t)nco * N;
+        void* od = scratch(a.i(2), P->vs * oslots, false);
+        void* oi = scratch(a.i(3), 4 * oslots, false);
+        check(sable_transpose_ell(g_ctx, P->dt, N, nc, nco, dd, (const int32_t*)di, od, (int32_t*)oi),
+              "transpose_ell");
+        drop_mirrors_in(a.i(2), P->vs * oslots);
+        drop_mirrors_in(a.i(3), 4 * oslots);
+        download(a.i(2), od, P->vs * oslots);
+        download(a.i(3), oi, 4 * oslots);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// memcpy(src, dst, bytes) inside the linear memory
+napi_value lm_memcpy(napi_env env, napi_callback_info info) {
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 3);
+        const size_t src = a.i(0), dst = a.i(1), n = a.i(2);
+        check_range(src, n);
+        check_range(dst, n);
+        memmove(g_mem.base + dst, g_mem.base + src, n);
+        return undef(env);
+    });
+}
+
+// ---- the three conversions the reference does in JS (now exports)
+// coo_csr(row, col, val, nnz, N, out_row_ptr, out_col, out_val)
+napi_value coo_csr(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 8);
+        ensure_ctx();
+        const int64_t nnz = a.i(3);
+        const int32_t N = (int32_t)a.i(4);
+        void* dr = mirror(a.i(0), 4 * nnz);
+        void* dc = mirror(a.i(1), 4 * nnz);
+        void* dv = mirror(a.i(2), P->vs * nnz);
+        void* orp = scratch(a.i(5), 4 * (size_t)(N + 1), false);
+        void* oc = scratch(a.i(6), 4 * nnz, false);
+        void* ov = scratch(a.i(7), P->vs * nnz, false);
+        check(sable_coo_csr(g_ctx, P->dt, N, nnz, (const int32_t*)dr, (const int32_t*)dc, dv, (int32_t*)orp,
+                            (int32_t*)oc, ov),
+              "coo_csr");
+        drop_mirrors_in(a.i(5), 4 * (size_t)(N + 1));
+        drop_mirrors_in(a.i(6), 4 * nnz);
+        drop_mirrors_in(a.i(7), P->vs * nnz);
+        download(a.i(5), orp, 4 * (size_t)(N + 1));
+        download(a.i(6), oc, 4 * nnz);
+        download(a.i(7), ov, P->vs * nnz);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// num_cols(row_ptr, N) -> i32
+napi_value num_cols(napi_env env, napi_callback_info info) {
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 2);
+        ensure_ctx();
+        const int32_t N = (int32_t)a.i(1);
+        void* drp = mirror(a.i(0), 4 * (size_t)(N + 1));
+        int32_t nc = 0;
+        check(sable_csr_num_cols(g_ctx, N, (const int32_t*)drp, &nc), "num_cols");
+        return num(env, nc);
+    });
+}
+
+// csr_ell(row_ptr, col, val, N, nc, out_indices, out_data)
+napi_value csr_ell(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 7);
+        ensure_ctx();
+        const int32_t N = (int32_t)a.i(3), nc = (int32_t)a.i(4);
+        const int32_t* hrp = host_i32(a.i(0), N + 1);
+        const int64_t nnz = hrp[N];
+        void* drp = mirror(a.i(0), 4 * (size_t)(N + 1));
+        void* dc = mirror(a.i(1), 4 * nnz);
+        void* dv = mirror(a.i(2), P->vs * nnz);
+        const size_t slots = (size_t)nc * N;
+        void* oi = scratch(a.i(5), 4 * slots, false);
+        void* od = scratch(a.i(6), P->vs * slots, false);
+        check(sable_csr_ell(g_ctx, P->dt, N, nc, (const int32_t*)drp, (const int32_t*)dc, dv, (int32_t*)oi, od),
+              "csr_ell");
+        drop_mirrors_in(a.i(5), 4 * slots);
+        drop_mirrors_in(a.i(6), P->vs * slots);
+        download(a.i(5), oi, 4 * slots);
+        download(a.i(6), od, P->vs * slots);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// num_diags(row_ptr, col, N) -> [nd, stride] (stride N, or N - min|offset|
+// for the FP64 sequential module as seq/src/sparse64.js:1066-1092)
+napi_value num_diags(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 3);
+        ensure_ctx();
+        const int32_t N = (int32_t)a.i(2);
+        const int32_t* hrp = host_i32(a.i(0), N + 1);
+        const int64_t nnz = hrp[N];
+        void* drp = mirror(a.i(0), 4 * (size_t)(N + 1));
+        void* dc = mirror(a.i(1), 4 * nnz);
+        int32_t nd = 0, mn = 0;
+        check(sable_csr_num_diags(g_ctx, N, (const int32_t*)drp, (const int32_t*)dc, &nd, &mn), "num_diags");
+        napi_value arr;
+        napi_create_array_with_length(env, 2, &arr);
+        napi_set_element(env, arr, 0, num(env, nd));
+        napi_set_element(env, arr, 1, num(env, P->stride_n_minus_min ? N - mn : N));
+        return arr;
+    });
+}
+
+// csr_dia(row_ptr, col, val, N, nd, stride, out_offset, out_data)
+napi_value csr_dia(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 8);
+        ensure_ctx();
+        const int32_t N = (int32_t)a.i(3), nd = (int32_t)a.i(4), stride = (int32_t)a.i(5);
+        const int32_t* hrp = host_i32(a.i(0), N + 1);
+        const int64_t nnz = hrp[N];
+        void* drp = mirror(a.i(0), 4 * (size_t)(N + 1));
+        void* dc = mirror(a.i(1), 4 * nnz);
+        void* dv = mirror(a.i(2), P->vs * nnz);
+        const size_t db = P->vs * (size_t)nd * stride;
+        void* oo = scratch(a.i(6), 4 * (size_t)(nd > 0 ? nd : 1), false);
+        void* od = scratch(a.i(7), db, false);
+        check(sable_csr_dia(g_ctx, P->dt, N, nd, stride, (const int32_t*)drp, (const int32_t*)dc, dv, (int32_t*)oo, od),
+              "csr_dia");
+        drop_mirrors_in(a.i(6), 4 * (size_t)nd);
+        drop_mirrors_in(a.i(7), db);
+        download(a.i(6), oo, 4 * (size_t)nd);
+        download(a.i(7), od, db);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// ---- triangular solves (parallel tree; the whole solve runs on the GPU in
+// the call of worker 0, the other workers return at once)
+
+// Level-set solve on the level-ordered matrix: b and y in level order.
+napi_value level_solve(napi_env env, Prec* P, const Args& a, size_t lp_at, size_t row_at, size_t b_at, size_t y_at,
+                       int32_t nl, int32_t N, int im) {
+    if (a.id() != 0 || nl <= 0 || im <= 0 || N <= 0) return undef(env);
+    ensure_ctx();
+    const int32_t* lp = host_i32(a.i(lp_at), nl + 1);
+    const int32_t* hrp = host_i32(a.i(row_at), N + 1);
+    const int64_t nnz = hrp[N];
+    void* drp = mirror(a.i(row_at), 4 * (size_t)(N + 1));
+    void* dc = mirror(a.i(row_at + 1), 4 * nnz);
+    void* dv = mirror(a.i(row_at + 2), P->vs * nnz);
+    void* db = scratch(a.i(b_at), P->vs * N, true);
+    void* dy = scratch(a.i(y_at), P->vs * N, false);
+    for (int it = 0; it < im; it++)
+        check(sable_spts_level_csr(g_ctx, P->dt, N, nl, lp, (const int32_t*)drp, (const int32_t*)dc, dv, db, dy, 1),
+              "spts_level_csr");
+    download(a.i(y_at), dy, P->vs * N);
+    release_scratch();
+    return undef(env);
+}
+
+// spts_level_csr(id, level_ptr, rowptr, col, val, b, y, perm, nlevels,
+//                barrier, flag, nthreads, N, inner_max)
+napi_value spts_level_csr(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 13);
+        return level_solve(env, P, a, 0, 1, 4, 5, (int32_t)a.i(7), (int32_t)a.i(11), (int)a.i(12));
+    });
+}
+
+// spts_csr_level_sync_free_wrapper(id, level_ptr, rowptr, col, val, b, y,
+//     perm, nlevels, barrier, flag, array_flag, nthreads, N, inner_max):
+// the same level-ordered arithmetic as spts_level_csr
+napi_value spts_csr_level_sync_free_wrapper(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 14);
+        return level_solve(env, P, a, 0, 1, 4, 5, (int32_t)a.i(7), (int32_t)a.i(12), (int)a.i(13));
+    });
+}
+
+// spts_csr_opt_level_sync_free_wrapper(id, level_ptr, rowptr, col, val, b,
+//     y, perm, nlevels, barrier, flag, global_level, global_rows,
+//     array_level, row_level, row_worker, worker_level, nthreads, N,
+//     inner_max): combined level + sync-free solve, level-ordered arithmetic
+napi_value spts_csr_opt_level_sync_free_wrapper(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 19);
+        return level_solve(env, P, a, 0, 1, 4, 5, (int32_t)a.i(7), (int32_t)a.i(17), (int)a.i(18));
+    });
+}
+
+// metadata(id, level_ptr, nlevels, nthreads, N, row_level, row_worker): the
+// per-worker ownership tables of the CPU combined solve; the GPU solve needs
+// none, so the call does nothing
+napi_value metadata(napi_env env, napi_callback_info info) {
+    return guarded(env, [&] {
+        (void)get_args(env, info, 6);
+        return undef(env);
+    });
+}
+
+// spts_csr_sync_free_wrapper(id, rowptr, col, val, b, y, N, barrier,
+//                            global_flag, flag, nthreads, inner_max)
+napi_value spts_csr_sync_free_wrapper(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 11);
+        if (a.id() != 0) return undef(env);
+        ensure_ctx();
+        const int32_t N = (int32_t)a.i(5);
+        const int im = (int)a.i(10);
+        if (im <= 0 || N <= 0) return undef(env);
+        const int32_t* hrp = host_i32(a.i(0), N + 1);
+        const int64_t nnz = hrp[N];
+        void* drp = mirror(a.i(0), 4 * (size_t)(N + 1));
+        void* dc = mirror(a.i(1), 4 * nnz);
+        void* dv = mirror(a.i(2), P->vs * nnz);
+        void* db = scratch(a.i(3), P->vs * N, true);
+        void* dy = scratch(a.i(4), P->vs * N, false);
+        for (int it = 0; it < im; it++)
+            check(sable_spts_syncfree_csr(g_ctx, P->dt, N, (const int32_t*)drp, (const int32_t*)dc, dv, db, dy, 1, 0),
+                  "spts_csr_sync_free");
+        download(a.i(4), dy, P->vs * N);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// ---- parallel-tree SpMV slices whose kernels take a row range
+
+// spmv_dia_col_wrapper(id, offset, data, start, end_incl, nd, N, stride, x,
+// y, inner_max): rows [start, end_incl] of y += A.x
+napi_value par_spmv_dia_col_wrapper(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 10);
+        ensure_ctx();
+        const int64_t r0 = a.i(2), r1 = a.i(3) + 1;
+        const int32_t nd = (int32_t)a.i(4), N = (int32_t)a.i(5), stride = (int32_t)a.i(6);
+        const size_t x = a.i(7), y = a.i(8);
+        const int im = (int)a.i(9);
+        if (nd <= 0 || N <= 0 || im <= 0 || r0 >= r1) return undef(env);
+        if (r0 < 0 || r1 > N) throw Err{"row range outside the matrix"};
+        void* doff = mirror(a.i(0), 4 * (size_t)nd);
+        void* dd = mirror(a.i(1), P->vs * (size_t)nd * stride);
+        void* dx = scratch(x, P->vs * N, true);
+        void* dy = scratch(y, P->vs * N, true);
+        // rows are independent: the whole product is formed and only this
+        // worker's rows are written back, each with the reference's order
+        check(sable_spmv_dia(g_ctx, P->dt, N, nd, stride, (const int32_t*)doff, dd, dx, dy, 1.0, im), "spmv_dia");
+        download(y + P->vs * r0, (char*)dy + P->vs * r0, P->vs * (r1 - r0));
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// spmv_ell_col_wrapper(id, indices, data, start, end, nc, N, x, y,
+// inner_max): rows [start, end) of y += A.x
+napi_value par_spmv_ell_col_wrapper(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 9);
+        ensure_ctx();
+        const int64_t r0 = a.i(2), r1 = a.i(3);
+        const int32_t nc = (int32_t)a.i(4), N = (int32_t)a.i(5);
+        const size_t x = a.i(6), y = a.i(7);
+        const int im = (int)a.i(8);
+        if (nc <= 0 || N <= 0 || im <= 0 || r0 >= r1) return undef(env);
+        if (r0 < 0 || r1 > N) throw Err{"row range outside the matrix"};
+        const size_t slots = (size_t)nc * N;
+        const int32_t ncol = max_of(a.i(0), slots) + 1;
+        void* di = mirror(a.i(0), 4 * slots);
+        void* dd = mirror(a.i(1), P->vs * slots);
+        void* dx = scratch(x, P->vs * (ncol > N ? ncol : N), true);
+        void* dy = scratch(y, P->vs * N, true);
+        check(sable_spmv_ell(g_ctx, P->dt, N, nc, (const int32_t*)di, dd, dx, dy, 1.0, im), "spmv_ell");
+        download(y + P->vs * r0, (char*)dy + P->vs * r0, P->vs * (r1 - r0));
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// sum(y, w, start, end[, N]): y[i] = y[i] + w[i] for i in [start, end)
+napi_value par_sum(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 4);
+        a.base = 0;  // sum has no worker id
+        ensure_ctx();
+        const int64_t r0 = a.i(2), r1 = a.i(3);
+        if (r0 >= r1) return undef(env);
+        const size_t y = a.i(0) + P->vs * r0, w = a.i(1) + P->vs * r0, n = r1 - r0;
+        void* dy = scratch(y, P->vs * n, true);
+        void* dw = scratch(w, P->vs * n, true);
+        check(sable_vec_add(g_ctx, P->dt, dy, dw, n), "sum");
+        download(y, dy, P->vs * n);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// ------------------------------------------------------------ module objects
+napi_value malloc_fn(napi_env env, napi_callback_info info) {
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 1);
+        return num(env, (double)lm_malloc((size_t)a.i(0)));
+    });
+}
+
+napi_value free_fn(napi_env env, napi_callback_info info) {
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 1);
+        const size_t off = a.i(0);
+        auto it = g_mem.used.find(off);
+        if (it != g_mem.used.end()) drop_mirrors_in(off, it->second);
+        lm_free(off);
+        return undef(env);
+    });
+}
+
+// One ArrayBuffer per JS environment (main thread or worker), all over the
+// same host memory, so that every `memory.buffer` sees the same bytes.
+struct EnvData {
+    napi_ref buffer = nullptr;
+};
+
+void env_cleanup(napi_env env, void* data, void*) {
+    EnvData* d = (EnvData*)data;
+    if (d->buffer) napi_delete_reference(env, d->buffer);
+    delete d;
+}
+
+napi_value buffer_getter(napi_env env, napi_callback_info) {
+    EnvData* d = nullptr;
+    napi_get_instance_data(env, (void**)&d);
+    napi_value ab;
+    if (d && d->buffer) {
+        napi_get_reference_value(env, d->buffer, &ab);
+        return ab;
+    }
+    napi_create_external_arraybuffer(env, g_mem.base, g_mem.size, nullptr, nullptr, &ab);
+    if (d) napi_create_reference(env, ab, 1, &d->buffer);
+    return ab;
+}
+
+napi_value stats_fn(napi_env env, napi_callback_info) {
+    std::lock_guard<std::mutex> hold(g_lock);
+    napi_value o;
+    napi_create_object(env, &o);
+    napi_set_named_property(env, o, "mirrors", num(env, (double)g_cache.size()));
+    napi_set_named_property(env, o, "linear_bytes", num(env, (double)g_mem.size));
+    napi_set_named_property(env, o, "pinned", num(env, g_mem.pinned ? 1 : 0));
+    napi_set_named_property(env, o, "allocations", num(env, (double)g_mem.used.size()));
+    return o;
+}
+
+void add_fn(napi_env env, napi_value obj, const char* name, napi_callback cb, void* data) {
+    napi_value f;
+    napi_create_function(env, name, NAPI_AUTO_LENGTH, cb, data, &f);
+    napi_set_named_property(env, obj, name, f);
+}
+
+struct Fn {
+    const char* name;
+    napi_callback cb;
+};
+
+// Serial tree: the exports of seq/src/sparse_opt_32.wat / sparse_opt_64.wat
+// on the hot path, plus the three conversions the reference does in JS.
+napi_value make_serial(napi_env env, Prec* P) {
+    napi_value m;
+    napi_create_object(env, &m);
+    const Fn fns[] = {
+        {"spmv_coo", spmv_coo}, {"spmv_coo_wrapper", spmv_coo_wrapper},
+        {"spmv_csr", spmv_csr}, {"spmv_csr_wrapper", spmv_csr_wrapper},
+        {"spmv_dia", spmv_dia}, {"spmv_dia_wrapper", spmv_dia_wrapper},
+        {"spmv_ell", spmv_ell}, {"spmv_ell_wrapper", spmv_ell_wrapper},
+        {"transpose_csr", transpose_csr}, {"transpose_dia", transpose_dia},
+        {"count_transpose_cols_ell", count_transpose_cols_ell}, {"transpose_ell", transpose_ell},
+        {"memcpy", lm_memcpy}, {"coo_csr", coo_csr}, {"num_cols", num_cols}, {"csr_ell", csr_ell},
+        {"num_diags", num_diags}, {"csr_dia", csr_dia},
+    };
+    for (auto& f : fns) add_fn(env, m, f.name, f.cb, P);
+    return m;
+}
+
+// Parallel tree: the exports that par/src/worker32.js calls on the hot path,
+// each with its leading worker id.
+napi_value make_parallel(napi_env env, Prec* P, Prec* serial) {
+    napi_value m;
+    napi_create_object(env, &m);
+    const Fn fns[] = {
+        {"spmv_coo_wrapper", spmv_coo_wrapper}, {"spmv_csr_wrapper", spmv_csr_wrapper},
+        {"spmv_dia_col_wrapper", par_spmv_dia_col_wrapper}, {"spmv_ell_col_wrapper", par_spmv_ell_col_wrapper},
+        {"memcpy", lm_memcpy}, {"spts_level_csr", spts_level_csr},
+        {"spts_csr_sync_free_wrapper", spts_csr_sync_free_wrapper},
+        {"spts_csr_level_sync_free_wrapper", spts_csr_level_sync_free_wrapper},
+        {"spts_csr_opt_level_sync_free_wrapper", spts_csr_opt_level_sync_free_wrapper},
+        {"metadata", metadata},
+    };
+    for (auto& f : fns) add_fn(env, m, f.name, f.cb, P);
+    add_fn(env, m, "sum", par_sum, serial);  // sum takes no worker id
+    return m;
+}
+
+napi_value Init(napi_env env, napi_value exports) {
+    {
+        std::lock_guard<std::mutex> hold(g_lock);
+        size_t mb = 1024;
+        if (const char* e = getenv("SABLE_LINEAR_MEMORY_MB")) mb = (size_t)atol(e);
+        if (!g_mem.base && !init_linear(mb << 20)) {
+            napi_throw_error(env, nullptr, "cannot allocate the linear memory");
+            return nullptr;
+        }
+    }
+    napi_set_instance_data(env, new EnvData(), env_cleanup, nullptr);
+    napi_value mem;
+    napi_create_object(env, &mem);
+    napi_property_descriptor d = {"buffer", nullptr, nullptr, buffer_getter, nullptr, nullptr, napi_default, nullptr};
+    napi_define_properties(env, mem, 1, &d);
+    napi_set_named_property(env, exports, "wasmMemory", mem);
+    napi_value mal;
+    napi_create_object(env, &mal);
+    add_fn(env, mal, "_malloc", malloc_fn, nullptr);
+    add_fn(env, mal, "_free", free_fn, nullptr);
+    napi_set_named_property(env, exports, "malloc_exports", mal);
+    napi_set_named_property(env, exports, "sparse_opt_32", make_serial(env, &g_p32));
+    napi_set_named_property(env, exports, "sparse_opt_64", make_serial(env, &g_p64));
+    napi_set_named_property(env, exports, "par_sparse_opt_32", make_parallel(env, &g_par32, &g_p32));
+    napi_set_named_property(env, exports, "par_sparse_opt_64", make_parallel(env, &g_par64, &g_p64));
+    add_fn(env, exports, "stats", stats_fn, nullptr);
+    return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)

@@ -1,0 +1,262 @@
+// flow.js -- drives the Node-API addon the way the reference's JS layer
+// drives its WebAssembly module: arrays live in the linear memory at offsets
+// returned by _malloc, typed-array views fill and read them, and kernels are
+// called with byte offsets and counts in the reference's argument orders.
+//
+// usage: node flow.js <dir> <32|64>
+// <dir>/inputs.json lists the input arrays (<name>.bin, raw little-endian)
+// and scalars; the script writes <dir>/out_<name>.bin for every result and
+// <dir>/outputs.json with the counts and checksums it computed.
+//
+// Flows (each after seq/tests/main32.js and par/tests/main32.js):
+//   coo_csr, num_cols + csr_ell, num_diags + csr_dia  (with the reference's
+//     storage guards, seq/src/sparse32.js:1449-1574)
+//   spmv_{coo,csr,dia,ell}_wrapper with inner_max, y cleared first
+//   transpose_csr / transpose_dia / count_transpose_cols_ell + transpose_ell
+//   parallel-tree slices: spmv_csr_wrapper / spmv_coo_wrapper + sum /
+//     spmv_dia_col_wrapper / spmv_ell_col_wrapper over W row ranges
+//   spts_csr_sync_free_wrapper, spts_level_csr,
+//     spts_csr_opt_level_sync_free_wrapper
+'use strict';
+
+const fs = require('fs');
+const path = require('path');
+
+const dir = process.argv[2];
+const bits = parseInt(process.argv[3] || '32', 10);
+const inst = require(path.join(__dirname, '..', '..', 'sciwasm', 'sparse_amd', 'node', 'sable_instance.js')).load(bits);
+const memory = inst.memory;
+const malloc_instance = inst.malloc_instance;
+const sparse_instance = inst.sparse_instance;
+const my_instance = inst.my_instance;
+const FArray = bits === 32 ? Float32Array : Float64Array;
+const VB = FArray.BYTES_PER_ELEMENT;
+const IB = Int32Array.BYTES_PER_ELEMENT;
+
+const spec = JSON.parse(fs.readFileSync(path.join(dir, 'inputs.json'), 'utf8'));
+const outputs = {};
+
+function step(name) {
+  process.stdout.write('step ' + name + '\n');
+}
+
+function load_array(name, Type) {
+  const buf = fs.readFileSync(path.join(dir, name + '.bin'));
+  const n = buf.length / Type.BYTES_PER_ELEMENT;
+  const off = malloc_instance.exports._malloc(Math.max(buf.length, 4));
+  new Uint8Array(memory.buffer, off, buf.length).set(buf);
+  return { off: off, n: n };
+}
+
+function alloc(n, Type) {
+  const off = malloc_instance.exports._malloc(Math.max(n * Type.BYTES_PER_ELEMENT, 4));
+  new Uint8Array(memory.buffer, off, n * Type.BYTES_PER_ELEMENT).fill(0);
+  return off;
+}
+
+function save(name, off, n, Type) {
+  const bytes = new Uint8Array(memory.buffer, off, n * Type.BYTES_PER_ELEMENT);
+  fs.writeFileSync(path.join(dir, 'out_' + name + '.bin'), Buffer.from(bytes));
+}
+
+function matlab_modulo(x, y) {
+  return x - Math.floor(x / y) * y;
+}
+
+// checksum of a result vector as the reference's tests print it
+function fletcher_sum_y(off, n) {
+  const y = new FArray(memory.buffer, off, n);
+  let s1 = 0, s2 = 0;
+  for (let i = 0; i < y.length; ++i) {
+    s1 = matlab_modulo(s1 + y[i], 255);
+    s2 = matlab_modulo(s2 + s1, 255);
+  }
+  return parseInt(s2 * 256 + s1);
+}
+
+const N = spec.N;
+const nnz = spec.nnz;
+const inner_max = spec.inner_max;
+const W = spec.workers;
+const E = sparse_instance.exports;
+const P = my_instance.exports;
+
+// ---------------------------------------------------------------- COO -> CSR
+step('coo_csr');
+const coo_row = load_array('coo_row', Int32Array);
+const coo_col = load_array('coo_col', Int32Array);
+const coo_val = load_array('coo_val', FArray);
+const x = load_array('x', FArray);
+const csr = { row: alloc(N + 1, Int32Array), col: alloc(nnz, Int32Array), val: alloc(nnz, FArray) };
+E.coo_csr(coo_row.off, coo_col.off, coo_val.off, nnz, N, csr.row, csr.col, csr.val);
+save('csr_row', csr.row, N + 1, Int32Array);
+save('csr_col', csr.col, nnz, Int32Array);
+save('csr_val', csr.val, nnz, FArray);
+
+// ---------------------------------------------------------------- CSR -> ELL
+step('csr_ell');
+const nc = E.num_cols(csr.row, N);
+outputs.ell_nc = nc;
+let ell = null;
+if (nc * N < Math.pow(2, 27) && (nc * N) / nnz <= 5) {
+  ell = { idx: alloc(nc * N, Int32Array), data: alloc(nc * N, FArray), nc: nc };
+  E.csr_ell(csr.row, csr.col, csr.val, N, nc, ell.idx, ell.data);
+  save('ell_idx', ell.idx, nc * N, Int32Array);
+  save('ell_data', ell.data, nc * N, FArray);
+} else {
+  console.log('not efficient to store this matrix in ELL');
+}
+
+// ---------------------------------------------------------------- CSR -> DIA
+step('csr_dia');
+const nds = E.num_diags(csr.row, csr.col, N);
+const nd = nds[0], stride = nds[1];
+outputs.dia_nd = nd;
+outputs.dia_stride = stride;
+let dia = null;
+if (nd * stride < Math.pow(2, 27) && (stride * nd) / nnz <= 5) {
+  dia = { off: alloc(nd, Int32Array), data: alloc(nd * stride, FArray), nd: nd, stride: stride };
+  E.csr_dia(csr.row, csr.col, csr.val, N, nd, stride, dia.off, dia.data);
+  save('dia_off', dia.off, nd, Int32Array);
+  save('dia_data', dia.data, nd * stride, FArray);
+} else {
+  console.log('not efficient to store this matrix in DIA');
+}
+
+// ---------------------------------------------------------------- SpMV
+function spmv_flow(name, call) {
+  step('spmv_' + name);
+  const y = alloc(N, FArray);
+  new FArray(memory.buffer, y, N).fill(0);
+  call(y);
+  save('y_' + name, y, N, FArray);
+  outputs['fletcher_' + name] = fletcher_sum_y(y, N);
+  return y;
+}
+
+spmv_flow('coo', (y) => E.spmv_coo_wrapper(coo_row.off, coo_col.off, coo_val.off, x.off, y, nnz, inner_max));
+spmv_flow('csr', (y) => E.spmv_csr_wrapper(csr.row, csr.col, csr.val, x.off, y, N, inner_max));
+spmv_flow('csr1', (y) => E.spmv_csr(csr.row, csr.col, csr.val, x.off, y, N));
+if (ell) spmv_flow('ell', (y) => E.spmv_ell_wrapper(ell.idx, ell.data, N, ell.nc, x.off, y, inner_max));
+if (dia) spmv_flow('dia', (y) => E.spmv_dia_wrapper(dia.off, dia.data, N, dia.nd, dia.stride, x.off, y, inner_max));
+// a second call on the unchanged matrix reuses its device copy
+spmv_flow('csr_again', (y) => E.spmv_csr_wrapper(csr.row, csr.col, csr.val, x.off, y, N, inner_max));
+outputs.mirrors_after_spmv = inst.stats().mirrors;
+
+// ---------------------------------------------------------------- transposes
+step('transpose_csr');
+const tcsr = { row: alloc(N + 1, Int32Array), col: alloc(nnz, Int32Array), val: alloc(nnz, FArray) };
+E.transpose_csr(csr.row, csr.col, csr.val, tcsr.row, tcsr.col, tcsr.val, N, nnz);
+save('tcsr_row', tcsr.row, N + 1, Int32Array);
+save('tcsr_col', tcsr.col, nnz, Int32Array);
+save('tcsr_val', tcsr.val, nnz, FArray);
+if (dia) {
+  step('transpose_dia');
+  const toff = alloc(dia.nd, Int32Array), tdata = alloc(dia.nd * dia.stride, FArray);
+  E.transpose_dia(dia.off, dia.data, toff, tdata, dia.nd, dia.stride, N);
+  save('tdia_off', toff, dia.nd, Int32Array);
+  save('tdia_data', tdata, dia.nd * dia.stride, FArray);
+}
+if (ell) {
+  step('transpose_ell');
+  const nnz_row = alloc(N, Int32Array);
+  const tnc = E.count_transpose_cols_ell(ell.data, ell.idx, nnz_row, ell.nc, N);
+  outputs.tell_nc = tnc;
+  const tdata = alloc(tnc * N, FArray), tidx = alloc(tnc * N, Int32Array);
+  E.transpose_ell(ell.data, ell.idx, tdata, tidx, nnz_row, ell.nc, N);
+  save('tell_idx', tidx, tnc * N, Int32Array);
+  save('tell_data', tdata, tnc * N, FArray);
+}
+
+// ---------------------------------------------------------------- parallel tree
+// Row ranges of W workers, the first N % W taking one extra row.
+function ranges(n, w) {
+  const out = [];
+  const len = Math.floor(n / w), rem = n % w;
+  let s = 0;
+  for (let i = 0; i < w; i++) {
+    const e = s + len + (i < rem ? 1 : 0);
+    out.push([s, e]);
+    s = e;
+  }
+  return out;
+}
+const R = ranges(N, W);
+const rowp = new Int32Array(memory.buffer, csr.row, N + 1);
+
+spmv_flow('par_csr', (y) => {
+  for (let id = 0; id < W; id++) {
+    const s = R[id][0], e = R[id][1];
+    P.spmv_csr_wrapper(id, csr.row + IB * s, csr.col, csr.val, x.off, y + VB * s, e - s, inner_max);
+  }
+});
+spmv_flow('par_coo', (y) => {
+  // nnz split across workers into private partial vectors, then summed
+  const per = Math.floor(nnz / W);
+  const w = [];
+  for (let id = 0; id < W; id++) {
+    const s = id * per, e = id === W - 1 ? nnz : (id + 1) * per;
+    const wy = alloc(N, FArray);
+    P.spmv_coo_wrapper(id, coo_row.off + IB * s, coo_col.off + IB * s, coo_val.off + VB * s, x.off, wy, e - s, 1);
+    w.push(wy);
+  }
+  for (let k = 0; k < W; k++)
+    for (let id = 0; id < W; id++) P.sum(y, w[k], R[id][0], R[id][1], N);
+});
+if (dia) {
+  spmv_flow('par_dia', (y) => {
+    for (let id = 0; id < W; id++)
+      P.spmv_dia_col_wrapper(id, dia.off, dia.data, R[id][0], R[id][1] - 1, dia.nd, N, dia.stride, x.off, y, inner_max);
+  });
+}
+if (ell) {
+  spmv_flow('par_ell', (y) => {
+    for (let id = 0; id < W; id++)
+      P.spmv_ell_col_wrapper(id, ell.idx, ell.data, R[id][0], R[id][1], ell.nc, N, x.off, y, inner_max);
+  });
+}
+outputs.row_ptr_last = rowp[N];
+
+// ---------------------------------------------------------------- SpTS
+if (spec.spts) {
+  const S = spec.spts;
+  const L = { row: load_array('L_row', Int32Array), col: load_array('L_col', Int32Array),
+              val: load_array('L_val', FArray) };
+  const b = load_array('L_b', FArray);
+  const barrier = alloc(4, Int32Array), flag = alloc(4, Int32Array), aflag = alloc(S.N, Int32Array);
+  step('spts_sync_free');
+  const y1 = alloc(S.N, FArray);
+  for (let id = 0; id < W; id++)
+    P.spts_csr_sync_free_wrapper(id, L.row.off, L.col.off, L.val.off, b.off, y1, S.N, barrier, flag, aflag, W, 1);
+  save('spts_syncfree', y1, S.N, FArray);
+
+  const Lr = { lp: load_array('Lr_level_ptr', Int32Array), perm: load_array('Lr_perm', Int32Array),
+               row: load_array('Lr_row', Int32Array), col: load_array('Lr_col', Int32Array),
+               val: load_array('Lr_val', FArray) };
+  const br = load_array('Lr_b', FArray);
+  step('spts_level');
+  const y2 = alloc(S.N, FArray);
+  for (let id = 0; id < W; id++)
+    P.spts_level_csr(id, Lr.lp.off, Lr.row.off, Lr.col.off, Lr.val.off, br.off, y2, Lr.perm.off, S.nlevels,
+                     barrier, flag, W, S.N, 1);
+  save('spts_level', y2, S.N, FArray);
+
+  step('spts_combined');
+  const y3 = alloc(S.N, FArray);
+  const gl = alloc(1, Int32Array), gr = alloc(1, Int32Array), al = alloc(W, Int32Array);
+  const rl = alloc(S.N, Int32Array), rw = alloc(S.N, Int32Array), wl = alloc(W, Int32Array);
+  for (let id = 0; id < W; id++) P.metadata(id, Lr.lp.off, S.nlevels, W, S.N, rl, rw);
+  for (let id = 0; id < W; id++)
+    P.spts_csr_opt_level_sync_free_wrapper(id, Lr.lp.off, Lr.row.off, Lr.col.off, Lr.val.off, br.off, y3,
+                                           Lr.perm.off, S.nlevels, barrier, flag, gl, gr, al, rl, rw, wl, W, S.N, 1);
+  save('spts_combined', y3, S.N, FArray);
+}
+
+// ---------------------------------------------------------------- _free
+step('free');
+const before = inst.stats().allocations;
+malloc_instance.exports._free(coo_row.off);
+outputs.freed = before - inst.stats().allocations;
+
+fs.writeFileSync(path.join(dir, 'outputs.json'), JSON.stringify(outputs));
+step('done');

@@ -1,0 +1,206 @@
+"""The Node-API drop-in (sciwasm/sparse_amd/node/): the addon that stands in
+for the reference's WebAssembly instances.
+
+CPU: node loads the addon without a GPU; it exports the reference's names
+(seq/src/sparse_opt_32.wat exports used by seq/src/sparse32.js, and the
+parallel tree's exports called from par/src/worker32.js); _malloc/_free and
+memcpy work on the linear memory; a kernel call without a GPU fails loudly.
+
+GPU: tests/node/flow.js drives every export the way the reference's JS does
+(typed-array views over _malloc offsets); its outputs are compared with the
+oracle: bit for bit for the conversions, transposes, DIA/ELL SpMV (serial
+and parallel-tree row slices) and the three triangular solves; CSR/COO SpMV
+(serial, slices and partial sums) within the summation-order bound.
+"""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from sciwasm.sparse_amd import gen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "sciwasm", "sparse_amd", "node", "sable_addon.node")
+FLOW = os.path.join(ROOT, "tests", "node", "flow.js")
+NODE = shutil.which("node")
+
+SERIAL = ["spmv_coo", "spmv_coo_wrapper", "spmv_csr", "spmv_csr_wrapper", "spmv_dia", "spmv_dia_wrapper",
+          "spmv_ell", "spmv_ell_wrapper", "transpose_csr", "transpose_dia", "count_transpose_cols_ell",
+          "transpose_ell", "memcpy", "coo_csr", "num_cols", "csr_ell", "num_diags", "csr_dia"]
+PARALLEL = ["spmv_coo_wrapper", "spmv_csr_wrapper", "spmv_dia_col_wrapper", "spmv_ell_col_wrapper", "memcpy",
+            "sum", "spts_level_csr", "spts_csr_sync_free_wrapper", "spts_csr_level_sync_free_wrapper",
+            "spts_csr_opt_level_sync_free_wrapper", "metadata"]
+
+needs_node = pytest.mark.skipif(NODE is None, reason="node is not installed")
+
+
+@needs_node
+def test_addon_loads_and_exports_reference_names():
+    assert os.path.exists(ADDON), "addon not built (run __graft_entry__.build())"
+    js = """//
+const m = require(process.argv[1]).load(32);
+const m64 = require(process.argv[1]).load(64);
+const out = {
+  serial: Object.keys(m.sparse_instance.exports), serial64: Object.keys(m64.sparse_instance.exports),
+  parallel: Object.keys(m.my_instance.exports), malloc: Object.keys(m.malloc_instance.exports),
+  bytes: m.memory.buffer.byteLength, same: m.memory.buffer === m64.memory.buffer,
+};
+const M = m.malloc_instance.exports;
+const a = M._malloc(100), b = M._malloc(1000), c = M._malloc(8);
+out.offsets = [a, b, c];
+M._free(b);
+out.reuse = M._malloc(900);
+new Int32Array(m.memory.buffer, a, 4).set([7, 8, 9, 10]);
+m.sparse_instance.exports.memcpy(a, c, 8);
+out.copied = Array.from(new Int32Array(m.memory.buffer, c, 2));
+try { m.sparse_instance.exports.spmv_csr_wrapper(a, a, a, a, a, 0, 1); out.kernel = 'ran'; }
+catch (e) { out.kernel = e.message; }
+try { m.sparse_instance.exports.memcpy(0, m.memory.buffer.byteLength - 4, 64); out.oob = 'no error'; }
+catch (e) { out.oob = e.message; }
+console.log(JSON.stringify(out));
+"""
+    js = js.replace("process.argv[1]", json.dumps(os.path.join(ROOT, "sciwasm", "sparse_amd", "node",
+                                                                  "sable_instance.js")))
+    r = subprocess.run([NODE, "-e", js], capture_output=True, text=True, timeout=120, cwd=ROOT,
+                       env=dict(os.environ, SABLE_LINEAR_MEMORY_MB="64"))
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert sorted(out["serial"]) == sorted(SERIAL)
+    assert sorted(out["serial64"]) == sorted(SERIAL)
+    assert sorted(out["parallel"]) == sorted(PARALLEL)
+    assert sorted(out["malloc"]) == ["_free", "_malloc"]
+    assert out["bytes"] == 64 << 20 and out["same"]
+    a, b, c = out["offsets"]
+    assert 0 < a < b < c and a % 64 == 0 and b - a >= 100 and c - b >= 1000
+    assert out["reuse"] == b  # first fit reuses the freed block
+    assert out["copied"] == [7, 8]
+    assert out["oob"] == "offset outside the linear memory"
+    if out["kernel"] != "ran":  # no GPU here: the error names the HIP failure
+        assert "sable_ctx_create" in out["kernel"]
+
+
+# ------------------------------------------------------------------ GPU flow
+def write_inputs(d, dtype, seed=0, workers=3, inner_max=3):
+    rng = np.random.default_rng(seed)
+    rp, col, _ = gen.laplacian_2d(40)
+    N = len(rp) - 1
+    nnz = len(col)
+    row = gen.csr_to_coo_rows(rp)
+    perm = rng.permutation(nnz)
+    coo_row, coo_col = row[perm].astype(np.int32), col[perm].astype(np.int32)
+    coo_val = rng.uniform(-1, 1, nnz).astype(dtype)
+    x = rng.uniform(-1, 1, N).astype(dtype)
+    Lrp, Lc, Lv = gen.random_lower(3000, kmax=12, seed=seed + 1, dtype=dtype)
+    n = len(Lrp) - 1
+    b = rng.uniform(-1, 1, n).astype(dtype)
+    nl, lp, lperm, order, _ = oracle.level_sets(n, Lrp, Lc)
+    rrp, rc, rv = oracle.level_reorder(n, Lrp, Lc, Lv, lperm, order)
+    br = b[lperm]
+    arrays = dict(coo_row=coo_row, coo_col=coo_col, coo_val=coo_val, x=x, L_row=Lrp, L_col=Lc, L_val=Lv, L_b=b,
+                  Lr_level_ptr=lp, Lr_perm=lperm, Lr_row=rrp, Lr_col=rc, Lr_val=rv, Lr_b=br)
+    for k, v in arrays.items():
+        np.ascontiguousarray(v).tofile(os.path.join(d, k + ".bin"))
+    spec = dict(N=N, nnz=nnz, inner_max=inner_max, workers=workers, spts=dict(N=n, nlevels=int(nl)))
+    with open(os.path.join(d, "inputs.json"), "w") as f:
+        json.dump(spec, f)
+    return spec, arrays
+
+
+def row_bound(rp, col, val, x, y0, dtype, extra=0):
+    u = np.finfo(dtype).eps / 2
+    k = np.diff(rp).astype(np.float64) + 1 + extra
+    g = k * u / (1 - k * u)
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    ab = np.abs(val.astype(np.float64) * x.astype(np.float64)[col])
+    s = np.bincount(rows, weights=ab, minlength=len(rp) - 1) + np.abs(y0.astype(np.float64))
+    return 2 * g * s + np.finfo(dtype).tiny
+
+
+def read_out(d, name, dtype):
+    return np.fromfile(os.path.join(d, "out_" + name + ".bin"), dtype=dtype)
+
+
+@needs_node
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
+def test_node_flow_matches_oracle(tmp_path, dtype):
+    d = str(tmp_path)
+    spec, a = write_inputs(d, dtype)
+    bits = "32" if dtype == np.float32 else "64"
+    print("node flow", bits, flush=True)
+    r = subprocess.run([NODE, FLOW, d, bits], capture_output=True, text=True, timeout=300, cwd=ROOT,
+                       env=dict(os.environ, SABLE_LINEAR_MEMORY_MB="256"))
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    out = json.load(open(os.path.join(d, "outputs.json")))
+    N, nnz, im, W = spec["N"], spec["nnz"], spec["inner_max"], spec["workers"]
+    i32 = np.int32
+
+    # conversions
+    rp, col, val = oracle.coo_csr(N, a["coo_row"], a["coo_col"], a["coo_val"])
+    assert np.array_equal(read_out(d, "csr_row", i32), rp)
+    assert np.array_equal(read_out(d, "csr_col", i32), col)
+    assert np.array_equal(read_out(d, "csr_val", dtype).view(np.uint8), val.view(np.uint8))
+    nc, eidx, edata = oracle.csr_ell(N, rp, col, val)
+    assert out["ell_nc"] == nc == 5
+    assert np.array_equal(read_out(d, "ell_idx", i32), eidx)
+    assert np.array_equal(read_out(d, "ell_data", dtype), edata)
+    nd, mn = oracle.num_diags(N, rp, col)
+    stride = N if dtype == np.float32 else N - mn
+    assert out["dia_nd"] == nd == 5 and out["dia_stride"] == stride
+    _, _, doff, ddata = oracle.csr_dia(N, rp, col, val, stride=stride)
+    assert np.array_equal(read_out(d, "dia_off", i32), doff)
+    assert np.array_equal(read_out(d, "dia_data", dtype), ddata)
+
+    # SpMV, y cleared then y += A.x inner_max times.  DIA and ELL keep the
+    # reference's per-row order: bit-exact, checksums equal.  CSR and COO sum
+    # in another order: within 2 gamma_{k+1} (|y0| + sum |a x|) per product.
+    z = np.zeros(N, dtype)
+    x = a["x"]
+    exact = {
+        "ell": oracle.spmv_ell(N, nc, eidx, edata, x, z, im),
+        "dia": oracle.spmv_dia(N, stride, doff, ddata, x, z, im),
+    }
+    exact["par_ell"], exact["par_dia"] = exact["ell"], exact["dia"]
+    for name, yref in exact.items():
+        y = read_out(d, "y_" + name, dtype)
+        assert np.array_equal(y.view(np.uint8), yref.view(np.uint8)), name
+        assert out["fletcher_" + name] == oracle.fletcher_sum(yref), name
+    y1 = oracle.spmv_csr(N, rp, col, val, x, z, 1)
+    yim = oracle.spmv_csr(N, rp, col, val, x, z, im)
+    b1 = row_bound(rp, col, val, x, z, dtype)
+    bim = b1 * im + row_bound(rp, col, val, x, yim, dtype) * (im - 1)
+    close = {"csr": (yim, bim), "csr_again": (yim, bim), "par_csr": (yim, bim), "csr1": (y1, b1),
+             "coo": (oracle.spmv_coo(a["coo_row"], a["coo_col"], a["coo_val"], x, z, im), bim),
+             # W partial products, each summed into y: W more roundings per row
+             "par_coo": (y1, row_bound(rp, col, val, x, z, dtype, extra=W))}
+    for name, (yref, bnd) in close.items():
+        y = read_out(d, "y_" + name, dtype)
+        err = np.abs(y.astype(np.float64) - yref.astype(np.float64))
+        assert np.all(err <= bnd), (name, float(err.max()))
+    assert out["mirrors_after_spmv"] > 0
+
+    # transposes
+    trp, tc, tv = oracle.transpose_csr(N, rp, col, val)
+    assert np.array_equal(read_out(d, "tcsr_row", i32), trp)
+    assert np.array_equal(read_out(d, "tcsr_col", i32), tc)
+    assert np.array_equal(read_out(d, "tcsr_val", dtype), tv)
+    too, tod = oracle.transpose_dia(N, stride, doff, ddata)
+    assert np.array_equal(read_out(d, "tdia_off", i32), too)
+    assert np.array_equal(read_out(d, "tdia_data", dtype), tod)
+    tnc, toi, tod2 = oracle.transpose_ell(N, nc, eidx, edata)
+    assert out["tell_nc"] == tnc
+    assert np.array_equal(read_out(d, "tell_idx", i32), toi)
+    assert np.array_equal(read_out(d, "tell_data", dtype), tod2)
+
+    # triangular solves in the reference's per-row order: bit-exact
+    n = spec["spts"]["N"]
+    ys = oracle.spts_syncfree(n, a["L_row"], a["L_col"], a["L_val"], a["L_b"])
+    assert np.array_equal(read_out(d, "spts_syncfree", dtype), ys)
+    yl = oracle.spts_level(n, a["Lr_row"], a["Lr_col"], a["Lr_val"], a["Lr_b"])
+    assert np.array_equal(read_out(d, "spts_level", dtype), yl)
+    assert np.array_equal(read_out(d, "spts_combined", dtype), yl)
+    assert out["freed"] == 1
