@@ -133,10 +133,22 @@ def headline(args, sp, ctx, T):
         nnz_s = int(rp[R])
         t, ycpu = oracle.time_spmv_csr_f64(R, rp, col, val, x, warm=10, reps=30)
         bytes_s = 4 * (R + 1) + 12 * nnz_s + 8 * N + 8 * R
-        cpu = {"value": round(bytes_s / t / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-               "sample": "serial C restatement of seq/src/sparse_opt_64.wat spmv_csr on rows [0, N/4) "
-                         "(%d rows, %d nnz) of the same matrix, 10 warm-up + 30 timed calls, %.4f s/call"
-                         % (R, nnz_s, t)}
+        serial = {"value": round(bytes_s / t / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+                  "sample": "serial C restatement of seq/src/sparse_opt_64.wat spmv_csr on rows [0, N/4) "
+                            "(%d rows, %d nnz) of the same matrix, 10 warm-up + 30 timed calls, %.4f s/call"
+                            % (R, nnz_s, t)}
+        try:
+            W = len(os.sched_getaffinity(0))
+        except AttributeError:
+            W = os.cpu_count() or 1
+        W = max(1, min(W, 64))
+        tw, yw = oracle.time_spmv_csr_f64_mt(N, rp, col, val, x, W, warm=3, reps=10)
+        cpu = {"value": round(bytes_step / tw / 1e9, 3), "unit": "GB/s", "cores": W, "kind": "port",
+               "sample": "the same C restatement on %d persistent threads over static_nnz row ranges "
+                         "(par/src/sparse32.js:2769-2821), whole matrix (%d rows, %d nnz), 3 warm-up + 10 "
+                         "timed calls, %.4f s/call; results identical to the serial restatement: %s"
+                         % (W, N, nnz, tw, bool(np.array_equal(yw[:R], ycpu))),
+               "serial": serial}
         lens = np.diff(rp[:R + 1]).astype(np.float64) + 1
         ab = np.add.reduceat(np.abs(val[:nnz_s] * x[col[:nnz_s]]), rp[:R])
         ab[np.diff(rp[:R + 1]) == 0] = 0
@@ -428,7 +440,8 @@ def main():
                     help="first run one 1 GiB device-to-device copy (a plain copy kernel of known size, used to "
                          "scale hardware byte counters)")
     ap.add_argument("--csr-kernel", type=str, default="0",
-                    help="comma list of CSR block kernels to time (0 basic, 1 staged, 2 persistent)")
+                    help="comma list of CSR block kernels to time (0 basic, 1 staged, 2 persistent, 3 basic with "
+                         "non-temporal matrix loads)")
     ap.add_argument("--block-weight", type=str, default="0",
                     help="comma list of row-block weights (nnz + 2*rows) to time with each panel width (0 = 1024)")
     ap.add_argument("--sections", type=str, default="c1,band,convert,spts,batch,serve",

@@ -112,7 +112,8 @@ SABLE_API int sable_spmv_csr_plan(sable_ctx_t* ctx, const sable_csr_plan_t* plan
 /* Selects the CSR block kernel for all later launches (process-wide): 0 =
  * basic block kernel, one workgroup per row block (default); 1 = the kernel
  * with staged row pointers and early y loads, one workgroup per row block;
- * 2 = that kernel on a persistent grid of 8 workgroups per CU. */
+ * 2 = that kernel on a persistent grid of 8 workgroups per CU; 3 = the basic
+ * kernel with non-temporal (streaming) col/val loads. */
 SABLE_API int sable_csr_kernel_select(int kernel);
 SABLE_API int sable_spmv_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
                              const int32_t* col, const void* val, const void* x, void* y,

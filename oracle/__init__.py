@@ -68,6 +68,9 @@ def lib():
         L.oracle_rel_residual.restype = ctypes.c_double
         L.oracle_time_spmv_csr_f64.argtypes = [_I, _I32P, _I32P, _F64P, _F64P, _F64P, _I, _I]
         L.oracle_time_spmv_csr_f64.restype = ctypes.c_double
+        L.oracle_time_spmv_csr_f64_mt.argtypes = [_I, _I32P, _I32P, _F64P, _F64P, _F64P, _I, _I, _I]
+        L.oracle_time_spmv_csr_f64_mt.restype = ctypes.c_double
+        L.oracle_static_nnz.argtypes = [_I, _I32P, _I, _I32P, _I32P]
         _LIB = L
     return _LIB
 
@@ -272,4 +275,21 @@ def time_spmv_csr_f64(nrows, rowptr, col, val, x, warm=10, reps=30):
     y = np.zeros(nrows, np.float64)
     t = lib().oracle_time_spmv_csr_f64(nrows, _i32(rowptr), _i32(col), np.ascontiguousarray(val, np.float64),
                                        np.ascontiguousarray(x, np.float64), y, warm, reps)
+    return t, y
+
+
+def static_nnz(nrows, rowptr, workers):
+    """Contiguous row ranges of about equal nnz (par/src/sparse32.js:2769-2821)."""
+    rs = np.zeros(workers, np.int32)
+    re = np.zeros(workers, np.int32)
+    lib().oracle_static_nnz(nrows, _i32(rowptr), workers, rs, re)
+    return rs, re
+
+
+def time_spmv_csr_f64_mt(nrows, rowptr, col, val, x, workers, warm=10, reps=30):
+    """Mean seconds per CSR SpMV over rows [0, nrows) with `workers`
+    persistent threads on static_nnz row ranges (C timer)."""
+    y = np.zeros(nrows, np.float64)
+    t = lib().oracle_time_spmv_csr_f64_mt(nrows, _i32(rowptr), _i32(col), np.ascontiguousarray(val, np.float64),
+                                          np.ascontiguousarray(x, np.float64), y, workers, warm, reps)
     return t, y

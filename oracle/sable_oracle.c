@@ -527,3 +527,77 @@ EXPORT double oracle_time_spmv_csr_f64(int nrows, const int *rowptr, const int *
     }
     return reps > 0 ? t / reps : 0.0;
 }
+
+/* Threaded CPU baseline: rows split into W contiguous ranges of about equal
+ * nnz by the greedy rule of static_nnz (par/src/sparse32.js:2769-2821: a
+ * worker takes rows while its running sum is below remaining_nnz /
+ * remaining_workers), one persistent thread per range, a barrier before and
+ * after each timed product (the reference's workers are persistent too and
+ * report back once per product). */
+#include <pthread.h>
+typedef struct {
+    int r0, r1, nrows;
+    const int *rowptr, *col;
+    const double *val, *x;
+    double *y;
+    int runs;
+    pthread_barrier_t *bar;
+} mt_job;
+
+static void *mt_worker(void *arg)
+{
+    mt_job *j = (mt_job *)arg;
+    for (int r = 0; r < j->runs; r++) {
+        pthread_barrier_wait(j->bar);
+        for (int i = j->r0; i < j->r1; i++) j->y[i] = 0.0;
+        if (j->r1 > j->r0)
+            oracle_spmv_csr_f64(j->r1 - j->r0, j->rowptr + j->r0, j->col, j->val, j->x, j->y + j->r0, 1);
+        pthread_barrier_wait(j->bar);
+    }
+    return NULL;
+}
+
+EXPORT void oracle_static_nnz(int nrows, const int *rowptr, int workers, int *row_start, int *row_end)
+{
+    long long rem = (long long)rowptr[nrows] - rowptr[0];
+    int rem_w = workers, next = 0;
+    for (int w = 0; w < workers; w++) {
+        if (next == nrows) { row_start[w] = row_end[w] = nrows; continue; }
+        long long ideal = rem / rem_w, got = 0;
+        row_start[w] = next;
+        while (next < nrows && got < ideal) { got += rowptr[next + 1] - rowptr[next]; next++; }
+        row_end[w] = next;
+        rem -= got;
+        rem_w--;
+    }
+    row_end[workers - 1] = nrows;
+}
+
+EXPORT double oracle_time_spmv_csr_f64_mt(int nrows, const int *rowptr, const int *col,
+                                          const double *val, const double *x, double *y,
+                                          int workers, int warm, int reps)
+{
+    if (workers < 1) workers = 1;
+    int *rs = (int *)malloc(sizeof(int) * workers), *re = (int *)malloc(sizeof(int) * workers);
+    mt_job *jobs = (mt_job *)malloc(sizeof(mt_job) * workers);
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * workers);
+    pthread_barrier_t bar;
+    pthread_barrier_init(&bar, NULL, (unsigned)workers + 1);
+    oracle_static_nnz(nrows, rowptr, workers, rs, re);
+    for (int w = 0; w < workers; w++) {
+        mt_job j = {rs[w], re[w], nrows, rowptr, col, val, x, y, warm + reps, &bar};
+        jobs[w] = j;
+        pthread_create(&th[w], NULL, mt_worker, &jobs[w]);
+    }
+    double t = 0;
+    for (int r = 0; r < warm + reps; r++) {
+        double t0 = now_s();
+        pthread_barrier_wait(&bar);
+        pthread_barrier_wait(&bar);
+        if (r >= warm) t += now_s() - t0;
+    }
+    for (int w = 0; w < workers; w++) pthread_join(th[w], NULL);
+    pthread_barrier_destroy(&bar);
+    free(rs); free(re); free(jobs); free(th);
+    return reps > 0 ? t / reps : 0.0;
+}

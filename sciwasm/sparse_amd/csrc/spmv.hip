@@ -92,7 +92,18 @@ __device__ __forceinline__ void write_y(T* y, int64_t r, T s, T beta, bool nonem
     }
 }
 
-template <typename T, int U>
+// Streaming loads of the matrix arrays: with NT the col/val loads carry the
+// non-temporal hint, so the L2 keeps x (the only reused data) instead of
+// matrix lines that are read exactly once.
+template <bool NT, typename V>
+__device__ __forceinline__ V ld_stream(const V* p) {
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+template <typename T, int U, bool NT>
 __global__ __launch_bounds__(BS) void k_csr_block(const int32_t* __restrict__ blk, int64_t nblocks,
                                                   const int32_t* __restrict__ rp,
                                                   const int32_t* __restrict__ col,
@@ -120,8 +131,8 @@ __global__ __launch_bounds__(BS) void k_csr_block(const int32_t* __restrict__ bl
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int kk = k + u * BS;
-                c[u] = kk < nnzb ? cb[kk] : 0;
-                v[u] = kk < nnzb ? vb[kk] : T(0);
+                c[u] = kk < nnzb ? ld_stream<NT>(cb + kk) : 0;
+                v[u] = kk < nnzb ? ld_stream<NT>(vb + kk) : T(0);
             }
             T xv[U];
 #pragma unroll
@@ -155,11 +166,14 @@ __global__ __launch_bounds__(BS) void k_csr_block(const int32_t* __restrict__ bl
                 int c[U];
                 T v[U];
 #pragma unroll
-                for (int u = 0; u < U; u++) { c[u] = col[k + u * BS]; v[u] = val[k + u * BS]; }
+                for (int u = 0; u < U; u++) {
+                    c[u] = ld_stream<NT>(col + k + u * BS);
+                    v[u] = ld_stream<NT>(val + k + u * BS);
+                }
 #pragma unroll
                 for (int u = 0; u < U; u++) s += v[u] * x[c[u]];
             }
-            for (; k < e; k += BS) s += val[k] * x[col[k]];
+            for (; k < e; k += BS) s += ld_stream<NT>(val + k) * x[ld_stream<NT>(col + k)];
             s = block_sum(s, red);
             if (tid == 0) write_y(y, r, s, beta, e > a);
         }
@@ -446,22 +460,26 @@ int plan_fill_stream(hipStream_t st, int32_t N, const int32_t* rp, int T, int C,
 
 int g_num_cu = 256;
 // 0: k_csr_block, one workgroup per block; 1: k_csr_persist with one
-// workgroup per block; 2: k_csr_persist with ~8 workgroups per CU.
+// workgroup per block; 2: k_csr_persist with ~8 workgroups per CU;
+// 3: k_csr_block with non-temporal col/val loads.
 int g_csr_kernel = 0;
 
 template <typename T>
 static void csr_launch_t(hipStream_t st, const int32_t* blk, int64_t nb, const int32_t* rp, const int32_t* col,
                          const void* val, const void* x, void* y, double beta, int lds_cap) {
     const size_t shm = (size_t)lds_cap * sizeof(T);
-    if (g_csr_kernel >= 1) {
+    if (g_csr_kernel == 1 || g_csr_kernel == 2) {
         // at least one workgroup per XCD, so that every XCD's block range is covered
         int64_t grid = g_csr_kernel == 2 ? (int64_t)g_num_cu * 8 : nb;
         if (grid > nb) grid = nb;
         if (grid < NUM_XCD) grid = NUM_XCD;
         hipLaunchKernelGGL((k_csr_persist<T, 8>), dim3((unsigned)grid), dim3(BS), shm, st, blk, nb, rp, col,
                            (const T*)val, (const T*)x, (T*)y, (T)beta, lds_cap);
+    } else if (g_csr_kernel == 3) {
+        hipLaunchKernelGGL((k_csr_block<T, 8, true>), dim3((unsigned)nb), dim3(BS), shm, st, blk, nb, rp, col,
+                           (const T*)val, (const T*)x, (T*)y, (T)beta, lds_cap);
     } else {
-        hipLaunchKernelGGL((k_csr_block<T, 8>), dim3((unsigned)nb), dim3(BS), shm, st, blk, nb, rp, col,
+        hipLaunchKernelGGL((k_csr_block<T, 8, false>), dim3((unsigned)nb), dim3(BS), shm, st, blk, nb, rp, col,
                            (const T*)val, (const T*)x, (T*)y, (T)beta, lds_cap);
     }
 }
@@ -562,7 +580,7 @@ SABLE_API int sable_csr_plan_info(const sable_csr_plan_t* p, int64_t* nblocks, i
 }
 
 SABLE_API int sable_csr_kernel_select(int kernel) {
-    SABLE_CHECK(kernel >= 0 && kernel <= 2, SABLE_EINVAL, "kernel must be 0, 1 or 2");
+    SABLE_CHECK(kernel >= 0 && kernel <= 3, SABLE_EINVAL, "kernel must be 0, 1, 2 or 3");
     g_csr_kernel = kernel;
     return SABLE_OK;
 }

@@ -303,3 +303,24 @@ def test_fletcher_known_answers():
     y = np.array([300.0, -1.5], np.float64)
     # s1 = 45, s2 = 45; s1 = 43.5, s2 = 88.5 -> 88.5*256 + 43.5 = 22699.5 -> 22699
     assert oracle.fletcher_sum(y) == 22699.0
+
+
+def test_static_nnz_partition_known_answer():
+    # row lengths 5,1,1,1,1,1,5,1 over 3 workers: ideal 16//3 = 5 -> rows [0,1);
+    # then 11//2 = 5 -> rows [1,6); the last worker takes the rest
+    lens = np.array([5, 1, 1, 1, 1, 1, 5, 1])
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    rs, re = oracle.static_nnz(8, rp, 3)
+    assert list(rs) == [0, 1, 6] and list(re) == [1, 6, 8]
+    # ideal 4//4 = 1 -> row 0; then ideal 1//3 = 0 -> empty ranges; the last
+    # worker (ideal 1//1) takes row 1
+    rs, re = oracle.static_nnz(2, np.array([0, 3, 4], np.int32), 4)
+    assert list(rs) == [0, 1, 1, 1] and list(re) == [1, 1, 1, 2]
+
+
+def test_threaded_cpu_baseline_matches_serial():
+    rp, col, val = gen.random_csr(5000, 9, seed=3, empty_rows=0.1)
+    x = np.random.default_rng(4).uniform(-1, 1, 5000)
+    _, y1 = oracle.time_spmv_csr_f64(5000, rp, col, val, x, warm=0, reps=1)
+    _, y4 = oracle.time_spmv_csr_f64_mt(5000, rp, col, val, x, 4, warm=0, reps=2)
+    assert np.array_equal(y1, y4)

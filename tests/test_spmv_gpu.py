@@ -50,7 +50,7 @@ CASES = [
 ]
 
 
-@pytest.fixture(params=[0, 1, 2], ids=["basic", "staged", "persistent"])
+@pytest.fixture(params=[0, 1, 2, 3], ids=["basic", "staged", "persistent", "streaming"])
 def csr_kernel(request):
     from sciwasm.sparse_amd import _lib
     _lib.call("sable_csr_kernel_select", request.param)
