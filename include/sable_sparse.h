@@ -311,7 +311,8 @@ typedef struct {
 
 /* y_k = A_k.x_k + beta_k*y_k for every item, nrhs in {1, 2, 4, 8} right-hand
  * sides, over total_blocks = sum of the items' block counts; workgroups = 0
- * picks a persistent grid of 4 per CU. */
+ * picks a persistent grid of 4 per CU.  x of every item must be 16-byte
+ * aligned (device allocations are). */
 SABLE_API int sable_batch_spmv(sable_ctx_t* ctx, sable_dtype_t dtype, int nrhs, const sable_batch_spmv_item_t* d_items,
                                int nitems, int64_t total_blocks, int workgroups);
 
@@ -365,6 +366,9 @@ typedef struct {
     int64_t arena_bytes;  /* device vector arena per slot (0 = 256 MiB) */
     int refine_iters;     /* refinement steps in mode 2 */
     int spts_lanes;       /* lanes per row of the solves (0 = 4) */
+    int group_rhs;        /* modes 1 and 2: SpMV requests on one matrix in a batch share
+                             one multi-RHS product, up to this many (0 = 8, 1 = off) */
+    int pad;
 } sable_server_config_t;
 
 SABLE_API int sable_server_create(sable_ctx_t* ctx, const sable_server_config_t* cfg, sable_server_t** out);

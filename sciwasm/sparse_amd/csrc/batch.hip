@@ -37,6 +37,34 @@ struct Lds {
     static constexpr int ELEMS = (2048 * NR < 8192) ? 2048 * NR : 8192;
 };
 
+// The NR values of row c of a row-major [N][NR] x: 16-byte vector loads when
+// a row is a multiple of 16 bytes (rows are then 16-byte aligned, since x is),
+// else one 8- or 4-byte load.
+template <typename T, int NR>
+__device__ __forceinline__ void load_xrow(const T* __restrict__ x, int c, T (&out)[NR]) {
+    constexpr int BYTES = NR * (int)sizeof(T);
+    const char* p = (const char*)(x + (int64_t)c * NR);
+    if constexpr (BYTES % 16 == 0) {
+#pragma unroll
+        for (int q = 0; q < BYTES / 16; q++) {
+            const uint4 w = *(const uint4*)(p + 16 * q);
+            __builtin_memcpy((char*)out + 16 * q, &w, 16);
+        }
+    } else if constexpr (BYTES == 8) {
+        const uint2 w = *(const uint2*)p;
+        __builtin_memcpy(out, &w, 8);
+    } else {
+        out[0] = *(const T*)p;
+    }
+}
+
+// Loads in flight per thread in the staged path: fewer for wide rows of x,
+// whose values already fill the registers.
+template <typename T, int NR>
+struct Unroll {
+    static constexpr int U = NR * (int)sizeof(T) >= 32 ? 2 : (NR * (int)sizeof(T) >= 16 ? 4 : 8);
+};
+
 template <typename T, int NR>
 __device__ __forceinline__ void block_spmv(const sable_batch_spmv_item_t& it, int64_t b, T* prod, T* red) {
     constexpr int LDS_ELEMS = Lds<NR>::ELEMS;
@@ -53,11 +81,31 @@ __device__ __forceinline__ void block_spmv(const sable_batch_spmv_item_t& it, in
     const int nnzb = (int)((int64_t)rp[r1] - nz0);
     const int R = r1 - r0;
     if (nnzb * NR <= LDS_ELEMS) {
-        for (int k = tid; k < nnzb; k += BB) {
-            const int c = col[nz0 + k];
-            const T v = val[nz0 + k];
+        constexpr int U = Unroll<T, NR>::U;
+        // all loads of a step are issued before any product is formed, with
+        // out-of-range slots predicated off
+        for (int k = tid; k < nnzb; k += U * BB) {
+            int c[U];
+            T v[U];
 #pragma unroll
-            for (int r = 0; r < NR; r++) prod[k * NR + r] = v * x[(int64_t)c * NR + r];
+            for (int u = 0; u < U; u++) {
+                const int kk = k + u * BB;
+                c[u] = kk < nnzb ? col[nz0 + kk] : 0;
+                v[u] = kk < nnzb ? val[nz0 + kk] : T(0);
+            }
+            T xr[U][NR];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (k + u * BB < nnzb) load_xrow<T, NR>(x, c[u], xr[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int kk = k + u * BB;
+                if (kk < nnzb) {
+#pragma unroll
+                    for (int r = 0; r < NR; r++) prod[kk * NR + r] = v[u] * xr[u][r];
+                }
+            }
         }
         __syncthreads();
         int L = WAVE;
@@ -98,8 +146,10 @@ __device__ __forceinline__ void block_spmv(const sable_batch_spmv_item_t& it, in
             for (int64_t k = a + tid; k < e; k += BB) {
                 const int c = col[k];
                 const T v = val[k];
+                T xr[NR];
+                load_xrow<T, NR>(x, c, xr);
 #pragma unroll
-                for (int r = 0; r < NR; r++) s[r] += v * x[(int64_t)c * NR + r];
+                for (int r = 0; r < NR; r++) s[r] += v * xr[r];
             }
 #pragma unroll
             for (int r = 0; r < NR; r++) {
@@ -336,6 +386,71 @@ static unsigned gsz(int64_t n) {
     return g > 16384 ? 16384 : g;
 }
 
+// One level of a batched level-set solve: G lanes per row; t = sum of
+// val*x over the row's off-diagonal entries, x[i] = (b[i] - t) / diag.
+// With G = 1 the per-row order is the level-set solve's t = 0 + v0*x0 + ...
+template <typename T, int G>
+__global__ __launch_bounds__(BB) void k_batch_level(const sable_batch_spts_item_t* __restrict__ items,
+                                                    const LevelSeg* __restrict__ segs, int nseg, int64_t total) {
+    const int64_t gid = ((int64_t)blockIdx.x * BB + threadIdx.x) / G;
+    const int l = threadIdx.x & (G - 1);
+    const bool valid = gid < total;
+    T s = T(0);
+    int32_t i = 0, e = 0;
+    const sable_batch_spts_item_t* it = nullptr;
+    if (valid) {
+        int lo = 0, hi = nseg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (segs[mid].prefix <= gid) lo = mid;
+            else hi = mid - 1;
+        }
+        const LevelSeg sg = segs[lo];
+        it = items + sg.item;
+        i = sg.rows[gid - sg.prefix];
+        const int32_t* __restrict__ rp = it->row_ptr;
+        const int32_t* __restrict__ col = it->col;
+        const T* __restrict__ val = (const T*)it->val;
+        const T* __restrict__ x = (const T*)it->x;
+        e = rp[i + 1] - 1;
+        for (int32_t j = rp[i] + l; j < e; j += G) s = add_rn(s, mul_rn(val[j], x[col[j]]));
+    }
+    if (G > 1) s = group_sum(s, G);
+    if (valid && l == 0) {
+        const T d = ((const T*)it->val)[e];
+        ((T*)it->x)[i] = div_rn(sub_rn(((const T*)it->b)[i], s), d);
+    }
+}
+
+int batch_level_launch(hipStream_t st, sable_dtype_t dtype, int lanes, const sable_batch_spts_item_t* items,
+                       const LevelSeg* d_segs, const LevelPlan& plan) {
+    for (int lv = 0; lv < plan.levels(); lv++) {
+        const int64_t rows = plan.rows[lv];
+        if (rows <= 0) continue;
+        const LevelSeg* sg = d_segs + plan.seg_lo[lv];
+        const int nseg = plan.seg_lo[lv + 1] - plan.seg_lo[lv];
+        const unsigned grid = (unsigned)((rows * lanes + BB - 1) / BB);
+#define LV_LAUNCH(T, G) hipLaunchKernelGGL((k_batch_level<T, G>), dim3(grid), dim3(BB), 0, st, items, sg, nseg, rows)
+#define LV_SWITCH(T)                        \
+        switch (lanes) {                    \
+            case 1: LV_LAUNCH(T, 1); break;  \
+            case 4: LV_LAUNCH(T, 4); break;  \
+            case 8: LV_LAUNCH(T, 8); break;  \
+            case 16: LV_LAUNCH(T, 16); break; \
+            default: LV_LAUNCH(T, 64); break; \
+        }
+        if (dtype == SABLE_F64) {
+            LV_SWITCH(double)
+        } else {
+            LV_SWITCH(float)
+        }
+#undef LV_SWITCH
+#undef LV_LAUNCH
+    }
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
 int batch_spts_launch(hipStream_t st, int num_cu, sable_dtype_t dtype, int lanes, const sable_batch_spts_item_t* items,
                       int nitems, int64_t total_rows, unsigned* ticket, int* err, long long spin) {
     if (total_rows <= 0) return SABLE_OK;
@@ -409,6 +524,29 @@ int batch_refine_launch(hipStream_t st, int num_cu, int lanes, const sable_batch
             default: hipLaunchKernelGGL((k_batch_residual<64>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
         }
         int rc = batch_spts_launch(st, num_cu, SABLE_F32, lanes, spts_view, nitems, total_rows, ticket, err, spin);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_batch_update, dim3(gsz(total_rows)), dim3(BB), 0, st, items, nitems, total_rows, first);
+    }
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+int batch_refine_level_launch(hipStream_t st, int lanes, const sable_batch_refine_item_t* items,
+                              sable_batch_spts_item_t* spts_view, int nitems, int64_t total_rows, int iters,
+                              const LevelSeg* d_segs, const LevelPlan& plan) {
+    if (total_rows <= 0) return SABLE_OK;
+    hipLaunchKernelGGL(k_refine_to_spts, dim3(1), dim3(BB), 0, st, items, spts_view, nitems);
+    const unsigned rgrid = (unsigned)((total_rows * lanes + BB - 1) / BB);
+    for (int it = 0; it <= iters; it++) {
+        const int first = it == 0;
+        switch (lanes) {
+            case 1: hipLaunchKernelGGL((k_batch_residual<1>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
+            case 4: hipLaunchKernelGGL((k_batch_residual<4>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
+            case 8: hipLaunchKernelGGL((k_batch_residual<8>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
+            case 16: hipLaunchKernelGGL((k_batch_residual<16>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
+            default: hipLaunchKernelGGL((k_batch_residual<64>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
+        }
+        int rc = batch_level_launch(st, SABLE_F32, lanes, spts_view, d_segs, plan);
         if (rc) return rc;
         hipLaunchKernelGGL(k_batch_update, dim3(gsz(total_rows)), dim3(BB), 0, st, items, nitems, total_rows, first);
     }

@@ -9,11 +9,19 @@
 // refinement work space) is exhausted, stages the inputs, and launches
 //   mode 0 (plain):   one request per batch, FP64, one SpMV launch or one
 //                     sync-free solve launch per request;
-//   mode 1 (batched): FP64, one persistent SpMV launch and one sync-free
-//                     solve launch for the whole batch;
+//   mode 1 (batched): FP64, one persistent SpMV launch per right-hand-side
+//                     width and one sync-free solve launch for the whole
+//                     batch; SpMV requests on the same matrix share one
+//                     multi-RHS product (up to `group_rhs` of them, x and y
+//                     row-major [N][w] with w = 1, 2, 4 or 8), so the
+//                     matrix is read once for all of them;
 //   mode 2 (mixed):   batched, SpMV with FP32 values/vectors and FP32
 //                     accumulation; solves in FP32 with FP64 residual and
 //                     update for `refine_iters` refinement steps.
+// Batched solves (modes 1 and 2) run level by level: every lower matrix
+// carries its level schedule from registration, and launch l of a batch
+// solves level l of all of the batch's solves at once (a batch needs as
+// many launches as its deepest matrix has levels).
 // A completion thread waits on each slot's event in order, copies results
 // out and records latencies.  Several slots in flight let a new batch be
 // staged and launched while earlier ones run on the GPU.
@@ -51,6 +59,10 @@ struct SMatrix {
     int64_t nblocks = 0;
     int lds_cap = 0;
     size_t bytes = 0;
+    // lower matrices: level schedule (level_ptr on the host, rows listed by
+    // level on the device) for the batched level-set solves
+    std::vector<int32_t> lp;
+    int32_t* perm = nullptr;
 };
 
 struct SRequest {
@@ -65,6 +77,7 @@ struct SRequest {
     int status = 0;
     bool done = false;
     size_t in_off = 0, out_off = 0, work_off = 0;
+    int32_t col = 0, ld = 1;  // SpMV: column of the request in its group's [N][ld] x and y
 };
 
 struct FillItem {
@@ -73,6 +86,7 @@ struct FillItem {
     int32_t n;
     int32_t is_f32;
     uint64_t seed;
+    int64_t stride;  // element i is written at ptr[i * stride]
 };
 
 struct Slot {
@@ -83,6 +97,8 @@ struct Slot {
     char* stage_out = nullptr;
     char* h_items = nullptr;  // pinned staging for item arrays
     char* d_items = nullptr;  // device item arrays
+    char* h_segs = nullptr;   // pinned staging for the level table of the batch's solves
+    char* d_segs = nullptr;
     unsigned long long* d_counter = nullptr;
     unsigned* d_ticket = nullptr;
     int* d_err = nullptr;
@@ -94,6 +110,7 @@ struct Slot {
 };
 
 constexpr size_t ITEM_BYTES = 1 << 20;  // per slot, per item kind
+constexpr size_t SEG_BYTES = 8 << 20;   // per slot: level segments of a batch's solves
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;
@@ -115,8 +132,8 @@ __global__ void k_fill_uniform(const FillItem* __restrict__ items, int nitems, i
         const int64_t i = g - it.begin;
         const double u = (double)(mix64(it.seed * 0x100000001B3ull + (uint64_t)i) >> 11) * (1.0 / 9007199254740992.0);
         const double v = 2.0 * u - 1.0;
-        if (it.is_f32) ((float*)it.ptr)[i] = (float)v;
-        else ((double*)it.ptr)[i] = v;
+        if (it.is_f32) ((float*)it.ptr)[i * it.stride] = (float)v;
+        else ((double*)it.ptr)[i * it.stride] = v;
     }
 }
 
@@ -157,12 +174,24 @@ struct sable_server_s {
     void scheduler();
     void completer();
     int launch_batch(Slot& s, std::vector<SRequest*>& batch);
-    size_t need_bytes(const SRequest* r) const {
-        const size_t n = (size_t)mats[r->mat].N;
-        if (r->op == 0) return 2 * al(n * vbytes_spmv());
-        return 2 * al(n * 8) + (cfg.mode == 2 ? 2 * al(n * 4) : 0);
-    }
+    int group_max() const { return cfg.mode == 0 ? 1 : std::max(1, std::min(8, cfg.group_rhs)); }
+    // Arena bytes that admitting r adds to a batch in which `before` SpMV
+    // requests on the same matrix precede it: SpMV requests on one matrix
+    // form groups of up to group_max() in arrival order, each holding x and
+    // y of width 1, 2, 4 or 8.
+    size_t need_bytes(const SRequest* r, int before) const;
 };
+
+static int width_for(size_t k) { return k == 0 ? 0 : k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : 8; }
+
+size_t sable_server_s::need_bytes(const SRequest* r, int before) const {
+    const size_t n = (size_t)mats[r->mat].N;
+    if (r->op == 1) return 2 * al(n * 8) + (cfg.mode == 2 ? 2 * al(n * 4) : 0);
+    const int g = group_max();
+    const int k = before % g;  // requests already in the open group
+    auto cost = [&](int w) { return w ? 2 * al(n * (size_t)w * vbytes_spmv()) : (size_t)0; };
+    return cost(width_for((size_t)k + 1)) - cost(width_for((size_t)k));
+}
 
 void sable_server_s::scheduler() {
     (void)hipSetDevice(ctx->device);
@@ -191,10 +220,12 @@ void sable_server_s::scheduler() {
             const size_t cap = (size_t)cfg.arena_bytes;
             size_t used_b = 0;
             const int maxb = cfg.mode == 0 ? 1 : cfg.max_batch;
+            std::unordered_map<int32_t, int> per_mat;  // SpMV requests per matrix in this batch
             while (!pending.empty() && (int)batch.size() < maxb) {
                 SRequest* r = pending.front();
-                const size_t nb = need_bytes(r);
-                if (nb > cap) {  // can never fit: reject
+                int& before = per_mat[r->op == 0 ? r->mat : -1 - r->mat];
+                const size_t nb = need_bytes(r, r->op == 0 ? before : 0);
+                if (need_bytes(r, 0) > cap) {  // can never fit: reject
                     pending.pop_front();
                     r->status = SABLE_EBUDGET;
                     r->done = true;
@@ -205,6 +236,7 @@ void sable_server_s::scheduler() {
                 }
                 if (used_b + nb > cap) break;
                 used_b += nb;
+                if (r->op == 0) before++;
                 batch.push_back(r);
                 pending.pop_front();
             }
@@ -229,23 +261,66 @@ void sable_server_s::scheduler() {
     }
 }
 
+namespace {
+// SpMV requests of one batch that share a matrix and one [N][w] x and y.
+struct SpmvGroup {
+    int32_t mat;
+    int w;  // 1, 2, 4 or 8 columns (requests, padded to a power of two)
+    std::vector<SRequest*> rs;
+    size_t x_off = 0, y_off = 0;
+};
+
+}  // namespace
+
 int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
     s.reqs = batch;
     const bool mixed = cfg.mode == 2;
     const size_t vs = vbytes_spmv();
+    // group SpMV requests by matrix (arrival order within a group); plain
+    // mode keeps one request per product
+    const int gmax = group_max();
+    std::vector<SpmvGroup> groups;
+    {
+        std::unordered_map<int32_t, size_t> open;
+        for (auto* r : batch) {
+            if (r->op != 0) continue;
+            auto it = open.find(r->mat);
+            if (it == open.end() || groups[it->second].rs.size() == (size_t)gmax) {
+                open[r->mat] = groups.size();
+                groups.push_back(SpmvGroup{r->mat, 1, {}, 0, 0});
+                it = open.find(r->mat);
+            }
+            groups[it->second].rs.push_back(r);
+        }
+        for (auto& g : groups) g.w = width_for(g.rs.size());
+    }
     // arena layout: inputs, then outputs, then refinement work
     size_t off = 0;
+    for (auto& g : groups) {
+        g.x_off = off;
+        off += al((size_t)mats[g.mat].N * g.w * vs);
+        for (size_t c = 0; c < g.rs.size(); c++) {
+            g.rs[c]->in_off = g.x_off;
+            g.rs[c]->col = (int32_t)c;
+            g.rs[c]->ld = g.w;
+        }
+    }
     for (auto* r : batch) {
-        const size_t n = (size_t)mats[r->mat].N;
+        if (r->op == 0) continue;
         r->in_off = off;
-        off += al(n * (r->op == 0 ? vs : 8));
+        off += al((size_t)mats[r->mat].N * 8);
     }
     const size_t in_end = off;
     s.out_begin = off;
+    for (auto& g : groups) {
+        g.y_off = off;
+        off += al((size_t)mats[g.mat].N * g.w * vs);
+        for (auto* r : g.rs) r->out_off = g.y_off;
+    }
     for (auto* r : batch) {
-        const size_t n = (size_t)mats[r->mat].N;
+        if (r->op == 0) continue;
         r->out_off = off;
-        off += al(n * (r->op == 0 ? vs : 8));
+        off += al((size_t)mats[r->mat].N * 8);
     }
     s.out_end = off;
     for (auto* r : batch) {
@@ -254,52 +329,65 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
             off += 2 * al((size_t)mats[r->mat].N * 4);
         }
     }
+    SABLE_CHECK(off <= (size_t)cfg.arena_bytes, SABLE_EINVAL, "batch exceeds the slot arena");
     hipStream_t st = s.stream;
-    // host inputs -> pinned staging -> device
+    // host inputs -> pinned staging -> device; seeded inputs are generated
+    // on the device
     bool any_host = false;
     std::vector<FillItem> fills;
     int64_t fill_total = 0;
     for (auto* r : batch) {
         const int32_t n = mats[r->mat].N;
         const bool f32 = r->op == 0 && mixed;
+        const size_t es = r->op == 0 ? vs : 8;
+        char* dst = s.arena + r->in_off + (size_t)r->col * es;
         if (r->seeded) {
-            fills.push_back(FillItem{s.arena + r->in_off, fill_total, n, f32 ? 1 : 0, r->seed});
+            fills.push_back(FillItem{dst, fill_total, n, f32 ? 1 : 0, r->seed, r->ld});
             fill_total += n;
         } else {
             any_host = true;
+            char* h = s.stage_in + r->in_off + (size_t)r->col * es;
             if (f32) {
-                float* d = (float*)(s.stage_in + r->in_off);
-                for (int32_t i = 0; i < n; i++) d[i] = (float)r->in[i];
+                float* d = (float*)h;
+                for (int32_t i = 0; i < n; i++) d[(size_t)i * r->ld] = (float)r->in[i];
+            } else if (r->ld == 1) {
+                memcpy(h, r->in, 8 * (size_t)n);
             } else {
-                memcpy(s.stage_in + r->in_off, r->in, 8 * (size_t)n);
+                double* d = (double*)h;
+                for (int32_t i = 0; i < n; i++) d[(size_t)i * r->ld] = r->in[i];
             }
         }
     }
     if (any_host) SABLE_HIP(hipMemcpyAsync(s.arena, s.stage_in, in_end, hipMemcpyHostToDevice, st));
-    // item arrays: [fill | spmv | spts or refine | spts view]
+    // item arrays: [fill | spmv (by width) | spts or refine | spts view]
     char* hi = s.h_items;
-    size_t fill_off = 0, spmv_off = ITEM_BYTES, spts_off = 2 * ITEM_BYTES, view_off = 3 * ITEM_BYTES;
+    const size_t fill_off = 0, spmv_off = ITEM_BYTES, spts_off = 2 * ITEM_BYTES, view_off = 3 * ITEM_BYTES;
     SABLE_CHECK(fills.size() * sizeof(FillItem) <= ITEM_BYTES, SABLE_EINVAL, "too many requests in a batch");
     memcpy(hi + fill_off, fills.data(), fills.size() * sizeof(FillItem));
-    std::vector<sable_batch_spmv_item_t> sv;
+    std::vector<sable_batch_spmv_item_t> sv[4];  // widths 1, 2, 4, 8
+    int64_t blk_total[4] = {0, 0, 0, 0};
+    for (auto& g : groups) {
+        const SMatrix& m = mats[g.mat];
+        const int wi = g.w == 1 ? 0 : g.w == 2 ? 1 : g.w == 4 ? 2 : 3;
+        sable_batch_spmv_item_t it;
+        it.row_ptr = m.rp;
+        it.col = m.col;
+        it.val = mixed ? (const void*)m.val32 : (const void*)m.val64;
+        it.blk = m.blk;
+        it.x = s.arena + g.x_off;
+        it.y = s.arena + g.y_off;
+        it.blk_begin = blk_total[wi];
+        it.beta = 0.0;
+        blk_total[wi] += m.nblocks;
+        sv[wi].push_back(it);
+    }
     std::vector<sable_batch_spts_item_t> tv;
     std::vector<sable_batch_refine_item_t> rv;
-    int64_t blk_total = 0, row_total = 0;
+    int64_t row_total = 0;
     for (auto* r : batch) {
+        if (r->op == 0) continue;
         const SMatrix& m = mats[r->mat];
-        if (r->op == 0) {
-            sable_batch_spmv_item_t it;
-            it.row_ptr = m.rp;
-            it.col = m.col;
-            it.val = mixed ? (const void*)m.val32 : (const void*)m.val64;
-            it.blk = m.blk;
-            it.x = s.arena + r->in_off;
-            it.y = s.arena + r->out_off;
-            it.blk_begin = blk_total;
-            it.beta = 0.0;
-            blk_total += m.nblocks;
-            sv.push_back(it);
-        } else if (!mixed) {
+        if (!mixed) {
             sable_batch_spts_item_t it;
             it.row_ptr = m.rp;
             it.col = m.col;
@@ -328,13 +416,20 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
             rv.push_back(it);
         }
     }
-    SABLE_CHECK(sv.size() * sizeof(sv[0]) <= ITEM_BYTES && tv.size() * sizeof(tv[0]) <= ITEM_BYTES &&
-                    rv.size() * sizeof(rv[0]) <= ITEM_BYTES,
+    size_t sbytes[4], sofs[4], sb = 0;
+    for (int wi = 0; wi < 4; wi++) {
+        sofs[wi] = spmv_off + sb;
+        sbytes[wi] = sv[wi].size() * sizeof(sable_batch_spmv_item_t);
+        sb += sbytes[wi];
+    }
+    SABLE_CHECK(sb <= ITEM_BYTES && tv.size() * sizeof(sable_batch_spts_item_t) <= ITEM_BYTES &&
+                    rv.size() * sizeof(sable_batch_refine_item_t) <= ITEM_BYTES,
                 SABLE_EINVAL, "too many requests in a batch");
-    memcpy(hi + spmv_off, sv.data(), sv.size() * sizeof(sv[0]));
+    for (int wi = 0; wi < 4; wi++)
+        if (sbytes[wi]) memcpy(hi + sofs[wi], sv[wi].data(), sbytes[wi]);
     if (mixed) memcpy(hi + spts_off, rv.data(), rv.size() * sizeof(rv[0]));
     else memcpy(hi + spts_off, tv.data(), tv.size() * sizeof(tv[0]));
-    const size_t fb = fills.size() * sizeof(FillItem), sb = sv.size() * sizeof(sv[0]);
+    const size_t fb = fills.size() * sizeof(FillItem);
     const size_t tb = mixed ? rv.size() * sizeof(rv[0]) : tv.size() * sizeof(tv[0]);
     if (fb) SABLE_HIP(hipMemcpyAsync(s.d_items + fill_off, hi + fill_off, fb, hipMemcpyHostToDevice, st));
     if (sb) SABLE_HIP(hipMemcpyAsync(s.d_items + spmv_off, hi + spmv_off, sb, hipMemcpyHostToDevice, st));
@@ -348,6 +443,39 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
     }
     const int lanes = cfg.spts_lanes > 0 ? cfg.spts_lanes : 4;
     const long long spin = 1ll << 24;
+    // level table of the batch's solves: level l lists, item by item, the
+    // rows of level l of every item that has one
+    LevelPlan plan;
+    std::vector<LevelSeg> segs;
+    if (cfg.mode != 0) {
+        std::vector<const SRequest*> solves;
+        for (auto* r : batch)
+            if (r->op == 1) solves.push_back(r);
+        int top = 0;
+        for (auto* r : solves) top = std::max(top, (int)mats[r->mat].lp.size() - 1);
+        plan.seg_lo.assign((size_t)top + 1, 0);
+        plan.rows.assign((size_t)top, 0);
+        for (int lv = 0; lv < top; lv++) {
+            plan.seg_lo[lv] = (int32_t)segs.size();
+            int64_t pre = 0;
+            for (size_t k = 0; k < solves.size(); k++) {
+                const SMatrix& m = mats[solves[k]->mat];
+                if (lv + 1 >= (int)m.lp.size()) continue;
+                const int32_t cnt = m.lp[lv + 1] - m.lp[lv];
+                if (cnt <= 0) continue;
+                segs.push_back(LevelSeg{m.perm + m.lp[lv], pre, (int32_t)k, cnt});
+                pre += cnt;
+            }
+            plan.rows[lv] = pre;
+        }
+        plan.seg_lo[top] = (int32_t)segs.size();
+        const size_t gb = segs.size() * sizeof(LevelSeg);
+        SABLE_CHECK(gb <= SEG_BYTES, SABLE_EINVAL, "too many solve levels in a batch");
+        if (gb) {
+            memcpy(s.h_segs, segs.data(), gb);
+            SABLE_HIP(hipMemcpyAsync(s.d_segs, s.h_segs, gb, hipMemcpyHostToDevice, st));
+        }
+    }
     SABLE_HIP(hipMemsetAsync(s.d_err, 0, sizeof(int), st));
     int rc = SABLE_OK;
     if (cfg.mode == 0) {
@@ -361,18 +489,19 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
             rc = syncfree_launch(st, ctx->num_cu, SABLE_F64, lanes, m.N, m.rp, m.col, m.val64, s.arena + r->in_off,
                                  s.arena + r->out_off, s.d_ticket, s.d_err, spin);
     } else {
-        if (!sv.empty())
-            rc = batch_spmv_launch(st, ctx->num_cu, mixed ? SABLE_F32 : SABLE_F64, 1,
-                                   (const sable_batch_spmv_item_t*)(s.d_items + spmv_off), (int)sv.size(), blk_total,
-                                   s.d_counter, 0);
+        static const int widths[4] = {1, 2, 4, 8};
+        for (int wi = 0; wi < 4 && !rc; wi++)
+            if (!sv[wi].empty())
+                rc = batch_spmv_launch(st, ctx->num_cu, mixed ? SABLE_F32 : SABLE_F64, widths[wi],
+                                       (const sable_batch_spmv_item_t*)(s.d_items + sofs[wi]), (int)sv[wi].size(),
+                                       blk_total[wi], s.d_counter, 0);
         if (!rc && !tv.empty())
-            rc = batch_spts_launch(st, ctx->num_cu, SABLE_F64, lanes,
-                                   (const sable_batch_spts_item_t*)(s.d_items + spts_off), (int)tv.size(), row_total,
-                                   s.d_ticket, s.d_err, spin);
+            rc = batch_level_launch(st, SABLE_F64, lanes, (const sable_batch_spts_item_t*)(s.d_items + spts_off),
+                                    (const LevelSeg*)s.d_segs, plan);
         if (!rc && !rv.empty())
-            rc = batch_refine_launch(st, ctx->num_cu, lanes, (const sable_batch_refine_item_t*)(s.d_items + spts_off),
-                                     (sable_batch_spts_item_t*)(s.d_items + view_off), (int)rv.size(), row_total,
-                                     cfg.refine_iters, s.d_ticket, s.d_err, spin);
+            rc = batch_refine_level_launch(st, lanes, (const sable_batch_refine_item_t*)(s.d_items + spts_off),
+                                           (sable_batch_spts_item_t*)(s.d_items + view_off), (int)rv.size(), row_total,
+                                           cfg.refine_iters, (const LevelSeg*)s.d_segs, plan);
     }
     SABLE_HIP(hipMemcpyAsync(s.stage_out + s.out_begin, s.arena + s.out_begin, s.out_end - s.out_begin,
                              hipMemcpyDeviceToHost, st));
@@ -399,11 +528,15 @@ void sable_server_s::completer() {
         for (auto* r : s.reqs) {
             if (r->out) {
                 const int32_t n = mats[r->mat].N;
+                const size_t ld = (size_t)r->ld;
                 if (r->op == 0 && mixed) {
-                    const float* src = (const float*)(s.stage_out + r->out_off);
-                    for (int32_t i = 0; i < n; i++) r->out[i] = (double)src[i];
-                } else {
+                    const float* src = (const float*)(s.stage_out + r->out_off) + r->col;
+                    for (int32_t i = 0; i < n; i++) r->out[i] = (double)src[i * ld];
+                } else if (ld == 1) {
                     memcpy(r->out, s.stage_out + r->out_off, 8 * (size_t)n);
+                } else {
+                    const double* src = (const double*)(s.stage_out + r->out_off) + r->col;
+                    for (int32_t i = 0; i < n; i++) r->out[i] = src[i * ld];
                 }
             }
         }
@@ -429,7 +562,7 @@ void sable_server_s::completer() {
 
 static void free_matrix(SMatrix& m) {
     if (m.plan) sable_csr_plan_destroy(m.plan);
-    void* ps[] = {m.rp, m.col, m.val64, m.val32};
+    void* ps[] = {m.rp, m.col, m.val64, m.val32, m.perm};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     m = SMatrix();
@@ -441,6 +574,8 @@ static void free_server(sable_server_s* s) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         if (sl.arena) (void)hipFree(sl.arena);
         if (sl.d_items) (void)hipFree(sl.d_items);
+        if (sl.d_segs) (void)hipFree(sl.d_segs);
+        if (sl.h_segs) (void)hipHostFree(sl.h_segs);
         if (sl.d_counter) (void)hipFree(sl.d_counter);
         if (sl.stage_in) (void)hipHostFree(sl.stage_in);
         if (sl.stage_out) (void)hipHostFree(sl.stage_out);
@@ -466,7 +601,9 @@ SABLE_API int sable_server_create(sable_ctx_t* ctx, const sable_server_config_t*
     if (s->cfg.arena_bytes <= 0) s->cfg.arena_bytes = 256ll << 20;
     if (s->cfg.refine_iters < 0) s->cfg.refine_iters = 2;
     if (s->cfg.max_batch > 4096) s->cfg.max_batch = 4096;
-    const int64_t slot_bytes = s->cfg.arena_bytes + 4 * (int64_t)ITEM_BYTES + 4096;
+    if (s->cfg.group_rhs <= 0) s->cfg.group_rhs = 8;
+    if (s->cfg.group_rhs > 8) s->cfg.group_rhs = 8;
+    const int64_t slot_bytes = s->cfg.arena_bytes + 4 * (int64_t)ITEM_BYTES + (int64_t)SEG_BYTES + 4096;
     if (cfg->hbm_budget > 0 && slot_bytes * s->cfg.slots > cfg->hbm_budget) {
         delete s;
         return fail(SABLE_EBUDGET, "batch arenas (%lld bytes) exceed the HBM budget", (long long)(slot_bytes * cfg->slots));
@@ -482,6 +619,8 @@ SABLE_API int sable_server_create(sable_ctx_t* ctx, const sable_server_config_t*
                   hipHostMalloc(&sl.stage_in, (size_t)s->cfg.arena_bytes, hipHostMallocDefault) == hipSuccess &&
                   hipHostMalloc(&sl.stage_out, (size_t)s->cfg.arena_bytes, hipHostMallocDefault) == hipSuccess &&
                   hipHostMalloc(&sl.h_items, 4 * ITEM_BYTES, hipHostMallocDefault) == hipSuccess &&
+                  hipMalloc(&sl.d_segs, SEG_BYTES) == hipSuccess &&
+                  hipHostMalloc(&sl.h_segs, SEG_BYTES, hipHostMallocDefault) == hipSuccess &&
                   hipHostMalloc(&sl.h_err, 256, hipHostMallocDefault) == hipSuccess;
         if (!ok) {
             (void)hipGetLastError();
@@ -531,8 +670,28 @@ SABLE_API int sable_server_add_matrix(sable_server_t* s, int32_t N, const int32_
     m.N = N;
     m.nnz = nnz;
     m.kind = kind;
+    // level schedule of a lower matrix (par/src/sparse32.js:2952-3007: level
+    // 0 for a one-entry row, else 1 + the largest level among its other
+    // columns; rows listed level by level, ascending within a level)
+    std::vector<int32_t> h_lp, h_perm;
+    if (kind == 1) {
+        std::vector<int32_t> level(N);
+        int32_t top = 0;
+        for (int32_t i = 0; i < N; i++) {
+            int32_t m = -1;
+            for (int32_t j = row_ptr[i]; j < row_ptr[i + 1] - 1; j++) m = std::max(m, level[col[j]]);
+            level[i] = m + 1;
+            top = std::max(top, level[i]);
+        }
+        h_lp.assign((size_t)top + 2, 0);
+        for (int32_t i = 0; i < N; i++) h_lp[(size_t)level[i] + 1]++;
+        for (int32_t l = 0; l <= top; l++) h_lp[(size_t)l + 1] += h_lp[l];
+        std::vector<int32_t> next(h_lp.begin(), h_lp.end() - 1);
+        h_perm.resize(N);
+        for (int32_t i = 0; i < N; i++) h_perm[next[level[i]]++] = i;
+    }
     const size_t bytes = al(4 * (size_t)(N + 1)) + al(4 * (size_t)nnz) + al(8 * (size_t)nnz) + al(4 * (size_t)nnz) +
-                         al(4 * ((size_t)(nnz + N) / 512 + 16));
+                         al(4 * ((size_t)(nnz + N) / 512 + 16)) + (kind == 1 ? al(4 * (size_t)N) : 0);
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->cfg.hbm_budget > 0 && s->used + (int64_t)bytes > s->cfg.hbm_budget)
@@ -545,14 +704,17 @@ SABLE_API int sable_server_add_matrix(sable_server_t* s, int32_t N, const int32_
     bool ok = hipMalloc(&m.rp, 4 * (size_t)(N + 1)) == hipSuccess &&
               hipMalloc(&m.col, 4 * (size_t)(nnz ? nnz : 1)) == hipSuccess &&
               hipMalloc(&m.val64, 8 * (size_t)(nnz ? nnz : 1)) == hipSuccess &&
-              hipMalloc(&m.val32, 4 * (size_t)(nnz ? nnz : 1)) == hipSuccess;
+              hipMalloc(&m.val32, 4 * (size_t)(nnz ? nnz : 1)) == hipSuccess &&
+              (kind == 0 || hipMalloc(&m.perm, 4 * (size_t)N) == hipSuccess);
     int rc = SABLE_OK;
     if (!ok) {
         (void)hipGetLastError();
         rc = fail(SABLE_ENOMEM, "matrix allocation failed");
     } else if (hipMemcpyAsync(m.rp, row_ptr, 4 * (size_t)(N + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
                hipMemcpyAsync(m.col, col, 4 * (size_t)nnz, hipMemcpyHostToDevice, st) != hipSuccess ||
-               hipMemcpyAsync(m.val64, val, 8 * (size_t)nnz, hipMemcpyHostToDevice, st) != hipSuccess) {
+               hipMemcpyAsync(m.val64, val, 8 * (size_t)nnz, hipMemcpyHostToDevice, st) != hipSuccess ||
+               (kind == 1 && hipMemcpyAsync(m.perm, h_perm.data(), 4 * (size_t)N, hipMemcpyHostToDevice, st) !=
+                                 hipSuccess)) {
         rc = fail(SABLE_EHIP, "matrix upload failed");
     } else {
         unsigned g = grid_for(nnz, 256);
@@ -568,9 +730,10 @@ SABLE_API int sable_server_add_matrix(sable_server_t* s, int32_t N, const int32_
         return rc;
     }
     SABLE_HIP(hipStreamSynchronize(st));
+    m.lp = std::move(h_lp);
     std::lock_guard<std::mutex> lk(s->mu);
     *matrix_id = (int32_t)s->mats.size();
-    s->mats.push_back(m);
+    s->mats.push_back(std::move(m));
     return SABLE_OK;
 }
 

@@ -41,10 +41,12 @@ def spmv_bound(rp, col, val, x, dtype):
     return (2 * k + 4) * u / (1 - (k + 2) * u) * s + 1e-300
 
 
-@pytest.mark.parametrize("mode", ["plain", "batched", "mixed"])
-def test_alone_equals_mix_and_accuracy(ctx, pool, mode):
+@pytest.mark.parametrize("mode,kw", [("plain", {}), ("batched", {}), ("batched", {"group_rhs": 1}), ("mixed", {}),
+                                     ("mixed", {"refine_iters": 1})],
+                         ids=["plain", "batched", "batched_ungrouped", "mixed", "mixed_refine1"])
+def test_alone_equals_mix_and_accuracy(ctx, pool, mode, kw):
     mats, lows = pool
-    S, gids, lids = make_server(mode, pool)
+    S, gids, lids = make_server(mode, pool, **kw)
     rng = np.random.default_rng(5)
     probes = []
     for k in range(6):
@@ -56,7 +58,8 @@ def test_alone_equals_mix_and_accuracy(ctx, pool, mode):
     for op, mid, vec, _ in probes:
         out, lat = S.wait(S.submit(mid, op, vec))
         alone.append(out.copy())
-    # the same requests interleaved with 200 others submitted at once
+    # the same requests interleaved with 200 others submitted at once (SpMV
+    # requests on one matrix then share multi-RHS products)
     ids = []
     for j in range(200):
         k = rng.integers(len(gids))
@@ -86,7 +89,9 @@ def test_alone_equals_mix_and_accuracy(ctx, pool, mode):
 
 def test_hbm_budget_admission(ctx, pool):
     mats, lows = pool
-    S = sp.Server(mode="batched", hbm_budget=3 * ((16 << 20) + (4 << 20) + 4096) + (4 << 20),
+    # per slot: the 16 MiB arena, 4 MiB of item arrays, 8 MiB of level
+    # segments and 4 KiB of counters; 4 MiB left for matrices
+    S = sp.Server(mode="batched", hbm_budget=3 * ((16 << 20) + (4 << 20) + (8 << 20) + 4096) + (4 << 20),
                   arena_bytes=16 << 20, slots=3)
     n, rp, c, v = mats[0]
     S.add_matrix(rp, c, v)
@@ -121,4 +126,23 @@ def test_open_loop_load(ctx, pool, mode):
     assert 0 < st["p50_us"] <= st["p99_us"] <= st["max_us"]
     if mode == "mixed":
         assert st["mean_batch"] >= 1.0
+    S.close()
+
+
+def test_grouped_requests_on_one_matrix(ctx, pool):
+    """Nine SpMV requests on one matrix submitted together (they share
+    multi-RHS products, of widths up to 8, with whichever requests the
+    scheduler admits into the same batch); each result equals the request
+    computed alone."""
+    mats, _ = pool
+    S = sp.Server(mode="batched", arena_bytes=64 << 20, max_batch=64)
+    n, rp, c, v = mats[3]
+    mid = S.add_matrix(rp, c, v)
+    rng = np.random.default_rng(9)
+    xs = [rng.uniform(-1, 1, n) for _ in range(9)]
+    alone = [S.wait(S.submit(mid, "spmv", x))[0].copy() for x in xs]
+    ids = [S.submit(mid, "spmv", x) for x in xs]
+    for k, i in enumerate(ids):
+        out, _ = S.wait(i)
+        assert np.array_equal(out, alone[k]), k
     S.close()
