@@ -42,6 +42,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Fabric/HBM bytes per product measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE (one counter per run, scaled by the readings of a 1 GiB copy:
+# FETCH x2, WRITE x1) for the headline matrix (seed 20261203) and schedule;
+# profiles/csr_spmv_fp64_zipf4M_panels524288*.md hold the runs.
+MEASURED_TRAFFIC = {
+    "panels_524288_bw1024_k0": {"per_product": 8.303e9, "per_launch": 1.038e9},
+    "panels_524288_bw1024_k3": {"per_product": 5.854e9, "per_launch": 0.7318e9},
+}
 
 
 def log(*a):
@@ -449,13 +457,13 @@ def main():
     ap.add_argument("--zipf-s", type=float, default=1.5)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--panel-cols", type=str, default="0,524288",
+    ap.add_argument("--panel-cols", type=str, default="524288",
                     help="comma list of column-panel widths to time (0 = 2 MiB of x); the best is reported")
     ap.add_argument("--no-rowblock", action="store_true", help="skip timing the unblocked row-block kernel")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy (a plain copy kernel of known size, used to "
                          "scale hardware byte counters)")
-    ap.add_argument("--csr-kernel", type=str, default="0",
+    ap.add_argument("--csr-kernel", type=str, default="0,3",
                     help="comma list of CSR block kernels to time (0 basic, 1 staged, 2 persistent, 3 basic with "
                          "non-temporal matrix loads)")
     ap.add_argument("--block-weight", type=str, default="0",
@@ -535,8 +543,13 @@ def main():
                    "N": head["N"], "nnz": head["nnz"], "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
                    "schedule": head["best"], "variants": head["variants"]},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "traffic": (MEASURED_TRAFFIC[head["best"]]["per_product"]
+                                 if head["best"] in MEASURED_TRAFFIC and head["N"] == 4194304 else None),
+                     "traffic_per_launch": (MEASURED_TRAFFIC[head["best"]]["per_launch"]
+                                            if head["best"] in MEASURED_TRAFFIC and head["N"] == 4194304 else None),
                      "kernel": "k_csr_block",
+                     "limiter": "L2 request rate of the random 8-byte x gathers (TCC busy ~90%), not HBM bytes",
                      "launches_per_step": head["variants"][head["best"]]["launches_per_step"]},
         "cpu_baseline": head["cpu"],
         "check": head["check"],
