@@ -196,6 +196,10 @@ def sec_c1(args, sp, ctx, T):
     for name, M in (("csr", A), ("ell", E), ("dia", D), ("coo", C)):
         ms = T(lambda: sp.spmv(M, xv, yv, beta=0.0), 200, 20)
         out[name] = {"us": r4(ms * 1e3), "gbs": r4(fmt_bytes[name] / (ms * 1e-3) / 1e9)}
+    for g in (4, 8):
+        pl = sp.CsrPlan(A, {"vector_lanes": g})
+        ms = T(lambda: pl.run(xv.vec, yv.vec, beta=0.0), 200, 20)
+        out["csr_vec%d" % g] = {"us": r4(ms * 1e3), "gbs": r4(fmt_bytes["csr"] / (ms * 1e-3) / 1e9)}
     # reference protocol (seq/tests/main32.js): x[i] = i, y = 0, y += A.x, checksum
     sp.init_x(xv)
     sp.clear_vec(yv)
@@ -240,6 +244,13 @@ def sec_band(args, sp, ctx, T):
         ms = T(lambda: sp.spmv(M, xv, yv, beta=0.0), 20, 3)
         out[name + "_spmv"] = {"ms": r4(ms), "gbs": r4(b / (ms * 1e-3) / 1e9),
                                "pct_hbm": r4(100 * b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)}
+    # CSR with the row-vector kernel (lanes per row around the row length 7)
+    for g, rr in ((8, 1), (8, 2), (8, 4), (8, 8), (8, 16), (4, 8), (16, 4)):
+        pl = sp.CsrPlan(A, {"vector_lanes": g, "vector_rows": rr})
+        ms = T(lambda: pl.run(xv.vec, yv.vec, beta=0.0), 20, 3)
+        out["csr_vec%dx%d_spmv" % (g, rr)] = {"ms": r4(ms), "gbs": r4(b_csr / (ms * 1e-3) / 1e9),
+                                              "pct_hbm": r4(100 * b_csr / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)}
+        del pl
     t_tr, Tt = T.once(lambda: sp.transpose(A))
     out["transpose_csr_ms"] = r4(t_tr)
     del Tt, E

@@ -100,6 +100,9 @@ typedef struct sable_csr_plan_s sable_csr_plan_t;
 typedef struct {
     int block_weight;  /* nnz + row_weight*rows per work block (0 = default) */
     int row_weight;    /* cost of one row in nnz units (0 = default)         */
+    int vector_lanes;  /* 0: row blocks; 2..64: the row-vector kernel with this many
+                          lanes per row (for short, regular rows) */
+    int vector_rows;   /* row-vector kernel: rows per lane group, 1, 2, 4, 8 or 16 (0 = 4) */
 } sable_csr_tune_t;
 SABLE_API int sable_csr_plan_create(sable_ctx_t* ctx, int32_t N, const int32_t* row_ptr,
                                     const sable_csr_tune_t* tune, sable_csr_plan_t** out);
@@ -144,6 +147,19 @@ SABLE_API int sable_spmv_csr_panels(sable_ctx_t* ctx, const sable_csr_panels_t* 
 SABLE_API int sable_spmv_coo(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int64_t nnz,
                              const int32_t* row, const int32_t* col, const void* val,
                              const void* x, void* y, double beta, int inner_max);
+/* The same product with the row analysis done once: the plan checks the
+ * rows (SABLE_EMALFORMED when one is out of range; synchronises once) and,
+ * for row-sorted input, keeps the row heads and row blocks, so that
+ * sable_spmv_coo_plan only launches.  `row` must not change while the
+ * plan is used. */
+typedef struct sable_coo_plan_s sable_coo_plan_t;
+SABLE_API int sable_coo_plan_create(sable_ctx_t* ctx, int32_t N, int64_t nnz, const int32_t* row,
+                                    sable_coo_plan_t** out);
+SABLE_API void sable_coo_plan_destroy(sable_coo_plan_t* plan);
+SABLE_API int sable_coo_plan_info(const sable_coo_plan_t* plan, int* sorted, int64_t* nblocks);
+SABLE_API int sable_spmv_coo_plan(sable_ctx_t* ctx, const sable_coo_plan_t* plan, sable_dtype_t dtype,
+                                  const int32_t* row, const int32_t* col, const void* val, const void* x, void* y,
+                                  double beta, int inner_max);
 
 /* DIA, row-indexed storage: entry (i, i+offset[k]) at
  * data[k*stride + i - (offset[k] < 0 ? N - stride : 0)].  Per-row order is

@@ -52,7 +52,7 @@ class LoadStats(ctypes.Structure):
 
 
 class CsrTune(ctypes.Structure):
-    _fields_ = [("block_weight", c_int), ("row_weight", c_int)]
+    _fields_ = [("block_weight", c_int), ("row_weight", c_int), ("vector_lanes", c_int), ("vector_rows", c_int)]
 
 
 # name -> argtypes; every function returns int unless listed in _RESTYPES.
@@ -88,6 +88,10 @@ SIGNATURES = {
     "sable_csr_panels_info": [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32), ctypes.POINTER(c_i64)],
     "sable_spmv_csr_panels": [c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_spmv_coo": [c_vp, c_int, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
+    "sable_coo_plan_create": [c_vp, c_i32, c_i64, c_vp, c_vpp],
+    "sable_coo_plan_destroy": [c_vp],
+    "sable_coo_plan_info": [c_vp, ctypes.POINTER(c_int), ctypes.POINTER(c_i64)],
+    "sable_spmv_coo_plan": [c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_spmv_dia": [c_vp, c_int, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_spmv_ell": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_coo_csr": [c_vp, c_int, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
@@ -128,6 +132,7 @@ SIGNATURES = {
 _RESTYPES = {
     "sable_ctx_destroy": None,
     "sable_csr_plan_destroy": None,
+    "sable_coo_plan_destroy": None,
     "sable_spts_destroy": None,
     "sable_csr_panels_destroy": None,
     "sable_server_destroy": None,

@@ -146,10 +146,22 @@ class sparse_COO_t:
     def __init__(self, row, col, val, N, nnz):
         self.row, self.col, self.val = row, col, val
         self.N, self.nnz = int(N), int(nnz)
+        self._plan = None
 
     @property
     def dtype(self):
         return self.val.dtype
+
+    def plan(self):
+        """Cached row analysis for SpMV (rows checked once; call
+        invalidate() after writing to the row array)."""
+        if self._plan is None:
+            from .spmv import CooPlan
+            self._plan = CooPlan(self)
+        return self._plan
+
+    def invalidate(self):
+        self._plan = None
 
 
 class sparse_CSR_t:

@@ -15,6 +15,16 @@
 #include <cstdlib>
 #include <cstring>
 
+struct sable_coo_plan_s {
+    int32_t N = 0;
+    int64_t nnz = 0;
+    bool sorted = false;       // rows non-decreasing: segmented (CSR block) path
+    int32_t* rp = nullptr;     // sorted: row heads, N + 1
+    int32_t* blk = nullptr;    // sorted: row blocks
+    int64_t nblocks = 0;
+    int lds_cap = 0;
+};
+
 struct sable_csr_plan_s {
     sable_ctx_t* ctx = nullptr;
     int32_t N = 0;
@@ -25,6 +35,8 @@ struct sable_csr_plan_s {
     int block_weight = 0;
     int row_weight = 0;
     int lds_cap = 0;  // elements staged per block
+    int vector_lanes = 0;  // > 0: the row-vector kernel with this many lanes per row
+    int vector_rows = 4;   // rows per lane group of the row-vector kernel
     bool owns_blk = false;
 };
 
@@ -180,6 +192,87 @@ __global__ __launch_bounds__(BS) void k_csr_block(const int32_t* __restrict__ bl
     }
 }
 
+// Row-vector kernel for matrices of short, regular rows: G lanes per row
+// (G >= the typical row length), each group handling R rows, R*G loads in
+// flight per group before any is used; no LDS and no block barrier, so a
+// row costs three dependent memory round trips (row_ptr, col/val, x).
+// Consecutive groups of a wave take consecutive rows, so the col/val loads
+// of a wave are contiguous.
+template <typename T, int G, int R, bool NT>
+__global__ __launch_bounds__(BS) void k_csr_vec(int32_t N, const int32_t* __restrict__ rp,
+                                                const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                const T* __restrict__ x, T* __restrict__ y, T beta) {
+    const int64_t gg = ((int64_t)blockIdx.x * BS + threadIdx.x) / G;
+    const int l = threadIdx.x & (G - 1);
+    const int64_t NG = (int64_t)gridDim.x * (BS / G);
+    int a[R], e[R];
+    T s[R];
+    int maxlen = 0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int64_t i = gg + k * NG;
+        a[k] = i < N ? rp[i] : 0;
+        e[k] = i < N ? rp[i + 1] : 0;
+        s[k] = T(0);
+        maxlen = max(maxlen, e[k] - a[k]);
+    }
+    for (int c = l; c < maxlen; c += G) {
+        int cc[R];
+        T vv[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int j = a[k] + c;
+            cc[k] = j < e[k] ? ld_stream<NT>(col + j) : 0;
+            vv[k] = j < e[k] ? ld_stream<NT>(val + j) : T(0);
+        }
+        T xv[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) xv[k] = (a[k] + c < e[k]) ? x[cc[k]] : T(0);
+#pragma unroll
+        for (int k = 0; k < R; k++) s[k] += vv[k] * xv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) s[k] = group_sum(s[k], G);
+    if (l == 0) {
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int64_t i = gg + k * NG;
+            if (i < N) write_y(y, i, s[k], beta, e[k] > a[k]);
+        }
+    }
+}
+
+template <typename T, int R>
+static void vec_launch_r(hipStream_t st, int G, int32_t N, const int32_t* rp, const int32_t* col, const void* val,
+                         const void* x, void* y, double beta) {
+    const int64_t groups = ((int64_t)N + R - 1) / R;
+    const unsigned grid = (unsigned)((groups * G + BS - 1) / BS);
+#define VEC_LAUNCH(GG)                                                                                     \
+    hipLaunchKernelGGL((k_csr_vec<T, GG, R, true>), dim3(grid), dim3(BS), 0, st, N, rp, col, (const T*)val, \
+                       (const T*)x, (T*)y, (T)beta)
+    switch (G) {
+        case 2: VEC_LAUNCH(2); break;
+        case 4: VEC_LAUNCH(4); break;
+        case 8: VEC_LAUNCH(8); break;
+        case 16: VEC_LAUNCH(16); break;
+        case 32: VEC_LAUNCH(32); break;
+        default: VEC_LAUNCH(64); break;
+    }
+#undef VEC_LAUNCH
+}
+
+template <typename T>
+static void vec_launch_t(hipStream_t st, int G, int R, int32_t N, const int32_t* rp, const int32_t* col,
+                         const void* val, const void* x, void* y, double beta) {
+    switch (R) {
+        case 1: vec_launch_r<T, 1>(st, G, N, rp, col, val, x, y, beta); break;
+        case 2: vec_launch_r<T, 2>(st, G, N, rp, col, val, x, y, beta); break;
+        case 8: vec_launch_r<T, 8>(st, G, N, rp, col, val, x, y, beta); break;
+        case 16: vec_launch_r<T, 16>(st, G, N, rp, col, val, x, y, beta); break;
+        default: vec_launch_r<T, 4>(st, G, N, rp, col, val, x, y, beta); break;
+    }
+}
+
 // Persistent variant of the block kernel.  A grid of about 8 workgroups
 // per CU walks the blocks (XCD x owns the contiguous x-th eighth of them;
 // its workgroups stride through it), so no workgroup is dispatched per
@@ -313,18 +406,29 @@ __global__ __launch_bounds__(BS) void k_dia(int32_t N, int32_t nd, int32_t strid
     for (int k = threadIdx.x; k < ndl; k += blockDim.x) offs[k] = offset[k];
     __syncthreads();
     const int64_t shiftneg = (int64_t)N - stride;
+    constexpr int U = 8;
     for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N;
          n += (int64_t)gridDim.x * blockDim.x) {
         T t = beta == T(1) ? y[n] : (beta == T(0) ? T(0) : beta * y[n]);
-        for (int k = 0; k < nd; k++) {
-            const int off = k < 1024 ? offs[k] : offset[k];
-            const int64_t c = n + off;
-            const int64_t slot = n - (off < 0 ? shiftneg : 0);
-            // entries outside the stored run of this diagonal are not read
-            if (c >= 0 && c < N && slot >= 0 && slot < stride) {
-                const T d = data[(int64_t)k * stride + slot];
-                t = add_rn(mul_rn(d, x[c]), t);
+        // U diagonals per step: all loads of a step are issued (predicated)
+        // before the products are added in diagonal order
+        for (int k0 = 0; k0 < nd; k0 += U) {
+            T d[U], xv[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = k0 + u;
+                const int off = k < nd ? (k < 1024 ? offs[k] : offset[k]) : 0;
+                const int64_t c = n + off;
+                const int64_t slot = n - (off < 0 ? shiftneg : 0);
+                // entries outside the stored run of this diagonal are not read
+                ok[u] = k < nd && c >= 0 && c < N && slot >= 0 && slot < stride;
+                d[u] = ok[u] ? data[(int64_t)k * stride + slot] : T(0);
+                xv[u] = ok[u] ? x[c] : T(0);
             }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (ok[u]) t = add_rn(mul_rn(d[u], xv[u]), t);
         }
         y[n] = t;
     }
@@ -340,25 +444,25 @@ __global__ __launch_bounds__(BS) void k_ell(int32_t N, int32_t nc, const int32_t
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (int64_t)gridDim.x * blockDim.x) {
         T t = beta == T(1) ? y[i] : (beta == T(0) ? T(0) : beta * y[i]);
-        int j = 0;
-        for (; j + 3 < nc; j += 4) {
-            int c[4];
-            T d[4];
+        // U slots per step, loads predicated on j < nc and all issued before
+        // the products are added in slot order (4 keeps the number of
+        // column-major streams a wave touches at once small)
+        constexpr int U = 4;
+        for (int j = 0; j < nc; j += U) {
+            int c[U];
+            T d[U];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < U; u++) {
                 const int64_t e = (int64_t)(j + u) * N + i;
-                c[u] = idx[e];
-                d[u] = data[e];
+                c[u] = j + u < nc ? idx[e] : 0;
+                d[u] = j + u < nc ? data[e] : T(0);
             }
-            T xv[4];
+            T xv[U];
 #pragma unroll
-            for (int u = 0; u < 4; u++) xv[u] = x[c[u]];
+            for (int u = 0; u < U; u++) xv[u] = j + u < nc ? x[c[u]] : T(0);
 #pragma unroll
-            for (int u = 0; u < 4; u++) t = add_rn(mul_rn(d[u], xv[u]), t);
-        }
-        for (; j < nc; j++) {
-            const int64_t e = (int64_t)j * N + i;
-            t = add_rn(mul_rn(data[e], x[idx[e]]), t);
+            for (int u = 0; u < U; u++)
+                if (j + u < nc) t = add_rn(mul_rn(d[u], xv[u]), t);
         }
         y[i] = t;
     }
@@ -516,6 +620,35 @@ static int dispatch_csr(sable_ctx_t* ctx, sable_dtype_t dt, const int32_t* blk, 
     return SABLE_OK;
 }
 
+// Unsorted COO: y = beta*y first, then one atomic add per entry (the order
+// of accumulation is not deterministic).
+static int coo_atomic_launch(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int64_t nnz, const int32_t* row,
+                             const int32_t* col, const void* val, const void* x, void* y, double beta,
+                             int inner_max) {
+    unsigned g = grid_for(N, BS);
+    if (g > 16384) g = 16384;
+    unsigned gz = grid_for(nnz, BS);
+    if (gz > 65536) gz = 65536;
+    for (int it = 0; it < inner_max; it++) {
+        if (beta != 1.0) {
+            if (dtype == SABLE_F32)
+                hipLaunchKernelGGL(k_scale<float>, dim3(g), dim3(BS), 0, ctx->stream, N, (float*)y, (float)beta);
+            else
+                hipLaunchKernelGGL(k_scale<double>, dim3(g), dim3(BS), 0, ctx->stream, N, (double*)y, beta);
+        }
+        if (nnz > 0) {
+            if (dtype == SABLE_F32)
+                hipLaunchKernelGGL(k_coo_atomic<float>, dim3(gz), dim3(BS), 0, ctx->stream, nnz, row, col,
+                                   (const float*)val, (const float*)x, (float*)y);
+            else
+                hipLaunchKernelGGL(k_coo_atomic<double>, dim3(gz), dim3(BS), 0, ctx->stream, nnz, row, col,
+                                   (const double*)val, (const double*)x, (double*)y);
+        }
+        SABLE_LAUNCH_CHECK();
+    }
+    return SABLE_OK;
+}
+
 }  // namespace sable
 
 using namespace sable;
@@ -542,6 +675,20 @@ SABLE_API int sable_csr_plan_create(sable_ctx_t* ctx, int32_t N, const int32_t* 
     p->block_weight = T;
     p->row_weight = C;
     p->lds_cap = lds_cap_for(T);
+    if (tune && tune->vector_lanes > 0) {
+        const int G = tune->vector_lanes;
+        if (G != 2 && G != 4 && G != 8 && G != 16 && G != 32 && G != 64) {
+            delete p;
+            return fail(SABLE_EINVAL, "vector_lanes must be 2, 4, 8, 16, 32 or 64");
+        }
+        p->vector_lanes = G;
+        const int R = tune->vector_rows > 0 ? tune->vector_rows : 4;
+        if (R != 1 && R != 2 && R != 4 && R != 8 && R != 16) {
+            delete p;
+            return fail(SABLE_EINVAL, "vector_rows must be 1, 2, 4, 8 or 16");
+        }
+        p->vector_rows = R;
+    }
     if (hipMalloc(&p->blk, sizeof(int32_t) * (size_t)(B + 1) + 256) != hipSuccess) {
         (void)hipGetLastError();
         delete p;
@@ -598,6 +745,19 @@ SABLE_API int sable_spmv_csr_plan(sable_ctx_t* ctx, const sable_csr_plan_t* p, s
     SABLE_CHECK(ctx && p, SABLE_EINVAL, "NULL argument");
     SABLE_CHECK(inner_max >= 0, SABLE_EINVAL, "negative inner_max");
     if (p->N == 0 || inner_max == 0) return SABLE_OK;
+    if (p->vector_lanes > 0) {
+        SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+        for (int it = 0; it < inner_max; it++) {
+            if (dtype == SABLE_F32)
+                vec_launch_t<float>(ctx->stream, p->vector_lanes, p->vector_rows, p->N, p->row_ptr, col, val, x, y,
+                                    beta);
+            else
+                vec_launch_t<double>(ctx->stream, p->vector_lanes, p->vector_rows, p->N, p->row_ptr, col, val, x, y,
+                                     beta);
+            SABLE_LAUNCH_CHECK();
+        }
+        return SABLE_OK;
+    }
     return dispatch_csr(ctx, dtype, p->blk, p->nblocks, p->row_ptr, col, val, x, y, beta, inner_max, p->lds_cap);
 }
 
@@ -653,34 +813,90 @@ SABLE_API int sable_spmv_coo(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, i
     SABLE_HIP(hipStreamSynchronize(ctx->stream));
     e = *(int*)ctx->pinned;
     SABLE_CHECK(!(e & 2), SABLE_EMALFORMED, "COO row index out of range");
-    if (e & 1) {
-        // Unsorted: atomics (order of accumulation is not deterministic).
-        unsigned g = grid_for(N, BS);
-        if (g > 16384) g = 16384;
-        unsigned gz = grid_for(nnz, BS);
-        if (gz > 65536) gz = 65536;
-        for (int it = 0; it < inner_max; it++) {
-            if (beta != 1.0) {
-                if (dtype == SABLE_F32)
-                    hipLaunchKernelGGL(k_scale<float>, dim3(g), dim3(BS), 0, ctx->stream, N, (float*)y, (float)beta);
-                else
-                    hipLaunchKernelGGL(k_scale<double>, dim3(g), dim3(BS), 0, ctx->stream, N, (double*)y, beta);
-            }
-            if (dtype == SABLE_F32)
-                hipLaunchKernelGGL(k_coo_atomic<float>, dim3(gz), dim3(BS), 0, ctx->stream, nnz, row, col,
-                                   (const float*)val, (const float*)x, (float*)y);
-            else
-                hipLaunchKernelGGL(k_coo_atomic<double>, dim3(gz), dim3(BS), 0, ctx->stream, nnz, row, col,
-                                   (const double*)val, (const double*)x, (double*)y);
-            SABLE_LAUNCH_CHECK();
-        }
-        return SABLE_OK;
-    }
+    if (e & 1) return coo_atomic_launch(ctx, dtype, N, nnz, row, col, val, x, y, beta, inner_max);
     hipLaunchKernelGGL(k_coo_heads, dim3(grid_for(nnz + 1, BS)), dim3(BS), 0, ctx->stream, nnz, N, row, rp);
     SABLE_LAUNCH_CHECK();
     rc = plan_fill(ctx, N, rp, T, C, B, blk, err, false);
     if (rc) return rc;
     return dispatch_csr(ctx, dtype, blk, B, rp, col, val, x, y, beta, inner_max, lds_cap_for(T));
+}
+
+SABLE_API int sable_coo_plan_create(sable_ctx_t* ctx, int32_t N, int64_t nnz, const int32_t* row,
+                                    sable_coo_plan_t** out) {
+    SABLE_CHECK(ctx && out && (row || nnz == 0), SABLE_EINVAL, "sable_coo_plan_create: NULL argument");
+    SABLE_CHECK(N >= 0 && nnz >= 0, SABLE_EINVAL, "negative size");
+    *out = nullptr;
+    auto* p = new sable_coo_plan_s();
+    p->N = N;
+    p->nnz = nnz;
+    int T, C;
+    tune_defaults(nullptr, &T, &C);
+    p->lds_cap = lds_cap_for(T);
+    int* err = nullptr;
+    int rc = SABLE_OK;
+    if (hipMalloc(&err, 256) != hipSuccess) {
+        (void)hipGetLastError();
+        delete p;
+        return fail(SABLE_ENOMEM, "COO plan allocation failed");
+    }
+    auto done = [&](int code) {
+        (void)hipFree(err);
+        if (code) sable_coo_plan_destroy(p);
+        else *out = p;
+        return code;
+    };
+    SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), ctx->stream));
+    if (nnz > 0) {
+        unsigned g = grid_for(nnz, BS);
+        if (g > 16384) g = 16384;
+        hipLaunchKernelGGL(k_coo_check, dim3(g), dim3(BS), 0, ctx->stream, nnz, N, row, err);
+        if (hipGetLastError() != hipSuccess) return done(fail(SABLE_EHIP, "COO check launch failed"));
+    }
+    if (hipMemcpyAsync(ctx->pinned, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return done(fail(SABLE_EHIP, "COO check failed"));
+    const int e = *(int*)ctx->pinned;
+    if (e & 2) return done(fail(SABLE_EMALFORMED, "COO row index out of range"));
+    p->sorted = !(e & 1);
+    if (p->sorted && N > 0) {
+        p->nblocks = nblocks_for(nnz, N, T, C);
+        if (hipMalloc(&p->rp, 4 * ((size_t)N + 1)) != hipSuccess ||
+            hipMalloc(&p->blk, 4 * ((size_t)p->nblocks + 1)) != hipSuccess) {
+            (void)hipGetLastError();
+            return done(fail(SABLE_ENOMEM, "COO plan allocation failed"));
+        }
+        hipLaunchKernelGGL(k_coo_heads, dim3(grid_for(nnz + 1, BS)), dim3(BS), 0, ctx->stream, nnz, N, row, p->rp);
+        if (hipGetLastError() != hipSuccess) return done(fail(SABLE_EHIP, "COO heads launch failed"));
+        rc = plan_fill(ctx, N, p->rp, T, C, p->nblocks, p->blk, err, false);
+        if (rc) return done(rc);
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) return done(fail(SABLE_EHIP, "COO plan failed"));
+    }
+    return done(SABLE_OK);
+}
+
+SABLE_API void sable_coo_plan_destroy(sable_coo_plan_t* p) {
+    if (!p) return;
+    if (p->rp) (void)hipFree(p->rp);
+    if (p->blk) (void)hipFree(p->blk);
+    delete p;
+}
+
+SABLE_API int sable_coo_plan_info(const sable_coo_plan_t* p, int* sorted, int64_t* nblocks) {
+    SABLE_CHECK(p, SABLE_EINVAL, "NULL plan");
+    if (sorted) *sorted = p->sorted ? 1 : 0;
+    if (nblocks) *nblocks = p->nblocks;
+    return SABLE_OK;
+}
+
+SABLE_API int sable_spmv_coo_plan(sable_ctx_t* ctx, const sable_coo_plan_t* p, sable_dtype_t dtype,
+                                  const int32_t* row, const int32_t* col, const void* val, const void* x, void* y,
+                                  double beta, int inner_max) {
+    SABLE_CHECK(ctx && p, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(inner_max >= 0, SABLE_EINVAL, "negative inner_max");
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    if (p->N == 0 || inner_max == 0) return SABLE_OK;
+    if (p->sorted) return dispatch_csr(ctx, dtype, p->blk, p->nblocks, p->rp, col, val, x, y, beta, inner_max, p->lds_cap);
+    return coo_atomic_launch(ctx, dtype, p->N, p->nnz, row, col, val, x, y, beta, inner_max);
 }
 
 SABLE_API int sable_spmv_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,

@@ -20,7 +20,8 @@ class CsrPlan:
         h = c_vp()
         t = None
         if tune is not None:
-            t = ctypes.byref(_lib.CsrTune(int(tune.get("block_weight", 0)), int(tune.get("row_weight", 0))))
+            t = ctypes.byref(_lib.CsrTune(int(tune.get("block_weight", 0)), int(tune.get("row_weight", 0)),
+                                          int(tune.get("vector_lanes", 0)), int(tune.get("vector_rows", 0))))
         call("sable_csr_plan_create", self.ctx.handle, A.N, A.row.ptr, t, ctypes.byref(h))
         self.handle = h
 
@@ -38,6 +39,37 @@ class CsrPlan:
         try:
             if self.handle:
                 _lib.load().sable_csr_plan_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class CooPlan:
+    """Row analysis of a COO matrix (sable_coo_plan_*): rows checked once;
+    row-sorted input keeps its row heads and row blocks, so a product is
+    launch-only."""
+
+    def __init__(self, A):
+        self.ctx = A.row.ctx
+        self.A = A
+        h = c_vp()
+        call("sable_coo_plan_create", self.ctx.handle, A.N, A.nnz, A.row.ptr, ctypes.byref(h))
+        self.handle = h
+
+    def info(self):
+        srt, nb = ctypes.c_int(), c_i64()
+        call("sable_coo_plan_info", self.handle, ctypes.byref(srt), ctypes.byref(nb))
+        return {"sorted": bool(srt.value), "nblocks": nb.value}
+
+    def run(self, x, y, beta=1.0, inner_max=1):
+        A = self.A
+        call("sable_spmv_coo_plan", self.ctx.handle, self.handle, dtype_code(A.dtype), A.row.ptr, A.col.ptr,
+             A.val.ptr, x.ptr, y.ptr, float(beta), int(inner_max))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.load().sable_coo_plan_destroy(self.handle)
                 self.handle = None
         except Exception:
             pass
@@ -92,8 +124,11 @@ def spmv(A, x_view, y_view, inner_max=1, beta=1.0, use_plan=True):
     dt = dtype_code(A.dtype)
     x, y = x_view.vec.ptr, y_view.vec.ptr
     if isinstance(A, sparse_COO_t):
-        call("sable_spmv_coo", ctx, dt, A.N, A.nnz, A.row.ptr, A.col.ptr, A.val.ptr, x, y, float(beta),
-             int(inner_max))
+        if use_plan:
+            A.plan().run(x_view.vec, y_view.vec, beta, inner_max)
+        else:
+            call("sable_spmv_coo", ctx, dt, A.N, A.nnz, A.row.ptr, A.col.ptr, A.val.ptr, x, y, float(beta),
+                 int(inner_max))
     elif isinstance(A, sparse_CSR_t):
         if use_plan:
             A.plan().run(x_view.vec, y_view.vec, beta, inner_max)

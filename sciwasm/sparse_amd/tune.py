@@ -10,7 +10,8 @@ so later matrices of the same class reuse the choice without measuring:
   reference's 5x guard and cheaper;
 * rows per wavefront: the block weight (nnz + row_weight * rows per
   256-thread workgroup) and row weight, which set how many rows share a
-  wavefront in the reduction (a block of R rows gets 256/R lanes per row);
+  wavefront in the reduction (a block of R rows gets 256/R lanes per row),
+  or, for short regular rows, the row-vector kernel's lanes per row;
 * bin thresholds: the LDS capacity of a block (twice the block weight)
   decides which rows take the whole-workgroup long-row path;
 * column panels for matrices whose x does not fit one XCD's L2.
@@ -64,6 +65,14 @@ class Autotuner:
         if N * A.dtype.itemsize > (2 << 20):
             out.append({"format": "panels", "block_weight": 1024, "row_weight": 2})
         lens = np.diff(np.asarray(row_ptr, np.int64))
+        if len(lens):
+            # row-vector kernel for short, regular rows: lanes around the mean
+            # row length, as long as the longest row needs few passes
+            mean, mx = float(lens.mean()), int(lens.max())
+            g0 = 1 << max(1, int(math.ceil(math.log2(max(mean, 2.0)))))
+            for g in (g0 // 2, g0, 2 * g0):
+                if 2 <= g <= 64 and mx <= 8 * g:
+                    out.append({"format": "csr", "vector_lanes": g})
         if len(lens) and lens.max() * N <= 5 * max(A.nnz, 1):
             out.append({"format": "ell"})
         return out
@@ -81,7 +90,7 @@ class Autotuner:
         best, best_ms, table = None, float("inf"), []
         for cand in self.candidates(A, row_ptr):
             if cand["format"] == "csr":
-                plan = CsrPlan(A, {"block_weight": cand["block_weight"], "row_weight": cand["row_weight"]})
+                plan = CsrPlan(A, {k: cand[k] for k in ("block_weight", "row_weight", "vector_lanes") if k in cand})
                 ms = self._time(lambda: plan.run(x, y, beta=0.0))
             elif cand["format"] == "panels":
                 pan = CsrPanels(A, 0, {"block_weight": cand["block_weight"], "row_weight": cand["row_weight"]})

@@ -188,6 +188,54 @@ def test_coo_empty_and_errors(ctx):
     assert e.value.name == "SABLE_EMALFORMED"
 
 
+@pytest.mark.parametrize("lanes,rows", [(2, 1), (8, 4), (8, 16), (64, 2)])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("name,make", CASES)
+def test_csr_vector_kernel_parity(ctx, lanes, rows, dtype, name, make):
+    """The row-vector kernel (G lanes per row, rows longer than G loop) meets
+    the CSR bound, y += A.x and y = A.x."""
+    rp, col, val = make(dtype)
+    N = len(rp) - 1
+    rng = np.random.default_rng(41)
+    x = rng.uniform(-1, 1, N).astype(dtype)
+    y0 = rng.uniform(-1, 1, N).astype(dtype)
+    A = sp.csr_from_host(N, rp, col, val)
+    pl = sp.CsrPlan(A, {"vector_lanes": lanes, "vector_rows": rows})
+    xv = sp.vec_from_host(x)
+    for beta in (1.0, 0.0):
+        yv = sp.vec_from_host(y0)
+        pl.run(xv.vec, yv.vec, beta=beta)
+        yb = y0 if beta else np.zeros(N, dtype)
+        yref = oracle.spmv_csr(N, rp, col, val, x, yb)
+        check_close(yv.array(), yref, bound(rp, col, val, x, yb, dtype))
+
+
+@pytest.mark.parametrize("order", ["sorted", "shuffled"])
+def test_coo_plan_matches_unplanned(ctx, order):
+    """The planned product (rows analysed once) equals the unplanned entry
+    point: bit for bit on row-sorted input (both run the same segmented
+    kernel), within the COO bound on unsorted input (atomics)."""
+    rp, col, val = gen.random_csr(9000, 6, seed=31, empty_rows=0.1)
+    N = len(rp) - 1
+    row = gen.csr_to_coo_rows(rp)
+    if order == "shuffled":
+        p = np.random.default_rng(32).permutation(len(row))
+        row, col, val = row[p], col[p], val[p]
+    x = np.random.default_rng(33).uniform(-1, 1, N)
+    y0 = np.random.default_rng(34).uniform(-1, 1, N)
+    A = sp.coo_from_host(N, row, col, val)
+    assert A.plan().info()["sorted"] == (order == "sorted")
+    ya, yb = sp.vec_from_host(y0), sp.vec_from_host(y0)
+    sp.spmv(A, sp.vec_from_host(x), ya, inner_max=2)
+    sp.spmv(A, sp.vec_from_host(x), yb, inner_max=2, use_plan=False)
+    if order == "sorted":
+        assert np.array_equal(ya.array(), yb.array())
+    yref = oracle.spmv_coo(row, col, val, x, y0, 2)
+    y1 = oracle.spmv_coo(row, col, val, x, y0, 1)
+    b = bound_coo(N, row, col, val, x, y0, np.float64) + bound_coo(N, row, col, val, x, y1, np.float64)
+    check_close(ya.array(), yref, b)
+
+
 def test_csr_malformed_row_ptr(ctx):
     rp = np.array([0, 3, 2, 4], np.int32)
     A = sp.csr_from_host(3, rp, np.zeros(4, np.int32), np.ones(4))
