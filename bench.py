@@ -131,6 +131,25 @@ def headline(args, sp, ctx, T):
                 log(key, info, "%.4f ms  %.1f GB/s" % (ms, variants[key]["gbs"]))
                 del pan
     L.call("sable_csr_kernel_select", 0)
+    for spec in [v for v in args.tiles.split(",") if v.strip()]:
+        tr, pc = (int(t) for t in spec.split("x"))
+        t1 = time.time()
+        tl = sp.CsrTiles(A, tr, pc)
+        info = tl.info()
+        build_s = time.time() - t1
+        for mode in [int(v) for v in args.tiles_mode.split(",") if v.strip()]:
+            L.call("sable_csr_tiles_select", mode)
+            ms = T(lambda: tl.run(xv.vec, yv.vec, beta=0.0), args.steps, args.warmup)
+            key = "tiles_%dx%d_%s" % (tr, pc, ["perpanel", "fused"][mode])
+            variants[key] = {"ms": ms, "gbs": bytes_step / (ms * 1e-3) / 1e9,
+                             "launches_per_step": 1 if mode else info["npanels"],
+                             "npanels": info["npanels"], "row_tiles": info["row_tiles"], "tile_rows": tr,
+                             "panel_cols": pc, "kernel": "k_tiles_fused" if mode else "k_tiles",
+                             "plan_build_s": round(build_s, 3)}
+            results[key] = yv.array()
+            log(key, info, "built in %.2fs, %.4f ms  %.1f GB/s" % (build_s, ms, variants[key]["gbs"]))
+        L.call("sable_csr_tiles_select", 1)
+        del tl
     best = min(variants, key=lambda k: variants[k]["ms"])
     ms, gbs, y = variants[best]["ms"], variants[best]["gbs"], results[best]
     log("best %s: %.4f ms/step, %.1f GB/s (%.1f%% of HBM peak)" % (best, ms, gbs, 100 * gbs / HBM_PEAK_GBS))
@@ -471,6 +490,10 @@ def main():
     ap.add_argument("--panel-cols", type=str, default="524288",
                     help="comma list of column-panel widths to time (0 = 2 MiB of x); the best is reported")
     ap.add_argument("--no-rowblock", action="store_true", help="skip timing the unblocked row-block kernel")
+    ap.add_argument("--tiles", type=str, default="16384x8192,16384x16384",
+                    help="comma list of column-sorted tile shapes ROWSxCOLS to time")
+    ap.add_argument("--tiles-mode", type=str, default="1",
+                    help="tile schedules to time: 0 one launch per panel, 1 one launch over all panels")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy (a plain copy kernel of known size, used to "
                          "scale hardware byte counters)")
@@ -559,7 +582,7 @@ def main():
                                  if head["best"] in MEASURED_TRAFFIC and head["N"] == 4194304 else None),
                      "traffic_per_launch": (MEASURED_TRAFFIC[head["best"]]["per_launch"]
                                             if head["best"] in MEASURED_TRAFFIC and head["N"] == 4194304 else None),
-                     "kernel": "k_csr_block",
+                     "kernel": head["variants"][head["best"]].get("kernel", "k_csr_block"),
                      "limiter": "L2 request rate of the random 8-byte x gathers (TCC busy ~90%), not HBM bytes",
                      "launches_per_step": head["variants"][head["best"]]["launches_per_step"]},
         "cpu_baseline": head["cpu"],

@@ -139,6 +139,31 @@ SABLE_API int sable_csr_panels_info(const sable_csr_panels_t* p, int32_t* npanel
 SABLE_API int sable_spmv_csr_panels(sable_ctx_t* ctx, const sable_csr_panels_t* p, const void* x, void* y, double beta,
                                     int inner_max);
 
+/* Column-sorted tiles (no reference counterpart; a schedule of the same
+ * CSR product for matrices whose columns are scattered over an x much larger
+ * than one XCD's L2): rows are cut into tiles of tile_rows rows and columns
+ * into panels of panel_cols columns (both powers of two, tile_rows *
+ * panel_cols <= 2^32, tile_rows values must fit the 160 KiB LDS; 0 = 16384
+ * rows (FP64) or 32768 (FP32) and 8192 columns); each (panel, tile) keeps
+ * its entries sorted by
+ * column, so that consecutive lanes gather neighbouring x values, and sums
+ * its rows in LDS.  The plan owns its copy of the
+ * matrix (row_ptr[0] must be 0).  Same y = A.x + beta*y semantics and
+ * tolerance as sable_spmv_csr; the summation order of a row depends on LDS
+ * atomic order, so results are not bit-reproducible from run to run. */
+typedef struct sable_csr_tiles_s sable_csr_tiles_t;
+SABLE_API int sable_csr_tiles_create(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                                     const int32_t* col, const void* val, int32_t tile_rows, int32_t panel_cols,
+                                     sable_csr_tiles_t** out);
+SABLE_API void sable_csr_tiles_destroy(sable_csr_tiles_t* p);
+SABLE_API int sable_csr_tiles_info(const sable_csr_tiles_t* p, int32_t* row_tiles, int32_t* npanels, int64_t* bytes);
+SABLE_API int sable_spmv_csr_tiles(sable_ctx_t* ctx, const sable_csr_tiles_t* p, const void* x, void* y, double beta,
+                                   int inner_max);
+/* Tile schedule for all later products (process-wide): 0 = one launch per
+ * panel, y updated after each; 1 = one launch, each row tile walks all
+ * panels with its row sums in LDS and writes y once (default). */
+SABLE_API int sable_csr_tiles_select(int mode);
+
 /* COO.  Row-sorted input runs a deterministic segmented reduction (row
  * heads found on the device, then the CSR block kernel); unsorted input is
  * reduced with device atomics.  Replaces spmv_coo / spmv_coo_wrapper

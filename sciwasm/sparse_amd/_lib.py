@@ -88,6 +88,11 @@ SIGNATURES = {
     "sable_csr_panels_info": [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32), ctypes.POINTER(c_i64)],
     "sable_spmv_csr_panels": [c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_spmv_coo": [c_vp, c_int, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
+    "sable_csr_tiles_create": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_vpp],
+    "sable_csr_tiles_destroy": [c_vp],
+    "sable_csr_tiles_info": [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32), ctypes.POINTER(c_i64)],
+    "sable_spmv_csr_tiles": [c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
+    "sable_csr_tiles_select": [c_int],
     "sable_coo_plan_create": [c_vp, c_i32, c_i64, c_vp, c_vpp],
     "sable_coo_plan_destroy": [c_vp],
     "sable_coo_plan_info": [c_vp, ctypes.POINTER(c_int), ctypes.POINTER(c_i64)],
@@ -133,6 +138,7 @@ _RESTYPES = {
     "sable_ctx_destroy": None,
     "sable_csr_plan_destroy": None,
     "sable_coo_plan_destroy": None,
+    "sable_csr_tiles_destroy": None,
     "sable_spts_destroy": None,
     "sable_csr_panels_destroy": None,
     "sable_server_destroy": None,

@@ -1,0 +1,381 @@
+// spmv_tiles.hip -- CSR SpMV over column-sorted tiles, for matrices whose
+// columns are scattered over an x much larger than one XCD's L2.
+//
+// With random columns every x[col] gather is its own L2 request: the row-
+// block kernel spends its time in the L2 request rate, not in HBM bytes.
+// This schedule regroups the matrix so that consecutive lanes of a wave read
+// neighbouring x values:
+//  * the rows are cut into tiles of `tile_rows` rows and the columns into
+//    panels of `panel_cols` columns (x panel = panel_cols values, sized for
+//    one XCD's L2);
+//  * within each (panel, row tile) the entries are sorted by column (stable:
+//    equal columns keep ascending rows) and stored as one 32-bit word,
+//    column-in-panel << row_bits | row-in-tile, plus the value;
+//  * one workgroup per row tile streams its entries panel by panel, gathers
+//    x (consecutive lanes hit the same or adjacent x lines, so the gathers
+//    of a wave coalesce into fewer L2 requests, and a panel narrow enough
+//    keeps the lines in flight within the CU's L1), and accumulates the
+//    products into the tile's rows held in LDS with LDS atomic adds; y is
+//    written once at the end (or, in the per-panel schedule, one launch per
+//    panel adds the panel's sums to y).
+// Each row's products are summed in an order that depends on the LDS atomic
+// order, so the result is not bit-reproducible from run to run; it meets the
+// same per-row bound as every other CSR schedule.
+#include "common.h"
+#include "prims.h"
+
+#include <algorithm>
+#include <cstring>
+
+struct sable_csr_tiles_s {
+    sable_ctx_t* ctx = nullptr;
+    sable_dtype_t dtype = SABLE_F64;
+    int32_t N = 0;
+    int64_t nnz = 0;
+    int32_t tile_rows = 0;
+    int32_t panel_cols = 0;
+    int row_bits = 0;
+    int col_bits = 0;
+    int32_t ntr = 0;        // row tiles
+    int32_t npanels = 0;
+    int32_t* tile_ptr = nullptr;  // npanels * ntr + 1 offsets, panel-major
+    uint32_t* packed = nullptr;   // column-in-panel << row_bits | row-in-tile
+    void* val = nullptr;
+    size_t bytes = 0;
+};
+
+namespace sable {
+
+constexpr int TILE_BS = 1024;
+constexpr int TILE_U = 4;
+
+static int log2_exact(int64_t v) {
+    int b = 0;
+    while (((int64_t)1 << b) < v) b++;
+    return ((int64_t)1 << b) == v ? b : -1;
+}
+
+__global__ void k_tile_rows(int32_t N, const int32_t* __restrict__ rp, int32_t* __restrict__ rows) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        for (int64_t j = rp[i]; j < rp[i + 1]; j++) rows[j] = (int32_t)i;
+}
+
+// key = (panel * ntr + row tile) << col_bits | column-in-panel: a panel's
+// entries are contiguous, row tile after row tile, each tile by column, so
+// that the workgroups working on one panel at the same time read one
+// compact region of memory
+__global__ void k_tile_keys(int64_t nnz, int32_t N, const int32_t* __restrict__ rows, const int32_t* __restrict__ col,
+                            int row_bits, int col_bits, int32_t ntr, uint64_t* __restrict__ keys,
+                            uint32_t* __restrict__ idx, int* __restrict__ err) {
+    const uint32_t cmask = (1u << col_bits) - 1u;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t c = col[k];
+        if (c < 0 || c >= N) {
+            atomicOr(err, 1);
+            keys[k] = 0;
+        } else {
+            const int64_t t = (int64_t)(c >> col_bits) * ntr + (rows[k] >> row_bits);
+            keys[k] = ((uint64_t)t << col_bits) | ((uint32_t)c & cmask);
+        }
+        idx[k] = (uint32_t)k;
+    }
+}
+
+template <typename T>
+__global__ void k_tile_pack(int64_t nnz, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ idx,
+                            const int32_t* __restrict__ rows, const T* __restrict__ val, int row_bits, int col_bits,
+                            uint32_t* __restrict__ packed, T* __restrict__ vout) {
+    const uint32_t cmask = (1u << col_bits) - 1u, rmask = (1u << row_bits) - 1u;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nnz; s += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = idx[s];
+        packed[s] = (((uint32_t)keys[s] & cmask) << row_bits) | ((uint32_t)rows[k] & rmask);
+        vout[s] = val[k];
+    }
+}
+
+// tile_ptr[t] = first sorted position whose tile id is >= t (t in [0, ntiles])
+__global__ void k_tile_heads(int64_t nnz, int64_t ntiles, int col_bits, const uint64_t* __restrict__ keys,
+                             int32_t* __restrict__ tile_ptr) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nnz) return;
+    const int64_t prev = s > 0 ? (int64_t)(keys[s - 1] >> col_bits) : -1;
+    const int64_t cur = s < nnz ? (int64_t)(keys[s] >> col_bits) : ntiles;
+    for (int64_t t = prev + 1; t <= cur; t++) tile_ptr[t] = (int32_t)s;
+}
+
+template <typename V>
+__device__ __forceinline__ V ld_nt(const V* p) {
+    return __builtin_nontemporal_load(p);
+}
+
+// One panel: workgroup = one row tile.  The tile's row sums live in LDS.
+template <typename T>
+__global__ __launch_bounds__(TILE_BS) void k_tiles(int32_t N, int32_t tile_rows, int row_bits, int32_t ntr,
+                                                   int32_t panel, int32_t panel_cols,
+                                                   const int32_t* __restrict__ tile_ptr,
+                                                   const uint32_t* __restrict__ packed, const T* __restrict__ val,
+                                                   const T* __restrict__ x, T* __restrict__ y, T beta, int first) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* acc = reinterpret_cast<T*>(smem_raw);
+    const int tr = blockIdx.x;
+    for (int r = threadIdx.x; r < tile_rows; r += TILE_BS) acc[r] = T(0);
+    __syncthreads();
+    const int64_t t = (int64_t)panel * ntr + tr;
+    const int64_t k0 = tile_ptr[t], k1 = tile_ptr[t + 1];
+    const T* __restrict__ xp = x + (int64_t)panel * panel_cols;
+    const uint32_t rmask = (1u << row_bits) - 1u;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += (int64_t)TILE_U * TILE_BS) {
+        uint32_t pk[TILE_U];
+        T v[TILE_U];
+#pragma unroll
+        for (int u = 0; u < TILE_U; u++) {
+            const int64_t kk = k + (int64_t)u * TILE_BS;
+            pk[u] = kk < k1 ? ld_nt(packed + kk) : 0u;
+            v[u] = kk < k1 ? ld_nt(val + kk) : T(0);
+        }
+        T xv[TILE_U];
+#pragma unroll
+        for (int u = 0; u < TILE_U; u++) xv[u] = (k + (int64_t)u * TILE_BS < k1) ? xp[pk[u] >> row_bits] : T(0);
+#pragma unroll
+        for (int u = 0; u < TILE_U; u++)
+            if (k + (int64_t)u * TILE_BS < k1) atomicAdd(&acc[pk[u] & rmask], v[u] * xv[u]);
+    }
+    __syncthreads();
+    const int64_t r0 = (int64_t)tr * tile_rows;
+    for (int r = threadIdx.x; r < tile_rows; r += TILE_BS) {
+        const int64_t i = r0 + r;
+        if (i >= N) break;
+        const T a = acc[r];
+        if (first) {
+            if (beta == T(0)) y[i] = a;
+            else if (beta == T(1)) { if (a != T(0)) y[i] = y[i] + a; }
+            else y[i] = beta * y[i] + a;
+        } else if (a != T(0)) {
+            y[i] = y[i] + a;
+        }
+    }
+}
+
+// All panels in one launch: the workgroup of a row tile walks the panels in
+// order with its row sums kept in LDS throughout, and writes y once.  The
+// workgroups start together and proceed through the panels at about the
+// same pace, so the x panels in use at any time stay few and L2-resident.
+template <typename T>
+__global__ __launch_bounds__(TILE_BS) void k_tiles_fused(int32_t N, int32_t tile_rows, int row_bits, int32_t ntr,
+                                                         int32_t npanels, int32_t panel_cols,
+                                                         const int32_t* __restrict__ tile_ptr,
+                                                         const uint32_t* __restrict__ packed,
+                                                         const T* __restrict__ val, const T* __restrict__ x,
+                                                         T* __restrict__ y, T beta) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* acc = reinterpret_cast<T*>(smem_raw);
+    const int tr = blockIdx.x;
+    for (int r = threadIdx.x; r < tile_rows; r += TILE_BS) acc[r] = T(0);
+    __syncthreads();
+    const uint32_t rmask = (1u << row_bits) - 1u;
+    for (int32_t panel = 0; panel < npanels; panel++) {
+        const int64_t t = (int64_t)panel * ntr + tr;
+        const int64_t k0 = tile_ptr[t], k1 = tile_ptr[t + 1];
+        const T* __restrict__ xp = x + (int64_t)panel * panel_cols;
+        for (int64_t k = k0 + threadIdx.x; k < k1; k += (int64_t)TILE_U * TILE_BS) {
+            uint32_t pk[TILE_U];
+            T v[TILE_U];
+#pragma unroll
+            for (int u = 0; u < TILE_U; u++) {
+                const int64_t kk = k + (int64_t)u * TILE_BS;
+                pk[u] = kk < k1 ? ld_nt(packed + kk) : 0u;
+                v[u] = kk < k1 ? ld_nt(val + kk) : T(0);
+            }
+            T xv[TILE_U];
+#pragma unroll
+            for (int u = 0; u < TILE_U; u++)
+                xv[u] = (k + (int64_t)u * TILE_BS < k1) ? xp[pk[u] >> row_bits] : T(0);
+#pragma unroll
+            for (int u = 0; u < TILE_U; u++)
+                if (k + (int64_t)u * TILE_BS < k1) atomicAdd(&acc[pk[u] & rmask], v[u] * xv[u]);
+        }
+    }
+    __syncthreads();
+    const int64_t r0 = (int64_t)tr * tile_rows;
+    for (int r = threadIdx.x; r < tile_rows; r += TILE_BS) {
+        const int64_t i = r0 + r;
+        if (i >= N) break;
+        const T a = acc[r];
+        if (beta == T(0)) y[i] = a;
+        else if (beta == T(1)) { if (a != T(0)) y[i] = y[i] + a; }
+        else y[i] = beta * y[i] + a;
+    }
+}
+
+// 0: one launch per panel; 1: one launch over all panels (process-wide).
+int g_tiles_mode = 1;
+
+static unsigned gs(int64_t n) {
+    unsigned g = grid_for(n, 256);
+    return g > 65536 ? 65536 : g;
+}
+
+template <typename T>
+static int tiles_run_t(sable_ctx_t* ctx, const sable_csr_tiles_s* p, const void* x, void* y, double beta) {
+    const size_t shm = (size_t)p->tile_rows * sizeof(T);
+    if (g_tiles_mode == 1) {
+        hipLaunchKernelGGL(k_tiles_fused<T>, dim3((unsigned)p->ntr), dim3(TILE_BS), shm, ctx->stream, p->N,
+                           p->tile_rows, p->row_bits, p->ntr, p->npanels, p->panel_cols, p->tile_ptr, p->packed,
+                           (const T*)p->val, (const T*)x, (T*)y, (T)beta);
+        SABLE_LAUNCH_CHECK();
+        return SABLE_OK;
+    }
+    for (int32_t q = 0; q < p->npanels; q++) {
+        hipLaunchKernelGGL(k_tiles<T>, dim3((unsigned)p->ntr), dim3(TILE_BS), shm, ctx->stream, p->N, p->tile_rows,
+                           p->row_bits, p->ntr, q, p->panel_cols, p->tile_ptr, p->packed, (const T*)p->val,
+                           (const T*)x, (T*)y, (T)beta, q == 0 ? 1 : 0);
+    }
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+}  // namespace sable
+
+using namespace sable;
+
+extern "C" {
+
+SABLE_API int sable_csr_tiles_create(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                                     const int32_t* col, const void* val, int32_t tile_rows, int32_t panel_cols,
+                                     sable_csr_tiles_t** out) {
+    SABLE_CHECK(ctx && out && row_ptr && col && val, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(N > 0, SABLE_EINVAL, "N must be positive");
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    *out = nullptr;
+    const size_t vs = dtype == SABLE_F64 ? 8 : 4;
+    if (tile_rows <= 0) tile_rows = dtype == SABLE_F64 ? 16384 : 32768;
+    if (panel_cols <= 0) panel_cols = 8192;
+    const int rb = log2_exact(tile_rows), cb = log2_exact(panel_cols);
+    SABLE_CHECK(rb >= 6 && cb >= 6, SABLE_EINVAL, "tile_rows and panel_cols must be powers of two >= 64");
+    SABLE_CHECK(rb + cb <= 32, SABLE_EINVAL, "tile_rows * panel_cols must not exceed 2^32");
+    SABLE_CHECK((size_t)tile_rows * vs <= 160 * 1024, SABLE_EINVAL, "a tile's row sums must fit the LDS (160 KiB)");
+    hipStream_t st = ctx->stream;
+    int32_t ends[2];
+    SABLE_HIP(hipMemcpyAsync(ends, row_ptr, 4, hipMemcpyDeviceToHost, st));
+    SABLE_HIP(hipMemcpyAsync(ends + 1, row_ptr + N, 4, hipMemcpyDeviceToHost, st));
+    SABLE_HIP(hipStreamSynchronize(st));
+    SABLE_CHECK(ends[0] == 0 && ends[1] >= 0, SABLE_EMALFORMED, "row_ptr must start at 0 and end at nnz >= 0");
+    const int64_t nnz = ends[1];
+    auto* p = new sable_csr_tiles_s();
+    p->ctx = ctx;
+    p->dtype = dtype;
+    p->N = N;
+    p->nnz = nnz;
+    p->tile_rows = tile_rows;
+    p->panel_cols = panel_cols;
+    p->row_bits = rb;
+    p->col_bits = cb;
+    p->ntr = (int32_t)(((int64_t)N + tile_rows - 1) / tile_rows);
+    p->npanels = (int32_t)(((int64_t)N + panel_cols - 1) / panel_cols);
+    const int64_t ntiles = (int64_t)p->ntr * p->npanels;
+    SABLE_CHECK(ntiles < (int64_t)1 << 30, SABLE_EINVAL, "too many tiles");
+    // build buffers: rows, keys, idx and the sort's work space
+    const size_t tmp_sort = sort_tmp_bytes(nnz);
+    const size_t need = Carver::round(4 * (size_t)nnz) + Carver::round(8 * (size_t)nnz) +
+                        Carver::round(4 * (size_t)nnz) + Carver::round(tmp_sort) + 256;
+    void* work = nullptr;
+    bool ok = hipMalloc(&work, need) == hipSuccess &&
+              hipMalloc(&p->tile_ptr, 4 * (size_t)(ntiles + 1)) == hipSuccess &&
+              hipMalloc(&p->packed, 4 * (size_t)(nnz ? nnz : 1)) == hipSuccess &&
+              hipMalloc(&p->val, vs * (size_t)(nnz ? nnz : 1)) == hipSuccess;
+    auto cleanup = [&](int code) {
+        if (work) (void)hipFree(work);
+        if (code) sable_csr_tiles_destroy(p);
+        else *out = p;
+        return code;
+    };
+    if (!ok) {
+        (void)hipGetLastError();
+        return cleanup(fail(SABLE_ENOMEM, "tile plan allocation failed (%zu bytes of work space)", need));
+    }
+    p->bytes = 4 * (size_t)(ntiles + 1) + (4 + vs) * (size_t)nnz;
+    Carver cv(work);
+    int32_t* rows = cv.take<int32_t>((size_t)nnz);
+    uint64_t* keys = cv.take<uint64_t>((size_t)nnz);
+    uint32_t* idx = cv.take<uint32_t>((size_t)nnz);
+    void* stmp = cv.take<char>(tmp_sort);
+    int* err = cv.take<int>(64);
+    SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_tile_rows, dim3(gs(N)), dim3(256), 0, st, N, row_ptr, rows);
+    if (nnz > 0)
+        hipLaunchKernelGGL(k_tile_keys, dim3(gs(nnz)), dim3(256), 0, st, nnz, N, rows, col, rb, cb, p->ntr, keys,
+                           idx, err);
+    if (hipGetLastError() != hipSuccess) return cleanup(fail(SABLE_EHIP, "tile key launch failed"));
+    if (hipMemcpyAsync(ctx->pinned, err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return cleanup(fail(SABLE_EHIP, "tile key check failed"));
+    if (*(int*)ctx->pinned) return cleanup(fail(SABLE_EMALFORMED, "column index outside [0, N)"));
+    if (nnz > 0) {
+        const int rc = sort_pairs_u64(st, keys, idx, nnz, bits_for(ntiles) + cb, stmp);
+        if (rc) return cleanup(rc);
+        if (dtype == SABLE_F64)
+            hipLaunchKernelGGL(k_tile_pack<double>, dim3(gs(nnz)), dim3(256), 0, st, nnz, keys, idx, rows,
+                               (const double*)val, rb, cb, p->packed, (double*)p->val);
+        else
+            hipLaunchKernelGGL(k_tile_pack<float>, dim3(gs(nnz)), dim3(256), 0, st, nnz, keys, idx, rows,
+                               (const float*)val, rb, cb, p->packed, (float*)p->val);
+    }
+    hipLaunchKernelGGL(k_tile_heads, dim3((unsigned)grid_for(nnz + 1, 256)), dim3(256), 0, st, nnz, ntiles, cb, keys,
+                       p->tile_ptr);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        return cleanup(fail(SABLE_EHIP, "tile plan build failed"));
+    // LDS above the default dynamic limit must be requested per kernel
+    const size_t shm = (size_t)tile_rows * vs;
+    hipError_t ea = dtype == SABLE_F64
+                        ? hipFuncSetAttribute((const void*)k_tiles<double>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm)
+                        : hipFuncSetAttribute((const void*)k_tiles<float>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (ea == hipSuccess)
+        ea = dtype == SABLE_F64 ? hipFuncSetAttribute((const void*)k_tiles_fused<double>,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm)
+                                : hipFuncSetAttribute((const void*)k_tiles_fused<float>,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (ea != hipSuccess) {
+        (void)hipGetLastError();
+        return cleanup(fail(SABLE_EINVAL, "tile of %zu LDS bytes not supported: %s", shm, hipGetErrorString(ea)));
+    }
+    return cleanup(SABLE_OK);
+}
+
+SABLE_API void sable_csr_tiles_destroy(sable_csr_tiles_t* p) {
+    if (!p) return;
+    if (p->ctx) (void)hipStreamSynchronize(p->ctx->stream);
+    if (p->tile_ptr) (void)hipFree(p->tile_ptr);
+    if (p->packed) (void)hipFree(p->packed);
+    if (p->val) (void)hipFree(p->val);
+    delete p;
+}
+
+SABLE_API int sable_csr_tiles_info(const sable_csr_tiles_t* p, int32_t* row_tiles, int32_t* npanels, int64_t* bytes) {
+    SABLE_CHECK(p, SABLE_EINVAL, "NULL plan");
+    if (row_tiles) *row_tiles = p->ntr;
+    if (npanels) *npanels = p->npanels;
+    if (bytes) *bytes = (int64_t)p->bytes;
+    return SABLE_OK;
+}
+
+SABLE_API int sable_csr_tiles_select(int mode) {
+    SABLE_CHECK(mode == 0 || mode == 1, SABLE_EINVAL, "mode must be 0 (per-panel launches) or 1 (one launch)");
+    g_tiles_mode = mode;
+    return SABLE_OK;
+}
+
+SABLE_API int sable_spmv_csr_tiles(sable_ctx_t* ctx, const sable_csr_tiles_t* p, const void* x, void* y, double beta,
+                                   int inner_max) {
+    SABLE_CHECK(ctx && p && x && y, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(inner_max >= 0, SABLE_EINVAL, "negative inner_max");
+    for (int it = 0; it < inner_max; it++) {
+        const int rc = p->dtype == SABLE_F64 ? tiles_run_t<double>(ctx, p, x, y, beta)
+                                             : tiles_run_t<float>(ctx, p, x, y, beta);
+        if (rc) return rc;
+    }
+    return SABLE_OK;
+}
+
+}  // extern "C"

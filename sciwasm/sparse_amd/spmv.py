@@ -44,6 +44,36 @@ class CsrPlan:
             pass
 
 
+class CsrTiles:
+    """Column-sorted tile plan (sable_csr_tiles_*): the matrix copied into
+    (column panel, row tile) groups sorted by column, rows summed in LDS,
+    one launch per panel."""
+
+    def __init__(self, A, tile_rows=0, panel_cols=0):
+        self.ctx = A.row.ctx
+        self.A = A
+        h = c_vp()
+        call("sable_csr_tiles_create", self.ctx.handle, dtype_code(A.dtype), A.N, A.row.ptr, A.col.ptr, A.val.ptr,
+             int(tile_rows), int(panel_cols), ctypes.byref(h))
+        self.handle = h
+
+    def info(self):
+        nt, npan, b = c_i32(), c_i32(), c_i64()
+        call("sable_csr_tiles_info", self.handle, ctypes.byref(nt), ctypes.byref(npan), ctypes.byref(b))
+        return {"row_tiles": nt.value, "npanels": npan.value, "bytes": b.value}
+
+    def run(self, x, y, beta=1.0, inner_max=1):
+        call("sable_spmv_csr_tiles", self.ctx.handle, self.handle, x.ptr, y.ptr, float(beta), int(inner_max))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.load().sable_csr_tiles_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
 class CooPlan:
     """Row analysis of a COO matrix (sable_coo_plan_*): rows checked once;
     row-sorted input keeps its row heads and row blocks, so a product is

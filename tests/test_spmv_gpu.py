@@ -210,6 +210,55 @@ def test_csr_vector_kernel_parity(ctx, lanes, rows, dtype, name, make):
         check_close(yv.array(), yref, bound(rp, col, val, x, yb, dtype))
 
 
+@pytest.fixture(params=[0, 1], ids=["perpanel", "fused"])
+def tiles_mode(request):
+    from sciwasm.sparse_amd import _lib
+    _lib.call("sable_csr_tiles_select", request.param)
+    yield request.param
+    _lib.call("sable_csr_tiles_select", 1)
+
+
+@pytest.mark.parametrize("tile_rows,panel_cols", [(64, 128), (256, 1024), (16384, 8192), (0, 0)])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("name,make", CASES)
+def test_csr_tiles_parity(ctx, tiles_mode, tile_rows, panel_cols, dtype, name, make):
+    """Column-sorted tiles: y = beta*y + A.x with beta 1, 0 and 0.5, and
+    inner_max = 2, within the CSR bound (small tiles and panels exercise
+    many tiles, partial last tiles and panels, and empty tiles)."""
+    rp, col, val = make(dtype)
+    N = len(rp) - 1
+    rng = np.random.default_rng(51)
+    x = rng.uniform(-1, 1, N).astype(dtype)
+    y0 = rng.uniform(-1, 1, N).astype(dtype)
+    A = sp.csr_from_host(N, rp, col, val)
+    pl = sp.CsrTiles(A, tile_rows, panel_cols)
+    info = pl.info()
+    trr = tile_rows or (16384 if dtype == np.float64 else 32768)
+    pcc = panel_cols or 8192
+    assert info["row_tiles"] == -(-N // trr) and info["npanels"] == -(-N // pcc)
+    xv = sp.vec_from_host(x)
+    for beta in (1.0, 0.0, 0.5):
+        yv = sp.vec_from_host(y0)
+        pl.run(xv.vec, yv.vec, beta=beta)
+        yb = (beta * y0.astype(np.float64)).astype(dtype)
+        yref = oracle.spmv_csr(N, rp, col, val, x, yb)
+        check_close(yv.array(), yref, bound(rp, col, val, x, yb, dtype) + np.finfo(dtype).eps * np.abs(y0))
+    yv = sp.vec_from_host(y0)
+    pl.run(xv.vec, yv.vec, beta=1.0, inner_max=2)
+    y1 = oracle.spmv_csr(N, rp, col, val, x, y0, 1)
+    y2 = oracle.spmv_csr(N, rp, col, val, x, y0, 2)
+    check_close(yv.array(), y2, bound(rp, col, val, x, y0, dtype) + bound(rp, col, val, x, y1, dtype))
+
+
+def test_csr_tiles_rejects_bad_shapes(ctx):
+    rp, col, val = gen.laplacian_2d(16)
+    A = sp.csr_from_host(256, rp, col, val)
+    for tr, pc in ((100, 128), (1 << 17, 1 << 16), (1 << 15, 1 << 10)):
+        with pytest.raises(SableError) as e:
+            sp.CsrTiles(A, tr, pc)
+        assert e.value.name == "SABLE_EINVAL", (tr, pc)
+
+
 @pytest.mark.parametrize("order", ["sorted", "shuffled"])
 def test_coo_plan_matches_unplanned(ctx, order):
     """The planned product (rows analysed once) equals the unplanned entry
