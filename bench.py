@@ -45,10 +45,11 @@ HBM_PEAK_GBS = 8000.0
 # Fabric/HBM bytes per product measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (one counter per run, scaled by the readings of a 1 GiB copy:
 # FETCH x2, WRITE x1) for the headline matrix (seed 20261203) and schedule;
-# profiles/csr_spmv_fp64_zipf4M_panels524288*.md hold the runs.
+# profiles/csr_spmv_fp64_zipf4M_*.md hold the runs.
 MEASURED_TRAFFIC = {
     "panels_524288_bw1024_k0": {"per_product": 8.303e9, "per_launch": 1.038e9},
     "panels_524288_bw1024_k3": {"per_product": 5.854e9, "per_launch": 0.7318e9},
+    "tiles_16384x8192_fused": {"per_product": 2.956e9, "per_launch": 2.956e9},
 }
 
 
@@ -583,7 +584,8 @@ def main():
                      "traffic_per_launch": (MEASURED_TRAFFIC[head["best"]]["per_launch"]
                                             if head["best"] in MEASURED_TRAFFIC and head["N"] == 4194304 else None),
                      "kernel": head["variants"][head["best"]].get("kernel", "k_csr_block"),
-                     "limiter": "L2 request rate of the random 8-byte x gathers (TCC busy ~90%), not HBM bytes",
+                     "limiter": "L2 request rate (TCC busy ~94%): streaming loads plus x lines missing the L1, "
+                                "not HBM bytes (fetched traffic ~1.13x the algorithmic bytes)",
                      "launches_per_step": head["variants"][head["best"]]["launches_per_step"]},
         "cpu_baseline": head["cpu"],
         "check": head["check"],
