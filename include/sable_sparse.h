@@ -144,7 +144,7 @@ SABLE_API int sable_spmv_csr_panels(sable_ctx_t* ctx, const sable_csr_panels_t* 
  * than one XCD's L2): rows are cut into tiles of tile_rows rows and columns
  * into panels of panel_cols columns (both powers of two, tile_rows *
  * panel_cols <= 2^32, tile_rows values must fit the 160 KiB LDS; 0 = 16384
- * rows (FP64) or 32768 (FP32) and 8192 columns); each (panel, tile) keeps
+ * rows and 8192 columns); each (panel, tile) keeps
  * its entries sorted by
  * column, so that consecutive lanes gather neighbouring x values, and sums
  * its rows in LDS.  The plan owns its copy of the
@@ -440,8 +440,9 @@ typedef struct {
 } sable_load_stats_t;
 /* Open-loop load: requests arrive at exponential gaps, choose a matrix
  * uniformly (solves: among the lower ones), inputs U(-1,1) generated on the
- * device from a per-request seed; latency runs from the scheduled arrival
- * to the completion of the request's batch. */
+ * device from a per-request seed and results left in HBM (nothing crosses
+ * PCIe); latency runs from the scheduled arrival to the completion of the
+ * request's batch. */
 SABLE_API int sable_server_run_load(sable_server_t* s, const sable_load_t* load, sable_load_stats_t* stats);
 SABLE_API int sable_server_info(sable_server_t* s, int64_t* hbm_used, int32_t* nmatrices);
 

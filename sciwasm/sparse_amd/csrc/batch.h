@@ -13,27 +13,31 @@ int batch_refine_launch(hipStream_t st, int num_cu, int lanes, const sable_batch
                         sable_batch_spts_item_t* spts_view, int nitems, int64_t total_rows, int iters,
                         unsigned* ticket, int* err, long long spin);
 
-// Batched level-set solve.  One level of one item: `count` rows of the item,
-// listed (natural row indices) at rows[0..count); `prefix` is the number of
-// rows of the same level in earlier items.
-struct LevelSeg {
+// Work of the server's batched solves, cut on the host into chunks of at
+// most one workgroup's rows (256 / lanes), so that no kernel searches for
+// the item a row belongs to.  rows != nullptr: the chunk's rows are
+// rows[0..count) of the item (level-ordered row lists); otherwise rows
+// first .. first + count - 1.
+struct RowChunk {
     const int32_t* rows;
-    int64_t prefix;
     int32_t item;
+    int32_t first;
     int32_t count;
+    int32_t pad;
 };
-// Level l of the batch owns segments [seg_lo[l], seg_lo[l+1]) and rows[l] rows.
+// Level l of a batch is chunks [chunk_lo[l], chunk_lo[l + 1]).
 struct LevelPlan {
-    std::vector<int32_t> seg_lo;
-    std::vector<int64_t> rows;
-    int levels() const { return (int)rows.size(); }
+    std::vector<int32_t> chunk_lo;
+    int levels() const { return chunk_lo.empty() ? 0 : (int)chunk_lo.size() - 1; }
 };
 // Every item solved level by level, one launch per level (all items' rows
 // of level l in launch l); x is written, never read before it is.
 int batch_level_launch(hipStream_t st, sable_dtype_t dtype, int lanes, const sable_batch_spts_item_t* items,
-                       const LevelSeg* d_segs, const LevelPlan& plan);
-// FP32 level-set solves with `iters` FP64 residual + update steps.
+                       const RowChunk* d_level_chunks, const LevelPlan& plan);
+// FP32 level-set solves with `iters` FP64 residual + update steps; the
+// residual and update run over d_row_chunks (natural row order).
 int batch_refine_level_launch(hipStream_t st, int lanes, const sable_batch_refine_item_t* items,
-                              sable_batch_spts_item_t* spts_view, int nitems, int64_t total_rows, int iters,
-                              const LevelSeg* d_segs, const LevelPlan& plan);
+                              sable_batch_spts_item_t* spts_view, int nitems, int iters,
+                              const RowChunk* d_level_chunks, const LevelPlan& plan, const RowChunk* d_row_chunks,
+                              int nrow_chunks);
 }  // namespace sable

@@ -386,51 +386,43 @@ static unsigned gsz(int64_t n) {
     return g > 16384 ? 16384 : g;
 }
 
-// One level of a batched level-set solve: G lanes per row; t = sum of
-// val*x over the row's off-diagonal entries, x[i] = (b[i] - t) / diag.
-// With G = 1 the per-row order is the level-set solve's t = 0 + v0*x0 + ...
+// One level of a batched level-set solve, one workgroup per chunk of the
+// level's rows: G lanes per row; t = sum of val*x over the row's
+// off-diagonal entries, x[i] = (b[i] - t) / diag.  With G = 1 the per-row
+// order is the level-set solve's t = 0 + v0*x0 + ...
 template <typename T, int G>
 __global__ __launch_bounds__(BB) void k_batch_level(const sable_batch_spts_item_t* __restrict__ items,
-                                                    const LevelSeg* __restrict__ segs, int nseg, int64_t total) {
-    const int64_t gid = ((int64_t)blockIdx.x * BB + threadIdx.x) / G;
+                                                    const RowChunk* __restrict__ chunks) {
+    const RowChunk c = chunks[blockIdx.x];
+    const int g = threadIdx.x / G;
     const int l = threadIdx.x & (G - 1);
-    const bool valid = gid < total;
+    const bool valid = g < c.count;
+    const sable_batch_spts_item_t& it = items[c.item];
     T s = T(0);
     int32_t i = 0, e = 0;
-    const sable_batch_spts_item_t* it = nullptr;
     if (valid) {
-        int lo = 0, hi = nseg - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (segs[mid].prefix <= gid) lo = mid;
-            else hi = mid - 1;
-        }
-        const LevelSeg sg = segs[lo];
-        it = items + sg.item;
-        i = sg.rows[gid - sg.prefix];
-        const int32_t* __restrict__ rp = it->row_ptr;
-        const int32_t* __restrict__ col = it->col;
-        const T* __restrict__ val = (const T*)it->val;
-        const T* __restrict__ x = (const T*)it->x;
+        i = c.rows ? c.rows[g] : c.first + g;
+        const int32_t* __restrict__ rp = it.row_ptr;
+        const int32_t* __restrict__ col = it.col;
+        const T* __restrict__ val = (const T*)it.val;
+        const T* __restrict__ x = (const T*)it.x;
         e = rp[i + 1] - 1;
         for (int32_t j = rp[i] + l; j < e; j += G) s = add_rn(s, mul_rn(val[j], x[col[j]]));
     }
     if (G > 1) s = group_sum(s, G);
     if (valid && l == 0) {
-        const T d = ((const T*)it->val)[e];
-        ((T*)it->x)[i] = div_rn(sub_rn(((const T*)it->b)[i], s), d);
+        const T d = ((const T*)it.val)[e];
+        ((T*)it.x)[i] = div_rn(sub_rn(((const T*)it.b)[i], s), d);
     }
 }
 
 int batch_level_launch(hipStream_t st, sable_dtype_t dtype, int lanes, const sable_batch_spts_item_t* items,
-                       const LevelSeg* d_segs, const LevelPlan& plan) {
+                       const RowChunk* d_level_chunks, const LevelPlan& plan) {
     for (int lv = 0; lv < plan.levels(); lv++) {
-        const int64_t rows = plan.rows[lv];
-        if (rows <= 0) continue;
-        const LevelSeg* sg = d_segs + plan.seg_lo[lv];
-        const int nseg = plan.seg_lo[lv + 1] - plan.seg_lo[lv];
-        const unsigned grid = (unsigned)((rows * lanes + BB - 1) / BB);
-#define LV_LAUNCH(T, G) hipLaunchKernelGGL((k_batch_level<T, G>), dim3(grid), dim3(BB), 0, st, items, sg, nseg, rows)
+        const int nch = plan.chunk_lo[lv + 1] - plan.chunk_lo[lv];
+        if (nch <= 0) continue;
+        const RowChunk* ch = d_level_chunks + plan.chunk_lo[lv];
+#define LV_LAUNCH(T, G) hipLaunchKernelGGL((k_batch_level<T, G>), dim3((unsigned)nch), dim3(BB), 0, st, items, ch)
 #define LV_SWITCH(T)                        \
         switch (lanes) {                    \
             case 1: LV_LAUNCH(T, 1); break;  \
@@ -449,6 +441,33 @@ int batch_level_launch(hipStream_t st, sable_dtype_t dtype, int lanes, const sab
     }
     SABLE_LAUNCH_CHECK();
     return SABLE_OK;
+}
+
+// Chunked residual r32 = b64 - L64 x64 (first pass: r32 = b64) and update
+// x64 += d32 (first pass: x64 = d32), one workgroup per chunk of rows.
+template <int G>
+__global__ __launch_bounds__(BB) void k_chunk_residual(const sable_batch_refine_item_t* __restrict__ items,
+                                                       const RowChunk* __restrict__ chunks, int first) {
+    const RowChunk c = chunks[blockIdx.x];
+    const int g = threadIdx.x / G;
+    const int l = threadIdx.x & (G - 1);
+    const bool valid = g < c.count;
+    const sable_batch_refine_item_t& it = items[c.item];
+    const int32_t i = c.first + g;
+    double s = 0.0;
+    if (valid && !first)
+        for (int32_t j = it.row_ptr[i] + l; j < it.row_ptr[i + 1]; j += G) s = fma(it.val64[j], it.x64[it.col[j]], s);
+    if (G > 1) s = group_sum(s, G);
+    if (valid && l == 0) it.r32[i] = (float)(it.b64[i] - s);
+}
+
+__global__ __launch_bounds__(BB) void k_chunk_update(const sable_batch_refine_item_t* __restrict__ items,
+                                                     const RowChunk* __restrict__ chunks, int first) {
+    const RowChunk c = chunks[blockIdx.x];
+    if ((int)threadIdx.x >= c.count) return;
+    const sable_batch_refine_item_t& it = items[c.item];
+    const int32_t i = c.first + threadIdx.x;
+    it.x64[i] = first ? (double)it.d32[i] : it.x64[i] + (double)it.d32[i];
 }
 
 int batch_spts_launch(hipStream_t st, int num_cu, sable_dtype_t dtype, int lanes, const sable_batch_spts_item_t* items,
@@ -532,23 +551,24 @@ int batch_refine_launch(hipStream_t st, int num_cu, int lanes, const sable_batch
 }
 
 int batch_refine_level_launch(hipStream_t st, int lanes, const sable_batch_refine_item_t* items,
-                              sable_batch_spts_item_t* spts_view, int nitems, int64_t total_rows, int iters,
-                              const LevelSeg* d_segs, const LevelPlan& plan) {
-    if (total_rows <= 0) return SABLE_OK;
+                              sable_batch_spts_item_t* spts_view, int nitems, int iters,
+                              const RowChunk* d_level_chunks, const LevelPlan& plan, const RowChunk* d_row_chunks,
+                              int nrow_chunks) {
+    if (nitems <= 0 || nrow_chunks <= 0) return SABLE_OK;
     hipLaunchKernelGGL(k_refine_to_spts, dim3(1), dim3(BB), 0, st, items, spts_view, nitems);
-    const unsigned rgrid = (unsigned)((total_rows * lanes + BB - 1) / BB);
     for (int it = 0; it <= iters; it++) {
         const int first = it == 0;
+        const dim3 g((unsigned)nrow_chunks);
         switch (lanes) {
-            case 1: hipLaunchKernelGGL((k_batch_residual<1>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
-            case 4: hipLaunchKernelGGL((k_batch_residual<4>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
-            case 8: hipLaunchKernelGGL((k_batch_residual<8>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
-            case 16: hipLaunchKernelGGL((k_batch_residual<16>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
-            default: hipLaunchKernelGGL((k_batch_residual<64>), dim3(rgrid), dim3(BB), 0, st, items, nitems, total_rows, first); break;
+            case 1: hipLaunchKernelGGL((k_chunk_residual<1>), g, dim3(BB), 0, st, items, d_row_chunks, first); break;
+            case 4: hipLaunchKernelGGL((k_chunk_residual<4>), g, dim3(BB), 0, st, items, d_row_chunks, first); break;
+            case 8: hipLaunchKernelGGL((k_chunk_residual<8>), g, dim3(BB), 0, st, items, d_row_chunks, first); break;
+            case 16: hipLaunchKernelGGL((k_chunk_residual<16>), g, dim3(BB), 0, st, items, d_row_chunks, first); break;
+            default: hipLaunchKernelGGL((k_chunk_residual<64>), g, dim3(BB), 0, st, items, d_row_chunks, first); break;
         }
-        int rc = batch_level_launch(st, SABLE_F32, lanes, spts_view, d_segs, plan);
+        int rc = batch_level_launch(st, SABLE_F32, lanes, spts_view, d_level_chunks, plan);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_batch_update, dim3(gsz(total_rows)), dim3(BB), 0, st, items, nitems, total_rows, first);
+        hipLaunchKernelGGL(k_chunk_update, g, dim3(BB), 0, st, items, d_row_chunks, first);
     }
     SABLE_LAUNCH_CHECK();
     return SABLE_OK;

@@ -54,6 +54,8 @@ int csr_plan_view(const sable_csr_plan_t* p, const int32_t** blk, int64_t* nb, i
 int64_t plan_nblocks(int64_t nnz, int32_t N, int T, int C);
 int plan_lds_cap(int T);
 int plan_fill_stream(hipStream_t st, int32_t N, const int32_t* rp, int T, int C, int64_t B, int32_t* blk, int* err);
+// y = A.x + beta*y with a column-sorted tile plan, on stream st.
+int tiles_launch(hipStream_t st, const sable_csr_tiles_t* p, const void* x, void* y, double beta);
 int syncfree_launch(hipStream_t st, int num_cu, sable_dtype_t dt, int lanes, int32_t N, const int32_t* rp,
                     const int32_t* col, const void* val, const void* b, void* x, unsigned* ticket, int* err,
                     long long spin);

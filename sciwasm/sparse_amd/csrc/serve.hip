@@ -63,7 +63,15 @@ struct SMatrix {
     // level on the device) for the batched level-set solves
     std::vector<int32_t> lp;
     int32_t* perm = nullptr;
+    // large general matrices: column-sorted tile plans (FP64 and FP32) for
+    // their products, which run one request per launch
+    sable_csr_tiles_t* tiles64 = nullptr;
+    sable_csr_tiles_t* tiles32 = nullptr;
 };
+
+// Matrices at least this large get tile plans: their x is far larger than
+// one XCD's L2, where the row-block kernel is bound by x gathers.
+constexpr int32_t TILE_MIN_ROWS = 1 << 20;
 
 struct SRequest {
     int64_t id = 0;
@@ -97,8 +105,8 @@ struct Slot {
     char* stage_out = nullptr;
     char* h_items = nullptr;  // pinned staging for item arrays
     char* d_items = nullptr;  // device item arrays
-    char* h_segs = nullptr;   // pinned staging for the level table of the batch's solves
-    char* d_segs = nullptr;
+    char* h_chunks = nullptr;  // pinned staging for the batch's work tables
+    char* d_chunks = nullptr;
     unsigned long long* d_counter = nullptr;
     unsigned* d_ticket = nullptr;
     int* d_err = nullptr;
@@ -110,7 +118,7 @@ struct Slot {
 };
 
 constexpr size_t ITEM_BYTES = 1 << 20;  // per slot, per item kind
-constexpr size_t SEG_BYTES = 8 << 20;   // per slot: level segments of a batch's solves
+
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;
@@ -119,17 +127,23 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-// U(-1,1) inputs from (seed, index) for seeded requests.
-__global__ void k_fill_uniform(const FillItem* __restrict__ items, int nitems, int64_t total) {
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
-        int lo = 0, hi = nitems - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (items[mid].begin <= g) lo = mid;
-            else hi = mid - 1;
-        }
-        const FillItem& it = items[lo];
-        const int64_t i = g - it.begin;
+// A chunk of one fill item: elements first .. first + count - 1.
+struct FillChunk {
+    int32_t fill;
+    int32_t first;
+    int32_t count;
+    int32_t pad;
+};
+constexpr int FILL_CHUNK = 1024;
+
+// U(-1,1) inputs from (seed, index) for seeded requests, one workgroup per
+// chunk of FILL_CHUNK elements.
+__global__ __launch_bounds__(256) void k_fill_uniform(const FillItem* __restrict__ items,
+                                                      const FillChunk* __restrict__ chunks) {
+    const FillChunk c = chunks[blockIdx.x];
+    const FillItem& it = items[c.fill];
+    for (int t = threadIdx.x; t < c.count; t += 256) {
+        const int64_t i = (int64_t)c.first + t;
         const double u = (double)(mix64(it.seed * 0x100000001B3ull + (uint64_t)i) >> 11) * (1.0 / 9007199254740992.0);
         const double v = 2.0 * u - 1.0;
         if (it.is_f32) ((float*)it.ptr)[i * it.stride] = (float)v;
@@ -171,6 +185,23 @@ struct sable_server_s {
     char errmsg[256] = "";
 
     size_t vbytes_spmv() const { return cfg.mode == 2 ? 4 : 8; }
+    // Work-table buffer per slot (fill, level and row chunks of a batch):
+    // an eighth of the arena, at least 8 MiB.  Admission keeps each batch's
+    // tables within it (table_need).
+    size_t chunk_bytes() const { return std::max((size_t)8 << 20, (size_t)cfg.arena_bytes / 8); }
+    int chunk_rows() const { return 256 / (cfg.spts_lanes > 0 ? cfg.spts_lanes : 4); }
+    // Upper bound of the table bytes request r adds: its fill chunks, and
+    // for a solve one level chunk per CH rows plus one per level, and one
+    // row chunk per CH rows.
+    size_t table_need(const SRequest* r) const {
+        const SMatrix& m = mats[r->mat];
+        size_t b = r->seeded ? ((size_t)m.N / 1024 + 1) * 16 : 0;
+        if (r->op == 1) {
+            const size_t ch = (size_t)chunk_rows();
+            b += ((size_t)m.N / ch + m.lp.size() + (size_t)m.N / ch + 2) * sizeof(RowChunk);
+        }
+        return b + 64;
+    }
     void scheduler();
     void completer();
     int launch_batch(Slot& s, std::vector<SRequest*>& batch);
@@ -187,7 +218,7 @@ static int width_for(size_t k) { return k == 0 ? 0 : k <= 1 ? 1 : k <= 2 ? 2 : k
 size_t sable_server_s::need_bytes(const SRequest* r, int before) const {
     const size_t n = (size_t)mats[r->mat].N;
     if (r->op == 1) return 2 * al(n * 8) + (cfg.mode == 2 ? 2 * al(n * 4) : 0);
-    const int g = group_max();
+    const int g = mats[r->mat].tiles64 ? 1 : group_max();
     const int k = before % g;  // requests already in the open group
     auto cost = [&](int w) { return w ? 2 * al(n * (size_t)w * vbytes_spmv()) : (size_t)0; };
     return cost(width_for((size_t)k + 1)) - cost(width_for((size_t)k));
@@ -221,11 +252,13 @@ void sable_server_s::scheduler() {
             size_t used_b = 0;
             const int maxb = cfg.mode == 0 ? 1 : cfg.max_batch;
             std::unordered_map<int32_t, int> per_mat;  // SpMV requests per matrix in this batch
+            size_t tables = 768;  // alignment slack of the three tables
             while (!pending.empty() && (int)batch.size() < maxb) {
                 SRequest* r = pending.front();
                 int& before = per_mat[r->op == 0 ? r->mat : -1 - r->mat];
                 const size_t nb = need_bytes(r, r->op == 0 ? before : 0);
-                if (need_bytes(r, 0) > cap) {  // can never fit: reject
+                const size_t tb = table_need(r);
+                if (need_bytes(r, 0) > cap || table_need(r) + 768 > chunk_bytes()) {  // can never fit: reject
                     pending.pop_front();
                     r->status = SABLE_EBUDGET;
                     r->done = true;
@@ -234,8 +267,9 @@ void sable_server_s::scheduler() {
                     cv_done.notify_all();
                     continue;
                 }
-                if (used_b + nb > cap) break;
+                if (used_b + nb > cap || tables + tb > chunk_bytes()) break;
                 used_b += nb;
+                tables += tb;
                 if (r->op == 0) before++;
                 batch.push_back(r);
                 pending.pop_front();
@@ -285,7 +319,8 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
         for (auto* r : batch) {
             if (r->op != 0) continue;
             auto it = open.find(r->mat);
-            if (it == open.end() || groups[it->second].rs.size() == (size_t)gmax) {
+            const size_t cap = mats[r->mat].tiles64 ? 1 : (size_t)gmax;
+            if (it == open.end() || groups[it->second].rs.size() == cap) {
                 open[r->mat] = groups.size();
                 groups.push_back(SpmvGroup{r->mat, 1, {}, 0, 0});
                 it = open.find(r->mat);
@@ -368,6 +403,7 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
     int64_t blk_total[4] = {0, 0, 0, 0};
     for (auto& g : groups) {
         const SMatrix& m = mats[g.mat];
+        if (m.tiles64) continue;  // launched on its own below
         const int wi = g.w == 1 ? 0 : g.w == 2 ? 1 : g.w == 4 ? 2 : 3;
         sable_batch_spmv_item_t it;
         it.row_ptr = m.rp;
@@ -434,55 +470,67 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
     if (fb) SABLE_HIP(hipMemcpyAsync(s.d_items + fill_off, hi + fill_off, fb, hipMemcpyHostToDevice, st));
     if (sb) SABLE_HIP(hipMemcpyAsync(s.d_items + spmv_off, hi + spmv_off, sb, hipMemcpyHostToDevice, st));
     if (tb) SABLE_HIP(hipMemcpyAsync(s.d_items + spts_off, hi + spts_off, tb, hipMemcpyHostToDevice, st));
-    if (!fills.empty()) {
-        unsigned g = grid_for(fill_total, 256);
-        if (g > 16384) g = 16384;
-        hipLaunchKernelGGL(k_fill_uniform, dim3(g), dim3(256), 0, st, (const FillItem*)(s.d_items + fill_off),
-                           (int)fills.size(), fill_total);
-        SABLE_LAUNCH_CHECK();
-    }
     const int lanes = cfg.spts_lanes > 0 ? cfg.spts_lanes : 4;
     const long long spin = 1ll << 24;
-    // level table of the batch's solves: level l lists, item by item, the
-    // rows of level l of every item that has one
+    // Work tables of the batch, cut into chunks on the host (layout in the
+    // slot's chunk buffer: fill chunks | level chunks | row chunks):
+    //  * fill chunks of FILL_CHUNK elements of each seeded input;
+    //  * level chunks: level l lists, item by item, the rows of level l of
+    //    every solve that has one, CH rows (one workgroup) per chunk;
+    //  * row chunks: every solve's rows in natural order, CH per chunk.
+    const int CH = 256 / lanes;
+    std::vector<FillChunk> fch;
+    for (size_t f = 0; f < fills.size(); f++)
+        for (int32_t a = 0; a < fills[f].n; a += FILL_CHUNK)
+            fch.push_back(FillChunk{(int32_t)f, a, std::min(FILL_CHUNK, fills[f].n - a), 0});
     LevelPlan plan;
-    std::vector<LevelSeg> segs;
+    std::vector<RowChunk> lch, rch;
     if (cfg.mode != 0) {
         std::vector<const SRequest*> solves;
         for (auto* r : batch)
             if (r->op == 1) solves.push_back(r);
         int top = 0;
         for (auto* r : solves) top = std::max(top, (int)mats[r->mat].lp.size() - 1);
-        plan.seg_lo.assign((size_t)top + 1, 0);
-        plan.rows.assign((size_t)top, 0);
+        plan.chunk_lo.assign((size_t)top + 1, 0);
         for (int lv = 0; lv < top; lv++) {
-            plan.seg_lo[lv] = (int32_t)segs.size();
-            int64_t pre = 0;
+            plan.chunk_lo[lv] = (int32_t)lch.size();
             for (size_t k = 0; k < solves.size(); k++) {
                 const SMatrix& m = mats[solves[k]->mat];
                 if (lv + 1 >= (int)m.lp.size()) continue;
-                const int32_t cnt = m.lp[lv + 1] - m.lp[lv];
-                if (cnt <= 0) continue;
-                segs.push_back(LevelSeg{m.perm + m.lp[lv], pre, (int32_t)k, cnt});
-                pre += cnt;
+                for (int32_t a = m.lp[lv]; a < m.lp[lv + 1]; a += CH)
+                    lch.push_back(RowChunk{m.perm + a, (int32_t)k, 0, std::min(CH, m.lp[lv + 1] - a), 0});
             }
-            plan.rows[lv] = pre;
         }
-        plan.seg_lo[top] = (int32_t)segs.size();
-        const size_t gb = segs.size() * sizeof(LevelSeg);
-        SABLE_CHECK(gb <= SEG_BYTES, SABLE_EINVAL, "too many solve levels in a batch");
-        if (gb) {
-            memcpy(s.h_segs, segs.data(), gb);
-            SABLE_HIP(hipMemcpyAsync(s.d_segs, s.h_segs, gb, hipMemcpyHostToDevice, st));
+        plan.chunk_lo[top] = (int32_t)lch.size();
+        for (size_t k = 0; k < solves.size(); k++) {
+            const int32_t n = mats[solves[k]->mat].N;
+            for (int32_t a = 0; a < n; a += CH) rch.push_back(RowChunk{nullptr, (int32_t)k, a, std::min(CH, n - a), 0});
         }
     }
+    const size_t fcb = fch.size() * sizeof(FillChunk), lcb = lch.size() * sizeof(RowChunk),
+                 rcb = rch.size() * sizeof(RowChunk);
+    const size_t lc_off = (fcb + 255) & ~(size_t)255, rc_off = lc_off + ((lcb + 255) & ~(size_t)255);
+    SABLE_CHECK(rc_off + rcb <= chunk_bytes(), SABLE_EINVAL, "batch work tables exceed the slot's chunk buffer");
+    if (fcb) memcpy(s.h_chunks, fch.data(), fcb);
+    if (lcb) memcpy(s.h_chunks + lc_off, lch.data(), lcb);
+    if (rcb) memcpy(s.h_chunks + rc_off, rch.data(), rcb);
+    if (rc_off + rcb) SABLE_HIP(hipMemcpyAsync(s.d_chunks, s.h_chunks, rc_off + rcb, hipMemcpyHostToDevice, st));
+    if (!fch.empty()) {
+        hipLaunchKernelGGL(k_fill_uniform, dim3((unsigned)fch.size()), dim3(256), 0, st,
+                           (const FillItem*)(s.d_items + fill_off), (const FillChunk*)s.d_chunks);
+        SABLE_LAUNCH_CHECK();
+    }
+    const RowChunk* d_lch = (const RowChunk*)(s.d_chunks + lc_off);
+    const RowChunk* d_rch = (const RowChunk*)(s.d_chunks + rc_off);
     SABLE_HIP(hipMemsetAsync(s.d_err, 0, sizeof(int), st));
     int rc = SABLE_OK;
     if (cfg.mode == 0) {
         // plain: exactly one request, its own single-matrix launch
         SRequest* r = batch[0];
         const SMatrix& m = mats[r->mat];
-        if (r->op == 0)
+        if (r->op == 0 && m.tiles64)
+            rc = tiles_launch(st, m.tiles64, s.arena + r->in_off, s.arena + r->out_off, 0.0);
+        else if (r->op == 0)
             rc = csr_launch(st, SABLE_F64, m.blk, m.nblocks, m.rp, m.col, m.val64, s.arena + r->in_off,
                             s.arena + r->out_off, 0.0, m.lds_cap);
         else
@@ -490,6 +538,11 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
                                  s.arena + r->out_off, s.d_ticket, s.d_err, spin);
     } else {
         static const int widths[4] = {1, 2, 4, 8};
+        for (auto& g : groups) {
+            const SMatrix& m = mats[g.mat];
+            if (!m.tiles64 || rc) continue;
+            rc = tiles_launch(st, mixed ? m.tiles32 : m.tiles64, s.arena + g.x_off, s.arena + g.y_off, 0.0);
+        }
         for (int wi = 0; wi < 4 && !rc; wi++)
             if (!sv[wi].empty())
                 rc = batch_spmv_launch(st, ctx->num_cu, mixed ? SABLE_F32 : SABLE_F64, widths[wi],
@@ -497,14 +550,19 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
                                        blk_total[wi], s.d_counter, 0);
         if (!rc && !tv.empty())
             rc = batch_level_launch(st, SABLE_F64, lanes, (const sable_batch_spts_item_t*)(s.d_items + spts_off),
-                                    (const LevelSeg*)s.d_segs, plan);
+                                    d_lch, plan);
         if (!rc && !rv.empty())
             rc = batch_refine_level_launch(st, lanes, (const sable_batch_refine_item_t*)(s.d_items + spts_off),
-                                           (sable_batch_spts_item_t*)(s.d_items + view_off), (int)rv.size(), row_total,
-                                           cfg.refine_iters, (const LevelSeg*)s.d_segs, plan);
+                                           (sable_batch_spts_item_t*)(s.d_items + view_off), (int)rv.size(),
+                                           cfg.refine_iters, d_lch, plan, d_rch, (int)rch.size());
     }
-    SABLE_HIP(hipMemcpyAsync(s.stage_out + s.out_begin, s.arena + s.out_begin, s.out_end - s.out_begin,
-                             hipMemcpyDeviceToHost, st));
+    // results go back to the host only for requests that brought a host
+    // output buffer (requests generated on the device keep theirs in HBM)
+    bool any_out = false;
+    for (auto* r : batch) any_out = any_out || r->out != nullptr;
+    if (any_out)
+        SABLE_HIP(hipMemcpyAsync(s.stage_out + s.out_begin, s.arena + s.out_begin, s.out_end - s.out_begin,
+                                 hipMemcpyDeviceToHost, st));
     SABLE_HIP(hipMemcpyAsync(s.h_err, s.d_err, sizeof(int), hipMemcpyDeviceToHost, st));
     SABLE_HIP(hipEventRecord(s.ev, st));
     return rc;
@@ -562,6 +620,8 @@ void sable_server_s::completer() {
 
 static void free_matrix(SMatrix& m) {
     if (m.plan) sable_csr_plan_destroy(m.plan);
+    if (m.tiles64) sable_csr_tiles_destroy(m.tiles64);
+    if (m.tiles32) sable_csr_tiles_destroy(m.tiles32);
     void* ps[] = {m.rp, m.col, m.val64, m.val32, m.perm};
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -574,8 +634,8 @@ static void free_server(sable_server_s* s) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         if (sl.arena) (void)hipFree(sl.arena);
         if (sl.d_items) (void)hipFree(sl.d_items);
-        if (sl.d_segs) (void)hipFree(sl.d_segs);
-        if (sl.h_segs) (void)hipHostFree(sl.h_segs);
+        if (sl.d_chunks) (void)hipFree(sl.d_chunks);
+        if (sl.h_chunks) (void)hipHostFree(sl.h_chunks);
         if (sl.d_counter) (void)hipFree(sl.d_counter);
         if (sl.stage_in) (void)hipHostFree(sl.stage_in);
         if (sl.stage_out) (void)hipHostFree(sl.stage_out);
@@ -603,7 +663,7 @@ SABLE_API int sable_server_create(sable_ctx_t* ctx, const sable_server_config_t*
     if (s->cfg.max_batch > 4096) s->cfg.max_batch = 4096;
     if (s->cfg.group_rhs <= 0) s->cfg.group_rhs = 8;
     if (s->cfg.group_rhs > 8) s->cfg.group_rhs = 8;
-    const int64_t slot_bytes = s->cfg.arena_bytes + 4 * (int64_t)ITEM_BYTES + (int64_t)SEG_BYTES + 4096;
+    const int64_t slot_bytes = s->cfg.arena_bytes + 4 * (int64_t)ITEM_BYTES + (int64_t)s->chunk_bytes() + 4096;
     if (cfg->hbm_budget > 0 && slot_bytes * s->cfg.slots > cfg->hbm_budget) {
         delete s;
         return fail(SABLE_EBUDGET, "batch arenas (%lld bytes) exceed the HBM budget", (long long)(slot_bytes * cfg->slots));
@@ -619,8 +679,8 @@ SABLE_API int sable_server_create(sable_ctx_t* ctx, const sable_server_config_t*
                   hipHostMalloc(&sl.stage_in, (size_t)s->cfg.arena_bytes, hipHostMallocDefault) == hipSuccess &&
                   hipHostMalloc(&sl.stage_out, (size_t)s->cfg.arena_bytes, hipHostMallocDefault) == hipSuccess &&
                   hipHostMalloc(&sl.h_items, 4 * ITEM_BYTES, hipHostMallocDefault) == hipSuccess &&
-                  hipMalloc(&sl.d_segs, SEG_BYTES) == hipSuccess &&
-                  hipHostMalloc(&sl.h_segs, SEG_BYTES, hipHostMallocDefault) == hipSuccess &&
+                  hipMalloc(&sl.d_chunks, s->chunk_bytes()) == hipSuccess &&
+                  hipHostMalloc(&sl.h_chunks, s->chunk_bytes(), hipHostMallocDefault) == hipSuccess &&
                   hipHostMalloc(&sl.h_err, 256, hipHostMallocDefault) == hipSuccess;
         if (!ok) {
             (void)hipGetLastError();
@@ -722,6 +782,23 @@ SABLE_API int sable_server_add_matrix(sable_server_t* s, int32_t N, const int32_
         if (nnz) hipLaunchKernelGGL(k_f64_to_f32, dim3(g), dim3(256), 0, st, nnz, m.val64, m.val32);
         rc = sable_csr_plan_create(s->ctx, N, m.rp, nullptr, &m.plan);
         if (!rc) csr_plan_view(m.plan, &m.blk, &m.nblocks, &m.lds_cap);
+        if (!rc && kind == 0 && N >= TILE_MIN_ROWS) {
+            rc = sable_csr_tiles_create(s->ctx, SABLE_F64, N, m.rp, m.col, m.val64, 0, 0, &m.tiles64);
+            if (!rc) rc = sable_csr_tiles_create(s->ctx, SABLE_F32, N, m.rp, m.col, m.val32, 0, 0, &m.tiles32);
+            if (!rc) {
+                int64_t b64 = 0, b32 = 0;
+                sable_csr_tiles_info(m.tiles64, nullptr, nullptr, &b64);
+                sable_csr_tiles_info(m.tiles32, nullptr, nullptr, &b32);
+                const size_t extra = (size_t)(b64 + b32);
+                std::lock_guard<std::mutex> lk(s->mu);
+                if (s->cfg.hbm_budget > 0 && s->used + (int64_t)extra > s->cfg.hbm_budget)
+                    rc = fail(SABLE_EBUDGET, "tile plans of %zu bytes exceed the HBM budget", extra);
+                else {
+                    s->used += (int64_t)extra;
+                    m.bytes += extra;
+                }
+            }
+        }
     }
     if (rc) {
         free_matrix(m);

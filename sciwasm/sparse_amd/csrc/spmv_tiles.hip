@@ -216,22 +216,26 @@ static unsigned gs(int64_t n) {
 }
 
 template <typename T>
-static int tiles_run_t(sable_ctx_t* ctx, const sable_csr_tiles_s* p, const void* x, void* y, double beta) {
+static int tiles_run_t(hipStream_t st, const sable_csr_tiles_s* p, const void* x, void* y, double beta) {
     const size_t shm = (size_t)p->tile_rows * sizeof(T);
     if (g_tiles_mode == 1) {
-        hipLaunchKernelGGL(k_tiles_fused<T>, dim3((unsigned)p->ntr), dim3(TILE_BS), shm, ctx->stream, p->N,
+        hipLaunchKernelGGL(k_tiles_fused<T>, dim3((unsigned)p->ntr), dim3(TILE_BS), shm, st, p->N,
                            p->tile_rows, p->row_bits, p->ntr, p->npanels, p->panel_cols, p->tile_ptr, p->packed,
                            (const T*)p->val, (const T*)x, (T*)y, (T)beta);
         SABLE_LAUNCH_CHECK();
         return SABLE_OK;
     }
     for (int32_t q = 0; q < p->npanels; q++) {
-        hipLaunchKernelGGL(k_tiles<T>, dim3((unsigned)p->ntr), dim3(TILE_BS), shm, ctx->stream, p->N, p->tile_rows,
+        hipLaunchKernelGGL(k_tiles<T>, dim3((unsigned)p->ntr), dim3(TILE_BS), shm, st, p->N, p->tile_rows,
                            p->row_bits, p->ntr, q, p->panel_cols, p->tile_ptr, p->packed, (const T*)p->val,
                            (const T*)x, (T*)y, (T)beta, q == 0 ? 1 : 0);
     }
     SABLE_LAUNCH_CHECK();
     return SABLE_OK;
+}
+
+int tiles_launch(hipStream_t st, const sable_csr_tiles_t* p, const void* x, void* y, double beta) {
+    return p->dtype == SABLE_F64 ? tiles_run_t<double>(st, p, x, y, beta) : tiles_run_t<float>(st, p, x, y, beta);
 }
 
 }  // namespace sable
@@ -248,7 +252,7 @@ SABLE_API int sable_csr_tiles_create(sable_ctx_t* ctx, sable_dtype_t dtype, int3
     SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
     *out = nullptr;
     const size_t vs = dtype == SABLE_F64 ? 8 : 4;
-    if (tile_rows <= 0) tile_rows = dtype == SABLE_F64 ? 16384 : 32768;
+    if (tile_rows <= 0) tile_rows = 16384;
     if (panel_cols <= 0) panel_cols = 8192;
     const int rb = log2_exact(tile_rows), cb = log2_exact(panel_cols);
     SABLE_CHECK(rb >= 6 && cb >= 6, SABLE_EINVAL, "tile_rows and panel_cols must be powers of two >= 64");
@@ -371,8 +375,7 @@ SABLE_API int sable_spmv_csr_tiles(sable_ctx_t* ctx, const sable_csr_tiles_t* p,
     SABLE_CHECK(ctx && p && x && y, SABLE_EINVAL, "NULL argument");
     SABLE_CHECK(inner_max >= 0, SABLE_EINVAL, "negative inner_max");
     for (int it = 0; it < inner_max; it++) {
-        const int rc = p->dtype == SABLE_F64 ? tiles_run_t<double>(ctx, p, x, y, beta)
-                                             : tiles_run_t<float>(ctx, p, x, y, beta);
+        const int rc = tiles_launch(ctx->stream, p, x, y, beta);
         if (rc) return rc;
     }
     return SABLE_OK;

@@ -146,3 +146,36 @@ def test_grouped_requests_on_one_matrix(ctx, pool):
         out, _ = S.wait(i)
         assert np.array_equal(out, alone[k]), k
     S.close()
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched", "mixed"])
+def test_large_matrix_tiles_alone_and_in_mix(ctx, pool, mode):
+    """A general matrix of 2^20 rows gets column-sorted tile plans and its
+    products run one request per tile launch.  The tiles sum each row in
+    LDS-atomic order, so alone and in the mix the results agree within the
+    stated tolerance (not bit for bit), and each meets the CSR bound against
+    the FP64 oracle."""
+    mats, lows = pool
+    n = 1 << 20
+    rp, c, v = gen.zipf_csr(n, 1.5, 256, seed=61)
+    S = sp.Server(mode=mode, arena_bytes=256 << 20)
+    big = S.add_matrix(rp, c, v)
+    small = [S.add_matrix(r, cc, vv) for _, r, cc, vv in mats[:4]]
+    rng = np.random.default_rng(62)
+    x = rng.uniform(-1, 1, n)
+    alone, _ = S.wait(S.submit(big, "spmv", x))
+    alone = alone.copy()
+    ids = [S.submit(small[k % 4], "spmv", rng.uniform(-1, 1, mats[k % 4][0])) for k in range(20)]
+    ids += [S.submit(big, "spmv", rng.uniform(-1, 1, n)) for _ in range(3)]
+    probe = S.submit(big, "spmv", x)
+    ids += [S.submit(small[k % 4], "spmv", rng.uniform(-1, 1, mats[k % 4][0])) for k in range(20)]
+    mixed_out, _ = S.wait(probe)
+    yref, _ = oracle.spmv_csr_fp64(n, rp, c, v, x)
+    dt = np.float32 if mode == "mixed" else np.float64
+    b = spmv_bound(rp, c, v, x, dt)
+    assert np.all(np.abs(alone - yref) <= b)
+    assert np.all(np.abs(mixed_out - yref) <= b)
+    assert np.all(np.abs(mixed_out - alone) <= 2 * b)
+    for i in ids:
+        S.wait(i)
+    S.close()

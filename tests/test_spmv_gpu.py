@@ -233,7 +233,7 @@ def test_csr_tiles_parity(ctx, tiles_mode, tile_rows, panel_cols, dtype, name, m
     A = sp.csr_from_host(N, rp, col, val)
     pl = sp.CsrTiles(A, tile_rows, panel_cols)
     info = pl.info()
-    trr = tile_rows or (16384 if dtype == np.float64 else 32768)
+    trr = tile_rows or 16384
     pcc = panel_cols or 8192
     assert info["row_tiles"] == -(-N // trr) and info["npanels"] == -(-N // pcc)
     xv = sp.vec_from_host(x)
