@@ -243,6 +243,13 @@ SABLE_API int sable_csr_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, in
                             const int32_t* row_ptr, const int32_t* col, const void* val, int32_t* out_offset,
                             void* out_data);
 
+/* COO -> CSC: the counting scatter by column with a stable per-column sort
+ * by row (duplicates kept in input order), i.e. coo_csr with the roles of
+ * row and column exchanged.  Replaces coo_csc (seq/src/sparse32.js:1629-1671). */
+SABLE_API int sable_coo_csc(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int64_t nnz, const int32_t* row,
+                            const int32_t* col, const void* val, int32_t* out_col_ptr, int32_t* out_row,
+                            void* out_val);
+
 /* CSR transpose (counting sort by column, rows ascending within each new
  * row; values moved bitwise).  Replaces transpose_csr
  * (seq/src/sparse_opt_32.wat:6348-6477, caller seq/src/sparse32.js:1116-1120). */
@@ -313,6 +320,23 @@ SABLE_API int sable_spts_status(sable_ctx_t* ctx, sable_spts_t* a);
  * reference has no mixed-precision solve). */
 SABLE_API int sable_spts_solve_refined(sable_ctx_t* ctx, sable_spts_t* a, const double* b, double* x, int iters,
                                        int inner);
+
+/* Lower L given in CSC with the diagonal first in each column (the serial
+ * tree's layout, seq/src/sparse32.js:1629-1671 builds it): the analysis
+ * transposes it into an owned CSR (rows in ascending column order,
+ * diagonal last) and analyses that, so every solve entry point above
+ * applies.  SABLE_EMALFORMED when a column is empty or does not start with
+ * its diagonal. */
+SABLE_API int sable_spts_csc_analyze(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* col_ptr,
+                                     const int32_t* row, const void* val, int lanes, sable_spts_t** out);
+/* The column sweep y = L^-1 b, inner_max times (y overwritten each time):
+ * per row the subtractions happen in ascending column order, then the
+ * division, so the sync-free solve at lanes = 1 on the transposed CSR
+ * performs the same operations in the same order and the result is
+ * bit-identical.  Replaces spts_csc / spts_csc_wrapper
+ * (seq/src/sparse_opt_32.wat:11-173).  Allocates and synchronises. */
+SABLE_API int sable_spts_csc(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* col_ptr,
+                             const int32_t* row, const void* val, const void* b, void* y, int inner_max);
 
 /* Reference-shaped level-set solve on an already level-ordered matrix with
  * level-ordered b and y; h_level_ptr on the host.  One launch per level.

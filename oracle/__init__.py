@@ -52,6 +52,7 @@ def lib():
             getattr(L, "oracle_spmv_ell_" + suf).argtypes = [_I, _I, _I32P, FP, FP, FP, _I]
             getattr(L, "oracle_spts_level_" + suf).argtypes = [_I, _I32P, _I32P, FP, FP, FP]
             getattr(L, "oracle_spts_syncfree_" + suf).argtypes = [_I, _I32P, _I32P, FP, FP, FP]
+            getattr(L, "oracle_spts_csc_" + suf).argtypes = [_I, _I32P, _I32P, FP, FP, FP]
             getattr(L, "oracle_fletcher_" + suf).argtypes = [_I, FP]
             getattr(L, "oracle_fletcher_" + suf).restype = ctypes.c_double
         L.oracle_num_cols.argtypes = [_I, _I32P]
@@ -249,6 +250,21 @@ def spts_syncfree(N, rowptr, col, val, b):
                                                        np.ascontiguousarray(val),
                                                        np.ascontiguousarray(b, val.dtype), y)
     return y
+
+
+def spts_csc(N, colptr, row, val, b):
+    """seq/src/sparse_opt_32.wat:11-146: lower L in CSC, diagonal first."""
+    y = np.zeros(N, val.dtype)
+    getattr(lib(), "oracle_spts_csc_" + _suf(val))(N, _i32(colptr), _i32(row), np.ascontiguousarray(val),
+                                                  np.ascontiguousarray(b, val.dtype), y)
+    return y
+
+
+def coo_csc(N, row, col, val):
+    """seq/src/sparse32.js:1629-1671 -> (colptr, row, val): the counting
+    scatter by column plus per-column stable sort by row, i.e. coo_csr with
+    the roles of row and column exchanged."""
+    return coo_csr(N, col, row, val)
 
 
 def spts_fp64(N, rowptr, col, val, b):

@@ -446,6 +446,29 @@ EXPORT void oracle_spts_syncfree_##SUF(int N, const int *rowptr, const int *col,
 DEF_SPTS(float, f32)
 DEF_SPTS(double, f64)
 
+/* Column-oriented forward substitution, seq/src/sparse_opt_32.wat:11-146
+ * (spts_csc) and :149-173 (spts_csc_wrapper): L in CSC with the diagonal
+ * first in each column; y = b; then for each column j in order:
+ * y[j] = y[j] / diag_j, and for the column's other entries (in stored order)
+ * y[row] = y[row] - val * y[j] (mul then sub).  The wat's 4-wide SIMD body
+ * performs the same per-element operations. */
+#define DEF_SPTS_CSC(T, SUF)                                                       \
+EXPORT void oracle_spts_csc_##SUF(int N, const int *colptr, const int *row,       \
+                                  const T *val, const T *b, T *y)                 \
+{                                                                                 \
+    for (int i = 0; i < N; i++) y[i] = b[i];                                      \
+    for (int j = 0; j < N; j++) {                                                 \
+        int k = colptr[j];                                                        \
+        y[j] = y[j] / val[k];                                                     \
+        for (k = k + 1; k < colptr[j + 1]; k++) {                                 \
+            T p = val[k] * y[j];                                                  \
+            y[row[k]] = y[row[k]] - p;                                            \
+        }                                                                         \
+    }                                                                             \
+}
+DEF_SPTS_CSC(float, f32)
+DEF_SPTS_CSC(double, f64)
+
 /* FP64 oracle forward substitution in long double. */
 EXPORT void oracle_spts_ld(int N, const int *rowptr, const int *col,
                            const double *val, const double *b, double *y)

@@ -7,22 +7,22 @@ the host-side object layer and binding.  Importing it does not touch the
 GPU; the first operation creates the default device context.
 """
 from ._lib import SableError, load as _load_library  # noqa: F401
-from .core import (Context, DeviceArray, sparse_COO_t, sparse_CSR_t, sparse_DIA_t, sparse_ELL_t,  # noqa: F401
-                   sparse_vec_t, coo_from_host, csr_from_host, dia_from_host, ell_from_host, create_vec,
-                   vec_from_host, clear_vec, init_x, fletcher_sum, fletcher_sum_y)
+from .core import (Context, DeviceArray, sparse_COO_t, sparse_CSC_t, sparse_CSR_t, sparse_DIA_t,  # noqa: F401
+                   sparse_ELL_t, sparse_vec_t, coo_from_host, csc_from_host, csr_from_host, dia_from_host,
+                   ell_from_host, create_vec, vec_from_host, clear_vec, init_x, fletcher_sum, fletcher_sum_y)
 from .spmv import spmv, CsrPlan, CsrPanels, CsrTiles, CooPlan  # noqa: F401
-from .convert import coo_csr, num_cols, csr_ell, num_diags, csr_dia, transpose  # noqa: F401
+from .convert import coo_csr, coo_csc, num_cols, csr_ell, num_diags, csr_dia, transpose  # noqa: F401
 from .spts import (SptsAnalysis, CSR_create_level_sets, CSR_create_level_sets_with_reorder,  # noqa: F401
-                   spts_level_csr, spts_syncfree)
+                   spts_level_csr, spts_syncfree, spts_csc)
 from .serve import Server, batch_spmv, batch_spts, batch_spts_refined, lower_dominant  # noqa: F401
 
 __all__ = [
     "Context", "DeviceArray", "SableError",
-    "sparse_COO_t", "sparse_CSR_t", "sparse_DIA_t", "sparse_ELL_t", "sparse_vec_t",
-    "coo_from_host", "csr_from_host", "dia_from_host", "ell_from_host",
+    "sparse_COO_t", "sparse_CSC_t", "sparse_CSR_t", "sparse_DIA_t", "sparse_ELL_t", "sparse_vec_t",
+    "coo_from_host", "csc_from_host", "csr_from_host", "dia_from_host", "ell_from_host",
     "create_vec", "vec_from_host", "clear_vec", "init_x", "fletcher_sum", "fletcher_sum_y",
     "spmv", "CsrPlan", "CsrPanels", "CsrTiles", "CooPlan",
-    "coo_csr", "num_cols", "csr_ell", "num_diags", "csr_dia", "transpose",
+    "coo_csr", "coo_csc", "num_cols", "csr_ell", "num_diags", "csr_dia", "transpose",
     "SptsAnalysis", "CSR_create_level_sets", "CSR_create_level_sets_with_reorder", "spts_level_csr",
-    "spts_syncfree", "Server", "batch_spmv", "batch_spts", "batch_spts_refined", "lower_dominant",
+    "spts_syncfree", "spts_csc", "Server", "batch_spmv", "batch_spts", "batch_spts_refined", "lower_dominant",
 ]

@@ -100,6 +100,7 @@ SIGNATURES = {
     "sable_spmv_dia": [c_vp, c_int, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_spmv_ell": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_coo_csr": [c_vp, c_int, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "sable_coo_csc": [c_vp, c_int, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "sable_csr_num_cols": [c_vp, c_i32, c_vp, ctypes.POINTER(c_i32)],
     "sable_csr_ell": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
     "sable_csr_num_diags": [c_vp, c_i32, c_vp, c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)],
@@ -117,6 +118,8 @@ SIGNATURES = {
     "sable_spts_solve_syncfree": [c_vp, c_vp, c_vp, c_vp],
     "sable_spts_status": [c_vp, c_vp],
     "sable_spts_solve_refined": [c_vp, c_vp, c_vp, c_vp, c_int, c_int],
+    "sable_spts_csc_analyze": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_int, c_vpp],
+    "sable_spts_csc": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_int],
     "sable_spts_level_csr": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int],
     "sable_spts_syncfree_csr": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, ctypes.c_longlong],
     "sable_csr_plan_blocks": [c_vp, c_vpp, ctypes.POINTER(c_i64)],

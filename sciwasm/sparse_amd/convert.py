@@ -14,7 +14,8 @@ import sys
 import numpy as np
 
 from ._lib import call, c_i32
-from .core import (DeviceArray, sparse_COO_t, sparse_CSR_t, sparse_DIA_t, sparse_ELL_t, dtype_code)
+from .core import (DeviceArray, sparse_COO_t, sparse_CSC_t, sparse_CSR_t, sparse_DIA_t, sparse_ELL_t,
+                   dtype_code)
 
 GUARD_SLOTS = 2 ** 27
 GUARD_RATIO = 5
@@ -29,6 +30,18 @@ def coo_csr(A):
     call("sable_coo_csr", ctx.handle, dtype_code(A.dtype), A.N, A.nnz, A.row.ptr, A.col.ptr, A.val.ptr,
          rp.ptr, col.ptr, val.ptr)
     return sparse_CSR_t(rp, col, val, A.N, A.nnz)
+
+
+def coo_csc(A):
+    """seq/src/sparse32.js:1629-1671 on the GPU (bit-identical): CSC with
+    rows ascending in each column, duplicates in input order."""
+    ctx = A.row.ctx
+    cp = DeviceArray(A.N + 1, np.int32, ctx)
+    row = DeviceArray(A.nnz, np.int32, ctx)
+    val = DeviceArray(A.nnz, A.dtype, ctx)
+    call("sable_coo_csc", ctx.handle, dtype_code(A.dtype), A.N, A.nnz, A.row.ptr, A.col.ptr, A.val.ptr,
+         cp.ptr, row.ptr, val.ptr)
+    return sparse_CSC_t(cp, row, val, A.N, A.nnz)
 
 
 def num_cols(A):

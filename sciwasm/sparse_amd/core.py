@@ -188,6 +188,19 @@ class sparse_CSR_t:
         return self._plan
 
 
+class sparse_CSC_t:
+    """seq/src/sparse32.js:110-123: col_ptr (ncols+1), row, val (nnz)."""
+
+    def __init__(self, col, row, val, ncols, nnz):
+        self.col, self.row, self.val = col, row, val
+        self.ncols, self.nnz = int(ncols), int(nnz)
+        self.N = self.ncols
+
+    @property
+    def dtype(self):
+        return self.val.dtype
+
+
 class sparse_DIA_t:
     """seq/src/sparse32.js:125-155: offset (nd), data (nd*stride)."""
 
@@ -237,6 +250,13 @@ def csr_from_host(N, row_ptr, col, val, ctx=None):
     val = np.ascontiguousarray(val)
     dtype_code(val.dtype)
     return sparse_CSR_t(DeviceArray.from_host(row_ptr, np.int32, ctx), DeviceArray.from_host(col, np.int32, ctx),
+                        DeviceArray.from_host(val, None, ctx), N, len(val))
+
+
+def csc_from_host(N, col_ptr, row, val, ctx=None):
+    val = np.ascontiguousarray(val)
+    dtype_code(val.dtype)
+    return sparse_CSC_t(DeviceArray.from_host(col_ptr, np.int32, ctx), DeviceArray.from_host(row, np.int32, ctx),
                         DeviceArray.from_host(val, None, ctx), N, len(val))
 
 

@@ -396,6 +396,13 @@ using namespace sable;
 
 extern "C" {
 
+SABLE_API int sable_coo_csc(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int64_t nnz, const int32_t* row,
+                            const int32_t* col, const void* val, int32_t* out_col_ptr, int32_t* out_row,
+                            void* out_val) {
+    // the same stable sort with the roles of row and column exchanged
+    return sable_coo_csr(ctx, dtype, N, nnz, col, row, val, out_col_ptr, out_row, out_val);
+}
+
 SABLE_API int sable_coo_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int64_t nnz, const int32_t* row,
                             const int32_t* col, const void* val, int32_t* out_row_ptr, int32_t* out_col,
                             void* out_val) {

@@ -62,6 +62,8 @@ struct sable_spts_s {
     int graph_lanes[2] = {0, 0};
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
+    // CSR copy owned by the analysis (built from a CSC input), else null
+    void* owned[3] = {nullptr, nullptr, nullptr};
 };
 
 namespace sable {
@@ -154,6 +156,15 @@ __global__ void k_check_diag(int32_t N, const int32_t* __restrict__ rp, const T*
 
 // Structure check for the raw sync-free entry: each row non-empty, the last
 // column is the row itself and the others are < row (flag 2).
+// CSC lower triangle as the column sweep reads it: every column starts with
+// its diagonal (seq/src/sparse_opt_32.wat:52-57 divides by the column's
+// first value).
+__global__ void k_check_csc_diag_first(int32_t N, const int32_t* __restrict__ cp, const int32_t* __restrict__ row,
+                                       int* err) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += (int64_t)gridDim.x * blockDim.x)
+        if (cp[j + 1] <= cp[j] || row[cp[j]] != (int32_t)j) atomicOr(err, 1);
+}
+
 __global__ void k_check_lower(int32_t N, const int32_t* __restrict__ rp, const int32_t* __restrict__ col, int* err) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t a = rp[i], e = rp[i + 1] - 1;
@@ -366,7 +377,8 @@ static void free_analysis(sable_spts_s* a) {
         if (a->graph[k]) (void)hipGraphDestroy(a->graph[k]);
     }
     void* ptrs[] = {a->d_perm, a->d_order, a->d_rp, a->d_col, a->d_val, a->d_bl, a->d_yl,
-                    a->d_val32, a->d_lval32, a->d_work, a->d_ticket, a->d_err};
+                    a->d_val32, a->d_lval32, a->d_work, a->d_ticket, a->d_err,
+                    a->owned[0], a->owned[1], a->owned[2]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete a;
@@ -672,6 +684,69 @@ SABLE_API int sable_spts_solve_refined(sable_ctx_t* ctx, sable_spts_t* a, const 
 // Reference-shaped level-set solve (par/src/sparse_opt_32.wat:11759-11984):
 // the matrix is already in level order, b and y are level-ordered and
 // h_level_ptr (nlevels+1 entries) is on the host.  One launch per level.
+SABLE_API int sable_spts_csc_analyze(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* col_ptr,
+                                     const int32_t* row, const void* val, int lanes, sable_spts_t** out) {
+    SABLE_CHECK(ctx && out && col_ptr && row && val, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(N > 0, SABLE_EINVAL, "N must be positive");
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    *out = nullptr;
+    hipStream_t st = ctx->stream;
+    int32_t ends[2];
+    SABLE_HIP(hipMemcpyAsync(ends, col_ptr, 4, hipMemcpyDeviceToHost, st));
+    SABLE_HIP(hipMemcpyAsync(ends + 1, col_ptr + N, 4, hipMemcpyDeviceToHost, st));
+    SABLE_HIP(hipStreamSynchronize(st));
+    SABLE_CHECK(ends[0] == 0 && ends[1] >= N, SABLE_EMALFORMED, "col_ptr must start at 0 and hold the N diagonals");
+    const int64_t nnz = ends[1];
+    const size_t vs = dtype == SABLE_F64 ? 8 : 4;
+    void *rp = nullptr, *cl = nullptr, *vl = nullptr;
+    int* err = nullptr;
+    auto release = [&]() {
+        void* ps[] = {rp, cl, vl, err};
+        for (void* q : ps)
+            if (q) (void)hipFree(q);
+    };
+    if (hipMalloc(&rp, 4 * ((size_t)N + 1)) != hipSuccess || hipMalloc(&cl, 4 * (size_t)nnz) != hipSuccess ||
+        hipMalloc(&vl, vs * (size_t)nnz) != hipSuccess || hipMalloc(&err, 256) != hipSuccess) {
+        (void)hipGetLastError();
+        release();
+        return fail(SABLE_ENOMEM, "CSC analysis allocation failed");
+    }
+    int rc = SABLE_OK, flag = 0;
+    if (hipMemsetAsync(err, 0, sizeof(int), st) != hipSuccess) rc = fail(SABLE_EHIP, "memset failed");
+    if (!rc) {
+        hipLaunchKernelGGL(k_check_csc_diag_first, dim3(gsz(N)), dim3(TB), 0, st, N, col_ptr, row, err);
+        rc = read_flag(ctx, err, &flag);
+        if (!rc && flag) rc = fail(SABLE_EMALFORMED, "a CSC column is empty or does not start with its diagonal");
+    }
+    // the CSR of L is the transpose of the CSR view of the CSC arrays (rows
+    // in ascending column order, diagonal last)
+    if (!rc) rc = sable_transpose_csr(ctx, dtype, N, col_ptr, row, val, (int32_t*)rp, (int32_t*)cl, vl);
+    if (!rc) rc = sable_spts_analyze(ctx, dtype, N, (const int32_t*)rp, (const int32_t*)cl, vl, lanes, out);
+    (void)hipFree(err);
+    err = nullptr;
+    if (rc) {
+        release();
+        return rc;
+    }
+    (*out)->owned[0] = rp;
+    (*out)->owned[1] = cl;
+    (*out)->owned[2] = vl;
+    return SABLE_OK;
+}
+
+SABLE_API int sable_spts_csc(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* col_ptr,
+                             const int32_t* row, const void* val, const void* b, void* y, int inner_max) {
+    SABLE_CHECK(inner_max >= 0, SABLE_EINVAL, "negative inner_max");
+    if (inner_max == 0 || N <= 0) return SABLE_OK;
+    sable_spts_t* a = nullptr;
+    int rc = sable_spts_csc_analyze(ctx, dtype, N, col_ptr, row, val, 1, &a);
+    if (rc) return rc;
+    for (int it = 0; it < inner_max && !rc; it++) rc = sable_spts_solve_syncfree(ctx, a, b, y);
+    if (!rc) rc = sable_spts_status(ctx, a);
+    sable_spts_destroy(a);
+    return rc;
+}
+
 SABLE_API int sable_spts_level_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nlevels,
                                    const int32_t* h_level_ptr, const int32_t* row_ptr, const int32_t* col,
                                    const void* val, const void* b, void* y, int lanes) {

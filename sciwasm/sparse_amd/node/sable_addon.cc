@@ -512,6 +512,61 @@ napi_value coo_csr(napi_env env, napi_callback_info info) {
     });
 }
 
+// coo_csc(row, col, val, nnz, N, out_col_ptr, out_row, out_val)
+napi_value coo_csc(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 8);
+        ensure_ctx();
+        const int64_t nnz = a.i(3);
+        const int32_t N = (int32_t)a.i(4);
+        void* dr = mirror(a.i(0), 4 * nnz);
+        void* dc = mirror(a.i(1), 4 * nnz);
+        void* dv = mirror(a.i(2), P->vs * nnz);
+        void* ocp = scratch(a.i(5), 4 * (size_t)(N + 1), false);
+        void* orow = scratch(a.i(6), 4 * nnz, false);
+        void* ov = scratch(a.i(7), P->vs * nnz, false);
+        check(sable_coo_csc(g_ctx, P->dt, N, nnz, (const int32_t*)dr, (const int32_t*)dc, dv, (int32_t*)ocp,
+                            (int32_t*)orow, ov),
+              "coo_csc");
+        drop_mirrors_in(a.i(5), 4 * (size_t)(N + 1));
+        drop_mirrors_in(a.i(6), 4 * nnz);
+        drop_mirrors_in(a.i(7), P->vs * nnz);
+        download(a.i(5), ocp, 4 * (size_t)(N + 1));
+        download(a.i(6), orow, 4 * nnz);
+        download(a.i(7), ov, P->vs * nnz);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// spts_csc(col_ptr, row, val, x, y, N[, inner_max]): y = L^-1 x, L lower in
+// CSC with the diagonal first in each column
+napi_value spts_csc_impl(napi_env env, napi_callback_info info, bool wrapper) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, wrapper ? 7 : 6);
+        ensure_ctx();
+        const int32_t N = (int32_t)a.i(5);
+        const int im = wrapper ? (int)a.i(6) : 1;
+        if (N <= 0 || im <= 0) return undef(env);
+        const int32_t* hcp = host_i32(a.i(0), N + 1);
+        const int64_t nnz = hcp[N];
+        if (hcp[0] != 0 || nnz < N) throw Err{"col_ptr must start at 0 and hold the N diagonals"};
+        void* dcp = mirror(a.i(0), 4 * (size_t)(N + 1));
+        void* drow = mirror(a.i(1), 4 * nnz);
+        void* dv = mirror(a.i(2), P->vs * nnz);
+        void* dx = scratch(a.i(3), P->vs * N, true);
+        void* dy = scratch(a.i(4), P->vs * N, false);
+        check(sable_spts_csc(g_ctx, P->dt, N, (const int32_t*)dcp, (const int32_t*)drow, dv, dx, dy, im), "spts_csc");
+        download(a.i(4), dy, P->vs * N);
+        release_scratch();
+        return undef(env);
+    });
+}
+napi_value spts_csc(napi_env env, napi_callback_info info) { return spts_csc_impl(env, info, false); }
+napi_value spts_csc_wrapper(napi_env env, napi_callback_info info) { return spts_csc_impl(env, info, true); }
+
 // num_cols(row_ptr, N) -> i32
 napi_value num_cols(napi_env env, napi_callback_info info) {
     return guarded(env, [&] {
@@ -845,7 +900,8 @@ napi_value make_serial(napi_env env, Prec* P) {
         {"transpose_csr", transpose_csr}, {"transpose_dia", transpose_dia},
         {"count_transpose_cols_ell", count_transpose_cols_ell}, {"transpose_ell", transpose_ell},
         {"memcpy", lm_memcpy}, {"coo_csr", coo_csr}, {"num_cols", num_cols}, {"csr_ell", csr_ell},
-        {"num_diags", num_diags}, {"csr_dia", csr_dia},
+        {"num_diags", num_diags}, {"csr_dia", csr_dia}, {"coo_csc", coo_csc},
+        {"spts_csc", spts_csc}, {"spts_csc_wrapper", spts_csc_wrapper},
     };
     for (auto& f : fns) add_fn(env, m, f.name, f.cb, P);
     return m;

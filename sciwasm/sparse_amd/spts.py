@@ -8,7 +8,10 @@
 * ``spts_syncfree(L, b, x)``: the sync-free solve
   (par/src/spts_opt_32.wat:8-260).
 * ``SptsAnalysis``: a device-resident analysis for repeated solves (level
-  schedule replayed from a hipGraph, sync-free, FP32 + FP64 refinement).
+  schedule replayed from a hipGraph, sync-free, FP32 + FP64 refinement),
+  of a CSR or of a CSC input.
+* ``spts_csc(A_csc, x, y, inner_max)``: the serial tree's column sweep
+  (seq/src/sparse_opt_32.wat:11-173).
 """
 import ctypes
 
@@ -16,7 +19,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import call, c_vp, c_i32
-from .core import DeviceArray, sparse_CSR_t, dtype_code
+from .core import DeviceArray, sparse_CSC_t, sparse_CSR_t, dtype_code
 
 
 class SptsAnalysis:
@@ -24,8 +27,13 @@ class SptsAnalysis:
         self.L = L
         self.ctx = L.row.ctx
         h = c_vp()
-        call("sable_spts_analyze", self.ctx.handle, dtype_code(L.dtype), L.N, L.row.ptr, L.col.ptr, L.val.ptr,
-             int(lanes), ctypes.byref(h))
+        if isinstance(L, sparse_CSC_t):
+            # the analysis keeps its own CSR copy of the CSC input
+            call("sable_spts_csc_analyze", self.ctx.handle, dtype_code(L.dtype), L.N, L.col.ptr, L.row.ptr,
+                 L.val.ptr, int(lanes), ctypes.byref(h))
+        else:
+            call("sable_spts_analyze", self.ctx.handle, dtype_code(L.dtype), L.N, L.row.ptr, L.col.ptr, L.val.ptr,
+                 int(lanes), ctypes.byref(h))
         self.handle = h
         nl, mw, ln = c_i32(), c_i32(), c_i32()
         call("sable_spts_info", h, ctypes.byref(nl), ctypes.byref(mw), ctypes.byref(ln))
@@ -105,3 +113,11 @@ def spts_syncfree(L, b_view, x_view, lanes=1, spin_limit=0):
     """Sync-free solve (natural order)."""
     call("sable_spts_syncfree_csr", L.row.ctx.handle, dtype_code(L.dtype), L.N, L.row.ptr, L.col.ptr, L.val.ptr,
          b_view.vec.ptr, x_view.vec.ptr, int(lanes), int(spin_limit))
+
+
+def spts_csc(A, x_view, y_view, inner_max=1):
+    """seq/src/sparse_opt_32.wat:11-173 (spts_csc_wrapper): y = L^-1 x for
+    a lower CSC with the diagonal first in each column, bit-identical to the
+    column sweep."""
+    call("sable_spts_csc", A.col.ctx.handle, dtype_code(A.dtype), A.N, A.col.ptr, A.row.ptr, A.val.ptr,
+         x_view.vec.ptr, y_view.vec.ptr, int(inner_max))

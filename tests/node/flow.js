@@ -17,6 +17,7 @@
 //     spmv_dia_col_wrapper / spmv_ell_col_wrapper over W row ranges
 //   spts_csr_sync_free_wrapper, spts_level_csr,
 //     spts_csr_opt_level_sync_free_wrapper
+//   coo_csc, spts_csc / spts_csc_wrapper (serial column sweep)
 'use strict';
 
 const fs = require('fs');
@@ -250,6 +251,28 @@ if (spec.spts) {
     P.spts_csr_opt_level_sync_free_wrapper(id, Lr.lp.off, Lr.row.off, Lr.col.off, Lr.val.off, br.off, y3,
                                            Lr.perm.off, S.nlevels, barrier, flag, gl, gr, al, rl, rw, wl, W, S.N, 1);
   save('spts_combined', y3, S.N, FArray);
+}
+
+// ---------------------------------------------------------------- CSC
+step('coo_csc');
+const csc = { col: alloc(N + 1, Int32Array), row: alloc(nnz, Int32Array), val: alloc(nnz, FArray) };
+E.coo_csc(coo_row.off, coo_col.off, coo_val.off, nnz, N, csc.col, csc.row, csc.val);
+save('csc_col', csc.col, N + 1, Int32Array);
+save('csc_row', csc.row, nnz, Int32Array);
+save('csc_val', csc.val, nnz, FArray);
+if (spec.spts) {
+  const S = spec.spts;
+  const Lc = { col: load_array('Lc_col', Int32Array), row: load_array('Lc_row', Int32Array),
+               val: load_array('Lc_val', FArray) };
+  const bc = load_array('L_b', FArray);
+  step('spts_csc');
+  const y4 = alloc(S.N, FArray);
+  E.spts_csc(Lc.col.off, Lc.row.off, Lc.val.off, bc.off, y4, S.N);
+  save('spts_csc', y4, S.N, FArray);
+  step('spts_csc_wrapper');
+  const y5 = alloc(S.N, FArray);
+  E.spts_csc_wrapper(Lc.col.off, Lc.row.off, Lc.val.off, bc.off, y5, S.N, inner_max);
+  save('spts_csc_wrapper', y5, S.N, FArray);
 }
 
 // ---------------------------------------------------------------- _free

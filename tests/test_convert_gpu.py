@@ -189,3 +189,14 @@ def test_convert_then_spmv_chain(ctx):
         sp.spmv(M, sp.vec_from_host(x), yv)
         ys.append(yv.array())
     assert np.array_equal(ys[0], ys[1]) and np.array_equal(ys[0], ys[2])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_coo_csc_bit_exact(ctx, dtype):
+    row, col, val = shuffled_coo(3000, 40000, 81, dtype)
+    A = sp.coo_from_host(3000, row, col, val)
+    C = sp.coo_csc(A)
+    cp, r, v = oracle.coo_csc(3000, row, col, val)
+    assert np.array_equal(C.col.numpy(), cp)
+    assert np.array_equal(C.row.numpy(), r)
+    assert np.array_equal(C.val.numpy().view(np.uint8), v.view(np.uint8))

@@ -30,7 +30,8 @@ NODE = shutil.which("node")
 
 SERIAL = ["spmv_coo", "spmv_coo_wrapper", "spmv_csr", "spmv_csr_wrapper", "spmv_dia", "spmv_dia_wrapper",
           "spmv_ell", "spmv_ell_wrapper", "transpose_csr", "transpose_dia", "count_transpose_cols_ell",
-          "transpose_ell", "memcpy", "coo_csr", "num_cols", "csr_ell", "num_diags", "csr_dia"]
+          "transpose_ell", "memcpy", "coo_csr", "num_cols", "csr_ell", "num_diags", "csr_dia", "coo_csc",
+          "spts_csc", "spts_csc_wrapper"]
 PARALLEL = ["spmv_coo_wrapper", "spmv_csr_wrapper", "spmv_dia_col_wrapper", "spmv_ell_col_wrapper", "memcpy",
             "sum", "spts_level_csr", "spts_csr_sync_free_wrapper", "spts_csr_level_sync_free_wrapper",
             "spts_csr_opt_level_sync_free_wrapper", "metadata"]
@@ -100,8 +101,12 @@ def write_inputs(d, dtype, seed=0, workers=3, inner_max=3):
     nl, lp, lperm, order, _ = oracle.level_sets(n, Lrp, Lc)
     rrp, rc, rv = oracle.level_reorder(n, Lrp, Lc, Lv, lperm, order)
     br = b[lperm]
+    # CSC of L: the transpose's CSR, rows ascending within each column, so
+    # the diagonal comes first
+    ccp, crow, cval = oracle.transpose_csr(n, Lrp, Lc, Lv)
     arrays = dict(coo_row=coo_row, coo_col=coo_col, coo_val=coo_val, x=x, L_row=Lrp, L_col=Lc, L_val=Lv, L_b=b,
-                  Lr_level_ptr=lp, Lr_perm=lperm, Lr_row=rrp, Lr_col=rc, Lr_val=rv, Lr_b=br)
+                  Lr_level_ptr=lp, Lr_perm=lperm, Lr_row=rrp, Lr_col=rc, Lr_val=rv, Lr_b=br,
+                  Lc_col=ccp, Lc_row=crow, Lc_val=cval)
     for k, v in arrays.items():
         np.ascontiguousarray(v).tofile(os.path.join(d, k + ".bin"))
     spec = dict(N=N, nnz=nnz, inner_max=inner_max, workers=workers, spts=dict(N=n, nlevels=int(nl)))
@@ -203,4 +208,13 @@ def test_node_flow_matches_oracle(tmp_path, dtype):
     yl = oracle.spts_level(n, a["Lr_row"], a["Lr_col"], a["Lr_val"], a["Lr_b"])
     assert np.array_equal(read_out(d, "spts_level", dtype), yl)
     assert np.array_equal(read_out(d, "spts_combined", dtype), yl)
+
+    # COO -> CSC and the column sweep: bit-exact
+    cp, crow, cval = oracle.coo_csc(N, a["coo_row"], a["coo_col"], a["coo_val"])
+    assert np.array_equal(read_out(d, "csc_col", i32), cp)
+    assert np.array_equal(read_out(d, "csc_row", i32), crow)
+    assert np.array_equal(read_out(d, "csc_val", dtype).view(np.uint8), cval.view(np.uint8))
+    yc = oracle.spts_csc(n, a["Lc_col"], a["Lc_row"], a["Lc_val"], a["L_b"])
+    assert np.array_equal(read_out(d, "spts_csc", dtype).view(np.uint8), yc.view(np.uint8))
+    assert np.array_equal(read_out(d, "spts_csc_wrapper", dtype).view(np.uint8), yc.view(np.uint8))
     assert out["freed"] == 1

@@ -324,3 +324,50 @@ def test_threaded_cpu_baseline_matches_serial():
     _, y1 = oracle.time_spmv_csr_f64(5000, rp, col, val, x, warm=0, reps=1)
     _, y4 = oracle.time_spmv_csr_f64_mt(5000, rp, col, val, x, 4, warm=0, reps=2)
     assert np.array_equal(y1, y4)
+
+
+def lower_csc_from_csr(N, rp, col, val):
+    """CSC of a lower CSR (diagonal last in each row) with the diagonal first
+    in each column: entries sorted by (column, row)."""
+    rows = np.repeat(np.arange(N), np.diff(rp))
+    order = np.lexsort((rows, col))
+    cp = np.zeros(N + 1, np.int32)
+    np.add.at(cp, col + 1, 1)
+    return np.cumsum(cp).astype(np.int32), rows[order].astype(np.int32), val[order]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_spts_csc_equals_row_order_and_manufactured(dtype):
+    """The column sweep performs, per row, the subtractions in ascending
+    column order and then the division: the same operations, in the same
+    order, as the sync-free row order on the CSR of the same matrix, so the
+    two must agree bit for bit; on an integer manufactured solution both are
+    exact."""
+    rp, col, val = gen.random_lower(400, kmax=12, seed=5, dtype=dtype)
+    N = len(rp) - 1
+    cp, row, cval = lower_csc_from_csr(N, rp, col, val)
+    b = np.random.default_rng(6).uniform(-1, 1, N).astype(dtype)
+    assert np.array_equal(oracle.spts_csc(N, cp, row, cval, b), oracle.spts_syncfree(N, rp, col, val, b))
+    # integer L with unit diagonal and integer x: b = L x is exact, so is x
+    rng = np.random.default_rng(7)
+    ival = rng.integers(-3, 4, len(col)).astype(dtype)
+    ival[rp[1:] - 1] = 1
+    x = rng.integers(-5, 6, N).astype(dtype)
+    bx = np.zeros(N, np.float64)
+    rows = np.repeat(np.arange(N), np.diff(rp))
+    np.add.at(bx, rows, ival.astype(np.float64) * x[col])
+    cp2, row2, cval2 = lower_csc_from_csr(N, rp, col, ival)
+    if np.max(np.abs(bx)) < 2 ** 20:
+        assert np.array_equal(oracle.spts_csc(N, cp2, row2, cval2, bx.astype(dtype)), x)
+
+
+def test_coo_csc_is_stable_col_row_sort():
+    rng = np.random.default_rng(8)
+    N, nnz = 50, 400
+    row = rng.integers(0, N, nnz).astype(np.int32)
+    col = rng.integers(0, N, nnz).astype(np.int32)
+    val = rng.standard_normal(nnz)
+    cp, r, v = oracle.coo_csc(N, row, col, val)
+    order = np.lexsort((np.arange(nnz), row, col))
+    assert np.array_equal(r, row[order]) and np.array_equal(v, val[order])
+    assert np.array_equal(cp, np.concatenate([[0], np.cumsum(np.bincount(col, minlength=N))]))

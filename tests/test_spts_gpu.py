@@ -176,3 +176,44 @@ def test_forced_spin_limit_reports_timeout(ctx):
     x = sp.create_vec(N, 0.0, np.float64)
     sp.spts_syncfree(L, b, x, lanes=1)          # the default bound solves it
     assert np.array_equal(x.array(), oracle.spts_syncfree(N, rp, c, v, np.ones(N)))
+
+
+def csc_of_lower(N, rp, col, val):
+    rows = np.repeat(np.arange(N), np.diff(rp))
+    order = np.lexsort((rows, col))
+    cp = np.concatenate([[0], np.cumsum(np.bincount(col, minlength=N))]).astype(np.int32)
+    return cp, rows[order].astype(np.int32), val[order]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("name", ["random_lower", "lap3d"])
+def test_spts_csc_bit_exact(ctx, dtype, name):
+    """The serial tree's column sweep (spts_csc_wrapper with inner_max = 2)
+    on the GPU: bit-identical to the oracle's column sweep."""
+    if name == "lap3d":
+        rp, col, val = gen.laplacian_3d_lower(12, dtype)
+    else:
+        rp, col, val = gen.random_lower(5000, kmax=16, seed=71, dtype=dtype)
+    N = len(rp) - 1
+    cp, row, cval = csc_of_lower(N, rp, col, val)
+    b = np.random.default_rng(72).uniform(-1, 1, N).astype(dtype)
+    A = sp.csc_from_host(N, cp, row, cval)
+    yv = sp.create_vec(N, 0.0, dtype)
+    sp.spts_csc(A, sp.vec_from_host(b), yv, inner_max=2)
+    yref = oracle.spts_csc(N, cp, row, cval, b)
+    assert np.array_equal(yv.array().view(np.uint8), yref.view(np.uint8))
+    # the analysis of the CSC input serves the other solves too
+    an = sp.SptsAnalysis(A, lanes=1)
+    x2 = sp.create_vec(N, 0.0, dtype)
+    an.solve_syncfree(sp.vec_from_host(b), x2)
+    assert np.array_equal(x2.array(), yref)
+
+
+def test_spts_csc_rejects_diagonal_not_first(ctx):
+    # column 0 holds rows (1, 0): its first entry is not the diagonal
+    cp = np.array([0, 2, 3], np.int32)
+    row = np.array([1, 0, 1], np.int32)
+    A = sp.csc_from_host(2, cp, row, np.array([0.5, 2.0, 4.0]))
+    with pytest.raises(SableError) as e:
+        sp.spts_csc(A, sp.vec_from_host(np.ones(2)), sp.create_vec(2, 0.0, np.float64))
+    assert e.value.name == "SABLE_EMALFORMED"
