@@ -275,6 +275,65 @@ SABLE_API int sable_count_transpose_cols_ell(sable_ctx_t* ctx, sable_dtype_t dty
 SABLE_API int sable_transpose_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc, int32_t nc_out,
                                   const void* data, const int32_t* indices, void* out_data, int32_t* out_indices);
 
+/* ------------------------------------------------------------ MatrixMarket ingest */
+/* Field of the banner (token 4 of "%%MatrixMarket matrix coordinate <field>
+ * <symmetry>"); every field but pattern carries a value per entry (complex:
+ * the real part, as the reference reads token 3 of each line). */
+typedef enum {
+    SABLE_MM_REAL = 0,
+    SABLE_MM_INTEGER = 1,
+    SABLE_MM_COMPLEX = 2,
+    SABLE_MM_PATTERN = 3,
+    SABLE_MM_OTHER = 4
+} sable_mm_field_t;
+
+/* mm_info of the reference (seq/src/sparse32.js:14-25). */
+typedef struct {
+    int64_t nrows, ncols, nentries; /* the size line */
+    int64_t nnz;        /* entries of the full COO: symmetric off-diagonals
+                         * twice, explicit zeros dropped, 0 -> nentries
+                         * (calculate_actual_nnz, read_matrix_MM_files) */
+    int64_t lower_nnz;  /* entries of the lower COO: nrows + strictly lower
+                         * nonzeros (create_LCOO_from_MM's L_nnz) */
+    int32_t field;      /* sable_mm_field_t */
+    int32_t symmetric;  /* symmetry token == "symmetric" (others read as general) */
+    int64_t body_offset; /* byte offset of the first line after the size line */
+} sable_mm_info_t;
+
+typedef struct sable_mm_s sable_mm_t;
+
+/* Header only, on the host: banner, '%' comment lines, size line.
+ * Replaces read_MM_header (seq/src/sparse32.js:1889-1911). */
+SABLE_API int sable_mm_header(const char* text, size_t len, sable_mm_info_t* info);
+
+/* Parses the whole text of a coordinate MatrixMarket file on the device.
+ * text is a host pointer (copied to HBM and released before returning) or,
+ * with text_on_device = 1, a device pointer that stays untouched.  Every
+ * entry line is checked: tokens must be decimal ("row col [value]"), indices
+ * inside the size line, and the entry count must match nentries
+ * (SABLE_EMALFORMED names the byte offset and the line).  Values are rounded
+ * to the nearest binary64 exactly as JavaScript's Number() does
+ * (synchronises).  Replaces read_matrix_MM_files + calculate_actual_nnz
+ * (seq/src/sparse32.js:1914-1981; seq/src/sparse64.js:1536-1564). */
+SABLE_API int sable_mm_parse(sable_ctx_t* ctx, const char* text, size_t len, int text_on_device, sable_mm_t** out);
+SABLE_API int sable_mm_get_info(const sable_mm_t* mm, sable_mm_info_t* info);
+/* Lines holding an entry or a blank, and values decided by the exact
+ * big-integer conversion (diagnostics). */
+SABLE_API int sable_mm_stats(const sable_mm_t* mm, int64_t* lines, int64_t* exact_values);
+
+/* The COO (0-based, sorted by (row, col)) into device arrays of info.nnz
+ * entries (lower = 0) or info.lower_nnz entries (lower = 1).  lower = 0
+ * replaces create_COO_from_MM (seq/src/sparse32.js:2100-2172): symmetric
+ * off-diagonals expanded to (r, c) and (c, r), explicit zeros dropped,
+ * pattern values 1; when every entry was zero the nentries outputs are zero.
+ * lower = 1 replaces create_LCOO_from_MM (seq/src/sparse32.js:1983-2097;
+ * par/src/spts32.js:14): strictly lower nonzeros (upper ones mirrored when
+ * symmetric) plus a unit diagonal on every row.  Equal (row, col) keys keep
+ * file order; the reference's quicksort leaves them in pivot order. */
+SABLE_API int sable_mm_coo(sable_ctx_t* ctx, const sable_mm_t* mm, sable_dtype_t dtype, int lower, int32_t* row,
+                           int32_t* col, void* val);
+SABLE_API void sable_mm_destroy(sable_mm_t* mm);
+
 /* ------------------------------------------------------------ triangular solve */
 /* L.x = b for a lower-triangular CSR whose rows each end with the diagonal
  * (the reference's layout; par/src/spts32.js:14-136 builds L = tril(A,-1)+I

@@ -72,6 +72,10 @@ def lib():
         L.oracle_time_spmv_csr_f64_mt.argtypes = [_I, _I32P, _I32P, _F64P, _F64P, _F64P, _I, _I, _I]
         L.oracle_time_spmv_csr_f64_mt.restype = ctypes.c_double
         L.oracle_static_nnz.argtypes = [_I, _I32P, _I, _I32P, _I32P]
+        for suf in ("f32", "f64"):
+            fn = getattr(L, "oracle_mm_coo_" + suf)
+            fn.argtypes = [ctypes.c_char_p, ctypes.c_size_t, _I, ctypes.POINTER(ctypes.c_int64), _VP, _VP, _VP]
+            fn.restype = _I
         _LIB = L
     return _LIB
 
@@ -309,3 +313,27 @@ def time_spmv_csr_f64_mt(nrows, rowptr, col, val, x, workers, warm=10, reps=30):
     t = lib().oracle_time_spmv_csr_f64_mt(nrows, _i32(rowptr), _i32(col), np.ascontiguousarray(val, np.float64),
                                           np.ascontiguousarray(x, np.float64), y, workers, warm, reps)
     return t, y
+
+
+def mm_coo(text, dtype=np.float32, lower=False):
+    """seq/src/sparse32.js:1889-2172 (32-bit tree) / seq/src/sparse64.js:
+    1536-1760 (64-bit): MatrixMarket text -> (info, row, col, val), with
+    create_COO_from_MM (lower=False) or create_LCOO_from_MM (lower=True).
+    info = dict(nrows, ncols, nentries, nnz, pattern, symmetric).  Raises
+    ValueError where the reference throws."""
+    if isinstance(text, str):
+        text = text.encode()
+    dtype = np.dtype(dtype)
+    suf = "f32" if dtype == np.float32 else "f64"
+    fn = getattr(lib(), "oracle_mm_coo_" + suf)
+    info6 = (ctypes.c_int64 * 6)()
+    if fn(text, len(text), int(lower), info6, None, None, None):
+        raise ValueError("the reference reader throws on this text")
+    nnz = int(info6[3])
+    row = np.zeros(max(nnz, 1), np.int32)
+    col = np.zeros(max(nnz, 1), np.int32)
+    val = np.zeros(max(nnz, 1), dtype)
+    fn(text, len(text), int(lower), info6, row.ctypes.data, col.ctypes.data, val.ctypes.data)
+    info = dict(nrows=int(info6[0]), ncols=int(info6[1]), nentries=int(info6[2]), nnz=nnz,
+                pattern=bool(info6[4]), symmetric=bool(info6[5]))
+    return info, row[:nnz], col[:nnz], val[:nnz]

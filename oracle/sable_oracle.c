@@ -624,3 +624,274 @@ EXPORT double oracle_time_spmv_csr_f64_mt(int nrows, const int *rowptr, const in
     free(rs); free(re); free(jobs); free(th);
     return reps > 0 ? t / reps : 0.0;
 }
+
+/* ================================================================ MatrixMarket
+ * seq/src/sparse32.js:1889-2172 (FP64: seq/src/sparse64.js:1536-1760), as
+ * the reference runs it: the text is cut into '\n'-terminated lines (the
+ * loader keeps an unterminated last line back, :2470-2485), every line is
+ * split on single spaces (String.split(" ")), and each token goes through
+ * JavaScript's Number() and, for the index arrays, the Int32Array store
+ * (ToInt32).  Values are held in a Float64Array, then stored to the COO's
+ * Float32Array (32-bit tree) or Float64Array (64-bit tree).  The COO is
+ * sorted by quick_sort_COO (:1577-1609) exactly as written; its arrays start
+ * zeroed, as freshly grown WebAssembly memory does. */
+
+typedef struct {
+    const char* p;
+    size_t n;
+} mm_str;
+
+/* Number(s) for the decimal literals this path meets (StringNumericLiteral
+ * without the hexadecimal/binary/octal forms): whitespace trimmed, empty ->
+ * 0, [+-]Infinity, otherwise strtod when the whole token is a decimal
+ * literal, NaN when it is not. */
+static double js_number(mm_str s) {
+    size_t a = 0, b = s.n;
+    while (a < b && (s.p[a] == ' ' || s.p[a] == '\t' || s.p[a] == '\r' || s.p[a] == '\n' || s.p[a] == '\v' ||
+                     s.p[a] == '\f'))
+        ++a;
+    while (b > a && (s.p[b - 1] == ' ' || s.p[b - 1] == '\t' || s.p[b - 1] == '\r' || s.p[b - 1] == '\n' ||
+                     s.p[b - 1] == '\v' || s.p[b - 1] == '\f'))
+        --b;
+    if (a == b) return 0.0;
+    char buf[4096];
+    size_t n = b - a;
+    if (n >= sizeof(buf)) return NAN;
+    memcpy(buf, s.p + a, n);
+    buf[n] = 0;
+    const char* q = buf;
+    int neg = 0;
+    if (*q == '+' || *q == '-') {
+        neg = *q == '-';
+        ++q;
+    }
+    if (strcmp(q, "Infinity") == 0) return neg ? -INFINITY : INFINITY;
+    /* digits [. digits] | . digits, then [eE][+-]digits */
+    const char* r = q;
+    int dig = 0;
+    while (*r >= '0' && *r <= '9') ++r, ++dig;
+    if (*r == '.') {
+        ++r;
+        while (*r >= '0' && *r <= '9') ++r, ++dig;
+    }
+    if (!dig) return NAN;
+    if (*r == 'e' || *r == 'E') {
+        ++r;
+        if (*r == '+' || *r == '-') ++r;
+        if (!(*r >= '0' && *r <= '9')) return NAN;
+        while (*r >= '0' && *r <= '9') ++r;
+    }
+    if (*r) return NAN;
+    return strtod(buf, NULL);
+}
+
+/* ECMAScript ToInt32 (the Int32Array store). */
+static int32_t js_to_int32(double d) {
+    if (!isfinite(d)) return 0;
+    d = trunc(d);
+    double m = fmod(d, 4294967296.0);
+    if (m < 0) m += 4294967296.0;
+    uint32_t u = (uint32_t)m;
+    return (int32_t)u;
+}
+
+/* The k-th piece of line.split(" "); .p == NULL when there are fewer. */
+static mm_str js_split_space(mm_str line, int k) {
+    size_t a = 0;
+    for (int i = 0; i < k; ++i) {
+        const char* sp = memchr(line.p + a, ' ', line.n - a);
+        if (!sp) {
+            mm_str none = {NULL, 0};
+            return none;
+        }
+        a = (size_t)(sp - line.p) + 1;
+    }
+    const char* sp = memchr(line.p + a, ' ', line.n - a);
+    mm_str r = {line.p + a, sp ? (size_t)(sp - line.p) - a : line.n - a};
+    return r;
+}
+
+static int mm_streq(mm_str s, const char* lit) {
+    size_t n = strlen(lit);
+    return s.p && s.n == n && memcmp(s.p, lit, n) == 0;
+}
+
+typedef struct {
+    int pattern, symmetric;
+    int64_t nrows, ncols, nentries, nnz;
+    int32_t* row; /* mm_info.row/col/val: nentries each */
+    int32_t* col;
+    double* val;
+} mm_info_t;
+
+static void mm_info_free(mm_info_t* m) {
+    free(m->row);
+    free(m->col);
+    free(m->val);
+}
+
+/* read_MM_header + read_matrix_MM_files + calculate_actual_nnz
+ * (:1889-1981).  Returns 0, or -1 when the reference throws. */
+static int oracle_mm_read_info(const char* text, size_t len, mm_info_t* m) {
+    memset(m, 0, sizeof(*m));
+    /* complete lines only */
+    size_t nl = 0;
+    for (size_t i = 0; i < len; ++i) nl += text[i] == '\n';
+    mm_str* lines = (mm_str*)malloc((nl + 1) * sizeof(mm_str));
+    size_t a = 0, k = 0;
+    for (size_t i = 0; i < len; ++i)
+        if (text[i] == '\n') {
+            lines[k].p = text + a;
+            lines[k].n = i - a;
+            ++k;
+            a = i + 1;
+        }
+    if (k == 0) {
+        free(lines);
+        return -1;
+    }
+    mm_str first = lines[0];
+    m->pattern = mm_streq(js_split_space(first, 3), "pattern");
+    m->symmetric = mm_streq(js_split_space(first, 4), "symmetric");
+    size_t n = 0;
+    while (n < k && lines[n].n > 0 && lines[n].p[0] == '%') ++n;
+    if (n >= k) {
+        free(lines);
+        return -1;
+    }
+    mm_str info = lines[n++];
+    m->nrows = (int64_t)js_number(js_split_space(info, 0));
+    m->ncols = (int64_t)js_number(js_split_space(info, 1));
+    m->nentries = (int64_t)js_number(js_split_space(info, 2));
+    if (m->nentries > (1LL << 27) || m->nentries < 0) {
+        free(lines);
+        return -1;
+    }
+    const int64_t ne = m->nentries;
+    m->row = (int32_t*)calloc((size_t)(ne ? ne : 1), 4);
+    m->col = (int32_t*)calloc((size_t)(ne ? ne : 1), 4);
+    m->val = (double*)calloc((size_t)(ne ? ne : 1), 8);
+    int64_t nnz = 0;
+    for (int64_t j = 0; n < k; ++n, ++j) {
+        mm_str c0 = js_split_space(lines[n], 0), c1 = js_split_space(lines[n], 1), c2 = js_split_space(lines[n], 2);
+        /* Number(undefined) is NaN */
+        const double r = c0.p ? js_number(c0) : NAN, c = c1.p ? js_number(c1) : NAN;
+        const int in = j < ne; /* typed-array stores past the end are dropped */
+        if (in) {
+            m->row[j] = js_to_int32(r);
+            m->col[j] = js_to_int32(c);
+        }
+        if (m->symmetric) {
+            if (!m->pattern) {
+                if (in) m->val[j] = c2.p ? js_number(c2) : NAN;
+                const double v = in ? m->val[j] : NAN; /* undefined compares false */
+                if (v < 0 || v > 0) nnz += (in && m->row[j] == m->col[j]) ? 1 : 2;
+            } else {
+                /* undefined == undefined past the end */
+                nnz += (!in || m->row[j] == m->col[j]) ? 1 : 2;
+            }
+        } else if (!m->pattern) {
+            if (in) m->val[j] = c2.p ? js_number(c2) : NAN;
+            const double v = in ? m->val[j] : NAN;
+            if (v < 0 || v > 0) ++nnz;
+        }
+    }
+    free(lines);
+    if (nnz == 0) nnz = ne;
+    if (nnz > (1LL << 28)) {
+        mm_info_free(m);
+        return -1;
+    }
+    m->nnz = nnz;
+    return 0;
+}
+
+#define DEF_MM_COO(T, SUF)                                                                                        \
+    /* quick_sort_COO (:1577-1609) */                                                                              \
+    static void mm_qsort_##SUF(int32_t* row, int32_t* col, T* val, int64_t left, int64_t right) {                   \
+        int64_t i = left, j = right;                                                                               \
+        const int64_t mid = (left + right) / 2;                                                                    \
+        const int32_t pivot = row[mid], pivot_col = col[mid];                                                      \
+        while (i <= j) {                                                                                           \
+            while (row[i] < pivot || (row[i] == pivot && col[i] < pivot_col)) ++i;                                 \
+            while (row[j] > pivot || (row[j] == pivot && col[j] > pivot_col)) --j;                                 \
+            if (i <= j) {                                                                                          \
+                int32_t tr = row[i]; row[i] = row[j]; row[j] = tr;                                                 \
+                int32_t tc = col[i]; col[i] = col[j]; col[j] = tc;                                                 \
+                T tv = val[i]; val[i] = val[j]; val[j] = tv;                                                       \
+                ++i;                                                                                               \
+                --j;                                                                                               \
+            }                                                                                                      \
+        }                                                                                                          \
+        if (left < j) mm_qsort_##SUF(row, col, val, left, j);                                                      \
+        if (i < right) mm_qsort_##SUF(row, col, val, i, right);                                                    \
+    }                                                                                                              \
+    /* create_COO_from_MM (:2100-2172) or, lower = 1, create_LCOO_from_MM (:1983-2097) into arrays of *nnz      \
+     * entries (call with row == NULL for the size).  Returns 0 or -1. */                                          \
+    EXPORT int oracle_mm_coo_##SUF(const char* text, size_t len, int lower, int64_t* info6, int32_t* orow,          \
+                                   int32_t* ocol, T* oval) {                                                       \
+        mm_info_t m;                                                                                               \
+        if (oracle_mm_read_info(text, len, &m)) return -1;                                                          \
+        const int64_t ne = m.nentries;                                                                             \
+        const int32_t* row = m.row;                                                                                \
+        const int32_t* col = m.col;                                                                                \
+        const double* val = m.val;                                                                                 \
+        int64_t nnz = m.nnz;                                                                                       \
+        if (lower) {                                                                                               \
+            nnz = m.nrows;                                                                                         \
+            for (int64_t n = 0; n < ne; ++n) {                                                                     \
+                if (m.symmetric) {                                                                                 \
+                    if (row[n] != col[n] && (m.pattern || val[n] > 0 || val[n] < 0)) ++nnz;                         \
+                } else if (row[n] > col[n] && (m.pattern || val[n] > 0 || val[n] < 0)) {                            \
+                    ++nnz;                                                                                         \
+                }                                                                                                  \
+            }                                                                                                      \
+        }                                                                                                          \
+        info6[0] = m.nrows; info6[1] = m.ncols; info6[2] = ne; info6[3] = nnz;                                     \
+        info6[4] = m.pattern; info6[5] = m.symmetric;                                                              \
+        if (!orow) {                                                                                               \
+            mm_info_free(&m);                                                                                      \
+            return 0;                                                                                              \
+        }                                                                                                          \
+        memset(orow, 0, (size_t)nnz * 4);                                                                          \
+        memset(ocol, 0, (size_t)nnz * 4);                                                                          \
+        memset(oval, 0, (size_t)nnz * sizeof(T));                                                                  \
+        int64_t i = 0;                                                                                             \
+        /* stores at i >= nnz are dropped like typed-array stores past the end */                                 \
+        if (lower) {                                                                                               \
+            for (int64_t n = 0; n < ne; ++n) {                                                                     \
+                const int kept = m.pattern || val[n] < 0 || val[n] > 0;                                            \
+                const T v = m.pattern ? (T)1.0 : (T)val[n];                                                        \
+                if (!kept) continue;                                                                               \
+                if (row[n] > col[n]) {                                                                             \
+                    if (i < nnz) { orow[i] = row[n] - 1; ocol[i] = col[n] - 1; oval[i] = v; }                      \
+                    ++i;                                                                                           \
+                } else if (m.symmetric && row[n] < col[n]) {                                                       \
+                    if (i < nnz) { orow[i] = col[n] - 1; ocol[i] = row[n] - 1; oval[i] = v; }                      \
+                    ++i;                                                                                           \
+                }                                                                                                  \
+            }                                                                                                      \
+            for (int64_t n = 0; n < m.nrows; ++n) {                                                                \
+                if (i < nnz) { orow[i] = (int32_t)n; ocol[i] = (int32_t)n; oval[i] = (T)1.0; }                     \
+                ++i;                                                                                               \
+            }                                                                                                      \
+        } else {                                                                                                   \
+            for (int64_t n = 0; n < ne; ++n) {                                                                     \
+                if (!m.pattern && !(val[n] < 0 || val[n] > 0)) continue;                                           \
+                const T v = m.pattern ? (T)1.0 : (T)val[n];                                                        \
+                if (i < nnz) { orow[i] = row[n] - 1; ocol[i] = col[n] - 1; oval[i] = v; }                          \
+                if (m.symmetric && row[n] != col[n]) {                                                             \
+                    if (i + 1 < nnz) { orow[i + 1] = col[n] - 1; ocol[i + 1] = row[n] - 1; oval[i + 1] = v; }      \
+                    i += 2;                                                                                        \
+                } else {                                                                                           \
+                    ++i;                                                                                           \
+                }                                                                                                  \
+            }                                                                                                      \
+        }                                                                                                          \
+        if (nnz > 0) mm_qsort_##SUF(orow, ocol, oval, 0, nnz - 1);                                                 \
+        mm_info_free(&m);                                                                                          \
+        return 0;                                                                                                  \
+    }
+
+DEF_MM_COO(float, f32)
+DEF_MM_COO(double, f64)

@@ -15,6 +15,7 @@ from .convert import coo_csr, coo_csc, num_cols, csr_ell, num_diags, csr_dia, tr
 from .spts import (SptsAnalysis, CSR_create_level_sets, CSR_create_level_sets_with_reorder,  # noqa: F401
                    spts_level_csr, spts_syncfree, spts_csc)
 from .serve import Server, batch_spmv, batch_spts, batch_spts_refined, lower_dominant  # noqa: F401
+from .mm import MMInfo, read_matrix_MM_files, create_COO_from_MM, create_LCOO_from_MM, mmread  # noqa: F401
 
 __all__ = [
     "Context", "DeviceArray", "SableError",
@@ -25,4 +26,5 @@ __all__ = [
     "coo_csr", "coo_csc", "num_cols", "csr_ell", "num_diags", "csr_dia", "transpose",
     "SptsAnalysis", "CSR_create_level_sets", "CSR_create_level_sets_with_reorder", "spts_level_csr",
     "spts_syncfree", "spts_csc", "Server", "batch_spmv", "batch_spts", "batch_spts_refined", "lower_dominant",
+    "MMInfo", "read_matrix_MM_files", "create_COO_from_MM", "create_LCOO_from_MM", "mmread",
 ]

@@ -55,6 +55,11 @@ class CsrTune(ctypes.Structure):
     _fields_ = [("block_weight", c_int), ("row_weight", c_int), ("vector_lanes", c_int), ("vector_rows", c_int)]
 
 
+class MMInfo(ctypes.Structure):
+    _fields_ = [("nrows", c_i64), ("ncols", c_i64), ("nentries", c_i64), ("nnz", c_i64), ("lower_nnz", c_i64),
+                ("field", ctypes.c_int32), ("symmetric", ctypes.c_int32), ("body_offset", c_i64)]
+
+
 # name -> argtypes; every function returns int unless listed in _RESTYPES.
 SIGNATURES = {
     "sable_ctx_create": [c_int, c_vp, c_vpp],
@@ -101,6 +106,12 @@ SIGNATURES = {
     "sable_spmv_ell": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_coo_csr": [c_vp, c_int, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "sable_coo_csc": [c_vp, c_int, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "sable_mm_header": [c_vp, c_sz, ctypes.POINTER(MMInfo)],
+    "sable_mm_parse": [c_vp, c_vp, c_sz, c_int, c_vpp],
+    "sable_mm_get_info": [c_vp, ctypes.POINTER(MMInfo)],
+    "sable_mm_stats": [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)],
+    "sable_mm_coo": [c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp],
+    "sable_mm_destroy": [c_vp],
     "sable_csr_num_cols": [c_vp, c_i32, c_vp, ctypes.POINTER(c_i32)],
     "sable_csr_ell": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
     "sable_csr_num_diags": [c_vp, c_i32, c_vp, c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)],
@@ -142,6 +153,7 @@ _RESTYPES = {
     "sable_csr_plan_destroy": None,
     "sable_coo_plan_destroy": None,
     "sable_csr_tiles_destroy": None,
+    "sable_mm_destroy": None,
     "sable_spts_destroy": None,
     "sable_csr_panels_destroy": None,
     "sable_server_destroy": None,

@@ -264,3 +264,75 @@ def concat_matrix(k, ns, row_start, row_ptr, col, val):
     a, b = row_start[k], row_start[k + 1]
     rp = (row_ptr[a:b + 1] - row_ptr[a]).astype(np.int32)
     return int(ns[k]), rp, col[row_ptr[a]:row_ptr[b]], val[row_ptr[a]:row_ptr[b]]
+
+
+def _digits_into(out, pos, x, width):
+    """Writes the decimal digits of x (>= 0, int64) at out[pos:pos+width]."""
+    for k in range(int(width.max()) if len(width) else 0):
+        m = k < width
+        p10 = np.power(10, np.maximum(width - 1 - k, 0), dtype=np.int64)
+        out[pos[m] + k] = (48 + (x[m] // p10[m]) % 10).astype(np.uint8)
+
+
+def _ndigits(x):
+    w = np.ones(len(x), np.int64)
+    t = x.copy()
+    while True:
+        t //= 10
+        nz = t > 0
+        if not nz.any():
+            return w
+        w += nz
+
+
+def mm_text(nrows, ncols, row, col, val_tokens=None, val_idx=None, field="real", symmetry="general",
+            comment=b"% generated\n"):
+    """Coordinate MatrixMarket text, built with numpy (no per-entry Python).
+
+    row, col: 1-based indices.  Values are tokens from the pool val_tokens
+    (bytes) chosen by val_idx; pattern files pass None.  Lines are
+    "row col value\\n" with single spaces."""
+    row = np.asarray(row, np.int64)
+    col = np.asarray(col, np.int64)
+    n = len(row)
+    head = b"%%%%MatrixMarket matrix coordinate %s %s\n" % (field.encode(), symmetry.encode())
+    head += comment + b"%d %d %d\n" % (nrows, ncols, n)
+    lr, lc = _ndigits(row), _ndigits(col)
+    lens = lr + 1 + lc + 1
+    if val_tokens is not None:
+        pool_len = np.array([len(t) for t in val_tokens], np.int64)
+        maxl = int(pool_len.max())
+        pool = np.full((len(val_tokens), maxl), 32, np.uint8)
+        for i, t in enumerate(val_tokens):
+            pool[i, :len(t)] = np.frombuffer(t, np.uint8)
+        val_idx = np.asarray(val_idx, np.int64)
+        lv = pool_len[val_idx]
+        lens = lens + 1 + lv
+    offs = np.zeros(n + 1, np.int64)
+    np.cumsum(lens, out=offs[1:])
+    offs += len(head)
+    out = np.empty(int(offs[-1]), np.uint8)
+    out[:len(head)] = np.frombuffer(head, np.uint8)
+    base = offs[:-1]
+    _digits_into(out, base, row, lr)
+    out[base + lr] = 32
+    _digits_into(out, base + lr + 1, col, lc)
+    p = base + lr + 1 + lc
+    if val_tokens is not None:
+        out[p] = 32
+        for k in range(maxl):
+            m = k < lv
+            out[p[m] + 1 + k] = pool[val_idx[m], k]
+        p = p + 1 + lv
+    out[p] = 10
+    return out.tobytes()
+
+
+def value_pool(count, seed=0, style="repr"):
+    """Distinct decimal tokens of random doubles: 'repr' (shortest round
+    trip) or 'e16' ('%.16e', 17 significant digits)."""
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal(count) * np.exp(rng.uniform(-8, 8, count))
+    if style == "repr":
+        return [repr(float(x)).encode() for x in v]
+    return [(b"%.16e" % float(x)) for x in v]
