@@ -306,6 +306,75 @@ def sec_convert(args, sp, ctx, T, host):
     return out
 
 
+# ------------------------------------------------------------------ ingest
+def sec_ingest(args, sp, ctx, T):
+    """MatrixMarket text already in HBM -> sorted COO on the device.  A
+    symmetric file (lower triangle stored, '%.16e' values, the layout of
+    SuiteSparse exports); checked at full size against numpy's own expansion
+    and sort of the generator's arrays."""
+    from sciwasm.sparse_amd import gen
+    rng = np.random.default_rng(args.seed + 9)
+    n, nent = args.ingest_rows, args.ingest_entries
+    r = rng.integers(0, n, int(nent * 1.05))
+    c = rng.integers(0, n, len(r))
+    r, c = np.maximum(r, c), np.minimum(r, c)
+    _, first = np.unique(r * n + c, return_index=True)
+    keep = rng.permutation(first)[:nent]
+    r, c = r[keep], c[keep]
+    del first, keep
+    pool = gen.value_pool(4096, seed=args.seed, style="e16")
+    vi = rng.integers(0, len(pool), len(r))
+    t0 = time.time()
+    text = gen.mm_text(n, n, r + 1, c + 1, pool, vi, symmetry="symmetric")
+    log("ingest: %d entries, %.1f MB text generated in %.1fs" % (len(r), len(text) / 1e6, time.time() - t0))
+    host = np.frombuffer(text, np.uint8)
+    t0 = time.perf_counter()
+    d = sp.DeviceArray.from_host(host, ctx=ctx)
+    ctx.sync()
+    h2d_s = time.perf_counter() - t0
+    parse_ms, coo_ms = [], []
+    info = A = None
+    for _ in range(3):
+        if info is not None:
+            info.free()
+        A = None
+        tp, info = T.once(lambda: sp.read_matrix_MM_files(d, ctx))
+        tc, A = T.once(lambda: sp.create_COO_from_MM(info, np.float64))
+        parse_ms.append(tp)
+        coo_ms.append(tc)
+    lines, exact = info.stats()
+    # full-size check: the expansion and (row, col) order computed by numpy
+    pv = np.array([float(t) for t in pool])
+    off = r != c
+    er = np.concatenate([r, c[off]])
+    ec = np.concatenate([c, r[off]])
+    ev = np.concatenate([pv[vi], pv[vi][off]])
+    order = np.argsort(er.astype(np.int64) * n + ec, kind="stable")
+    ok = (A.nnz == len(er) and np.array_equal(A.row.numpy(), er[order]) and
+          np.array_equal(A.col.numpy(), ec[order]) and
+          np.array_equal(A.val.numpy().view(np.uint64), ev[order].view(np.uint64)))
+    tp, tc = min(parse_ms), min(coo_ms)
+    out = {"workload": "symmetric MatrixMarket text, N=%d, %d stored entries, '%%.16e' values" % (n, len(r)),
+           "text_bytes": len(text), "entries": len(r), "coo_nnz": int(A.nnz), "exact_values": int(exact),
+           "parse_ms": r4(tp), "coo_ms": r4(tc), "total_ms": r4(tp + tc),
+           "parse_text_gbs": r4(len(text) / (tp * 1e-3) / 1e9),
+           "entries_per_s": r4(len(r) / ((tp + tc) * 1e-3)),
+           "h2d_s": r4(h2d_s), "text_gbs_with_h2d": r4(len(text) / (h2d_s + (tp + tc) * 1e-3) / 1e9),
+           "check_full_size": bool(ok)}
+    if not args.no_cpu:
+        import oracle
+        k = min(len(r), 1000000)
+        sample = gen.mm_text(n, n, r[:k] + 1, c[:k] + 1, pool, vi[:k], symmetry="symmetric")
+        t0 = time.perf_counter()
+        oi = oracle.mm_coo(sample, np.float64)[0]
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"entries_per_s": r4(k / dt), "cores": 1, "kind": "port",
+                               "sample": "first %d entries of the same file (%d-entry COO)" % (k, oi["nnz"])}
+    info.free()
+    log("ingest:", out)
+    return out
+
+
 # ------------------------------------------------------------------ spts
 def sec_spts(args, sp, ctx, T):
     import oracle
@@ -503,7 +572,9 @@ def main():
                          "non-temporal matrix loads)")
     ap.add_argument("--block-weight", type=str, default="0",
                     help="comma list of row-block weights (nnz + 2*rows) to time with each panel width (0 = 1024)")
-    ap.add_argument("--sections", type=str, default="c1,band,convert,spts,batch,serve",
+    ap.add_argument("--ingest-rows", type=int, default=4194304)
+    ap.add_argument("--ingest-entries", type=int, default=8000000)
+    ap.add_argument("--sections", type=str, default="c1,band,convert,ingest,spts,batch,serve",
                     help="comma list of extra sections (empty for the headline only)")
     ap.add_argument("--batch-count", type=int, default=2048)
     ap.add_argument("--serve-rates", type=str, default="2000,10000,30000,60000,120000,240000")
@@ -542,6 +613,8 @@ def main():
         run("band", lambda: sec_band(args, sp, ctx, T))
     if "convert" in sections:
         run("convert", lambda: sec_convert(args, sp, ctx, T, host))
+    if "ingest" in sections:
+        run("ingest", lambda: sec_ingest(args, sp, ctx, T))
     if "spts" in sections:
         run("spts", lambda: sec_spts(args, sp, ctx, T))
     pool = lows = None

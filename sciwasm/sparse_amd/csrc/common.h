@@ -43,6 +43,10 @@ struct sable_ctx_s {
     size_t scratch_bytes = 0;
     // small pinned host buffer for reading device flags / counters
     void* pinned = nullptr;
+    // stream-ordered pool for per-call temporaries (created on first use;
+    // keeps its memory until the context is destroyed)
+    hipMemPool_t pool = nullptr;
+    bool pool_tried = false;
     std::mutex mu;
 };
 
@@ -60,6 +64,11 @@ int syncfree_launch(hipStream_t st, int num_cu, sable_dtype_t dt, int lanes, int
                     const int32_t* col, const void* val, const void* b, void* x, unsigned* ticket, int* err,
                     long long spin);
 int ctx_scratch(sable_ctx_t* ctx, size_t bytes, void** out);
+// Temporary device memory ordered on ctx->stream: usable by work enqueued on
+// that stream after the call, released by ctx_pool_free once the work that
+// uses it has been enqueued.  Without a memory pool it is plain hipMalloc.
+int ctx_pool_alloc(sable_ctx_t* ctx, size_t bytes, void** out);
+void ctx_pool_free(sable_ctx_t* ctx, void* p);
 int read_device_i64(sable_ctx_t* ctx, const void* dsrc, size_t count, int64_t* host_out);
 
 constexpr int WAVE = 64;

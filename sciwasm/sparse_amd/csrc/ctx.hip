@@ -45,6 +45,39 @@ int ctx_scratch(sable_ctx_t* ctx, size_t bytes, void** out) {
     return SABLE_OK;
 }
 
+int ctx_pool_alloc(sable_ctx_t* ctx, size_t bytes, void** out) {
+    if (bytes == 0) bytes = 256;
+    *out = nullptr;
+    if (!ctx->pool_tried) {
+        ctx->pool_tried = true;
+        hipMemPoolProps props;
+        memset(&props, 0, sizeof(props));
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = ctx->device;
+        if (hipMemPoolCreate(&ctx->pool, &props) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(ctx->pool, hipMemPoolAttrReleaseThreshold, &keep);
+        } else {
+            (void)hipGetLastError();
+            ctx->pool = nullptr;
+        }
+    }
+    hipError_t e = ctx->pool ? hipMallocFromPoolAsync(out, bytes, ctx->pool, ctx->stream) : hipMalloc(out, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *out = nullptr;
+        return fail(SABLE_ENOMEM, "temporary allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    }
+    return SABLE_OK;
+}
+
+void ctx_pool_free(sable_ctx_t* ctx, void* p) {
+    if (!p) return;
+    if (ctx->pool) (void)hipFreeAsync(p, ctx->stream);
+    else (void)hipFree(p);
+}
+
 int read_device_i64(sable_ctx_t* ctx, const void* dsrc, size_t count, int64_t* host_out) {
     SABLE_CHECK(count <= 64, SABLE_EINVAL, "read_device_i64: count %zu > 64", count);
     SABLE_HIP(hipMemcpyAsync(ctx->pinned, dsrc, count * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -124,6 +157,7 @@ SABLE_API void sable_ctx_destroy(sable_ctx_t* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->pool) (void)hipMemPoolDestroy(ctx->pool);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -183,8 +183,10 @@ __global__ __launch_bounds__(MB) void k_mm_parse(const uint8_t* __restrict__ t, 
                                                  int32_t* __restrict__ rows, int32_t* __restrict__ cols,
                                                  double* __restrict__ vals, int32_t* __restrict__ cnt,
                                                  int32_t* __restrict__ amb_list,
-                                                 unsigned long long* __restrict__ counters) {
+                                                 unsigned long long* __restrict__ counters,
+                                                 int32_t* __restrict__ partials) {
     __shared__ __align__(16) uint8_t stage[STAGE_BYTES + 32];
+    __shared__ int wsum[3][MB / WAVE];
     const int64_t l0 = (int64_t)blockIdx.x * MB;
     const int64_t l1 = l0 + MB < nlines ? l0 + MB : nlines;
     const int64_t s0 = starts[l0];
@@ -217,13 +219,42 @@ __global__ __launch_bounds__(MB) void k_mm_parse(const uint8_t* __restrict__ t, 
             parse_one(GSrc{t, e}, l, p, pattern, symmetric, nrows, ncols, rows, cols, vals, cnt, amb_list, counters,
                       &n_entry, &n_full, &n_lower);
     }
+    // per-workgroup sums; k_mm_reduce adds them (one device-wide atomic per
+    // wave on three addresses serialises across the XCDs)
     n_entry = wave_sum(n_entry);
     n_full = wave_sum(n_full);
     n_lower = wave_sum(n_lower);
+    const int wv = threadIdx.x / WAVE;
     if ((threadIdx.x & (WAVE - 1)) == 0) {
-        if (n_entry) atomicAdd(&counters[C_ENTRIES], (unsigned long long)n_entry);
-        if (n_full) atomicAdd(&counters[C_NNZ], (unsigned long long)n_full);
-        if (n_lower) atomicAdd(&counters[C_LOWER], (unsigned long long)n_lower);
+        wsum[0][wv] = n_entry;
+        wsum[1][wv] = n_full;
+        wsum[2][wv] = n_lower;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int t = 0;
+        for (int w = 0; w < MB / WAVE; ++w) t += wsum[threadIdx.x][w];
+        partials[3 * (int64_t)blockIdx.x + threadIdx.x] = t;
+    }
+}
+
+// counters[C_ENTRIES, C_NNZ, C_LOWER] += sums of the per-workgroup partials.
+__global__ __launch_bounds__(1024) void k_mm_reduce(const int32_t* __restrict__ partials, int64_t nblocks,
+                                                    unsigned long long* __restrict__ counters) {
+    unsigned long long s[3] = {0, 0, 0};
+    for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x)
+        for (int k = 0; k < 3; ++k) s[k] += (unsigned long long)partials[3 * b + k];
+    __shared__ unsigned long long red[3][1024 / WAVE];
+    for (int k = 0; k < 3; ++k) {
+        unsigned long long v = s[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+        if ((threadIdx.x & (WAVE - 1)) == 0) red[k][threadIdx.x / WAVE] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        unsigned long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x / WAVE); ++w) t += red[threadIdx.x][w];
+        counters[C_ENTRIES + threadIdx.x] += t;
     }
 }
 
@@ -427,6 +458,7 @@ int parse_header(const char* h, size_t n, bool at_end, sable_mm_info_t* info) {
 struct sable_mm_s {
     sable_mm_info_t info;
     int64_t nlines = 0;
+    void* block = nullptr;  // one allocation holding the four arrays below
     int32_t* rows = nullptr;
     int32_t* cols = nullptr;
     double* vals = nullptr;
@@ -437,10 +469,7 @@ struct sable_mm_s {
 
 static void mm_free(sable_mm_t* m) {
     if (!m) return;
-    if (m->rows) (void)hipFree(m->rows);
-    if (m->cols) (void)hipFree(m->cols);
-    if (m->vals) (void)hipFree(m->vals);
-    if (m->cnt) (void)hipFree(m->cnt);
+    if (m->block) (void)hipFree(m->block);
     delete m;
 }
 
@@ -464,27 +493,27 @@ static int mm_parse_device(sable_ctx_t* ctx, const uint8_t* t, int64_t len, sabl
     SABLE_CHECK(!info.symmetric || info.nrows == info.ncols, SABLE_EMALFORMED, "a symmetric matrix must be square");
     sable_mm_t* m = new sable_mm_t();
     m->info = info;
-    // 1. line heads: per-tile counts and their scan live in a buffer of
-    // their own, so the scratch can grow for starts[] without losing them
+    // 1. line heads.  Temporaries come from the context's stream-ordered
+    // pool, so repeated parses do not pay for device allocation.
     const int64_t mis = (int64_t)((uintptr_t)t & 15);
     const int64_t a0 = b0 - ((b0 + mis) & 15);
     const int64_t ntiles = len > a0 ? (len - a0 + TILE_BYTES - 1) / TILE_BYTES : 0;
     SABLE_CHECK(ntiles < 0x7fffffffLL, SABLE_EINVAL, "text too large");
     int64_t nlines = 0;
-    int32_t* tiles = nullptr;  // [counts | offsets | total | scan tmp]
+    void* tiles = nullptr;  // [counts | offsets | total | scan tmp]
+    void* work = nullptr;   // [starts | amb | counters | partials]
     auto cleanup = [&](int rc) {
-        if (tiles) (void)hipFree(tiles);
+        ctx_pool_free(ctx, tiles);
+        ctx_pool_free(ctx, work);
+        tiles = work = nullptr;
         if (rc) mm_free(m);
         return rc;
     };
     int64_t counters[C_COUNT] = {0, 0, 0, 0, -1};
     if (ntiles > 0) {
         const size_t tb = Carver::round(4 * (size_t)ntiles);
-        if (hipMalloc(&tiles, 2 * tb + 256 + scan_tmp_bytes(ntiles)) != hipSuccess) {
-            (void)hipGetLastError();
-            tiles = nullptr;
-            return cleanup(fail(SABLE_ENOMEM, "line-head tile arrays"));
-        }
+        int rc = ctx_pool_alloc(ctx, 2 * tb + 256 + scan_tmp_bytes(ntiles), &tiles);
+        if (rc) return cleanup(rc);
         Carver tv(tiles);
         int32_t* tc = tv.take<int32_t>(ntiles);
         int32_t* to = tv.take<int32_t>(ntiles);
@@ -492,7 +521,7 @@ static int mm_parse_device(sable_ctx_t* ctx, const uint8_t* t, int64_t len, sabl
         void* stmp = tv.p + tv.used;
         hipLaunchKernelGGL(k_mm_count, dim3((unsigned)ntiles), dim3(MB), 0, st, t, len, b0, a0, tc);
         SABLE_LAUNCH_CHECK();
-        int rc = scan_exclusive_i32(st, tc, to, ntiles, tot, stmp);
+        rc = scan_exclusive_i32(st, tc, to, ntiles, tot, stmp);
         if (rc) return cleanup(rc);
         SABLE_HIP(hipMemcpyAsync(ctx->pinned, tot, 4, hipMemcpyDeviceToHost, st));
         SABLE_HIP(hipStreamSynchronize(st));
@@ -502,21 +531,28 @@ static int mm_parse_device(sable_ctx_t* ctx, const uint8_t* t, int64_t len, sabl
     }
     m->nlines = nlines;
     const size_t L = (size_t)(nlines ? nlines : 1);
-    if (hipMalloc(&m->rows, 4 * L) != hipSuccess || hipMalloc(&m->cols, 4 * L) != hipSuccess ||
-        hipMalloc(&m->vals, 8 * L) != hipSuccess || hipMalloc(&m->cnt, 4 * L) != hipSuccess) {
+    // rows | cols | cnt | vals in one allocation
+    if (hipMalloc(&m->block, 20 * L + 64) != hipSuccess) {
         (void)hipGetLastError();
+        m->block = nullptr;
         return cleanup(fail(SABLE_ENOMEM, "allocation for %lld parsed lines failed", (long long)nlines));
     }
+    m->vals = (double*)m->block;
+    m->rows = (int32_t*)((char*)m->block + 8 * L);
+    m->cols = m->rows + L;
+    m->cnt = m->cols + L;
     if (nlines > 0) {
         // 2. starts, then the parse
-        const size_t need = Carver::round(8 * (size_t)nlines) + Carver::round(4 * (size_t)nlines) + 256;
-        void* s;
-        int rc = ctx_scratch(ctx, need, &s);
+        const int64_t nblk = (nlines + MB - 1) / MB;
+        const size_t need = Carver::round(8 * (size_t)nlines) + Carver::round(4 * (size_t)nlines) + 256 +
+                            Carver::round(12 * (size_t)nblk);
+        int rc = ctx_pool_alloc(ctx, need, &work);
         if (rc) return cleanup(rc);
-        Carver cv(s);
+        Carver cv(work);
         int64_t* starts = cv.take<int64_t>(nlines);
         int32_t* amb = cv.take<int32_t>(nlines);
         unsigned long long* ctr = cv.take<unsigned long long>(C_COUNT);
+        int32_t* partials = cv.take<int32_t>(3 * nblk);
         Carver tv(tiles);
         (void)tv.take<int32_t>(ntiles);
         const int32_t* to = tv.take<int32_t>(ntiles);
@@ -527,7 +563,9 @@ static int mm_parse_device(sable_ctx_t* ctx, const uint8_t* t, int64_t len, sabl
         SABLE_HIP(hipMemcpyAsync(ctr, ctx->pinned, sizeof(init), hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_mm_parse, dim3(gs(nlines)), dim3(MB), 0, st, t, len, starts, nlines,
                            (int)(info.field == SABLE_MM_PATTERN), (int)info.symmetric, info.nrows, info.ncols, m->rows,
-                           m->cols, m->vals, m->cnt, amb, ctr);
+                           m->cols, m->vals, m->cnt, amb, ctr, partials);
+        SABLE_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_mm_reduce, dim3(1), dim3(1024), 0, st, partials, nblk, ctr);
         SABLE_LAUNCH_CHECK();
         rc = read_device_i64(ctx, ctr, C_COUNT, counters);
         if (rc) return cleanup(rc);
@@ -542,7 +580,6 @@ static int mm_parse_device(sable_ctx_t* ctx, const uint8_t* t, int64_t len, sabl
         }
     }
     cleanup(SABLE_OK);
-    tiles = nullptr;
     const uint64_t err = (uint64_t)counters[C_ERR];
     if (err != ~0ull) {
         const int64_t pos = (int64_t)(err >> 8);

@@ -567,6 +567,105 @@ napi_value spts_csc_impl(napi_env env, napi_callback_info info, bool wrapper) {
 napi_value spts_csc(napi_env env, napi_callback_info info) { return spts_csc_impl(env, info, false); }
 napi_value spts_csc_wrapper(napi_env env, napi_callback_info info) { return spts_csc_impl(env, info, true); }
 
+// mmread(text[, lower]) -> {row_index, col_index, val_index, N, nnz, nrows,
+// ncols, nentries, field, symmetry}: the whole MatrixMarket text (Buffer,
+// TypedArray, ArrayBuffer or string) parsed on the device; the COO arrays are
+// allocated in the linear memory with _malloc.  lower = 1 builds the lower
+// COO with a unit diagonal.  Replaces mmread / read_matrix_MM_files +
+// create_COO_from_MM (seq/src/sparse32.js:1953-2172, 2498-2510) and
+// create_LCOO_from_MM (:1983-2097).
+napi_value mmread(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        size_t argc = 2;
+        napi_value argv[2];
+        napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr);
+        if (argc < 1) throw Err{"mmread(text[, lower])"};
+        int64_t lower = 0;
+        if (argc > 1) napi_get_value_int64(env, argv[1], &lower);
+        const char* text = nullptr;
+        size_t len = 0;
+        std::string copy;
+        bool is = false;
+        if (napi_is_buffer(env, argv[0], &is) == napi_ok && is) {
+            void* d = nullptr;
+            napi_get_buffer_info(env, argv[0], &d, &len);
+            text = (const char*)d;
+        } else if (napi_is_typedarray(env, argv[0], &is) == napi_ok && is) {
+            napi_typedarray_type t;
+            size_t n = 0, off = 0;
+            void* d = nullptr;
+            napi_value ab;
+            napi_get_typedarray_info(env, argv[0], &t, &n, &d, &ab, &off);
+            size_t es = (t == napi_int8_array || t == napi_uint8_array || t == napi_uint8_clamped_array) ? 1
+                        : (t == napi_int16_array || t == napi_uint16_array) ? 2
+                        : (t == napi_float64_array || t == napi_bigint64_array || t == napi_biguint64_array) ? 8
+                        : 4;
+            text = (const char*)d;
+            len = n * es;
+        } else if (napi_is_arraybuffer(env, argv[0], &is) == napi_ok && is) {
+            void* d = nullptr;
+            napi_get_arraybuffer_info(env, argv[0], &d, &len);
+            text = (const char*)d;
+        } else {
+            size_t n = 0;
+            if (napi_get_value_string_utf8(env, argv[0], nullptr, 0, &n) != napi_ok)
+                throw Err{"mmread: text must be a Buffer, TypedArray, ArrayBuffer or string"};
+            copy.resize(n + 1);
+            napi_get_value_string_utf8(env, argv[0], &copy[0], n + 1, &n);
+            copy.resize(n);
+            text = copy.data();
+            len = n;
+        }
+        if (!len) throw Err{"mmread: empty text"};
+        ensure_ctx();
+        sable_mm_t* mm = nullptr;
+        check(sable_mm_parse(g_ctx, text, len, 0, &mm), "mmread");
+        sable_mm_info_t mi;
+        sable_mm_get_info(mm, &mi);
+        const int64_t n = lower ? mi.lower_nnz : mi.nnz;
+        const size_t ro = lm_malloc(4 * (size_t)n), co = lm_malloc(4 * (size_t)n), vo = lm_malloc(P->vs * (size_t)n);
+        if (n > 0 && (!ro || !co || !vo)) {
+            if (ro) lm_free(ro);
+            if (co) lm_free(co);
+            if (vo) lm_free(vo);
+            sable_mm_destroy(mm);
+            throw Err{"mmread: the linear memory cannot hold the COO"};
+        }
+        if (n > 0) {
+            void* dr = scratch(ro, 4 * (size_t)n, false);
+            void* dc = scratch(co, 4 * (size_t)n, false);
+            void* dv = scratch(vo, P->vs * (size_t)n, false);
+            const int rc = sable_mm_coo(g_ctx, mm, P->dt, (int)lower, (int32_t*)dr, (int32_t*)dc, dv);
+            if (rc != SABLE_OK) {
+                sable_mm_destroy(mm);
+                check(rc, "mmread");
+            }
+            download(ro, dr, 4 * (size_t)n);
+            download(co, dc, 4 * (size_t)n);
+            download(vo, dv, P->vs * (size_t)n);
+            release_scratch();
+        }
+        sable_mm_destroy(mm);
+        static const char* fields[] = {"real", "integer", "complex", "pattern", "other"};
+        napi_value o, s1, s2;
+        napi_create_object(env, &o);
+        napi_set_named_property(env, o, "row_index", num(env, (double)ro));
+        napi_set_named_property(env, o, "col_index", num(env, (double)co));
+        napi_set_named_property(env, o, "val_index", num(env, (double)vo));
+        napi_set_named_property(env, o, "N", num(env, (double)mi.nrows));
+        napi_set_named_property(env, o, "nnz", num(env, (double)n));
+        napi_set_named_property(env, o, "nrows", num(env, (double)mi.nrows));
+        napi_set_named_property(env, o, "ncols", num(env, (double)mi.ncols));
+        napi_set_named_property(env, o, "nentries", num(env, (double)mi.nentries));
+        napi_create_string_utf8(env, fields[mi.field >= 0 && mi.field <= 4 ? mi.field : 4], NAPI_AUTO_LENGTH, &s1);
+        napi_create_string_utf8(env, mi.symmetric ? "symmetric" : "general", NAPI_AUTO_LENGTH, &s2);
+        napi_set_named_property(env, o, "field", s1);
+        napi_set_named_property(env, o, "symmetry", s2);
+        return o;
+    });
+}
+
 // num_cols(row_ptr, N) -> i32
 napi_value num_cols(napi_env env, napi_callback_info info) {
     return guarded(env, [&] {
@@ -901,7 +1000,7 @@ napi_value make_serial(napi_env env, Prec* P) {
         {"count_transpose_cols_ell", count_transpose_cols_ell}, {"transpose_ell", transpose_ell},
         {"memcpy", lm_memcpy}, {"coo_csr", coo_csr}, {"num_cols", num_cols}, {"csr_ell", csr_ell},
         {"num_diags", num_diags}, {"csr_dia", csr_dia}, {"coo_csc", coo_csc},
-        {"spts_csc", spts_csc}, {"spts_csc_wrapper", spts_csc_wrapper},
+        {"spts_csc", spts_csc}, {"spts_csc_wrapper", spts_csc_wrapper}, {"mmread", mmread},
     };
     for (auto& f : fns) add_fn(env, m, f.name, f.cb, P);
     return m;

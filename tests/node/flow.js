@@ -18,6 +18,7 @@
 //   spts_csr_sync_free_wrapper, spts_level_csr,
 //     spts_csr_opt_level_sync_free_wrapper
 //   coo_csc, spts_csc / spts_csc_wrapper (serial column sweep)
+//   mmread (full and lower COO from MatrixMarket text)
 'use strict';
 
 const fs = require('fs');
@@ -273,6 +274,22 @@ if (spec.spts) {
   const y5 = alloc(S.N, FArray);
   E.spts_csc_wrapper(Lc.col.off, Lc.row.off, Lc.val.off, bc.off, y5, S.N, inner_max);
   save('spts_csc_wrapper', y5, S.N, FArray);
+}
+
+// ---------------------------------------------------------------- MatrixMarket
+if (spec.mm) {
+  step('mmread');
+  const text = fs.readFileSync(path.join(dir, 'mm.mtx'));
+  const A = E.mmread(text);
+  outputs.mm = { N: A.N, nnz: A.nnz, field: A.field, symmetry: A.symmetry };
+  save('mm_row', A.row_index, A.nnz, Int32Array);
+  save('mm_col', A.col_index, A.nnz, Int32Array);
+  save('mm_val', A.val_index, A.nnz, FArray);
+  const L = E.mmread(text, 1);
+  outputs.mm_lower_nnz = L.nnz;
+  save('mml_row', L.row_index, L.nnz, Int32Array);
+  save('mml_col', L.col_index, L.nnz, Int32Array);
+  save('mml_val', L.val_index, L.nnz, FArray);
 }
 
 // ---------------------------------------------------------------- _free

@@ -31,7 +31,7 @@ NODE = shutil.which("node")
 SERIAL = ["spmv_coo", "spmv_coo_wrapper", "spmv_csr", "spmv_csr_wrapper", "spmv_dia", "spmv_dia_wrapper",
           "spmv_ell", "spmv_ell_wrapper", "transpose_csr", "transpose_dia", "count_transpose_cols_ell",
           "transpose_ell", "memcpy", "coo_csr", "num_cols", "csr_ell", "num_diags", "csr_dia", "coo_csc",
-          "spts_csc", "spts_csc_wrapper"]
+          "spts_csc", "spts_csc_wrapper", "mmread"]
 PARALLEL = ["spmv_coo_wrapper", "spmv_csr_wrapper", "spmv_dia_col_wrapper", "spmv_ell_col_wrapper", "memcpy",
             "sum", "spts_level_csr", "spts_csr_sync_free_wrapper", "spts_csr_level_sync_free_wrapper",
             "spts_csr_opt_level_sync_free_wrapper", "metadata"]
@@ -109,10 +109,25 @@ def write_inputs(d, dtype, seed=0, workers=3, inner_max=3):
                   Lc_col=ccp, Lc_row=crow, Lc_val=cval)
     for k, v in arrays.items():
         np.ascontiguousarray(v).tofile(os.path.join(d, k + ".bin"))
-    spec = dict(N=N, nnz=nnz, inner_max=inner_max, workers=workers, spts=dict(N=n, nlevels=int(nl)))
+    with open(os.path.join(d, "mm.mtx"), "wb") as f:
+        f.write(mm_text(seed))
+    spec = dict(N=N, nnz=nnz, inner_max=inner_max, workers=workers, spts=dict(N=n, nlevels=int(nl)), mm=True)
     with open(os.path.join(d, "inputs.json"), "w") as f:
         json.dump(spec, f)
     return spec, arrays
+
+
+def mm_text(seed):
+    """A symmetric MatrixMarket file with distinct keys and a few explicit zeros."""
+    rng = np.random.default_rng(seed + 7)
+    n = 300
+    r = rng.integers(1, n + 1, 2500)
+    c = rng.integers(1, n + 1, 2500)
+    r, c = np.maximum(r, c), np.minimum(r, c)
+    _, first = np.unique(r * (n + 1) + c, return_index=True)
+    first = np.sort(first)
+    pool = gen.value_pool(50, seed=seed) + [b"0"]
+    return gen.mm_text(n, n, r[first], c[first], pool, rng.integers(0, len(pool), len(first)), symmetry="symmetric")
 
 
 def row_bound(rp, col, val, x, y0, dtype, extra=0):
@@ -217,4 +232,14 @@ def test_node_flow_matches_oracle(tmp_path, dtype):
     yc = oracle.spts_csc(n, a["Lc_col"], a["Lc_row"], a["Lc_val"], a["L_b"])
     assert np.array_equal(read_out(d, "spts_csc", dtype).view(np.uint8), yc.view(np.uint8))
     assert np.array_equal(read_out(d, "spts_csc_wrapper", dtype).view(np.uint8), yc.view(np.uint8))
+
+    # MatrixMarket text -> COO in the linear memory: bit-exact
+    text = open(os.path.join(d, "mm.mtx"), "rb").read()
+    for pre, lower in (("mm", False), ("mml", True)):
+        info, r, c, v = oracle.mm_coo(text, dtype, lower=lower)
+        assert np.array_equal(read_out(d, pre + "_row", i32), r)
+        assert np.array_equal(read_out(d, pre + "_col", i32), c)
+        assert np.array_equal(read_out(d, pre + "_val", dtype).view(np.uint8), v.view(np.uint8))
+    assert out["mm"] == dict(N=300, nnz=int(oracle.mm_coo(text, dtype)[0]["nnz"]), field="real",
+                             symmetry="symmetric")
     assert out["freed"] == 1
