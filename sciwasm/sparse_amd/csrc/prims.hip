@@ -4,13 +4,11 @@
 // sum pass, a recursive scan of the tile sums, then a per-tile scan that
 // adds the tile offset.  In place is allowed.
 //
-// Radix sort: 8-bit digits, tiles of 2048 keys (256 threads x 8 rounds,
-// striped so that a round covers 256 consecutive keys).  Per pass: digit
-// histogram per tile (digit-major layout), exclusive scan of the histogram,
-// then a scatter in which each key's rank among equal digits of its tile is
-// found with wave ballots (8 ballots give the set of lanes holding the same
-// digit) plus per-wave counts in LDS.  Equal digits keep their input order,
-// so the sort is stable.
+// Radix sort: 8-bit digits, tiles of 4096 keys (4 waves x 1024 contiguous
+// keys).  Per pass: digit histogram per tile (digit-major layout), exclusive
+// scan of the histogram, then a scatter that ranks the tile's keys by digit
+// with wave ballots, reorders them in LDS and writes each digit's run
+// contiguously.  Equal digits keep their input order, so the sort is stable.
 #include "common.h"
 #include "prims.h"
 
@@ -115,45 +113,80 @@ int scan_exclusive_i32(hipStream_t st, const int32_t* in, int32_t* out, int64_t 
 }
 
 // ---------------------------------------------------------------- radix sort
-constexpr int RS_BS = 256;
-constexpr int RS_IT = 8;
-constexpr int RS_TILE = RS_BS * RS_IT;
+constexpr int RS_BS = 256;                 // 4 waves
+constexpr int RS_WAVES = RS_BS / WAVE;
+constexpr int RS_ROUNDS = 16;              // 64-key rounds per wave
+constexpr int RS_WKEYS = WAVE * RS_ROUNDS; // keys per wave (contiguous)
+constexpr int RS_TILE = RS_BS * RS_ROUNDS; // 4096 keys per workgroup
 constexpr int RADIX = 256;
 
+// Per-tile digit histogram, digit-major: counts[d * ntiles + tile].  Each
+// wave counts into its own LDS row so that skewed digits do not serialise
+// on one counter.
 __global__ __launch_bounds__(RS_BS) void k_rs_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
                                                    int32_t* __restrict__ counts, int64_t ntiles) {
-    __shared__ int hist[RADIX];
-    hist[threadIdx.x] = 0;
+    __shared__ int hist[RS_WAVES][RADIX];
+    const int tid = threadIdx.x, w = tid / WAVE;
+#pragma unroll
+    for (int k = 0; k < RS_WAVES; k++) hist[k][tid] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-#pragma unroll
-    for (int r = 0; r < RS_IT; r++) {
-        int64_t i = base + r * RS_BS + threadIdx.x;
-        if (i < n) atomicAdd(&hist[(int)((keys[i] >> shift) & (RADIX - 1))], 1);
+#pragma unroll 4
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        const int64_t i = base + r * RS_BS + tid;
+        if (i < n) atomicAdd(&hist[w][(int)((keys[i] >> shift) & (RADIX - 1))], 1);
     }
     __syncthreads();
-    counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < RS_WAVES; k++) t += hist[k][tid];
+    counts[(int64_t)tid * ntiles + blockIdx.x] = t;
 }
 
+// One stable scatter pass over a 4096-key tile.
+//  1. Wave w ranks its 1024 contiguous keys, 64 per round: the 8 digit-bit
+//     ballots give each lane the set of lanes with its digit; a lane's rank
+//     is the number of such lanes below it plus the wave's running count of
+//     that digit (kept in LDS by the group's lowest lane).
+//  2. Tile-local start of each digit (scan over digits) and each wave's
+//     offset inside it give every key its position in the tile sorted by
+//     digit; keys and values are placed there in LDS.
+//  3. The tile is written out in sorted order: consecutive threads write
+//     consecutive positions of one bucket, so the stores coalesce into runs
+//     instead of one scattered 8-byte store per key.
 __global__ __launch_bounds__(RS_BS) void k_rs_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                       int64_t n, int shift, const int32_t* __restrict__ offsets,
                                                       int64_t ntiles) {
-    __shared__ int running[RADIX];
-    __shared__ int wcnt[RS_BS / WAVE][RADIX];
+    __shared__ uint64_t sk[RS_TILE];
+    __shared__ uint32_t sv[RS_TILE];
+    __shared__ int wcnt[RS_WAVES][RADIX];  // running, then exclusive-over-waves counts
+    __shared__ int dstart[RADIX];           // tile-local start of each digit
+    __shared__ int gbase[RADIX];            // global start of this tile's run of each digit
+    __shared__ int scan_lds[RS_WAVES];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
-    running[tid] = offsets[(int64_t)tid * ntiles + blockIdx.x];
 #pragma unroll
-    for (int k = 0; k < RS_BS / WAVE; k++) wcnt[k][tid] = 0;
+    for (int k = 0; k < RS_WAVES; k++) wcnt[k][tid] = 0;
+    gbase[tid] = offsets[(int64_t)tid * ntiles + blockIdx.x];
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-    for (int r = 0; r < RS_IT; r++) {
-        const int64_t i = base + r * RS_BS + tid;
+    const int64_t tbase = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t wbase = tbase + (int64_t)w * RS_WKEYS;
+    uint64_t key[RS_ROUNDS];
+    uint32_t val[RS_ROUNDS];
+    int rank[RS_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        const int64_t i = wbase + r * WAVE + lane;
         const bool valid = i < n;
-        uint64_t key = valid ? kin[i] : 0;
-        uint32_t val = valid ? vin[i] : 0;
-        const int d = (int)((key >> shift) & (RADIX - 1));
+        key[r] = valid ? kin[i] : 0;
+        val[r] = valid ? vin[i] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        const int64_t i = wbase + r * WAVE + lane;
+        const bool valid = i < n;
+        const int d = (int)((key[r] >> shift) & (RADIX - 1));
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -161,24 +194,47 @@ __global__ __launch_bounds__(RS_BS) void k_rs_scatter(const uint64_t* __restrict
             const uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
         }
-        const int rank = __popcll(peers & lt);
-        if (valid && rank == 0) wcnt[w][d] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            int pos = running[d] + rank;
-            for (int k = 0; k < w; k++) pos += wcnt[k][d];
-            kout[pos] = key;
-            vout[pos] = val;
-        }
-        __syncthreads();
-        int add = 0;
+        const int below = __popcll(peers & lt);
+        int before = 0;
+        if (valid) before = wcnt[w][d];
+        rank[r] = valid ? before + below : -1;
+        // all lanes have read the running count before the leader bumps it
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == 0) wcnt[w][d] = before + __popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // per digit: exclusive over waves, and the tile total
+    int tot = 0;
 #pragma unroll
-        for (int k = 0; k < RS_BS / WAVE; k++) {
-            add += wcnt[k][tid];
-            wcnt[k][tid] = 0;
+    for (int k = 0; k < RS_WAVES; k++) {
+        const int c = wcnt[k][tid];
+        wcnt[k][tid] = tot;
+        tot += c;
+    }
+    int all;
+    const int st = block_excl_scan(tot, &all, scan_lds);
+    dstart[tid] = st;
+    gbase[tid] -= st;  // global position = gbase[d] + tile-local position
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        if (rank[r] >= 0) {
+            const int d = (int)((key[r] >> shift) & (RADIX - 1));
+            const int pos = dstart[d] + wcnt[w][d] + rank[r];
+            sk[pos] = key[r];
+            sv[pos] = val[r];
         }
-        running[tid] += add;
-        __syncthreads();
+    }
+    __syncthreads();
+    const int64_t tn = (n - tbase) < RS_TILE ? (n - tbase) : RS_TILE;
+#pragma unroll 4
+    for (int i = tid; i < tn; i += RS_BS) {
+        const uint64_t k = sk[i];
+        const int d = (int)((k >> shift) & (RADIX - 1));
+        const int64_t g = (int64_t)gbase[d] + i;
+        kout[g] = k;
+        vout[g] = sv[i];
     }
 }
 
