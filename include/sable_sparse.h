@@ -250,6 +250,13 @@ SABLE_API int sable_coo_csc(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, in
                             const int32_t* col, const void* val, int32_t* out_col_ptr, int32_t* out_row,
                             void* out_val);
 
+/* Sorts a COO in place by (row, col), stably (equal keys keep their order).
+ * For distinct (row, col) keys this is the order of quick_sort_COO
+ * (seq/src/sparse32.js:1577-1609), used by the in-place COO transpose
+ * (seq/src/sparse32.js:285-295) and the MatrixMarket reader. */
+SABLE_API int sable_coo_sort(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int64_t nnz, int32_t* row,
+                             int32_t* col, void* val);
+
 /* CSR transpose (counting sort by column, rows ascending within each new
  * row; values moved bitwise).  Replaces transpose_csr
  * (seq/src/sparse_opt_32.wat:6348-6477, caller seq/src/sparse32.js:1116-1120). */
@@ -274,6 +281,91 @@ SABLE_API int sable_count_transpose_cols_ell(sable_ctx_t* ctx, sable_dtype_t dty
  * (seq/src/sparse_opt_32.wat:6634-6701). */
 SABLE_API int sable_transpose_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc, int32_t nc_out,
                                   const void* data, const int32_t* indices, void* out_data, int32_t* out_indices);
+
+/* ------------------------------------------------------------ element-wise ops */
+/* Operation on each stored value (seq/src/sparse32.js:671-1105 out of
+ * place, :200-660 in place).  expm1, log1p, sin, tan and pow run in double
+ * precision and round to the value type, as the reference's WebAssembly
+ * calls into JavaScript's Math (pow with Math.pow's NaN cases); the rest are
+ * the WebAssembly instructions. */
+typedef enum {
+    SABLE_EW_EXPM1 = 0,
+    SABLE_EW_LOG1P = 1,
+    SABLE_EW_SIN = 2,
+    SABLE_EW_TAN = 3,
+    SABLE_EW_POW = 4,      /* param = exponent p */
+    SABLE_EW_DEG2RAD = 5,  /* param = pi: v * (pi / 180), pi / 180 rounded in the value type */
+    SABLE_EW_RAD2DEG = 6,  /* param = pi: v / (pi / 180) */
+    SABLE_EW_SIGN = 7,     /* v > 0 ? 1 : -1 (the COO/CSR exports) */
+    SABLE_EW_SIGN0 = 8,    /* v == 0 ? 0 : (v > 0 ? 1 : -1) (the DIA/ELL exports) */
+    SABLE_EW_ABS = 9,
+    SABLE_EW_NEG = 10,
+    SABLE_EW_SQRT = 11,
+    SABLE_EW_CEIL = 12,
+    SABLE_EW_FLOOR = 13,
+    SABLE_EW_TRUNC = 14,
+    SABLE_EW_NEAREST = 15, /* round half to even */
+    SABLE_EW_MULTIPLY = 16 /* param = scalar */
+} sable_ew_op_t;
+
+/* out[i] = op(in[i]) for i < n: COO/CSR values (n = nnz) or all ELL slots
+ * (n = ncols*N).  in == out is the in-place form.  Replaces <op>_coo,
+ * self_<op>_coo, <op>_ell, self_<op>_ell and self_multiply_scalar_coo
+ * (seq/src/sparse_opt_32.wat:597-1699, 4977-6347; sparse_opt_64.wat). */
+SABLE_API int sable_ew_values(sable_ctx_t* ctx, sable_dtype_t dtype, int op, double param, int64_t n, const void* in,
+                              void* out);
+/* DIA: op on the in-range slots of every diagonal only (rows max(0,-k) to
+ * min(N, N-k)), restricted to rows [r0, r1) (0, N for the whole matrix);
+ * padding slots of out are not written.  Replaces <op>_dia / self_<op>_dia
+ * (seq/src/sparse_opt_32.wat:2137-4976) and the worker-range forms
+ * (par/src/sparse_opt_32.wat, e.g. self_expm1_dia(id, offset, data,
+ * start_row, end_row, ...)). */
+SABLE_API int sable_ew_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int op, double param, int32_t N, int32_t nd,
+                           int32_t stride, const int32_t* offset, int32_t r0, int32_t r1, const void* in, void* out);
+/* ELL rows [r0, r1) of every slot column (column-major slots k*N + i).
+ * Replaces the parallel tree's <op>_ell / self_<op>_ell worker forms
+ * (par/src/sparse_opt_32.wat, (id, [scalar,] data, start_row, end_row,
+ * ncols, N)); sable_ew_values covers the serial form. */
+SABLE_API int sable_ew_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int op, double param, int32_t N, int32_t nc,
+                           int32_t r0, int32_t r1, const void* in, void* out);
+
+/* Number of values v with v > 0 || v < 0 (synchronises).  Replaces
+ * count_nonzeros_coo (seq/src/sparse_opt_32.wat:500-521). */
+SABLE_API int sable_count_nonzeros(sable_ctx_t* ctx, sable_dtype_t dtype, int64_t n, const void* val, int64_t* count);
+/* COO without its zero (and NaN) values, order kept; outputs hold *count
+ * entries (size them with sable_count_nonzeros).  Replaces copy_nonzeros_coo
+ * (seq/src/sparse_opt_32.wat:523-546), eliminate_zeros (seq/src/sparse32.js:
+ * 1140-1155). */
+SABLE_API int sable_eliminate_zeros_coo(sable_ctx_t* ctx, sable_dtype_t dtype, int64_t nnz, const int32_t* row,
+                                        const int32_t* col, const void* val, int32_t* out_row, int32_t* out_col,
+                                        void* out_val, int64_t* count);
+/* CSR without its zero values: out_row_ptr (N+1), out_col/out_val (*count).
+ * Replaces copy_nonzeros_csr (seq/src/sparse_opt_32.wat:548-595; its outer
+ * loop also visits a row N past the end, which is not repeated here). */
+SABLE_API int sable_eliminate_zeros_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                                        const int32_t* col, const void* val, int32_t* out_row_ptr, int32_t* out_col,
+                                        void* out_val, int64_t* count);
+/* DIA: diagonals with a nonzero in range, and nonzeros in range
+ * (synchronises).  Replaces count_nonzeros_dia (:1948-2039). */
+SABLE_API int sable_count_nonzeros_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
+                                       const int32_t* offset, const void* data, int32_t* count_diags, int64_t* count);
+/* DIA keeping the diagonals that hold a nonzero (offsets in order) in
+ * count_diags*stride zero-filled slots; as the reference does, the kept
+ * diagonal's nonzeros are packed in row order from the slot of its first
+ * row (zeros inside a diagonal are squeezed out, moving later values up).
+ * Replaces copy_nonzeros_dia (:2041-2135). */
+SABLE_API int sable_eliminate_zeros_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
+                                        const int32_t* offset, const void* data, int32_t* out_offset, void* out_data,
+                                        int32_t* count_diags);
+/* ELL: the largest per-row count of nonzero slots, and the total
+ * (synchronises).  Replaces count_nonzeros_ell (:5821-5893). */
+SABLE_API int sable_count_nonzeros_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc, const void* data,
+                                       int32_t* count_cols, int64_t* count);
+/* ELL of width nc_out (= count_cols): each row's nonzero slots in slot
+ * order, the rest zero.  Replaces copy_nonzeros_ell (:5895-5949). */
+SABLE_API int sable_eliminate_zeros_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc, int32_t nc_out,
+                                        const void* data, const int32_t* indices, void* out_data,
+                                        int32_t* out_indices);
 
 /* ------------------------------------------------------------ MatrixMarket ingest */
 /* Field of the banner (token 4 of "%%MatrixMarket matrix coordinate <field>

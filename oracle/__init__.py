@@ -337,3 +337,104 @@ def mm_coo(text, dtype=np.float32, lower=False):
     info = dict(nrows=int(info6[0]), ncols=int(info6[1]), nentries=int(info6[2]), nnz=nnz,
                 pattern=bool(info6[4]), symmetric=bool(info6[5]))
     return info, row[:nnz], col[:nnz], val[:nnz]
+
+
+# ------------------------------------------------------------------ element-wise
+EW_OPS = ["expm1", "log1p", "sin", "tan", "pow", "deg2rad", "rad2deg", "sign", "sign0", "abs", "neg", "sqrt", "ceil",
+          "floor", "trunc", "nearest", "multiply"]
+
+
+def _ewlib(name, suf):
+    L = lib()
+    fn = getattr(L, "oracle_" + name + "_" + suf)
+    return fn
+
+
+def ew_values(op, param, v):
+    """<op>_coo / <op>_ell (seq/src/sparse_opt_32.wat:597-1699, 4977-6347)."""
+    v = np.ascontiguousarray(v)
+    out = np.empty_like(v)
+    fn = _ewlib("ew_values", _suf(v))
+    fn.argtypes = [_I, ctypes.c_double, ctypes.c_int64, _VP, _VP]
+    fn(EW_OPS.index(op) if isinstance(op, str) else op, float(param), v.size, v.ctypes.data, out.ctypes.data)
+    return out
+
+
+def ew_dia(op, param, N, stride, offset, data, out=None):
+    """<op>_dia (seq/src/sparse_opt_32.wat:2137-4976): in-range slots only;
+    other slots of out are left as given (zeros by default)."""
+    data = np.ascontiguousarray(data)
+    out = np.zeros_like(data) if out is None else np.ascontiguousarray(out).copy()
+    fn = _ewlib("ew_dia", _suf(data))
+    fn.argtypes = [_I, ctypes.c_double, _I, _I, _I, _I32P, _VP, _VP]
+    fn(EW_OPS.index(op) if isinstance(op, str) else op, float(param), N, len(offset), stride, _i32(offset),
+       data.ctypes.data, out.ctypes.data)
+    return out
+
+
+def count_nonzeros(v):
+    v = np.ascontiguousarray(v)
+    fn = _ewlib("count_nonzeros", _suf(v))
+    fn.argtypes = [ctypes.c_int64, _VP]
+    fn.restype = ctypes.c_int64
+    return int(fn(v.size, v.ctypes.data))
+
+
+def copy_nonzeros_coo(row, col, val):
+    val = np.ascontiguousarray(val)
+    c = count_nonzeros(val)
+    orow, ocol, oval = np.zeros(c, np.int32), np.zeros(c, np.int32), np.zeros(c, val.dtype)
+    fn = _ewlib("copy_nonzeros_coo", _suf(val))
+    fn.argtypes = [ctypes.c_int64, _I32P, _I32P, _VP, _VP, _VP, _VP]
+    fn(val.size, _i32(row), _i32(col), val.ctypes.data, orow.ctypes.data, ocol.ctypes.data, oval.ctypes.data)
+    return orow, ocol, oval
+
+
+def copy_nonzeros_csr(N, rowptr, col, val):
+    val = np.ascontiguousarray(val)
+    c = count_nonzeros(val)
+    orp, ocol, oval = np.zeros(N + 1, np.int32), np.zeros(c, np.int32), np.zeros(c, val.dtype)
+    fn = _ewlib("copy_nonzeros_csr", _suf(val))
+    fn.argtypes = [_I, _I32P, _I32P, _VP, _VP, _VP, _VP]
+    fn(N, _i32(rowptr), _i32(col), val.ctypes.data, orp.ctypes.data, ocol.ctypes.data, oval.ctypes.data)
+    return orp, ocol, oval
+
+
+def count_nonzeros_dia(N, stride, offset, data):
+    data = np.ascontiguousarray(data)
+    out2 = np.zeros(2, np.int64)
+    fn = _ewlib("count_nonzeros_dia", _suf(data))
+    fn.argtypes = [_I, _I, _I, _I32P, _VP, _VP]
+    fn(N, len(offset), stride, _i32(offset), data.ctypes.data, out2.ctypes.data)
+    return int(out2[0]), int(out2[1])
+
+
+def copy_nonzeros_dia(N, stride, offset, data):
+    data = np.ascontiguousarray(data)
+    cd, _ = count_nonzeros_dia(N, stride, offset, data)
+    ooff = np.zeros(cd, np.int32)
+    odata = np.zeros(cd * stride, data.dtype)
+    fn = _ewlib("copy_nonzeros_dia", _suf(data))
+    fn.argtypes = [_I, _I, _I, _I32P, _VP, _VP, _VP, ctypes.c_int64]
+    fn(N, len(offset), stride, _i32(offset), data.ctypes.data, ooff.ctypes.data, odata.ctypes.data, odata.size)
+    return ooff, odata
+
+
+def count_nonzeros_ell(N, nc, data):
+    data = np.ascontiguousarray(data)
+    out2 = np.zeros(2, np.int64)
+    fn = _ewlib("count_nonzeros_ell", _suf(data))
+    fn.argtypes = [_I, _I, _VP, _VP]
+    fn(N, nc, data.ctypes.data, out2.ctypes.data)
+    return int(out2[0]), int(out2[1])
+
+
+def copy_nonzeros_ell(N, nc, data, idx):
+    data = np.ascontiguousarray(data)
+    cc, _ = count_nonzeros_ell(N, nc, data)
+    odata = np.zeros(cc * N, data.dtype)
+    oidx = np.zeros(cc * N, np.int32)
+    fn = _ewlib("copy_nonzeros_ell", _suf(data))
+    fn.argtypes = [_I, _I, _I, _VP, _I32P, _VP, _VP]
+    fn(N, nc, cc, data.ctypes.data, _i32(idx), odata.ctypes.data, oidx.ctypes.data)
+    return cc, odata, oidx

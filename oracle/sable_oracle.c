@@ -895,3 +895,167 @@ static int oracle_mm_read_info(const char* text, size_t len, mm_info_t* m) {
 
 DEF_MM_COO(float, f32)
 DEF_MM_COO(double, f64)
+
+/* ================================================================ element-wise
+ * seq/src/sparse_opt_32.wat / sparse_opt_64.wat exports behind
+ * seq/src/sparse32.js:200-1207.  Op codes follow include/sable_sparse.h
+ * (sable_ew_op_t).  Transcendental ops: JavaScript Math in double on the
+ * widened value, rounded back (the module imports Math.expm1 ... Math.pow,
+ * seq/src/sparse32.js:8); Math.pow differs from C pow for a NaN exponent
+ * and for |x| == 1 with an infinite exponent (both NaN). */
+static double js_pow(double x, double y) {
+    if (isnan(y)) return NAN;
+    if (fabs(x) == 1.0 && isinf(y)) return NAN;
+    return pow(x, y);
+}
+
+#define DEF_EW(T, SUF)                                                                                            \
+    static T ew_##SUF(int op, T param, T v) {                                                                     \
+        switch (op) {                                                                                             \
+            case 0: return (T)expm1((double)v);                                                                   \
+            case 1: return (T)log1p((double)v);                                                                   \
+            case 2: return (T)sin((double)v);                                                                     \
+            case 3: return (T)tan((double)v);                                                                     \
+            case 4: return (T)js_pow((double)v, (double)param);                                                   \
+            case 5: return v * param;                 /* deg2rad: param = pi/180 in T */                         \
+            case 6: return v / param;                 /* rad2deg */                                               \
+            case 7: return v > 0 ? (T)1 : (T)-1;      /* sign, COO/CSR exports (:1137-1183) */                   \
+            case 8: return v == 0 ? (T)0 : (v > 0 ? (T)1 : (T)-1); /* sign, DIA/ELL exports (:2879-2960) */      \
+            case 9: return (T)fabs((double)v);                                                                    \
+            case 10: return -v;                                                                                   \
+            case 11: return (sizeof(T) == 4) ? (T)sqrtf((float)v) : (T)sqrt((double)v);                           \
+            case 12: return (sizeof(T) == 4) ? (T)ceilf((float)v) : (T)ceil((double)v);                           \
+            case 13: return (sizeof(T) == 4) ? (T)floorf((float)v) : (T)floor((double)v);                         \
+            case 14: return (sizeof(T) == 4) ? (T)truncf((float)v) : (T)trunc((double)v);                         \
+            case 15: return (sizeof(T) == 4) ? (T)rintf((float)v) : (T)rint((double)v); /* f32.nearest */        \
+            case 16: return v * param;                /* self_multiply_scalar_coo (:788-830) */                  \
+            default: return v;                                                                                    \
+        }                                                                                                         \
+    }                                                                                                             \
+    static T ew_param_##SUF(int op, double param) {                                                               \
+        /* pi / 180 rounded in T (f32.div / f64.div at :683-700) */                                               \
+        if (op == 5 || op == 6) return (T)((T)param / (T)180);                                                    \
+        return (T)param;                                                                                          \
+    }                                                                                                             \
+    /* <op>_coo, self_<op>_coo, <op>_ell, self_<op>_ell: every value */                                           \
+    EXPORT void oracle_ew_values_##SUF(int op, double param, int64_t n, const T* in, T* out) {                    \
+        const T p = ew_param_##SUF(op, param);                                                                    \
+        for (int64_t i = 0; i < n; ++i) out[i] = ew_##SUF(op, p, in[i]);                                          \
+    }                                                                                                             \
+    /* <op>_dia, self_<op>_dia: rows max(0,-k) .. min(N, N-k) of each diagonal (:2137-2207) */                    \
+    EXPORT void oracle_ew_dia_##SUF(int op, double param, int N, int nd, int stride, const int32_t* offset,         \
+                                    const T* in, T* out) {                                                        \
+        const T p = ew_param_##SUF(op, param);                                                                    \
+        for (int d = 0; d < nd; ++d) {                                                                            \
+            const int k = offset[d];                                                                              \
+            const int64_t base = (int64_t)d * stride - (k < 0 ? (int64_t)(N - stride) : 0);                       \
+            const int lo = k < 0 ? -k : 0, hi = k > 0 ? N - k : N;                                                \
+            for (int r = lo; r < hi; ++r) out[base + r] = ew_##SUF(op, p, in[base + r]);                          \
+        }                                                                                                         \
+    }                                                                                                             \
+    /* count_nonzeros_coo (:500-521): v > 0 || v < 0 */                                                            \
+    EXPORT int64_t oracle_count_nonzeros_##SUF(int64_t n, const T* v) {                                           \
+        int64_t c = 0;                                                                                            \
+        for (int64_t i = 0; i < n; ++i) c += (v[i] > 0 || v[i] < 0);                                              \
+        return c;                                                                                                 \
+    }                                                                                                             \
+    /* copy_nonzeros_coo (:523-546) */                                                                             \
+    EXPORT void oracle_copy_nonzeros_coo_##SUF(int64_t n, const int32_t* row, const int32_t* col, const T* val,     \
+                                               int32_t* orow, int32_t* ocol, T* oval) {                           \
+        int64_t o = 0;                                                                                            \
+        for (int64_t i = 0; i < n; ++i)                                                                           \
+            if (val[i] > 0 || val[i] < 0) {                                                                       \
+                orow[o] = row[i];                                                                                 \
+                ocol[o] = col[i];                                                                                 \
+                oval[o] = val[i];                                                                                 \
+                ++o;                                                                                              \
+            }                                                                                                     \
+    }                                                                                                             \
+    /* copy_nonzeros_csr (:548-595), rows 0..N-1 */                                                                \
+    EXPORT void oracle_copy_nonzeros_csr_##SUF(int N, const int32_t* rp, const int32_t* col, const T* val,        \
+                                               int32_t* orp, int32_t* ocol, T* oval) {                            \
+        int64_t o = 0;                                                                                            \
+        orp[0] = 0;                                                                                               \
+        for (int i = 0; i < N; ++i) {                                                                             \
+            for (int j = rp[i]; j < rp[i + 1]; ++j)                                                               \
+                if (val[j] > 0 || val[j] < 0) {                                                                   \
+                    ocol[o] = col[j];                                                                             \
+                    oval[o] = val[j];                                                                             \
+                    ++o;                                                                                          \
+                }                                                                                                 \
+            orp[i + 1] = (int32_t)o;                                                                              \
+        }                                                                                                         \
+    }                                                                                                             \
+    /* count_nonzeros_dia (:1948-2039) -> out2 = {count_diags, count} */                                          \
+    EXPORT void oracle_count_nonzeros_dia_##SUF(int N, int nd, int stride, const int32_t* offset, const T* data,   \
+                                                int64_t* out2) {                                                  \
+        int64_t cd = 0, nnz = 0;                                                                                  \
+        for (int d = 0; d < nd; ++d) {                                                                            \
+            const int k = offset[d];                                                                              \
+            const int64_t base = (int64_t)d * stride - (k < 0 ? (int64_t)(N - stride) : 0);                       \
+            const int lo = k < 0 ? -k : 0, hi = k > 0 ? N - k : N;                                                \
+            int64_t c = 0;                                                                                        \
+            for (int r = lo; r < hi; ++r) c += (data[base + r] > 0 || data[base + r] < 0);                        \
+            if (c) {                                                                                              \
+                nnz += c;                                                                                         \
+                ++cd;                                                                                             \
+            }                                                                                                     \
+        }                                                                                                         \
+        out2[0] = cd;                                                                                             \
+        out2[1] = nnz;                                                                                            \
+    }                                                                                                             \
+    /* copy_nonzeros_dia (:2041-2135): the j-th kept diagonal's nonzeros, in row order, from slot               \
+     * j*stride + (first row) onwards (for stride < N, k < 0, the slot of the first row in that layout);        \
+     * out_data zero-filled first as eliminate_zeros does (seq/src/sparse32.js:1176-1179) */                     \
+    EXPORT void oracle_copy_nonzeros_dia_##SUF(int N, int nd, int stride, const int32_t* offset, const T* data,    \
+                                               int32_t* ooff, T* odata, int64_t out_slots) {                      \
+        for (int64_t i = 0; i < out_slots; ++i) odata[i] = 0;                                                     \
+        int64_t j = 0;                                                                                            \
+        for (int d = 0; d < nd; ++d) {                                                                            \
+            const int k = offset[d];                                                                              \
+            const int64_t shift = (k < 0 ? (int64_t)(N - stride) : 0);                                            \
+            const int64_t base = (int64_t)d * stride - shift;                                                     \
+            const int lo = k < 0 ? -k : 0, hi = k > 0 ? N - k : N;                                                \
+            int64_t o = j * stride - shift + lo, c = 0;                                                           \
+            for (int r = lo; r < hi; ++r)                                                                         \
+                if (data[base + r] > 0 || data[base + r] < 0) {                                                   \
+                    odata[o++] = data[base + r];                                                                  \
+                    ++c;                                                                                          \
+                }                                                                                                 \
+            if (c) ooff[j++] = k;                                                                                 \
+        }                                                                                                         \
+    }                                                                                                             \
+    /* count_nonzeros_ell (:5821-5893) -> out2 = {count_cols, count} */                                           \
+    EXPORT void oracle_count_nonzeros_ell_##SUF(int N, int nc, const T* data, int64_t* out2) {                    \
+        int64_t cc = 0, nnz = 0;                                                                                  \
+        for (int i = 0; i < N; ++i) {                                                                             \
+            int64_t c = 0;                                                                                        \
+            for (int k = 0; k < nc; ++k) c += (data[(int64_t)k * N + i] > 0 || data[(int64_t)k * N + i] < 0);     \
+            if (c > cc) cc = c;                                                                                   \
+            nnz += c;                                                                                             \
+        }                                                                                                         \
+        out2[0] = cc;                                                                                             \
+        out2[1] = nnz;                                                                                            \
+    }                                                                                                             \
+    /* copy_nonzeros_ell (:5895-5949) into zero-filled nc_out-wide arrays (seq/src/sparse32.js:1196-1203) */      \
+    EXPORT void oracle_copy_nonzeros_ell_##SUF(int N, int nc, int nc_out, const T* data, const int32_t* idx,       \
+                                               T* odata, int32_t* oidx) {                                         \
+        for (int64_t i = 0; i < (int64_t)nc_out * N; ++i) {                                                       \
+            odata[i] = 0;                                                                                         \
+            oidx[i] = 0;                                                                                          \
+        }                                                                                                         \
+        for (int i = 0; i < N; ++i) {                                                                             \
+            int o = 0;                                                                                            \
+            for (int k = 0; k < nc; ++k) {                                                                        \
+                const T v = data[(int64_t)k * N + i];                                                             \
+                if (v > 0 || v < 0) {                                                                             \
+                    odata[(int64_t)o * N + i] = v;                                                                \
+                    oidx[(int64_t)o * N + i] = idx[(int64_t)k * N + i];                                           \
+                    ++o;                                                                                          \
+                }                                                                                                 \
+            }                                                                                                     \
+        }                                                                                                         \
+    }
+
+DEF_EW(float, f32)
+DEF_EW(double, f64)

@@ -16,6 +16,8 @@ from .spts import (SptsAnalysis, CSR_create_level_sets, CSR_create_level_sets_wi
                    spts_level_csr, spts_syncfree, spts_csc)
 from .serve import Server, batch_spmv, batch_spts, batch_spts_refined, lower_dominant  # noqa: F401
 from .mm import MMInfo, read_matrix_MM_files, create_COO_from_MM, create_LCOO_from_MM, mmread  # noqa: F401
+from .ewise import (expm1, log1p, sin, tan, pow, deg2rad, rad2deg, sign, abs, neg, sqrt, ceil, floor,  # noqa: F401
+                    trunc, nearest, eliminate_zeros, count_nonzeros)
 
 __all__ = [
     "Context", "DeviceArray", "SableError",
@@ -27,4 +29,6 @@ __all__ = [
     "SptsAnalysis", "CSR_create_level_sets", "CSR_create_level_sets_with_reorder", "spts_level_csr",
     "spts_syncfree", "spts_csc", "Server", "batch_spmv", "batch_spts", "batch_spts_refined", "lower_dominant",
     "MMInfo", "read_matrix_MM_files", "create_COO_from_MM", "create_LCOO_from_MM", "mmread",
+    "expm1", "log1p", "sin", "tan", "pow", "deg2rad", "rad2deg", "sign", "abs", "neg", "sqrt", "ceil", "floor",
+    "trunc", "nearest", "eliminate_zeros", "count_nonzeros",
 ]

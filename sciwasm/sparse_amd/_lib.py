@@ -106,6 +106,18 @@ SIGNATURES = {
     "sable_spmv_ell": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_coo_csr": [c_vp, c_int, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "sable_coo_csc": [c_vp, c_int, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "sable_coo_sort": [c_vp, c_int, c_i32, c_i64, c_vp, c_vp, c_vp],
+    "sable_ew_values": [c_vp, c_int, c_int, c_dbl, c_i64, c_vp, c_vp],
+    "sable_ew_dia": [c_vp, c_int, c_int, c_dbl, c_i32, c_i32, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp],
+    "sable_ew_ell": [c_vp, c_int, c_int, c_dbl, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp],
+    "sable_count_nonzeros": [c_vp, c_int, c_i64, c_vp, ctypes.POINTER(c_i64)],
+    "sable_eliminate_zeros_coo": [c_vp, c_int, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_i64)],
+    "sable_eliminate_zeros_csr": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_i64)],
+    "sable_count_nonzeros_dia": [c_vp, c_int, c_i32, c_i32, c_i32, c_vp, c_vp, ctypes.POINTER(c_i32),
+                                 ctypes.POINTER(c_i64)],
+    "sable_eliminate_zeros_dia": [c_vp, c_int, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_i32)],
+    "sable_count_nonzeros_ell": [c_vp, c_int, c_i32, c_i32, c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i64)],
+    "sable_eliminate_zeros_ell": [c_vp, c_int, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp],
     "sable_mm_header": [c_vp, c_sz, ctypes.POINTER(MMInfo)],
     "sable_mm_parse": [c_vp, c_vp, c_sz, c_int, c_vpp],
     "sable_mm_get_info": [c_vp, ctypes.POINTER(MMInfo)],

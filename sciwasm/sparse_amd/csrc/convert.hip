@@ -389,6 +389,50 @@ static int transpose_csr_t(sable_ctx_t* ctx, int32_t N, const int32_t* rp, const
 
 using namespace sable;
 
+// Sorted COO in place: row and column from the sorted keys, values gathered.
+template <typename T>
+__global__ void k_coo_unkey(int64_t nnz, int colbits, const uint64_t* __restrict__ keys,
+                            const uint32_t* __restrict__ idx, const T* __restrict__ val, int32_t* __restrict__ orow,
+                            int32_t* __restrict__ ocol, T* __restrict__ oval) {
+    const uint64_t mask = (1ull << colbits) - 1;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+        orow[p] = (int32_t)(keys[p] >> colbits);
+        ocol[p] = (int32_t)(keys[p] & mask);
+        oval[p] = val[idx[p]];
+    }
+}
+
+template <typename T>
+static int coo_sort_t(sable_ctx_t* ctx, int32_t N, int64_t nnz, int32_t* row, int32_t* col, void* val) {
+    hipStream_t st = ctx->stream;
+    const int bits = bits_for(N > 0 ? N - 1 : 0);
+    size_t need = Carver::round(8 * nnz) + Carver::round(4 * nnz) + Carver::round(sizeof(T) * nnz) + 256 +
+                  sort_tmp_bytes(nnz);
+    void* s;
+    int rc = ctx_scratch(ctx, need, &s);
+    if (rc) return rc;
+    Carver cv(s);
+    uint64_t* keys = cv.take<uint64_t>(nnz);
+    uint32_t* idx = cv.take<uint32_t>(nnz);
+    T* vcopy = cv.take<T>(nnz);
+    int* err = cv.take<int>(1);
+    void* stmp = cv.p + cv.used;
+    SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_coo_keys, dim3(gs(nnz)), dim3(CB), 0, st, nnz, N, bits, row, col, keys, idx, err);
+    SABLE_LAUNCH_CHECK();
+    int e = 0;
+    rc = read_err(ctx, err, &e);
+    if (rc) return rc;
+    SABLE_CHECK(!e, SABLE_EMALFORMED, "COO index outside [0, N)");
+    rc = sort_pairs_u64(st, keys, idx, nnz, 2 * bits, stmp);
+    if (rc) return rc;
+    SABLE_HIP(hipMemcpyAsync(vcopy, val, sizeof(T) * nnz, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_coo_unkey<T>, dim3(gs(nnz)), dim3(CB), 0, st, nnz, bits, keys, idx, (const T*)vcopy, row,
+                       col, (T*)val);
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
 #define DT_DISPATCH(dtype, FN, ...)                                                   \
     ((dtype) == SABLE_F32 ? FN<float>(__VA_ARGS__)                                   \
                           : (dtype) == SABLE_F64 ? FN<double>(__VA_ARGS__)           \
@@ -410,6 +454,15 @@ SABLE_API int sable_coo_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, in
     SABLE_CHECK(N >= 0 && nnz >= 0, SABLE_EINVAL, "negative size");
     SABLE_CHECK(nnz <= 0x7fffffffLL, SABLE_EINVAL, "nnz exceeds the int32 row_ptr range");
     return DT_DISPATCH(dtype, coo_csr_t, ctx, N, nnz, row, col, val, out_row_ptr, out_col, out_val);
+}
+
+SABLE_API int sable_coo_sort(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int64_t nnz, int32_t* row,
+                             int32_t* col, void* val) {
+    SABLE_CHECK(ctx, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(N >= 0 && nnz >= 0 && nnz < 0xffffffffLL, SABLE_EINVAL, "bad size");
+    if (nnz <= 1) return SABLE_OK;
+    SABLE_CHECK(row && col && val, SABLE_EINVAL, "NULL array");
+    return DT_DISPATCH(dtype, coo_sort_t, ctx, N, nnz, row, col, val);
 }
 
 SABLE_API int sable_transpose_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,

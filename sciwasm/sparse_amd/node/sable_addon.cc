@@ -205,6 +205,12 @@ struct Args {
         return r;
     }
     int64_t id() const { return base ? Args{env, v, 0}.i(0) : 0; }
+    double d(size_t k) const {
+        double r = 0;
+        k += base;
+        if (k >= v.size() || napi_get_value_double(env, v[k], &r) != napi_ok) throw Err{"expected a number argument"};
+        return r;
+    }
 };
 
 // Module flavour, carried as the callback data of every export.
@@ -920,6 +926,315 @@ napi_value par_sum(napi_env env, napi_callback_info info) {
     });
 }
 
+// ------------------------------------------------------------ element-wise exports
+// <op>_coo / self_<op>_coo, <op>_dia / self_<op>_dia, <op>_ell /
+// self_<op>_ell (seq/src/sparse_opt_32.wat:597-6347; the parallel tree's
+// forms take a leading worker id and, for DIA and ELL, a row range:
+// par/src/sparse_opt_32.wat).  Signatures:
+//   COO  [scalar,] in_val, out_val, len            self: [scalar,] val, len
+//   DIA  [scalar,] offset, in, out, [r0, r1,] ndiags, stride, N
+//   ELL  [scalar,] in, out, [r0, r1,] ncols, N
+// where scalar is p (pow), pi (deg2rad, rad2deg) or the multiplier.
+enum EwFmt { EW_COO = 0, EW_DIA = 1, EW_ELL = 2 };
+struct EwDesc {
+    Prec* P;
+    EwFmt fmt;
+    int op;
+    bool self;
+    bool scalar;
+};
+
+napi_value ew_export(napi_env env, napi_callback_info info) {
+    void* data = nullptr;
+    napi_get_cb_info(env, info, nullptr, nullptr, nullptr, &data);
+    const EwDesc* D = (const EwDesc*)data;
+    Prec* P = D->P;
+    return guarded(env, [&] {
+        const bool par = P->lead != 0;
+        size_t n = D->scalar ? 1 : 0;
+        if (D->fmt == EW_COO) n += D->self ? 2 : 3;
+        else if (D->fmt == EW_DIA) n += (D->self ? 5 : 6) + (par ? 2 : 0);
+        else n += (D->self ? 3 : 4) + (par ? 2 : 0);
+        size_t argc = n + P->lead;
+        std::vector<napi_value> v(argc);
+        napi_get_cb_info(env, info, &argc, v.data(), nullptr, nullptr);
+        if (argc < n + P->lead) throw Err{"too few arguments"};
+        Args a{env, v, P->lead};
+        size_t k = 0;
+        const double param = D->scalar ? a.d(k++) : 0.0;
+        ensure_ctx();
+        if (D->fmt == EW_COO) {
+            const size_t in = a.i(k++);
+            const size_t out = D->self ? in : (size_t)a.i(k++);
+            const int64_t len = a.i(k++);
+            if (len <= 0) return undef(env);
+            const size_t bytes = P->vs * (size_t)len;
+            void* din = scratch(in, bytes, true);
+            void* dout = D->self ? din : scratch(out, bytes, false);
+            check(sable_ew_values(g_ctx, P->dt, D->op, param, len, din, dout), "element-wise");
+            drop_mirrors_in(out, bytes);
+            download(out, dout, bytes);
+        } else if (D->fmt == EW_DIA) {
+            const size_t off = a.i(k++), in = a.i(k++);
+            const size_t out = D->self ? in : (size_t)a.i(k++);
+            int64_t r0 = 0, r1 = 0;
+            if (par) {
+                r0 = a.i(k++);
+                r1 = a.i(k++);
+            }
+            const int32_t nd = (int32_t)a.i(k++), stride = (int32_t)a.i(k++), N = (int32_t)a.i(k++);
+            if (!par) r1 = N;
+            if (nd <= 0 || N <= 0 || r0 >= r1) return undef(env);
+            const size_t bytes = P->vs * (size_t)nd * stride;
+            void* doff = mirror(off, 4 * (size_t)nd);
+            void* din = scratch(in, bytes, true);
+            // out is uploaded too: slots outside the diagonals' rows keep their bytes
+            void* dout = D->self ? din : scratch(out, bytes, true);
+            check(sable_ew_dia(g_ctx, P->dt, D->op, param, N, nd, stride, (const int32_t*)doff, (int32_t)r0,
+                               (int32_t)r1, din, dout),
+                  "element-wise");
+            drop_mirrors_in(out, bytes);
+            download(out, dout, bytes);
+        } else {
+            const size_t in = a.i(k++);
+            const size_t out = D->self ? in : (size_t)a.i(k++);
+            int64_t r0 = 0, r1 = 0;
+            if (par) {
+                r0 = a.i(k++);
+                r1 = a.i(k++);
+            }
+            const int32_t nc = (int32_t)a.i(k++), N = (int32_t)a.i(k++);
+            if (!par) r1 = N;
+            if (nc <= 0 || N <= 0 || r0 >= r1) return undef(env);
+            const size_t bytes = P->vs * (size_t)nc * N;
+            void* din = scratch(in, bytes, true);
+            void* dout = D->self ? din : scratch(out, bytes, true);
+            check(sable_ew_ell(g_ctx, P->dt, D->op, param, N, nc, (int32_t)r0, (int32_t)r1, din, dout),
+                  "element-wise");
+            drop_mirrors_in(out, bytes);
+            download(out, dout, bytes);
+        }
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// count_nonzeros_coo(val, nnz) -> i32 (seq/src/sparse_opt_32.wat:500-521)
+napi_value count_nonzeros_coo(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 2);
+        ensure_ctx();
+        const int64_t n = a.i(1);
+        if (n <= 0) return num(env, 0);
+        void* dv = mirror(a.i(0), P->vs * n);
+        int64_t c = 0;
+        check(sable_count_nonzeros(g_ctx, P->dt, n, dv, &c), "count_nonzeros_coo");
+        return num(env, (double)c);
+    });
+}
+
+// copy_nonzeros_coo(in_row, in_col, in_val, out_row, out_col, out_val, nnz) (:523-546)
+napi_value copy_nonzeros_coo(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 7);
+        ensure_ctx();
+        const int64_t n = a.i(6);
+        if (n <= 0) return undef(env);
+        void* dr = mirror(a.i(0), 4 * n);
+        void* dc = mirror(a.i(1), 4 * n);
+        void* dv = mirror(a.i(2), P->vs * n);
+        int64_t c = 0;
+        check(sable_count_nonzeros(g_ctx, P->dt, n, dv, &c), "copy_nonzeros_coo");
+        void* orow = scratch(a.i(3), 4 * c, false);
+        void* ocol = scratch(a.i(4), 4 * c, false);
+        void* oval = scratch(a.i(5), P->vs * c, false);
+        check(sable_eliminate_zeros_coo(g_ctx, P->dt, n, (const int32_t*)dr, (const int32_t*)dc, dv, (int32_t*)orow,
+                                        (int32_t*)ocol, oval, &c),
+              "copy_nonzeros_coo");
+        for (int j = 3; j < 6; ++j) drop_mirrors_in(a.i(j), (j == 5 ? P->vs : 4) * c);
+        download(a.i(3), orow, 4 * c);
+        download(a.i(4), ocol, 4 * c);
+        download(a.i(5), oval, P->vs * c);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// copy_nonzeros_csr(in_row_ptr, in_col, in_val, out_row_ptr, out_col, out_val, N) (:548-595)
+napi_value copy_nonzeros_csr(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 7);
+        ensure_ctx();
+        const int32_t N = (int32_t)a.i(6);
+        if (N <= 0) return undef(env);
+        const int32_t* hrp = host_i32(a.i(0), N + 1);
+        const int64_t nnz = (int64_t)hrp[N] - hrp[0];
+        if (nnz < 0) throw Err{"row_ptr[N] < row_ptr[0]"};
+        void* drp = mirror(a.i(0), 4 * (size_t)(N + 1));
+        void* dc = mirror(a.i(1), 4 * (size_t)(hrp[N]));
+        void* dv = mirror(a.i(2), P->vs * (size_t)(hrp[N]));
+        int64_t c = 0;
+        check(sable_count_nonzeros(g_ctx, P->dt, nnz, (const char*)dv + P->vs * hrp[0], &c), "copy_nonzeros_csr");
+        void* orp = scratch(a.i(3), 4 * (size_t)(N + 1), false);
+        void* ocol = scratch(a.i(4), 4 * c, false);
+        void* oval = scratch(a.i(5), P->vs * c, false);
+        check(sable_eliminate_zeros_csr(g_ctx, P->dt, N, (const int32_t*)drp, (const int32_t*)dc, dv, (int32_t*)orp,
+                                        (int32_t*)ocol, oval, &c),
+              "copy_nonzeros_csr");
+        drop_mirrors_in(a.i(3), 4 * (size_t)(N + 1));
+        drop_mirrors_in(a.i(4), 4 * c);
+        drop_mirrors_in(a.i(5), P->vs * c);
+        download(a.i(3), orp, 4 * (size_t)(N + 1));
+        download(a.i(4), ocol, 4 * c);
+        download(a.i(5), oval, P->vs * c);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+napi_value pair(napi_env env, double x, double y) {
+    napi_value arr;
+    napi_create_array_with_length(env, 2, &arr);
+    napi_set_element(env, arr, 0, num(env, x));
+    napi_set_element(env, arr, 1, num(env, y));
+    return arr;
+}
+
+// count_nonzeros_dia(offset, data, ndiags, stride, N) -> [count_diags, count] (:1948-2039)
+napi_value count_nonzeros_dia(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 5);
+        ensure_ctx();
+        const int32_t nd = (int32_t)a.i(2), stride = (int32_t)a.i(3), N = (int32_t)a.i(4);
+        if (nd <= 0 || N <= 0) return pair(env, 0, 0);
+        void* doff = mirror(a.i(0), 4 * (size_t)nd);
+        void* dd = mirror(a.i(1), P->vs * (size_t)nd * stride);
+        int32_t cd = 0;
+        int64_t c = 0;
+        check(sable_count_nonzeros_dia(g_ctx, P->dt, N, nd, stride, (const int32_t*)doff, dd, &cd, &c),
+              "count_nonzeros_dia");
+        return pair(env, cd, (double)c);
+    });
+}
+
+// copy_nonzeros_dia(in_offset, in_data, out_offset, out_data, ndiags, stride, N) (:2041-2135)
+napi_value copy_nonzeros_dia(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 7);
+        ensure_ctx();
+        const int32_t nd = (int32_t)a.i(4), stride = (int32_t)a.i(5), N = (int32_t)a.i(6);
+        if (nd <= 0 || N <= 0) return undef(env);
+        void* doff = mirror(a.i(0), 4 * (size_t)nd);
+        void* dd = mirror(a.i(1), P->vs * (size_t)nd * stride);
+        int32_t cd = 0;
+        int64_t c = 0;
+        check(sable_count_nonzeros_dia(g_ctx, P->dt, N, nd, stride, (const int32_t*)doff, dd, &cd, &c),
+              "copy_nonzeros_dia");
+        if (cd == 0) return undef(env);
+        void* ooff = scratch(a.i(2), 4 * (size_t)cd, false);
+        void* odata = scratch(a.i(3), P->vs * (size_t)cd * stride, false);
+        check(sable_eliminate_zeros_dia(g_ctx, P->dt, N, nd, stride, (const int32_t*)doff, dd, (int32_t*)ooff, odata,
+                                        &cd),
+              "copy_nonzeros_dia");
+        drop_mirrors_in(a.i(2), 4 * (size_t)cd);
+        drop_mirrors_in(a.i(3), P->vs * (size_t)cd * stride);
+        download(a.i(2), ooff, 4 * (size_t)cd);
+        download(a.i(3), odata, P->vs * (size_t)cd * stride);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// count_nonzeros_ell(data, nnz_row, ncols, N) -> [count_cols, count] (:5821-5893)
+napi_value count_nonzeros_ell(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 4);
+        ensure_ctx();
+        const int32_t nc = (int32_t)a.i(2), N = (int32_t)a.i(3);
+        if (nc <= 0 || N <= 0) return pair(env, 0, 0);
+        void* dd = mirror(a.i(0), P->vs * (size_t)nc * N);
+        int32_t cc = 0;
+        int64_t c = 0;
+        check(sable_count_nonzeros_ell(g_ctx, P->dt, N, nc, dd, &cc, &c), "count_nonzeros_ell");
+        return pair(env, cc, (double)c);
+    });
+}
+
+// copy_nonzeros_ell(in_data, in_indices, out_data, out_indices, nnz_row, ncols, N) (:5895-5949):
+// the output width is the largest per-row nonzero count
+napi_value copy_nonzeros_ell(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 7);
+        ensure_ctx();
+        const int32_t nc = (int32_t)a.i(5), N = (int32_t)a.i(6);
+        if (nc <= 0 || N <= 0) return undef(env);
+        const size_t slots = (size_t)nc * N;
+        void* dd = mirror(a.i(0), P->vs * slots);
+        void* di = mirror(a.i(1), 4 * slots);
+        int32_t cc = 0;
+        int64_t c = 0;
+        check(sable_count_nonzeros_ell(g_ctx, P->dt, N, nc, dd, &cc, &c), "copy_nonzeros_ell");
+        if (cc == 0) return undef(env);
+        const size_t oslots = (size_t)cc * N;
+        void* od = scratch(a.i(2), P->vs * oslots, false);
+        void* oi = scratch(a.i(3), 4 * oslots, false);
+        check(sable_eliminate_zeros_ell(g_ctx, P->dt, N, nc, cc, dd, (const int32_t*)di, od, (int32_t*)oi),
+              "copy_nonzeros_ell");
+        drop_mirrors_in(a.i(2), P->vs * oslots);
+        drop_mirrors_in(a.i(3), 4 * oslots);
+        download(a.i(2), od, P->vs * oslots);
+        download(a.i(3), oi, 4 * oslots);
+        release_scratch();
+        return undef(env);
+    });
+}
+
+// Registers every element-wise export of one module.  The descriptors live
+// for the process: each JS environment that loads the addon gets functions
+// over the same ones.
+const char* const kEwNames[] = {"expm1", "log1p", "sin", "tan", "pow", "deg2rad", "rad2deg", "sign", "sign",
+                                "abs", "neg", "sqrt", "ceil", "floor", "trunc", "nearest", "multiply_scalar"};
+
+void add_fn(napi_env env, napi_value obj, const char* name, napi_callback cb, void* data);
+
+void add_ew(napi_env env, napi_value m, Prec* P) {
+    static std::mutex mu;
+    static std::map<Prec*, std::vector<EwDesc*>> made;
+    std::vector<EwDesc*>* list;
+    {
+        std::lock_guard<std::mutex> hold(mu);
+        list = &made[P];
+        if (list->empty()) {
+            const bool par = P->lead != 0;
+            for (int fmt = 0; fmt < 3; ++fmt)
+                for (int op = SABLE_EW_EXPM1; op <= SABLE_EW_MULTIPLY; ++op) {
+                    const bool coo = fmt == EW_COO;
+                    if (op == SABLE_EW_SIGN && !coo) continue;   // DIA/ELL sign keeps zero
+                    if (op == SABLE_EW_SIGN0 && coo) continue;
+                    const bool scalar = op == SABLE_EW_POW || op == SABLE_EW_DEG2RAD || op == SABLE_EW_RAD2DEG ||
+                                        op == SABLE_EW_MULTIPLY;
+                    for (int self = 0; self < 2; ++self) {
+                        // serial tree: multiply_scalar only as self_multiply_scalar_coo
+                        if (op == SABLE_EW_MULTIPLY && !par && (!coo || !self)) continue;
+                        list->push_back(new EwDesc{P, (EwFmt)fmt, op, self != 0, scalar});
+                    }
+                }
+        }
+    }
+    static const char* const fmts[] = {"coo", "dia", "ell"};
+    for (EwDesc* D : *list) {
+        std::string name = std::string(D->self ? "self_" : "") + kEwNames[D->op] + "_" + fmts[D->fmt];
+        add_fn(env, m, name.c_str(), ew_export, D);
+    }
+}
+
 // ------------------------------------------------------------ module objects
 napi_value malloc_fn(napi_env env, napi_callback_info info) {
     return guarded(env, [&] {
@@ -1001,8 +1316,13 @@ napi_value make_serial(napi_env env, Prec* P) {
         {"memcpy", lm_memcpy}, {"coo_csr", coo_csr}, {"num_cols", num_cols}, {"csr_ell", csr_ell},
         {"num_diags", num_diags}, {"csr_dia", csr_dia}, {"coo_csc", coo_csc},
         {"spts_csc", spts_csc}, {"spts_csc_wrapper", spts_csc_wrapper}, {"mmread", mmread},
+        {"count_nonzeros_coo", count_nonzeros_coo}, {"copy_nonzeros_coo", copy_nonzeros_coo},
+        {"copy_nonzeros_csr", copy_nonzeros_csr}, {"count_nonzeros_dia", count_nonzeros_dia},
+        {"copy_nonzeros_dia", copy_nonzeros_dia}, {"count_nonzeros_ell", count_nonzeros_ell},
+        {"copy_nonzeros_ell", copy_nonzeros_ell},
     };
     for (auto& f : fns) add_fn(env, m, f.name, f.cb, P);
+    add_ew(env, m, P);
     return m;
 }
 
@@ -1022,6 +1342,7 @@ napi_value make_parallel(napi_env env, Prec* P, Prec* serial) {
     };
     for (auto& f : fns) add_fn(env, m, f.name, f.cb, P);
     add_fn(env, m, "sum", par_sum, serial);  // sum takes no worker id
+    add_ew(env, m, P);
     return m;
 }
 

@@ -19,6 +19,8 @@
 //     spts_csr_opt_level_sync_free_wrapper
 //   coo_csc, spts_csc / spts_csc_wrapper (serial column sweep)
 //   mmread (full and lower COO from MatrixMarket text)
+//   element-wise exports (COO/DIA/ELL, serial and worker-range forms) and
+//     count_nonzeros_* / copy_nonzeros_* on a value array with zeros
 'use strict';
 
 const fs = require('fs');
@@ -274,6 +276,102 @@ if (spec.spts) {
   const y5 = alloc(S.N, FArray);
   E.spts_csc_wrapper(Lc.col.off, Lc.row.off, Lc.val.off, bc.off, y5, S.N, inner_max);
   save('spts_csc_wrapper', y5, S.N, FArray);
+}
+
+// ---------------------------------------------------------------- element-wise
+function copy_of(src, n, Type) {
+  const dst = alloc(n, Type);
+  new Uint8Array(memory.buffer, dst, n * Type.BYTES_PER_ELEMENT).set(
+      new Uint8Array(memory.buffer, src, n * Type.BYTES_PER_ELEMENT));
+  return dst;
+}
+step('element-wise coo');
+{
+  const o1 = alloc(nnz, FArray);
+  E.sqrt_coo(coo_val.off, o1, nnz);
+  save('ew_sqrt_coo', o1, nnz, FArray);
+  const o2 = alloc(nnz, FArray);
+  E.deg2rad_coo(Math.PI, coo_val.off, o2, nnz);
+  save('ew_deg2rad_coo', o2, nnz, FArray);
+  const o3 = alloc(nnz, FArray);
+  E.expm1_coo(coo_val.off, o3, nnz);
+  save('ew_expm1_coo', o3, nnz, FArray);
+  const s1 = copy_of(coo_val.off, nnz, FArray);
+  E.self_sign_coo(s1, nnz);
+  save('ew_self_sign_coo', s1, nnz, FArray);
+  const s2 = copy_of(coo_val.off, nnz, FArray);
+  E.self_multiply_scalar_coo(2.5, s2, nnz);
+  save('ew_self_multiply_coo', s2, nnz, FArray);
+}
+if (dia) {
+  step('element-wise dia');
+  const n = dia.nd * dia.stride;
+  const o = alloc(n, FArray);
+  new FArray(memory.buffer, o, n).fill(7);  // slots outside the diagonals keep 7
+  E.neg_dia(dia.off, dia.data, o, dia.nd, dia.stride, N);
+  save('ew_neg_dia', o, n, FArray);
+  const s = copy_of(dia.data, n, FArray);
+  E.self_sign_dia(dia.off, s, dia.nd, dia.stride, N);
+  save('ew_self_sign_dia', s, n, FArray);
+  const q = copy_of(dia.data, n, FArray);
+  for (let id = 0; id < W; id++) P.self_abs_dia(id, dia.off, q, R[id][0], R[id][1], dia.nd, dia.stride, N);
+  save('ew_par_abs_dia', q, n, FArray);
+}
+if (ell) {
+  step('element-wise ell');
+  const n = ell.nc * N;
+  const o = alloc(n, FArray);
+  E.nearest_ell(ell.data, o, ell.nc, N);
+  save('ew_nearest_ell', o, n, FArray);
+  const q = alloc(n, FArray);
+  for (let id = 0; id < W; id++) P.multiply_scalar_ell(id, -2, ell.data, q, R[id][0], R[id][1], ell.nc, N);
+  save('ew_par_mul_ell', q, n, FArray);
+}
+step('element-wise par coo');
+{
+  const o = alloc(nnz, FArray);
+  const per = Math.floor(nnz / W);
+  for (let id = 0; id < W; id++) {
+    const s0 = id * per, e0 = id === W - 1 ? nnz : (id + 1) * per;
+    P.ceil_coo(id, coo_val.off + VB * s0, o + VB * s0, e0 - s0);
+  }
+  save('ew_par_ceil_coo', o, nnz, FArray);
+}
+
+// ---------------------------------------------------------------- eliminate_zeros
+step('nonzeros');
+{
+  const zv = load_array('z_val', FArray);
+  const c = E.count_nonzeros_coo(zv.off, nnz);
+  outputs.nz_count = c;
+  const orow = alloc(c, Int32Array), ocol = alloc(c, Int32Array), oval = alloc(c, FArray);
+  E.copy_nonzeros_coo(coo_row.off, coo_col.off, zv.off, orow, ocol, oval, nnz);
+  save('nz_coo_row', orow, c, Int32Array);
+  save('nz_coo_col', ocol, c, Int32Array);
+  save('nz_coo_val', oval, c, FArray);
+  const zc = load_array('z_csr_val', FArray);
+  const prow = alloc(N + 1, Int32Array), pcol = alloc(c, Int32Array), pval = alloc(c, FArray);
+  E.copy_nonzeros_csr(csr.row, csr.col, zc.off, prow, pcol, pval, N);
+  save('nz_csr_row', prow, N + 1, Int32Array);
+  save('nz_csr_col', pcol, c, Int32Array);
+  save('nz_csr_val', pval, c, FArray);
+  if (dia) {
+    const r = E.count_nonzeros_dia(dia.off, dia.data, dia.nd, dia.stride, N);
+    outputs.nz_dia = [r[0], r[1]];
+    const ooff = alloc(r[0], Int32Array), odata = alloc(r[0] * dia.stride, FArray);
+    E.copy_nonzeros_dia(dia.off, dia.data, ooff, odata, dia.nd, dia.stride, N);
+    save('nz_dia_off', ooff, r[0], Int32Array);
+    save('nz_dia_data', odata, r[0] * dia.stride, FArray);
+  }
+  if (ell) {
+    const nnz_row = alloc(N, Int32Array);
+    const r = E.count_nonzeros_ell(ell.data, nnz_row, ell.nc, N);
+    outputs.nz_ell = [r[0], r[1]];
+    const od = alloc(r[0] * N, FArray), oi = alloc(r[0] * N, Int32Array);
+    E.copy_nonzeros_ell(ell.data, ell.idx, od, oi, nnz_row, ell.nc, N);
+    save('nz_ell_data', od, r[0] * N, FArray);
+    save('nz_ell_idx', oi, r[0] * N, Int32Array);
+  }
 }
 
 // ---------------------------------------------------------------- MatrixMarket

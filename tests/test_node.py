@@ -31,10 +31,17 @@ NODE = shutil.which("node")
 SERIAL = ["spmv_coo", "spmv_coo_wrapper", "spmv_csr", "spmv_csr_wrapper", "spmv_dia", "spmv_dia_wrapper",
           "spmv_ell", "spmv_ell_wrapper", "transpose_csr", "transpose_dia", "count_transpose_cols_ell",
           "transpose_ell", "memcpy", "coo_csr", "num_cols", "csr_ell", "num_diags", "csr_dia", "coo_csc",
-          "spts_csc", "spts_csc_wrapper", "mmread"]
+          "spts_csc", "spts_csc_wrapper", "mmread", "count_nonzeros_coo", "copy_nonzeros_coo", "copy_nonzeros_csr",
+          "count_nonzeros_dia", "copy_nonzeros_dia", "count_nonzeros_ell", "copy_nonzeros_ell"]
 PARALLEL = ["spmv_coo_wrapper", "spmv_csr_wrapper", "spmv_dia_col_wrapper", "spmv_ell_col_wrapper", "memcpy",
             "sum", "spts_level_csr", "spts_csr_sync_free_wrapper", "spts_csr_level_sync_free_wrapper",
             "spts_csr_opt_level_sync_free_wrapper", "metadata"]
+# element-wise exports (seq/src/sparse_opt_32.wat, par/src/sparse_opt_32.wat): <op>_<fmt> and self_<op>_<fmt>
+EW_OPS = ["expm1", "log1p", "sin", "tan", "pow", "deg2rad", "rad2deg", "sign", "abs", "neg", "sqrt", "ceil",
+          "floor", "trunc", "nearest"]
+EW = [p + op + "_" + f for f in ("coo", "dia", "ell") for op in EW_OPS for p in ("", "self_")]
+SERIAL += EW + ["self_multiply_scalar_coo"]
+PARALLEL += EW + [p + "multiply_scalar_" + f for f in ("coo", "dia", "ell") for p in ("", "self_")]
 
 needs_node = pytest.mark.skipif(NODE is None, reason="node is not installed")
 
@@ -104,7 +111,13 @@ def write_inputs(d, dtype, seed=0, workers=3, inner_max=3):
     # CSC of L: the transpose's CSR, rows ascending within each column, so
     # the diagonal comes first
     ccp, crow, cval = oracle.transpose_csr(n, Lrp, Lc, Lv)
+    z_val = coo_val.copy()
+    z_val[rng.random(nnz) < 0.3] = 0
+    # the same values in CSR order, for copy_nonzeros_csr
+    order = np.lexsort((coo_col, coo_row))
+    z_csr_val = z_val[order]
     arrays = dict(coo_row=coo_row, coo_col=coo_col, coo_val=coo_val, x=x, L_row=Lrp, L_col=Lc, L_val=Lv, L_b=b,
+                  z_val=z_val, z_csr_val=z_csr_val,
                   Lr_level_ptr=lp, Lr_perm=lperm, Lr_row=rrp, Lr_col=rc, Lr_val=rv, Lr_b=br,
                   Lc_col=ccp, Lc_row=crow, Lc_val=cval)
     for k, v in arrays.items():
@@ -232,6 +245,48 @@ def test_node_flow_matches_oracle(tmp_path, dtype):
     yc = oracle.spts_csc(n, a["Lc_col"], a["Lc_row"], a["Lc_val"], a["L_b"])
     assert np.array_equal(read_out(d, "spts_csc", dtype).view(np.uint8), yc.view(np.uint8))
     assert np.array_equal(read_out(d, "spts_csc_wrapper", dtype).view(np.uint8), yc.view(np.uint8))
+
+    # element-wise exports: bit-exact except expm1 (JavaScript's Math in
+    # the reference: within 1 ulp of the oracle after rounding to the type)
+    def ew(name):
+        return read_out(d, "ew_" + name, dtype)
+
+    def eq(a, b):
+        return np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
+
+    cv = a["coo_val"]
+    with np.errstate(all="ignore"):
+        assert eq(ew("sqrt_coo"), oracle.ew_values("sqrt", 0, cv))
+        assert eq(ew("deg2rad_coo"), oracle.ew_values("deg2rad", np.pi, cv))
+        assert eq(ew("self_sign_coo"), oracle.ew_values("sign", 0, cv))
+        assert eq(ew("self_multiply_coo"), oracle.ew_values("multiply", 2.5, cv))
+        assert eq(ew("par_ceil_coo"), oracle.ew_values("ceil", 0, cv))
+        e1, e2 = ew("expm1_coo"), oracle.ew_values("expm1", 0, cv)
+        it = np.int32 if dtype == np.float32 else np.int64
+        assert np.all(np.abs(e1.view(it).astype(np.int64) - e2.view(it).astype(np.int64)) <= 2)
+        assert eq(ew("neg_dia"), oracle.ew_dia("neg", 0, N, stride, doff, ddata, out=np.full_like(ddata, 7)))
+        assert eq(ew("self_sign_dia"), oracle.ew_dia("sign0", 0, N, stride, doff, ddata, out=ddata))
+        assert eq(ew("par_abs_dia"), oracle.ew_dia("abs", 0, N, stride, doff, ddata, out=ddata))
+        assert eq(ew("nearest_ell"), oracle.ew_values("nearest", 0, edata))
+        assert eq(ew("par_mul_ell"), oracle.ew_values("multiply", -2, edata))
+
+    # count_nonzeros_* / copy_nonzeros_*: bit-exact
+    zv = a["z_val"]
+    assert out["nz_count"] == oracle.count_nonzeros(zv)
+    r_, c_, v_ = oracle.copy_nonzeros_coo(a["coo_row"], a["coo_col"], zv)
+    assert np.array_equal(read_out(d, "nz_coo_row", i32), r_) and np.array_equal(read_out(d, "nz_coo_col", i32), c_)
+    assert np.array_equal(read_out(d, "nz_coo_val", dtype), v_)
+    orp, oc, ov = oracle.copy_nonzeros_csr(N, rp, col, a["z_csr_val"])
+    assert np.array_equal(read_out(d, "nz_csr_row", i32), orp) and np.array_equal(read_out(d, "nz_csr_col", i32), oc)
+    assert np.array_equal(read_out(d, "nz_csr_val", dtype), ov)
+    assert out["nz_dia"] == list(oracle.count_nonzeros_dia(N, stride, doff, ddata))
+    ooff, odata = oracle.copy_nonzeros_dia(N, stride, doff, ddata)
+    assert np.array_equal(read_out(d, "nz_dia_off", i32), ooff)
+    assert np.array_equal(read_out(d, "nz_dia_data", dtype), odata)
+    assert out["nz_ell"] == list(oracle.count_nonzeros_ell(N, nc, edata))
+    cc, od_, oi_ = oracle.copy_nonzeros_ell(N, nc, edata, eidx)
+    assert np.array_equal(read_out(d, "nz_ell_data", dtype), od_)
+    assert np.array_equal(read_out(d, "nz_ell_idx", i32), oi_)
 
     # MatrixMarket text -> COO in the linear memory: bit-exact
     text = open(os.path.join(d, "mm.mtx"), "rb").read()
