@@ -367,6 +367,24 @@ SABLE_API int sable_eliminate_zeros_ell(sable_ctx_t* ctx, sable_dtype_t dtype, i
                                         const void* data, const int32_t* indices, void* out_data,
                                         int32_t* out_indices);
 
+/* ------------------------------------------------------------ row reordering */
+/* Rows in ascending nnz order, stable: out_perm[i] = the old row placed at
+ * new row i; out_row_ptr/out_col/out_val the permuted CSR (row_ptr[0] = 0);
+ * out_nnz_row[r] = nnz of old row r (may be NULL); bins = the number of rows
+ * with 0, <= 1, <= 2 and <= 3 entries (num_zero_rows ... num_three_rows),
+ * which select the short-row SpMV variants (synchronises).  Replaces
+ * sort_rows_by_nnz (par/src/sparse32.js:2148-2218; sparse64.js:1297-1367),
+ * whose row_ptr stores go through a zero-length view and are lost; the
+ * row_ptr here is the one that loop computes. */
+SABLE_API int sable_csr_sort_rows_by_nnz(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                                         const int32_t* col, const void* val, int32_t* out_row_ptr, int32_t* out_col,
+                                         void* out_val, int32_t* out_perm, int32_t* out_nnz_row, int32_t* bins);
+/* scatter = 1: out[perm[i]] = in[i] (results of the reordered matrix back to
+ * the original row order; sort_y_rows_by_nnz, par/src/sparse32.js:2220-2235);
+ * scatter = 0: out[i] = in[perm[i]]. */
+SABLE_API int sable_permute_rows(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* perm, int scatter,
+                                 const void* in, void* out);
+
 /* ------------------------------------------------------------ MatrixMarket ingest */
 /* Field of the banner (token 4 of "%%MatrixMarket matrix coordinate <field>
  * <symmetry>"); every field but pattern carries a value per entry (complex:

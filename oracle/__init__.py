@@ -438,3 +438,27 @@ def copy_nonzeros_ell(N, nc, data, idx):
     fn.argtypes = [_I, _I, _I, _VP, _I32P, _VP, _VP]
     fn(N, nc, cc, data.ctypes.data, _i32(idx), odata.ctypes.data, oidx.ctypes.data)
     return cc, odata, oidx
+
+
+# ------------------------------------------------------------------ row reordering
+def sort_rows_by_nnz(N, rowptr, col, val):
+    """par/src/sparse32.js:2148-2218 -> (rowptr, col, val, perm, nnz_row, bins4)."""
+    val = np.ascontiguousarray(val)
+    nnz = len(val)
+    orp, ocol, oval = np.zeros(N + 1, np.int32), np.zeros(nnz, np.int32), np.zeros(nnz, val.dtype)
+    perm, nnz_row, bins = np.zeros(N, np.int32), np.zeros(N, np.int32), np.zeros(4, np.int32)
+    fn = getattr(lib(), "oracle_sort_rows_by_nnz_" + _suf(val))
+    fn.argtypes = [_I, _I32P, _I32P, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
+    fn(N, _i32(rowptr), _i32(col), val.ctypes.data, orp.ctypes.data, ocol.ctypes.data, oval.ctypes.data,
+       perm.ctypes.data, nnz_row.ctypes.data, bins.ctypes.data)
+    return orp, ocol, oval, perm, nnz_row, bins
+
+
+def sort_y_rows_by_nnz(perm, y):
+    """par/src/sparse32.js:2220-2235: out[perm[i]] = y[i]."""
+    y = np.ascontiguousarray(y)
+    out = np.zeros_like(y)
+    fn = getattr(lib(), "oracle_sort_y_rows_" + _suf(y))
+    fn.argtypes = [_I, _I32P, _VP, _VP]
+    fn(len(perm), _i32(perm), y.ctypes.data, out.ctypes.data)
+    return out

@@ -1059,3 +1059,52 @@ static double js_pow(double x, double y) {
 
 DEF_EW(float, f32)
 DEF_EW(double, f64)
+
+/* ================================================================ row reordering
+ * sort_rows_by_nnz (par/src/sparse32.js:2148-2218): counting sort of the
+ * rows by nnz, stable; the new row_ptr is the one its loop computes (the
+ * reference stores it through a zero-length view).  bins4 = cumulative
+ * counts of rows with 0, <=1, <=2, <=3 entries. */
+#define DEF_SORT_ROWS(T, SUF)                                                                                      \
+    EXPORT void oracle_sort_rows_by_nnz_##SUF(int N, const int32_t* rp, const int32_t* col, const T* val,          \
+                                              int32_t* orp, int32_t* ocol, T* oval, int32_t* perm,               \
+                                              int32_t* nnz_row, int32_t* bins4) {                                 \
+        /* frequencies up to the longest row (the reference sizes them N + 1, which drops rows longer  \
+         * than N, possible only with duplicate entries, and reads undefined bins when N < 3) */      \
+        int32_t maxl = 3;                                                                                         \
+        for (int i = 0; i < N; ++i) {                                                                             \
+            nnz_row[i] = rp[i + 1] - rp[i];                                                                       \
+            if (nnz_row[i] > maxl) maxl = nnz_row[i];                                                             \
+        }                                                                                                         \
+        int32_t* freq = (int32_t*)calloc((size_t)maxl + 1, 4);                                                    \
+        int32_t* start = (int32_t*)calloc((size_t)maxl + 1, 4);                                                   \
+        for (int i = 0; i < N; ++i) freq[nnz_row[i]]++;                                                           \
+        start[0] = 0;                                                                                             \
+        for (int i = 1; i <= maxl; ++i) start[i] = start[i - 1] + freq[i - 1];                                    \
+        bins4[0] = freq[0];                                                                                       \
+        bins4[1] = bins4[0] + freq[1];                                                                            \
+        bins4[2] = bins4[1] + freq[2];                                                                            \
+        bins4[3] = bins4[2] + freq[3];                                                                            \
+        for (int i = 0; i < N; ++i) perm[start[nnz_row[i]]++] = i;                                                \
+        int64_t j = 0;                                                                                            \
+        orp[0] = 0;                                                                                               \
+        for (int i = 0; i < N; ++i) {                                                                             \
+            int64_t k = rp[perm[i]];                                                                              \
+            int t = nnz_row[perm[i]];                                                                             \
+            orp[i + 1] = orp[i] + t;                                                                              \
+            while (t != 0) {                                                                                      \
+                ocol[j] = col[k];                                                                                 \
+                oval[j++] = val[k++];                                                                             \
+                t--;                                                                                              \
+            }                                                                                                     \
+        }                                                                                                         \
+        free(freq);                                                                                               \
+        free(start);                                                                                              \
+    }                                                                                                             \
+    /* sort_y_rows_by_nnz (:2220-2235) */                                                                          \
+    EXPORT void oracle_sort_y_rows_##SUF(int N, const int32_t* perm, const T* y, T* out) {                        \
+        for (int i = 0; i < N; ++i) out[perm[i]] = y[i];                                                          \
+    }
+
+DEF_SORT_ROWS(float, f32)
+DEF_SORT_ROWS(double, f64)

@@ -371,3 +371,32 @@ def test_coo_csc_is_stable_col_row_sort():
     order = np.lexsort((np.arange(nnz), row, col))
     assert np.array_equal(r, row[order]) and np.array_equal(v, val[order])
     assert np.array_equal(cp, np.concatenate([[0], np.cumsum(np.bincount(col, minlength=N))]))
+
+
+# ------------------------------------------------------------------ row reordering
+def test_sort_rows_by_nnz_hand():
+    rp = np.array([0, 3, 3, 4, 6, 9], np.int32)
+    col = np.arange(9, dtype=np.int32)
+    val = np.arange(9, dtype=np.float64)
+    orp, oc, ov, perm, nnz_row, bins = oracle.sort_rows_by_nnz(5, rp, col, val)
+    # lengths 3,0,1,2,3 -> rows 1,2,3,0,4 (ties keep their order)
+    assert perm.tolist() == [1, 2, 3, 0, 4]
+    assert orp.tolist() == [0, 0, 1, 3, 6, 9] and oc.tolist() == [3, 4, 5, 0, 1, 2, 6, 7, 8]
+    assert nnz_row.tolist() == [3, 0, 1, 2, 3] and bins.tolist() == [1, 2, 3, 5]
+
+
+def test_sort_rows_by_nnz_is_a_stable_sort_and_spmv_invariant():
+    from sciwasm.sparse_amd import gen
+    rp, col, val = gen.zipf_csr(3000, 1.4, 64, seed=3)
+    N = len(rp) - 1
+    orp, oc, ov, perm, nnz_row, bins = oracle.sort_rows_by_nnz(N, rp, col, val)
+    lens = np.diff(rp)
+    assert np.array_equal(perm, np.argsort(lens, kind="stable"))
+    assert np.array_equal(np.diff(orp), lens[perm])
+    assert bins.tolist() == [int((lens <= k).sum()) for k in range(4)]
+    x = np.random.default_rng(0).standard_normal(N)
+    z = np.zeros(N)
+    y = oracle.spmv_csr(N, rp, col, val, x, z, 1)
+    ys = oracle.spmv_csr(N, orp, oc, ov, x, z, 1)
+    # each row keeps its entries in order, so each product is bit-identical
+    assert np.array_equal(oracle.sort_y_rows_by_nnz(perm, ys), y)
