@@ -379,6 +379,18 @@ SABLE_API int sable_eliminate_zeros_ell(sable_ctx_t* ctx, sable_dtype_t dtype, i
 SABLE_API int sable_csr_sort_rows_by_nnz(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
                                          const int32_t* col, const void* val, int32_t* out_row_ptr, int32_t* out_col,
                                          void* out_val, int32_t* out_perm, int32_t* out_nnz_row, int32_t* bins);
+/* Locality ordering of a CSR whose rows are sorted by nnz: within each
+ * bucket of <= 1024 rows of equal nnz, a greedy walk that appends the
+ * unvisited row most similar (Jaccard similarity of the 16-column cache
+ * lines it touches) to one of the last w rows placed, from each of those
+ * rows' top-w candidate lists.  out_perm[i] = perm_in[r] for the input row r
+ * placed at new row i (perm_in may be NULL); the permuted CSR goes to
+ * out_row_ptr/out_col/out_val.  w in [1, 16] (synchronises).  Replaces
+ * reorder_NN (par/src/sparse32.js:2501-2622), including its list rules; its
+ * row_ptr stores go through a zero-length view as in sort_rows_by_nnz. */
+SABLE_API int sable_csr_reorder_nn(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                                   const int32_t* col, const void* val, const int32_t* perm_in, int w,
+                                   int32_t* out_row_ptr, int32_t* out_col, void* out_val, int32_t* out_perm);
 /* scatter = 1: out[perm[i]] = in[i] (results of the reordered matrix back to
  * the original row order; sort_y_rows_by_nnz, par/src/sparse32.js:2220-2235);
  * scatter = 0: out[i] = in[perm[i]]. */

@@ -462,3 +462,13 @@ def sort_y_rows_by_nnz(perm, y):
     fn.argtypes = [_I, _I32P, _VP, _VP]
     fn(len(perm), _i32(perm), y.ctypes.data, out.ctypes.data)
     return out
+
+
+def reorder_nn(N, rowptr, col, w=16):
+    """par/src/sparse32.js:2501-2622 on a CSR with rows sorted by nnz: the
+    greedy window ordering of each bucket (perm[i] = input row at new row i)."""
+    perm = np.zeros(N, np.int32)
+    fn = lib().oracle_reorder_nn
+    fn.argtypes = [_I, _I32P, _I32P, _I, _VP]
+    fn(N, _i32(rowptr), _i32(col), w, perm.ctypes.data)
+    return perm

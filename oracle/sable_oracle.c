@@ -1108,3 +1108,142 @@ DEF_EW(double, f64)
 
 DEF_SORT_ROWS(float, f32)
 DEF_SORT_ROWS(double, f64)
+
+/* reorder_NN (par/src/sparse32.js:2501-2622) with reorder_subset
+ * (:2439-2492), find_next_row (:2330-2419) and assign_w_min_elems
+ * (:2275-2327), over a CSR whose rows are sorted by nnz.  out_perm[i] = the
+ * row of the input placed at new row i (the caller composes it with the
+ * sort's permutation).  Similarities are computed in double and kept in
+ * float32 lists, compared against the double value, as the reference's
+ * Float32Array lists are. */
+static void nn_assign(float* jac, int32_t* nid, int32_t next, double max, int w) {
+    int i = 0;
+    while (i < w && jac[i] >= 0 && (double)jac[i] >= max) i++;
+    if (i == w) return;
+    if (jac[i] == -1) {
+        jac[i] = (float)max;
+        nid[i] = next;
+        return;
+    }
+    int j = w - 1;
+    while (jac[j] == -1) j--;
+    while (j != i) {
+        jac[j] = jac[j - 1];
+        nid[j] = nid[j - 1];
+        j--;
+    }
+    jac[i] = (float)max;
+    nid[i] = next;
+}
+
+/* last_used has N/16 + 1 lines; a typed-array store past its end is dropped
+ * and a load past it gives undefined, which equals no row */
+static void nn_find(int index, int8_t* visited, const int32_t* nlines, float* jac, int32_t* nid, int32_t* last_used,
+                    const int32_t* rp, const int32_t* col, int N, int w, int start, int end, int32_t* next_out,
+                    double* max_out) {
+    const int L = N / 16 + 1;
+    if (visited[index] == 3) {
+        visited[index] = 1;
+        for (int k = 0; k < w; ++k) {
+            jac[k] = -1;
+            nid[k] = N;
+        }
+        for (int j = rp[index]; j < rp[index + 1]; ++j)
+            if (col[j] >= 0 && col[j] / 16 < L && last_used[col[j] / 16] != index) last_used[col[j] / 16] = index;
+        for (int i = start; i < end; ++i) {
+            if (visited[i] != 0) continue;
+            int j = rp[i], alines = 0;
+            while (j < rp[i + 1]) {
+                if (col[j] >= 0 && col[j] / 16 < L && last_used[col[j] / 16] == index) alines++;
+                j++;
+                while (j < rp[i + 1] && col[j - 1] / 16 == col[j] / 16) j++;
+            }
+            const int dissim = nlines[index] + nlines[i] - alines;
+            nn_assign(jac, nid, i, (double)alines / (double)dissim, w);
+        }
+    }
+    int32_t next = N;
+    double max = 0;
+    for (int k = 0; k < w; ++k)
+        if (jac[k] >= 0 && nid[k] < N && visited[nid[k]] == 0) {
+            next = nid[k];
+            max = jac[k];
+            break;
+        }
+    *next_out = next;
+    *max_out = max;
+}
+
+EXPORT void oracle_reorder_nn(int N, const int32_t* rp, const int32_t* col, int w, int32_t* perm) {
+    int8_t* visited = (int8_t*)calloc((size_t)N + 1, 1);
+    int32_t* nlines = (int32_t*)calloc((size_t)N + 1, 4);
+    int32_t maxl = 0;
+    for (int i = 0; i < N; ++i)
+        if (rp[i + 1] - rp[i] > maxl) maxl = rp[i + 1] - rp[i];
+    int32_t* freq = (int32_t*)calloc((size_t)maxl + 1, 4);
+    for (int i = 0; i < N; ++i) {
+        perm[i] = i;
+        freq[rp[i + 1] - rp[i]]++;
+        if (rp[i + 1] == rp[i]) visited[i] = 2;
+    }
+    for (int i = 0; i < N; ++i) {
+        if (visited[i]) continue;
+        int j = rp[i];
+        while (j < rp[i + 1]) {
+            nlines[i]++;
+            j++;
+            while (j < rp[i + 1] && col[j - 1] / 16 == col[j] / 16) j++;
+        }
+    }
+    int index = 0;
+    while (index < N && visited[index] != 0) index++;
+    const int B = 1024;
+    const int L = N / 16 + 1;
+    int32_t* last_used = (int32_t*)malloc((size_t)L * 4);
+    float* jac = (float*)malloc((size_t)w * w * sizeof(float));
+    int32_t* nid = (int32_t*)malloc((size_t)w * w * 4);
+    int32_t* win = (int32_t*)malloc((size_t)w * 4);
+    int start = index;
+    for (int c = 1; c <= maxl; ++c) {
+        while (freq[c] > 0) {
+            const int take = freq[c] >= B ? B : freq[c];
+            const int end = start + take;
+            freq[c] -= take;
+            /* reorder_subset(start, end) */
+            for (int k = 0; k < L; ++k) last_used[k] = -1;
+            for (int k = 0; k < w * w; ++k) {
+                jac[k] = 0;
+                nid[k] = 0;
+            }
+            int idx = start, pos = start, nnb = 0, count = 0;
+            while (idx != N) {
+                visited[idx] = 3;
+                win[count++] = idx;
+                if (nnb < w) nnb++;
+                if (count == w) count = 0;
+                perm[pos++] = idx;
+                int32_t next = N;
+                double max = 0;
+                for (int s = 0; s < nnb; ++s) {
+                    int32_t nx;
+                    double mx;
+                    nn_find(win[s], visited, nlines, jac + (size_t)s * w, nid + (size_t)s * w, last_used, rp, col, N,
+                            w, start, end, &nx, &mx);
+                    if (max <= mx && nx != N) {
+                        max = mx;
+                        next = nx;
+                    }
+                }
+                idx = next;
+            }
+            start = end;
+        }
+    }
+    free(visited);
+    free(nlines);
+    free(freq);
+    free(last_used);
+    free(jac);
+    free(nid);
+    free(win);
+}

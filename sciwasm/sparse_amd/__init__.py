@@ -18,7 +18,7 @@ from .serve import Server, batch_spmv, batch_spts, batch_spts_refined, lower_dom
 from .mm import MMInfo, read_matrix_MM_files, create_COO_from_MM, create_LCOO_from_MM, mmread  # noqa: F401
 from .ewise import (expm1, log1p, sin, tan, pow, deg2rad, rad2deg, sign, abs, neg, sqrt, ceil, floor,  # noqa: F401
                     trunc, nearest, eliminate_zeros, count_nonzeros)
-from .reorder import sort_rows_by_nnz, sort_y_rows_by_nnz, permute_rows  # noqa: F401
+from .reorder import sort_rows_by_nnz, sort_y_rows_by_nnz, permute_rows, reorder_NN  # noqa: F401
 
 __all__ = [
     "Context", "DeviceArray", "SableError",
@@ -32,4 +32,5 @@ __all__ = [
     "MMInfo", "read_matrix_MM_files", "create_COO_from_MM", "create_LCOO_from_MM", "mmread",
     "expm1", "log1p", "sin", "tan", "pow", "deg2rad", "rad2deg", "sign", "abs", "neg", "sqrt", "ceil", "floor",
     "trunc", "nearest", "eliminate_zeros", "count_nonzeros", "sort_rows_by_nnz", "sort_y_rows_by_nnz", "permute_rows",
+    "reorder_NN",
 ]

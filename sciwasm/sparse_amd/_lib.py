@@ -121,6 +121,7 @@ SIGNATURES = {
     "sable_csr_sort_rows_by_nnz": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                    ctypes.POINTER(c_i32)],
     "sable_permute_rows": [c_vp, c_int, c_i32, c_vp, c_int, c_vp, c_vp],
+    "sable_csr_reorder_nn": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
     "sable_mm_header": [c_vp, c_sz, ctypes.POINTER(MMInfo)],
     "sable_mm_parse": [c_vp, c_vp, c_sz, c_int, c_vpp],
     "sable_mm_get_info": [c_vp, ctypes.POINTER(MMInfo)],

@@ -8,6 +8,12 @@
 //                                    rows with 0, <=1, <=2, <=3 entries that
 //                                    select the short-row SpMV variants
 //   sort_y_rows_by_nnz   :2220-2235  y_new[perm[i]] = y[i]
+//   reorder_NN           :2501-2622  greedy window ordering of each bucket
+//                                    of <= 1024 equal-nnz rows by the
+//                                    Jaccard similarity of their 16-column
+//                                    cache-line sets (reorder_subset
+//                                    :2439-2492, find_next_row :2330-2419,
+//                                    assign_w_min_elems :2275-2327)
 //
 // The reference writes the new row_ptr through a view sized by the
 // undefined A_csr_new.nrows, i.e. a zero-length view, so its row_ptr is
@@ -18,6 +24,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 using namespace sable;
 
@@ -82,6 +89,17 @@ __global__ void k_copy_rows(int32_t N, const int32_t* __restrict__ perm, const i
     }
 }
 
+__global__ void k_iota(int32_t N, int32_t* __restrict__ p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) p[i] = (int32_t)i;
+}
+
+__global__ void k_row_len_of(int32_t N, const int32_t* __restrict__ rp, const int32_t* __restrict__ perm,
+                             int32_t* __restrict__ len) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) len[i] = rp[perm[i] + 1] - rp[perm[i]];
+}
+
 template <typename T>
 __global__ void k_scatter_rows(int32_t N, const int32_t* __restrict__ perm, const T* __restrict__ y,
                                T* __restrict__ out) {
@@ -94,6 +112,161 @@ __global__ void k_gather_rows(int32_t N, const int32_t* __restrict__ perm, const
                               T* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N) out[i] = x[perm[i]];
+}
+
+// ---------------------------------------------------------------- reorder_NN
+constexpr int NN_B = 1024;  // rows per bucket (the reference's B)
+constexpr int NN_W = 16;    // largest window
+
+// Runs of consecutive entries in the same 16-column line, per row.
+__global__ void k_nlines(int32_t N, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                         int32_t* __restrict__ nlines) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int32_t n = 0;
+    for (int32_t j = rp[i]; j < rp[i + 1];) {
+        ++n;
+        ++j;
+        while (j < rp[i + 1] && (col[j - 1] >> 4) == (col[j] >> 4)) ++j;
+    }
+    nlines[i] = n;
+}
+
+// assign_w_min_elems on a list held in registers: the first position whose
+// entry is -1 or below m takes m; the entries from there to the last valid
+// one move down by one, the last valid one being overwritten (as the
+// reference's loop does, also when the list is not full).
+__device__ __forceinline__ void nn_insert(float (&lj)[NN_W], int32_t (&ln)[NN_W], int32_t cand, double m, int w) {
+    int ins = w, last = -1;
+#pragma unroll
+    for (int k = NN_W - 1; k >= 0; --k)
+        if (k < w && !(lj[k] >= 0.f && (double)lj[k] >= m)) ins = k;
+#pragma unroll
+    for (int k = 0; k < NN_W; ++k)
+        if (k < w && lj[k] != -1.f) last = k;
+    if (ins >= w) return;
+    if (lj[0] == -1.f && ins == 0) {  // empty list
+        lj[0] = (float)m;
+        ln[0] = cand;
+        return;
+    }
+    bool empty_slot = false;
+#pragma unroll
+    for (int k = 0; k < NN_W; ++k)
+        if (k == ins) empty_slot = lj[k] == -1.f;
+    if (!empty_slot) {
+#pragma unroll
+        for (int k = NN_W - 1; k >= 1; --k)
+            if (k > ins && k <= last) {
+                lj[k] = lj[k - 1];
+                ln[k] = ln[k - 1];
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < NN_W; ++k)
+        if (k == ins) {
+            lj[k] = (float)m;
+            ln[k] = cand;
+        }
+}
+
+// One workgroup per bucket [bstart, bend) of the nnz-sorted rows.
+__global__ __launch_bounds__(RB) void k_reorder_nn(int32_t N, const int32_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ col,
+                                                   const int32_t* __restrict__ nlines,
+                                                   const int32_t* __restrict__ bstart,
+                                                   const int32_t* __restrict__ bend, int32_t nbuckets, int w,
+                                                   int32_t* __restrict__ lu_pool, int64_t L,
+                                                   int32_t* __restrict__ perm) {
+    __shared__ int8_t vis[NN_B];
+    __shared__ double cj[NN_B];
+    __shared__ float jac[NN_W][NN_W];
+    __shared__ int32_t nid[NN_W][NN_W];
+    __shared__ int32_t s_next;
+    const int tid = threadIdx.x;
+    int32_t* lu = lu_pool + (int64_t)blockIdx.x * L;
+    for (int b = blockIdx.x; b < nbuckets; b += gridDim.x) {
+        const int32_t start = bstart[b], end = bend[b], n = end - start;
+        for (int i = tid; i < n; i += RB) vis[i] = 0;
+        __syncthreads();
+        int32_t index = start, pos = start;
+        int nnb = 0, count = 0;
+        while (index != N) {
+            const int slot = count;
+            if (tid == 0) {
+                vis[index - start] = 3;
+                perm[pos] = index;
+            }
+            ++pos;
+            if (++count == w) count = 0;
+            if (nnb < w) ++nnb;
+            // find_next_row for the new window row: its lines, then the
+            // similarity of every unvisited row of the bucket
+            // lines past N/16 (columns >= N) are not tracked, as the
+            // reference's Int32Array(N/16 + 1) drops those stores
+            for (int32_t j = rp[index] + tid; j < rp[index + 1]; j += RB) {
+                const int32_t ln = col[j] >> 4;
+                if (ln >= 0 && ln < L) lu[ln] = index;
+            }
+            __threadfence_block();
+            __syncthreads();
+            for (int i = tid; i < n; i += RB) {
+                if (vis[i] != 0) continue;
+                const int32_t c = start + i, e = rp[c + 1];
+                int32_t al = 0;
+                for (int32_t j = rp[c]; j < e;) {
+                    const int32_t ln = col[j] >> 4;
+                    if (ln >= 0 && ln < L && lu[ln] == index) ++al;
+                    ++j;
+                    while (j < e && (col[j - 1] >> 4) == (col[j] >> 4)) ++j;
+                }
+                cj[i] = (double)al / (double)(nlines[index] + nlines[c] - al);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                vis[index - start] = 1;
+                float lj[NN_W];
+                int32_t ln[NN_W];
+#pragma unroll
+                for (int k = 0; k < NN_W; ++k) {
+                    lj[k] = -1.f;
+                    ln[k] = N;
+                }
+                for (int i = 0; i < n; ++i)
+                    if (vis[i] == 0) nn_insert(lj, ln, start + i, cj[i], w);
+#pragma unroll
+                for (int k = 0; k < NN_W; ++k) {
+                    jac[slot][k] = lj[k];
+                    nid[slot][k] = ln[k];
+                }
+                // next row: each slot's best unvisited candidate; a later
+                // slot wins a tie
+                int32_t next = N;
+                double mx = 0;
+                for (int s2 = 0; s2 < nnb; ++s2) {
+                    int32_t nx = N;
+                    double m = 0;
+                    for (int k = 0; k < w; ++k) {
+                        const float v = jac[s2][k];
+                        const int32_t q = nid[s2][k];
+                        if (v >= 0.f && q < N && vis[q - start] == 0) {
+                            nx = q;
+                            m = v;
+                            break;
+                        }
+                    }
+                    if (mx <= m && nx != N) {
+                        mx = m;
+                        next = nx;
+                    }
+                }
+                s_next = next;
+            }
+            __syncthreads();
+            index = s_next;
+            __syncthreads();
+        }
+    }
 }
 
 template <typename T>
@@ -184,5 +357,90 @@ SABLE_API int sable_permute_rows(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t 
         return fail(SABLE_EINVAL, "bad dtype");
     }
     SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+SABLE_API int sable_csr_reorder_nn(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                                   const int32_t* col, const void* val, const int32_t* perm_in, int w,
+                                   int32_t* out_row_ptr, int32_t* out_col, void* out_val, int32_t* out_perm) {
+    SABLE_CHECK(ctx && row_ptr && out_row_ptr && out_perm, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(N >= 0, SABLE_EINVAL, "negative size");
+    SABLE_CHECK(w >= 1 && w <= NN_W, SABLE_EINVAL, "window w=%d outside [1, %d]", w, NN_W);
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    if (N == 0) return SABLE_OK;
+    hipStream_t st = ctx->stream;
+    // buckets from the row-length histogram, as reorder_NN forms them
+    std::vector<int32_t> hrp((size_t)N + 1);
+    SABLE_HIP(hipMemcpyAsync(hrp.data(), row_ptr, 4 * ((size_t)N + 1), hipMemcpyDeviceToHost, st));
+    SABLE_HIP(hipStreamSynchronize(st));
+    SABLE_CHECK(hrp[0] == 0, SABLE_EINVAL, "row_ptr[0] must be 0");
+    int32_t maxl = 0, first = -1;
+    for (int32_t i = 0; i < N; ++i) {
+        const int32_t l = hrp[i + 1] - hrp[i];
+        SABLE_CHECK(l >= 0, SABLE_EMALFORMED, "row_ptr decreases at row %d", i);
+        if (l > maxl) maxl = l;
+        if (first < 0 && l > 0) first = i;
+    }
+    std::vector<int64_t> freq((size_t)maxl + 1, 0);
+    for (int32_t i = 0; i < N; ++i) freq[hrp[i + 1] - hrp[i]]++;
+    std::vector<int32_t> bs, be;
+    int64_t start = first < 0 ? N : first;
+    for (int32_t c = 1; c <= maxl; ++c)
+        while (freq[c] > 0) {
+            const int64_t take = std::min<int64_t>(freq[c], NN_B);
+            bs.push_back((int32_t)start);
+            be.push_back((int32_t)(start + take));
+            freq[c] -= take;
+            start += take;
+        }
+    const int32_t nb = (int32_t)bs.size();
+    const int64_t L = (int64_t)N / 16 + 1;
+    const int nwg = (int)std::max<int64_t>(1, std::min<int64_t>(nb, std::min<int64_t>(512, 4 * (int64_t)ctx->num_cu)));
+    const size_t need = Carver::round(4 * (size_t)N) * 3 + Carver::round(4 * (size_t)std::max(nb, 1)) * 2 +
+                        Carver::round(4 * (size_t)nwg * L) + 256 + scan_tmp_bytes(N);
+    void* s;
+    int rc = ctx_scratch(ctx, need, &s);
+    if (rc) return rc;
+    Carver cv(s);
+    int32_t* nl = cv.take<int32_t>(N);
+    int32_t* pnn = cv.take<int32_t>(N);
+    int32_t* nlen = cv.take<int32_t>(N);
+    int32_t* dbs = cv.take<int32_t>(std::max(nb, 1));
+    int32_t* dbe = cv.take<int32_t>(std::max(nb, 1));
+    int32_t* lu = cv.take<int32_t>((size_t)nwg * L);
+    void* tmp = cv.p + cv.used;
+    // identity, then each bucket's walk overwrites its own positions
+    hipLaunchKernelGGL(k_iota, dim3(gs(N)), dim3(RB), 0, st, N, pnn);
+    SABLE_LAUNCH_CHECK();
+    if (nb > 0) {
+        SABLE_HIP(hipMemcpyAsync(dbs, bs.data(), 4 * (size_t)nb, hipMemcpyHostToDevice, st));
+        SABLE_HIP(hipMemcpyAsync(dbe, be.data(), 4 * (size_t)nb, hipMemcpyHostToDevice, st));
+        SABLE_HIP(hipMemsetAsync(lu, 0xff, 4 * (size_t)nwg * L, st));
+        hipLaunchKernelGGL(k_nlines, dim3(gs(N)), dim3(RB), 0, st, N, row_ptr, col, nl);
+        SABLE_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_reorder_nn, dim3(nwg), dim3(RB), 0, st, N, row_ptr, col, nl, dbs, dbe, nb, w, lu, L,
+                           pnn);
+        SABLE_LAUNCH_CHECK();
+    }
+    // rows of the input in the new order
+    hipLaunchKernelGGL(k_row_len_of, dim3(gs(N)), dim3(RB), 0, st, N, row_ptr, pnn, nlen);
+    SABLE_LAUNCH_CHECK();
+    rc = scan_exclusive_i32(st, nlen, out_row_ptr, N, out_row_ptr + N, tmp);
+    if (rc) return rc;
+    const unsigned gc = gs((int64_t)std::min<int64_t>(N, 1 << 20) * WAVE);
+    if (dtype == SABLE_F32)
+        hipLaunchKernelGGL(k_copy_rows<float>, dim3(gc), dim3(RB), 0, st, N, pnn, row_ptr, col, (const float*)val,
+                           out_row_ptr, out_col, (float*)out_val);
+    else
+        hipLaunchKernelGGL(k_copy_rows<double>, dim3(gc), dim3(RB), 0, st, N, pnn, row_ptr, col, (const double*)val,
+                           out_row_ptr, out_col, (double*)out_val);
+    SABLE_LAUNCH_CHECK();
+    // permutation of the original rows
+    if (perm_in)
+        hipLaunchKernelGGL(k_gather_rows<int32_t>, dim3(gs(N)), dim3(RB), 0, st, N, pnn, perm_in, out_perm);
+    else
+        SABLE_HIP(hipMemcpyAsync(out_perm, pnn, 4 * (size_t)N, hipMemcpyDeviceToDevice, st));
+    SABLE_LAUNCH_CHECK();
+    SABLE_HIP(hipStreamSynchronize(st));  // the bucket vectors are host memory of this call
     return SABLE_OK;
 }

@@ -35,6 +35,30 @@ def sort_rows_by_nnz(A):
     return B
 
 
+def reorder_NN(A, w=16):
+    """par/src/sparse32.js:2501-2622: locality ordering of a matrix returned
+    by sort_rows_by_nnz (rows of equal nnz regrouped so that consecutive rows
+    share cache lines of x).  The result carries the composed permutation and
+    the short-row counts of A."""
+    if not isinstance(A, sparse_CSR_t):
+        raise TypeError("reorder_NN takes a CSR matrix")
+    ctx = A.row.ctx
+    N = A.N
+    rp = DeviceArray(N + 1, np.int32, ctx)
+    col = DeviceArray(A.nnz, np.int32, ctx)
+    val = DeviceArray(A.nnz, A.dtype, ctx)
+    perm = DeviceArray(N, np.int32, ctx)
+    pin = getattr(A, "permutation", None)
+    call("sable_csr_reorder_nn", ctx.handle, dtype_code(A.dtype), N, A.row.ptr, A.col.ptr, A.val.ptr,
+         pin.ptr if pin is not None else None, int(w), rp.ptr, col.ptr, val.ptr, perm.ptr)
+    B = sparse_CSR_t(rp, col, val, N, A.nnz)
+    B.permutation = perm
+    for k in ("nnz_row", "num_zero_rows", "num_one_rows", "num_two_rows", "num_three_rows"):
+        if hasattr(A, k):
+            setattr(B, k, getattr(A, k))
+    return B
+
+
 def sort_y_rows_by_nnz(y_view, A):
     """y_new[A.permutation[i]] = y[i]; y_view then holds y_new (the
     reference swaps y_view's storage the same way)."""

@@ -400,3 +400,49 @@ def test_sort_rows_by_nnz_is_a_stable_sort_and_spmv_invariant():
     ys = oracle.spmv_csr(N, orp, oc, ov, x, z, 1)
     # each row keeps its entries in order, so each product is bit-identical
     assert np.array_equal(oracle.sort_y_rows_by_nnz(perm, ys), y)
+
+
+def test_reorder_nn_hand():
+    """64 rows, rows 0-3 hold one entry each in cache lines 0, 2, 0, 2
+    (columns 0, 40, 5, 45); the rest are empty.  After the sort by nnz they
+    are rows 60-63.  Worked by hand through reorder_subset: 60 starts; its
+    list gets 61 (similarity 0), then 62 (1.0) overwrites 61 in place (the
+    insertion replaces the last entry instead of shifting it), then 63 (0);
+    62 follows; from 62 the tie between 63 (slot 0) and 61 (slot 1) goes to
+    the later slot, 61; then 63."""
+    N = 64
+    rp = np.zeros(N + 1, np.int32)
+    rp[1:5] = [1, 2, 3, 4]
+    rp[5:] = 4
+    col = np.array([0, 40, 5, 45], np.int32)
+    val = np.ones(4, np.float64)
+    orp, oc, _, sperm, _, _ = oracle.sort_rows_by_nnz(N, rp, col, val)
+    assert sperm[60:].tolist() == [0, 1, 2, 3]
+    perm = oracle.reorder_nn(N, orp, oc, 16)
+    assert perm[:60].tolist() == list(range(60))
+    assert perm[60:].tolist() == [60, 62, 61, 63]
+    assert sperm[perm][60:].tolist() == [0, 2, 1, 3]
+
+
+def test_reorder_nn_is_a_permutation_that_groups_shared_lines():
+    from sciwasm.sparse_amd import gen
+    rp, col, val = gen.laplacian_2d(48)  # 2304 rows, 5-point stencil
+    N = len(rp) - 1
+    # scatter the rows, so that neighbouring grid rows are far apart
+    p = np.random.default_rng(0).permutation(N)
+    lens = np.diff(rp)[p]
+    srp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    scol = np.concatenate([col[rp[r]:rp[r + 1]] for r in p]).astype(np.int32)
+    orp, oc, ov, sperm, _, _ = oracle.sort_rows_by_nnz(N, srp, scol, np.ones(len(scol)))
+    perm = oracle.reorder_nn(N, orp, oc, 16)
+    assert np.array_equal(np.sort(perm), np.arange(N))
+
+    def shared(rpx, colx, order):
+        # cache lines shared by consecutive rows of the ordering
+        s = 0
+        for a, b in zip(order[:-1], order[1:]):
+            la = set((colx[rpx[a]:rpx[a + 1]] // 16).tolist())
+            lb = set((colx[rpx[b]:rpx[b + 1]] // 16).tolist())
+            s += len(la & lb)
+        return s
+    assert shared(orp, oc, perm) > 2 * shared(orp, oc, np.arange(N))

@@ -57,3 +57,37 @@ def test_permute_rows_round_trip(ctx):
     assert np.array_equal(g.numpy(), x[perm])
     back = sp.permute_rows(g, dp, scatter=True)
     assert np.array_equal(back.numpy(), x)
+
+
+def scattered_laplacian(n, seed):
+    rp, col, val = gen.laplacian_2d(n)
+    N = len(rp) - 1
+    p = np.random.default_rng(seed).permutation(N)
+    lens = np.diff(rp)[p]
+    srp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    idx = np.concatenate([np.arange(rp[r], rp[r + 1]) for r in p])
+    return N, srp, col[idx].astype(np.int32), val[idx]
+
+
+@pytest.mark.parametrize("w", [16, 4, 1])
+@pytest.mark.parametrize("kind", ["laplacian", "random", "zipf"])
+def test_reorder_nn_matches_oracle(ctx, kind, w):
+    if kind == "laplacian":
+        N, rp, col, val = scattered_laplacian(150, 1)  # 22500 rows, buckets of 1024 and remainders
+    elif kind == "random":
+        rp, col, val = gen.random_csr(30000, 3, seed=2, dtype=np.float64, empty_rows=0.1)
+        N = len(rp) - 1
+    else:
+        rp, col, val = gen.zipf_csr(40000, 1.6, 256, seed=3)
+        N = len(rp) - 1
+    print("reorder_nn", kind, w, N, flush=True)
+    A = sp.csr_from_host(N, rp, col, val, ctx)
+    S = sp.sort_rows_by_nnz(A)
+    R = sp.reorder_NN(S, w)
+    orp, oc, ov, sperm, _, _ = oracle.sort_rows_by_nnz(N, rp, col, val)
+    pnn = oracle.reorder_nn(N, orp, oc, w)
+    final = sperm[pnn]
+    assert np.array_equal(R.permutation.numpy(), final)
+    assert np.array_equal(R.row.numpy(), np.concatenate([[0], np.cumsum(np.diff(rp)[final])]))
+    assert np.array_equal(R.col.numpy(), np.concatenate([col[rp[r]:rp[r + 1]] for r in final]))
+    assert R.num_one_rows == S.num_one_rows
