@@ -375,6 +375,81 @@ def sec_ingest(args, sp, ctx, T):
     return out
 
 
+# ------------------------------------------------------------------ element-wise
+def sec_ewise(args, sp, ctx, T, host):
+    """Element-wise ops on the headline matrix's values (206M FP64 values):
+    out of place and in place (each value read and written once, 16 B), and
+    eliminate_zeros on a CSR copy with 10% of its values zeroed."""
+    from sciwasm.sparse_amd._lib import call
+    rp, col, val, _ = host
+    n = len(val)
+    a = sp.DeviceArray.from_host(val, ctx=ctx)
+    b = sp.DeviceArray(n, np.float64, ctx)
+    ops = sp.ewise.OPS
+    out = {"values": n}
+    for name, src_dst in (("sqrt", (a, b)), ("neg_in_place", (a, a)), ("expm1", (a, b)), ("sin", (a, b))):
+        op = ops.index(name.split("_")[0])
+        ms = T(lambda: call("sable_ew_values", ctx.handle, 1, op, 0.0, n, src_dst[0].ptr, src_dst[1].ptr),
+               max(3, args.steps // 4), 1)
+        out[name] = {"ms": r4(ms), "gbs": r4(16.0 * n / (ms * 1e-3) / 1e9),
+                     "pct_hbm": r4(100 * 16.0 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)}
+    del a, b
+    z = val.copy()
+    z[np.random.default_rng(7).random(n) < 0.1] = 0
+    A = sp.csr_from_host(len(rp) - 1, rp, col, z, ctx)
+    del z
+    t, B = T.once(lambda: sp.ewise._compacted(A))
+    # algorithmic bytes: read col+val (12 B) and write the kept 12 B, row_ptr twice
+    byt = 12.0 * n + 12.0 * B.nnz + 8.0 * (A.N + 1)
+    out["eliminate_zeros_csr"] = {"ms": r4(t), "kept": int(B.nnz), "gbs": r4(byt / (t * 1e-3) / 1e9)}
+    log("ewise:", out)
+    return out
+
+
+# ------------------------------------------------------------------ reordering
+def sec_reorder(args, sp, ctx, T, host):
+    """sort_rows_by_nnz on the headline matrix; reorder_NN on a 2D Laplacian
+    whose rows were scattered, with the CSR product timed before and after
+    (the locality the ordering gives back to the x gathers)."""
+    from sciwasm.sparse_amd import gen
+    rp, col, val, _ = host
+    N = len(rp) - 1
+    A = sp.csr_from_host(N, rp, col, val, ctx)
+    t_sort, S = T.once(lambda: sp.sort_rows_by_nnz(A))
+    out = {"sort_rows_by_nnz": {"N": N, "nnz": len(val), "ms": r4(t_sort),
+                                "gbs": r4((24.0 * len(val) + 16.0 * N) / (t_sort * 1e-3) / 1e9),
+                                "bins": [S.num_zero_rows, S.num_one_rows, S.num_two_rows, S.num_three_rows]}}
+    del A, S
+    n = args.reorder_grid
+    lrp, lcol, lval = gen.laplacian_2d(n)
+    M = len(lrp) - 1
+    p = np.random.default_rng(3).permutation(M)
+    lens = np.diff(lrp)[p]
+    srp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    starts = lrp[p]
+    idx = (np.repeat(starts - np.concatenate([[0], np.cumsum(lens)[:-1]]), lens) + np.arange(srp[-1])).astype(np.int64)
+    scol, sval = lcol[idx].astype(np.int32), lval[idx]
+    L = sp.csr_from_host(M, srp, scol, sval, ctx)
+    x = sp.vec_from_host(np.random.default_rng(4).standard_normal(M), ctx=ctx)
+    y = sp.create_vec(M, 0.0, np.float64, ctx)
+
+    def spmv_ms(Mx):
+        pl = Mx.plan()
+        return T(lambda: sp.spmv(Mx, x, y), args.steps, 2), pl
+    t_scat, _ = spmv_ms(L)
+    t_s2, S = T.once(lambda: sp.sort_rows_by_nnz(L))
+    t_nn, R = T.once(lambda: sp.reorder_NN(S, 16))
+    t_re, _ = spmv_ms(R)
+    byt = 4.0 * (M + 1) + 12.0 * len(sval) + 16.0 * M
+    out["reorder_NN"] = {"matrix": "2D Laplacian %dx%d, rows scattered" % (n, n), "N": M, "nnz": len(sval),
+                         "sort_ms": r4(t_s2), "reorder_nn_ms": r4(t_nn),
+                         "spmv_scattered_ms": r4(t_scat), "spmv_reordered_ms": r4(t_re),
+                         "spmv_scattered_gbs": r4(byt / (t_scat * 1e-3) / 1e9),
+                         "spmv_reordered_gbs": r4(byt / (t_re * 1e-3) / 1e9)}
+    log("reorder:", out)
+    return out
+
+
 # ------------------------------------------------------------------ spts
 def sec_spts(args, sp, ctx, T):
     import oracle
@@ -574,7 +649,8 @@ def main():
                     help="comma list of row-block weights (nnz + 2*rows) to time with each panel width (0 = 1024)")
     ap.add_argument("--ingest-rows", type=int, default=4194304)
     ap.add_argument("--ingest-entries", type=int, default=8000000)
-    ap.add_argument("--sections", type=str, default="c1,band,convert,ingest,spts,batch,serve",
+    ap.add_argument("--reorder-grid", type=int, default=1000)
+    ap.add_argument("--sections", type=str, default="c1,band,convert,ingest,ewise,reorder,spts,batch,serve",
                     help="comma list of extra sections (empty for the headline only)")
     ap.add_argument("--batch-count", type=int, default=2048)
     ap.add_argument("--serve-rates", type=str, default="2000,10000,30000,60000,120000,240000")
@@ -615,6 +691,10 @@ def main():
         run("convert", lambda: sec_convert(args, sp, ctx, T, host))
     if "ingest" in sections:
         run("ingest", lambda: sec_ingest(args, sp, ctx, T))
+    if "ewise" in sections:
+        run("ewise", lambda: sec_ewise(args, sp, ctx, T, host))
+    if "reorder" in sections:
+        run("reorder", lambda: sec_reorder(args, sp, ctx, T, host))
     if "spts" in sections:
         run("spts", lambda: sec_spts(args, sp, ctx, T))
     pool = lows = None

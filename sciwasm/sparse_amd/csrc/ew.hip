@@ -94,6 +94,50 @@ __global__ void k_ew_self(Op<T> f, int64_t n, T* v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) v[i] = f(v[i]);
 }
 
+// Bulk of a 16-byte aligned array: 16-byte vectors on a grid of 4
+// workgroups per CU that strides through the array (the fastest of 1-8
+// vectors in flight per thread and 1-256 workgroups per CU measured on
+// MI355X: about the device-to-device copy rate).  in and out may be the
+// same array (each vector is read, then written, by one thread).
+constexpr int EW_U = 1;
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef float4 type;
+    static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+    typedef double2 type;
+    static constexpr int n = 2;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ew_vec(Op<T> f, int64_t nvec, const typename Vec16<T>::type* in,
+                                                typename Vec16<T>::type* out) {
+    typedef typename Vec16<T>::type V;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += stride * EW_U) {
+        V v[EW_U];
+#pragma unroll
+        for (int u = 0; u < EW_U; ++u) {
+            const int64_t k = i + u * stride;
+            if (k < nvec) v[u] = in[k];
+        }
+#pragma unroll
+        for (int u = 0; u < EW_U; ++u) {
+            const int64_t k = i + u * stride;
+            if (k < nvec) {
+                T* e = (T*)&v[u];
+#pragma unroll
+                for (int c = 0; c < Vec16<T>::n; ++c) e[c] = f(e[c]);
+                out[k] = v[u];
+            }
+        }
+    }
+}
+
 // DIA slot range of diagonal d: rows [lo, hi), data index base + row.  The
 // range is clamped to the diagonal's own stride slots, so an offset that
 // does not fit the stride (|k| < N - stride) cannot reach another diagonal.
@@ -290,6 +334,25 @@ Op<T> make_op(int op, double param) {
 template <typename T>
 int ew_values_t(sable_ctx_t* ctx, int op, double param, int64_t n, const void* in, void* out) {
     const Op<T> f = make_op<T>(op, param);
+    typedef typename Vec16<T>::type V;
+    const int64_t per = Vec16<T>::n;
+    if ((((uintptr_t)in | (uintptr_t)out) & 15) == 0 && n >= per) {
+        const int64_t nvec = n / per, rest = n - nvec * per;
+        const int64_t blocks = std::min<int64_t>((nvec + 256 * EW_U - 1) / (256 * EW_U), 4 * (int64_t)ctx->num_cu);
+        hipLaunchKernelGGL(k_ew_vec<T>, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0, ctx->stream, f,
+                           nvec, (const V*)in, (V*)out);
+        SABLE_LAUNCH_CHECK();
+        if (rest) {
+            const T* ti = (const T*)in + nvec * per;
+            T* to = (T*)out + nvec * per;
+            if (in == out)
+                hipLaunchKernelGGL(k_ew_self<T>, dim3(1), dim3(EB), 0, ctx->stream, f, rest, to);
+            else
+                hipLaunchKernelGGL(k_ew<T>, dim3(1), dim3(EB), 0, ctx->stream, f, rest, ti, to);
+            SABLE_LAUNCH_CHECK();
+        }
+        return SABLE_OK;
+    }
     if (in == out)
         hipLaunchKernelGGL(k_ew_self<T>, dim3(gs_cap(n)), dim3(EB), 0, ctx->stream, f, n, (T*)out);
     else
