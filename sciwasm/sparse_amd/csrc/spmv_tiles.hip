@@ -65,7 +65,7 @@ __global__ void k_tile_rows(int32_t N, const int32_t* __restrict__ rp, int32_t* 
 // that the workgroups working on one panel at the same time read one
 // compact region of memory
 __global__ void k_tile_keys(int64_t nnz, int32_t N, const int32_t* __restrict__ rows, const int32_t* __restrict__ col,
-                            int row_bits, int col_bits, int32_t ntr, uint64_t* __restrict__ keys,
+                            int32_t tile_rows, int col_bits, int32_t ntr, uint64_t* __restrict__ keys,
                             uint32_t* __restrict__ idx, int* __restrict__ err) {
     const uint32_t cmask = (1u << col_bits) - 1u;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
@@ -74,7 +74,7 @@ __global__ void k_tile_keys(int64_t nnz, int32_t N, const int32_t* __restrict__ 
             atomicOr(err, 1);
             keys[k] = 0;
         } else {
-            const int64_t t = (int64_t)(c >> col_bits) * ntr + (rows[k] >> row_bits);
+            const int64_t t = (int64_t)(c >> col_bits) * ntr + (rows[k] / tile_rows);
             keys[k] = ((uint64_t)t << col_bits) | ((uint32_t)c & cmask);
         }
         idx[k] = (uint32_t)k;
@@ -83,12 +83,13 @@ __global__ void k_tile_keys(int64_t nnz, int32_t N, const int32_t* __restrict__ 
 
 template <typename T>
 __global__ void k_tile_pack(int64_t nnz, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ idx,
-                            const int32_t* __restrict__ rows, const T* __restrict__ val, int row_bits, int col_bits,
-                            uint32_t* __restrict__ packed, T* __restrict__ vout) {
-    const uint32_t cmask = (1u << col_bits) - 1u, rmask = (1u << row_bits) - 1u;
+                            const int32_t* __restrict__ rows, const T* __restrict__ val, int32_t tile_rows,
+                            int row_bits, int col_bits, uint32_t* __restrict__ packed, T* __restrict__ vout) {
+    const uint32_t cmask = (1u << col_bits) - 1u;
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nnz; s += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t k = idx[s];
-        packed[s] = (((uint32_t)keys[s] & cmask) << row_bits) | ((uint32_t)rows[k] & rmask);
+        const uint32_t rin = (uint32_t)(rows[k] % tile_rows);
+        packed[s] = (((uint32_t)keys[s] & cmask) << row_bits) | rin;
         vout[s] = val[k];
     }
 }
@@ -254,8 +255,11 @@ SABLE_API int sable_csr_tiles_create(sable_ctx_t* ctx, sable_dtype_t dtype, int3
     const size_t vs = dtype == SABLE_F64 ? 8 : 4;
     if (tile_rows <= 0) tile_rows = 16384;
     if (panel_cols <= 0) panel_cols = 8192;
-    const int rb = log2_exact(tile_rows), cb = log2_exact(panel_cols);
-    SABLE_CHECK(rb >= 6 && cb >= 6, SABLE_EINVAL, "tile_rows and panel_cols must be powers of two >= 64");
+    // tile_rows: a multiple of 64 (row-in-tile takes bits_for(tile_rows - 1)
+    // bits of the packed word); panel_cols: a power of two
+    SABLE_CHECK(tile_rows >= 64 && tile_rows % 64 == 0, SABLE_EINVAL, "tile_rows must be a multiple of 64");
+    const int rb = bits_for(tile_rows - 1), cb = log2_exact(panel_cols);
+    SABLE_CHECK(cb >= 6, SABLE_EINVAL, "panel_cols must be a power of two >= 64");
     SABLE_CHECK(rb + cb <= 32, SABLE_EINVAL, "tile_rows * panel_cols must not exceed 2^32");
     SABLE_CHECK((size_t)tile_rows * vs <= 160 * 1024, SABLE_EINVAL, "a tile's row sums must fit the LDS (160 KiB)");
     hipStream_t st = ctx->stream;
@@ -307,8 +311,8 @@ SABLE_API int sable_csr_tiles_create(sable_ctx_t* ctx, sable_dtype_t dtype, int3
     SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
     hipLaunchKernelGGL(k_tile_rows, dim3(gs(N)), dim3(256), 0, st, N, row_ptr, rows);
     if (nnz > 0)
-        hipLaunchKernelGGL(k_tile_keys, dim3(gs(nnz)), dim3(256), 0, st, nnz, N, rows, col, rb, cb, p->ntr, keys,
-                           idx, err);
+        hipLaunchKernelGGL(k_tile_keys, dim3(gs(nnz)), dim3(256), 0, st, nnz, N, rows, col, tile_rows, cb, p->ntr,
+                           keys, idx, err);
     if (hipGetLastError() != hipSuccess) return cleanup(fail(SABLE_EHIP, "tile key launch failed"));
     if (hipMemcpyAsync(ctx->pinned, err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
@@ -319,10 +323,10 @@ SABLE_API int sable_csr_tiles_create(sable_ctx_t* ctx, sable_dtype_t dtype, int3
         if (rc) return cleanup(rc);
         if (dtype == SABLE_F64)
             hipLaunchKernelGGL(k_tile_pack<double>, dim3(gs(nnz)), dim3(256), 0, st, nnz, keys, idx, rows,
-                               (const double*)val, rb, cb, p->packed, (double*)p->val);
+                               (const double*)val, tile_rows, rb, cb, p->packed, (double*)p->val);
         else
             hipLaunchKernelGGL(k_tile_pack<float>, dim3(gs(nnz)), dim3(256), 0, st, nnz, keys, idx, rows,
-                               (const float*)val, rb, cb, p->packed, (float*)p->val);
+                               (const float*)val, tile_rows, rb, cb, p->packed, (float*)p->val);
     }
     hipLaunchKernelGGL(k_tile_heads, dim3((unsigned)grid_for(nnz + 1, 256)), dim3(256), 0, st, nnz, ntiles, cb, keys,
                        p->tile_ptr);

@@ -218,7 +218,8 @@ def tiles_mode(request):
     _lib.call("sable_csr_tiles_select", 1)
 
 
-@pytest.mark.parametrize("tile_rows,panel_cols", [(64, 128), (256, 1024), (16384, 8192), (0, 0)])
+@pytest.mark.parametrize("tile_rows,panel_cols", [(64, 128), (192, 256), (256, 1024), (16384, 8192), (20480, 8192),
+                                                 (0, 0)])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("name,make", CASES)
 def test_csr_tiles_parity(ctx, tiles_mode, tile_rows, panel_cols, dtype, name, make):
