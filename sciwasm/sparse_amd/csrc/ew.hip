@@ -94,22 +94,23 @@ __global__ void k_ew_self(Op<T> f, int64_t n, T* v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) v[i] = f(v[i]);
 }
 
-// Bulk of a 16-byte aligned array: 16-byte vectors on a grid of 4
-// workgroups per CU that strides through the array (the fastest of 1-8
-// vectors in flight per thread and 1-256 workgroups per CU measured on
-// MI355X: about the device-to-device copy rate).  in and out may be the
-// same array (each vector is read, then written, by one thread).
-constexpr int EW_U = 1;
+// Bulk of a 16-byte aligned array: 16-byte vectors, EW_U of them loaded per
+// thread before any is computed, with non-temporal loads and stores (the
+// values are touched once), on a grid of up to 16384 workgroups; the
+// steadiest of the shapes measured on MI355X, near the device-to-device
+// copy rate.  in and out may be the same array (each vector is read, then
+// written, by one thread).
+constexpr int EW_U = 4;
 template <typename T>
 struct Vec16;
 template <>
 struct Vec16<float> {
-    typedef float4 type;
+    typedef float type __attribute__((ext_vector_type(4)));
     static constexpr int n = 4;
 };
 template <>
 struct Vec16<double> {
-    typedef double2 type;
+    typedef double type __attribute__((ext_vector_type(2)));
     static constexpr int n = 2;
 };
 
@@ -123,16 +124,15 @@ __global__ __launch_bounds__(256) void k_ew_vec(Op<T> f, int64_t nvec, const typ
 #pragma unroll
         for (int u = 0; u < EW_U; ++u) {
             const int64_t k = i + u * stride;
-            if (k < nvec) v[u] = in[k];
+            if (k < nvec) v[u] = __builtin_nontemporal_load(in + k);
         }
 #pragma unroll
         for (int u = 0; u < EW_U; ++u) {
             const int64_t k = i + u * stride;
             if (k < nvec) {
-                T* e = (T*)&v[u];
 #pragma unroll
-                for (int c = 0; c < Vec16<T>::n; ++c) e[c] = f(e[c]);
-                out[k] = v[u];
+                for (int c = 0; c < Vec16<T>::n; ++c) v[u][c] = f(v[u][c]);
+                __builtin_nontemporal_store(v[u], out + k);
             }
         }
     }
@@ -338,7 +338,7 @@ int ew_values_t(sable_ctx_t* ctx, int op, double param, int64_t n, const void* i
     const int64_t per = Vec16<T>::n;
     if ((((uintptr_t)in | (uintptr_t)out) & 15) == 0 && n >= per) {
         const int64_t nvec = n / per, rest = n - nvec * per;
-        const int64_t blocks = std::min<int64_t>((nvec + 256 * EW_U - 1) / (256 * EW_U), 4 * (int64_t)ctx->num_cu);
+        const int64_t blocks = std::min<int64_t>((nvec + 256 * EW_U - 1) / (256 * EW_U), 16384);
         hipLaunchKernelGGL(k_ew_vec<T>, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0, ctx->stream, f,
                            nvec, (const V*)in, (V*)out);
         SABLE_LAUNCH_CHECK();
