@@ -8,7 +8,8 @@ nonzeros), per-row columns uniform without replacement, values N(0,1),
 x ~ U(-1,1), int32 indices, float64 values; inputs resident in HBM.  One
 step is one whole product.  Bytes per step (algorithmic) = 4(N+1) +
 12 nnz + 8N (x once) + 8N (y).  The fastest of the measured schedules is
-reported (row-block CSR, or the column-panel blocked plan of the same CSR).
+reported (row-block CSR, the column-panel blocked plan, or the column-sorted
+tile plan of the same CSR).
 
 Also measured and reported under "extra" (each section bounded to seconds):
   c1      FP32 5-point Laplacian 256x256 (65,536 rows): CSR/ELL/DIA/COO
@@ -16,7 +17,13 @@ Also measured and reported under "extra" (each section bounded to seconds):
   band    FP64 7-diagonal band, 16,777,216 rows (offsets 0, +-1, +-64,
           +-4096): csr_dia / csr_ell / coo_csr / transpose on the GPU and
           DIA/ELL/CSR SpMV GB/s
+  zipf16  the same product on the sparser Zipf(s = 1.73) matrix of the same
+          N (mean about 16, about 67M nonzeros), two tile shapes
   convert coo_csr of the headline matrix (shuffled COO, 206M entries)
+  ingest  MatrixMarket text parsed on the device and expanded to COO
+  ewise   element-wise ops and eliminate_zeros on the headline values
+  reorder sort_rows_by_nnz of the headline matrix, reorder_NN of a
+          scattered 2D Laplacian
   spts    lower solves, 3D 7-point Laplacian 128^3 and a random 1,048,576-row
           lower matrix: analysis, level-set (hipGraph), sync-free, FP32 +
           FP64 refinement, residuals
@@ -240,6 +247,41 @@ def sec_c1(args, sp, ctx, T):
 
 
 # ------------------------------------------------------------------ band
+def sec_zipf16(args, sp, ctx, T):
+    """The same FP64 CSR SpMV on the sparser Zipf matrix whose size matches
+    the ~16 mean / ~67 M nnz quoted beside the literal s = 1.5 law: row
+    lengths Zipf(s = 1.73) truncated to [1, 4096]."""
+    from sciwasm.sparse_amd import gen
+    N = args.rows
+    rp, col, val = gen.zipf_csr(N, 1.73, 4096, seed=args.seed, dtype=np.float64)
+    nnz = int(rp[-1])
+    x = np.random.default_rng(args.seed + 1).uniform(-1.0, 1.0, N)
+    A = sp.csr_from_host(N, rp, col, val, ctx)
+    xv, yv = sp.vec_from_host(x, ctx=ctx), sp.create_vec(N, 0.0, np.float64, ctx=ctx)
+    bytes_step = 4 * (N + 1) + 12 * nnz + 8 * N + 8 * N
+    out = {"N": N, "nnz": nnz, "mean_row": round(nnz / N, 3), "zipf_s": 1.73, "bytes_step": bytes_step}
+    R = N // 8
+    ab = np.abs(val[:rp[R]] * x[col[:rp[R]]])
+    yref = np.add.reduceat(val[:rp[R]] * x[col[:rp[R]]], rp[:R])
+    sab = np.add.reduceat(ab, rp[:R])
+    lens = np.diff(rp[:R + 1])
+    yref[lens == 0] = 0
+    sab[lens == 0] = 0
+    u = np.finfo(np.float64).eps / 2
+    bnd = 2 * (lens + 1) * u / (1 - (lens + 1) * u) * sab + 1e-300
+    for tr, pc in ((16384, 16384), (16384, 8192)):
+        tl = sp.CsrTiles(A, tr, pc)
+        ms = T(lambda: tl.run(xv.vec, yv.vec, beta=0.0), args.steps, args.warmup)
+        y = yv.array()[:R]
+        gbs = bytes_step / (ms * 1e-3) / 1e9
+        out["tiles_%dx%d_fused" % (tr, pc)] = {"ms": r4(ms), "gbs": r4(gbs), "pct_hbm": r4(100 * gbs / HBM_PEAK_GBS),
+                                              "rows_checked": R,
+                                              "within_csr_bound": bool(np.all(np.abs(y - yref) <= bnd))}
+        del tl
+    log("zipf16:", out)
+    return out
+
+
 def sec_band(args, sp, ctx, T):
     from sciwasm.sparse_amd import gen
     N = 1 << 24
@@ -650,7 +692,7 @@ def main():
     ap.add_argument("--ingest-rows", type=int, default=4194304)
     ap.add_argument("--ingest-entries", type=int, default=8000000)
     ap.add_argument("--reorder-grid", type=int, default=1000)
-    ap.add_argument("--sections", type=str, default="c1,band,convert,ingest,ewise,reorder,spts,batch,serve",
+    ap.add_argument("--sections", type=str, default="c1,band,zipf16,convert,ingest,ewise,reorder,spts,batch,serve",
                     help="comma list of extra sections (empty for the headline only)")
     ap.add_argument("--batch-count", type=int, default=2048)
     ap.add_argument("--serve-rates", type=str, default="2000,10000,30000,60000,120000,240000")
@@ -687,6 +729,8 @@ def main():
         run("c1", lambda: sec_c1(args, sp, ctx, T))
     if "band" in sections:
         run("band", lambda: sec_band(args, sp, ctx, T))
+    if "zipf16" in sections:
+        run("zipf16", lambda: sec_zipf16(args, sp, ctx, T))
     if "convert" in sections:
         run("convert", lambda: sec_convert(args, sp, ctx, T, host))
     if "ingest" in sections:
