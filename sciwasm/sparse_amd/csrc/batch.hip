@@ -65,8 +65,116 @@ struct Unroll {
     static constexpr int U = NR * (int)sizeof(T) >= 32 ? 2 : (NR * (int)sizeof(T) >= 16 ? 4 : 8);
 };
 
+// Pipelined path for a block whose entries fit one step (every thread
+// loads at most U of them and the products fit the LDS staging buffer):
+// the entries (and the first row extents each thread reduces) are loaded
+// into a Stage, the products are formed into LDS, and the next block's
+// Stage is loaded before this block's rows are reduced, so that its loads
+// are in flight during the barrier and the reduction.
 template <typename T, int NR>
-__device__ __forceinline__ void block_spmv(const sable_batch_spmv_item_t& it, int64_t b, T* prod, T* red) {
+struct Stage {
+    static constexpr int U = Unroll<T, NR>::U;
+    int c[U];
+    T v[U];
+    int a, e;  // entry range (block-relative) of the thread's first reduced row
+};
+
+template <typename T, int NR>
+__device__ __forceinline__ bool one_step(int nnzb) {
+    return nnzb <= Stage<T, NR>::U * BB && nnzb * NR <= Lds<NR>::ELEMS;
+}
+
+// lanes per row of the reduction: the widest power of two <= WAVE that
+// gives every row of the block a group of lanes in one pass, else 1
+__device__ __forceinline__ int reduce_lanes(int R) {
+    int L = WAVE;
+    while (L > 1 && L * R > BB) L >>= 1;
+    return L;
+}
+
+template <typename T, int NR>
+__device__ __forceinline__ void stage_load(Stage<T, NR>& st, const sable_batch_spmv_item_t& it, int r0, int r1,
+                                           int64_t nz0, int nnzb) {
+    const int tid = threadIdx.x;
+    const int32_t* __restrict__ col = it.col;
+    const T* __restrict__ val = (const T*)it.val;
+#pragma unroll
+    for (int u = 0; u < Stage<T, NR>::U; u++) {
+        const int kk = tid + u * BB;
+        st.c[u] = kk < nnzb ? col[nz0 + kk] : 0;
+        st.v[u] = kk < nnzb ? val[nz0 + kk] : T(0);
+    }
+    const int L = reduce_lanes(r1 - r0);
+    const int row = r0 + tid / L;
+    st.a = row < r1 ? (int)((int64_t)it.row_ptr[row] - nz0) : 0;
+    st.e = row < r1 ? (int)((int64_t)it.row_ptr[row + 1] - nz0) : 0;
+}
+
+template <typename T, int NR>
+__device__ __forceinline__ void stage_prod(const Stage<T, NR>& st, const sable_batch_spmv_item_t& it, int nnzb,
+                                           T* prod) {
+    constexpr int U = Stage<T, NR>::U;
+    const int tid = threadIdx.x;
+    const T* __restrict__ x = (const T*)it.x;
+    T xr[U][NR];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (tid + u * BB < nnzb) load_xrow<T, NR>(x, st.c[u], xr[u]);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int kk = tid + u * BB;
+        if (kk < nnzb) {
+#pragma unroll
+            for (int r = 0; r < NR; r++) prod[kk * NR + r] = st.v[u] * xr[u][r];
+        }
+    }
+}
+
+template <typename T, int NR>
+__device__ __forceinline__ void stage_reduce(const sable_batch_spmv_item_t& it, int r0, int r1, int64_t nz0, int a0,
+                                             int e0, const T* prod) {
+    const int tid = threadIdx.x;
+    const int R = r1 - r0;
+    const int L = reduce_lanes(R);
+    const int groups = BB / L, g = tid / L, l = tid & (L - 1);
+    T* __restrict__ y = (T*)it.y;
+    const T beta = (T)it.beta;
+    for (int rb = 0; rb < R; rb += groups) {
+        const int row = r0 + rb + g;
+        T s[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) s[r] = T(0);
+        int a = 0, e = 0;
+        if (row < r1) {
+            if (rb == 0) {
+                a = a0;
+                e = e0;
+            } else {
+                a = (int)((int64_t)it.row_ptr[row] - nz0);
+                e = (int)((int64_t)it.row_ptr[row + 1] - nz0);
+            }
+            for (int kk = a + l; kk < e; kk += L)
+#pragma unroll
+                for (int r = 0; r < NR; r++) s[r] += prod[kk * NR + r];
+        }
+#pragma unroll
+        for (int r = 0; r < NR; r++) s[r] = group_sum(s[r], L);
+        if (l == 0 && row < r1) {
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                T* yp = &y[(int64_t)row * NR + r];
+                if (beta == T(0)) *yp = s[r];
+                else if (beta == T(1)) { if (e > a) *yp = *yp + s[r]; }
+                else *yp = beta * *yp + s[r];
+            }
+        }
+    }
+}
+
+// One row block [r0, r1) of an item whose entries are [nz0, nz0 + nnzb).
+template <typename T, int NR>
+__device__ __forceinline__ void block_spmv(const sable_batch_spmv_item_t& it, int r0, int r1, int64_t nz0, int nnzb,
+                                           T* prod, T* red) {
     constexpr int LDS_ELEMS = Lds<NR>::ELEMS;
     const int32_t* __restrict__ rp = it.row_ptr;
     const int32_t* __restrict__ col = it.col;
@@ -74,11 +182,8 @@ __device__ __forceinline__ void block_spmv(const sable_batch_spmv_item_t& it, in
     const T* __restrict__ x = (const T*)it.x;
     T* __restrict__ y = (T*)it.y;
     const T beta = (T)it.beta;
-    const int r0 = it.blk[b], r1 = it.blk[b + 1];
     if (r0 >= r1) return;
     const int tid = threadIdx.x;
-    const int64_t nz0 = rp[r0];
-    const int nnzb = (int)((int64_t)rp[r1] - nz0);
     const int R = r1 - r0;
     if (nnzb * NR <= LDS_ELEMS) {
         constexpr int U = Unroll<T, NR>::U;
@@ -137,7 +242,40 @@ __device__ __forceinline__ void block_spmv(const sable_batch_spmv_item_t& it, in
             }
         }
         __syncthreads();  // prod is reused by the next block of this workgroup
+    } else if (R > 1) {
+        // several rows whose products do not fit the LDS: a group of L
+        // lanes per row, L the power of two (at most a wave) at or above
+        // the block's mean row length
+        int L = 1;
+        while (L < WAVE && L * R < nnzb) L <<= 1;
+        const int grp = tid / L, lane = tid & (L - 1);
+        for (int row = r0 + grp; row < r1; row += BB / L) {
+            const int64_t a = rp[row], e = rp[row + 1];
+            T s[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) s[r] = T(0);
+            for (int64_t k = a + lane; k < e; k += L) {
+                const int c = col[k];
+                const T v = val[k];
+                T xr[NR];
+                load_xrow<T, NR>(x, c, xr);
+#pragma unroll
+                for (int r = 0; r < NR; r++) s[r] += v * xr[r];
+            }
+#pragma unroll
+            for (int r = 0; r < NR; r++) s[r] = group_sum(s[r], L);
+            if (lane == 0) {
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    T* yp = &y[(int64_t)row * NR + r];
+                    if (beta == T(0)) *yp = s[r];
+                    else if (beta == T(1)) { if (e > a) *yp = *yp + s[r]; }
+                    else *yp = beta * *yp + s[r];
+                }
+            }
+        }
     } else {
+        // one long row: the whole workgroup
         for (int row = r0; row < r1; row++) {
             const int64_t a = rp[row], e = rp[row + 1];
             T s[NR];
@@ -171,45 +309,94 @@ __device__ __forceinline__ void block_spmv(const sable_batch_spmv_item_t& it, in
     }
 }
 
+// The items' block begins are searched in LDS (each workgroup copies them
+// once) when they fit beside the staging buffer within the default 64 KiB
+// of LDS per workgroup (lds_items = nitems); otherwise (lds_items = 0)
+// items[] is searched in global memory.
+constexpr int BATCH_LDS_ITEMS = 4096;
+// Blocks claimed per counter increment: the first BATCH_CH threads resolve
+// one block each (owning item, row range, entry range) in parallel, so the
+// chain of dependent loads behind a block is paid once per claim.
+constexpr int BATCH_CH = 8;
+
 template <typename T, int NR>
 __global__ __launch_bounds__(BB) void k_batch_spmv(const sable_batch_spmv_item_t* __restrict__ items, int nitems,
-                                                   int64_t total, unsigned long long* __restrict__ counter) {
+                                                   int64_t total, unsigned long long* __restrict__ counter,
+                                                   int lds_items) {
     __shared__ __align__(16) T prod[Lds<NR>::ELEMS];
     __shared__ T red[BB / WAVE];
     __shared__ long long next;
-    __shared__ int64_t begins[64];
-    // the first 64 block begins are kept in LDS for the common small batch
-    const int nb = nitems < 64 ? nitems : 64;
-    for (int k = threadIdx.x; k < nb; k += BB) begins[k] = items[k].blk_begin;
+    __shared__ sable_batch_spmv_item_t s_it[BATCH_CH];
+    __shared__ int s_r0[BATCH_CH], s_r1[BATCH_CH], s_nnz[BATCH_CH];
+    __shared__ int64_t s_nz0[BATCH_CH];
+    extern __shared__ __align__(16) int64_t begins[];  // lds_items entries
+    const bool in_lds = lds_items == nitems;
+    if (in_lds)
+        for (int k = threadIdx.x; k < nitems; k += BB) begins[k] = items[k].blk_begin;
+    // the next claim is made while the current one runs
+    long long ahead = 0;
+    if (threadIdx.x == 0) ahead = (long long)atomicAdd(counter, 1ull);
     __syncthreads();
     while (true) {
-        if (threadIdx.x == 0) next = (long long)atomicAdd(counter, 1ull);
+        if (threadIdx.x == 0) next = ahead;
         __syncthreads();
-        const int64_t g = next;
-        __syncthreads();
-        if (g >= total) break;
-        int k;
-        if (nitems <= 64) {
-            k = 0;
+        const int64_t g0 = next * BATCH_CH;
+        if (g0 >= total) break;
+        if (threadIdx.x == 0) ahead = (long long)atomicAdd(counter, 1ull);
+        const int cnt = (int)min((int64_t)BATCH_CH, total - g0);
+        if (threadIdx.x < cnt) {
+            const int64_t g = g0 + threadIdx.x;
+            // items[].blk_begin is ascending: the owning item is the last
+            // one that begins at or before g
             int lo = 0, hi = nitems - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (begins[mid] <= g) lo = mid;
-                else hi = mid - 1;
+            if (in_lds) {
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (begins[mid] <= g) lo = mid;
+                    else hi = mid - 1;
+                }
+            } else {
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (items[mid].blk_begin <= g) lo = mid;
+                    else hi = mid - 1;
+                }
             }
-            k = lo;
-        } else {
-            // items[].blk_begin is ascending: binary search in global memory
-            int lo = 0, hi = nitems - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (items[mid].blk_begin <= g) lo = mid;
-                else hi = mid - 1;
-            }
-            k = lo;
+            const sable_batch_spmv_item_t it = items[lo];
+            const int64_t b = g - it.blk_begin;
+            const int r0 = it.blk[b], r1 = it.blk[b + 1];
+            const int64_t nz0 = it.row_ptr[r0];
+            s_it[threadIdx.x] = it;
+            s_r0[threadIdx.x] = r0;
+            s_r1[threadIdx.x] = r1;
+            s_nz0[threadIdx.x] = nz0;
+            s_nnz[threadIdx.x] = (int)((int64_t)it.row_ptr[r1] - nz0);
         }
-        const sable_batch_spmv_item_t it = items[k];
-        block_spmv<T, NR>(it, g - it.blk_begin, prod, red);
+        __syncthreads();
+        Stage<T, NR> st;
+        bool staged = false;  // st holds block q's loads (uniform over the workgroup)
+        for (int q = 0; q < cnt; q++) {
+            const int r0 = s_r0[q], r1 = s_r1[q], nnzb = s_nnz[q];
+            const int64_t nz0 = s_nz0[q];
+            if (r0 >= r1) continue;
+            if (!one_step<T, NR>(nnzb)) {
+                block_spmv<T, NR>(s_it[q], r0, r1, nz0, nnzb, prod, red);
+                continue;
+            }
+            const sable_batch_spmv_item_t it = s_it[q];
+            if (!staged) stage_load<T, NR>(st, it, r0, r1, nz0, nnzb);
+            stage_prod<T, NR>(st, it, nnzb, prod);
+            const int a0 = st.a, e0 = st.e;
+            staged = false;
+            if (q + 1 < cnt && s_r0[q + 1] < s_r1[q + 1] && one_step<T, NR>(s_nnz[q + 1])) {
+                stage_load<T, NR>(st, s_it[q + 1], s_r0[q + 1], s_r1[q + 1], s_nz0[q + 1], s_nnz[q + 1]);
+                staged = true;
+            }
+            __syncthreads();
+            stage_reduce<T, NR>(it, r0, r1, nz0, a0, e0, prod);
+            __syncthreads();  // prod is rewritten by the next block
+        }
+        __syncthreads();  // the claim's descriptors are rewritten by the next claim
     }
 }
 
@@ -506,9 +693,16 @@ int batch_spmv_launch(hipStream_t st, int num_cu, sable_dtype_t dtype, int nrhs,
     if (total_blocks <= 0 || nitems <= 0) return SABLE_OK;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, counter);
     int64_t grid = wgs > 0 ? wgs : (int64_t)num_cu * 4;
-    if (grid > total_blocks) grid = total_blocks;
-#define BS_LAUNCH(T, NR) \
-    hipLaunchKernelGGL((k_batch_spmv<T, NR>), dim3((unsigned)grid), dim3(BB), 0, st, items, nitems, total_blocks, counter)
+    const int64_t claims = (total_blocks + BATCH_CH - 1) / BATCH_CH;
+    if (grid > claims) grid = claims;
+#define BS_LAUNCH(T, NR)                                                                                          \
+    do {                                                                                                           \
+        const size_t fixed = sizeof(T) * (Lds<NR>::ELEMS + BB / WAVE) + 1024;                                     \
+        const size_t want = sizeof(int64_t) * (size_t)nitems;                                                     \
+        const bool fits = nitems <= BATCH_LDS_ITEMS && fixed + want <= 65536;                                     \
+        hipLaunchKernelGGL((k_batch_spmv<T, NR>), dim3((unsigned)grid), dim3(BB), fits ? want : 0, st, items,     \
+                           nitems, total_blocks, counter, fits ? nitems : 0);                                      \
+    } while (0)
 #define BS_SWITCH(T)                        \
     switch (nrhs) {                         \
         case 1: BS_LAUNCH(T, 1); break;     \

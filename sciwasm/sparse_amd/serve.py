@@ -34,14 +34,17 @@ def _upload_items(items, ctx):
     return DeviceArray.from_host(np.frombuffer(raw, np.uint8), ctx=ctx)
 
 
-def batch_spmv(mats, xs, ys, nrhs=1, beta=0.0, workgroups=0):
+def batch_spmv(mats, xs, ys, nrhs=1, beta=0.0, workgroups=0, plans=None):
     """ys[k] = mats[k].x[k] (+ beta*ys[k]) for every k in one launch.  x and y
-    are DeviceArrays of N*nrhs elements, row-major [N][nrhs]."""
+    are DeviceArrays of N*nrhs elements, row-major [N][nrhs].  plans[k]
+    (CsrPlan) gives matrix k's row blocks; by default each matrix's own
+    plan is used."""
     ctx = mats[0].row.ctx
     items, total = [], 0
-    for A, x, y in zip(mats, xs, ys):
+    for k, (A, x, y) in enumerate(zip(mats, xs, ys)):
         blk, nb = c_vp(), c_i64()
-        call("sable_csr_plan_blocks", A.plan().handle, ctypes.byref(blk), ctypes.byref(nb))
+        pl = plans[k] if plans is not None else A.plan()
+        call("sable_csr_plan_blocks", pl.handle, ctypes.byref(blk), ctypes.byref(nb))
         items.append(BatchSpmvItem(A.row.ptr, A.col.ptr, A.val.ptr, blk.value, x.ptr, y.ptr, total, float(beta)))
         total += nb.value
     d = _upload_items(items, ctx)

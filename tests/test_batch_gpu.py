@@ -68,6 +68,32 @@ def test_batch_spmv_multi_rhs(ctx, mats, nrhs, dtype):
             assert np.all(d <= bound(rp, c, v.astype(dtype), X[k][:, r], dtype)), (k, r)
 
 
+@pytest.mark.parametrize("nrhs", [1, 8])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batch_spmv_blocks_beyond_lds(ctx, mats, nrhs, dtype):
+    """Plans with blocks of up to 4096 weight: blocks whose products do not
+    fit the LDS staging buffer take the wave-per-row path, one-row blocks
+    the whole-workgroup path; results within the CSR bound."""
+    rng = np.random.default_rng(3)
+    As, plans, xs, ys, X = [], [], [], [], []
+    sub = mats[:10]
+    for n, rp, c, v in sub:
+        A = sp.csr_from_host(n, rp, c, v.astype(dtype))
+        As.append(A)
+        plans.append(sp.CsrPlan(A, {"block_weight": 4096, "row_weight": 1}))
+        x = rng.uniform(-1, 1, (n, nrhs)).astype(dtype)
+        X.append(x)
+        xs.append(DeviceArray.from_host(x.ravel()))
+        ys.append(DeviceArray.zeros(n * nrhs, dtype))
+    sp.batch_spmv(As, xs, ys, nrhs=nrhs, plans=plans)
+    for k, (n, rp, c, v) in enumerate(sub):
+        Y = ys[k].numpy().reshape(n, nrhs)
+        for r in range(nrhs):
+            yref = oracle.spmv_csr(n, rp, c, v.astype(dtype), X[k][:, r].copy(), np.zeros(n, dtype))
+            d = np.abs(Y[:, r].astype(np.float64) - yref)
+            assert np.all(d <= bound(rp, c, v.astype(dtype), X[k][:, r], dtype)), (k, r)
+
+
 def lower_set(mats, dtype=np.float64):
     out = []
     for n, rp, c, v in mats:
