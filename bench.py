@@ -512,6 +512,7 @@ def sec_spts(args, sp, ctx, T):
         res = {"N": N, "nnz": nnz, "nlevels": an.nlevels, "lanes": an.lanes, "analysis_s": r4(t_an)}
         for meth, run in (("level_graph", lambda: an.solve_level(bv, xv)),
                           ("syncfree", lambda: an.solve_syncfree(bv, xv)),
+                          ("combined", lambda: an.solve_combined(bv, xv)),
                           ("fp32_refine2", lambda: an.solve_refined(bv, xv, iters=2))):
             ms = T(run, 10, 2)
             an.status()

@@ -491,6 +491,15 @@ SABLE_API int sable_spts_solve_level(sable_ctx_t* ctx, sable_spts_t* a, const vo
  * spts_csr_sync_free(_wrapper) (par/src/spts_opt_32.wat:8-260). */
 SABLE_API int sable_spts_solve_syncfree(sable_ctx_t* ctx, sable_spts_t* a, const void* b, void* x);
 
+/* Level + sync-free combined solve, b and x in natural order: the
+ * sync-free kernel over the analysis's level-ordered matrix, with the
+ * level solve's arithmetic, so that the result equals sable_spts_solve_level
+ * bit for bit at every lane count.  Replaces the combined solve
+ * spts_csr_opt_level_sync_free_wrapper (par/src/sparse_opt_32.wat:12846-13353)
+ * and spts_csr_level_sync_free_wrapper, whose arithmetic is spts_csr's
+ * (:11679-11757). */
+SABLE_API int sable_spts_solve_combined(sable_ctx_t* ctx, sable_spts_t* a, const void* b, void* x);
+
 /* Synchronises; SABLE_ETIMEOUT if a sync-free solve of this analysis hit
  * its spin limit since the last call. */
 SABLE_API int sable_spts_status(sable_ctx_t* ctx, sable_spts_t* a);

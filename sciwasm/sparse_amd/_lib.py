@@ -143,6 +143,7 @@ SIGNATURES = {
     "sable_spts_get_schedule": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "sable_spts_solve_level": [c_vp, c_vp, c_vp, c_vp],
     "sable_spts_solve_syncfree": [c_vp, c_vp, c_vp, c_vp],
+    "sable_spts_solve_combined": [c_vp, c_vp, c_vp, c_vp],
     "sable_spts_status": [c_vp, c_vp],
     "sable_spts_solve_refined": [c_vp, c_vp, c_vp, c_vp, c_int, c_int],
     "sable_spts_csc_analyze": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_int, c_vpp],

@@ -188,7 +188,11 @@ __global__ void k_sentinel(int32_t N, T* __restrict__ x, unsigned* __restrict__ 
     if (blockIdx.x == 0 && threadIdx.x == 0) *ticket = 0u;
 }
 
-template <typename T, int G>
+// LVL: the level solve's arithmetic (t = 0, t += val * x over the
+// off-diagonals, x = (b - t) / diag) also for G = 1, where the plain
+// sync-free solve follows the reference's t = b, t -= val * x.  With LVL
+// every lane count gives the level solve's result bit for bit.
+template <typename T, int G, bool LVL = false>
 __global__ __launch_bounds__(TB) void k_syncfree(int32_t N, const int32_t* __restrict__ rp,
                                                  const int32_t* __restrict__ col, const T* __restrict__ val,
                                                  const T* __restrict__ b, T* x, unsigned* __restrict__ ticket,
@@ -212,7 +216,7 @@ __global__ __launch_bounds__(TB) void k_syncfree(int32_t N, const int32_t* __res
             j = rp[i] + l;
             e = rp[i + 1] - 1;
             bi = b[i];
-            if (G == 1) s = bi;
+            if (G == 1 && !LVL) s = bi;
         }
         long long spins = 0;
         while (__ballot(!published) != 0ull) {
@@ -221,7 +225,7 @@ __global__ __launch_bounds__(TB) void k_syncfree(int32_t N, const int32_t* __res
                 while (j < e) {
                     const T v = load_agent(&x[col[j]]);
                     if (Sent<T>::is(v)) break;
-                    if (G == 1)
+                    if (G == 1 && !LVL)
                         s = sub_rn(s, mul_rn(val[j], v));
                     else
                         s = add_rn(s, mul_rn(val[j], v));
@@ -236,7 +240,7 @@ __global__ __launch_bounds__(TB) void k_syncfree(int32_t N, const int32_t* __res
             if (G > 1) tot = group_sum(s, G);
             if (!published && gdone) {
                 if (l == 0) {
-                    const T y = (G == 1) ? div_rn(tot, val[e]) : div_rn(sub_rn(bi, tot), val[e]);
+                    const T y = (G == 1 && !LVL) ? div_rn(tot, val[e]) : div_rn(sub_rn(bi, tot), val[e]);
                     store_agent(&x[i], y);
                 }
                 published = true;
@@ -329,20 +333,35 @@ static int levels_on_stream(hipStream_t st, int G, const std::vector<int32_t>& l
 
 template <typename T>
 static void launch_syncfree(hipStream_t st, int G, int num_cu, int32_t N, const int32_t* rp, const int32_t* col,
-                            const T* val, const T* b, T* x, unsigned* ticket, int* err, long long spin) {
+                            const T* val, const T* b, T* x, unsigned* ticket, int* err, long long spin,
+                            bool lvl = false) {
     hipLaunchKernelGGL(k_sentinel<T>, dim3(gsz(N)), dim3(TB), 0, st, N, x, ticket);
     const int64_t waves_needed = ((int64_t)N * G + WAVE - 1) / WAVE;
     int64_t blocks = (waves_needed + (TB / WAVE) - 1) / (TB / WAVE);
     const int64_t cap = (int64_t)num_cu * 8;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
-    switch (G) {
-        case 1: hipLaunchKernelGGL((k_syncfree<T, 1>), dim3((unsigned)blocks), dim3(TB), 0, st, N, rp, col, val, b, x, ticket, err, spin); break;
-        case 4: hipLaunchKernelGGL((k_syncfree<T, 4>), dim3((unsigned)blocks), dim3(TB), 0, st, N, rp, col, val, b, x, ticket, err, spin); break;
-        case 8: hipLaunchKernelGGL((k_syncfree<T, 8>), dim3((unsigned)blocks), dim3(TB), 0, st, N, rp, col, val, b, x, ticket, err, spin); break;
-        case 16: hipLaunchKernelGGL((k_syncfree<T, 16>), dim3((unsigned)blocks), dim3(TB), 0, st, N, rp, col, val, b, x, ticket, err, spin); break;
-        default: hipLaunchKernelGGL((k_syncfree<T, 64>), dim3((unsigned)blocks), dim3(TB), 0, st, N, rp, col, val, b, x, ticket, err, spin); break;
+#define SF_LAUNCH(GG, L)                                                                                    \
+    hipLaunchKernelGGL((k_syncfree<T, GG, L>), dim3((unsigned)blocks), dim3(TB), 0, st, N, rp, col, val, b, x, \
+                       ticket, err, spin)
+    if (lvl) {
+        switch (G) {
+            case 1: SF_LAUNCH(1, true); break;
+            case 4: SF_LAUNCH(4, true); break;
+            case 8: SF_LAUNCH(8, true); break;
+            case 16: SF_LAUNCH(16, true); break;
+            default: SF_LAUNCH(64, true); break;
+        }
+    } else {
+        switch (G) {
+            case 1: SF_LAUNCH(1, false); break;
+            case 4: SF_LAUNCH(4, false); break;
+            case 8: SF_LAUNCH(8, false); break;
+            case 16: SF_LAUNCH(16, false); break;
+            default: SF_LAUNCH(64, false); break;
+        }
     }
+#undef SF_LAUNCH
 }
 
 int syncfree_launch(hipStream_t st, int num_cu, sable_dtype_t dt, int lanes, int32_t N, const int32_t* rp,
@@ -618,6 +637,34 @@ SABLE_API int sable_spts_solve_syncfree(sable_ctx_t* ctx, sable_spts_t* a, const
     else
         launch_syncfree<float>(ctx->stream, a->lanes, ctx->num_cu, a->N, a->rp, a->col, (const float*)a->val,
                                (const float*)b, (float*)x, a->d_ticket, a->d_err, DEFAULT_SPIN);
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+// The reference's level + sync-free combination: the sync-free kernel over
+// the analysis's level-ordered matrix (waves take rows in level order, so
+// a wave's rows mostly share a level and rarely wait on each other), with
+// the level solve's arithmetic.
+SABLE_API int sable_spts_solve_combined(sable_ctx_t* ctx, sable_spts_t* a, const void* b, void* x) {
+    SABLE_CHECK(ctx && a && b && x, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(ctx == a->ctx, SABLE_EINVAL, "analysis belongs to another context");
+    hipStream_t st = ctx->stream;
+    const int32_t N = a->N;
+    if (a->dtype == SABLE_F64) {
+        hipLaunchKernelGGL(k_gather_perm<double>, dim3(gsz(N)), dim3(TB), 0, st, N, a->d_perm, (const double*)b,
+                           (double*)a->d_bl);
+        launch_syncfree<double>(st, a->lanes, ctx->num_cu, N, a->d_rp, a->d_col, (const double*)a->d_val,
+                                (const double*)a->d_bl, (double*)a->d_yl, a->d_ticket, a->d_err, DEFAULT_SPIN, true);
+        hipLaunchKernelGGL(k_scatter_perm<double>, dim3(gsz(N)), dim3(TB), 0, st, N, a->d_perm,
+                           (const double*)a->d_yl, (double*)x);
+    } else {
+        hipLaunchKernelGGL(k_gather_perm<float>, dim3(gsz(N)), dim3(TB), 0, st, N, a->d_perm, (const float*)b,
+                           (float*)a->d_bl);
+        launch_syncfree<float>(st, a->lanes, ctx->num_cu, N, a->d_rp, a->d_col, (const float*)a->d_val,
+                               (const float*)a->d_bl, (float*)a->d_yl, a->d_ticket, a->d_err, DEFAULT_SPIN, true);
+        hipLaunchKernelGGL(k_scatter_perm<float>, dim3(gsz(N)), dim3(TB), 0, st, N, a->d_perm, (const float*)a->d_yl,
+                           (float*)x);
+    }
     SABLE_LAUNCH_CHECK();
     return SABLE_OK;
 }

@@ -62,6 +62,11 @@ class SptsAnalysis:
     def solve_syncfree(self, b, x):
         call("sable_spts_solve_syncfree", self.ctx.handle, self.handle, b.vec.ptr, x.vec.ptr)
 
+    def solve_combined(self, b, x):
+        """Level + sync-free combined: the sync-free kernel over the
+        level-ordered matrix; equal to solve_level bit for bit."""
+        call("sable_spts_solve_combined", self.ctx.handle, self.handle, b.vec.ptr, x.vec.ptr)
+
     def solve_refined(self, b, x, iters=2, inner="syncfree"):
         call("sable_spts_solve_refined", self.ctx.handle, self.handle, b.vec.ptr, x.vec.ptr, int(iters),
              1 if inner == "syncfree" else 0)

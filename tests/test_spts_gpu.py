@@ -85,6 +85,30 @@ def test_multilane_residual(ctx, lanes, name):
         assert r <= 1e-12, (method, r)
 
 
+@pytest.mark.parametrize("lanes", [1, 4, 16, 64])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("name", list(MATS))
+def test_combined_equals_level(ctx, lanes, dtype, name):
+    """The combined solve (sync-free over the level-ordered matrix) uses the
+    level solve's arithmetic: bit-identical to it at every lane count, and
+    at lanes = 1 to the reference-order level solve of the oracle."""
+    rp, c, v = MATS[name]()
+    v = v.astype(dtype)
+    N = len(rp) - 1
+    b = np.random.default_rng(9).uniform(-1, 1, N).astype(dtype)
+    an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v), lanes=lanes)
+    bv = sp.vec_from_host(b)
+    xl = sp.create_vec(N, 0.0, dtype)
+    an.solve_level(bv, xl)
+    for _ in range(2):
+        xc = sp.create_vec(N, 0.0, dtype)
+        an.solve_combined(bv, xc)
+        an.status()
+        assert np.array_equal(xc.array(), xl.array())
+    if lanes == 1:
+        assert np.array_equal(xc.array(), host_level_solution(N, rp, c, v, b))
+
+
 def test_graph_replays_match_eager(ctx):
     rp, c, v = gen.random_lower(30000, 16, seed=5)
     N = len(rp) - 1
