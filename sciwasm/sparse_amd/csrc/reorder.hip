@@ -223,8 +223,12 @@ __global__ __launch_bounds__(RB) void k_reorder_nn(int32_t N, const int32_t* __r
                 cj[i] = (double)al / (double)(nlines[index] + nlines[c] - al);
             }
             __syncthreads();
-            if (tid == 0) {
-                vis[index - start] = 1;
+            // The candidates are folded into the list in row order by the
+            // first wave, every lane holding the same list.  A candidate is
+            // a no-op exactly when the list holds w valid (>= 0) entries,
+            // all >= its score (nn_insert then finds no position); a ballot
+            // skips those, and the others are inserted one by one, in order.
+            if (tid < WAVE) {
                 float lj[NN_W];
                 int32_t ln[NN_W];
 #pragma unroll
@@ -232,35 +236,71 @@ __global__ __launch_bounds__(RB) void k_reorder_nn(int32_t N, const int32_t* __r
                     lj[k] = -1.f;
                     ln[k] = N;
                 }
-                for (int i = 0; i < n; ++i)
-                    if (vis[i] == 0) nn_insert(lj, ln, start + i, cj[i], w);
+                auto changes = [&](bool cand, double m) {
+                    bool full = true;
+                    float thr = 0.f;
 #pragma unroll
-                for (int k = 0; k < NN_W; ++k) {
-                    jac[slot][k] = lj[k];
-                    nid[slot][k] = ln[k];
+                    for (int k = 0; k < NN_W; ++k)
+                        if (k < w) {
+                            full = full && lj[k] >= 0.f;
+                            thr = (k == 0 || lj[k] < thr) ? lj[k] : thr;
+                        }
+                    return cand && !(full && (double)thr >= m);
+                };
+                for (int base = 0; base < n; base += WAVE) {
+                    const int i = base + tid;
+                    const bool cand = i < n && vis[i] == 0;
+                    const double m = cand ? cj[i] : 0.0;
+                    uint64_t mask = __ballot(changes(cand, m));
+                    while (mask != 0ull) {
+                        const int j = __ffsll((unsigned long long)mask) - 1;
+                        const double mj = __shfl(m, j, WAVE);
+                        nn_insert(lj, ln, start + base + j, mj, w);
+                        mask &= mask - 1ull;
+                        mask &= __ballot(changes(cand, m));
+                    }
                 }
-                // next row: each slot's best unvisited candidate; a later
-                // slot wins a tie
-                int32_t next = N;
-                double mx = 0;
-                for (int s2 = 0; s2 < nnb; ++s2) {
-                    int32_t nx = N;
-                    double m = 0;
+                if (tid == 0) {
+#pragma unroll
+                    for (int k = 0; k < NN_W; ++k) {
+                        jac[slot][k] = lj[k];
+                        nid[slot][k] = ln[k];
+                    }
+                    vis[index - start] = 1;
+                }
+                // next row: each slot's first unvisited candidate, then the
+                // slot with the largest score, a later slot winning a tie
+                // (lane s2 < nnb <= NN_W takes slot s2; LDS accesses of one
+                // wave complete in order, so lane 0's writes are seen)
+                int32_t nx = N;
+                double m = 0;
+                if (tid < nnb) {
                     for (int k = 0; k < w; ++k) {
-                        const float v = jac[s2][k];
-                        const int32_t q = nid[s2][k];
+                        const float v = jac[tid][k];
+                        const int32_t q = nid[tid][k];
                         if (v >= 0.f && q < N && vis[q - start] == 0) {
                             nx = q;
                             m = v;
                             break;
                         }
                     }
-                    if (mx <= m && nx != N) {
-                        mx = m;
-                        next = nx;
+                }
+                bool ok = nx != N;
+                int s2 = tid;
+#pragma unroll
+                for (int off = NN_W / 2; off > 0; off >>= 1) {
+                    const double m2 = __shfl_xor(m, off, WAVE);
+                    const int32_t nx2 = __shfl_xor(nx, off, WAVE);
+                    const int s22 = __shfl_xor(s2, off, WAVE);
+                    const bool ok2 = __shfl_xor((int)ok, off, WAVE) != 0;
+                    if (ok2 && (!ok || m2 > m || (m2 == m && s22 > s2))) {
+                        m = m2;
+                        nx = nx2;
+                        s2 = s22;
+                        ok = true;
                     }
                 }
-                s_next = next;
+                if (tid == 0) s_next = ok ? nx : N;
             }
             __syncthreads();
             index = s_next;
