@@ -19,7 +19,8 @@ Also measured and reported under "extra" (each section bounded to seconds):
           DIA/ELL/CSR SpMV GB/s
   zipf16  the same product on the sparser Zipf(s = 1.73) matrix of the same
           N (mean about 16, about 67M nonzeros), two tile shapes
-  convert coo_csr of the headline matrix (shuffled COO, 206M entries)
+  convert coo_csr of the headline matrix (shuffled COO, 206M entries); COO
+          SpMV of the same matrix, shuffled (atomics) and row-sorted
   ingest  MatrixMarket text parsed on the device and expanded to COO
   ewise   element-wise ops and eliminate_zeros on the headline values
   reorder sort_rows_by_nnz of the headline matrix, reorder_NN of a
@@ -336,14 +337,43 @@ def sec_convert(args, sp, ctx, T, host):
     row = gen.csr_to_coo_rows(rp)
     p = np.random.default_rng(4).permutation(nnz)
     C = sp.coo_from_host(N, row[p], col[p], val[p], ctx)
-    del row, p
+    del p
     t_cc, B = T.once(lambda: sp.coo_csr(C))
     ok = bool(np.array_equal(B.row.numpy(), rp))
-    del B, C
+    del B
     A = sp.csr_from_host(N, rp, col, val, ctx)
     E = sp.csr_ell(A)
     out = {"coo_csr_ms": r4(t_cc), "coo_csr_entries_per_s": r4(nnz / (t_cc * 1e-3)), "row_ptr_ok": ok,
            "csr_ell": "refused by the reference guard" if E is None else "accepted"}
+    # COO SpMV of the same matrix (16 B per entry + x + y): the shuffled COO
+    # reduces with device atomics, the row-sorted COO with the segmented
+    # reduction of its plan; both checked on the first N/8 rows against
+    # the CSR bound
+    R = N // 8
+    pr = val[:rp[R]] * x[col[:rp[R]]]
+    lens = np.diff(rp[:R + 1])
+    yref = np.add.reduceat(pr, rp[:R])
+    sab = np.add.reduceat(np.abs(pr), rp[:R])
+    yref[lens == 0] = 0
+    sab[lens == 0] = 0
+    u = np.finfo(np.float64).eps / 2
+    bnd = 2 * (lens + 1) * u / (1 - (lens + 1) * u) * sab + 1e-300
+    xv, yv = sp.vec_from_host(x, ctx=ctx), sp.create_vec(N, 0.0, np.float64, ctx=ctx)
+    b_coo = 16 * nnz + 16 * N
+    Cs = sp.coo_from_host(N, row, col, val, ctx)
+    t1 = time.time()
+    tl = sp.CsrTiles.from_coo(Cs, 16384, 8192)
+    tiles_build_s = time.time() - t1
+    for name, pl in (("coo_shuffled_spmv", sp.CooPlan(C)), ("coo_sorted_spmv", sp.CooPlan(Cs)),
+                     ("coo_sorted_tiles_spmv", tl)):
+        ms = T(lambda: pl.run(xv.vec, yv.vec, beta=0.0), 10, 2)
+        y = yv.array()[:R]
+        out[name] = {"ms": r4(ms), "gbs": r4(b_coo / (ms * 1e-3) / 1e9),
+                     "pct_hbm": r4(100 * b_coo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS), "plan": pl.info(),
+                     "within_csr_bound_rows": R, "within_csr_bound": bool(np.all(np.abs(y - yref) <= bnd))}
+        del pl
+    out["coo_sorted_tiles_spmv"]["plan_build_s"] = r4(tiles_build_s)
+    del C, Cs, tl, row
     log("convert:", out)
     return out
 

@@ -182,6 +182,11 @@ SABLE_API int sable_coo_plan_create(sable_ctx_t* ctx, int32_t N, int64_t nnz, co
                                     sable_coo_plan_t** out);
 SABLE_API void sable_coo_plan_destroy(sable_coo_plan_t* plan);
 SABLE_API int sable_coo_plan_info(const sable_coo_plan_t* plan, int* sorted, int64_t* nblocks);
+/* Device row heads (N + 1 entries, owned by the plan) of a row-sorted COO:
+ * with them the COO's col and val arrays form a CSR, e.g. for a tile plan.
+ * SABLE_EINVAL when the rows are not sorted.  No reference counterpart
+ * (the reference's COO product has no plan). */
+SABLE_API int sable_coo_plan_row_ptr(const sable_coo_plan_t* plan, const int32_t** d_row_ptr);
 SABLE_API int sable_spmv_coo_plan(sable_ctx_t* ctx, const sable_coo_plan_t* plan, sable_dtype_t dtype,
                                   const int32_t* row, const int32_t* col, const void* val, const void* x, void* y,
                                   double beta, int inner_max);

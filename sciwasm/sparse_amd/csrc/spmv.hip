@@ -888,6 +888,13 @@ SABLE_API int sable_coo_plan_info(const sable_coo_plan_t* p, int* sorted, int64_
     return SABLE_OK;
 }
 
+SABLE_API int sable_coo_plan_row_ptr(const sable_coo_plan_t* p, const int32_t** d_row_ptr) {
+    SABLE_CHECK(p && d_row_ptr, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(p->sorted && p->rp, SABLE_EINVAL, "the COO rows are not sorted: no row heads");
+    *d_row_ptr = p->rp;
+    return SABLE_OK;
+}
+
 SABLE_API int sable_spmv_coo_plan(sable_ctx_t* ctx, const sable_coo_plan_t* p, sable_dtype_t dtype,
                                   const int32_t* row, const int32_t* col, const void* val, const void* x, void* y,
                                   double beta, int inner_max) {

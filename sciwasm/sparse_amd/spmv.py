@@ -6,6 +6,8 @@ for every format; ``spmv(A, x, y, beta=0)`` is the plain product.
 import ctypes
 import sys
 
+import numpy as np
+
 from . import _lib
 from ._lib import call, c_vp, c_i64, c_i32
 from .core import (sparse_COO_t, sparse_CSR_t, sparse_DIA_t, sparse_ELL_t, dtype_code)
@@ -65,6 +67,13 @@ class CsrTiles:
     def run(self, x, y, beta=1.0, inner_max=1):
         call("sable_spmv_csr_tiles", self.ctx.handle, self.handle, x.ptr, y.ptr, float(beta), int(inner_max))
 
+    @classmethod
+    def from_coo(cls, A, tile_rows=0, panel_cols=0):
+        """A tile plan of a row-sorted COO: its row heads (from a CooPlan)
+        with its col and val arrays form the CSR the plan copies."""
+        rp = CooPlan(A).row_ptr()
+        return cls(sparse_CSR_t(rp, A.col, A.val, A.N, A.nnz), tile_rows, panel_cols)
+
     def __del__(self):
         try:
             if self.handle:
@@ -90,6 +99,15 @@ class CooPlan:
         srt, nb = ctypes.c_int(), c_i64()
         call("sable_coo_plan_info", self.handle, ctypes.byref(srt), ctypes.byref(nb))
         return {"sorted": bool(srt.value), "nblocks": nb.value}
+
+    def row_ptr(self):
+        """A copy of the row heads of a row-sorted COO (raises otherwise)."""
+        from .core import DeviceArray
+        p = c_vp()
+        call("sable_coo_plan_row_ptr", self.handle, ctypes.byref(p))
+        d = DeviceArray(self.A.N + 1, np.int32, self.ctx)
+        call("sable_memcpy_d2d", self.ctx.handle, d.ptr, p.value, d.nbytes)
+        return d
 
     def run(self, x, y, beta=1.0, inner_max=1):
         A = self.A

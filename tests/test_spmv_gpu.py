@@ -251,6 +251,31 @@ def test_csr_tiles_parity(ctx, tiles_mode, tile_rows, panel_cols, dtype, name, m
     check_close(yv.array(), y2, bound(rp, col, val, x, y0, dtype) + bound(rp, col, val, x, y1, dtype))
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("name,make", CASES)
+def test_csr_tiles_from_sorted_coo(ctx, dtype, name, make):
+    """A tile plan built from a row-sorted COO (its plan's row heads with
+    its col and val) gives the CSR product within the CSR bound; an
+    unsorted COO has no row heads."""
+    rp, col, val = make(dtype)
+    N = len(rp) - 1
+    row = np.repeat(np.arange(N), np.diff(rp)).astype(np.int32)
+    rng = np.random.default_rng(53)
+    x = rng.uniform(-1, 1, N).astype(dtype)
+    C = sp.coo_from_host(N, row, col, val)
+    pl = sp.CsrTiles.from_coo(C, 256, 1024)
+    yv = sp.create_vec(N, 0.0, dtype)
+    pl.run(sp.vec_from_host(x).vec, yv.vec, beta=0.0)
+    z = np.zeros(N, dtype)
+    check_close(yv.array(), oracle.spmv_csr(N, rp, col, val, x, z), bound(rp, col, val, x, z, dtype))
+    if len(row) > 1 and row[0] != row[-1]:
+        p = np.arange(len(row))[::-1]
+        U = sp.coo_from_host(N, row[p], col[p], val[p])
+        with pytest.raises(SableError) as e:
+            sp.CsrTiles.from_coo(U)
+        assert e.value.name == "SABLE_EINVAL"
+
+
 def test_csr_tiles_rejects_bad_shapes(ctx):
     rp, col, val = gen.laplacian_2d(16)
     A = sp.csr_from_host(256, rp, col, val)
