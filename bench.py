@@ -364,8 +364,11 @@ def sec_convert(args, sp, ctx, T, host):
     t1 = time.time()
     tl = sp.CsrTiles.from_coo(Cs, 16384, 8192)
     tiles_build_s = time.time() - t1
+    t1 = time.time()
+    tu = sp.CsrTiles.from_coo(C, 16384, 8192)
+    tiles_build_u = time.time() - t1
     for name, pl in (("coo_shuffled_spmv", sp.CooPlan(C)), ("coo_sorted_spmv", sp.CooPlan(Cs)),
-                     ("coo_sorted_tiles_spmv", tl)):
+                     ("coo_sorted_tiles_spmv", tl), ("coo_shuffled_tiles_spmv", tu)):
         ms = T(lambda: pl.run(xv.vec, yv.vec, beta=0.0), 10, 2)
         y = yv.array()[:R]
         out[name] = {"ms": r4(ms), "gbs": r4(b_coo / (ms * 1e-3) / 1e9),
@@ -373,7 +376,8 @@ def sec_convert(args, sp, ctx, T, host):
                      "within_csr_bound_rows": R, "within_csr_bound": bool(np.all(np.abs(y - yref) <= bnd))}
         del pl
     out["coo_sorted_tiles_spmv"]["plan_build_s"] = r4(tiles_build_s)
-    del C, Cs, tl, row
+    out["coo_shuffled_tiles_spmv"]["plan_build_s"] = r4(tiles_build_u)
+    del C, Cs, tl, tu, row
     log("convert:", out)
     return out
 

@@ -69,10 +69,14 @@ class CsrTiles:
 
     @classmethod
     def from_coo(cls, A, tile_rows=0, panel_cols=0):
-        """A tile plan of a row-sorted COO: its row heads (from a CooPlan)
-        with its col and val arrays form the CSR the plan copies."""
-        rp = CooPlan(A).row_ptr()
-        return cls(sparse_CSR_t(rp, A.col, A.val, A.N, A.nnz), tile_rows, panel_cols)
+        """A tile plan of a COO.  Row-sorted: its row heads (from a CooPlan)
+        with its col and val arrays form the CSR the plan copies; otherwise
+        the COO is first sorted on the device (coo_csr)."""
+        cp = CooPlan(A)
+        if cp.info()["sorted"]:
+            return cls(sparse_CSR_t(cp.row_ptr(), A.col, A.val, A.N, A.nnz), tile_rows, panel_cols)
+        from .convert import coo_csr
+        return cls(coo_csr(A), tile_rows, panel_cols)
 
     def __del__(self):
         try:

@@ -255,8 +255,9 @@ def test_csr_tiles_parity(ctx, tiles_mode, tile_rows, panel_cols, dtype, name, m
 @pytest.mark.parametrize("name,make", CASES)
 def test_csr_tiles_from_sorted_coo(ctx, dtype, name, make):
     """A tile plan built from a row-sorted COO (its plan's row heads with
-    its col and val) gives the CSR product within the CSR bound; an
-    unsorted COO has no row heads."""
+    its col and val), or from the same entries shuffled (sorted first),
+    gives the CSR product within the CSR bound; an unsorted COO's plan has
+    no row heads."""
     rp, col, val = make(dtype)
     N = len(rp) - 1
     row = np.repeat(np.arange(N), np.diff(rp)).astype(np.int32)
@@ -269,11 +270,15 @@ def test_csr_tiles_from_sorted_coo(ctx, dtype, name, make):
     z = np.zeros(N, dtype)
     check_close(yv.array(), oracle.spmv_csr(N, rp, col, val, x, z), bound(rp, col, val, x, z, dtype))
     if len(row) > 1 and row[0] != row[-1]:
-        p = np.arange(len(row))[::-1]
+        p = np.random.default_rng(54).permutation(len(row))
         U = sp.coo_from_host(N, row[p], col[p], val[p])
         with pytest.raises(SableError) as e:
-            sp.CsrTiles.from_coo(U)
+            sp.CooPlan(U).row_ptr()
         assert e.value.name == "SABLE_EINVAL"
+        pu = sp.CsrTiles.from_coo(U, 256, 1024)
+        yu = sp.create_vec(N, 0.0, dtype)
+        pu.run(sp.vec_from_host(x).vec, yu.vec, beta=0.0)
+        check_close(yu.array(), oracle.spmv_csr(N, rp, col, val, x, z), bound(rp, col, val, x, z, dtype))
 
 
 def test_csr_tiles_rejects_bad_shapes(ctx):
