@@ -3,9 +3,11 @@
 // SpMV: every matrix brings its own row-block schedule (sable_csr_plan_t).
 // The batch's blocks form one global work list (item k owns global blocks
 // [blk_begin_k, blk_begin_k + nblocks_k)); a persistent grid of workgroups
-// pops block indices from a device counter, finds the owning item by binary
-// search and runs the same block kernel as single-matrix CSR SpMV.  Items
-// are ordered by the caller (the server puts blocks of long rows first).
+// claims them BATCH_CH at a time from a device counter, finds each block's
+// item by binary search and runs the block with the single-matrix CSR
+// block arithmetic (products staged in LDS, each row summed in order).
+// Items are ordered by the caller (the server puts blocks of long rows
+// first).
 // Up to 8 right-hand sides share each matrix read: x and y are row-major
 // [N][nrhs], so one gather of x[col] fetches all right-hand sides.
 //
