@@ -34,30 +34,33 @@ constexpr int RB = 256;
 
 unsigned gs(int64_t n) { return grid_for(n, RB); }
 
-// nnz of every row; keys = nnz (for the stable sort), payload = row.
-__global__ void k_row_lengths(int32_t N, const int32_t* __restrict__ rp, int32_t* __restrict__ len,
-                              uint64_t* __restrict__ keys, uint32_t* __restrict__ rows, int* __restrict__ bins) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
-    if (i < N) {
+// nnz of every row; keys = nnz (for the stable sort), payload = row.  The
+// counts of rows of 0-3 entries are summed per workgroup (grid-stride, then
+// waves, then LDS) and added with one atomic per counter and workgroup.
+__global__ __launch_bounds__(RB) void k_row_lengths(int32_t N, const int32_t* __restrict__ rp,
+                                                    int32_t* __restrict__ len, uint64_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ rows, int* __restrict__ bins) {
+    __shared__ int part[RB / WAVE][4];
+    int b[4] = {0, 0, 0, 0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t l = rp[i + 1] - rp[i];
         len[i] = l;
         keys[i] = (uint64_t)(uint32_t)l;
         rows[i] = (uint32_t)i;
-        b0 = l == 0;
-        b1 = l == 1;
-        b2 = l == 2;
-        b3 = l == 3;
+#pragma unroll
+        for (int q = 0; q < 4; q++) b[q] += l == q;
     }
-    b0 = wave_sum(b0);
-    b1 = wave_sum(b1);
-    b2 = wave_sum(b2);
-    b3 = wave_sum(b3);
-    if ((threadIdx.x & (WAVE - 1)) == 0) {
-        if (b0) atomicAdd(&bins[0], b0);
-        if (b1) atomicAdd(&bins[1], b1);
-        if (b2) atomicAdd(&bins[2], b2);
-        if (b3) atomicAdd(&bins[3], b3);
+#pragma unroll
+    for (int q = 0; q < 4; q++) b[q] = wave_sum(b[q]);
+    const int w = threadIdx.x / WAVE;
+    if ((threadIdx.x & (WAVE - 1)) == 0)
+#pragma unroll
+        for (int q = 0; q < 4; q++) part[w][q] = b[q];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        int t = 0;
+        for (int k = 0; k < RB / WAVE; k++) t += part[k][threadIdx.x];
+        if (t) atomicAdd(&bins[threadIdx.x], t);
     }
 }
 
@@ -333,7 +336,7 @@ int sort_rows_t(sable_ctx_t* ctx, int32_t N, const int32_t* rp, const int32_t* c
     int* bins = cv.take<int>(4);
     void* tmp = cv.p + cv.used;
     SABLE_HIP(hipMemsetAsync(bins, 0, 16, st));
-    hipLaunchKernelGGL(k_row_lengths, dim3(gs(N)), dim3(RB), 0, st, N, rp, len, keys, rows, bins);
+    hipLaunchKernelGGL(k_row_lengths, dim3(std::min(gs(N), 2048u)), dim3(RB), 0, st, N, rp, len, keys, rows, bins);
     SABLE_LAUNCH_CHECK();
     rc = sort_pairs_u64(st, keys, rows, N, bits_for(nnz), tmp);
     if (rc) return rc;
