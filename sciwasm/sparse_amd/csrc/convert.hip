@@ -288,6 +288,13 @@ static unsigned gs(int64_t n) {
     return g > 32768 ? 32768 : g;
 }
 
+// Grid of the grid-stride reductions that end in one atomic per wave on a
+// single counter: few enough workgroups that those atomics do not queue.
+static unsigned gs_reduce(int64_t n) {
+    unsigned g = grid_for(n, CB);
+    return g > 1024 ? 1024 : g;
+}
+
 static int read_err(sable_ctx_t* ctx, int* err_dev, int* out) {
     SABLE_HIP(hipMemcpyAsync(ctx->pinned, err_dev, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     SABLE_HIP(hipStreamSynchronize(ctx->stream));
@@ -484,7 +491,7 @@ SABLE_API int sable_csr_num_cols(sable_ctx_t* ctx, int32_t N, const int32_t* row
     if (rc) return rc;
     int* out = (int*)s;
     SABLE_HIP(hipMemsetAsync(out, 0, sizeof(int), ctx->stream));
-    hipLaunchKernelGGL(k_max_row_len, dim3(gs(N)), dim3(CB), 0, ctx->stream, N, row_ptr, out);
+    hipLaunchKernelGGL(k_max_row_len, dim3(gs_reduce(N)), dim3(CB), 0, ctx->stream, N, row_ptr, out);
     SABLE_LAUNCH_CHECK();
     return read_i32(ctx, out, 1, nc);
 }
@@ -531,7 +538,7 @@ static int diag_hist(sable_ctx_t* ctx, int32_t N, const int32_t* rp, const int32
     hipLaunchKernelGGL(k_fill_t<int32_t>, dim3(gs(nd2 + 1)), dim3(CB), 0, st, ind, nd2 + 1);
     hipLaunchKernelGGL(k_set4, dim3(1), dim3(1), 0, st, stats, 0, 0x7fffffff, 0, 0);
     hipLaunchKernelGGL(k_diag_hist, dim3(gs(N)), dim3(CB), 0, st, N, rp, col, ind, stats + 2);
-    hipLaunchKernelGGL(k_diag_flags, dim3(gs(nd2)), dim3(CB), 0, st, nd2, N, ind, flags, stats);
+    hipLaunchKernelGGL(k_diag_flags, dim3(gs_reduce(nd2)), dim3(CB), 0, st, nd2, N, ind, flags, stats);
     SABLE_LAUNCH_CHECK();
     *ind_out = ind;
     *flags_out = flags;
@@ -643,7 +650,7 @@ SABLE_API int sable_count_transpose_cols_ell(sable_ctx_t* ctx, sable_dtype_t dty
                            (const double*)data, indices, cnt, st4 + 1);
     else
         return fail(SABLE_EINVAL, "bad dtype");
-    hipLaunchKernelGGL(k_max_i32, dim3(gs(N)), dim3(CB), 0, st, (int64_t)N, cnt, st4);
+    hipLaunchKernelGGL(k_max_i32, dim3(gs_reduce(N)), dim3(CB), 0, st, (int64_t)N, cnt, st4);
     SABLE_LAUNCH_CHECK();
     int32_t h[2];
     rc = read_i32(ctx, (int32_t*)st4, 2, h);

@@ -275,13 +275,16 @@ __global__ __launch_bounds__(EB) void k_dia_pack(int32_t N, int32_t stride, cons
 // ELL: per-row nonzero counts; count_cols = max, nnz = total.
 template <typename T>
 __global__ void k_ell_count(int32_t N, int32_t nc, const T* __restrict__ data, unsigned long long* __restrict__ acc) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int c = 0;
-    if (i < N)
+    int m = 0;
+    long long s = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        int c = 0;
         for (int32_t k = 0; k < nc; ++k) c += nonzero(data[(int64_t)k * N + i]) ? 1 : 0;
-    int m = c;
+        m = max(m, c);
+        s += c;
+    }
     for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, WAVE));
-    const int s = wave_sum(c);
+    s = wave_sum(s);
     if ((threadIdx.x & (WAVE - 1)) == 0) {
         if (m) atomicMax(&acc[0], (unsigned long long)m);
         if (s) atomicAdd(&acc[1], (unsigned long long)s);
@@ -622,7 +625,9 @@ static int count_ell_t(sable_ctx_t* ctx, int32_t N, int32_t nc, const void* data
     if (rc) return rc;
     unsigned long long* acc = (unsigned long long*)s;
     SABLE_HIP(hipMemsetAsync(acc, 0, 16, ctx->stream));
-    hipLaunchKernelGGL(k_ell_count<T>, dim3(gs(N)), dim3(EB), 0, ctx->stream, N, nc, (const T*)data, acc);
+    // grid-stride on a capped grid: one atomic per wave on each counter
+    hipLaunchKernelGGL(k_ell_count<T>, dim3(std::min(gs(N), 1024u)), dim3(EB), 0, ctx->stream, N, nc, (const T*)data,
+                       acc);
     SABLE_LAUNCH_CHECK();
     int64_t h[2];
     rc = read_device_i64(ctx, acc, 2, h);
