@@ -120,6 +120,7 @@ __global__ void k_gather_rows(int32_t N, const int32_t* __restrict__ perm, const
 // ---------------------------------------------------------------- reorder_NN
 constexpr int NN_B = 1024;  // rows per bucket (the reference's B)
 constexpr int NN_W = 16;    // largest window
+constexpr int NN_RUNS = 12;  // rows of at most this many entries keep their line runs in LDS
 
 // Runs of consecutive entries in the same 16-column line, per row.
 __global__ void k_nlines(int32_t N, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
@@ -186,11 +187,30 @@ __global__ __launch_bounds__(RB) void k_reorder_nn(int32_t N, const int32_t* __r
     __shared__ float jac[NN_W][NN_W];
     __shared__ int32_t nid[NN_W][NN_W];
     __shared__ int32_t s_next;
+    // line runs of every row of a bucket of short rows (nnz <= NN_RUNS):
+    // run r of row i is runs[i * NN_RUNS + r], -1 for a line >= L
+    __shared__ int32_t runs[NN_B * NN_RUNS];
+    __shared__ uint8_t nruns[NN_B];
     const int tid = threadIdx.x;
     int32_t* lu = lu_pool + (int64_t)blockIdx.x * L;
     for (int b = blockIdx.x; b < nbuckets; b += gridDim.x) {
         const int32_t start = bstart[b], end = bend[b], n = end - start;
-        for (int i = tid; i < n; i += RB) vis[i] = 0;
+        // the rows of a bucket have equal nnz
+        const bool in_lds = n > 0 && rp[start + 1] - rp[start] <= NN_RUNS;
+        for (int i = tid; i < n; i += RB) {
+            vis[i] = 0;
+            if (in_lds) {
+                const int32_t c = start + i, e = rp[c + 1];
+                int r = 0;
+                for (int32_t j = rp[c]; j < e;) {
+                    const int32_t ln = col[j] >> 4;
+                    runs[i * NN_RUNS + r++] = (ln >= 0 && ln < L) ? ln : -1;
+                    ++j;
+                    while (j < e && (col[j - 1] >> 4) == (col[j] >> 4)) ++j;
+                }
+                nruns[i] = (uint8_t)r;
+            }
+        }
         __syncthreads();
         int32_t index = start, pos = start;
         int nnb = 0, count = 0;
@@ -207,23 +227,46 @@ __global__ __launch_bounds__(RB) void k_reorder_nn(int32_t N, const int32_t* __r
             // similarity of every unvisited row of the bucket
             // lines past N/16 (columns >= N) are not tracked, as the
             // reference's Int32Array(N/16 + 1) drops those stores
-            for (int32_t j = rp[index] + tid; j < rp[index + 1]; j += RB) {
-                const int32_t ln = col[j] >> 4;
-                if (ln >= 0 && ln < L) lu[ln] = index;
-            }
-            __threadfence_block();
-            __syncthreads();
-            for (int i = tid; i < n; i += RB) {
-                if (vis[i] != 0) continue;
-                const int32_t c = start + i, e = rp[c + 1];
-                int32_t al = 0;
-                for (int32_t j = rp[c]; j < e;) {
-                    const int32_t ln = col[j] >> 4;
-                    if (ln >= 0 && ln < L && lu[ln] == index) ++al;
-                    ++j;
-                    while (j < e && (col[j - 1] >> 4) == (col[j] >> 4)) ++j;
+            if (in_lds) {
+                // the window row's valid lines are its own runs in LDS; a
+                // candidate's run counts when its line is among them (the
+                // line table's lu[ln] == index)
+                __syncthreads();
+                const int ii = index - start, ni = nruns[ii];
+                for (int i = tid; i < n; i += RB) {
+                    if (vis[i] != 0) continue;
+                    const int nc = nruns[i];
+                    int32_t al = 0;
+                    for (int r = 0; r < nc; ++r) {
+                        const int32_t ln = runs[i * NN_RUNS + r];
+                        if (ln < 0) continue;
+                        for (int q = 0; q < ni; ++q)
+                            if (runs[ii * NN_RUNS + q] == ln) {
+                                ++al;
+                                break;
+                            }
+                    }
+                    cj[i] = (double)al / (double)(ni + nc - al);
                 }
-                cj[i] = (double)al / (double)(nlines[index] + nlines[c] - al);
+            } else {
+                for (int32_t j = rp[index] + tid; j < rp[index + 1]; j += RB) {
+                    const int32_t ln = col[j] >> 4;
+                    if (ln >= 0 && ln < L) lu[ln] = index;
+                }
+                __threadfence_block();
+                __syncthreads();
+                for (int i = tid; i < n; i += RB) {
+                    if (vis[i] != 0) continue;
+                    const int32_t c = start + i, e = rp[c + 1];
+                    int32_t al = 0;
+                    for (int32_t j = rp[c]; j < e;) {
+                        const int32_t ln = col[j] >> 4;
+                        if (ln >= 0 && ln < L && lu[ln] == index) ++al;
+                        ++j;
+                        while (j < e && (col[j - 1] >> 4) == (col[j] >> 4)) ++j;
+                    }
+                    cj[i] = (double)al / (double)(nlines[index] + nlines[c] - al);
+                }
             }
             __syncthreads();
             // The candidates are folded into the list in row order by the
