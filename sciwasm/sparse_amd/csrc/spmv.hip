@@ -410,8 +410,12 @@ __global__ __launch_bounds__(BS) void k_dia(int32_t N, int32_t nd, int32_t strid
     for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N;
          n += (int64_t)gridDim.x * blockDim.x) {
         T t = beta == T(1) ? y[n] : (beta == T(0) ? T(0) : beta * y[n]);
-        // U diagonals per step: all loads of a step are issued (predicated)
-        // before the products are added in diagonal order
+        // U diagonals per step: all loads of a step are issued before the
+        // products are added in diagonal order.  Entries outside the stored
+        // run of a diagonal load a clamped address and are skipped, so the
+        // loads are straight-line code (a guarded load makes every later
+        // wait conservative); the diagonals' data are read once
+        // (non-temporal), x is reused by neighbouring rows.
         for (int k0 = 0; k0 < nd; k0 += U) {
             T d[U], xv[U];
             bool ok[U];
@@ -421,10 +425,12 @@ __global__ __launch_bounds__(BS) void k_dia(int32_t N, int32_t nd, int32_t strid
                 const int off = k < nd ? (k < 1024 ? offs[k] : offset[k]) : 0;
                 const int64_t c = n + off;
                 const int64_t slot = n - (off < 0 ? shiftneg : 0);
-                // entries outside the stored run of this diagonal are not read
                 ok[u] = k < nd && c >= 0 && c < N && slot >= 0 && slot < stride;
-                d[u] = ok[u] ? data[(int64_t)k * stride + slot] : T(0);
-                xv[u] = ok[u] ? x[c] : T(0);
+                const int64_t kc = k < nd ? k : nd - 1;
+                const int64_t sc = slot < 0 ? 0 : (slot >= stride ? stride - 1 : slot);
+                const int64_t cc = c < 0 ? 0 : (c >= N ? N - 1 : c);
+                d[u] = __builtin_nontemporal_load(data + kc * stride + sc);
+                xv[u] = x[cc];
             }
 #pragma unroll
             for (int u = 0; u < U; u++)
@@ -914,12 +920,15 @@ SABLE_API int sable_spmv_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, i
     SABLE_CHECK(stride >= 0 && stride <= N, SABLE_EINVAL, "stride %d outside [0, N=%d]", stride, N);
     if (N == 0 || inner_max == 0) return SABLE_OK;
     unsigned g = grid_for(N, BS);
+    // with stride 0 no diagonal stores an entry (and the kernel's clamped
+    // loads need stride >= 1): y = beta*y
+    const int32_t ndk = stride > 0 ? nd : 0;
     for (int it = 0; it < inner_max; it++) {
         if (dtype == SABLE_F32)
-            hipLaunchKernelGGL(k_dia<float>, dim3(g), dim3(BS), 0, ctx->stream, N, nd, stride, offset,
+            hipLaunchKernelGGL(k_dia<float>, dim3(g), dim3(BS), 0, ctx->stream, N, ndk, stride, offset,
                                (const float*)data, (const float*)x, (float*)y, (float)beta);
         else if (dtype == SABLE_F64)
-            hipLaunchKernelGGL(k_dia<double>, dim3(g), dim3(BS), 0, ctx->stream, N, nd, stride, offset,
+            hipLaunchKernelGGL(k_dia<double>, dim3(g), dim3(BS), 0, ctx->stream, N, ndk, stride, offset,
                                (const double*)data, (const double*)x, (double*)y, beta);
         else
             return fail(SABLE_EINVAL, "bad dtype");
