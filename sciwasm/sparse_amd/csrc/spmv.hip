@@ -450,22 +450,24 @@ __global__ __launch_bounds__(BS) void k_ell(int32_t N, int32_t nc, const int32_t
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (int64_t)gridDim.x * blockDim.x) {
         T t = beta == T(1) ? y[i] : (beta == T(0) ? T(0) : beta * y[i]);
-        // U slots per step, loads predicated on j < nc and all issued before
-        // the products are added in slot order (4 keeps the number of
-        // column-major streams a wave touches at once small)
+        // U slots per step, all loads issued before the products are added
+        // in slot order (4 keeps the number of column-major streams a wave
+        // touches at once small).  Slots past the width load the last slot
+        // and are not added, so the loads are straight-line code; the
+        // matrix is read once (non-temporal).
         constexpr int U = 4;
         for (int j = 0; j < nc; j += U) {
             int c[U];
             T d[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int64_t e = (int64_t)(j + u) * N + i;
-                c[u] = j + u < nc ? idx[e] : 0;
-                d[u] = j + u < nc ? data[e] : T(0);
+                const int64_t e = (int64_t)min(j + u, nc - 1) * N + i;
+                c[u] = __builtin_nontemporal_load(idx + e);
+                d[u] = __builtin_nontemporal_load(data + e);
             }
             T xv[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) xv[u] = j + u < nc ? x[c[u]] : T(0);
+            for (int u = 0; u < U; u++) xv[u] = x[c[u]];
 #pragma unroll
             for (int u = 0; u < U; u++)
                 if (j + u < nc) t = add_rn(mul_rn(d[u], xv[u]), t);
