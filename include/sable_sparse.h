@@ -142,12 +142,12 @@ SABLE_API int sable_spmv_csr_panels(sable_ctx_t* ctx, const sable_csr_panels_t* 
 /* Column-sorted tiles (no reference counterpart; a schedule of the same
  * CSR product for matrices whose columns are scattered over an x much larger
  * than one XCD's L2): rows are cut into tiles of tile_rows rows and columns
- * into panels of panel_cols columns (both powers of two, tile_rows *
- * panel_cols <= 2^32, tile_rows values must fit the 160 KiB LDS; 0 = 16384
- * rows and 8192 columns); each (panel, tile) keeps
- * its entries sorted by
- * column, so that consecutive lanes gather neighbouring x values, and sums
- * its rows in LDS.  The plan owns its copy of the
+ * into panels of panel_cols columns (tile_rows a multiple of 64 and at most
+ * 20480, panel_cols a power of two >= 64, tile_rows * panel_cols <= 2^32;
+ * 0 = 16384 rows and 8192 columns); each (panel, tile) keeps its entries
+ * sorted by column, so that consecutive lanes gather neighbouring x values,
+ * and sums its rows in LDS in FP64 (products in the value type, each row's
+ * sum rounded to it once).  The plan owns its copy of the
  * matrix (row_ptr[0] must be 0).  Same y = A.x + beta*y semantics and
  * tolerance as sable_spmv_csr; the summation order of a row depends on LDS
  * atomic order, so results are not bit-reproducible from run to run. */
@@ -163,6 +163,39 @@ SABLE_API int sable_spmv_csr_tiles(sable_ctx_t* ctx, const sable_csr_tiles_t* p,
  * panel, y updated after each; 1 = one launch, each row tile walks all
  * panels with its row sums in LDS and writes y once (default). */
 SABLE_API int sable_csr_tiles_select(int mode);
+
+/* Many tile plans in one persistent launch, one right-hand side (no
+ * reference counterpart; the batched form of sable_spmv_csr_tiles for
+ * batches of matrices whose x outgrows what a row block's random gathers
+ * can hit in the L1).  The row tiles of all items form one global work
+ * list: item k owns units [unit_begin_k, unit_begin_k + ntr_k).
+ * Workgroups claim units from a device counter; each one sums its tile's
+ * rows in LDS over the item's panels and writes y = A.x + beta*y for those
+ * rows.  Items are filled by sable_csr_tiles_batch_item, live in DEVICE
+ * memory and are ordered by ascending unit_begin; all plans share `dtype`.
+ * max_tile_rows (>= every item's tile_rows, at most 20480: the row sums are
+ * FP64) sizes the LDS of a workgroup.  An item whose tile_rows exceeds it
+ * is skipped and the call fails with SABLE_EINVAL (the call synchronises to
+ * read that flag).  workgroups = 0 picks 4 per CU. */
+typedef struct {
+    const int32_t* tile_ptr;
+    const uint32_t* packed;
+    const void* val;
+    const void* x;
+    void* y;
+    int64_t unit_begin; /* first global unit (row tile) of this item */
+    double beta;
+    int32_t N;
+    int32_t tile_rows;
+    int32_t ntr;        /* row tiles = units of this item */
+    int32_t npanels;
+    int32_t panel_cols;
+    int32_t row_bits;
+} sable_batch_tiles_item_t;
+SABLE_API int sable_csr_tiles_batch_item(const sable_csr_tiles_t* p, const void* x, void* y, double beta,
+                                         int64_t unit_begin, sable_batch_tiles_item_t* out);
+SABLE_API int sable_batch_spmv_tiles(sable_ctx_t* ctx, sable_dtype_t dtype, const sable_batch_tiles_item_t* d_items,
+                                     int nitems, int64_t total_units, int32_t max_tile_rows, int workgroups);
 
 /* COO.  Row-sorted input runs a deterministic segmented reduction (row
  * heads found on the device, then the CSR block kernel); unsorted input is

@@ -14,7 +14,7 @@ from .spmv import spmv, CsrPlan, CsrPanels, CsrTiles, CooPlan  # noqa: F401
 from .convert import coo_csr, coo_csc, num_cols, csr_ell, num_diags, csr_dia, transpose  # noqa: F401
 from .spts import (SptsAnalysis, CSR_create_level_sets, CSR_create_level_sets_with_reorder,  # noqa: F401
                    spts_level_csr, spts_syncfree, spts_csc)
-from .serve import Server, batch_spmv, batch_spts, batch_spts_refined, lower_dominant  # noqa: F401
+from .serve import Server, batch_spmv, batch_spmv_tiles, batch_tile_plans, batch_spts, batch_spts_refined, lower_dominant  # noqa: F401
 from .mm import MMInfo, read_matrix_MM_files, create_COO_from_MM, create_LCOO_from_MM, mmread  # noqa: F401
 from .ewise import (expm1, log1p, sin, tan, pow, deg2rad, rad2deg, sign, abs, neg, sqrt, ceil, floor,  # noqa: F401
                     trunc, nearest, eliminate_zeros, count_nonzeros)
@@ -28,7 +28,7 @@ __all__ = [
     "spmv", "CsrPlan", "CsrPanels", "CsrTiles", "CooPlan",
     "coo_csr", "coo_csc", "num_cols", "csr_ell", "num_diags", "csr_dia", "transpose",
     "SptsAnalysis", "CSR_create_level_sets", "CSR_create_level_sets_with_reorder", "spts_level_csr",
-    "spts_syncfree", "spts_csc", "Server", "batch_spmv", "batch_spts", "batch_spts_refined", "lower_dominant",
+    "spts_syncfree", "spts_csc", "Server", "batch_spmv", "batch_spmv_tiles", "batch_tile_plans", "batch_spts", "batch_spts_refined", "lower_dominant",
     "MMInfo", "read_matrix_MM_files", "create_COO_from_MM", "create_LCOO_from_MM", "mmread",
     "expm1", "log1p", "sin", "tan", "pow", "deg2rad", "rad2deg", "sign", "abs", "neg", "sqrt", "ceil", "floor",
     "trunc", "nearest", "eliminate_zeros", "count_nonzeros", "sort_rows_by_nnz", "sort_y_rows_by_nnz", "permute_rows",

@@ -109,6 +109,12 @@ __device__ __forceinline__ V ld_nt(const V* p) {
     return __builtin_nontemporal_load(p);
 }
 
+// Row sums are held in LDS in FP64 for either value type: each product is
+// formed in the value type and added to its row with an FP64 LDS atomic,
+// and the sum is rounded to the value type once.  On gfx950 the FP32 LDS
+// atomic add has measured about 2.5x slower than the FP64 one in this loop,
+// and the FP64 sum only tightens the FP32 result.
+
 // One panel: workgroup = one row tile.  The tile's row sums live in LDS.
 template <typename T>
 __global__ __launch_bounds__(TILE_BS) void k_tiles(int32_t N, int32_t tile_rows, int row_bits, int32_t ntr,
@@ -117,9 +123,9 @@ __global__ __launch_bounds__(TILE_BS) void k_tiles(int32_t N, int32_t tile_rows,
                                                    const uint32_t* __restrict__ packed, const T* __restrict__ val,
                                                    const T* __restrict__ x, T* __restrict__ y, T beta, int first) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* acc = reinterpret_cast<T*>(smem_raw);
+    double* acc = reinterpret_cast<double*>(smem_raw);
     const int tr = blockIdx.x;
-    for (int r = threadIdx.x; r < tile_rows; r += TILE_BS) acc[r] = T(0);
+    for (int r = threadIdx.x; r < tile_rows; r += TILE_BS) acc[r] = 0.0;
     __syncthreads();
     const int64_t t = (int64_t)panel * ntr + tr;
     const int64_t k0 = tile_ptr[t], k1 = tile_ptr[t + 1];
@@ -139,14 +145,14 @@ __global__ __launch_bounds__(TILE_BS) void k_tiles(int32_t N, int32_t tile_rows,
         for (int u = 0; u < TILE_U; u++) xv[u] = (k + (int64_t)u * TILE_BS < k1) ? xp[pk[u] >> row_bits] : T(0);
 #pragma unroll
         for (int u = 0; u < TILE_U; u++)
-            if (k + (int64_t)u * TILE_BS < k1) atomicAdd(&acc[pk[u] & rmask], v[u] * xv[u]);
+            if (k + (int64_t)u * TILE_BS < k1) atomicAdd(&acc[pk[u] & rmask], (double)(v[u] * xv[u]));
     }
     __syncthreads();
     const int64_t r0 = (int64_t)tr * tile_rows;
     for (int r = threadIdx.x; r < tile_rows; r += TILE_BS) {
         const int64_t i = r0 + r;
         if (i >= N) break;
-        const T a = acc[r];
+        const T a = (T)acc[r];
         if (first) {
             if (beta == T(0)) y[i] = a;
             else if (beta == T(1)) { if (a != T(0)) y[i] = y[i] + a; }
@@ -169,9 +175,9 @@ __global__ __launch_bounds__(TILE_BS) void k_tiles_fused(int32_t N, int32_t tile
                                                          const T* __restrict__ val, const T* __restrict__ x,
                                                          T* __restrict__ y, T beta) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* acc = reinterpret_cast<T*>(smem_raw);
+    double* acc = reinterpret_cast<double*>(smem_raw);
     const int tr = blockIdx.x;
-    for (int r = threadIdx.x; r < tile_rows; r += TILE_BS) acc[r] = T(0);
+    for (int r = threadIdx.x; r < tile_rows; r += TILE_BS) acc[r] = 0.0;
     __syncthreads();
     const uint32_t rmask = (1u << row_bits) - 1u;
     for (int32_t panel = 0; panel < npanels; panel++) {
@@ -193,7 +199,7 @@ __global__ __launch_bounds__(TILE_BS) void k_tiles_fused(int32_t N, int32_t tile
                 xv[u] = (k + (int64_t)u * TILE_BS < k1) ? xp[pk[u] >> row_bits] : T(0);
 #pragma unroll
             for (int u = 0; u < TILE_U; u++)
-                if (k + (int64_t)u * TILE_BS < k1) atomicAdd(&acc[pk[u] & rmask], v[u] * xv[u]);
+                if (k + (int64_t)u * TILE_BS < k1) atomicAdd(&acc[pk[u] & rmask], (double)(v[u] * xv[u]));
         }
     }
     __syncthreads();
@@ -201,11 +207,108 @@ __global__ __launch_bounds__(TILE_BS) void k_tiles_fused(int32_t N, int32_t tile
     for (int r = threadIdx.x; r < tile_rows; r += TILE_BS) {
         const int64_t i = r0 + r;
         if (i >= N) break;
-        const T a = acc[r];
+        const T a = (T)acc[r];
         if (beta == T(0)) y[i] = a;
         else if (beta == T(1)) { if (a != T(0)) y[i] = y[i] + a; }
         else y[i] = beta * y[i] + a;
     }
+}
+
+// Many tile plans in one persistent launch: a workgroup claims one unit
+// (item, row tile) at a time, finds its item by binary search over the
+// items' unit_begin, and runs the row tile as k_tiles_fused does.  The
+// claiming thread searches while the others wait at the barrier; a unit is
+// a whole row tile, so the search is paid once per tile_rows rows.
+// The item's arrays are read through global-address-space pointers:
+// pointers loaded from memory are otherwise generic, and generic loads
+// share the LDS counter, so every wait for a gather would also wait for the
+// LDS atomics in flight.
+constexpr int BT_BS = 256;
+#define SABLE_GLOBAL __attribute__((address_space(1)))
+
+template <typename T>
+__global__ __launch_bounds__(BT_BS) void k_batch_tiles(const sable_batch_tiles_item_t* __restrict__ items,
+                                                       int nitems, int64_t total, int32_t max_rows,
+                                                       unsigned long long* __restrict__ counter,
+                                                       int* __restrict__ err) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* acc = reinterpret_cast<double*>(smem_raw);
+    __shared__ long long s_g;
+    __shared__ int s_k;
+    while (true) {
+        if (threadIdx.x == 0) {
+            const long long g = (long long)atomicAdd(counter, 1ull);
+            int lo = 0;
+            if (g < total) {
+                int hi = nitems - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (items[mid].unit_begin <= g) lo = mid;
+                    else hi = mid - 1;
+                }
+            }
+            s_g = g;
+            s_k = lo;
+        }
+        __syncthreads();
+        const long long g = s_g;
+        if (g >= total) break;
+        const sable_batch_tiles_item_t it = items[s_k];
+        if (it.tile_rows > max_rows) {
+            if (threadIdx.x == 0) atomicOr(err, 1);
+            __syncthreads();  // s_g and s_k are read before the next claim rewrites them
+            continue;
+        }
+        const int32_t tr = (int32_t)(g - it.unit_begin);
+        const int64_t r0 = (int64_t)tr * it.tile_rows;
+        const int32_t nrows = (int32_t)min((int64_t)it.tile_rows, (int64_t)it.N - r0);
+        for (int r = threadIdx.x; r < nrows; r += BT_BS) acc[r] = 0.0;
+        __syncthreads();
+        const uint32_t rmask = (1u << it.row_bits) - 1u;
+        const int row_bits = it.row_bits;
+        const SABLE_GLOBAL int32_t* tile_ptr = (const SABLE_GLOBAL int32_t*)it.tile_ptr;
+        const SABLE_GLOBAL uint32_t* packed = (const SABLE_GLOBAL uint32_t*)it.packed;
+        const SABLE_GLOBAL T* val = (const SABLE_GLOBAL T*)it.val;
+        for (int32_t panel = 0; panel < it.npanels; panel++) {
+            const int64_t t = (int64_t)panel * it.ntr + tr;
+            const int64_t k0 = tile_ptr[t], k1 = tile_ptr[t + 1];
+            const SABLE_GLOBAL T* xp = (const SABLE_GLOBAL T*)it.x + (int64_t)panel * it.panel_cols;
+            for (int64_t k = k0 + threadIdx.x; k < k1; k += (int64_t)TILE_U * BT_BS) {
+                uint32_t pk[TILE_U];
+                T v[TILE_U];
+#pragma unroll
+                for (int u = 0; u < TILE_U; u++) {
+                    const int64_t kk = k + (int64_t)u * BT_BS;
+                    pk[u] = kk < k1 ? __builtin_nontemporal_load(packed + kk) : 0u;
+                    v[u] = kk < k1 ? __builtin_nontemporal_load(val + kk) : T(0);
+                }
+                T xv[TILE_U];
+#pragma unroll
+                for (int u = 0; u < TILE_U; u++)
+                    xv[u] = (k + (int64_t)u * BT_BS < k1) ? xp[pk[u] >> row_bits] : T(0);
+#pragma unroll
+                for (int u = 0; u < TILE_U; u++)
+                    if (k + (int64_t)u * BT_BS < k1) atomicAdd(&acc[pk[u] & rmask], (double)(v[u] * xv[u]));
+            }
+        }
+        __syncthreads();
+        SABLE_GLOBAL T* y = (SABLE_GLOBAL T*)it.y;
+        const T beta = (T)it.beta;
+        for (int r = threadIdx.x; r < nrows; r += BT_BS) {
+            const int64_t i = r0 + r;
+            const T a = (T)acc[r];
+            if (beta == T(0)) y[i] = a;
+            else if (beta == T(1)) { if (a != T(0)) y[i] = y[i] + a; }
+            else y[i] = beta * y[i] + a;
+        }
+        __syncthreads();  // acc, s_g and s_k are rewritten for the next unit
+    }
+}
+#undef SABLE_GLOBAL
+
+__global__ void k_batch_tiles_reset(unsigned long long* counter, int* err) {
+    *counter = 0ull;
+    *err = 0;
 }
 
 // 0: one launch per panel; 1: one launch over all panels (process-wide).
@@ -218,7 +321,7 @@ static unsigned gs(int64_t n) {
 
 template <typename T>
 static int tiles_run_t(hipStream_t st, const sable_csr_tiles_s* p, const void* x, void* y, double beta) {
-    const size_t shm = (size_t)p->tile_rows * sizeof(T);
+    const size_t shm = (size_t)p->tile_rows * sizeof(double);
     if (g_tiles_mode == 1) {
         hipLaunchKernelGGL(k_tiles_fused<T>, dim3((unsigned)p->ntr), dim3(TILE_BS), shm, st, p->N,
                            p->tile_rows, p->row_bits, p->ntr, p->npanels, p->panel_cols, p->tile_ptr, p->packed,
@@ -261,7 +364,8 @@ SABLE_API int sable_csr_tiles_create(sable_ctx_t* ctx, sable_dtype_t dtype, int3
     const int rb = bits_for(tile_rows - 1), cb = log2_exact(panel_cols);
     SABLE_CHECK(cb >= 6, SABLE_EINVAL, "panel_cols must be a power of two >= 64");
     SABLE_CHECK(rb + cb <= 32, SABLE_EINVAL, "tile_rows * panel_cols must not exceed 2^32");
-    SABLE_CHECK((size_t)tile_rows * vs <= 160 * 1024, SABLE_EINVAL, "a tile's row sums must fit the LDS (160 KiB)");
+    SABLE_CHECK((size_t)tile_rows * sizeof(double) <= 160 * 1024, SABLE_EINVAL,
+                "a tile's FP64 row sums must fit the LDS (160 KiB): tile_rows <= 20480");
     hipStream_t st = ctx->stream;
     int32_t ends[2];
     SABLE_HIP(hipMemcpyAsync(ends, row_ptr, 4, hipMemcpyDeviceToHost, st));
@@ -333,7 +437,7 @@ SABLE_API int sable_csr_tiles_create(sable_ctx_t* ctx, sable_dtype_t dtype, int3
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
         return cleanup(fail(SABLE_EHIP, "tile plan build failed"));
     // LDS above the default dynamic limit must be requested per kernel
-    const size_t shm = (size_t)tile_rows * vs;
+    const size_t shm = (size_t)tile_rows * sizeof(double);
     hipError_t ea = dtype == SABLE_F64
                         ? hipFuncSetAttribute((const void*)k_tiles<double>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm)
@@ -382,6 +486,70 @@ SABLE_API int sable_spmv_csr_tiles(sable_ctx_t* ctx, const sable_csr_tiles_t* p,
         const int rc = tiles_launch(ctx->stream, p, x, y, beta);
         if (rc) return rc;
     }
+    return SABLE_OK;
+}
+
+SABLE_API int sable_csr_tiles_batch_item(const sable_csr_tiles_t* p, const void* x, void* y, double beta,
+                                         int64_t unit_begin, sable_batch_tiles_item_t* out) {
+    SABLE_CHECK(p && x && y && out, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(unit_begin >= 0, SABLE_EINVAL, "negative unit_begin");
+    out->tile_ptr = p->tile_ptr;
+    out->packed = p->packed;
+    out->val = p->val;
+    out->x = x;
+    out->y = y;
+    out->unit_begin = unit_begin;
+    out->beta = beta;
+    out->N = p->N;
+    out->tile_rows = p->tile_rows;
+    out->ntr = p->ntr;
+    out->npanels = p->npanels;
+    out->panel_cols = p->panel_cols;
+    out->row_bits = p->row_bits;
+    return SABLE_OK;
+}
+
+SABLE_API int sable_batch_spmv_tiles(sable_ctx_t* ctx, sable_dtype_t dtype, const sable_batch_tiles_item_t* d_items,
+                                     int nitems, int64_t total_units, int32_t max_tile_rows, int workgroups) {
+    SABLE_CHECK(ctx && (d_items || nitems == 0), SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    SABLE_CHECK(nitems >= 0 && total_units >= 0 && workgroups >= 0, SABLE_EINVAL, "negative size");
+    if (nitems == 0 || total_units == 0) return SABLE_OK;
+    const size_t vs = sizeof(double);  // row sums are held in FP64 for either dtype
+    SABLE_CHECK(max_tile_rows > 0 && (size_t)max_tile_rows * vs <= 160 * 1024, SABLE_EINVAL,
+                "max_tile_rows must be positive and its row sums must fit the LDS (160 KiB)");
+    void* s;
+    int rc = ctx_scratch(ctx, 256, &s);
+    if (rc) return rc;
+    unsigned long long* counter = (unsigned long long*)s;
+    int* err = (int*)((char*)s + 64);
+    hipStream_t st = ctx->stream;
+    const size_t shm = (size_t)max_tile_rows * vs;
+    if (shm > 65536 - 256) {  // with the kernel's static LDS, above the default limit
+        const hipError_t ea =
+            dtype == SABLE_F64
+                ? hipFuncSetAttribute((const void*)k_batch_tiles<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)shm)
+                : hipFuncSetAttribute((const void*)k_batch_tiles<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)shm);
+        if (ea != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SABLE_EINVAL, "tile of %zu LDS bytes not supported: %s", shm, hipGetErrorString(ea));
+        }
+    }
+    int64_t grid = workgroups > 0 ? workgroups : (int64_t)ctx->num_cu * 4;
+    if (grid > total_units) grid = total_units;
+    hipLaunchKernelGGL(k_batch_tiles_reset, dim3(1), dim3(1), 0, st, counter, err);
+    if (dtype == SABLE_F64)
+        hipLaunchKernelGGL(k_batch_tiles<double>, dim3((unsigned)grid), dim3(BT_BS), shm, st, d_items, nitems,
+                           total_units, max_tile_rows, counter, err);
+    else
+        hipLaunchKernelGGL(k_batch_tiles<float>, dim3((unsigned)grid), dim3(BT_BS), shm, st, d_items, nitems,
+                           total_units, max_tile_rows, counter, err);
+    SABLE_LAUNCH_CHECK();
+    SABLE_HIP(hipMemcpyAsync(ctx->pinned, err, sizeof(int), hipMemcpyDeviceToHost, st));
+    SABLE_HIP(hipStreamSynchronize(st));
+    SABLE_CHECK(*(int*)ctx->pinned == 0, SABLE_EINVAL, "an item's tile_rows exceeds max_tile_rows");
     return SABLE_OK;
 }
 

@@ -53,6 +53,63 @@ def batch_spmv(mats, xs, ys, nrhs=1, beta=0.0, workgroups=0, plans=None):
     ctx.sync()
 
 
+class BatchTilesItem(ctypes.Structure):
+    _fields_ = [("tile_ptr", c_vp), ("packed", c_vp), ("val", c_vp), ("x", c_vp), ("y", c_vp),
+                ("unit_begin", c_i64), ("beta", c_dbl), ("N", c_i32), ("tile_rows", c_i32), ("ntr", c_i32),
+                ("npanels", c_i32), ("panel_cols", c_i32), ("row_bits", c_i32)]
+
+
+def batch_tile_plans(mats, tile_rows=4096):
+    """One column-sorted tile plan per matrix for batch_spmv_tiles: row tiles
+    of up to `tile_rows` rows and a single column panel (the next power of
+    two >= N), so that each row tile gathers its x in ascending column
+    order.  Matrices with N = 0 get None."""
+    from .spmv import CsrTiles
+    plans = []
+    for A in mats:
+        if A.N == 0:
+            plans.append(None)
+            continue
+        tr = min(int(tile_rows), (A.N + 63) // 64 * 64)
+        pc = 64
+        while pc < A.N:
+            pc *= 2
+        plans.append(CsrTiles(A, tile_rows=tr, panel_cols=pc))
+    return plans
+
+
+def batch_tiles_items(plans, xs, ys, beta=0.0):
+    """The device item list of batch_spmv_tiles: (items, nitems, total_units,
+    max_tile_rows, dtype, ctx), or None when every plan is None."""
+    items, total, maxr, ctx, dt = [], 0, 0, None, None
+    for p, x, y in zip(plans, xs, ys):
+        if p is None:
+            continue
+        if dt is None:
+            ctx, dt = p.ctx, np.dtype(p.A.dtype)
+        elif np.dtype(p.A.dtype) != dt:
+            raise ValueError("batch_spmv_tiles: all plans must share one dtype")
+        it = BatchTilesItem()
+        call("sable_csr_tiles_batch_item", p.handle, x.ptr, y.ptr, float(beta), total, ctypes.byref(it))
+        items.append(it)
+        total += it.ntr
+        maxr = max(maxr, it.tile_rows)
+    if not items:
+        return None
+    return _upload_items(items, ctx), len(items), total, maxr, dt, ctx
+
+
+def batch_spmv_tiles(plans, xs, ys, beta=0.0, workgroups=0):
+    """ys[k] = A_k.xs[k] (+ beta*ys[k]) for every tile plan (CsrTiles) in one
+    persistent launch, one right-hand side.  None plans (N = 0) are
+    skipped."""
+    b = batch_tiles_items(plans, xs, ys, beta)
+    if b is None:
+        return
+    d, n, total, maxr, dt, ctx = b
+    call("sable_batch_spmv_tiles", ctx.handle, dtype_code(dt), d.ptr, n, total, maxr, int(workgroups))
+
+
 def batch_spts(mats, bs, xs, lanes=4):
     ctx = mats[0].row.ctx
     items, total = [], 0

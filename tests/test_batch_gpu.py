@@ -12,7 +12,7 @@ import pytest
 
 import oracle
 import sciwasm.sparse_amd as sp
-from sciwasm.sparse_amd import gen, DeviceArray
+from sciwasm.sparse_amd import gen, DeviceArray, SableError
 
 pytestmark = pytest.mark.gpu
 
@@ -92,6 +92,97 @@ def test_batch_spmv_blocks_beyond_lds(ctx, mats, nrhs, dtype):
             yref = oracle.spmv_csr(n, rp, c, v.astype(dtype), X[k][:, r].copy(), np.zeros(n, dtype))
             d = np.abs(Y[:, r].astype(np.float64) - yref)
             assert np.all(d <= bound(rp, c, v.astype(dtype), X[k][:, r], dtype)), (k, r)
+
+
+def tiles_case(dtype):
+    """Uneven matrices for the batched tile launch: the batch law plus sizes
+    at and around the 64-row and 4096-row tile edges, a matrix with empty
+    rows and an empty matrix."""
+    out = [(n, rp, c, v.astype(dtype)) for n, rp, c, v in gen.batch_csr(16, rows_lo=512, rows_hi=20000, seed=21)]
+    for k, n in enumerate((1, 63, 64, 65, 4095, 4097, 9000)):
+        rp, c, v = gen.random_csr(n, 9, seed=40 + k, dtype=dtype)
+        out.append((n, rp, c, v))
+    rp, c, v = gen.random_csr(7000, 6, seed=50, dtype=dtype, empty_rows=0.3)
+    out.append((7000, rp, c, v))
+    out.append((0, np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0, dtype)))
+    return out
+
+
+@pytest.mark.parametrize("beta", [0.0, 1.0, 0.5])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batch_spmv_tiles(ctx, dtype, beta):
+    """batch_spmv_tiles against the oracle: every matrix's rows within the
+    CSR bound (the LDS sums may add in any order), plus one rounding of
+    beta*y0 + sum for beta != 0."""
+    rng = np.random.default_rng(5)
+    case = tiles_case(dtype)
+    As, xs, ys, X, Y0 = [], [], [], [], []
+    for n, rp, c, v in case:
+        A = sp.csr_from_host(n, rp, c, v) if n else None
+        As.append(A)
+        x = rng.uniform(-1, 1, n).astype(dtype)
+        y0 = rng.uniform(-1, 1, n).astype(dtype)
+        X.append(x)
+        Y0.append(y0)
+        xs.append(DeviceArray.from_host(x) if n else None)
+        ys.append(DeviceArray.from_host(y0) if n else None)
+    made = iter(sp.batch_tile_plans([A for A in As if A is not None]))
+    plans = [next(made) if A is not None else None for A in As]
+    sp.batch_spmv_tiles(plans, xs, ys, beta=beta)
+    ctx.sync()
+    u = np.finfo(dtype).eps / 2
+    for k, (n, rp, c, v) in enumerate(case):
+        if n == 0:
+            assert plans[k] is None
+            continue
+        yref = oracle.spmv_csr(n, rp, c, v, X[k].copy(), np.zeros(n, dtype)).astype(np.float64)
+        b = bound(rp, c, v, X[k], dtype)
+        by0 = beta * Y0[k].astype(np.float64)
+        want = by0 + yref
+        tol = b + 2 * u * (np.abs(by0) + np.abs(yref) + b)
+        d = np.abs(ys[k].numpy().astype(np.float64) - want)
+        assert np.all(d <= tol), (k, n, float(np.max(d - tol)))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batch_spmv_tiles_many_panels(ctx, dtype):
+    """Plans with small tiles and several panels (the general tile layout)
+    in one batch, equal to each plan's own one-launch product within the
+    bound, and max_tile_rows below an item's tile_rows refused after a
+    launch that skips that item."""
+    import ctypes
+    from sciwasm.sparse_amd import _lib
+    from sciwasm.sparse_amd.serve import BatchTilesItem, _upload_items
+    from sciwasm.sparse_amd.core import dtype_code
+    rng = np.random.default_rng(6)
+    shapes = [(3000, 128, 256), (5000, 256, 1024), (777, 64, 64), (12000, 1024, 2048)]
+    As, plans, xs, ys, X, M = [], [], [], [], [], []
+    for k, (n, tr, pc) in enumerate(shapes):
+        rp, c, v = gen.random_csr(n, 11, seed=60 + k, dtype=dtype)
+        M.append((n, rp, c, v))
+        A = sp.csr_from_host(n, rp, c, v)
+        As.append(A)
+        plans.append(sp.CsrTiles(A, tile_rows=tr, panel_cols=pc))
+        x = rng.uniform(-1, 1, n).astype(dtype)
+        X.append(x)
+        xs.append(DeviceArray.from_host(x))
+        ys.append(DeviceArray.zeros(n, dtype))
+    sp.batch_spmv_tiles(plans, xs, ys)
+    ctx.sync()
+    for k, (n, rp, c, v) in enumerate(M):
+        yref = oracle.spmv_csr(n, rp, c, v, X[k].copy(), np.zeros(n, dtype))
+        d = np.abs(ys[k].numpy().astype(np.float64) - yref)
+        assert np.all(d <= bound(rp, c, v, X[k], dtype)), k
+    items, total = [], 0
+    for p, x, y in zip(plans, xs, ys):
+        it = BatchTilesItem()
+        _lib.call("sable_csr_tiles_batch_item", p.handle, x.ptr, y.ptr, 0.0, total, ctypes.byref(it))
+        items.append(it)
+        total += it.ntr
+    d = _upload_items(items, ctx)
+    with pytest.raises(SableError) as e:
+        _lib.call("sable_batch_spmv_tiles", ctx.handle, dtype_code(dtype), d.ptr, len(items), total, 512, 0)
+    assert e.value.name == "SABLE_EINVAL"
 
 
 def lower_set(mats, dtype=np.float64):
