@@ -164,17 +164,19 @@ SABLE_API int sable_spmv_csr_tiles(sable_ctx_t* ctx, const sable_csr_tiles_t* p,
  * panels with its row sums in LDS and writes y once (default). */
 SABLE_API int sable_csr_tiles_select(int mode);
 
-/* Many tile plans in one persistent launch, one right-hand side (no
- * reference counterpart; the batched form of sable_spmv_csr_tiles for
- * batches of matrices whose x outgrows what a row block's random gathers
- * can hit in the L1).  The row tiles of all items form one global work
+/* Many tile plans in one persistent launch (no reference counterpart; the
+ * batched form of sable_spmv_csr_tiles for batches of matrices whose x
+ * outgrows what a row block's random gathers can hit in the L1), with nrhs
+ * in {1, 2, 4, 8} right-hand sides: x and y of every item are row-major
+ * [N][nrhs], and x must be 16-byte aligned (device allocations are).  The row tiles of all items form one global work
  * list: item k owns units [unit_begin_k, unit_begin_k + ntr_k).
  * Workgroups claim units from a device counter; each one sums its tile's
  * rows in LDS over the item's panels and writes y = A.x + beta*y for those
  * rows.  Items are filled by sable_csr_tiles_batch_item, live in DEVICE
  * memory and are ordered by ascending unit_begin; all plans share `dtype`.
- * max_tile_rows (>= every item's tile_rows, at most 20480: the row sums are
- * FP64) sizes the LDS of a workgroup.  An item whose tile_rows exceeds it
+ * max_tile_rows (>= every item's tile_rows; max_tile_rows * nrhs FP64 row
+ * sums must fit the 160 KiB LDS, i.e. at most 20480 / nrhs) sizes the LDS
+ * of a workgroup.  An item whose tile_rows exceeds it
  * is skipped and the call fails with SABLE_EINVAL (the call synchronises to
  * read that flag).  workgroups = 0 picks 4 per CU. */
 typedef struct {
@@ -194,8 +196,9 @@ typedef struct {
 } sable_batch_tiles_item_t;
 SABLE_API int sable_csr_tiles_batch_item(const sable_csr_tiles_t* p, const void* x, void* y, double beta,
                                          int64_t unit_begin, sable_batch_tiles_item_t* out);
-SABLE_API int sable_batch_spmv_tiles(sable_ctx_t* ctx, sable_dtype_t dtype, const sable_batch_tiles_item_t* d_items,
-                                     int nitems, int64_t total_units, int32_t max_tile_rows, int workgroups);
+SABLE_API int sable_batch_spmv_tiles(sable_ctx_t* ctx, sable_dtype_t dtype, int nrhs,
+                                     const sable_batch_tiles_item_t* d_items, int nitems, int64_t total_units,
+                                     int32_t max_tile_rows, int workgroups);
 
 /* COO.  Row-sorted input runs a deterministic segmented reduction (row
  * heads found on the device, then the CSR block kernel); unsorted input is

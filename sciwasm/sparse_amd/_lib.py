@@ -99,7 +99,7 @@ SIGNATURES = {
     "sable_spmv_csr_tiles": [c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_csr_tiles_select": [c_int],
     "sable_csr_tiles_batch_item": [c_vp, c_vp, c_vp, c_dbl, c_i64, c_vp],
-    "sable_batch_spmv_tiles": [c_vp, c_int, c_vp, c_int, c_i64, c_i32, c_int],
+    "sable_batch_spmv_tiles": [c_vp, c_int, c_int, c_vp, c_int, c_i64, c_i32, c_int],
     "sable_coo_plan_row_ptr": [c_vp, c_vpp],
     "sable_coo_plan_create": [c_vp, c_i32, c_i64, c_vp, c_vpp],
     "sable_coo_plan_destroy": [c_vp],

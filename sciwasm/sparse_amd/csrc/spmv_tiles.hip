@@ -219,6 +219,10 @@ __global__ __launch_bounds__(TILE_BS) void k_tiles_fused(int32_t N, int32_t tile
 // items' unit_begin, and runs the row tile as k_tiles_fused does.  The
 // claiming thread searches while the others wait at the barrier; a unit is
 // a whole row tile, so the search is paid once per tile_rows rows.
+// NR right-hand sides: x and y are row-major [N][NR], so one gather brings
+// an entry's NR x values; the LDS row sums are RHS-major (sum r of row q at
+// r * tile_rows + q), so that the 64 lanes of one atomic instruction, on
+// random rows of one right-hand side, spread over the LDS banks.
 // The item's arrays are read through global-address-space pointers:
 // pointers loaded from memory are otherwise generic, and generic loads
 // share the LDS counter, so every wait for a gather would also wait for the
@@ -226,11 +230,35 @@ __global__ __launch_bounds__(TILE_BS) void k_tiles_fused(int32_t N, int32_t tile
 constexpr int BT_BS = 256;
 #define SABLE_GLOBAL __attribute__((address_space(1)))
 
-template <typename T>
+// The NR values of row c of a row-major [N][NR] x: 16-byte loads when a row
+// is a multiple of 16 bytes (x is then 16-byte aligned), else one load.
+typedef unsigned int tile_u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int tile_u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename T, int NR>
+__device__ __forceinline__ void tile_xrow(const SABLE_GLOBAL T* x, uint32_t c, T (&out)[NR]) {
+    constexpr int BYTES = NR * (int)sizeof(T);
+    const SABLE_GLOBAL char* p = (const SABLE_GLOBAL char*)(x + (int64_t)c * NR);
+    if constexpr (BYTES % 16 == 0) {
+#pragma unroll
+        for (int q = 0; q < BYTES / 16; q++) {
+            const tile_u32x4 w = *(const SABLE_GLOBAL tile_u32x4*)(p + 16 * q);
+            __builtin_memcpy((char*)out + 16 * q, &w, 16);
+        }
+    } else if constexpr (BYTES == 8) {
+        const tile_u32x2 w = *(const SABLE_GLOBAL tile_u32x2*)p;
+        __builtin_memcpy(out, &w, 8);
+    } else {
+        out[0] = *(const SABLE_GLOBAL T*)p;
+    }
+}
+
+template <typename T, int NR>
 __global__ __launch_bounds__(BT_BS) void k_batch_tiles(const sable_batch_tiles_item_t* __restrict__ items,
                                                        int nitems, int64_t total, int32_t max_rows,
                                                        unsigned long long* __restrict__ counter,
                                                        int* __restrict__ err) {
+    constexpr int U = NR * (int)sizeof(T) >= 32 ? 2 : TILE_U;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* acc = reinterpret_cast<double*>(smem_raw);
     __shared__ long long s_g;
@@ -262,7 +290,8 @@ __global__ __launch_bounds__(BT_BS) void k_batch_tiles(const sable_batch_tiles_i
         const int32_t tr = (int32_t)(g - it.unit_begin);
         const int64_t r0 = (int64_t)tr * it.tile_rows;
         const int32_t nrows = (int32_t)min((int64_t)it.tile_rows, (int64_t)it.N - r0);
-        for (int r = threadIdx.x; r < nrows; r += BT_BS) acc[r] = 0.0;
+        const int32_t stride = it.tile_rows;
+        for (int q = threadIdx.x; q < stride * NR; q += BT_BS) acc[q] = 0.0;
         __syncthreads();
         const uint32_t rmask = (1u << it.row_bits) - 1u;
         const int row_bits = it.row_bits;
@@ -272,34 +301,43 @@ __global__ __launch_bounds__(BT_BS) void k_batch_tiles(const sable_batch_tiles_i
         for (int32_t panel = 0; panel < it.npanels; panel++) {
             const int64_t t = (int64_t)panel * it.ntr + tr;
             const int64_t k0 = tile_ptr[t], k1 = tile_ptr[t + 1];
-            const SABLE_GLOBAL T* xp = (const SABLE_GLOBAL T*)it.x + (int64_t)panel * it.panel_cols;
-            for (int64_t k = k0 + threadIdx.x; k < k1; k += (int64_t)TILE_U * BT_BS) {
-                uint32_t pk[TILE_U];
-                T v[TILE_U];
+            const SABLE_GLOBAL T* xp = (const SABLE_GLOBAL T*)it.x + (int64_t)panel * it.panel_cols * NR;
+            for (int64_t k = k0 + threadIdx.x; k < k1; k += (int64_t)U * BT_BS) {
+                uint32_t pk[U];
+                T v[U];
 #pragma unroll
-                for (int u = 0; u < TILE_U; u++) {
+                for (int u = 0; u < U; u++) {
                     const int64_t kk = k + (int64_t)u * BT_BS;
                     pk[u] = kk < k1 ? __builtin_nontemporal_load(packed + kk) : 0u;
                     v[u] = kk < k1 ? __builtin_nontemporal_load(val + kk) : T(0);
                 }
-                T xv[TILE_U];
+                T xv[U][NR];
 #pragma unroll
-                for (int u = 0; u < TILE_U; u++)
-                    xv[u] = (k + (int64_t)u * BT_BS < k1) ? xp[pk[u] >> row_bits] : T(0);
+                for (int u = 0; u < U; u++) {
+                    if (k + (int64_t)u * BT_BS < k1) tile_xrow<T, NR>(xp, pk[u] >> row_bits, xv[u]);
+                    else {
 #pragma unroll
-                for (int u = 0; u < TILE_U; u++)
-                    if (k + (int64_t)u * BT_BS < k1) atomicAdd(&acc[pk[u] & rmask], (double)(v[u] * xv[u]));
+                        for (int r = 0; r < NR; r++) xv[u][r] = T(0);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (k + (int64_t)u * BT_BS < k1) {
+                        const uint32_t row = pk[u] & rmask;
+#pragma unroll
+                        for (int r = 0; r < NR; r++) atomicAdd(&acc[r * stride + row], (double)(v[u] * xv[u][r]));
+                    }
             }
         }
         __syncthreads();
-        SABLE_GLOBAL T* y = (SABLE_GLOBAL T*)it.y;
+        SABLE_GLOBAL T* y = (SABLE_GLOBAL T*)it.y + r0 * NR;
         const T beta = (T)it.beta;
-        for (int r = threadIdx.x; r < nrows; r += BT_BS) {
-            const int64_t i = r0 + r;
-            const T a = (T)acc[r];
-            if (beta == T(0)) y[i] = a;
-            else if (beta == T(1)) { if (a != T(0)) y[i] = y[i] + a; }
-            else y[i] = beta * y[i] + a;
+        for (int q = threadIdx.x; q < nrows * NR; q += BT_BS) {
+            const int rr = q / NR, r = q - rr * NR;
+            const T a = (T)acc[r * stride + rr];
+            if (beta == T(0)) y[q] = a;
+            else if (beta == T(1)) { if (a != T(0)) y[q] = y[q] + a; }
+            else y[q] = beta * y[q] + a;
         }
         __syncthreads();  // acc, s_g and s_k are rewritten for the next unit
     }
@@ -509,15 +547,18 @@ SABLE_API int sable_csr_tiles_batch_item(const sable_csr_tiles_t* p, const void*
     return SABLE_OK;
 }
 
-SABLE_API int sable_batch_spmv_tiles(sable_ctx_t* ctx, sable_dtype_t dtype, const sable_batch_tiles_item_t* d_items,
-                                     int nitems, int64_t total_units, int32_t max_tile_rows, int workgroups) {
+SABLE_API int sable_batch_spmv_tiles(sable_ctx_t* ctx, sable_dtype_t dtype, int nrhs,
+                                     const sable_batch_tiles_item_t* d_items, int nitems, int64_t total_units,
+                                     int32_t max_tile_rows, int workgroups) {
     SABLE_CHECK(ctx && (d_items || nitems == 0), SABLE_EINVAL, "NULL argument");
     SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    SABLE_CHECK(nrhs == 1 || nrhs == 2 || nrhs == 4 || nrhs == 8, SABLE_EINVAL, "nrhs must be 1, 2, 4 or 8");
     SABLE_CHECK(nitems >= 0 && total_units >= 0 && workgroups >= 0, SABLE_EINVAL, "negative size");
     if (nitems == 0 || total_units == 0) return SABLE_OK;
-    const size_t vs = sizeof(double);  // row sums are held in FP64 for either dtype
+    // row sums are held in FP64 for either dtype, nrhs per row
+    const size_t vs = sizeof(double) * (size_t)nrhs;
     SABLE_CHECK(max_tile_rows > 0 && (size_t)max_tile_rows * vs <= 160 * 1024, SABLE_EINVAL,
-                "max_tile_rows must be positive and its row sums must fit the LDS (160 KiB)");
+                "max_tile_rows * nrhs FP64 row sums must fit the LDS (160 KiB)");
     void* s;
     int rc = ctx_scratch(ctx, 256, &s);
     if (rc) return rc;
@@ -525,27 +566,37 @@ SABLE_API int sable_batch_spmv_tiles(sable_ctx_t* ctx, sable_dtype_t dtype, cons
     int* err = (int*)((char*)s + 64);
     hipStream_t st = ctx->stream;
     const size_t shm = (size_t)max_tile_rows * vs;
-    if (shm > 65536 - 256) {  // with the kernel's static LDS, above the default limit
-        const hipError_t ea =
-            dtype == SABLE_F64
-                ? hipFuncSetAttribute((const void*)k_batch_tiles<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)shm)
-                : hipFuncSetAttribute((const void*)k_batch_tiles<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)shm);
-        if (ea != hipSuccess) {
-            (void)hipGetLastError();
-            return fail(SABLE_EINVAL, "tile of %zu LDS bytes not supported: %s", shm, hipGetErrorString(ea));
-        }
-    }
     int64_t grid = workgroups > 0 ? workgroups : (int64_t)ctx->num_cu * 4;
     if (grid > total_units) grid = total_units;
     hipLaunchKernelGGL(k_batch_tiles_reset, dim3(1), dim3(1), 0, st, counter, err);
-    if (dtype == SABLE_F64)
-        hipLaunchKernelGGL(k_batch_tiles<double>, dim3((unsigned)grid), dim3(BT_BS), shm, st, d_items, nitems,
-                           total_units, max_tile_rows, counter, err);
-    else
-        hipLaunchKernelGGL(k_batch_tiles<float>, dim3((unsigned)grid), dim3(BT_BS), shm, st, d_items, nitems,
-                           total_units, max_tile_rows, counter, err);
+    hipError_t ea = hipSuccess;
+#define BT_LAUNCH(T, NR)                                                                                       \
+    do {                                                                                                        \
+        if (shm > 65536 - 256) /* with the kernel's static LDS, above the default limit */                     \
+            ea = hipFuncSetAttribute((const void*)k_batch_tiles<T, NR>,                                         \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);                     \
+        if (ea == hipSuccess)                                                                                   \
+            hipLaunchKernelGGL((k_batch_tiles<T, NR>), dim3((unsigned)grid), dim3(BT_BS), shm, st, d_items,     \
+                               nitems, total_units, max_tile_rows, counter, err);                               \
+    } while (0)
+#define BT_SWITCH(T)                     \
+    switch (nrhs) {                      \
+        case 1: BT_LAUNCH(T, 1); break;  \
+        case 2: BT_LAUNCH(T, 2); break;  \
+        case 4: BT_LAUNCH(T, 4); break;  \
+        default: BT_LAUNCH(T, 8); break; \
+    }
+    if (dtype == SABLE_F64) {
+        BT_SWITCH(double)
+    } else {
+        BT_SWITCH(float)
+    }
+#undef BT_SWITCH
+#undef BT_LAUNCH
+    if (ea != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SABLE_EINVAL, "tile of %zu LDS bytes not supported: %s", shm, hipGetErrorString(ea));
+    }
     SABLE_LAUNCH_CHECK();
     SABLE_HIP(hipMemcpyAsync(ctx->pinned, err, sizeof(int), hipMemcpyDeviceToHost, st));
     SABLE_HIP(hipStreamSynchronize(st));

@@ -59,12 +59,17 @@ class BatchTilesItem(ctypes.Structure):
                 ("npanels", c_i32), ("panel_cols", c_i32), ("row_bits", c_i32)]
 
 
-def batch_tile_plans(mats, tile_rows=4096):
+def batch_tile_plans(mats, tile_rows=0, nrhs=1):
     """One column-sorted tile plan per matrix for batch_spmv_tiles: row tiles
     of up to `tile_rows` rows and a single column panel (the next power of
     two >= N), so that each row tile gathers its x in ascending column
-    order.  Matrices with N = 0 get None."""
+    order.  tile_rows * nrhs of the later products must not exceed 20480;
+    tile_rows = 0 picks min(4096, 8192 / nrhs), the fastest measured (64 KiB
+    of FP64 row sums or less, so two or more workgroups share a CU).
+    Matrices with N = 0 get None."""
     from .spmv import CsrTiles
+    if not tile_rows:
+        tile_rows = min(4096, 8192 // int(nrhs))
     plans = []
     for A in mats:
         if A.N == 0:
@@ -99,15 +104,16 @@ def batch_tiles_items(plans, xs, ys, beta=0.0):
     return _upload_items(items, ctx), len(items), total, maxr, dt, ctx
 
 
-def batch_spmv_tiles(plans, xs, ys, beta=0.0, workgroups=0):
+def batch_spmv_tiles(plans, xs, ys, nrhs=1, beta=0.0, workgroups=0):
     """ys[k] = A_k.xs[k] (+ beta*ys[k]) for every tile plan (CsrTiles) in one
-    persistent launch, one right-hand side.  None plans (N = 0) are
-    skipped."""
+    persistent launch; x and y are DeviceArrays of N*nrhs elements,
+    row-major [N][nrhs] (nrhs in 1, 2, 4, 8; the plans' tile_rows * nrhs
+    must not exceed 20480).  None plans (N = 0) are skipped."""
     b = batch_tiles_items(plans, xs, ys, beta)
     if b is None:
         return
     d, n, total, maxr, dt, ctx = b
-    call("sable_batch_spmv_tiles", ctx.handle, dtype_code(dt), d.ptr, n, total, maxr, int(workgroups))
+    call("sable_batch_spmv_tiles", ctx.handle, dtype_code(dt), int(nrhs), d.ptr, n, total, maxr, int(workgroups))
 
 
 def batch_spts(mats, bs, xs, lanes=4):

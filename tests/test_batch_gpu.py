@@ -144,6 +144,36 @@ def test_batch_spmv_tiles(ctx, dtype, beta):
         assert np.all(d <= tol), (k, n, float(np.max(d - tol)))
 
 
+@pytest.mark.parametrize("nrhs,beta", [(2, 0.0), (4, 0.5), (8, 0.0), (8, 1.0)])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batch_spmv_tiles_multi_rhs(ctx, dtype, nrhs, beta):
+    """nrhs right-hand sides, x and y row-major [N][nrhs]: every column
+    within the CSR bound of its own product (plus one rounding of
+    beta*y0 + sum), with the default tiles and, for 4, the largest that
+    the LDS holds (4096 rows, 128 KiB of row sums)."""
+    rng = np.random.default_rng(7 + nrhs)
+    case = [c for c in tiles_case(dtype) if c[0] > 0]
+    As = [sp.csr_from_host(n, rp, c, v) for n, rp, c, v in case]
+    plans = sp.batch_tile_plans(As, tile_rows=4096 if nrhs == 4 else 0, nrhs=nrhs)
+    X = [rng.uniform(-1, 1, (n, nrhs)).astype(dtype) for n, _, _, _ in case]
+    Y0 = [rng.uniform(-1, 1, (n, nrhs)).astype(dtype) for n, _, _, _ in case]
+    xs = [DeviceArray.from_host(x.ravel()) for x in X]
+    ys = [DeviceArray.from_host(y.ravel()) for y in Y0]
+    sp.batch_spmv_tiles(plans, xs, ys, nrhs=nrhs, beta=beta)
+    ctx.sync()
+    u = np.finfo(dtype).eps / 2
+    for k, (n, rp, c, v) in enumerate(case):
+        Y = ys[k].numpy().reshape(n, nrhs).astype(np.float64)
+        for r in range(nrhs):
+            x = X[k][:, r].copy()
+            yref = oracle.spmv_csr(n, rp, c, v, x, np.zeros(n, dtype)).astype(np.float64)
+            b = bound(rp, c, v, x, dtype)
+            by0 = beta * Y0[k][:, r].astype(np.float64)
+            tol = b + 2 * u * (np.abs(by0) + np.abs(yref) + b)
+            d = np.abs(Y[:, r] - (by0 + yref))
+            assert np.all(d <= tol), (k, n, r)
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_batch_spmv_tiles_many_panels(ctx, dtype):
     """Plans with small tiles and several panels (the general tile layout)
@@ -181,7 +211,11 @@ def test_batch_spmv_tiles_many_panels(ctx, dtype):
         total += it.ntr
     d = _upload_items(items, ctx)
     with pytest.raises(SableError) as e:
-        _lib.call("sable_batch_spmv_tiles", ctx.handle, dtype_code(dtype), d.ptr, len(items), total, 512, 0)
+        _lib.call("sable_batch_spmv_tiles", ctx.handle, dtype_code(dtype), 1, d.ptr, len(items), total, 512, 0)
+    assert e.value.name == "SABLE_EINVAL"
+    # 8 right-hand sides of 4096-row tiles need 256 KiB of row sums: refused before any launch
+    with pytest.raises(SableError) as e:
+        _lib.call("sable_batch_spmv_tiles", ctx.handle, dtype_code(dtype), 8, d.ptr, len(items), total, 4096, 0)
     assert e.value.name == "SABLE_EINVAL"
 
 
