@@ -30,7 +30,9 @@ Also measured and reported under "extra" (each section bounded to seconds):
           FP64 refinement, residuals
   batch   2,048 matrices in one persistent launch vs one launch per matrix
   serve   open-loop Poisson serving, plain FP64 one launch per request vs
-          batched FP32 + FP64 refinement: solves/s, p50/p99 latency
+          batched FP64 and batched FP32 + FP64 refinement (also with the
+          pool's SpMV as batched tile plans, "mixed+tiles"): solves/s,
+          p50/p99 latency
 The CPU baseline is the serial C restatement of the reference kernels on
 this host (the reference's own Wasm needs a WAT assembler and Node >= 16.4,
 which this image does not have).
@@ -687,9 +689,13 @@ def sec_serve(args, sp, ctx, pool, lows, big):
     nsmall = min(512, len(ns))
     modes = [m for m in args.serve_modes.split(",") if m]
     for mode in modes:
-        # refinement: one FP64 residual + update step after the FP32 solve
-        S = sp.Server(mode=mode, arena_bytes=args.serve_arena_mb << 20, max_batch=args.serve_max_batch, slots=3,
-                      refine_iters=args.refine_iters, spts_lanes=4, group_rhs=args.group_rhs, ctx=ctx)
+        # "<mode>+tiles": the batched tile schedule for the pool's SpMV
+        # (rows summed in LDS-atomic order); refinement: one FP64 residual +
+        # update step after the FP32 solve
+        base, _, opt = mode.partition("+")
+        S = sp.Server(mode=base, arena_bytes=args.serve_arena_mb << 20, max_batch=args.serve_max_batch, slots=3,
+                      refine_iters=args.refine_iters, spts_lanes=4, group_rhs=args.group_rhs,
+                      batch_tiles=opt == "tiles", ctx=ctx)
         t0 = time.time()
         for k in range(nsmall):
             a, b = rs[k], rs[k + 1]
@@ -722,6 +728,8 @@ def sec_serve(args, sp, ctx, pool, lows, big):
     out["max_solves_per_s"] = best
     if "mixed" in best and "plain" in best:
         out["speedup_mixed_vs_plain"] = r4(best["mixed"] / max(best["plain"], 1e-9))
+    if "mixed+tiles" in best and "plain" in best:
+        out["speedup_mixed_tiles_vs_plain"] = r4(best["mixed+tiles"] / max(best["plain"], 1e-9))
     out["config"] = {"max_batch": args.serve_max_batch, "arena_mb_per_slot": args.serve_arena_mb, "slots": 3,
                      "refine_iters": args.refine_iters, "group_rhs": args.group_rhs}
     return out
@@ -760,7 +768,7 @@ def main():
     ap.add_argument("--serve-rates", type=str, default="2000,10000,30000,60000,120000,240000")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-large", type=int, default=8)
-    ap.add_argument("--serve-modes", type=str, default="plain,batched,mixed")
+    ap.add_argument("--serve-modes", type=str, default="plain,batched,mixed,mixed+tiles")
     ap.add_argument("--serve-max-batch", type=int, default=4096)
     ap.add_argument("--serve-arena-mb", type=int, default=2048)
     ap.add_argument("--refine-iters", type=int, default=1)

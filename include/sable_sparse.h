@@ -664,7 +664,12 @@ typedef struct {
     int spts_lanes;       /* lanes per row of the solves (0 = 4) */
     int group_rhs;        /* modes 1 and 2: SpMV requests on one matrix in a batch share
                              one multi-RHS product, up to this many (0 = 8, 1 = off) */
-    int pad;
+    int batch_tiles;      /* modes 1 and 2, matrices below 2^20 rows: 0 = row-block batch
+                             (each row summed in a fixed order, so a result is the same
+                             bits alone and in any mix); 1 = column-sorted tile plans in
+                             the mode's precision, one batched tile launch per width
+                             (faster; rows summed in LDS-atomic order, so results agree
+                             within the CSR bound) */
 } sable_server_config_t;
 
 SABLE_API int sable_server_create(sable_ctx_t* ctx, const sable_server_config_t* cfg, sable_server_t** out);

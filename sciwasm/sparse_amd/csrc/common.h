@@ -60,6 +60,9 @@ int plan_lds_cap(int T);
 int plan_fill_stream(hipStream_t st, int32_t N, const int32_t* rp, int T, int C, int64_t B, int32_t* blk, int* err);
 // y = A.x + beta*y with a column-sorted tile plan, on stream st.
 int tiles_launch(hipStream_t st, const sable_csr_tiles_t* p, const void* x, void* y, double beta);
+int batch_tiles_launch(hipStream_t st, int num_cu, sable_dtype_t dtype, int nrhs, const sable_batch_tiles_item_t* d_items,
+                       int nitems, int64_t total_units, int32_t max_tile_rows, unsigned long long* counter, int* err,
+                       int workgroups);
 int syncfree_launch(hipStream_t st, int num_cu, sable_dtype_t dt, int lanes, int32_t N, const int32_t* rp,
                     const int32_t* col, const void* val, const void* b, void* x, unsigned* ticket, int* err,
                     long long spin);

@@ -67,6 +67,9 @@ struct SMatrix {
     // their products, which run one request per launch
     sable_csr_tiles_t* tiles64 = nullptr;
     sable_csr_tiles_t* tiles32 = nullptr;
+    // smaller general matrices in the batched and mixed modes: a column-
+    // sorted tile plan in the mode's precision for the batched tile launch
+    sable_csr_tiles_t* btiles = nullptr;
 };
 
 // Matrices at least this large get tile plans: their x is far larger than
@@ -400,11 +403,23 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
     SABLE_CHECK(fills.size() * sizeof(FillItem) <= ITEM_BYTES, SABLE_EINVAL, "too many requests in a batch");
     memcpy(hi + fill_off, fills.data(), fills.size() * sizeof(FillItem));
     std::vector<sable_batch_spmv_item_t> sv[4];  // widths 1, 2, 4, 8
-    int64_t blk_total[4] = {0, 0, 0, 0};
+    std::vector<sable_batch_tiles_item_t> bt[4];  // the same, matrices with a batch tile plan
+    int64_t blk_total[4] = {0, 0, 0, 0}, unit_total[4] = {0, 0, 0, 0};
+    int32_t bt_rows[4] = {0, 0, 0, 0};
     for (auto& g : groups) {
         const SMatrix& m = mats[g.mat];
         if (m.tiles64) continue;  // launched on its own below
         const int wi = g.w == 1 ? 0 : g.w == 2 ? 1 : g.w == 4 ? 2 : 3;
+        if (m.btiles) {
+            sable_batch_tiles_item_t ti;
+            const int trc = sable_csr_tiles_batch_item(m.btiles, s.arena + g.x_off, s.arena + g.y_off, 0.0,
+                                                       unit_total[wi], &ti);
+            if (trc) return trc;
+            unit_total[wi] += ti.ntr;
+            bt_rows[wi] = std::max(bt_rows[wi], ti.tile_rows);
+            bt[wi].push_back(ti);
+            continue;
+        }
         sable_batch_spmv_item_t it;
         it.row_ptr = m.rp;
         it.col = m.col;
@@ -452,17 +467,24 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
             rv.push_back(it);
         }
     }
-    size_t sbytes[4], sofs[4], sb = 0;
+    size_t sbytes[4], sofs[4], tbytes[4], tofs[4], sb = 0;
     for (int wi = 0; wi < 4; wi++) {
         sofs[wi] = spmv_off + sb;
         sbytes[wi] = sv[wi].size() * sizeof(sable_batch_spmv_item_t);
         sb += sbytes[wi];
     }
+    for (int wi = 0; wi < 4; wi++) {
+        tofs[wi] = spmv_off + sb;
+        tbytes[wi] = bt[wi].size() * sizeof(sable_batch_tiles_item_t);
+        sb += tbytes[wi];
+    }
     SABLE_CHECK(sb <= ITEM_BYTES && tv.size() * sizeof(sable_batch_spts_item_t) <= ITEM_BYTES &&
                     rv.size() * sizeof(sable_batch_refine_item_t) <= ITEM_BYTES,
                 SABLE_EINVAL, "too many requests in a batch");
-    for (int wi = 0; wi < 4; wi++)
+    for (int wi = 0; wi < 4; wi++) {
         if (sbytes[wi]) memcpy(hi + sofs[wi], sv[wi].data(), sbytes[wi]);
+        if (tbytes[wi]) memcpy(hi + tofs[wi], bt[wi].data(), tbytes[wi]);
+    }
     if (mixed) memcpy(hi + spts_off, rv.data(), rv.size() * sizeof(rv[0]));
     else memcpy(hi + spts_off, tv.data(), tv.size() * sizeof(tv[0]));
     const size_t fb = fills.size() * sizeof(FillItem);
@@ -548,6 +570,14 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
                 rc = batch_spmv_launch(st, ctx->num_cu, mixed ? SABLE_F32 : SABLE_F64, widths[wi],
                                        (const sable_batch_spmv_item_t*)(s.d_items + sofs[wi]), (int)sv[wi].size(),
                                        blk_total[wi], s.d_counter, 0);
+        // the items come from the matrices' own plans, so no tile exceeds
+        // bt_rows; the launch's flag word is the batch's error word, which
+        // the solves below set only on failure
+        for (int wi = 0; wi < 4 && !rc; wi++)
+            if (!bt[wi].empty())
+                rc = batch_tiles_launch(st, ctx->num_cu, mixed ? SABLE_F32 : SABLE_F64, widths[wi],
+                                        (const sable_batch_tiles_item_t*)(s.d_items + tofs[wi]), (int)bt[wi].size(),
+                                        unit_total[wi], bt_rows[wi], s.d_counter, s.d_err, 0);
         if (!rc && !tv.empty())
             rc = batch_level_launch(st, SABLE_F64, lanes, (const sable_batch_spts_item_t*)(s.d_items + spts_off),
                                     d_lch, plan);
@@ -622,6 +652,7 @@ static void free_matrix(SMatrix& m) {
     if (m.plan) sable_csr_plan_destroy(m.plan);
     if (m.tiles64) sable_csr_tiles_destroy(m.tiles64);
     if (m.tiles32) sable_csr_tiles_destroy(m.tiles32);
+    if (m.btiles) sable_csr_tiles_destroy(m.btiles);
     void* ps[] = {m.rp, m.col, m.val64, m.val32, m.perm};
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -796,6 +827,27 @@ SABLE_API int sable_server_add_matrix(sable_server_t* s, int32_t N, const int32_
                 else {
                     s->used += (int64_t)extra;
                     m.bytes += extra;
+                }
+            }
+        }
+        if (!rc && kind == 0 && N > 0 && N < TILE_MIN_ROWS && s->cfg.mode != 0 && s->cfg.batch_tiles) {
+            // rows per tile: min(4096, 8192 / widest group), so that a
+            // group's FP64 row sums take at most 64 KiB of LDS; one panel
+            const int32_t tr = std::min(std::min(4096, 8192 / s->group_max()) / 64 * 64, (N + 63) / 64 * 64);
+            int32_t pc = 64;
+            while (pc < N) pc *= 2;
+            const bool mx = s->cfg.mode == 2;
+            rc = sable_csr_tiles_create(s->ctx, mx ? SABLE_F32 : SABLE_F64, N, m.rp, m.col,
+                                        mx ? (const void*)m.val32 : (const void*)m.val64, tr, pc, &m.btiles);
+            if (!rc) {
+                int64_t bb = 0;
+                sable_csr_tiles_info(m.btiles, nullptr, nullptr, &bb);
+                std::lock_guard<std::mutex> lk(s->mu);
+                if (s->cfg.hbm_budget > 0 && s->used + bb > s->cfg.hbm_budget)
+                    rc = fail(SABLE_EBUDGET, "batch tile plan of %lld bytes exceeds the HBM budget", (long long)bb);
+                else {
+                    s->used += bb;
+                    m.bytes += (size_t)bb;
                 }
             }
         }

@@ -349,6 +349,48 @@ __global__ void k_batch_tiles_reset(unsigned long long* counter, int* err) {
     *err = 0;
 }
 
+// Launch only (no wait, no check of the items): counter and err are reset
+// on the stream first.  max_tile_rows * nrhs * 8 must fit the LDS.
+int batch_tiles_launch(hipStream_t st, int num_cu, sable_dtype_t dtype, int nrhs, const sable_batch_tiles_item_t* d_items,
+                       int nitems, int64_t total_units, int32_t max_tile_rows, unsigned long long* counter, int* err,
+                       int workgroups) {
+    if (nitems <= 0 || total_units <= 0) return SABLE_OK;
+    const size_t shm = (size_t)max_tile_rows * sizeof(double) * (size_t)nrhs;
+    int64_t grid = workgroups > 0 ? workgroups : (int64_t)num_cu * 4;
+    if (grid > total_units) grid = total_units;
+    hipLaunchKernelGGL(k_batch_tiles_reset, dim3(1), dim3(1), 0, st, counter, err);
+    hipError_t ea = hipSuccess;
+#define BT_LAUNCH(T, NR)                                                                                       \
+    do {                                                                                                        \
+        if (shm > 65536 - 256) /* with the kernel's static LDS, above the default limit */                     \
+            ea = hipFuncSetAttribute((const void*)k_batch_tiles<T, NR>,                                         \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);                     \
+        if (ea == hipSuccess)                                                                                   \
+            hipLaunchKernelGGL((k_batch_tiles<T, NR>), dim3((unsigned)grid), dim3(BT_BS), shm, st, d_items,     \
+                               nitems, total_units, max_tile_rows, counter, err);                               \
+    } while (0)
+#define BT_SWITCH(T)                     \
+    switch (nrhs) {                      \
+        case 1: BT_LAUNCH(T, 1); break;  \
+        case 2: BT_LAUNCH(T, 2); break;  \
+        case 4: BT_LAUNCH(T, 4); break;  \
+        default: BT_LAUNCH(T, 8); break; \
+    }
+    if (dtype == SABLE_F64) {
+        BT_SWITCH(double)
+    } else {
+        BT_SWITCH(float)
+    }
+#undef BT_SWITCH
+#undef BT_LAUNCH
+    if (ea != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SABLE_EINVAL, "tile of %zu LDS bytes not supported: %s", shm, hipGetErrorString(ea));
+    }
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
 // 0: one launch per panel; 1: one launch over all panels (process-wide).
 int g_tiles_mode = 1;
 
@@ -565,39 +607,9 @@ SABLE_API int sable_batch_spmv_tiles(sable_ctx_t* ctx, sable_dtype_t dtype, int 
     unsigned long long* counter = (unsigned long long*)s;
     int* err = (int*)((char*)s + 64);
     hipStream_t st = ctx->stream;
-    const size_t shm = (size_t)max_tile_rows * vs;
-    int64_t grid = workgroups > 0 ? workgroups : (int64_t)ctx->num_cu * 4;
-    if (grid > total_units) grid = total_units;
-    hipLaunchKernelGGL(k_batch_tiles_reset, dim3(1), dim3(1), 0, st, counter, err);
-    hipError_t ea = hipSuccess;
-#define BT_LAUNCH(T, NR)                                                                                       \
-    do {                                                                                                        \
-        if (shm > 65536 - 256) /* with the kernel's static LDS, above the default limit */                     \
-            ea = hipFuncSetAttribute((const void*)k_batch_tiles<T, NR>,                                         \
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);                     \
-        if (ea == hipSuccess)                                                                                   \
-            hipLaunchKernelGGL((k_batch_tiles<T, NR>), dim3((unsigned)grid), dim3(BT_BS), shm, st, d_items,     \
-                               nitems, total_units, max_tile_rows, counter, err);                               \
-    } while (0)
-#define BT_SWITCH(T)                     \
-    switch (nrhs) {                      \
-        case 1: BT_LAUNCH(T, 1); break;  \
-        case 2: BT_LAUNCH(T, 2); break;  \
-        case 4: BT_LAUNCH(T, 4); break;  \
-        default: BT_LAUNCH(T, 8); break; \
-    }
-    if (dtype == SABLE_F64) {
-        BT_SWITCH(double)
-    } else {
-        BT_SWITCH(float)
-    }
-#undef BT_SWITCH
-#undef BT_LAUNCH
-    if (ea != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(SABLE_EINVAL, "tile of %zu LDS bytes not supported: %s", shm, hipGetErrorString(ea));
-    }
-    SABLE_LAUNCH_CHECK();
+    rc = batch_tiles_launch(st, ctx->num_cu, dtype, nrhs, d_items, nitems, total_units, max_tile_rows, counter, err,
+                            workgroups);
+    if (rc) return rc;
     SABLE_HIP(hipMemcpyAsync(ctx->pinned, err, sizeof(int), hipMemcpyDeviceToHost, st));
     SABLE_HIP(hipStreamSynchronize(st));
     SABLE_CHECK(*(int*)ctx->pinned == 0, SABLE_EINVAL, "an item's tile_rows exceeds max_tile_rows");

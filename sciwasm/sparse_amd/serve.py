@@ -149,10 +149,10 @@ class Server:
     """The native request server (sable_server_*)."""
 
     def __init__(self, mode="mixed", hbm_budget=0, max_batch=256, slots=3, arena_bytes=256 << 20, refine_iters=2,
-                 spts_lanes=4, group_rhs=8, ctx=None):
+                 spts_lanes=4, group_rhs=8, batch_tiles=False, ctx=None):
         self.ctx = ctx if ctx is not None else Context.default()
         cfg = _lib.ServerConfig(int(hbm_budget), MODES[mode], int(max_batch), int(slots), int(arena_bytes),
-                                int(refine_iters), int(spts_lanes), int(group_rhs), 0)
+                                int(refine_iters), int(spts_lanes), int(group_rhs), int(bool(batch_tiles)))
         h = c_vp()
         call("sable_server_create", self.ctx.handle, ctypes.byref(cfg), ctypes.byref(h))
         self.handle = h

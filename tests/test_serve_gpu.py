@@ -87,6 +87,47 @@ def test_alone_equals_mix_and_accuracy(ctx, pool, mode, kw):
     S.close()
 
 
+@pytest.mark.parametrize("mode", ["batched", "mixed"])
+def test_batch_tiles_alone_and_in_mix(ctx, pool, mode):
+    """batch_tiles: the pool's SpMV runs as column-sorted tile plans in one
+    batched launch per width, rows summed in LDS-atomic order.  Alone and in
+    a mix (multi-RHS groups included) each SpMV result meets the CSR bound
+    against the FP64 oracle, so the two agree within twice that bound; the
+    solves are untouched and stay bit-identical."""
+    mats, lows = pool
+    S, gids, lids = make_server(mode, pool, batch_tiles=True)
+    rng = np.random.default_rng(15)
+    probes = []
+    for k in range(6):
+        n, rp, c, v = mats[k]
+        probes.append(("spmv", gids[k], rng.uniform(-1, 1, n), (n, rp, c, v)))
+        n, rp, c, v = lows[k]
+        probes.append(("spts", lids[k], rng.uniform(-1, 1, n), (n, rp, c, v)))
+    alone = [S.wait(S.submit(mid, op, vec))[0].copy() for op, mid, vec, _ in probes]
+    ids = []
+    for j in range(120):
+        k = rng.integers(len(gids))
+        ids.append(S.submit(gids[k], "spmv", rng.uniform(-1, 1, mats[k][0])))
+    probe_ids = [S.submit(mid, op, vec) for op, mid, vec, _ in probes]
+    for k in range(6):
+        ids.append(S.submit(gids[k], "spmv", rng.uniform(-1, 1, mats[k][0])))
+    dt = np.float32 if mode == "mixed" else np.float64
+    for p, (op, mid, vec, (n, rp, c, v)) in enumerate(probes):
+        out, _ = S.wait(probe_ids[p])
+        if op == "spmv":
+            yref, _ = oracle.spmv_csr_fp64(n, rp, c, v, vec)
+            b = spmv_bound(rp, c, v, vec, dt)
+            assert np.all(np.abs(out - yref) <= b), (mode, p)
+            assert np.all(np.abs(alone[p] - yref) <= b), (mode, p)
+            assert np.all(np.abs(out - alone[p]) <= 2 * b), (mode, p)
+        else:
+            assert np.array_equal(out, alone[p]), (mode, p)
+            assert oracle.rel_residual(n, rp, c, v, out, vec) <= 1e-12, (mode, p)
+    for i in ids:
+        S.wait(i)
+    S.close()
+
+
 def test_hbm_budget_admission(ctx, pool):
     mats, lows = pool
     # per slot: the 16 MiB arena, 4 MiB of item arrays, 8 MiB of level
