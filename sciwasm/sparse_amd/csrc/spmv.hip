@@ -477,14 +477,20 @@ __global__ __launch_bounds__(BS) void k_ell(int32_t N, int32_t nc, const int32_t
 }
 
 // ------------------------------------------------------------------ COO
-// Flags rows out of order (bit 1) or out of [0, N) (bit 2).
+// Flags rows out of order (bit 1) or out of [0, N) (bit 2).  Each thread
+// collects its flags over a grid-stride loop and each wave ORs them into
+// err once: rows out of order are the normal case of a shuffled COO, and
+// one atomic per entry on one address would serialise the whole check.
 __global__ void k_coo_check(int64_t nnz, int32_t N, const int32_t* __restrict__ row, int* __restrict__ err) {
+    int f = 0;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
          k += (int64_t)gridDim.x * blockDim.x) {
         const int r = row[k];
-        if (r < 0 || r >= N) atomicOr(err, 2);
-        if (k + 1 < nnz && row[k + 1] < r) atomicOr(err, 1);
+        if (r < 0 || r >= N) f |= 2;
+        if (k + 1 < nnz && row[k + 1] < r) f |= 1;
     }
+    const int w = (__ballot(f & 1) ? 1 : 0) | (__ballot(f & 2) ? 2 : 0);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && w) atomicOr(err, w);
 }
 
 // Row heads of a row-sorted COO become a row pointer: thread k in [0, nnz]
@@ -812,7 +818,7 @@ SABLE_API int sable_spmv_coo(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, i
     SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), ctx->stream));
     if (nnz > 0) {
         unsigned g = grid_for(nnz, BS);
-        if (g > 16384) g = 16384;
+        if (g > 2048) g = 2048;
         hipLaunchKernelGGL(k_coo_check, dim3(g), dim3(BS), 0, ctx->stream, nnz, N, row, err);
         SABLE_LAUNCH_CHECK();
     }
@@ -856,7 +862,7 @@ SABLE_API int sable_coo_plan_create(sable_ctx_t* ctx, int32_t N, int64_t nnz, co
     SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), ctx->stream));
     if (nnz > 0) {
         unsigned g = grid_for(nnz, BS);
-        if (g > 16384) g = 16384;
+        if (g > 2048) g = 2048;
         hipLaunchKernelGGL(k_coo_check, dim3(g), dim3(BS), 0, ctx->stream, nnz, N, row, err);
         if (hipGetLastError() != hipSuccess) return done(fail(SABLE_EHIP, "COO check launch failed"));
     }
