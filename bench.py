@@ -633,7 +633,7 @@ def batch_tiles(sp, ctx, T, mats, xs, ys, nrhs, plans_built, bytes_, ms_rowblock
     from sciwasm.sparse_amd.serve import batch_tiles_items
     plans, build_s = plans_built
     ref = [y.numpy()[:A.N * nrhs].astype(np.float64) for A, y in zip(mats[:32], ys[:32])]
-    d, n, total, maxr, dt, _ = batch_tiles_items(plans, xs, ys)
+    d, n, total, maxr, dt, _ = batch_tiles_items(plans, xs, ys, 0.0, nrhs)
     code = 1 if dt == np.float64 else 0
     ms = T(lambda: L.call("sable_batch_spmv_tiles", ctx.handle, code, nrhs, d.ptr, n, total, maxr, 0), 10, 2)
     err = 0.0

@@ -42,6 +42,7 @@ def batch_spmv(mats, xs, ys, nrhs=1, beta=0.0, workgroups=0, plans=None):
     ctx = mats[0].row.ctx
     items, total = [], 0
     for k, (A, x, y) in enumerate(zip(mats, xs, ys)):
+        _check_vectors(A, x, y, nrhs, "batch_spmv")
         blk, nb = c_vp(), c_i64()
         pl = plans[k] if plans is not None else A.plan()
         call("sable_csr_plan_blocks", pl.handle, ctypes.byref(blk), ctypes.byref(nb))
@@ -83,13 +84,26 @@ def batch_tile_plans(mats, tile_rows=0, nrhs=1):
     return plans
 
 
-def batch_tiles_items(plans, xs, ys, beta=0.0):
+def _check_vectors(A, x, y, nrhs, what):
+    """x and y must hold N*nrhs values of A's dtype: the kernels index them
+    as row-major [N][nrhs] without bounds checks."""
+    need = A.N * int(nrhs)
+    for v, name in ((x, "x"), (y, "y")):
+        if v.dtype != np.dtype(A.dtype) or v.n < need:
+            raise ValueError("%s: %s holds %d %s values, the product needs %d %s" %
+                             (what, name, v.n, v.dtype.name, need, np.dtype(A.dtype).name))
+
+
+def batch_tiles_items(plans, xs, ys, beta=0.0, nrhs=1):
     """The device item list of batch_spmv_tiles: (items, nitems, total_units,
     max_tile_rows, dtype, ctx), or None when every plan is None."""
+    if not (len(plans) == len(xs) == len(ys)):
+        raise ValueError("batch_spmv_tiles: plans, xs and ys differ in length")
     items, total, maxr, ctx, dt = [], 0, 0, None, None
     for p, x, y in zip(plans, xs, ys):
         if p is None:
             continue
+        _check_vectors(p.A, x, y, nrhs, "batch_spmv_tiles")
         if dt is None:
             ctx, dt = p.ctx, np.dtype(p.A.dtype)
         elif np.dtype(p.A.dtype) != dt:
@@ -109,7 +123,7 @@ def batch_spmv_tiles(plans, xs, ys, nrhs=1, beta=0.0, workgroups=0):
     persistent launch; x and y are DeviceArrays of N*nrhs elements,
     row-major [N][nrhs] (nrhs in 1, 2, 4, 8; the plans' tile_rows * nrhs
     must not exceed 20480).  None plans (N = 0) are skipped."""
-    b = batch_tiles_items(plans, xs, ys, beta)
+    b = batch_tiles_items(plans, xs, ys, beta, nrhs)
     if b is None:
         return
     d, n, total, maxr, dt, ctx = b
