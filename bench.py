@@ -316,6 +316,15 @@ def sec_band(args, sp, ctx, T):
         out["csr_vec%dx%d_spmv" % (g, rr)] = {"ms": r4(ms), "gbs": r4(b_csr / (ms * 1e-3) / 1e9),
                                               "pct_hbm": r4(100 * b_csr / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)}
         del pl
+    # CSR through column-sorted tiles (16384-row tiles over 8192-column panels)
+    yc = yv.array()
+    tl = sp.CsrTiles(A)
+    ms = T(lambda: tl.run(xv.vec, yv.vec, beta=0.0), 20, 3)
+    out["csr_tiles_spmv"] = {"ms": r4(ms), "gbs": r4(b_csr / (ms * 1e-3) / 1e9),
+                             "pct_hbm": r4(100 * b_csr / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS),
+                             "max_rel_diff_vs_csr": float(np.max(np.abs(yv.array() - yc)) /
+                                                          max(float(np.max(np.abs(yc))), 1e-300))}
+    del tl
     t_tr, Tt = T.once(lambda: sp.transpose(A))
     out["transpose_csr_ms"] = r4(t_tr)
     del Tt, E
