@@ -228,6 +228,7 @@ __global__ __launch_bounds__(TILE_BS) void k_tiles_fused(int32_t N, int32_t tile
 // share the LDS counter, so every wait for a gather would also wait for the
 // LDS atomics in flight.
 constexpr int BT_BS = 256;
+constexpr int BT_XCDS = 8;  // claim ranges (MI355X has 8 XCDs); counters used: BT_XCDS
 #define SABLE_GLOBAL __attribute__((address_space(1)))
 
 // The NR values of row c of a row-major [N][NR] x: 16-byte loads when a row
@@ -263,9 +264,28 @@ __global__ __launch_bounds__(BT_BS) void k_batch_tiles(const sable_batch_tiles_i
     double* acc = reinterpret_cast<double*>(smem_raw);
     __shared__ long long s_g;
     __shared__ int s_k;
+    // claim ranges: the units are cut into BT_XCDS contiguous ranges with a
+    // counter each; a workgroup claims from range blockIdx % BT_XCDS (the XCD
+    // it is dispatched to) and, once that range is used up, from the others
+    // in turn.  Consecutive units mostly belong to one matrix, so its x is
+    // fetched into few XCDs' L2s.  Every unit is handed out exactly once
+    // whatever the mapping; the mapping only decides whose L2 it warms.
+    int exhausted = 0;  // ranges this workgroup found used up (thread 0)
+    const int q0 = (int)(blockIdx.x % BT_XCDS);
     while (true) {
         if (threadIdx.x == 0) {
-            const long long g = (long long)atomicAdd(counter, 1ull);
+            long long g = total;
+            for (int i = 0; i < BT_XCDS; i++) {
+                const int q = (q0 + i) % BT_XCDS;
+                if (exhausted & (1 << q)) continue;
+                const long long e = total * (q + 1) / BT_XCDS;
+                const long long c = total * q / BT_XCDS + (long long)atomicAdd(&counter[q], 1ull);
+                if (c < e) {
+                    g = c;
+                    break;
+                }
+                exhausted |= 1 << q;
+            }
             int lo = 0;
             if (g < total) {
                 int hi = nitems - 1;
@@ -345,12 +365,13 @@ __global__ __launch_bounds__(BT_BS) void k_batch_tiles(const sable_batch_tiles_i
 #undef SABLE_GLOBAL
 
 __global__ void k_batch_tiles_reset(unsigned long long* counter, int* err) {
-    *counter = 0ull;
+    for (int q = 0; q < BT_XCDS; q++) counter[q] = 0ull;
     *err = 0;
 }
 
-// Launch only (no wait, no check of the items): counter and err are reset
-// on the stream first.  max_tile_rows * nrhs * 8 must fit the LDS.
+// Launch only (no wait, no check of the items): the BT_XCDS counters at
+// `counter` and err are reset on the stream first.  max_tile_rows * nrhs * 8
+// must fit the LDS.
 int batch_tiles_launch(hipStream_t st, int num_cu, sable_dtype_t dtype, int nrhs, const sable_batch_tiles_item_t* d_items,
                        int nitems, int64_t total_units, int32_t max_tile_rows, unsigned long long* counter, int* err,
                        int workgroups) {
@@ -604,8 +625,8 @@ SABLE_API int sable_batch_spmv_tiles(sable_ctx_t* ctx, sable_dtype_t dtype, int 
     void* s;
     int rc = ctx_scratch(ctx, 256, &s);
     if (rc) return rc;
-    unsigned long long* counter = (unsigned long long*)s;
-    int* err = (int*)((char*)s + 64);
+    unsigned long long* counter = (unsigned long long*)s;  // BT_XCDS counters
+    int* err = (int*)((char*)s + 128);
     hipStream_t st = ctx->stream;
     rc = batch_tiles_launch(st, ctx->num_cu, dtype, nrhs, d_items, nitems, total_units, max_tile_rows, counter, err,
                             workgroups);
