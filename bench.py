@@ -29,6 +29,8 @@ Also measured and reported under "extra" (each section bounded to seconds):
           lower matrix: analysis, level-set (hipGraph), sync-free, FP32 +
           FP64 refinement, residuals
   batch   2,048 matrices in one persistent launch vs one launch per matrix
+  batch_tiles  (not in the default list) only the FP64 batched tile product
+          of the same pool, three launches, for hardware-counter runs
   serve   open-loop Poisson serving, plain FP64 one launch per request vs
           batched FP64 and batched FP32 + FP64 refinement (also with the
           pool's SpMV as batched tile plans, "mixed+tiles"): solves/s,
@@ -654,6 +656,36 @@ def batch_tiles(sp, ctx, T, mats, xs, ys, nrhs, plans_built, bytes_, ms_rowblock
     return out
 
 
+def sec_batch_tiles(args, sp, ctx, pool):
+    """Only the batched tile product of the batch pool, FP64, one right-hand
+    side, three launches (and, with --calib-copy, a 1 GiB device copy first):
+    a short program for hardware-counter runs of k_batch_tiles."""
+    from sciwasm.sparse_amd import _lib as L
+    from sciwasm.sparse_amd.serve import batch_tiles_items
+    ns, rs, rp, col, val = pool
+    mats, xs, ys = [], [], []
+    for k in range(len(ns)):
+        a, b = rs[k], rs[k + 1]
+        lrp = (rp[a:b + 1] - rp[a]).astype(np.int32)
+        mats.append(sp.csr_from_host(int(ns[k]), lrp, col[rp[a]:rp[b]], val[rp[a]:rp[b]], ctx))
+        x = sp.DeviceArray(int(ns[k]), np.float64, ctx)
+        x.fill(0.5)
+        xs.append(x)
+        ys.append(sp.DeviceArray(int(ns[k]), np.float64, ctx))
+    plans = sp.batch_tile_plans(mats)
+    d, n, total, maxr, dt, _ = batch_tiles_items(plans, xs, ys)
+    if args.calib_copy:
+        a = sp.DeviceArray(1 << 27, np.float64, ctx)
+        b = sp.DeviceArray(1 << 27, np.float64, ctx)
+        a.fill(1.0)
+        L.call("sable_memcpy_d2d", ctx.handle, b.ptr, a.ptr, a.nbytes)
+        del a, b
+    for _ in range(3):
+        L.call("sable_batch_spmv_tiles", ctx.handle, 1, 1, d.ptr, n, total, maxr, 0)
+    ctx.sync()
+    return {"matrices": len(mats), "units": total, "launches": 3}
+
+
 def make_lower_pool(pool, count):
     """Diagonally dominant lower triangles of the first `count` matrices."""
     from sciwasm.sparse_amd import lower_dominant
@@ -821,13 +853,15 @@ def main():
     if "spts" in sections:
         run("spts", lambda: sec_spts(args, sp, ctx, T))
     pool = lows = None
-    if "batch" in sections or "serve" in sections:
+    if "batch" in sections or "serve" in sections or "batch_tiles" in sections:
         from sciwasm.sparse_amd import gen
         t0 = time.time()
         pool = gen.batch_csr_concat(args.batch_count, seed=args.seed + 4)
         log("pool generated: %d matrices, %d rows, %d nnz in %.1fs" % (len(pool[0]), pool[1][-1], pool[2][-1],
                                                                       time.time() - t0))
-        lows = make_lower_pool(pool, min(512, args.batch_count))
+        lows = make_lower_pool(pool, min(512, args.batch_count)) if "batch_tiles" not in sections else None
+    if "batch_tiles" in sections:
+        run("batch_tiles", lambda: sec_batch_tiles(args, sp, ctx, pool))
     if "batch" in sections:
         run("batch", lambda: sec_batch(args, sp, ctx, T, pool))
         run("batch_spts", lambda: sec_batch_spts(args, sp, ctx, T, lows))
