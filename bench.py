@@ -14,6 +14,10 @@ tile plan of the same CSR).
 Also measured and reported under "extra" (each section bounded to seconds):
   c1      FP32 5-point Laplacian 256x256 (65,536 rows): CSR/ELL/DIA/COO
           SpMV in microseconds and GB/s, reference checksum protocol
+  addon   the reference's call surface: spmv_csr_wrapper through the
+          Node-API addon (node sciwasm/sparse_amd/node/bench_node.js) on the
+          c1 matrix (inner_max 1 and 500) and on the headline matrix
+          (inner_max 1 and 10), x and y across PCIe on every call
   band    FP64 7-diagonal band, 16,777,216 rows (offsets 0, +-1, +-64,
           +-4096): csr_dia / csr_ell / coo_csr / transpose on the GPU and
           DIA/ELL/CSR SpMV GB/s
@@ -248,6 +252,104 @@ def sec_c1(args, sp, ctx, T):
     t = (time.perf_counter() - t0) / reps
     out["cpu_serial_us"] = r4(t * 1e6)
     log("c1:", out)
+    return out
+
+
+# ------------------------------------------------------------------ addon
+def _addon_run(d, bits, rp, col, val, x, warmup, calls, inner, linear_mb):
+    """Writes the matrix for bench_node.js and runs it in a child Node process
+    (the addon holds its own HIP context there)."""
+    import shutil
+    import subprocess
+    N, nnz = len(rp) - 1, len(col)
+    for name, a in (("rp", rp), ("col", col), ("val", val), ("x", x)):
+        np.ascontiguousarray(a).tofile(os.path.join(d, name + ".bin"))
+    with open(os.path.join(d, "spec.json"), "w") as f:
+        json.dump({"N": N, "nnz": nnz}, f)
+    node = shutil.which("node")
+    if node is None:
+        raise RuntimeError("node is not installed")
+    script = os.path.join(ROOT, "sciwasm", "sparse_amd", "node", "bench_node.js")
+    r = subprocess.run([node, script, d, str(bits), str(warmup), str(calls), ",".join(str(i) for i in inner)],
+                       capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, SABLE_LINEAR_MEMORY_MB=str(linear_mb)))
+    if r.returncode != 0:
+        raise RuntimeError("bench_node.js failed: " + (r.stdout[-2000:] + r.stderr[-2000:]))
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    y = np.fromfile(os.path.join(d, "y.bin"), dtype=val.dtype)
+    return out, y
+
+
+def _row_check(rp, col, val, x, y, R):
+    """y[:R] against the reference-order oracle, within the CSR bound."""
+    import oracle
+    yref = oracle.spmv_csr(R, rp[:R + 1], col[:rp[R]], val[:rp[R]], x, np.zeros(R, val.dtype), 1)
+    lens = np.diff(rp[:R + 1]).astype(np.float64) + 1
+    ab = np.add.reduceat(np.abs(val[:rp[R]].astype(np.float64) * x.astype(np.float64)[col[:rp[R]]]), rp[:R])
+    ab[np.diff(rp[:R + 1]) == 0] = 0
+    u = np.finfo(val.dtype).eps / 2
+    bnd = 2 * lens * u / (1 - lens * u) * ab + np.finfo(val.dtype).tiny
+    return bool(np.all(np.abs(y[:R].astype(np.float64) - yref.astype(np.float64)) <= bnd))
+
+
+def sec_addon(args, sp, ctx, host):
+    """The reference's call surface, timed: spmv_csr_wrapper through the
+    Node-API addon (bench_node.js), matrix resident after the first call,
+    x and y across PCIe on every call, on C1 (FP32 Laplacian 256^2, inner_max
+    1 and 500) and on the headline matrix (FP64, inner_max 1 and 10)."""
+    import tempfile
+    import shutil
+    import oracle
+    from sciwasm.sparse_amd import gen
+    names = {-1: "none", 0: "row blocks", 1: "row vector", 2: "column-sorted tiles"}
+    out = {}
+    base = tempfile.mkdtemp(prefix="sable_addon_")
+    try:
+        rp, col, val = gen.laplacian_2d(256, np.float32)
+        N = len(rp) - 1
+        x = np.random.default_rng(1).uniform(-1, 1, N).astype(np.float32)
+        d = os.path.join(base, "c1")
+        os.makedirs(d)
+        r, y = _addon_run(d, 32, rp, col, val, x, 20, 200, (1, 500), 64)
+        t0 = time.perf_counter()
+        reps = 300
+        oracle.spmv_csr(N, rp, col, val, x, np.zeros(N, np.float32), reps)
+        cpu_us = (time.perf_counter() - t0) / reps * 1e6
+        b = 4 * (N + 1) + 8 * len(col) + 8 * N
+        out["c1"] = {"N": N, "nnz": len(col), "schedule": names.get(r["schedule"], r["schedule"]),
+                     "first_call_ms": r4(r["first_call_ms"]),
+                     "call_us_inner1": r4(r["runs"]["1"]["median_ms"] * 1e3),
+                     "call_ms_inner500": r4(r["runs"]["500"]["median_ms"]),
+                     "product_us_inner500": r4(r["runs"]["500"]["median_ms"] * 1e3 / 500),
+                     "gbs_inner500": r4(b * 500 / (r["runs"]["500"]["median_ms"] * 1e-3) / 1e9),
+                     "uploads_in_timed_calls": r["runs"]["1"]["uploads"] + r["runs"]["500"]["uploads"],
+                     "cpu_serial_us": r4(cpu_us), "within_csr_bound": _row_check(rp, col, val, x, y, N),
+                     "write_tracking": r["write_tracking"]}
+        log("addon c1:", out["c1"])
+        if host is not None:
+            rp, col, val, x = host
+            N, nnz = len(rp) - 1, len(col)
+            d = os.path.join(base, "c2a")
+            os.makedirs(d)
+            lin = int((4 * (N + 1) + 12 * nnz + 16 * N) / (1 << 20)) + 256
+            r, y = _addon_run(d, 64, rp, col, val, x, 2, 10, (1, 10), lin)
+            b = 4 * (N + 1) + 12 * nnz + 16 * N
+            pcie = 24 * N  # x and y up, y down per call
+            c1m, c10 = r["runs"]["1"]["median_ms"], r["runs"]["10"]["median_ms"]
+            # per-call cost = transfers + products: the two call shapes give
+            # the transfer part and the product part apart
+            prod = (c10 - c1m) / 9
+            out["c2a"] = {"N": N, "nnz": nnz, "schedule": names.get(r["schedule"], r["schedule"]),
+                          "load_s": r4(r["load_s"]), "first_call_ms": r4(r["first_call_ms"]),
+                          "call_ms_inner1": r4(c1m), "call_ms_inner10": r4(c10),
+                          "product_ms_from_calls": r4(prod), "transfer_ms_per_call": r4(c1m - prod),
+                          "pcie_gbs": r4(pcie / ((c1m - prod) * 1e-3) / 1e9) if c1m > prod else None,
+                          "gbs_inner1": r4(b / (c1m * 1e-3) / 1e9), "gbs_inner10": r4(10 * b / (c10 * 1e-3) / 1e9),
+                          "uploads_in_timed_calls": r["runs"]["1"]["uploads"] + r["runs"]["10"]["uploads"],
+                          "rows_checked": N // 4, "within_csr_bound": _row_check(rp, col, val, x, y, N // 4)}
+            log("addon c2a:", out["c2a"])
+    finally:
+        shutil.rmtree(base, ignore_errors=True)
     return out
 
 
@@ -803,7 +905,8 @@ def main():
     ap.add_argument("--ingest-rows", type=int, default=4194304)
     ap.add_argument("--ingest-entries", type=int, default=8000000)
     ap.add_argument("--reorder-grid", type=int, default=1000)
-    ap.add_argument("--sections", type=str, default="c1,band,zipf16,convert,ingest,ewise,reorder,spts,batch,serve",
+    ap.add_argument("--sections", type=str,
+                    default="c1,addon,band,zipf16,convert,ingest,ewise,reorder,spts,batch,serve",
                     help="comma list of extra sections (empty for the headline only)")
     ap.add_argument("--batch-count", type=int, default=2048)
     ap.add_argument("--serve-rates", type=str, default="2000,10000,30000,60000,120000,240000")
@@ -838,6 +941,8 @@ def main():
 
     if "c1" in sections:
         run("c1", lambda: sec_c1(args, sp, ctx, T))
+    if "addon" in sections:
+        run("addon", lambda: sec_addon(args, sp, ctx, host))
     if "band" in sections:
         run("band", lambda: sec_band(args, sp, ctx, T))
     if "zipf16" in sections:

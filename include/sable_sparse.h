@@ -71,6 +71,14 @@ SABLE_API int sable_host_free(void* p);
 SABLE_API int sable_memcpy_h2d(sable_ctx_t* ctx, void* dst, const void* src, size_t bytes);
 SABLE_API int sable_memcpy_d2h(sable_ctx_t* ctx, void* dst, const void* src, size_t bytes);
 SABLE_API int sable_memcpy_d2d(sable_ctx_t* ctx, void* dst, const void* src, size_t bytes);
+/* Copies between PAGEABLE host memory and the device through the context's
+ * two pinned 4 MiB staging halves, the host copy of one half overlapping the
+ * DMA of the other (the addon's write-tracked linear memory is pageable).
+ * h2d returns once the host bytes have been read (the device copy is ordered
+ * on the context stream); d2h returns with the bytes in place.  One caller at
+ * a time per context. */
+SABLE_API int sable_memcpy_h2d_staged(sable_ctx_t* ctx, void* dst, const void* src, size_t bytes);
+SABLE_API int sable_memcpy_d2h_staged(sable_ctx_t* ctx, void* dst, const void* src, size_t bytes);
 /* Fill with a byte value using a kernel (no hipMemset nodes). */
 SABLE_API int sable_fill_bytes(sable_ctx_t* ctx, void* dst, int byte, size_t bytes);
 /* Replaces clear_vec (seq/src/sparse32.js:1319-1322) and create_vec's fill. */
@@ -79,6 +87,11 @@ SABLE_API int sable_fill_value(sable_ctx_t* ctx, sable_dtype_t dtype, void* dst,
  * vectors after a parallel COO SpMV.  Replaces sum
  * (par/src/sparse_opt_32.wat:1420-1481, driven from par/src/sparse32.js:2675-2691). */
 SABLE_API int sable_vec_add(sable_ctx_t* ctx, sable_dtype_t dtype, void* y, const void* w, int64_t n);
+
+/* Largest element of a device int32 array (-1 for n = 0; synchronises):
+ * the x and y lengths of a resident COO/ELL, found once per upload instead
+ * of scanning the host arrays on every call. */
+SABLE_API int sable_max_i32(sable_ctx_t* ctx, const int32_t* a, int64_t n, int32_t* out);
 
 /* Timing on the context stream (HIP events). */
 SABLE_API int sable_event_create(void** ev);
@@ -118,6 +131,12 @@ SABLE_API int sable_spmv_csr_plan(sable_ctx_t* ctx, const sable_csr_plan_t* plan
  * 2 = that kernel on a persistent grid of 8 workgroups per CU; 3 = the basic
  * kernel with non-temporal (streaming) col/val loads. */
 SABLE_API int sable_csr_kernel_select(int kernel);
+/* Plan-free CSR product: the row-block plan is built on the device on every
+ * call (block weight raised when the default would need more than N/2 +
+ * 1024 blocks) and a persistent grid runs it, so the call neither
+ * synchronises nor reads anything back.  A row_ptr that decreases leaves y
+ * unchanged (no error is reported without a synchronisation; the plan
+ * entry points check it). */
 SABLE_API int sable_spmv_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
                              const int32_t* col, const void* val, const void* x, void* y,
                              double beta, int inner_max);
@@ -159,6 +178,12 @@ SABLE_API void sable_csr_tiles_destroy(sable_csr_tiles_t* p);
 SABLE_API int sable_csr_tiles_info(const sable_csr_tiles_t* p, int32_t* row_tiles, int32_t* npanels, int64_t* bytes);
 SABLE_API int sable_spmv_csr_tiles(sable_ctx_t* ctx, const sable_csr_tiles_t* p, const void* x, void* y, double beta,
                                    int inner_max);
+/* The same plan for a matrix of N rows whose columns lie in [0, ncols)
+ * (e.g. a row slice of a larger matrix); sable_csr_tiles_create is this
+ * with ncols = N. */
+SABLE_API int sable_csr_tiles_create_rect(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t ncols,
+                                          const int32_t* row_ptr, const int32_t* col, const void* val,
+                                          int32_t tile_rows, int32_t panel_cols, sable_csr_tiles_t** out);
 /* Tile schedule for all later products (process-wide): 0 = one launch per
  * panel, y updated after each; 1 = one launch, each row tile walks all
  * panels with its row sums in LDS and writes y once (default). */
@@ -199,6 +224,45 @@ SABLE_API int sable_csr_tiles_batch_item(const sable_csr_tiles_t* p, const void*
 SABLE_API int sable_batch_spmv_tiles(sable_ctx_t* ctx, sable_dtype_t dtype, int nrhs,
                                      const sable_batch_tiles_item_t* d_items, int nitems, int64_t total_units,
                                      int32_t max_tile_rows, int workgroups);
+
+/* Autotuned CSR product for a resident matrix: the schedule behind the
+ * reference's spmv_csr / spmv_csr_wrapper (seq/src/sparse_opt_32.wat:227-288,
+ * seq/src/sparse_opt_64.wat:239-300, par/src/sparse_opt_32.wat:1712-1771)
+ * and spmv_coo when the Node-API addon keeps the matrix on the device.
+ * Candidate schedules (row blocks at three weights, the row-vector kernel
+ * with lanes around the mean row length, column-sorted tiles) are built and
+ * timed with HIP events on the context stream and the fastest is kept; the
+ * choice is cached per matrix class (row-length statistics, value type,
+ * whether x fits one XCD's L2), so later matrices of a class skip the timing.
+ * row_ptr may be a slice (row_ptr[0] != 0, col/val indexed by the absolute
+ * values); ncols <= 0 measures the x length as the largest column + 1.
+ * Creation synchronises; products only launch.  The caller's row_ptr, col
+ * and val must stay valid and unchanged while the plan is used (a tile
+ * schedule keeps its own copy of col and val).  Flags:
+ * SABLE_AUTO_DETERMINISTIC keeps to the bit-reproducible schedules (no
+ * tiles); SABLE_AUTO_NO_TIMING takes a rule of thumb instead of timing. */
+enum { SABLE_AUTO_DETERMINISTIC = 1, SABLE_AUTO_NO_TIMING = 2 };
+enum { SABLE_SCHED_BLOCKS = 0, SABLE_SCHED_VECTOR = 1, SABLE_SCHED_TILES = 2 };
+typedef struct sable_csr_auto_s sable_csr_auto_t;
+typedef struct {
+    int schedule;        /* SABLE_SCHED_*                                          */
+    int param0, param1;  /* blocks: block/row weight; vector: lanes/rows; tiles:
+                            tile rows/panel columns                               */
+    int32_t ncols;       /* x length                                              */
+    int64_t nnz;
+    int64_t extra_bytes; /* device bytes of the schedule's own copy of the matrix */
+    float ms;            /* measured time of one product (0 when taken from the cache) */
+    int from_cache;      /* 1: the class's cached choice, not timed again           */
+} sable_csr_auto_info_t;
+SABLE_API int sable_csr_auto_create(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t ncols,
+                                    const int32_t* row_ptr, const int32_t* col, const void* val, int flags,
+                                    sable_csr_auto_t** out);
+SABLE_API void sable_csr_auto_destroy(sable_csr_auto_t* plan);
+SABLE_API int sable_csr_auto_info(const sable_csr_auto_t* plan, sable_csr_auto_info_t* info);
+SABLE_API int sable_spmv_csr_auto(sable_ctx_t* ctx, const sable_csr_auto_t* plan, const void* x, void* y,
+                                  double beta, int inner_max);
+/* Forgets every cached class choice (process-wide). */
+SABLE_API void sable_csr_auto_cache_clear(void);
 
 /* COO.  Row-sorted input runs a deterministic segmented reduction (row
  * heads found on the device, then the CSR block kernel); unsorted input is
@@ -245,6 +309,39 @@ SABLE_API int sable_spmv_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, i
                              const int32_t* indices, const void* data, const void* x, void* y,
                              double beta, int inner_max);
 
+/* Rows [r0, r1) only of the DIA and ELL products above (the other rows of y
+ * are not touched), each row with the same order and bits.  The parallel
+ * tree's column-major slices: spmv_dia_col_wrapper
+ * (par/src/sparse_opt_32.wat:9112-9246), spmv_dia_col_basic_wrapper
+ * (:8991-9111) and spmv_bdia_col_wrapper (:8757-8965), whose blocked and
+ * basic sweeps add each row's diagonals in the same stored order; and
+ * spmv_ell_col_wrapper (:11055-11106), spmv_ell_col_gs_wrapper
+ * (:11291-11400) and spmv_bell_col_gs_wrapper (:11130-11290), whose SIMD
+ * runs across consecutive rows, so each row adds its slots in order. */
+SABLE_API int sable_spmv_dia_range(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
+                                   const int32_t* offset, const void* data, int32_t r0, int32_t r1, const void* x,
+                                   void* y, double beta, int inner_max);
+SABLE_API int sable_spmv_ell_range(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc,
+                                   const int32_t* indices, const void* data, int32_t r0, int32_t r1, const void* x,
+                                   void* y, double beta, int inner_max);
+
+/* Row-major ("major: row") DIA and ELL of the parallel tree, y += A.x on
+ * rows [r0, r1) (inner_max times).  DIA: data[i*nd + k] = a(i, i +
+ * offset[k]), columns outside [0, N) skipped; replaces spmv_dia /
+ * spmv_dia_wrapper (par/src/sparse_opt_32.wat:8213-8290).  ELL:
+ * data/indices[i*nc + k]; order 0 adds the slots in order and leaves the
+ * row at the first negative index, replacing spmv_ell / spmv_ell_wrapper
+ * (:8292-8367); order 1 is the gs order of spmv_ell_row_gs_wrapper
+ * (:8369-8756): for nc >= 4 the first nc % 4 slots go into the running sum,
+ * the rest into four lane sums that are added to it in lane order.  Both
+ * bit-identical to the reference. */
+SABLE_API int sable_spmv_dia_rowmajor(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd,
+                                      const int32_t* offset, const void* data, int32_t r0, int32_t r1,
+                                      const void* x, void* y, int inner_max);
+SABLE_API int sable_spmv_ell_rowmajor(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc,
+                                      const int32_t* indices, const void* data, int32_t r0, int32_t r1,
+                                      const void* x, void* y, int order, int inner_max);
+
 
 /* ------------------------------------------------------------ conversions */
 /* All conversions are bit-identical to the reference algorithms.  The
@@ -283,6 +380,18 @@ SABLE_API int sable_csr_num_diags(sable_ctx_t* ctx, int32_t N, const int32_t* ro
 SABLE_API int sable_csr_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
                             const int32_t* row_ptr, const int32_t* col, const void* val, int32_t* out_offset,
                             void* out_data);
+
+/* Row-major ("major: row") conversions of the parallel tree: ELL slots at
+ * i*nc + k (padding value 0, index 0), replacing csr_ell
+ * (par/src/sparse32.js:1840-1880); DIA data[i*nd + k] with stride N and the
+ * offsets ascending (a later duplicate overwrites), replacing csr_dia
+ * (par/src/sparse32.js:1912-1973).  Bit-identical to the reference. */
+SABLE_API int sable_csr_ell_rowmajor(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc,
+                                     const int32_t* row_ptr, const int32_t* col, const void* val,
+                                     int32_t* out_indices, void* out_data);
+SABLE_API int sable_csr_dia_rowmajor(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd,
+                                     const int32_t* row_ptr, const int32_t* col, const void* val,
+                                     int32_t* out_offset, void* out_data);
 
 /* COO -> CSC: the counting scatter by column with a stable per-column sort
  * by row (duplicates kept in input order), i.e. coo_csr with the roles of

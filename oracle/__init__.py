@@ -50,6 +50,10 @@ def lib():
             getattr(L, "oracle_spmv_csr_" + suf).argtypes = [_I, _I32P, _I32P, FP, FP, FP, _I]
             getattr(L, "oracle_spmv_dia_" + suf).argtypes = [_I, _I, _I, _I32P, FP, FP, FP, _I]
             getattr(L, "oracle_spmv_ell_" + suf).argtypes = [_I, _I, _I32P, FP, FP, FP, _I]
+            getattr(L, "oracle_csr_ell_row_" + suf).argtypes = [_I, _I, _I32P, _I32P, FP, _I32P, FP]
+            getattr(L, "oracle_csr_dia_row_" + suf).argtypes = [_I, _I, _I32P, _I32P, FP, _I32P, FP]
+            getattr(L, "oracle_spmv_dia_row_" + suf).argtypes = [_I, _I, _I32P, FP, _I, _I, FP, FP, _I]
+            getattr(L, "oracle_spmv_ell_row_" + suf).argtypes = [_I, _I, _I32P, FP, _I, _I, FP, FP, _I, _I]
             getattr(L, "oracle_spts_level_" + suf).argtypes = [_I, _I32P, _I32P, FP, FP, FP]
             getattr(L, "oracle_spts_syncfree_" + suf).argtypes = [_I, _I32P, _I32P, FP, FP, FP]
             getattr(L, "oracle_spts_csc_" + suf).argtypes = [_I, _I32P, _I32P, FP, FP, FP]
@@ -221,6 +225,43 @@ def spmv_ell(N, nc, indices, data, x, y, inner_max=1):
     getattr(lib(), "oracle_spmv_ell_" + _suf(data))(N, nc, _i32(indices),
                                                    np.ascontiguousarray(data),
                                                    np.ascontiguousarray(x, data.dtype), y, inner_max)
+    return y
+
+
+def csr_ell_row(N, rowptr, col, val, nc=None):
+    """Row-major ELL (par/src/sparse32.js:1840-1880): (nc, indices, data)."""
+    rowptr, col, val = _i32(rowptr), _i32(col), np.ascontiguousarray(val)
+    if nc is None:
+        nc = num_cols(N, rowptr)
+    idx = np.zeros(nc * N, np.int32)
+    data = np.zeros(nc * N, val.dtype)
+    getattr(lib(), "oracle_csr_ell_row_" + _suf(val))(N, nc, rowptr, col, val, idx, data)
+    return nc, idx, data
+
+
+def csr_dia_row(N, rowptr, col, val):
+    """Row-major DIA (par/src/sparse32.js:1912-1973): (nd, offset, data)."""
+    rowptr, col, val = _i32(rowptr), _i32(col), np.ascontiguousarray(val)
+    nd, _ = num_diags(N, rowptr, col)
+    off = np.zeros(max(nd, 1), np.int32)
+    data = np.zeros(max(nd * N, 1), val.dtype)
+    getattr(lib(), "oracle_csr_dia_row_" + _suf(val))(N, nd, rowptr, col, val, off, data)
+    return nd, off[:nd], data[:nd * N]
+
+
+def spmv_dia_row(N, offset, data, x, y, r0=0, r1=None, inner_max=1):
+    y = np.array(y, dtype=data.dtype, copy=True)
+    r1 = N if r1 is None else r1
+    getattr(lib(), "oracle_spmv_dia_row_" + _suf(data))(N, len(offset), _i32(offset), np.ascontiguousarray(data),
+                                                       r0, r1, np.ascontiguousarray(x, data.dtype), y, inner_max)
+    return y
+
+
+def spmv_ell_row(N, nc, indices, data, x, y, r0=0, r1=None, gs=False, inner_max=1):
+    y = np.array(y, dtype=data.dtype, copy=True)
+    r1 = N if r1 is None else r1
+    getattr(lib(), "oracle_spmv_ell_row_" + _suf(data))(N, nc, _i32(indices), np.ascontiguousarray(data), r0, r1,
+                                                       np.ascontiguousarray(x, data.dtype), y, int(gs), inner_max)
     return y
 
 

@@ -392,6 +392,101 @@ EXPORT void oracle_spmv_ell_##SUF(int N, int nc, const int *indices,            
 DEF_SPMV(float, f32)
 DEF_SPMV(double, f64)
 
+/* Row-major ("major: row") storage of the parallel tree. */
+#define DEF_ROWMAJOR(T, SUF)                                                      \
+/* csr_ell, row-major: par/src/sparse32.js:1840-1880: data[i*nc+k],              \
+ * indices[i*nc+k], padding value 0 and index 0. */                             \
+EXPORT void oracle_csr_ell_row_##SUF(int N, int nc, const int *rowptr,           \
+                                     const int *col, const T *val, int *indices, \
+                                     T *data)                                    \
+{                                                                                 \
+    memset(indices, 0, sizeof(int) * (size_t)nc * N);                            \
+    memset(data, 0, sizeof(T) * (size_t)nc * N);                                 \
+    for (int i = 0; i < N; i++) {                                                 \
+        int k = 0;                                                                \
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++) {                         \
+            data[(size_t)i * nc + k] = val[j];                                    \
+            indices[(size_t)i * nc + k] = col[j];                                 \
+            k++;                                                                  \
+        }                                                                         \
+    }                                                                             \
+}                                                                                 \
+/* csr_dia, row-major: par/src/sparse32.js:1912-1973: offsets ascending over     \
+ * the occupied diagonals (stride N), data[i*nd + k] = a(i, i+offset[k]) with    \
+ * the first k whose offset matches; a later duplicate overwrites. */           \
+EXPORT void oracle_csr_dia_row_##SUF(int N, int nd, const int *rowptr,           \
+                                     const int *col, const T *val, int *offset,  \
+                                     T *data)                                    \
+{                                                                                 \
+    memset(data, 0, sizeof(T) * (size_t)nd * N);                                 \
+    int *ind = (int *)calloc((size_t)(2 * N > 1 ? 2 * N - 1 : 1), sizeof(int));  \
+    for (int i = 0; i < N; i++)                                                   \
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++) ind[N + col[j] - i - 1]++;\
+    int diag_no = -((2 * N - 1) / 2), index = 0;                                 \
+    for (int i = 0; i < 2 * N - 1; i++) {                                         \
+        if (ind[i]) offset[index++] = diag_no;                                    \
+        diag_no++;                                                                \
+    }                                                                             \
+    free(ind);                                                                    \
+    for (int i = 0; i < N; i++)                                                   \
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++)                           \
+            for (int k = 0; k < nd; k++)                                          \
+                if (col[j] - i == offset[k]) {                                    \
+                    data[(size_t)i * nd + k] = val[j];                            \
+                    break;                                                        \
+                }                                                                 \
+}                                                                                 \
+/* spmv_dia, row-major: par/src/sparse_opt_32.wat:8213-8269: rows [r0, r1),      \
+ * t = y[i]; for each diagonal k with c = offset[k] + i in [0, N):               \
+ * t = d*x[c] + t; y[i] = t. */                                                  \
+EXPORT void oracle_spmv_dia_row_##SUF(int N, int nd, const int *offset,          \
+                                      const T *data, int r0, int r1, const T *x, \
+                                      T *y, int inner_max)                       \
+{                                                                                 \
+    if (r0 >= r1 || nd <= 0) return;                                              \
+    for (int it = 0; it < inner_max; it++)                                        \
+        for (int i = r0; i < r1; i++) {                                           \
+            T t = y[i];                                                           \
+            for (int k = 0; k < nd; k++) {                                        \
+                int c = offset[k] + i;                                            \
+                if (c >= 0 && c < N) { T p = data[(size_t)i * nd + k] * x[c]; t = p + t; } \
+            }                                                                     \
+            y[i] = t;                                                             \
+        }                                                                         \
+}                                                                                 \
+/* spmv_ell, row-major: par/src/sparse_opt_32.wat:8292-8346 (gs = 0): slots in   \
+ * order from t = y[i], leaving the row at the first negative index; and         \
+ * spmv_ell_row_gs :8369-8756 (gs = 1): for nc >= 4 the first nc % 4 slots go    \
+ * into t, slot h + 4q + l into lane l (each lane from 0), and                   \
+ * y[i] = (((t + l0) + l1) + l2) + l3; narrower rows take the plain order. */    \
+EXPORT void oracle_spmv_ell_row_##SUF(int N, int nc, const int *indices,         \
+                                      const T *data, int r0, int r1, const T *x, \
+                                      T *y, int gs, int inner_max)               \
+{                                                                                 \
+    (void)N;                                                                      \
+    if (r0 >= r1 || nc <= 0) return;                                              \
+    for (int it = 0; it < inner_max; it++)                                        \
+        for (int i = r0; i < r1; i++) {                                           \
+            const int *ci = indices + (size_t)i * nc;                             \
+            const T *d = data + (size_t)i * nc;                                   \
+            T t = y[i];                                                           \
+            if (!gs || nc < 4) {                                                  \
+                for (int k = 0; k < nc && ci[k] >= 0; k++) { T p = d[k] * x[ci[k]]; t = p + t; } \
+                y[i] = t;                                                         \
+            } else {                                                              \
+                int h = nc % 4;                                                   \
+                for (int k = 0; k < h; k++) { T p = d[k] * x[ci[k]]; t = p + t; } \
+                T l[4] = {0, 0, 0, 0};                                            \
+                for (int k = h; k < nc; k += 4)                                   \
+                    for (int q = 0; q < 4; q++) { T p = d[k + q] * x[ci[k + q]]; l[q] = p + l[q]; } \
+                t = t + l[0]; t = t + l[1]; t = t + l[2]; t = t + l[3];           \
+                y[i] = t;                                                         \
+            }                                                                     \
+        }                                                                         \
+}
+DEF_ROWMAJOR(float, f32)
+DEF_ROWMAJOR(double, f64)
+
 /* FP64 oracle for CSR SpMV: y_ld[i] = y0[i] + sum_j v*x in long double. */
 EXPORT void oracle_spmv_csr_ld(int N, const int *rowptr, const int *col,
                                const double *val, const double *x, const double *y0,

@@ -10,7 +10,7 @@ from ._lib import SableError, load as _load_library  # noqa: F401
 from .core import (Context, DeviceArray, sparse_COO_t, sparse_CSC_t, sparse_CSR_t, sparse_DIA_t,  # noqa: F401
                    sparse_ELL_t, sparse_vec_t, coo_from_host, csc_from_host, csr_from_host, dia_from_host,
                    ell_from_host, create_vec, vec_from_host, clear_vec, init_x, fletcher_sum, fletcher_sum_y)
-from .spmv import spmv, CsrPlan, CsrPanels, CsrTiles, CooPlan  # noqa: F401
+from .spmv import spmv, CsrPlan, CsrPanels, CsrTiles, CooPlan, CsrAuto  # noqa: F401
 from .convert import coo_csr, coo_csc, num_cols, csr_ell, num_diags, csr_dia, transpose  # noqa: F401
 from .spts import (SptsAnalysis, CSR_create_level_sets, CSR_create_level_sets_with_reorder,  # noqa: F401
                    spts_level_csr, spts_syncfree, spts_csc)
@@ -26,7 +26,7 @@ __all__ = [
     "sparse_COO_t", "sparse_CSC_t", "sparse_CSR_t", "sparse_DIA_t", "sparse_ELL_t", "sparse_vec_t",
     "coo_from_host", "csc_from_host", "csr_from_host", "dia_from_host", "ell_from_host",
     "create_vec", "vec_from_host", "clear_vec", "init_x", "fletcher_sum", "fletcher_sum_y",
-    "spmv", "CsrPlan", "CsrPanels", "CsrTiles", "CooPlan", "Autotuner", "row_stats_key",
+    "spmv", "CsrPlan", "CsrPanels", "CsrTiles", "CooPlan", "CsrAuto", "Autotuner", "row_stats_key",
     "coo_csr", "coo_csc", "num_cols", "csr_ell", "num_diags", "csr_dia", "transpose",
     "SptsAnalysis", "CSR_create_level_sets", "CSR_create_level_sets_with_reorder", "spts_level_csr",
     "spts_syncfree", "spts_csc", "Server", "batch_spmv", "batch_spmv_tiles", "batch_tile_plans", "batch_spts", "batch_spts_refined", "lower_dominant",

@@ -55,6 +55,11 @@ class CsrTune(ctypes.Structure):
     _fields_ = [("block_weight", c_int), ("row_weight", c_int), ("vector_lanes", c_int), ("vector_rows", c_int)]
 
 
+class CsrAutoInfo(ctypes.Structure):
+    _fields_ = [("schedule", c_int), ("param0", c_int), ("param1", c_int), ("ncols", c_i32), ("nnz", c_i64),
+                ("extra_bytes", c_i64), ("ms", c_flt), ("from_cache", c_int)]
+
+
 class MMInfo(ctypes.Structure):
     _fields_ = [("nrows", c_i64), ("ncols", c_i64), ("nentries", c_i64), ("nnz", c_i64), ("lower_nnz", c_i64),
                 ("field", ctypes.c_int32), ("symmetric", ctypes.c_int32), ("body_offset", c_i64)]
@@ -75,6 +80,21 @@ SIGNATURES = {
     "sable_memcpy_h2d": [c_vp, c_vp, c_vp, c_sz],
     "sable_memcpy_d2h": [c_vp, c_vp, c_vp, c_sz],
     "sable_memcpy_d2d": [c_vp, c_vp, c_vp, c_sz],
+    "sable_memcpy_h2d_staged": [c_vp, c_vp, c_vp, c_sz],
+    "sable_memcpy_d2h_staged": [c_vp, c_vp, c_vp, c_sz],
+    "sable_max_i32": [c_vp, c_vp, c_i64, ctypes.POINTER(c_i32)],
+    "sable_csr_tiles_create_rect": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_vpp],
+    "sable_csr_auto_create": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_int, c_vpp],
+    "sable_csr_auto_destroy": [c_vp],
+    "sable_csr_auto_info": [c_vp, c_vp],
+    "sable_spmv_csr_auto": [c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
+    "sable_csr_auto_cache_clear": [],
+    "sable_spmv_dia_range": [c_vp, c_int, c_i32, c_i32, c_i32, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_dbl, c_int],
+    "sable_spmv_ell_range": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_dbl, c_int],
+    "sable_spmv_dia_rowmajor": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_int],
+    "sable_spmv_ell_rowmajor": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_int, c_int],
+    "sable_csr_ell_rowmajor": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "sable_csr_dia_rowmajor": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
     "sable_fill_bytes": [c_vp, c_vp, c_int, c_sz],
     "sable_fill_value": [c_vp, c_int, c_vp, c_i64, c_dbl],
     "sable_vec_add": [c_vp, c_int, c_vp, c_vp, c_i64],
@@ -173,6 +193,8 @@ _RESTYPES = {
     "sable_csr_plan_destroy": None,
     "sable_coo_plan_destroy": None,
     "sable_csr_tiles_destroy": None,
+    "sable_csr_auto_destroy": None,
+    "sable_csr_auto_cache_clear": None,
     "sable_mm_destroy": None,
     "sable_spts_destroy": None,
     "sable_csr_panels_destroy": None,

@@ -47,6 +47,11 @@ struct sable_ctx_s {
     // keeps its memory until the context is destroyed)
     hipMemPool_t pool = nullptr;
     bool pool_tried = false;
+    // pinned staging halves for copies from/to pageable host memory
+    void* stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    bool stage_used[2] = {false, false};
+    int stage_next = 0;
     std::mutex mu;
 };
 

@@ -94,7 +94,9 @@ __global__ void k_max_row_len(int32_t N, const int32_t* __restrict__ rp, int* __
     if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(out, m);
 }
 
-template <typename T>
+// ROW: row-major slots (slot k of row i at i*nc + k, "major: row",
+// par/src/sparse32.js:1840-1880); otherwise column-major (k*N + i).
+template <typename T, bool ROW = false>
 __global__ void k_csr_ell(int32_t N, int32_t nc, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                           const T* __restrict__ val, int32_t* __restrict__ oidx, T* __restrict__ odata,
                           int* __restrict__ err) {
@@ -103,7 +105,7 @@ __global__ void k_csr_ell(int32_t N, int32_t nc, const int32_t* __restrict__ rp,
         const int len = rp[i + 1] - (int)a;
         if (len > nc) atomicOr(err, 1);
         for (int k = 0; k < nc; k++) {
-            const int64_t e = (int64_t)k * N + i;
+            const int64_t e = ROW ? i * nc + k : (int64_t)k * N + i;
             if (k < len) {
                 oidx[e] = col[a + k];
                 odata[e] = val[a + k];
@@ -165,9 +167,12 @@ __global__ void k_diag_offsets(int64_t nd2, int32_t N, const int32_t* __restrict
         if (flags[d]) offset[pos[d]] = (int32_t)(d - (N - 1));
 }
 
-template <typename T>
+// ROW: row-major data[i*nd + k] ("major: row", par/src/sparse32.js:1912-1973,
+// stride N); otherwise diagonal-major data[k*stride + slot].
+template <typename T, bool ROW = false>
 __global__ void k_csr_dia(int32_t N, int32_t stride, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                          const T* __restrict__ val, const int32_t* __restrict__ pos, T* __restrict__ data) {
+                          const T* __restrict__ val, const int32_t* __restrict__ pos, T* __restrict__ data,
+                          int32_t nd = 0) {
     const int64_t shiftneg = (int64_t)N - stride;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
         for (int64_t j = rp[i]; j < rp[i + 1]; j++) {
@@ -175,6 +180,10 @@ __global__ void k_csr_dia(int32_t N, int32_t stride, const int32_t* __restrict__
             if (c < 0 || c >= N) continue;
             const int off = c - (int)i;
             const int64_t k = pos[(int64_t)N - 1 + off];
+            if (ROW) {
+                data[i * nd + k] = val[j];
+                continue;
+            }
             const int64_t slot = i - (off < 0 ? shiftneg : 0);
             if (slot >= 0 && slot < stride) data[k * stride + slot] = val[j];
         }
@@ -496,8 +505,8 @@ SABLE_API int sable_csr_num_cols(sable_ctx_t* ctx, int32_t N, const int32_t* row
     return read_i32(ctx, out, 1, nc);
 }
 
-SABLE_API int sable_csr_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc, const int32_t* row_ptr,
-                            const int32_t* col, const void* val, int32_t* out_indices, void* out_data) {
+static int csr_ell_any(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc, const int32_t* row_ptr,
+                       const int32_t* col, const void* val, int32_t* out_indices, void* out_data, bool row_major) {
     SABLE_CHECK(ctx && row_ptr, SABLE_EINVAL, "NULL argument");
     SABLE_CHECK(N >= 0 && nc >= 0, SABLE_EINVAL, "negative size");
     if (N == 0 || nc == 0) return SABLE_OK;
@@ -506,20 +515,36 @@ SABLE_API int sable_csr_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, in
     if (rc) return rc;
     int* err = (int*)s;
     SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), ctx->stream));
-    if (dtype == SABLE_F32)
-        hipLaunchKernelGGL(k_csr_ell<float>, dim3(gs(N)), dim3(CB), 0, ctx->stream, N, nc, row_ptr, col,
-                           (const float*)val, out_indices, (float*)out_data, err);
-    else if (dtype == SABLE_F64)
-        hipLaunchKernelGGL(k_csr_ell<double>, dim3(gs(N)), dim3(CB), 0, ctx->stream, N, nc, row_ptr, col,
-                           (const double*)val, out_indices, (double*)out_data, err);
-    else
+#define CSR_ELL_LAUNCH(TT, RR)                                                                              \
+    hipLaunchKernelGGL((k_csr_ell<TT, RR>), dim3(gs(N)), dim3(CB), 0, ctx->stream, N, nc, row_ptr, col,     \
+                       (const TT*)val, out_indices, (TT*)out_data, err)
+    if (dtype == SABLE_F32) {
+        if (row_major) CSR_ELL_LAUNCH(float, true);
+        else CSR_ELL_LAUNCH(float, false);
+    } else if (dtype == SABLE_F64) {
+        if (row_major) CSR_ELL_LAUNCH(double, true);
+        else CSR_ELL_LAUNCH(double, false);
+    } else {
         return fail(SABLE_EINVAL, "bad dtype");
+    }
+#undef CSR_ELL_LAUNCH
     SABLE_LAUNCH_CHECK();
     int e = 0;
     rc = read_err(ctx, err, &e);
     if (rc) return rc;
     SABLE_CHECK(!e, SABLE_EINVAL, "a row is longer than nc=%d", nc);
     return SABLE_OK;
+}
+
+SABLE_API int sable_csr_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc, const int32_t* row_ptr,
+                            const int32_t* col, const void* val, int32_t* out_indices, void* out_data) {
+    return csr_ell_any(ctx, dtype, N, nc, row_ptr, col, val, out_indices, out_data, false);
+}
+
+SABLE_API int sable_csr_ell_rowmajor(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc,
+                                     const int32_t* row_ptr, const int32_t* col, const void* val,
+                                     int32_t* out_indices, void* out_data) {
+    return csr_ell_any(ctx, dtype, N, nc, row_ptr, col, val, out_indices, out_data, true);
 }
 
 static int diag_hist(sable_ctx_t* ctx, int32_t N, const int32_t* rp, const int32_t* col, int32_t** ind_out,
@@ -568,9 +593,9 @@ SABLE_API int sable_csr_num_diags(sable_ctx_t* ctx, int32_t N, const int32_t* ro
     return SABLE_OK;
 }
 
-SABLE_API int sable_csr_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
-                            const int32_t* row_ptr, const int32_t* col, const void* val, int32_t* out_offset,
-                            void* out_data) {
+static int csr_dia_any(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
+                       const int32_t* row_ptr, const int32_t* col, const void* val, int32_t* out_offset,
+                       void* out_data, bool row_major) {
     SABLE_CHECK(ctx && row_ptr, SABLE_EINVAL, "NULL argument");
     SABLE_CHECK(N >= 0 && nd >= 0 && stride >= 0 && stride <= N, SABLE_EINVAL, "bad sizes");
     SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
@@ -592,17 +617,38 @@ SABLE_API int sable_csr_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, in
     if (nd > 0 && stride < N) SABLE_CHECK(N - stride <= h[1], SABLE_EINVAL, "stride %d < N - min|offset|", stride);
     hipLaunchKernelGGL(k_diag_offsets, dim3(gs(nd2)), dim3(CB), 0, st, nd2, N, flags, pos, out_offset);
     const int64_t total = (int64_t)nd * stride;
+#define CSR_DIA_LAUNCH(TT, RR)                                                                                 \
+    hipLaunchKernelGGL(k_fill_t<TT>, dim3(gs(total)), dim3(CB), 0, st, (TT*)out_data, total);                  \
+    hipLaunchKernelGGL((k_csr_dia<TT, RR>), dim3(gs(N)), dim3(CB), 0, st, N, stride, row_ptr, col, (const TT*)val, \
+                       pos, (TT*)out_data, nd)
     if (dtype == SABLE_F32) {
-        hipLaunchKernelGGL(k_fill_t<float>, dim3(gs(total)), dim3(CB), 0, st, (float*)out_data, total);
-        hipLaunchKernelGGL(k_csr_dia<float>, dim3(gs(N)), dim3(CB), 0, st, N, stride, row_ptr, col, (const float*)val,
-                           pos, (float*)out_data);
+        if (row_major) {
+            CSR_DIA_LAUNCH(float, true);
+        } else {
+            CSR_DIA_LAUNCH(float, false);
+        }
     } else {
-        hipLaunchKernelGGL(k_fill_t<double>, dim3(gs(total)), dim3(CB), 0, st, (double*)out_data, total);
-        hipLaunchKernelGGL(k_csr_dia<double>, dim3(gs(N)), dim3(CB), 0, st, N, stride, row_ptr, col,
-                           (const double*)val, pos, (double*)out_data);
+        if (row_major) {
+            CSR_DIA_LAUNCH(double, true);
+        } else {
+            CSR_DIA_LAUNCH(double, false);
+        }
     }
+#undef CSR_DIA_LAUNCH
     SABLE_LAUNCH_CHECK();
     return SABLE_OK;
+}
+
+SABLE_API int sable_csr_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
+                            const int32_t* row_ptr, const int32_t* col, const void* val, int32_t* out_offset,
+                            void* out_data) {
+    return csr_dia_any(ctx, dtype, N, nd, stride, row_ptr, col, val, out_offset, out_data, false);
+}
+
+SABLE_API int sable_csr_dia_rowmajor(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd,
+                                     const int32_t* row_ptr, const int32_t* col, const void* val,
+                                     int32_t* out_offset, void* out_data) {
+    return csr_dia_any(ctx, dtype, N, nd, N, row_ptr, col, val, out_offset, out_data, true);
 }
 
 SABLE_API int sable_transpose_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,

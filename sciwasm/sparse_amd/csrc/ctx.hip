@@ -78,6 +78,20 @@ void ctx_pool_free(sable_ctx_t* ctx, void* p) {
     else (void)hipFree(p);
 }
 
+constexpr size_t STAGE_BYTES = (size_t)4 << 20;
+
+static int ensure_stage(sable_ctx_t* ctx) {
+    for (int k = 0; k < 2; k++) {
+        if (!ctx->stage[k] && hipHostMalloc(&ctx->stage[k], STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->stage[k] = nullptr;
+            return fail(SABLE_ENOMEM, "pinned staging buffer allocation failed");
+        }
+        if (!ctx->stage_ev[k]) SABLE_HIP(hipEventCreateWithFlags(&ctx->stage_ev[k], hipEventDisableTiming));
+    }
+    return SABLE_OK;
+}
+
 int read_device_i64(sable_ctx_t* ctx, const void* dsrc, size_t count, int64_t* host_out) {
     SABLE_CHECK(count <= 64, SABLE_EINVAL, "read_device_i64: count %zu > 64", count);
     SABLE_HIP(hipMemcpyAsync(ctx->pinned, dsrc, count * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -103,6 +117,18 @@ __global__ void k_fill_value(T* __restrict__ p, T v, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_max_i32_init(int* out) {
+    if (threadIdx.x == 0) *out = INT32_MIN;
+}
+
+__global__ void k_max_i32_reduce(int64_t n, const int32_t* __restrict__ a, int* __restrict__ out) {
+    int m = INT32_MIN;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = max(m, a[i]);
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, WAVE));
+    if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(out, m);
 }
 
 template <typename T>
@@ -159,6 +185,10 @@ SABLE_API void sable_ctx_destroy(sable_ctx_t* ctx) {
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->pool) (void)hipMemPoolDestroy(ctx->pool);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    for (int k = 0; k < 2; k++) {
+        if (ctx->stage[k]) (void)hipHostFree(ctx->stage[k]);
+        if (ctx->stage_ev[k]) (void)hipEventDestroy(ctx->stage_ev[k]);
+    }
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -225,6 +255,53 @@ SABLE_API int sable_memcpy_d2h(sable_ctx_t* ctx, void* dst, const void* src, siz
     return SABLE_OK;
 }
 
+SABLE_API int sable_memcpy_h2d_staged(sable_ctx_t* ctx, void* dst, const void* src, size_t bytes) {
+    SABLE_CHECK(ctx && (bytes == 0 || (dst && src)), SABLE_EINVAL, "NULL argument");
+    if (!bytes) return SABLE_OK;
+    int rc = ensure_stage(ctx);
+    if (rc) return rc;
+    int k = ctx->stage_next;
+    for (size_t off = 0; off < bytes; off += STAGE_BYTES, k ^= 1) {
+        const size_t n = bytes - off < STAGE_BYTES ? bytes - off : STAGE_BYTES;
+        if (ctx->stage_used[k]) SABLE_HIP(hipEventSynchronize(ctx->stage_ev[k]));
+        memcpy(ctx->stage[k], (const char*)src + off, n);
+        SABLE_HIP(hipMemcpyAsync((char*)dst + off, ctx->stage[k], n, hipMemcpyHostToDevice, ctx->stream));
+        SABLE_HIP(hipEventRecord(ctx->stage_ev[k], ctx->stream));
+        ctx->stage_used[k] = true;
+    }
+    ctx->stage_next = k;
+    return SABLE_OK;
+}
+
+SABLE_API int sable_memcpy_d2h_staged(sable_ctx_t* ctx, void* dst, const void* src, size_t bytes) {
+    SABLE_CHECK(ctx && (bytes == 0 || (dst && src)), SABLE_EINVAL, "NULL argument");
+    if (!bytes) return SABLE_OK;
+    int rc = ensure_stage(ctx);
+    if (rc) return rc;
+    const size_t nch = (bytes + STAGE_BYTES - 1) / STAGE_BYTES;
+    auto len = [&](size_t c) { return c + 1 < nch ? STAGE_BYTES : bytes - c * STAGE_BYTES; };
+    for (size_t c = 0; c <= nch; c++) {
+        if (c < nch) {
+            // chunk c into half c % 2 (its previous chunk was copied out in the last step)
+            const int k = (int)(c & 1);
+            if (ctx->stage_used[k]) SABLE_HIP(hipEventSynchronize(ctx->stage_ev[k]));
+            SABLE_HIP(hipMemcpyAsync(ctx->stage[k], (const char*)src + c * STAGE_BYTES, len(c), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+            SABLE_HIP(hipEventRecord(ctx->stage_ev[k], ctx->stream));
+            ctx->stage_used[k] = true;
+        }
+        if (c >= 1) {
+            // chunk c - 1 is copied out while chunk c is in flight
+            const int k = (int)((c - 1) & 1);
+            SABLE_HIP(hipEventSynchronize(ctx->stage_ev[k]));
+            memcpy((char*)dst + (c - 1) * STAGE_BYTES, ctx->stage[k], len(c - 1));
+            ctx->stage_used[k] = false;
+        }
+    }
+    ctx->stage_next = 0;
+    return SABLE_OK;
+}
+
 SABLE_API int sable_memcpy_d2d(sable_ctx_t* ctx, void* dst, const void* src, size_t bytes) {
     SABLE_CHECK(ctx, SABLE_EINVAL, "ctx is NULL");
     if (!bytes) return SABLE_OK;
@@ -278,6 +355,27 @@ SABLE_API int sable_vec_add(sable_ctx_t* ctx, sable_dtype_t dtype, void* y, cons
     else
         return fail(SABLE_EINVAL, "bad dtype %d", (int)dtype);
     SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+SABLE_API int sable_max_i32(sable_ctx_t* ctx, const int32_t* a, int64_t n, int32_t* out) {
+    SABLE_CHECK(ctx && out && (a || n == 0), SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(n >= 0, SABLE_EINVAL, "negative length");
+    *out = -1;
+    if (n == 0) return SABLE_OK;
+    void* s;
+    int rc = ctx_scratch(ctx, 256, &s);
+    if (rc) return rc;
+    int* dm = (int*)s;
+    // at most 1024 workgroups: one atomic per wave on one address
+    unsigned g = grid_for(n, 256);
+    if (g > 1024) g = 1024;
+    hipLaunchKernelGGL(k_max_i32_init, dim3(1), dim3(64), 0, ctx->stream, dm);
+    hipLaunchKernelGGL(k_max_i32_reduce, dim3(g), dim3(256), 0, ctx->stream, n, a, dm);
+    SABLE_LAUNCH_CHECK();
+    SABLE_HIP(hipMemcpyAsync(ctx->pinned, dm, 4, hipMemcpyDeviceToHost, ctx->stream));
+    SABLE_HIP(hipStreamSynchronize(ctx->stream));
+    memcpy(out, ctx->pinned, 4);
     return SABLE_OK;
 }
 

@@ -78,6 +78,56 @@ __global__ void k_plan_fill(int32_t N, const int32_t* __restrict__ rp, int64_t C
     for (int64_t b = blo; b <= bhi; b++) blk[b] = (int32_t)i;
 }
 
+// The block count of a plan built on the device without reading nnz on the
+// host: weight W = nnz + C*N, block weight Teff = max(T, ceil(W / cap)) so
+// that at most cap blocks are needed, B = ceil(W / Teff).  Every thread of
+// the fill kernel and of the persistent kernel derives the same B from
+// row_ptr[0] and row_ptr[N].
+__device__ __forceinline__ void dev_plan_shape(int32_t N, const int32_t* __restrict__ rp, int64_t C, int64_t T,
+                                               int64_t cap, int64_t* Teff, int64_t* B) {
+    const int64_t nnz = (int64_t)rp[N] - rp[0];
+    const int64_t W = (nnz > 0 ? nnz : 0) + C * N;
+    int64_t t = (W + cap - 1) / cap;
+    if (t < T) t = T;
+    *Teff = t;
+    *B = W <= 0 ? 0 : (W + t - 1) / t;
+}
+
+__global__ void k_devplan_init(int64_t* shape, int64_t C, int64_t T, int64_t cap, int* err) {
+    if (threadIdx.x == 0) {
+        shape[0] = C;
+        shape[1] = T;
+        shape[2] = cap;
+        *err = 0;
+    }
+}
+
+// k_plan_fill with the shape computed on the device; blk holds cap + 1
+// entries.  A row_ptr that decreases anywhere, or ends below its start,
+// raises *err (the product kernel then writes nothing).
+__global__ void k_plan_fill_dev(int32_t N, const int32_t* __restrict__ rp, int64_t C, int64_t T, int64_t cap,
+                                int32_t* __restrict__ blk, int* __restrict__ err) {
+    int64_t Teff, B;
+    dev_plan_shape(N, rp, C, T, cap, &Teff, &B);
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > N) return;
+    if (i == 0 && rp[N] < rp[0]) atomicOr(err, 1);
+    int64_t base = rp[0];
+    int64_t kprev = -1;
+    if (i > 0) kprev = ((int64_t)rp[i - 1] - base) + C * (i - 1);
+    int64_t blo = kprev < 0 ? 0 : kprev / Teff + 1;
+    int64_t bhi;
+    if (i < N) {
+        int64_t a = rp[i], e = rp[i + 1];
+        if (e < a) atomicOr(err, 1);
+        bhi = ((a - base) + C * i) / Teff;
+        if (bhi > B) bhi = B;
+    } else {
+        bhi = B;
+    }
+    for (int64_t b = blo; b <= bhi; b++) blk[b] = (int32_t)i;
+}
+
 template <typename T>
 __device__ __forceinline__ T block_sum(T v, T* red) {
     v = wave_sum(v);
@@ -282,17 +332,27 @@ static void vec_launch_t(hipStream_t st, int G, int R, int32_t N, const int32_t*
 // the same step as col/val.
 constexpr int RPS_CAP = 520;
 
+// dshape != nullptr: the plan was built on the device (k_plan_fill_dev);
+// dshape = {row weight, block weight, cap} and the block count is derived
+// from row_ptr, with nothing done when *err is set.
 template <typename T, int U>
 __global__ __launch_bounds__(BS) void k_csr_persist(const int32_t* __restrict__ blk, int64_t nblocks,
                                                     const int32_t* __restrict__ rp,
                                                     const int32_t* __restrict__ col,
                                                     const T* __restrict__ val, const T* __restrict__ x,
-                                                    T* __restrict__ y, T beta, int lds_cap) {
+                                                    T* __restrict__ y, T beta, int lds_cap,
+                                                    int32_t N = 0, const int64_t* __restrict__ dshape = nullptr,
+                                                    const int* __restrict__ derr = nullptr) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* prod = reinterpret_cast<T*>(smem_raw);
     __shared__ T red[BS / WAVE];
     __shared__ int32_t rps[RPS_CAP + 1];
     const int tid = threadIdx.x;
+    if (dshape) {
+        if (*derr) return;
+        int64_t Teff;
+        dev_plan_shape(N, rp, dshape[0], dshape[1], dshape[2], &Teff, &nblocks);
+    }
     const int64_t G = gridDim.x;
     const int64_t xcd = blockIdx.x % NUM_XCD, idx = blockIdx.x / NUM_XCD;
     const int64_t gx = G / NUM_XCD + (xcd < G % NUM_XCD ? 1 : 0);  // workgroups on this XCD
@@ -400,14 +460,14 @@ template <typename T>
 __global__ __launch_bounds__(BS) void k_dia(int32_t N, int32_t nd, int32_t stride,
                                             const int32_t* __restrict__ offset,
                                             const T* __restrict__ data, const T* __restrict__ x,
-                                            T* __restrict__ y, T beta) {
+                                            T* __restrict__ y, T beta, int32_t r0, int32_t r1) {
     __shared__ int32_t offs[1024];
     const int ndl = nd < 1024 ? nd : 1024;
     for (int k = threadIdx.x; k < ndl; k += blockDim.x) offs[k] = offset[k];
     __syncthreads();
     const int64_t shiftneg = (int64_t)N - stride;
     constexpr int U = 8;
-    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N;
+    for (int64_t n = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < r1;
          n += (int64_t)gridDim.x * blockDim.x) {
         T t = beta == T(1) ? y[n] : (beta == T(0) ? T(0) : beta * y[n]);
         // U diagonals per step: all loads of a step are issued before the
@@ -446,8 +506,8 @@ __global__ __launch_bounds__(BS) void k_dia(int32_t N, int32_t nd, int32_t strid
 template <typename T>
 __global__ __launch_bounds__(BS) void k_ell(int32_t N, int32_t nc, const int32_t* __restrict__ idx,
                                             const T* __restrict__ data, const T* __restrict__ x,
-                                            T* __restrict__ y, T beta) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+                                            T* __restrict__ y, T beta, int32_t r0, int32_t r1) {
+    for (int64_t i = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < r1;
          i += (int64_t)gridDim.x * blockDim.x) {
         T t = beta == T(1) ? y[i] : (beta == T(0) ? T(0) : beta * y[i]);
         // U slots per step, all loads issued before the products are added
@@ -473,6 +533,68 @@ __global__ __launch_bounds__(BS) void k_ell(int32_t N, int32_t nc, const int32_t
                 if (j + u < nc) t = add_rn(mul_rn(d[u], xv[u]), t);
         }
         y[i] = t;
+    }
+}
+
+// ------------------------------------------------------- row-major layouts
+// The parallel tree's "major: row" storage (par/src/sparse32.js:1840-1880,
+// 1912-1973): row i's slots at [i*w, (i+1)*w).  One thread per row of
+// [r0, r1); t starts from y[i] and every slot adds with a rounded multiply
+// then add, in the reference's order:
+//  * DIA (par/src/sparse_opt_32.wat:8213-8269): diagonals in stored order,
+//    skipping columns outside [0, N);
+//  * ELL (:8292-8346): slots in order, leaving the row at the first
+//    negative index;
+//  * ELL, gs order (:8369-8756), for width >= 4: the first width % 4 slots
+//    into t, the rest into four lane sums l0..l3 (slot s into lane
+//    (s - width % 4) % 4, each lane from 0), then y = (((t + l0) + l1) + l2)
+//    + l3; narrower rows take the plain order.
+template <typename T>
+__global__ __launch_bounds__(BS) void k_dia_rm(int32_t N, int32_t nd, const int32_t* __restrict__ offset,
+                                               const T* __restrict__ data, const T* __restrict__ x,
+                                               T* __restrict__ y, int32_t r0, int32_t r1) {
+    __shared__ int32_t offs[1024];
+    const int ndl = nd < 1024 ? nd : 1024;
+    for (int k = threadIdx.x; k < ndl; k += blockDim.x) offs[k] = offset[k];
+    __syncthreads();
+    for (int64_t i = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < r1;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        T t = y[i];
+        const T* d = data + i * nd;
+        for (int k = 0; k < nd; k++) {
+            const int64_t c = i + (k < 1024 ? offs[k] : offset[k]);
+            if (c >= 0 && c < N) t = add_rn(mul_rn(d[k], x[c]), t);
+        }
+        y[i] = t;
+    }
+}
+
+template <typename T, bool GS>
+__global__ __launch_bounds__(BS) void k_ell_rm(int32_t nc, const int32_t* __restrict__ idx,
+                                               const T* __restrict__ data, const T* __restrict__ x,
+                                               T* __restrict__ y, int32_t r0, int32_t r1) {
+    for (int64_t i = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < r1;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        T t = y[i];
+        const int32_t* ci = idx + i * nc;
+        const T* d = data + i * nc;
+        if (!GS || nc < 4) {
+            for (int k = 0; k < nc; k++) {
+                const int32_t c = ci[k];
+                if (c < 0) break;
+                t = add_rn(mul_rn(d[k], x[c]), t);
+            }
+            y[i] = t;
+        } else {
+            const int h = nc & 3;
+            for (int k = 0; k < h; k++) t = add_rn(mul_rn(d[k], x[ci[k]]), t);
+            T l[4] = {T(0), T(0), T(0), T(0)};
+            for (int k = h; k < nc; k += 4) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) l[q] = add_rn(mul_rn(d[k + q], x[ci[k + q]]), l[q]);
+            }
+            y[i] = add_rn(add_rn(add_rn(add_rn(t, l[0]), l[1]), l[2]), l[3]);
+        }
     }
 }
 
@@ -663,6 +785,44 @@ static int coo_atomic_launch(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, i
     return SABLE_OK;
 }
 
+// Scratch of a device-built plan: the shape {C, T, cap}, the error flag and
+// cap + 1 block boundaries.
+size_t devplan_bytes(int64_t cap) { return 256 + sizeof(int32_t) * (size_t)(cap + 1); }
+
+// y = A.x + beta*y, inner_max times, with the block plan built on the
+// device into `scratch` (devplan_bytes(cap) bytes): no host synchronisation
+// and no allocation.  A malformed row_ptr leaves y unchanged.
+int devplan_spmv(hipStream_t st, int num_cu, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                 const int32_t* col, const void* val, const void* x, void* y, double beta, int inner_max,
+                 void* scratch, int64_t cap) {
+    int T, C;
+    tune_defaults(nullptr, &T, &C);
+    int64_t* shape = (int64_t*)scratch;
+    int* err = (int*)((char*)scratch + 64);
+    int32_t* blk = (int32_t*)((char*)scratch + 256);
+    // shape and flag are written by a kernel (no memset node in captured work)
+    hipLaunchKernelGGL(k_devplan_init, dim3(1), dim3(64), 0, st, shape, (int64_t)C, (int64_t)T, cap, err);
+    hipLaunchKernelGGL(k_plan_fill_dev, dim3(grid_for((int64_t)N + 1, BS)), dim3(BS), 0, st, N, row_ptr, (int64_t)C,
+                       (int64_t)T, cap, blk, err);
+    SABLE_LAUNCH_CHECK();
+    const int lds_cap = lds_cap_for(T);
+    int64_t grid = (int64_t)num_cu * 8;
+    if (grid > cap) grid = cap;
+    if (grid < NUM_XCD) grid = NUM_XCD;
+    for (int it = 0; it < inner_max; it++) {
+        if (dtype == SABLE_F32)
+            hipLaunchKernelGGL((k_csr_persist<float, 8>), dim3((unsigned)grid), dim3(BS), (size_t)lds_cap * 4, st, blk,
+                               (int64_t)0, row_ptr, col, (const float*)val, (const float*)x, (float*)y, (float)beta,
+                               lds_cap, N, (const int64_t*)shape, (const int*)err);
+        else
+            hipLaunchKernelGGL((k_csr_persist<double, 8>), dim3((unsigned)grid), dim3(BS), (size_t)lds_cap * 8, st,
+                               blk, (int64_t)0, row_ptr, col, (const double*)val, (const double*)x, (double*)y, beta,
+                               lds_cap, N, (const int64_t*)shape, (const int*)err);
+        SABLE_LAUNCH_CHECK();
+    }
+    return SABLE_OK;
+}
+
 }  // namespace sable
 
 using namespace sable;
@@ -780,21 +940,15 @@ SABLE_API int sable_spmv_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, c
                              int inner_max) {
     SABLE_CHECK(ctx && row_ptr, SABLE_EINVAL, "NULL argument");
     SABLE_CHECK(N >= 0 && inner_max >= 0, SABLE_EINVAL, "negative size");
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
     if (N == 0 || inner_max == 0) return SABLE_OK;
-    int T, C;
-    tune_defaults(nullptr, &T, &C);
-    int64_t nnz;
-    int rc = read_nnz(ctx, N, row_ptr, &nnz);
-    if (rc) return rc;
-    int64_t B = nblocks_for(nnz, N, T, C);
+    // Block capacity from N alone: matrices whose default-weight plan would
+    // need more blocks get proportionally heavier blocks.
+    const int64_t cap = (int64_t)N / 2 + 1024;
     void* s;
-    rc = ctx_scratch(ctx, sizeof(int32_t) * (size_t)(B + 1) + 256, &s);
+    int rc = ctx_scratch(ctx, devplan_bytes(cap), &s);
     if (rc) return rc;
-    int32_t* blk = (int32_t*)s;
-    int* err = (int*)((char*)s + ((sizeof(int32_t) * (size_t)(B + 1) + 3) & ~(size_t)3));
-    rc = plan_fill(ctx, N, row_ptr, T, C, B, blk, err, false);
-    if (rc) return rc;
-    return dispatch_csr(ctx, dtype, blk, B, row_ptr, col, val, x, y, beta, inner_max, lds_cap_for(T));
+    return devplan_spmv(ctx->stream, ctx->num_cu, dtype, N, row_ptr, col, val, x, y, beta, inner_max, s, cap);
 }
 
 SABLE_API int sable_spmv_coo(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int64_t nnz, const int32_t* row,
@@ -920,26 +1074,54 @@ SABLE_API int sable_spmv_coo_plan(sable_ctx_t* ctx, const sable_coo_plan_t* p, s
     return coo_atomic_launch(ctx, dtype, p->N, p->nnz, row, col, val, x, y, beta, inner_max);
 }
 
-SABLE_API int sable_spmv_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
-                             const int32_t* offset, const void* data, const void* x, void* y, double beta,
-                             int inner_max) {
+SABLE_API int sable_spmv_dia_range(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
+                                   const int32_t* offset, const void* data, int32_t r0, int32_t r1, const void* x,
+                                   void* y, double beta, int inner_max) {
     SABLE_CHECK(ctx, SABLE_EINVAL, "NULL ctx");
     SABLE_CHECK(N >= 0 && nd >= 0 && inner_max >= 0, SABLE_EINVAL, "negative size");
     SABLE_CHECK(stride >= 0 && stride <= N, SABLE_EINVAL, "stride %d outside [0, N=%d]", stride, N);
-    if (N == 0 || inner_max == 0) return SABLE_OK;
-    unsigned g = grid_for(N, BS);
+    SABLE_CHECK(r0 >= 0 && r0 <= r1 && r1 <= N, SABLE_EINVAL, "row range [%d, %d) outside [0, %d)", r0, r1, N);
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    if (r0 == r1 || inner_max == 0) return SABLE_OK;
+    unsigned g = grid_for(r1 - r0, BS);
     // with stride 0 no diagonal stores an entry (and the kernel's clamped
     // loads need stride >= 1): y = beta*y
     const int32_t ndk = stride > 0 ? nd : 0;
     for (int it = 0; it < inner_max; it++) {
         if (dtype == SABLE_F32)
             hipLaunchKernelGGL(k_dia<float>, dim3(g), dim3(BS), 0, ctx->stream, N, ndk, stride, offset,
-                               (const float*)data, (const float*)x, (float*)y, (float)beta);
-        else if (dtype == SABLE_F64)
-            hipLaunchKernelGGL(k_dia<double>, dim3(g), dim3(BS), 0, ctx->stream, N, ndk, stride, offset,
-                               (const double*)data, (const double*)x, (double*)y, beta);
+                               (const float*)data, (const float*)x, (float*)y, (float)beta, r0, r1);
         else
-            return fail(SABLE_EINVAL, "bad dtype");
+            hipLaunchKernelGGL(k_dia<double>, dim3(g), dim3(BS), 0, ctx->stream, N, ndk, stride, offset,
+                               (const double*)data, (const double*)x, (double*)y, beta, r0, r1);
+        SABLE_LAUNCH_CHECK();
+    }
+    return SABLE_OK;
+}
+
+SABLE_API int sable_spmv_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd, int32_t stride,
+                             const int32_t* offset, const void* data, const void* x, void* y, double beta,
+                             int inner_max) {
+    return sable_spmv_dia_range(ctx, dtype, N, nd, stride, offset, data, 0, N, x, y, beta, inner_max);
+}
+
+SABLE_API int sable_spmv_ell_range(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc,
+                                   const int32_t* indices, const void* data, int32_t r0, int32_t r1, const void* x,
+                                   void* y, double beta, int inner_max) {
+    SABLE_CHECK(ctx, SABLE_EINVAL, "NULL ctx");
+    SABLE_CHECK(N >= 0 && nc >= 0 && inner_max >= 0, SABLE_EINVAL, "negative size");
+    SABLE_CHECK(r0 >= 0 && r0 <= r1 && r1 <= N, SABLE_EINVAL, "row range [%d, %d) outside [0, %d)", r0, r1, N);
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    // The reference returns without touching y when nc or N is not positive.
+    if (r0 == r1 || nc == 0 || inner_max == 0) return SABLE_OK;
+    unsigned g = grid_for(r1 - r0, BS);
+    for (int it = 0; it < inner_max; it++) {
+        if (dtype == SABLE_F32)
+            hipLaunchKernelGGL(k_ell<float>, dim3(g), dim3(BS), 0, ctx->stream, N, nc, indices, (const float*)data,
+                               (const float*)x, (float*)y, (float)beta, r0, r1);
+        else
+            hipLaunchKernelGGL(k_ell<double>, dim3(g), dim3(BS), 0, ctx->stream, N, nc, indices, (const double*)data,
+                               (const double*)x, (double*)y, beta, r0, r1);
         SABLE_LAUNCH_CHECK();
     }
     return SABLE_OK;
@@ -947,20 +1129,57 @@ SABLE_API int sable_spmv_dia(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, i
 
 SABLE_API int sable_spmv_ell(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc, const int32_t* indices,
                              const void* data, const void* x, void* y, double beta, int inner_max) {
+    return sable_spmv_ell_range(ctx, dtype, N, nc, indices, data, 0, N, x, y, beta, inner_max);
+}
+
+SABLE_API int sable_spmv_dia_rowmajor(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nd,
+                                      const int32_t* offset, const void* data, int32_t r0, int32_t r1,
+                                      const void* x, void* y, int inner_max) {
     SABLE_CHECK(ctx, SABLE_EINVAL, "NULL ctx");
-    SABLE_CHECK(N >= 0 && nc >= 0 && inner_max >= 0, SABLE_EINVAL, "negative size");
-    // The reference returns without touching y when nc or N is not positive.
-    if (N == 0 || nc == 0 || inner_max == 0) return SABLE_OK;
-    unsigned g = grid_for(N, BS);
+    SABLE_CHECK(N >= 0 && nd >= 0 && inner_max >= 0, SABLE_EINVAL, "negative size");
+    SABLE_CHECK(r0 >= 0 && r1 <= N, SABLE_EINVAL, "row range [%d, %d) outside [0, %d)", r0, r1, N);
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    // the reference returns at once for an empty range or no diagonals
+    if (r0 >= r1 || nd == 0 || inner_max == 0) return SABLE_OK;
+    unsigned g = grid_for(r1 - r0, BS);
     for (int it = 0; it < inner_max; it++) {
         if (dtype == SABLE_F32)
-            hipLaunchKernelGGL(k_ell<float>, dim3(g), dim3(BS), 0, ctx->stream, N, nc, indices,
-                               (const float*)data, (const float*)x, (float*)y, (float)beta);
-        else if (dtype == SABLE_F64)
-            hipLaunchKernelGGL(k_ell<double>, dim3(g), dim3(BS), 0, ctx->stream, N, nc, indices,
-                               (const double*)data, (const double*)x, (double*)y, beta);
+            hipLaunchKernelGGL(k_dia_rm<float>, dim3(g), dim3(BS), 0, ctx->stream, N, nd, offset, (const float*)data,
+                               (const float*)x, (float*)y, r0, r1);
         else
-            return fail(SABLE_EINVAL, "bad dtype");
+            hipLaunchKernelGGL(k_dia_rm<double>, dim3(g), dim3(BS), 0, ctx->stream, N, nd, offset,
+                               (const double*)data, (const double*)x, (double*)y, r0, r1);
+        SABLE_LAUNCH_CHECK();
+    }
+    return SABLE_OK;
+}
+
+SABLE_API int sable_spmv_ell_rowmajor(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t nc,
+                                      const int32_t* indices, const void* data, int32_t r0, int32_t r1,
+                                      const void* x, void* y, int order, int inner_max) {
+    SABLE_CHECK(ctx, SABLE_EINVAL, "NULL ctx");
+    SABLE_CHECK(N >= 0 && nc >= 0 && inner_max >= 0, SABLE_EINVAL, "negative size");
+    SABLE_CHECK(r0 >= 0 && r1 <= N, SABLE_EINVAL, "row range [%d, %d) outside [0, %d)", r0, r1, N);
+    SABLE_CHECK(order == 0 || order == 1, SABLE_EINVAL, "order must be 0 (plain) or 1 (gs)");
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    if (r0 >= r1 || nc == 0 || inner_max == 0) return SABLE_OK;
+    unsigned g = grid_for(r1 - r0, BS);
+    for (int it = 0; it < inner_max; it++) {
+        if (dtype == SABLE_F32) {
+            if (order)
+                hipLaunchKernelGGL((k_ell_rm<float, true>), dim3(g), dim3(BS), 0, ctx->stream, nc, indices,
+                                   (const float*)data, (const float*)x, (float*)y, r0, r1);
+            else
+                hipLaunchKernelGGL((k_ell_rm<float, false>), dim3(g), dim3(BS), 0, ctx->stream, nc, indices,
+                                   (const float*)data, (const float*)x, (float*)y, r0, r1);
+        } else {
+            if (order)
+                hipLaunchKernelGGL((k_ell_rm<double, true>), dim3(g), dim3(BS), 0, ctx->stream, nc, indices,
+                                   (const double*)data, (const double*)x, (double*)y, r0, r1);
+            else
+                hipLaunchKernelGGL((k_ell_rm<double, false>), dim3(g), dim3(BS), 0, ctx->stream, nc, indices,
+                                   (const double*)data, (const double*)x, (double*)y, r0, r1);
+        }
         SABLE_LAUNCH_CHECK();
     }
     return SABLE_OK;

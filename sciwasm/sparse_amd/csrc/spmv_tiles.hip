@@ -64,13 +64,13 @@ __global__ void k_tile_rows(int32_t N, const int32_t* __restrict__ rp, int32_t* 
 // entries are contiguous, row tile after row tile, each tile by column, so
 // that the workgroups working on one panel at the same time read one
 // compact region of memory
-__global__ void k_tile_keys(int64_t nnz, int32_t N, const int32_t* __restrict__ rows, const int32_t* __restrict__ col,
+__global__ void k_tile_keys(int64_t nnz, int32_t ncols, const int32_t* __restrict__ rows, const int32_t* __restrict__ col,
                             int32_t tile_rows, int col_bits, int32_t ntr, uint64_t* __restrict__ keys,
                             uint32_t* __restrict__ idx, int* __restrict__ err) {
     const uint32_t cmask = (1u << col_bits) - 1u;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
         const int32_t c = col[k];
-        if (c < 0 || c >= N) {
+        if (c < 0 || c >= ncols) {
             atomicOr(err, 1);
             keys[k] = 0;
         } else {
@@ -452,8 +452,14 @@ extern "C" {
 SABLE_API int sable_csr_tiles_create(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
                                      const int32_t* col, const void* val, int32_t tile_rows, int32_t panel_cols,
                                      sable_csr_tiles_t** out) {
+    return sable_csr_tiles_create_rect(ctx, dtype, N, N, row_ptr, col, val, tile_rows, panel_cols, out);
+}
+
+SABLE_API int sable_csr_tiles_create_rect(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int32_t ncols,
+                                          const int32_t* row_ptr, const int32_t* col, const void* val,
+                                          int32_t tile_rows, int32_t panel_cols, sable_csr_tiles_t** out) {
     SABLE_CHECK(ctx && out && row_ptr && col && val, SABLE_EINVAL, "NULL argument");
-    SABLE_CHECK(N > 0, SABLE_EINVAL, "N must be positive");
+    SABLE_CHECK(N > 0 && ncols > 0, SABLE_EINVAL, "N and ncols must be positive");
     SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
     *out = nullptr;
     const size_t vs = dtype == SABLE_F64 ? 8 : 4;
@@ -484,7 +490,7 @@ SABLE_API int sable_csr_tiles_create(sable_ctx_t* ctx, sable_dtype_t dtype, int3
     p->row_bits = rb;
     p->col_bits = cb;
     p->ntr = (int32_t)(((int64_t)N + tile_rows - 1) / tile_rows);
-    p->npanels = (int32_t)(((int64_t)N + panel_cols - 1) / panel_cols);
+    p->npanels = (int32_t)(((int64_t)ncols + panel_cols - 1) / panel_cols);
     const int64_t ntiles = (int64_t)p->ntr * p->npanels;
     SABLE_CHECK(ntiles < (int64_t)1 << 30, SABLE_EINVAL, "too many tiles");
     // build buffers: rows, keys, idx and the sort's work space
@@ -516,13 +522,13 @@ SABLE_API int sable_csr_tiles_create(sable_ctx_t* ctx, sable_dtype_t dtype, int3
     SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
     hipLaunchKernelGGL(k_tile_rows, dim3(gs(N)), dim3(256), 0, st, N, row_ptr, rows);
     if (nnz > 0)
-        hipLaunchKernelGGL(k_tile_keys, dim3(gs(nnz)), dim3(256), 0, st, nnz, N, rows, col, tile_rows, cb, p->ntr,
+        hipLaunchKernelGGL(k_tile_keys, dim3(gs(nnz)), dim3(256), 0, st, nnz, ncols, rows, col, tile_rows, cb, p->ntr,
                            keys, idx, err);
     if (hipGetLastError() != hipSuccess) return cleanup(fail(SABLE_EHIP, "tile key launch failed"));
     if (hipMemcpyAsync(ctx->pinned, err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return cleanup(fail(SABLE_EHIP, "tile key check failed"));
-    if (*(int*)ctx->pinned) return cleanup(fail(SABLE_EMALFORMED, "column index outside [0, N)"));
+    if (*(int*)ctx->pinned) return cleanup(fail(SABLE_EMALFORMED, "column index outside [0, ncols)"));
     if (nnz > 0) {
         const int rc = sort_pairs_u64(st, keys, idx, nnz, bits_for(ntiles) + cb, stmp);
         if (rc) return cleanup(rc);

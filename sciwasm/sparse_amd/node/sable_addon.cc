@@ -7,26 +7,50 @@
 //   * malloc_instance.exports._malloc/_free returning such offsets, and
 //   * sparse_instance.exports.<kernel>(offsets..., sizes...).
 // This addon provides the same three things: `wasmMemory.buffer` is an
-// ArrayBuffer over host memory owned by the addon (pinned when possible),
-// `malloc_exports` allocates inside it, and `sparse_opt_32` / `sparse_opt_64`
-// carry every hot-path export with the reference's names and argument
-// orders.  Each kernel call mirrors the arrays it reads to HBM, runs the HIP
-// kernel and copies the outputs back into the linear memory.  Matrix arrays
-// are cached on the device keyed by (offset, length) and revalidated with a
-// 64-bit hash of their bytes, so repeated SpMV calls on an unchanged matrix
-// move only x and y.
+// ArrayBuffer over host memory owned by the addon, `malloc_exports`
+// allocates inside it, and `sparse_opt_32` / `sparse_opt_64` (and the
+// parallel tree's modules) carry every hot-path export with the reference's
+// names and argument orders.
+//
+// Matrices stay resident on the device between calls:
+//  * every array an export only reads is mirrored to HBM once, keyed by
+//    (offset, length); the mirror stays valid while its bytes are unchanged;
+//  * changes are found by write tracking, not by hashing: the whole pages of
+//    a mirrored array are made read-only, and the first write to such a page
+//    (from JS through a typed array, or from any thread) takes a fault that
+//    the addon's SIGSEGV handler turns into "page written": it counts the
+//    write on the page and makes the page writable again.  A call re-checks a
+//    mirror in O(1) when no page anywhere was written since its last check,
+//    else by summing the write counts of its pages; the partial pages at the
+//    two ends of an array are hashed (at most 8 KiB);
+//  * a CSR or COO matrix keeps an autotuned product schedule
+//    (sable_csr_auto_*) built on its first product and rebuilt only when one
+//    of its arrays changes;
+//  * x and y move per call through two grow-only device vectors and the
+//    context's pinned staging buffers; nothing is allocated per product.
+// SABLE_WRITE_TRACKING=0 replaces the tracking by a full hash of each
+// mirrored array on every call; SABLE_LINEAR_PINNED=1 places the linear
+// memory in pinned memory (which is then never write-protected).
 #include <node_api.h>
 
+#include <signal.h>
+#include <sys/mman.h>
+#include <unistd.h>
+
+#include <atomic>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
 #include "../../../include/sable_sparse.h"
+#include "write_track.h"
 
 namespace {
 
@@ -35,26 +59,38 @@ struct Linear {
     uint8_t* base = nullptr;
     size_t size = 0;
     bool pinned = false;
-    std::map<size_t, size_t> used;  // offset -> bytes
+    bool mapped = false;
+    std::map<size_t, size_t> used;   // offset -> bytes
     std::map<size_t, size_t> freeb;  // offset -> bytes
 };
 Linear g_mem;
 sable_ctx_t* g_ctx = nullptr;
 
+// ------------------------------------------------------------ write tracking
+// Page-granular write tracking of the linear memory (write_track.h).
+using sable_track::full_pages;
+using sable_track::protect_pages;
+using sable_track::unprotect_range;
+using sable_track::wsum;
+
 bool init_linear(size_t bytes) {
+    bytes = (bytes + 65535) & ~(size_t)65535;
+    const char* pin = getenv("SABLE_LINEAR_PINNED");
     void* p = nullptr;
-    if (sable_host_alloc(bytes, &p) == SABLE_OK) {
+    if (pin && atoi(pin) == 1 && sable_host_alloc(bytes, &p) == SABLE_OK) {
         g_mem.pinned = true;
+        memset(p, 0, bytes);
     } else {
-        p = aligned_alloc(4096, (bytes + 4095) & ~(size_t)4095);
-        g_mem.pinned = false;
+        p = mmap(nullptr, bytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+        if (p == MAP_FAILED) return false;
+        g_mem.mapped = true;  // anonymous memory reads as zero
     }
-    if (!p) return false;
-    memset(p, 0, bytes);
     g_mem.base = (uint8_t*)p;
     g_mem.size = bytes;
     // offset 0 stays reserved so that 0 can mean "no allocation"
     g_mem.freeb[64] = bytes - 64;
+    const char* wt = getenv("SABLE_WRITE_TRACKING");
+    if (g_mem.mapped && !(wt && atoi(wt) == 0)) sable_track::init(g_mem.base, g_mem.size);
     return true;
 }
 
@@ -108,14 +144,6 @@ uint64_t hash_bytes(const uint8_t* p, size_t n) {
     return h;
 }
 
-struct Mirror {
-    void* d = nullptr;
-    size_t bytes = 0;
-    uint64_t hash = 0;
-};
-std::map<std::pair<size_t, size_t>, Mirror> g_cache;  // read-only inputs
-std::vector<std::pair<void*, size_t>> g_scratch;      // per-call vectors
-
 struct Err {
     std::string msg;
 };
@@ -132,33 +160,138 @@ void check_range(size_t off, size_t bytes) {
     if (off > g_mem.size || bytes > g_mem.size - off) throw Err{"offset outside the linear memory"};
 }
 
-// Device copy of a read-only region, reused while its bytes are unchanged.
-void* mirror(size_t off, size_t bytes) {
-    check_range(off, bytes);
-    const uint64_t h = hash_bytes(g_mem.base + off, bytes);
-    Mirror& m = g_cache[{off, bytes}];
-    if (m.d && m.hash == h) return m.d;
-    if (!m.d) check(sable_device_alloc(g_ctx, bytes ? bytes : 64, &m.d), "device alloc");
-    check(sable_memcpy_h2d(g_ctx, m.d, g_mem.base + off, bytes), "upload");
-    m.bytes = bytes;
-    m.hash = h;
-    return m.d;
+void upload(void* d, size_t off, size_t bytes) {
+    if (!bytes) return;
+    if (g_mem.pinned)
+        check(sable_memcpy_h2d(g_ctx, d, g_mem.base + off, bytes), "upload");
+    else
+        check(sable_memcpy_h2d_staged(g_ctx, d, g_mem.base + off, bytes), "upload");
 }
 
-// Fresh device copy of a region for this call (inputs that change per call,
-// or outputs); `upload` copies the current host bytes in.
-void* scratch(size_t off, size_t bytes, bool upload) {
+void download_raw(size_t off, const void* d, size_t bytes) {
+    check_range(off, bytes);
+    if (!bytes) return;
+    unprotect_range(off, bytes);
+    if (g_mem.pinned)
+        check(sable_memcpy_d2h(g_ctx, g_mem.base + off, d, bytes), "download");
+    else
+        check(sable_memcpy_d2h_staged(g_ctx, g_mem.base + off, d, bytes), "download");
+}
+
+// A device copy of a region that exports only read.  `version` changes on
+// every (re)upload, so that plans built over a mirror know when to rebuild.
+struct Mirror {
+    void* d = nullptr;
+    size_t off = 0, bytes = 0;
+    uint64_t version = 0;
+    bool tracked = false;
+    size_t p0 = 0, p1 = 0;   // whole pages, write-protected
+    uint64_t sum = 0;        // their write counts at upload
+    uint64_t seen = 0;       // g_faults when last found valid
+    uint64_t hash = 0;       // partial pages (tracked) or the whole region
+};
+std::map<std::pair<size_t, size_t>, std::unique_ptr<Mirror>> g_cache;
+uint64_t g_version = 0;
+
+uint64_t edge_hash(const Mirror& m) {
+    const size_t a = m.p0 << sable_track::g_page_shift, b = m.p1 << sable_track::g_page_shift;
+    if (a >= b) return hash_bytes(g_mem.base + m.off, m.bytes);
+    return hash_bytes(g_mem.base + m.off, a - m.off) * 31u + hash_bytes(g_mem.base + b, m.off + m.bytes - b);
+}
+
+bool mirror_valid(Mirror& m) {
+    if (!m.tracked) return hash_bytes(g_mem.base + m.off, m.bytes) == m.hash;
+    if (edge_hash(m) != m.hash) return false;
+    const uint64_t f = sable_track::faults();
+    if (f == m.seen) return true;
+    if (wsum(m.p0, m.p1) != m.sum) return false;
+    m.seen = f;
+    return true;
+}
+
+uint64_t g_uploads = 0, g_upload_bytes = 0;  // mirror uploads (statistics)
+int g_last_sched = -1;                         // SABLE_SCHED_* of the last plan built
+double g_last_sched_ms = 0;
+
+void fill_mirror(Mirror& m) {
+    // order: counts first, then protection, then the copy, so that a write
+    // racing with the copy is always seen by the next check
+    g_uploads++;
+    g_upload_bytes += m.bytes;
+    m.version = ++g_version;
+    m.seen = sable_track::faults();
+    if (m.tracked) {
+        m.sum = wsum(m.p0, m.p1);
+        protect_pages(m.p0, m.p1);
+    }
+    m.hash = m.tracked ? edge_hash(m) : hash_bytes(g_mem.base + m.off, m.bytes);
+    upload(m.d, m.off, m.bytes);
+}
+
+Mirror& mirror_rec(size_t off, size_t bytes) {
+    check_range(off, bytes);
+    auto& slot = g_cache[{off, bytes}];
+    if (slot && slot->d && mirror_valid(*slot)) return *slot;
+    if (!slot) {
+        slot.reset(new Mirror());
+        slot->off = off;
+        slot->bytes = bytes;
+        // arrays of 16 KiB or more are write-tracked; smaller ones hashed
+        slot->tracked = sable_track::g_on && bytes >= 4 * sable_track::g_page;
+        if (slot->tracked) full_pages(off, bytes, &slot->p0, &slot->p1);
+        check(sable_device_alloc(g_ctx, bytes ? bytes : 64, &slot->d), "device alloc");
+    }
+    fill_mirror(*slot);
+    return *slot;
+}
+
+void* mirror(size_t off, size_t bytes) { return mirror_rec(off, bytes).d; }
+
+// After an export wrote `bytes` at `off` from device memory d (the bytes
+// are now the same on both sides), d becomes that region's mirror, so
+// that a later export reading the result does not upload it again.
+void adopt(size_t off, size_t bytes, void* d) {
+    auto& slot = g_cache[{off, bytes}];
+    if (slot && slot->d) check(sable_device_free(g_ctx, slot->d), "device free");
+    slot.reset(new Mirror());
+    Mirror& m = *slot;
+    m.off = off;
+    m.bytes = bytes;
+    m.d = d;
+    m.tracked = sable_track::g_on && bytes >= 4 * sable_track::g_page;
+    if (m.tracked) full_pages(off, bytes, &m.p0, &m.p1);
+    m.version = ++g_version;
+    m.seen = sable_track::faults();
+    if (m.tracked) {
+        m.sum = wsum(m.p0, m.p1);
+        protect_pages(m.p0, m.p1);
+    }
+    m.hash = m.tracked ? edge_hash(m) : hash_bytes(g_mem.base + off, bytes);
+}
+
+// Fresh device buffers for one call's outputs and per-call inputs.
+std::vector<std::pair<void*, size_t>> g_scratch;
+
+void* scratch(size_t off, size_t bytes, bool up) {
     check_range(off, bytes);
     void* d = nullptr;
     check(sable_device_alloc(g_ctx, bytes ? bytes : 64, &d), "device alloc");
     g_scratch.push_back({d, bytes});
-    if (upload) check(sable_memcpy_h2d(g_ctx, d, g_mem.base + off, bytes), "upload");
+    if (up) upload(d, off, bytes);
     return d;
 }
 
+// Device bytes to the linear memory.  A whole per-call output buffer
+// (from scratch) is kept as the mirror of the region it was written to.
 void download(size_t off, const void* d, size_t bytes) {
-    check_range(off, bytes);
-    check(sable_memcpy_d2h(g_ctx, g_mem.base + off, d, bytes), "download");
+    download_raw(off, d, bytes);
+    for (auto it = g_scratch.begin(); it != g_scratch.end(); ++it)
+        if (it->first == d && it->second == bytes) {
+            void* keep = it->first;
+            g_scratch.erase(it);
+            adopt(off, bytes, keep);
+            break;
+        }
 }
 
 void release_scratch() {
@@ -166,29 +299,98 @@ void release_scratch() {
     g_scratch.clear();
 }
 
+// Two grow-only device vectors for x and y of the products.
+struct VecPool {
+    void* d[2] = {nullptr, nullptr};
+    size_t bytes[2] = {0, 0};
+} g_vec;
+
+void* vec(int k, size_t bytes) {
+    if (bytes > g_vec.bytes[k]) {
+        if (g_vec.d[k]) check(sable_device_free(g_ctx, g_vec.d[k]), "device free");
+        g_vec.d[k] = nullptr;
+        g_vec.bytes[k] = 0;
+        const size_t want = bytes + bytes / 4 + 256;
+        check(sable_device_alloc(g_ctx, want, &g_vec.d[k]), "device alloc");
+        g_vec.bytes[k] = want;
+    }
+    return g_vec.d[k];
+}
+
+void* vec_in(int k, size_t off, size_t bytes) {
+    check_range(off, bytes);
+    void* d = vec(k, bytes);
+    upload(d, off, bytes);
+    return d;
+}
+
 const int32_t* host_i32(size_t off, size_t count) {
     check_range(off, count * 4);
     return (const int32_t*)(g_mem.base + off);
 }
 
-int32_t max_of(size_t off, size_t count) {
-    const int32_t* p = host_i32(off, count);
-    int32_t m = -1;
-    for (size_t i = 0; i < count; i++) m = p[i] > m ? p[i] : m;
-    return m;
-}
+struct ResidentBase {
+    virtual ~ResidentBase() {}
+};
+std::map<std::tuple<int, size_t, size_t, size_t, int64_t>, std::unique_ptr<ResidentBase>> g_resident;
 
 void drop_mirrors_in(size_t off, size_t bytes) {
     // outputs written into the linear memory invalidate overlapping mirrors
     for (auto it = g_cache.begin(); it != g_cache.end();) {
         const size_t a = it->first.first, b = a + it->first.second;
         if (a < off + bytes && off < b) {
-            sable_device_free(g_ctx, it->second.d);
+            sable_device_free(g_ctx, it->second->d);
             it = g_cache.erase(it);
         } else {
             ++it;
         }
     }
+}
+
+// Resident matrices whose arrays lie in [off, off + bytes) (freed memory).
+void drop_resident_in(size_t off, size_t bytes) {
+    for (auto it = g_resident.begin(); it != g_resident.end();) {
+        const size_t a = std::get<1>(it->first), b = std::get<2>(it->first), c = std::get<3>(it->first);
+        auto in = [&](size_t v) { return v >= off && v < off + bytes; };
+        if (in(a) || in(b) || in(c)) it = g_resident.erase(it);
+        else ++it;
+    }
+}
+
+// ------------------------------------------------------------ resident matrices
+// A CSR matrix (or a CSR built from a COO) with its autotuned schedule.
+struct ResidentCsr : ResidentBase {
+    sable_csr_auto_t* plan = nullptr;
+    uint64_t v[3] = {0, 0, 0};  // mirror versions the plan was built on
+    int32_t ncols = 0;
+    // COO input: the sorted CSR owned here, or the COO plan's row heads
+    sable_coo_plan_t* coo = nullptr;
+    void* own[3] = {nullptr, nullptr, nullptr};
+    int32_t N = 0;
+    ~ResidentCsr() override { clear(); }
+    void clear() {
+        if (plan) sable_csr_auto_destroy(plan);
+        if (coo) sable_coo_plan_destroy(coo);
+        for (void*& p : own)
+            if (p) {
+                sable_device_free(g_ctx, p);
+                p = nullptr;
+            }
+        plan = nullptr;
+        coo = nullptr;
+    }
+};
+
+int auto_flags() {
+    const char* e = getenv("SABLE_DETERMINISTIC");
+    return (e && atoi(e) == 1) ? SABLE_AUTO_DETERMINISTIC : 0;
+}
+
+template <typename R>
+R& resident(int kind, size_t a, size_t b, size_t c, int64_t n) {
+    auto& slot = g_resident[std::make_tuple(kind, a, b, c, n)];
+    if (!slot) slot.reset(new R());
+    return *static_cast<R*>(slot.get());
 }
 
 // ------------------------------------------------------------ N-API helpers
@@ -273,6 +475,95 @@ Prec g_p64{SABLE_F64, 8, true, 0};
 Prec g_par32{SABLE_F32, 4, false, 1};
 Prec g_par64{SABLE_F64, 8, false, 1};
 
+// The resident CSR (row_ptr, col, val) of N rows with its schedule; row_ptr
+// may be a slice: col/val are mirrored from the first entry it references
+// and the device base pointers shifted back, so that the absolute row_ptr
+// values index them.
+ResidentCsr& csr_resident(Prec* P, size_t rp, size_t col, size_t val, int32_t N) {
+    const int32_t* hrp = host_i32(rp, (size_t)N + 1);
+    const int64_t j0 = hrp[0], j1 = hrp[N];
+    if (j0 < 0 || j1 < j0) throw Err{"row_ptr is not monotone"};
+    Mirror& mr = mirror_rec(rp, 4 * ((size_t)N + 1));
+    Mirror& mc = mirror_rec(col + 4 * j0, 4 * (j1 - j0));
+    Mirror& mv = mirror_rec(val + P->vs * j0, P->vs * (j1 - j0));
+    ResidentCsr& R = resident<ResidentCsr>((int)P->dt, rp, col, val, N);
+    if (!R.plan || R.v[0] != mr.version || R.v[1] != mc.version || R.v[2] != mv.version) {
+        R.clear();
+        const int32_t* dc = (const int32_t*)((char*)mc.d - 4 * j0);
+        const void* dv = (const char*)mv.d - P->vs * j0;
+        check(sable_csr_auto_create(g_ctx, P->dt, N, 0, (const int32_t*)mr.d, dc, dv, auto_flags(), &R.plan),
+              "CSR schedule");
+        sable_csr_auto_info_t inf;
+        check(sable_csr_auto_info(R.plan, &inf), "CSR schedule");
+        R.ncols = inf.ncols;
+        g_last_sched = inf.schedule;
+        g_last_sched_ms = inf.ms;
+        R.N = N;
+        R.v[0] = mr.version;
+        R.v[1] = mc.version;
+        R.v[2] = mv.version;
+    }
+    return R;
+}
+
+// The resident COO (row, col, val) of nnz entries: row-sorted input keeps
+// its row heads (the COO's own col/val form the CSR), other input is sorted
+// once on the device into a CSR owned here (duplicates kept, so every
+// entry's product is still added to its row).  N = largest row + 1.
+ResidentCsr& coo_resident(Prec* P, size_t row, size_t col, size_t val, int64_t nnz) {
+    Mirror& mr = mirror_rec(row, 4 * (size_t)nnz);
+    Mirror& mc = mirror_rec(col, 4 * (size_t)nnz);
+    Mirror& mv = mirror_rec(val, P->vs * (size_t)nnz);
+    ResidentCsr& R = resident<ResidentCsr>(16 + (int)P->dt, row, col, val, nnz);
+    if (!R.plan || R.v[0] != mr.version || R.v[1] != mc.version || R.v[2] != mv.version) {
+        R.clear();
+        int32_t mx = -1;
+        check(sable_max_i32(g_ctx, (const int32_t*)mr.d, nnz, &mx), "spmv_coo");
+        const int32_t N = mx + 1;
+        if (N <= 0) throw Err{"spmv_coo: negative row index"};
+        check(sable_coo_plan_create(g_ctx, N, nnz, (const int32_t*)mr.d, &R.coo), "spmv_coo");
+        int sorted = 0;
+        int64_t nb = 0;
+        check(sable_coo_plan_info(R.coo, &sorted, &nb), "spmv_coo");
+        const int32_t* rp;
+        const int32_t* c = (const int32_t*)mc.d;
+        const void* v = mv.d;
+        if (sorted) {
+            check(sable_coo_plan_row_ptr(R.coo, &rp), "spmv_coo");
+        } else {
+            check(sable_device_alloc(g_ctx, 4 * ((size_t)N + 1), &R.own[0]), "device alloc");
+            check(sable_device_alloc(g_ctx, 4 * (size_t)nnz, &R.own[1]), "device alloc");
+            check(sable_device_alloc(g_ctx, P->vs * (size_t)nnz, &R.own[2]), "device alloc");
+            check(sable_coo_csr(g_ctx, P->dt, N, nnz, (const int32_t*)mr.d, c, v, (int32_t*)R.own[0],
+                                (int32_t*)R.own[1], R.own[2]),
+                  "spmv_coo");
+            rp = (const int32_t*)R.own[0];
+            c = (const int32_t*)R.own[1];
+            v = R.own[2];
+        }
+        check(sable_csr_auto_create(g_ctx, P->dt, N, 0, rp, c, v, auto_flags(), &R.plan), "COO schedule");
+        sable_csr_auto_info_t inf;
+        check(sable_csr_auto_info(R.plan, &inf), "COO schedule");
+        R.ncols = inf.ncols;
+        g_last_sched = inf.schedule;
+        g_last_sched_ms = inf.ms;
+        R.N = N;
+        R.v[0] = mr.version;
+        R.v[1] = mc.version;
+        R.v[2] = mv.version;
+    }
+    return R;
+}
+
+// One product call: x (ncols values) and y (N values) in, y += A.x
+// inner_max times on the resident schedule, y out.
+void run_resident(Prec* P, ResidentCsr& R, size_t x, size_t y, int im) {
+    void* dx = vec_in(0, x, P->vs * (size_t)R.ncols);
+    void* dy = vec_in(1, y, P->vs * (size_t)R.N);
+    check(sable_spmv_csr_auto(g_ctx, R.plan, dx, dy, 1.0, im), "spmv");
+    download(y, dy, P->vs * (size_t)R.N);
+}
+
 // spmv_coo(row, col, val, x, y, len[, inner_max])
 napi_value spmv_coo_impl(napi_env env, napi_callback_info info, bool wrapper) {
     Prec* P = prec_of(env, info);
@@ -283,16 +574,7 @@ napi_value spmv_coo_impl(napi_env env, napi_callback_info info, bool wrapper) {
         const int64_t nnz = a.i(5);
         const int im = wrapper ? (int)a.i(6) : 1;
         if (nnz <= 0 || im <= 0) return undef(env);
-        const int32_t N = max_of(row, nnz) + 1, ncol = max_of(col, nnz) + 1;
-        void* dr = mirror(row, 4 * nnz);
-        void* dc = mirror(col, 4 * nnz);
-        void* dv = mirror(val, P->vs * nnz);
-        void* dx = scratch(x, P->vs * ncol, true);
-        void* dy = scratch(y, P->vs * N, true);
-        check(sable_spmv_coo(g_ctx, P->dt, N, nnz, (const int32_t*)dr, (const int32_t*)dc, dv, dx, dy, 1.0, im),
-              "spmv_coo");
-        download(y, dy, P->vs * N);
-        release_scratch();
+        run_resident(P, coo_resident(P, row, col, val, nnz), x, y, im);
         return undef(env);
     });
 }
@@ -307,21 +589,34 @@ napi_value spmv_csr_impl(napi_env env, napi_callback_info info, bool wrapper) {
         const int32_t N = (int32_t)a.i(5);
         const int im = wrapper ? (int)a.i(6) : 1;
         if (N <= 0 || im <= 0) return undef(env);
-        const int32_t* hrp = host_i32(rp, N + 1);
-        const int64_t j0 = hrp[0], j1 = hrp[N];
-        if (j1 < j0) throw Err{"row_ptr is not monotone"};
-        const int32_t ncol = j1 > j0 ? max_of(col + 4 * j0, j1 - j0) + 1 : 1;
-        void* drp = mirror(rp, 4 * (size_t)(N + 1));
-        // col/val mirrored from the first referenced entry; the base pointer is
-        // shifted back so that the absolute row_ptr values index it
-        char* dc = (char*)mirror(col + 4 * j0, 4 * (j1 - j0)) - 4 * j0;
-        char* dv = (char*)mirror(val + P->vs * j0, P->vs * (j1 - j0)) - P->vs * j0;
-        void* dx = scratch(x, P->vs * ncol, true);
-        void* dy = scratch(y, P->vs * N, true);
-        check(sable_spmv_csr(g_ctx, P->dt, N, (const int32_t*)drp, (const int32_t*)dc, dv, dx, dy, 1.0, im),
-              "spmv_csr");
-        download(y, dy, P->vs * N);
-        release_scratch();
+        run_resident(P, csr_resident(P, rp, col, val, N), x, y, im);
+        return undef(env);
+    });
+}
+
+// Parallel-tree CSR variants with short-row groups
+// (par/src/sparse_opt_32.wat:1791-1989 and the unroll/gs forms):
+// (id, rowptr, col, val, x, y, len, n1, n2, n3, inner_max) covers n1 rows of
+// one entry, n2 of two, n3 of three, then len rows of any length, all from
+// rowptr on; each row adds its entries to y in order, so the product is
+// the plain CSR product over the n1 + n2 + n3 + len rows.  The group sizes
+// are checked against rowptr (the reference would read them as given).
+napi_value spmv_csr_short_impl(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 10);
+        ensure_ctx();
+        const size_t rp = a.i(0), col = a.i(1), val = a.i(2), x = a.i(3), y = a.i(4);
+        const int64_t len = a.i(5), n1 = a.i(6), n2 = a.i(7), n3 = a.i(8);
+        const int im = (int)a.i(9);
+        if (len < 0 || n1 < 0 || n2 < 0 || n3 < 0) throw Err{"negative row count"};
+        const int64_t N = n1 + n2 + n3 + len;
+        if (N <= 0 || im <= 0) return undef(env);
+        if (N > 0x7fffffff) throw Err{"too many rows"};
+        const int32_t* h = host_i32(rp, (size_t)N + 1);
+        if (h[n1] - h[0] != n1 || h[n1 + n2] - h[n1] != 2 * n2 || h[n1 + n2 + n3] - h[n1 + n2] != 3 * n3)
+            throw Err{"short-row group sizes do not match rowptr"};
+        run_resident(P, csr_resident(P, rp, col, val, (int32_t)N), x, y, im);
         return undef(env);
     });
 }
@@ -338,13 +633,24 @@ napi_value spmv_dia_impl(napi_env env, napi_callback_info info, bool wrapper) {
         if (nd <= 0 || N <= 0 || im <= 0) return undef(env);
         void* doff = mirror(off, 4 * (size_t)nd);
         void* dd = mirror(data, P->vs * (size_t)nd * stride);
-        void* dx = scratch(x, P->vs * N, true);
-        void* dy = scratch(y, P->vs * N, true);
+        void* dx = vec_in(0, x, P->vs * N);
+        void* dy = vec_in(1, y, P->vs * N);
         check(sable_spmv_dia(g_ctx, P->dt, N, nd, stride, (const int32_t*)doff, dd, dx, dy, 1.0, im), "spmv_dia");
         download(y, dy, P->vs * N);
-        release_scratch();
         return undef(env);
     });
+}
+
+// Largest index of an int32 mirror, found once per upload.
+std::map<const Mirror*, std::pair<uint64_t, int32_t>> g_maxidx;
+int32_t max_index(const Mirror& m, int64_t n) {
+    auto& e = g_maxidx[&m];
+    if (e.first != m.version) {
+        int32_t mx = -1;
+        check(sable_max_i32(g_ctx, (const int32_t*)m.d, n, &mx), "index scan");
+        e = {m.version, mx};
+    }
+    return e.second;
 }
 
 // spmv_ell(indices, data, N, nc, x, y[, inner_max])
@@ -358,14 +664,13 @@ napi_value spmv_ell_impl(napi_env env, napi_callback_info info, bool wrapper) {
         const int im = wrapper ? (int)a.i(6) : 1;
         if (nc <= 0 || N <= 0 || im <= 0) return undef(env);
         const size_t slots = (size_t)nc * N;
-        const int32_t ncol = max_of(idx, slots) + 1;
-        void* di = mirror(idx, 4 * slots);
+        Mirror& mi = mirror_rec(idx, 4 * slots);
         void* dd = mirror(data, P->vs * slots);
-        void* dx = scratch(x, P->vs * (ncol > N ? ncol : N), true);
-        void* dy = scratch(y, P->vs * N, true);
-        check(sable_spmv_ell(g_ctx, P->dt, N, nc, (const int32_t*)di, dd, dx, dy, 1.0, im), "spmv_ell");
+        const int32_t ncol = max_index(mi, (int64_t)slots) + 1;
+        void* dx = vec_in(0, x, P->vs * (ncol > N ? ncol : N));
+        void* dy = vec_in(1, y, P->vs * N);
+        check(sable_spmv_ell(g_ctx, P->dt, N, nc, (const int32_t*)mi.d, dd, dx, dy, 1.0, im), "spmv_ell");
         download(y, dy, P->vs * N);
-        release_scratch();
         return undef(env);
     });
 }
@@ -484,6 +789,7 @@ napi_value lm_memcpy(napi_env env, napi_callback_info info) {
         const size_t src = a.i(0), dst = a.i(1), n = a.i(2);
         check_range(src, n);
         check_range(dst, n);
+        unprotect_range(dst, n);
         memmove(g_mem.base + dst, g_mem.base + src, n);
         return undef(env);
     });
@@ -546,6 +852,43 @@ napi_value coo_csc(napi_env env, napi_callback_info info) {
     });
 }
 
+// A lower-triangular matrix with its solve analysis (level schedule, the
+// level-ordered copy), kept while its arrays are unchanged.  One lane per
+// row, so every solve keeps the reference's per-row order.
+struct ResidentSpts : ResidentBase {
+    sable_spts_t* a = nullptr;
+    uint64_t v[3] = {0, 0, 0};
+    ~ResidentSpts() override {
+        if (a) sable_spts_destroy(a);
+    }
+};
+
+enum { KIND_SPTS_CSR = 32, KIND_SPTS_CSC = 48 };
+
+ResidentSpts& spts_resident(Prec* P, int kind, size_t p, size_t i, size_t v, int32_t N) {
+    const int32_t* h = host_i32(p, (size_t)N + 1);
+    const int64_t nnz = h[N];
+    if (h[0] != 0 || nnz < N) throw Err{"the pointer array must start at 0 and hold the N diagonals"};
+    Mirror& mp = mirror_rec(p, 4 * ((size_t)N + 1));
+    Mirror& mi = mirror_rec(i, 4 * (size_t)nnz);
+    Mirror& mv = mirror_rec(v, P->vs * (size_t)nnz);
+    ResidentSpts& R = resident<ResidentSpts>(kind + (int)P->dt, p, i, v, N);
+    if (!R.a || R.v[0] != mp.version || R.v[1] != mi.version || R.v[2] != mv.version) {
+        if (R.a) sable_spts_destroy(R.a);
+        R.a = nullptr;
+        if (kind == KIND_SPTS_CSC)
+            check(sable_spts_csc_analyze(g_ctx, P->dt, N, (const int32_t*)mp.d, (const int32_t*)mi.d, mv.d, 1, &R.a),
+                  "spts_csc");
+        else
+            check(sable_spts_analyze(g_ctx, P->dt, N, (const int32_t*)mp.d, (const int32_t*)mi.d, mv.d, 1, &R.a),
+                  "spts analysis");
+        R.v[0] = mp.version;
+        R.v[1] = mi.version;
+        R.v[2] = mv.version;
+    }
+    return R;
+}
+
 // spts_csc(col_ptr, row, val, x, y, N[, inner_max]): y = L^-1 x, L lower in
 // CSC with the diagonal first in each column
 napi_value spts_csc_impl(napi_env env, napi_callback_info info, bool wrapper) {
@@ -556,17 +899,12 @@ napi_value spts_csc_impl(napi_env env, napi_callback_info info, bool wrapper) {
         const int32_t N = (int32_t)a.i(5);
         const int im = wrapper ? (int)a.i(6) : 1;
         if (N <= 0 || im <= 0) return undef(env);
-        const int32_t* hcp = host_i32(a.i(0), N + 1);
-        const int64_t nnz = hcp[N];
-        if (hcp[0] != 0 || nnz < N) throw Err{"col_ptr must start at 0 and hold the N diagonals"};
-        void* dcp = mirror(a.i(0), 4 * (size_t)(N + 1));
-        void* drow = mirror(a.i(1), 4 * nnz);
-        void* dv = mirror(a.i(2), P->vs * nnz);
-        void* dx = scratch(a.i(3), P->vs * N, true);
-        void* dy = scratch(a.i(4), P->vs * N, false);
-        check(sable_spts_csc(g_ctx, P->dt, N, (const int32_t*)dcp, (const int32_t*)drow, dv, dx, dy, im), "spts_csc");
+        ResidentSpts& R = spts_resident(P, KIND_SPTS_CSC, a.i(0), a.i(1), a.i(2), N);
+        void* dx = vec_in(0, a.i(3), P->vs * N);
+        void* dy = vec(1, P->vs * N);
+        for (int it = 0; it < im; it++) check(sable_spts_solve_syncfree(g_ctx, R.a, dx, dy), "spts_csc");
+        check(sable_spts_status(g_ctx, R.a), "spts_csc");
         download(a.i(4), dy, P->vs * N);
-        release_scratch();
         return undef(env);
     });
 }
@@ -685,8 +1023,12 @@ napi_value num_cols(napi_env env, napi_callback_info info) {
     });
 }
 
-// csr_ell(row_ptr, col, val, N, nc, out_indices, out_data)
-napi_value csr_ell(napi_env env, napi_callback_info info) {
+// csr_ell(row_ptr, col, val, N, nc, out_indices, out_data); csr_ell_row: the
+// same into the parallel tree's row-major layout
+napi_value csr_ell_any(napi_env env, napi_callback_info info, bool row_major);
+napi_value csr_ell(napi_env env, napi_callback_info info) { return csr_ell_any(env, info, false); }
+napi_value csr_ell_row(napi_env env, napi_callback_info info) { return csr_ell_any(env, info, true); }
+napi_value csr_ell_any(napi_env env, napi_callback_info info, bool row_major) {
     Prec* P = prec_of(env, info);
     return guarded(env, [&] {
         Args a = get_args(env, info, 7);
@@ -700,7 +1042,8 @@ napi_value csr_ell(napi_env env, napi_callback_info info) {
         const size_t slots = (size_t)nc * N;
         void* oi = scratch(a.i(5), 4 * slots, false);
         void* od = scratch(a.i(6), P->vs * slots, false);
-        check(sable_csr_ell(g_ctx, P->dt, N, nc, (const int32_t*)drp, (const int32_t*)dc, dv, (int32_t*)oi, od),
+        check((row_major ? sable_csr_ell_rowmajor : sable_csr_ell)(g_ctx, P->dt, N, nc, (const int32_t*)drp,
+                                                                  (const int32_t*)dc, dv, (int32_t*)oi, od),
               "csr_ell");
         drop_mirrors_in(a.i(5), 4 * slots);
         drop_mirrors_in(a.i(6), P->vs * slots);
@@ -733,8 +1076,12 @@ napi_value num_diags(napi_env env, napi_callback_info info) {
     });
 }
 
-// csr_dia(row_ptr, col, val, N, nd, stride, out_offset, out_data)
-napi_value csr_dia(napi_env env, napi_callback_info info) {
+// csr_dia(row_ptr, col, val, N, nd, stride, out_offset, out_data); csr_dia_row:
+// the parallel tree's row-major layout (stride N)
+napi_value csr_dia_any(napi_env env, napi_callback_info info, bool row_major);
+napi_value csr_dia(napi_env env, napi_callback_info info) { return csr_dia_any(env, info, false); }
+napi_value csr_dia_row(napi_env env, napi_callback_info info) { return csr_dia_any(env, info, true); }
+napi_value csr_dia_any(napi_env env, napi_callback_info info, bool row_major) {
     Prec* P = prec_of(env, info);
     return guarded(env, [&] {
         Args a = get_args(env, info, 8);
@@ -748,7 +1095,11 @@ napi_value csr_dia(napi_env env, napi_callback_info info) {
         const size_t db = P->vs * (size_t)nd * stride;
         void* oo = scratch(a.i(6), 4 * (size_t)(nd > 0 ? nd : 1), false);
         void* od = scratch(a.i(7), db, false);
-        check(sable_csr_dia(g_ctx, P->dt, N, nd, stride, (const int32_t*)drp, (const int32_t*)dc, dv, (int32_t*)oo, od),
+        if (row_major && stride != N) throw Err{"row-major DIA storage has stride N"};
+        check(row_major ? sable_csr_dia_rowmajor(g_ctx, P->dt, N, nd, (const int32_t*)drp, (const int32_t*)dc, dv,
+                                                 (int32_t*)oo, od)
+                        : sable_csr_dia(g_ctx, P->dt, N, nd, stride, (const int32_t*)drp, (const int32_t*)dc, dv,
+                                        (int32_t*)oo, od),
               "csr_dia");
         drop_mirrors_in(a.i(6), 4 * (size_t)nd);
         drop_mirrors_in(a.i(7), db);
@@ -762,24 +1113,20 @@ napi_value csr_dia(napi_env env, napi_callback_info info) {
 // ---- triangular solves (parallel tree; the whole solve runs on the GPU in
 // the call of worker 0, the other workers return at once)
 
-// Level-set solve on the level-ordered matrix: b and y in level order.
+// Level-set solve on the level-ordered matrix: b and y in level order.  The
+// level-ordered matrix is its own level order (rows sorted by level), so its
+// analysis reproduces the caller's level_ptr and the solve is the
+// reference's level-by-level arithmetic, t = 0 + v0*y0 + ...; y = (b - t)/d.
 napi_value level_solve(napi_env env, Prec* P, const Args& a, size_t lp_at, size_t row_at, size_t b_at, size_t y_at,
                        int32_t nl, int32_t N, int im) {
     if (a.id() != 0 || nl <= 0 || im <= 0 || N <= 0) return undef(env);
     ensure_ctx();
-    const int32_t* lp = host_i32(a.i(lp_at), nl + 1);
-    const int32_t* hrp = host_i32(a.i(row_at), N + 1);
-    const int64_t nnz = hrp[N];
-    void* drp = mirror(a.i(row_at), 4 * (size_t)(N + 1));
-    void* dc = mirror(a.i(row_at + 1), 4 * nnz);
-    void* dv = mirror(a.i(row_at + 2), P->vs * nnz);
-    void* db = scratch(a.i(b_at), P->vs * N, true);
-    void* dy = scratch(a.i(y_at), P->vs * N, false);
-    for (int it = 0; it < im; it++)
-        check(sable_spts_level_csr(g_ctx, P->dt, N, nl, lp, (const int32_t*)drp, (const int32_t*)dc, dv, db, dy, 1),
-              "spts_level_csr");
+    (void)host_i32(a.i(lp_at), (size_t)nl + 1);
+    ResidentSpts& R = spts_resident(P, KIND_SPTS_CSR, a.i(row_at), a.i(row_at + 1), a.i(row_at + 2), N);
+    void* db = vec_in(0, a.i(b_at), P->vs * N);
+    void* dy = vec(1, P->vs * N);
+    for (int it = 0; it < im; it++) check(sable_spts_solve_level(g_ctx, R.a, db, dy), "spts_level_csr");
     download(a.i(y_at), dy, P->vs * N);
-    release_scratch();
     return undef(env);
 }
 
@@ -837,52 +1184,65 @@ napi_value spts_csr_sync_free_wrapper(napi_env env, napi_callback_info info) {
         const int32_t N = (int32_t)a.i(5);
         const int im = (int)a.i(10);
         if (im <= 0 || N <= 0) return undef(env);
-        const int32_t* hrp = host_i32(a.i(0), N + 1);
-        const int64_t nnz = hrp[N];
-        void* drp = mirror(a.i(0), 4 * (size_t)(N + 1));
-        void* dc = mirror(a.i(1), 4 * nnz);
-        void* dv = mirror(a.i(2), P->vs * nnz);
-        void* db = scratch(a.i(3), P->vs * N, true);
-        void* dy = scratch(a.i(4), P->vs * N, false);
-        for (int it = 0; it < im; it++)
-            check(sable_spts_syncfree_csr(g_ctx, P->dt, N, (const int32_t*)drp, (const int32_t*)dc, dv, db, dy, 1, 0),
-                  "spts_csr_sync_free");
+        ResidentSpts& R = spts_resident(P, KIND_SPTS_CSR, a.i(0), a.i(1), a.i(2), N);
+        void* db = vec_in(0, a.i(3), P->vs * N);
+        void* dy = vec(1, P->vs * N);
+        for (int it = 0; it < im; it++) check(sable_spts_solve_syncfree(g_ctx, R.a, db, dy), "spts_csr_sync_free");
+        check(sable_spts_status(g_ctx, R.a), "spts_csr_sync_free");
         download(a.i(4), dy, P->vs * N);
-        release_scratch();
         return undef(env);
     });
 }
 
 // ---- parallel-tree SpMV slices whose kernels take a row range
 
-// spmv_dia_col_wrapper(id, offset, data, start, end_incl, nd, N, stride, x,
-// y, inner_max): rows [start, end_incl] of y += A.x
-napi_value par_spmv_dia_col_wrapper(napi_env env, napi_callback_info info) {
+// The worker's rows [r0, r1) of y on the device: the pooled vector holds
+// only those rows, and the pointer handed to a row-range kernel is shifted
+// back by r0 (the kernel touches rows r0..r1-1 only).
+char* y_rows(Prec* P, size_t y, int64_t r0, int64_t r1) {
+    return (char*)vec_in(1, y + P->vs * r0, P->vs * (size_t)(r1 - r0)) - P->vs * r0;
+}
+
+void y_rows_out(Prec* P, size_t y, const char* dy, int64_t r0, int64_t r1) {
+    download(y + P->vs * r0, dy + P->vs * r0, P->vs * (size_t)(r1 - r0));
+}
+
+// Column-major DIA slices: spmv_dia_col_wrapper and spmv_dia_col_basic_wrapper
+// (id, offset, data, start, end_incl, nd, N, stride, x, y, inner_max) and
+// spmv_bdia_col_wrapper (id, offset, data, istart, iend, start, end_incl, nd,
+// N, stride, x, y, inner_max): rows [start, end_incl] of y += A.x, each row
+// adding its diagonals in stored order (bit-identical).  istart/iend are the
+// blocked kernel's work arrays; the device sweep needs none and leaves them
+// untouched.
+napi_value dia_col_slice(napi_env env, napi_callback_info info, bool blocked) {
     Prec* P = prec_of(env, info);
     return guarded(env, [&] {
-        Args a = get_args(env, info, 10);
+        Args a = get_args(env, info, blocked ? 12 : 10);
         ensure_ctx();
-        const int64_t r0 = a.i(2), r1 = a.i(3) + 1;
-        const int32_t nd = (int32_t)a.i(4), N = (int32_t)a.i(5), stride = (int32_t)a.i(6);
-        const size_t x = a.i(7), y = a.i(8);
-        const int im = (int)a.i(9);
+        const size_t k = blocked ? 2 : 0;
+        const int64_t r0 = a.i(2 + k), r1 = a.i(3 + k) + 1;
+        const int32_t nd = (int32_t)a.i(4 + k), N = (int32_t)a.i(5 + k), stride = (int32_t)a.i(6 + k);
+        const size_t x = a.i(7 + k), y = a.i(8 + k);
+        const int im = (int)a.i(9 + k);
         if (nd <= 0 || N <= 0 || im <= 0 || r0 >= r1) return undef(env);
         if (r0 < 0 || r1 > N) throw Err{"row range outside the matrix"};
         void* doff = mirror(a.i(0), 4 * (size_t)nd);
         void* dd = mirror(a.i(1), P->vs * (size_t)nd * stride);
-        void* dx = scratch(x, P->vs * N, true);
-        void* dy = scratch(y, P->vs * N, true);
-        // rows are independent: the whole product is formed and only this
-        // worker's rows are written back, each with the reference's order
-        check(sable_spmv_dia(g_ctx, P->dt, N, nd, stride, (const int32_t*)doff, dd, dx, dy, 1.0, im), "spmv_dia");
-        download(y + P->vs * r0, (char*)dy + P->vs * r0, P->vs * (r1 - r0));
-        release_scratch();
+        void* dx = vec_in(0, x, P->vs * N);
+        char* dy = y_rows(P, y, r0, r1);
+        check(sable_spmv_dia_range(g_ctx, P->dt, N, nd, stride, (const int32_t*)doff, dd, (int32_t)r0, (int32_t)r1,
+                                   dx, dy, 1.0, im),
+              "spmv_dia");
+        y_rows_out(P, y, dy, r0, r1);
         return undef(env);
     });
 }
+napi_value par_spmv_dia_col_wrapper(napi_env env, napi_callback_info info) { return dia_col_slice(env, info, false); }
+napi_value par_spmv_bdia_col_wrapper(napi_env env, napi_callback_info info) { return dia_col_slice(env, info, true); }
 
-// spmv_ell_col_wrapper(id, indices, data, start, end, nc, N, x, y,
-// inner_max): rows [start, end) of y += A.x
+// Column-major ELL slices: spmv_ell_col_wrapper, spmv_ell_col_gs_wrapper and
+// spmv_bell_col_gs_wrapper (id, indices, data, start, end, nc, N, x, y,
+// inner_max): rows [start, end) of y += A.x, slots in order per row.
 napi_value par_spmv_ell_col_wrapper(napi_env env, napi_callback_info info) {
     Prec* P = prec_of(env, info);
     return guarded(env, [&] {
@@ -895,17 +1255,73 @@ napi_value par_spmv_ell_col_wrapper(napi_env env, napi_callback_info info) {
         if (nc <= 0 || N <= 0 || im <= 0 || r0 >= r1) return undef(env);
         if (r0 < 0 || r1 > N) throw Err{"row range outside the matrix"};
         const size_t slots = (size_t)nc * N;
-        const int32_t ncol = max_of(a.i(0), slots) + 1;
-        void* di = mirror(a.i(0), 4 * slots);
+        Mirror& mi = mirror_rec(a.i(0), 4 * slots);
         void* dd = mirror(a.i(1), P->vs * slots);
-        void* dx = scratch(x, P->vs * (ncol > N ? ncol : N), true);
-        void* dy = scratch(y, P->vs * N, true);
-        check(sable_spmv_ell(g_ctx, P->dt, N, nc, (const int32_t*)di, dd, dx, dy, 1.0, im), "spmv_ell");
-        download(y + P->vs * r0, (char*)dy + P->vs * r0, P->vs * (r1 - r0));
-        release_scratch();
+        const int32_t ncol = max_index(mi, (int64_t)slots) + 1;
+        void* dx = vec_in(0, x, P->vs * (ncol > N ? ncol : N));
+        char* dy = y_rows(P, y, r0, r1);
+        check(sable_spmv_ell_range(g_ctx, P->dt, N, nc, (const int32_t*)mi.d, dd, (int32_t)r0, (int32_t)r1, dx, dy,
+                                   1.0, im),
+              "spmv_ell");
+        y_rows_out(P, y, dy, r0, r1);
         return undef(env);
     });
 }
+
+// Row-major DIA slice: spmv_dia_wrapper(id, offset, data, start, end, nd, N,
+// x, y, inner_max), rows [start, end), data[i*nd + k].
+napi_value par_spmv_dia_row_wrapper(napi_env env, napi_callback_info info) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 9);
+        ensure_ctx();
+        const int64_t r0 = a.i(2), r1 = a.i(3);
+        const int32_t nd = (int32_t)a.i(4), N = (int32_t)a.i(5);
+        const size_t x = a.i(6), y = a.i(7);
+        const int im = (int)a.i(8);
+        if (nd <= 0 || N <= 0 || im <= 0 || r0 >= r1) return undef(env);
+        if (r0 < 0 || r1 > N) throw Err{"row range outside the matrix"};
+        void* doff = mirror(a.i(0), 4 * (size_t)nd);
+        void* dd = mirror(a.i(1), P->vs * (size_t)nd * N);
+        void* dx = vec_in(0, x, P->vs * N);
+        char* dy = y_rows(P, y, r0, r1);
+        check(sable_spmv_dia_rowmajor(g_ctx, P->dt, N, nd, (const int32_t*)doff, dd, (int32_t)r0, (int32_t)r1, dx,
+                                      dy, im),
+              "spmv_dia");
+        y_rows_out(P, y, dy, r0, r1);
+        return undef(env);
+    });
+}
+
+// Row-major ELL slices: spmv_ell_wrapper and spmv_ell_row_gs_wrapper (id,
+// indices, data, start, end, nc, N, x, y, inner_max), rows [start, end),
+// slots at i*nc + k, in the plain or the gs order.
+napi_value ell_row_slice(napi_env env, napi_callback_info info, int order) {
+    Prec* P = prec_of(env, info);
+    return guarded(env, [&] {
+        Args a = get_args(env, info, 9);
+        ensure_ctx();
+        const int64_t r0 = a.i(2), r1 = a.i(3);
+        const int32_t nc = (int32_t)a.i(4), N = (int32_t)a.i(5);
+        const size_t x = a.i(6), y = a.i(7);
+        const int im = (int)a.i(8);
+        if (nc <= 0 || N <= 0 || im <= 0 || r0 >= r1) return undef(env);
+        if (r0 < 0 || r1 > N) throw Err{"row range outside the matrix"};
+        const size_t slots = (size_t)nc * N;
+        Mirror& mi = mirror_rec(a.i(0), 4 * slots);
+        void* dd = mirror(a.i(1), P->vs * slots);
+        const int32_t ncol = max_index(mi, (int64_t)slots) + 1;
+        void* dx = vec_in(0, x, P->vs * (ncol > N ? ncol : N));
+        char* dy = y_rows(P, y, r0, r1);
+        check(sable_spmv_ell_rowmajor(g_ctx, P->dt, N, nc, (const int32_t*)mi.d, dd, (int32_t)r0, (int32_t)r1, dx,
+                                      dy, order, im),
+              "spmv_ell");
+        y_rows_out(P, y, dy, r0, r1);
+        return undef(env);
+    });
+}
+napi_value par_spmv_ell_row_wrapper(napi_env env, napi_callback_info info) { return ell_row_slice(env, info, 0); }
+napi_value par_spmv_ell_row_gs_wrapper(napi_env env, napi_callback_info info) { return ell_row_slice(env, info, 1); }
 
 // sum(y, w, start, end[, N]): y[i] = y[i] + w[i] for i in [start, end)
 napi_value par_sum(napi_env env, napi_callback_info info) {
@@ -917,11 +1333,10 @@ napi_value par_sum(napi_env env, napi_callback_info info) {
         const int64_t r0 = a.i(2), r1 = a.i(3);
         if (r0 >= r1) return undef(env);
         const size_t y = a.i(0) + P->vs * r0, w = a.i(1) + P->vs * r0, n = r1 - r0;
-        void* dy = scratch(y, P->vs * n, true);
-        void* dw = scratch(w, P->vs * n, true);
+        void* dy = vec_in(1, y, P->vs * n);
+        void* dw = vec_in(0, w, P->vs * n);
         check(sable_vec_add(g_ctx, P->dt, dy, dw, n), "sum");
         download(y, dy, P->vs * n);
-        release_scratch();
         return undef(env);
     });
 }
@@ -1248,7 +1663,13 @@ napi_value free_fn(napi_env env, napi_callback_info info) {
         Args a = get_args(env, info, 1);
         const size_t off = a.i(0);
         auto it = g_mem.used.find(off);
-        if (it != g_mem.used.end()) drop_mirrors_in(off, it->second);
+        if (it != g_mem.used.end()) {
+            // the region's mirrors and resident matrices go; its pages become
+            // writable so that the next owner's writes do not fault
+            drop_resident_in(off, it->second);
+            drop_mirrors_in(off, it->second);
+            unprotect_range(off, it->second);
+        }
         lm_free(off);
         return undef(env);
     });
@@ -1287,6 +1708,13 @@ napi_value stats_fn(napi_env env, napi_callback_info) {
     napi_set_named_property(env, o, "linear_bytes", num(env, (double)g_mem.size));
     napi_set_named_property(env, o, "pinned", num(env, g_mem.pinned ? 1 : 0));
     napi_set_named_property(env, o, "allocations", num(env, (double)g_mem.used.size()));
+    napi_set_named_property(env, o, "write_tracking", num(env, sable_track::g_on ? 1 : 0));
+    napi_set_named_property(env, o, "writes_caught", num(env, (double)sable_track::faults()));
+    napi_set_named_property(env, o, "uploads", num(env, (double)g_uploads));
+    napi_set_named_property(env, o, "upload_bytes", num(env, (double)g_upload_bytes));
+    napi_set_named_property(env, o, "resident", num(env, (double)g_resident.size()));
+    napi_set_named_property(env, o, "schedule", num(env, g_last_sched));
+    napi_set_named_property(env, o, "schedule_ms", num(env, g_last_sched_ms));
     return o;
 }
 
@@ -1315,6 +1743,7 @@ napi_value make_serial(napi_env env, Prec* P) {
         {"count_transpose_cols_ell", count_transpose_cols_ell}, {"transpose_ell", transpose_ell},
         {"memcpy", lm_memcpy}, {"coo_csr", coo_csr}, {"num_cols", num_cols}, {"csr_ell", csr_ell},
         {"num_diags", num_diags}, {"csr_dia", csr_dia}, {"coo_csc", coo_csc},
+        {"csr_ell_row", csr_ell_row}, {"csr_dia_row", csr_dia_row},
         {"spts_csc", spts_csc}, {"spts_csc_wrapper", spts_csc_wrapper}, {"mmread", mmread},
         {"count_nonzeros_coo", count_nonzeros_coo}, {"copy_nonzeros_coo", copy_nonzeros_coo},
         {"copy_nonzeros_csr", copy_nonzeros_csr}, {"count_nonzeros_dia", count_nonzeros_dia},
@@ -1333,7 +1762,20 @@ napi_value make_parallel(napi_env env, Prec* P, Prec* serial) {
     napi_create_object(env, &m);
     const Fn fns[] = {
         {"spmv_coo_wrapper", spmv_coo_wrapper}, {"spmv_csr_wrapper", spmv_csr_wrapper},
-        {"spmv_dia_col_wrapper", par_spmv_dia_col_wrapper}, {"spmv_ell_col_wrapper", par_spmv_ell_col_wrapper},
+        // the SIMD variants compute the same products (CSR/COO within the
+        // summation bound, the column-major DIA/ELL ones bit for bit)
+        {"spmv_coo_gs_wrapper", spmv_coo_wrapper}, {"spmv_csr_gs_wrapper", spmv_csr_wrapper},
+        {"spmv_csr_short_rows_wrapper", spmv_csr_short_impl}, {"spmv_csr_gs_short_wrapper", spmv_csr_short_impl},
+        {"spmv_csr_unroll2_wrapper", spmv_csr_short_impl}, {"spmv_csr_unroll3_wrapper", spmv_csr_short_impl},
+        {"spmv_csr_unroll4_wrapper", spmv_csr_short_impl}, {"spmv_csr_unroll6_wrapper", spmv_csr_short_impl},
+        {"spmv_dia_col_wrapper", par_spmv_dia_col_wrapper},
+        {"spmv_dia_col_basic_wrapper", par_spmv_dia_col_wrapper},
+        {"spmv_bdia_col_wrapper", par_spmv_bdia_col_wrapper},
+        {"spmv_dia_wrapper", par_spmv_dia_row_wrapper},
+        {"spmv_ell_col_wrapper", par_spmv_ell_col_wrapper}, {"spmv_ell_col_gs_wrapper", par_spmv_ell_col_wrapper},
+        {"spmv_bell_col_gs_wrapper", par_spmv_ell_col_wrapper},
+        {"spmv_ell_wrapper", par_spmv_ell_row_wrapper}, {"spmv_ell_row_gs_wrapper", par_spmv_ell_row_gs_wrapper},
+        {"csr_ell_row", csr_ell_row}, {"csr_dia_row", csr_dia_row},
         {"memcpy", lm_memcpy}, {"spts_level_csr", spts_level_csr},
         {"spts_csr_sync_free_wrapper", spts_csr_sync_free_wrapper},
         {"spts_csr_level_sync_free_wrapper", spts_csr_level_sync_free_wrapper},

@@ -127,6 +127,46 @@ class CooPlan:
             pass
 
 
+class CsrAuto:
+    """Autotuned CSR schedule of a resident matrix (sable_csr_auto_*): the
+    candidates are timed on the device and the fastest kept; the choice is
+    cached per matrix class.  deterministic=True keeps to the bit-
+    reproducible schedules."""
+
+    NAMES = {0: "blocks", 1: "vector", 2: "tiles"}
+
+    def __init__(self, A, deterministic=False, timing=True, ncols=0):
+        self.ctx = A.row.ctx
+        self.A = A
+        h = c_vp()
+        flags = (1 if deterministic else 0) | (0 if timing else 2)
+        call("sable_csr_auto_create", self.ctx.handle, dtype_code(A.dtype), A.N, int(ncols), A.row.ptr, A.col.ptr,
+             A.val.ptr, flags, ctypes.byref(h))
+        self.handle = h
+
+    def info(self):
+        i = _lib.CsrAutoInfo()
+        call("sable_csr_auto_info", self.handle, ctypes.byref(i))
+        return {"schedule": self.NAMES.get(i.schedule, i.schedule), "param0": i.param0, "param1": i.param1,
+                "ncols": i.ncols, "nnz": i.nnz, "extra_bytes": i.extra_bytes, "ms": i.ms,
+                "from_cache": bool(i.from_cache)}
+
+    def run(self, x, y, beta=1.0, inner_max=1):
+        call("sable_spmv_csr_auto", self.ctx.handle, self.handle, x.ptr, y.ptr, float(beta), int(inner_max))
+
+    @staticmethod
+    def clear_cache():
+        _lib.load().sable_csr_auto_cache_clear()
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.load().sable_csr_auto_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
 class CsrPanels:
     """Column-panel blocked CSR plan (sable_csr_panels_*): a panel-major copy
     of the matrix, one launch per panel of panel_cols columns."""

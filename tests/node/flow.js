@@ -86,13 +86,28 @@ const E = sparse_instance.exports;
 const P = my_instance.exports;
 
 // ---------------------------------------------------------------- COO -> CSR
+// The reference's conversion shape (seq/src/sparse32.js:1674, sparse64.js:1255,
+// par/src/sparse32.js:2096): coo_csr(A_coo) allocates A_csr with allocate_CSR
+// and returns it; with the addon its body calls the coo_csr export (the
+// edited function INTEGRATION.md shows).  Callers use the return value.
+function allocate_CSR(N_, nnz_) {
+  return { row_index: alloc(N_ + 1, Int32Array), col_index: alloc(nnz_, Int32Array), val_index: alloc(nnz_, FArray),
+           N: N_, nnz: nnz_ };
+}
+function coo_csr(A_coo) {
+  const A_csr = allocate_CSR(A_coo.N, A_coo.nnz);
+  sparse_instance.exports.coo_csr(A_coo.row_index, A_coo.col_index, A_coo.val_index, A_coo.nnz, A_coo.N,
+                                  A_csr.row_index, A_csr.col_index, A_csr.val_index);
+  return A_csr;
+}
 step('coo_csr');
 const coo_row = load_array('coo_row', Int32Array);
 const coo_col = load_array('coo_col', Int32Array);
 const coo_val = load_array('coo_val', FArray);
 const x = load_array('x', FArray);
-const csr = { row: alloc(N + 1, Int32Array), col: alloc(nnz, Int32Array), val: alloc(nnz, FArray) };
-E.coo_csr(coo_row.off, coo_col.off, coo_val.off, nnz, N, csr.row, csr.col, csr.val);
+const A_coo = { row_index: coo_row.off, col_index: coo_col.off, val_index: coo_val.off, N: N, nnz: nnz };
+const A_csr = coo_csr(A_coo);
+const csr = { row: A_csr.row_index, col: A_csr.col_index, val: A_csr.val_index };
 save('csr_row', csr.row, N + 1, Int32Array);
 save('csr_col', csr.col, nnz, Int32Array);
 save('csr_val', csr.val, nnz, FArray);
@@ -143,9 +158,26 @@ spmv_flow('csr', (y) => E.spmv_csr_wrapper(csr.row, csr.col, csr.val, x.off, y, 
 spmv_flow('csr1', (y) => E.spmv_csr(csr.row, csr.col, csr.val, x.off, y, N));
 if (ell) spmv_flow('ell', (y) => E.spmv_ell_wrapper(ell.idx, ell.data, N, ell.nc, x.off, y, inner_max));
 if (dia) spmv_flow('dia', (y) => E.spmv_dia_wrapper(dia.off, dia.data, N, dia.nd, dia.stride, x.off, y, inner_max));
-// a second call on the unchanged matrix reuses its device copy
+// a second call on the unchanged matrix reuses its device copy: no upload
+const up0 = inst.stats().uploads;
 spmv_flow('csr_again', (y) => E.spmv_csr_wrapper(csr.row, csr.col, csr.val, x.off, y, N, inner_max));
+outputs.uploads_again = inst.stats().uploads - up0;
 outputs.mirrors_after_spmv = inst.stats().mirrors;
+outputs.write_tracking = inst.stats().write_tracking;
+// a JS write into the matrix (through a typed array) is seen: the next
+// product uses the new values, and only the changed array is uploaded
+{
+  const v = new FArray(memory.buffer, csr.val, nnz);
+  const keep = [v[7], v[nnz - 1]];
+  v[7] = 3.0;
+  v[nnz - 1] = -2.0;
+  const up1 = inst.stats().uploads;
+  spmv_flow('csr_mod', (y) => E.spmv_csr_wrapper(csr.row, csr.col, csr.val, x.off, y, N, 1));
+  outputs.uploads_mod = inst.stats().uploads - up1;
+  save('csr_val_mod', csr.val, nnz, FArray);
+  v[7] = keep[0];
+  v[nnz - 1] = keep[1];
+}
 
 // ---------------------------------------------------------------- transposes
 step('transpose_csr');
@@ -220,6 +252,86 @@ if (ell) {
   });
 }
 outputs.row_ptr_last = rowp[N];
+
+// the parallel tree's SIMD variants (par/src/worker32.js:127-399), with the
+// reference's argument orders
+spmv_flow('par_csr_gs', (y) => {
+  for (let id = 0; id < W; id++) {
+    const s = R[id][0], e = R[id][1];
+    P.spmv_csr_gs_wrapper(id, csr.row + IB * s, csr.col, csr.val, x.off, y + VB * s, e - s, inner_max);
+  }
+});
+spmv_flow('par_coo_gs', (y) => {
+  P.spmv_coo_gs_wrapper(0, coo_row.off, coo_col.off, coo_val.off, x.off, y, nnz, inner_max);
+});
+if (spec.short_rows) {
+  // rows sorted into runs of 1, 2 and 3 entries, then longer rows; worker id
+  // takes [one, end) with the group boundaries one <= two <= three <= four
+  const S = spec.short_rows;
+  const sr = { row: load_array('sr_row', Int32Array), col: load_array('sr_col', Int32Array),
+               val: load_array('sr_val', FArray) };
+  const names = ['spmv_csr_short_rows_wrapper', 'spmv_csr_gs_short_wrapper', 'spmv_csr_unroll2_wrapper',
+                 'spmv_csr_unroll3_wrapper', 'spmv_csr_unroll4_wrapper', 'spmv_csr_unroll6_wrapper'];
+  for (const fn of names) {
+    step(fn);
+    const y = alloc(S.N, FArray);
+    const one = S.one, two = S.two, three = S.three, four = S.four, end = S.N;
+    P[fn](0, sr.row.off + IB * one, sr.col.off, sr.val.off, x.off, y + VB * one, end - four, two - one, three - two,
+          four - three, inner_max);
+    save('y_' + fn, y, S.N, FArray);
+  }
+  let bad = 'no error';
+  try {
+    // rows 0 and 1 hold one entry each, not three
+    P.spmv_csr_short_rows_wrapper(0, sr.row.off, sr.col.off, sr.val.off, x.off, alloc(S.N, FArray), S.N - 2, 0, 0, 2, 1);
+  } catch (e) { bad = e.message; }
+  outputs.short_rows_mismatch = bad;
+}
+if (dia) {
+  spmv_flow('par_dia_basic', (y) => {
+    for (let id = 0; id < W; id++)
+      P.spmv_dia_col_basic_wrapper(id, dia.off, dia.data, R[id][0], R[id][1] - 1, dia.nd, N, dia.stride, x.off, y,
+                                   inner_max);
+  });
+  spmv_flow('par_bdia', (y) => {
+    const istart = alloc(dia.nd, Int32Array), iend = alloc(dia.nd, Int32Array);
+    for (let id = 0; id < W; id++)
+      P.spmv_bdia_col_wrapper(id, dia.off, dia.data, istart, iend, R[id][0], R[id][1] - 1, dia.nd, N, dia.stride,
+                              x.off, y, inner_max);
+  });
+}
+if (ell) {
+  spmv_flow('par_ell_gs', (y) => {
+    for (let id = 0; id < W; id++)
+      P.spmv_ell_col_gs_wrapper(id, ell.idx, ell.data, R[id][0], R[id][1], ell.nc, N, x.off, y, inner_max);
+  });
+  spmv_flow('par_bell_gs', (y) => {
+    for (let id = 0; id < W; id++)
+      P.spmv_bell_col_gs_wrapper(id, ell.idx, ell.data, R[id][0], R[id][1], ell.nc, N, x.off, y, inner_max);
+  });
+}
+// row-major storage: csr_ell / csr_dia with major "row"
+// (par/src/sparse32.js:1840-1880, 1912-1973) and its products
+step('csr_ell_row');
+const rell = { idx: alloc(nc * N, Int32Array), data: alloc(nc * N, FArray) };
+P.csr_ell_row(csr.row, csr.col, csr.val, N, nc, rell.idx, rell.data);
+save('rell_idx', rell.idx, nc * N, Int32Array);
+save('rell_data', rell.data, nc * N, FArray);
+spmv_flow('par_ell_row', (y) => {
+  for (let id = 0; id < W; id++) P.spmv_ell_wrapper(id, rell.idx, rell.data, R[id][0], R[id][1], nc, N, x.off, y, inner_max);
+});
+spmv_flow('par_ell_row_gs', (y) => {
+  for (let id = 0; id < W; id++)
+    P.spmv_ell_row_gs_wrapper(id, rell.idx, rell.data, R[id][0], R[id][1], nc, N, x.off, y, inner_max);
+});
+step('csr_dia_row');
+const rdia = { off: alloc(nd, Int32Array), data: alloc(nd * N, FArray) };
+P.csr_dia_row(csr.row, csr.col, csr.val, N, nd, N, rdia.off, rdia.data);
+save('rdia_off', rdia.off, nd, Int32Array);
+save('rdia_data', rdia.data, nd * N, FArray);
+spmv_flow('par_dia_row', (y) => {
+  for (let id = 0; id < W; id++) P.spmv_dia_wrapper(id, rdia.off, rdia.data, R[id][0], R[id][1], nd, N, x.off, y, inner_max);
+});
 
 // ---------------------------------------------------------------- SpTS
 if (spec.spts) {

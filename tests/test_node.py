@@ -32,10 +32,18 @@ SERIAL = ["spmv_coo", "spmv_coo_wrapper", "spmv_csr", "spmv_csr_wrapper", "spmv_
           "spmv_ell", "spmv_ell_wrapper", "transpose_csr", "transpose_dia", "count_transpose_cols_ell",
           "transpose_ell", "memcpy", "coo_csr", "num_cols", "csr_ell", "num_diags", "csr_dia", "coo_csc",
           "spts_csc", "spts_csc_wrapper", "mmread", "count_nonzeros_coo", "copy_nonzeros_coo", "copy_nonzeros_csr",
-          "count_nonzeros_dia", "copy_nonzeros_dia", "count_nonzeros_ell", "copy_nonzeros_ell"]
+          "count_nonzeros_dia", "copy_nonzeros_dia", "count_nonzeros_ell", "copy_nonzeros_ell",
+          "csr_ell_row", "csr_dia_row"]
 PARALLEL = ["spmv_coo_wrapper", "spmv_csr_wrapper", "spmv_dia_col_wrapper", "spmv_ell_col_wrapper", "memcpy",
             "sum", "spts_level_csr", "spts_csr_sync_free_wrapper", "spts_csr_level_sync_free_wrapper",
-            "spts_csr_opt_level_sync_free_wrapper", "metadata"]
+            "spts_csr_opt_level_sync_free_wrapper", "metadata",
+            # the SIMD variants par/src/worker32.js calls (:127-399) and the
+            # row-major conversions (par/src/sparse32.js:1840-1973)
+            "spmv_coo_gs_wrapper", "spmv_csr_gs_wrapper", "spmv_csr_short_rows_wrapper", "spmv_csr_gs_short_wrapper",
+            "spmv_csr_unroll2_wrapper", "spmv_csr_unroll3_wrapper", "spmv_csr_unroll4_wrapper",
+            "spmv_csr_unroll6_wrapper", "spmv_dia_col_basic_wrapper", "spmv_bdia_col_wrapper", "spmv_dia_wrapper",
+            "spmv_ell_col_gs_wrapper", "spmv_bell_col_gs_wrapper", "spmv_ell_wrapper", "spmv_ell_row_gs_wrapper",
+            "csr_ell_row", "csr_dia_row"]
 # element-wise exports (seq/src/sparse_opt_32.wat, par/src/sparse_opt_32.wat): <op>_<fmt> and self_<op>_<fmt>
 EW_OPS = ["expm1", "log1p", "sin", "tan", "pow", "deg2rad", "rad2deg", "sign", "abs", "neg", "sqrt", "ceil",
           "floor", "trunc", "nearest"]
@@ -116,7 +124,15 @@ def write_inputs(d, dtype, seed=0, workers=3, inner_max=3):
     # the same values in CSR order, for copy_nonzeros_csr
     order = np.lexsort((coo_col, coo_row))
     z_csr_val = z_val[order]
+    # rows in runs of one, two and three entries after 50 rows of another
+    # worker, then longer rows (the short-row variants' layout)
+    lens = np.concatenate([rng.integers(4, 9, 50), np.full(100, 1), np.full(150, 2), np.full(150, 3),
+                           rng.integers(4, 10, N - 450)])
+    sr_row = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    sr_col = np.concatenate([np.sort(rng.choice(N, k, replace=False)) for k in lens]).astype(np.int32)
+    sr_val = rng.uniform(-1, 1, len(sr_col)).astype(dtype)
     arrays = dict(coo_row=coo_row, coo_col=coo_col, coo_val=coo_val, x=x, L_row=Lrp, L_col=Lc, L_val=Lv, L_b=b,
+                  sr_row=sr_row, sr_col=sr_col, sr_val=sr_val,
                   z_val=z_val, z_csr_val=z_csr_val,
                   Lr_level_ptr=lp, Lr_perm=lperm, Lr_row=rrp, Lr_col=rc, Lr_val=rv, Lr_b=br,
                   Lc_col=ccp, Lc_row=crow, Lc_val=cval)
@@ -124,7 +140,8 @@ def write_inputs(d, dtype, seed=0, workers=3, inner_max=3):
         np.ascontiguousarray(v).tofile(os.path.join(d, k + ".bin"))
     with open(os.path.join(d, "mm.mtx"), "wb") as f:
         f.write(mm_text(seed))
-    spec = dict(N=N, nnz=nnz, inner_max=inner_max, workers=workers, spts=dict(N=n, nlevels=int(nl)), mm=True)
+    spec = dict(N=N, nnz=nnz, inner_max=inner_max, workers=workers, spts=dict(N=n, nlevels=int(nl)), mm=True,
+                short_rows=dict(N=N, one=50, two=150, three=300, four=450))
     with open(os.path.join(d, "inputs.json"), "w") as f:
         json.dump(spec, f)
     return spec, arrays
@@ -215,6 +232,46 @@ def test_node_flow_matches_oracle(tmp_path, dtype):
         err = np.abs(y.astype(np.float64) - yref.astype(np.float64))
         assert np.all(err <= bnd), (name, float(err.max()))
     assert out["mirrors_after_spmv"] > 0
+    # the unchanged matrix is not uploaded again; a JS write to its values is
+    # seen by the next product, which uploads that one array again
+    assert out["uploads_again"] == 0
+    assert out["write_tracking"] == 1
+    assert out["uploads_mod"] == 1
+    vmod = read_out(d, "csr_val_mod", dtype)
+    assert not np.array_equal(vmod, val)
+    ymod = oracle.spmv_csr(N, rp, col, vmod, x, z, 1)
+    err = np.abs(read_out(d, "y_csr_mod", dtype).astype(np.float64) - ymod)
+    assert np.all(err <= row_bound(rp, col, vmod, x, z, dtype))
+
+    # parallel-tree SIMD variants: CSR/COO within the bound, column-major
+    # DIA/ELL bit for bit (each row keeps its stored order)
+    for name, (yref, bnd) in {"par_csr_gs": close["csr"], "par_coo_gs": close["coo"]}.items():
+        err = np.abs(read_out(d, "y_" + name, dtype).astype(np.float64) - yref.astype(np.float64))
+        assert np.all(err <= bnd), name
+    for name, base in (("par_dia_basic", "dia"), ("par_bdia", "dia"), ("par_ell_gs", "ell"), ("par_bell_gs", "ell")):
+        assert np.array_equal(read_out(d, "y_" + name, dtype).view(np.uint8), exact[base].view(np.uint8)), name
+    sr = spec["short_rows"]
+    srp, scol, sval = a["sr_row"], a["sr_col"], a["sr_val"]
+    ysr = oracle.spmv_csr(N, srp, scol, sval, x, z, im)
+    bsr = row_bound(srp, scol, sval, x, z, dtype) * im + row_bound(srp, scol, sval, x, ysr, dtype) * (im - 1)
+    for fn in ("spmv_csr_short_rows_wrapper", "spmv_csr_gs_short_wrapper", "spmv_csr_unroll2_wrapper",
+               "spmv_csr_unroll3_wrapper", "spmv_csr_unroll4_wrapper", "spmv_csr_unroll6_wrapper"):
+        y = read_out(d, "y_" + fn, dtype)
+        assert np.all(y[:sr["one"]] == 0), fn  # another worker's rows are not touched
+        err = np.abs(y[sr["one"]:].astype(np.float64) - ysr[sr["one"]:])
+        assert np.all(err <= bsr[sr["one"]:]), fn
+    assert "short-row group sizes" in out["short_rows_mismatch"]
+    # row-major storage: conversions and products bit-exact
+    _, ri, rd = oracle.csr_ell_row(N, rp, col, val)
+    assert np.array_equal(read_out(d, "rell_idx", i32), ri)
+    assert np.array_equal(read_out(d, "rell_data", dtype), rd)
+    _, roff, rdd = oracle.csr_dia_row(N, rp, col, val)
+    assert np.array_equal(read_out(d, "rdia_off", i32), roff)
+    assert np.array_equal(read_out(d, "rdia_data", dtype), rdd)
+    for name, yref in (("par_ell_row", oracle.spmv_ell_row(N, nc, ri, rd, x, z, inner_max=im)),
+                       ("par_ell_row_gs", oracle.spmv_ell_row(N, nc, ri, rd, x, z, gs=True, inner_max=im)),
+                       ("par_dia_row", oracle.spmv_dia_row(N, roff, rdd, x, z, inner_max=im))):
+        assert np.array_equal(read_out(d, "y_" + name, dtype).view(np.uint8), yref.view(np.uint8)), name
 
     # transposes
     trp, tc, tv = oracle.transpose_csr(N, rp, col, val)

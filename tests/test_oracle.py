@@ -446,3 +446,90 @@ def test_reorder_nn_is_a_permutation_that_groups_shared_lines():
             s += len(la & lb)
         return s
     assert shared(orp, oc, perm) > 2 * shared(orp, oc, np.arange(N))
+
+
+# ------------------------------------------------------------------ row-major layouts
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_rowmajor_conversions_are_the_transposed_layouts(dtype):
+    rng = np.random.default_rng(11)
+    rp, col, _ = gen.laplacian_2d(9)
+    N = len(rp) - 1
+    val = rng.uniform(-1, 1, len(col)).astype(dtype)
+    nc, ci, cd = oracle.csr_ell(N, rp, col, val)
+    nr, ri, rd = oracle.csr_ell_row(N, rp, col, val)
+    assert nr == nc == 5
+    assert np.array_equal(ri.reshape(N, nc), ci.reshape(nc, N).T)
+    assert np.array_equal(rd.reshape(N, nc), cd.reshape(nc, N).T)
+    nd, _, off, dd = oracle.csr_dia(N, rp, col, val)
+    nd2, off2, rdd = oracle.csr_dia_row(N, rp, col, val)
+    assert nd2 == nd == 5 and np.array_equal(off2, off) and list(off) == [-9, -1, 0, 1, 9]
+    assert np.array_equal(rdd.reshape(N, nd), dd.reshape(nd, N).T)
+    # dense round trip of the row-major DIA
+    A = dense_from_csr(N, rp, col, val)
+    B = np.zeros_like(A)
+    for i in range(N):
+        for k in range(nd):
+            c = i + off2[k]
+            if 0 <= c < N:
+                B[i, c] = rdd[i * nd + k]
+    assert np.array_equal(A, B)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_rowmajor_spmv_equals_column_major_bits(dtype):
+    rng = np.random.default_rng(12)
+    rp, col, _ = gen.laplacian_2d(12)
+    N = len(rp) - 1
+    val = rng.uniform(-1, 1, len(col)).astype(dtype)
+    x = rng.uniform(-1, 1, N).astype(dtype)
+    y0 = rng.uniform(-1, 1, N).astype(dtype)
+    nd, _, off, dd = oracle.csr_dia(N, rp, col, val)
+    _, _, rdd = oracle.csr_dia_row(N, rp, col, val)
+    yc = oracle.spmv_dia(N, N, off, dd, x, y0, 3)
+    yr = oracle.spmv_dia_row(N, off, rdd, x, y0, inner_max=3)
+    assert np.array_equal(yc.view(np.uint8), yr.view(np.uint8))
+    nc, ci, cd = oracle.csr_ell(N, rp, col, val)
+    _, ri, rd = oracle.csr_ell_row(N, rp, col, val)
+    ye = oracle.spmv_ell(N, nc, ci, cd, x, y0, 2)
+    yer = oracle.spmv_ell_row(N, nc, ri, rd, x, y0, inner_max=2)
+    assert np.array_equal(ye.view(np.uint8), yer.view(np.uint8))
+    # a row range touches only its rows
+    part = oracle.spmv_dia_row(N, off, rdd, x, y0, r0=20, r1=50)
+    full = oracle.spmv_dia_row(N, off, rdd, x, y0)
+    assert np.array_equal(part[20:50], full[20:50])
+    assert np.array_equal(part[:20], y0[:20]) and np.array_equal(part[50:], y0[50:])
+
+
+def test_rowmajor_ell_stops_at_negative_index_and_gs_order():
+    f = np.float32
+    # plain order leaves the row at the first negative index
+    idx = np.array([0, 1, -1, 2], np.int32)
+    data = np.array([1, 2, 4, 8], f)
+    x = np.array([1, 1, 1], f)
+    y = oracle.spmv_ell_row(1, 4, idx, data, x, np.zeros(1, f))
+    assert y[0] == 3
+    # gs order, width 8: lanes take slots k and k + 4; with s0 = 2^24,
+    # s1 = 1, s4 = -2^24 the lane sums are (0, 1, 0, 0), total 1, where the
+    # sequential order rounds 2^24 + 1 to 2^24 and ends at 0
+    idx = np.zeros(8, np.int32)
+    data = np.array([2.0 ** 24, 1, 0, 0, -2.0 ** 24, 0, 0, 0], f)
+    x1 = np.ones(1, f)
+    assert oracle.spmv_ell_row(1, 8, idx, data, x1, np.zeros(1, f), gs=True)[0] == 1
+    assert oracle.spmv_ell_row(1, 8, idx, data, x1, np.zeros(1, f), gs=False)[0] == 0
+    # width 5: slot 0 is the head, added to y before the lanes
+    data5 = np.array([1, 2.0 ** 24, 1, 0, 0], f)
+    y5 = oracle.spmv_ell_row(1, 5, np.zeros(5, np.int32), data5, x1, np.array([2.0 ** 24], f), gs=True)
+    # head: 2^24 + 1 -> 2^24; lanes (2^24, 1, 0, 0): ((2^24 + 2^24) + 1) -> 2^25
+    assert y5[0] == 2.0 ** 25
+    # width < 4 takes the plain order
+    d3 = np.array([1, 2, 3], f)
+    assert oracle.spmv_ell_row(1, 3, np.zeros(3, np.int32), d3, x1, np.zeros(1, f), gs=True)[0] == 6
+    # exact integer products agree with the dense product in both orders
+    rng = np.random.default_rng(5)
+    N, nc = 40, 6
+    ci = rng.integers(0, N, N * nc).astype(np.int32)
+    cd = rng.integers(-5, 6, N * nc).astype(np.float64)
+    xv = rng.integers(-3, 4, N).astype(np.float64)
+    ref = (cd.reshape(N, nc) * xv[ci.reshape(N, nc)]).sum(1)
+    for gs in (False, True):
+        assert np.array_equal(oracle.spmv_ell_row(N, nc, ci, cd, xv, np.zeros(N), gs=gs), ref)
