@@ -531,10 +531,16 @@ ResidentCsr& coo_resident(Prec* P, size_t row, size_t col, size_t val, int64_t n
         if (sorted) {
             check(sable_coo_plan_row_ptr(R.coo, &rp), "spmv_coo");
         } else {
-            check(sable_device_alloc(g_ctx, 4 * ((size_t)N + 1), &R.own[0]), "device alloc");
+            // sorted as a square matrix of max(rows, columns) (columns may lie
+            // beyond the last row); rows past the last stored one are empty,
+            // so the first N + 1 row heads are the CSR of the N rows
+            int32_t mc = -1;
+            check(sable_max_i32(g_ctx, c, nnz, &mc), "spmv_coo");
+            const int32_t Ns = mc + 1 > N ? mc + 1 : N;
+            check(sable_device_alloc(g_ctx, 4 * ((size_t)Ns + 1), &R.own[0]), "device alloc");
             check(sable_device_alloc(g_ctx, 4 * (size_t)nnz, &R.own[1]), "device alloc");
             check(sable_device_alloc(g_ctx, P->vs * (size_t)nnz, &R.own[2]), "device alloc");
-            check(sable_coo_csr(g_ctx, P->dt, N, nnz, (const int32_t*)mr.d, c, v, (int32_t*)R.own[0],
+            check(sable_coo_csr(g_ctx, P->dt, Ns, nnz, (const int32_t*)mr.d, c, v, (int32_t*)R.own[0],
                                 (int32_t*)R.own[1], R.own[2]),
                   "spmv_coo");
             rp = (const int32_t*)R.own[0];
@@ -1775,7 +1781,6 @@ napi_value make_parallel(napi_env env, Prec* P, Prec* serial) {
         {"spmv_ell_col_wrapper", par_spmv_ell_col_wrapper}, {"spmv_ell_col_gs_wrapper", par_spmv_ell_col_wrapper},
         {"spmv_bell_col_gs_wrapper", par_spmv_ell_col_wrapper},
         {"spmv_ell_wrapper", par_spmv_ell_row_wrapper}, {"spmv_ell_row_gs_wrapper", par_spmv_ell_row_gs_wrapper},
-        {"csr_ell_row", csr_ell_row}, {"csr_dia_row", csr_dia_row},
         {"memcpy", lm_memcpy}, {"spts_level_csr", spts_level_csr},
         {"spts_csr_sync_free_wrapper", spts_csr_sync_free_wrapper},
         {"spts_csr_level_sync_free_wrapper", spts_csr_level_sync_free_wrapper},
@@ -1784,6 +1789,9 @@ napi_value make_parallel(napi_env env, Prec* P, Prec* serial) {
     };
     for (auto& f : fns) add_fn(env, m, f.name, f.cb, P);
     add_fn(env, m, "sum", par_sum, serial);  // sum takes no worker id
+    // the row-major conversions are JS functions of the parallel tree (no worker id)
+    add_fn(env, m, "csr_ell_row", csr_ell_row, serial);
+    add_fn(env, m, "csr_dia_row", csr_dia_row, serial);
     add_ew(env, m, P);
     return m;
 }
