@@ -667,6 +667,12 @@ int batch_spts_launch(hipStream_t st, int num_cu, sable_dtype_t dtype, int lanes
     const int64_t cap = (int64_t)num_cu * 8;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
+    // rows are claimed from a 32-bit ticket, at most WAVE per wave past the
+    // end before every wave sees it exhausted: the last claim must not wrap
+    const int64_t overshoot = blocks * (BB / WAVE) * (int64_t)WAVE;
+    SABLE_CHECK(total_rows + overshoot < ((int64_t)1 << 32), SABLE_EINVAL,
+                "batched solve of %lld rows: the 32-bit row ticket holds fewer than %lld rows",
+                (long long)total_rows, (long long)(((int64_t)1 << 32) - overshoot));
 #define SF_LAUNCH(T, G)                                                                                          \
     hipLaunchKernelGGL((k_batch_syncfree<T, G>), dim3((unsigned)blocks), dim3(BB), 0, st, items, nitems, total_rows, \
                        ticket, err, spin)

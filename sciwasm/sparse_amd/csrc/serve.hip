@@ -161,6 +161,49 @@ __global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __r
 
 static size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// The server's matrices: append-only, at addresses that never change.  The
+// scheduler and completion threads and the submitting threads index it
+// without the server lock while add_matrix appends under it: a matrix is
+// complete before the count that covers it is published (release), and
+// every id is checked against the count (acquire) before it is used.  A
+// fixed directory of chunks holds up to MAX matrices, so no append moves
+// the directory a reader may be walking.
+class MatrixTable {
+  public:
+    static constexpr int32_t CHUNK = 1024, DIRS = 1024, MAX = CHUNK * DIRS;
+    MatrixTable() {
+        for (auto& d : dir_) d.store(nullptr, std::memory_order_relaxed);
+    }
+    ~MatrixTable() {
+        for (auto& d : dir_) delete[] d.load(std::memory_order_relaxed);
+    }
+    MatrixTable(const MatrixTable&) = delete;
+    MatrixTable& operator=(const MatrixTable&) = delete;
+    int32_t size() const { return n_.load(std::memory_order_acquire); }
+    SMatrix& operator[](int32_t i) { return dir_[i / CHUNK].load(std::memory_order_acquire)[i % CHUNK]; }
+    const SMatrix& operator[](int32_t i) const {
+        return dir_[i / CHUNK].load(std::memory_order_acquire)[i % CHUNK];
+    }
+    // One writer at a time (the caller holds the server lock).  Returns the
+    // new id, or -1 when the table is full.
+    int32_t push_back(SMatrix&& m) {
+        const int32_t i = n_.load(std::memory_order_relaxed);
+        if (i >= MAX) return -1;
+        SMatrix* c = dir_[i / CHUNK].load(std::memory_order_relaxed);
+        if (!c) {
+            c = new SMatrix[CHUNK];
+            dir_[i / CHUNK].store(c, std::memory_order_release);
+        }
+        c[i % CHUNK] = std::move(m);
+        n_.store(i + 1, std::memory_order_release);
+        return i;
+    }
+
+  private:
+    std::atomic<SMatrix*> dir_[DIRS];
+    std::atomic<int32_t> n_{0};
+};
+
 }  // namespace sable
 
 using namespace sable;
@@ -169,7 +212,7 @@ struct sable_server_s {
     sable_ctx_t* ctx = nullptr;
     sable_server_config_t cfg{};
     int64_t used = 0;
-    std::deque<SMatrix> mats;
+    MatrixTable mats;
     std::vector<Slot> slots;
     std::mutex mu;
     std::condition_variable cv_pending, cv_slot, cv_done, cv_complete;
@@ -660,7 +703,7 @@ static void free_matrix(SMatrix& m) {
 }
 
 static void free_server(sable_server_s* s) {
-    for (auto& m : s->mats) free_matrix(m);
+    for (int32_t k = 0; k < s->mats.size(); k++) free_matrix(s->mats[k]);
     for (auto& sl : s->slots) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         if (sl.arena) (void)hipFree(sl.arena);
@@ -853,16 +896,24 @@ SABLE_API int sable_server_add_matrix(sable_server_t* s, int32_t N, const int32_
         }
     }
     if (rc) {
+        // m.bytes counts the matrix and every plan already charged
+        const int64_t charged = (int64_t)m.bytes;
         free_matrix(m);
         std::lock_guard<std::mutex> lk(s->mu);
-        s->used -= (int64_t)bytes;
+        s->used -= charged;
         return rc;
     }
     SABLE_HIP(hipStreamSynchronize(st));
     m.lp = std::move(h_lp);
     std::lock_guard<std::mutex> lk(s->mu);
-    *matrix_id = (int32_t)s->mats.size();
-    s->mats.push_back(std::move(m));
+    const int64_t charged = (int64_t)m.bytes;
+    const int32_t id = s->mats.push_back(std::move(m));
+    if (id < 0) {
+        free_matrix(m);
+        s->used -= charged;
+        return fail(SABLE_EINVAL, "the server holds at most %d matrices", MatrixTable::MAX);
+    }
+    *matrix_id = id;
     return SABLE_OK;
 }
 

@@ -144,6 +144,33 @@ def test_batch_spmv_tiles(ctx, dtype, beta):
         assert np.all(d <= tol), (k, n, float(np.max(d - tol)))
 
 
+@pytest.mark.parametrize("few_units", [False, True], ids=["all", "under8units"])
+@pytest.mark.parametrize("workgroups", [1, 3, 7])
+def test_batch_spmv_tiles_small_grids(ctx, workgroups, few_units):
+    """Grids of fewer workgroups than claim ranges (8): the ranges without a
+    home workgroup are drained by stealing, and a batch of fewer than 8
+    units is complete; every matrix within the CSR bound."""
+    dtype = np.float64
+    rng = np.random.default_rng(6)
+    case = [c for c in tiles_case(dtype) if c[0] > 0]
+    if few_units:
+        case = [c for c in case if c[0] <= 4096][:3]  # one row tile each
+    As = [sp.csr_from_host(n, rp, c, v) for n, rp, c, v in case]
+    X = [rng.uniform(-1, 1, n).astype(dtype) for n, _, _, _ in case]
+    xs = [DeviceArray.from_host(x) for x in X]
+    ys = [DeviceArray.zeros(n, dtype) for n, _, _, _ in case]
+    plans = sp.batch_tile_plans(As)
+    total = sum(p.info()["row_tiles"] for p in plans)
+    if few_units:
+        assert total < 8
+    sp.batch_spmv_tiles(plans, xs, ys, workgroups=workgroups)
+    ctx.sync()
+    for k, (n, rp, c, v) in enumerate(case):
+        yref = oracle.spmv_csr(n, rp, c, v, X[k], np.zeros(n, dtype))
+        d = np.abs(ys[k].numpy() - yref)
+        assert np.all(d <= bound(rp, c, v, X[k], dtype)), (workgroups, k)
+
+
 @pytest.mark.parametrize("nrhs,beta", [(2, 0.0), (4, 0.5), (8, 0.0), (8, 1.0)])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_batch_spmv_tiles_multi_rhs(ctx, dtype, nrhs, beta):
