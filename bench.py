@@ -643,6 +643,12 @@ def sec_reorder(args, sp, ctx, T, host):
 
 # ------------------------------------------------------------------ spts
 def sec_spts(args, sp, ctx, T):
+    """Lower solves on the 3D Laplacian 128^3 and the random 1M-row lower
+    matrix: the row-block schedule (default block size and a sweep), the
+    level graph and the tuned choice between them beside it (unless
+    --no-spts-graph), sync-free, combined and FP32 + FP64 refinement; residuals against the long-double oracle, and
+    the row-block level solve checked bit for bit against the oracle's
+    reference-order level solve."""
     import oracle
     from sciwasm.sparse_amd import gen
     out = {}
@@ -658,24 +664,66 @@ def sec_spts(args, sp, ctx, T):
         bv = sp.vec_from_host(b, ctx=ctx)
         xv = sp.create_vec(N, 0.0, np.float64, ctx=ctx)
         flops = 2 * nnz - N
-        res = {"N": N, "nnz": nnz, "nlevels": an.nlevels, "lanes": an.lanes, "analysis_s": r4(t_an)}
-        for meth, run in (("level_graph", lambda: an.solve_level(bv, xv)),
-                          ("syncfree", lambda: an.solve_syncfree(bv, xv)),
-                          ("combined", lambda: an.solve_combined(bv, xv)),
-                          ("fp32_refine2", lambda: an.solve_refined(bv, xv, iters=2))):
-            ms = T(run, 10, 2)
+        bytes_solve = 4 * (N + 1) + 12 * nnz + 16 * N
+        res = {"N": N, "nnz": nnz, "nlevels": an.nlevels, "lanes": an.lanes, "analysis_s": r4(t_an),
+               "schedule": an.schedule_info()}
+
+        def timed(key, run, reps=10):
+            ms = T(run, reps, 2)
             an.status()
             rr = oracle.rel_residual(N, rp, col, val, xv.array(), b)
-            res[meth] = {"ms": r4(ms), "gflops": r4(flops / (ms * 1e-3) / 1e9), "rel_residual": rr}
-        an1 = sp.SptsAnalysis(L, lanes=1)
-        ms = T(lambda: an1.solve_syncfree(bv, xv), 10, 2)
-        res["syncfree_lanes1_ms"] = r4(ms)
+            res[key] = {"ms": r4(ms), "gflops": r4(flops / (ms * 1e-3) / 1e9),
+                        "gbs": r4(bytes_solve / (ms * 1e-3) / 1e9), "rel_residual": rr}
+            return ms
+
+        t0 = time.perf_counter()
+        nl, lp, perm, order, _ = oracle.level_sets(N, rp, col)
+        nrp, nc_, nv = oracle.level_reorder(N, rp, col, val, perm, order)
+        yl = oracle.spts_level(N, nrp, nc_, nv, b[perm])
+        want = np.empty_like(yl)
+        want[perm] = yl
+        res["cpu_level_ms"] = r4((time.perf_counter() - t0) * 1e3)
+        best = None
+        for B in [0] + [int(v) for v in args.spts_blocks.split(",") if v.strip()]:
+            an.set_schedule("blocks", B)
+            key = "blocks_%d" % an.schedule_info()["block_rows"]
+            ms = timed(key, lambda: an.solve_level(bv, xv))
+            res[key]["bit_exact_vs_oracle_level"] = bool(np.array_equal(xv.array(), want))
+            res[key]["nsteps"] = an.schedule_info()["nsteps"]
+            if best is None or ms < best[1]:
+                best = (an.schedule_info()["block_rows"], ms)
+        an.set_schedule("blocks", best[0])
+        res["blocks_best"] = best[0]
+        want_sf = oracle.spts_syncfree(N, rp, col, val, b)
+        timed("syncfree_blocks", lambda: an.solve_syncfree(bv, xv))
+        res["syncfree_blocks"]["bit_exact_vs_oracle_syncfree"] = bool(np.array_equal(xv.array(), want_sf))
+        timed("combined", lambda: an.solve_combined(bv, xv))
+        res["combined"]["bit_exact_vs_oracle_level"] = bool(np.array_equal(xv.array(), want))
+        timed("fp32_refine1", lambda: an.solve_refined(bv, xv, iters=1, inner="level"))
+        timed("fp32_refine2", lambda: an.solve_refined(bv, xv, iters=2, inner="level"))
+        timed("fp32_refine1_syncfree", lambda: an.solve_refined(bv, xv, iters=1, inner="syncfree"))
+        if not args.no_spts_graph:
+            an.set_schedule("graph")
+            timed("level_graph", lambda: an.solve_level(bv, xv))
+            res["level_graph"]["bit_exact_vs_oracle_level"] = bool(np.array_equal(xv.array(), want))
+            timed("syncfree_natural", lambda: an.solve_syncfree(bv, xv))
+            res["syncfree_natural"]["bit_exact_vs_oracle_syncfree"] = bool(
+                np.array_equal(xv.array(), want_sf))
+            lanes0 = an.lanes
+            t0 = time.time()
+            tuned = an.tune()
+            tuned["tune_s"] = r4(time.time() - t0)
+            timed("level_tuned", lambda: an.solve_level(bv, xv))
+            res["level_tuned"]["bit_exact_vs_oracle_level"] = bool(np.array_equal(xv.array(), want))
+            res["level_tuned"]["choice"] = tuned
+            an.set_lanes(lanes0)
+            an.set_schedule("blocks")
         t0 = time.perf_counter()
         oracle.spts_syncfree(N, rp, col, val, b)
         res["cpu_serial_ms"] = r4((time.perf_counter() - t0) * 1e3)
         out[name] = res
         log("spts", name, res)
-        del an, an1, L
+        del an, L
     return out
 
 
@@ -916,6 +964,10 @@ def main():
     ap.add_argument("--serve-max-batch", type=int, default=4096)
     ap.add_argument("--serve-arena-mb", type=int, default=2048)
     ap.add_argument("--refine-iters", type=int, default=1)
+    ap.add_argument("--spts-blocks", type=str, default="2048,4096,8192,16384",
+                    help="comma list of row-block sizes of the triangular solves to time")
+    ap.add_argument("--no-spts-graph", action="store_true",
+                    help="leave the hipGraph level schedule out of the spts section")
     ap.add_argument("--group-rhs", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -610,8 +610,11 @@ SABLE_API void sable_mm_destroy(sable_mm_t* mm);
 /* L.x = b for a lower-triangular CSR whose rows each end with the diagonal
  * (the reference's layout; par/src/spts32.js:14-136 builds L = tril(A,-1)+I
  * that way).  lanes = lanes per row (1, 4, 8, 16 or 64; 0 = chosen from the
- * mean row length).  With lanes = 1 both solves keep the reference's per-row
- * operation order and are bit-identical to it. */
+ * mean row length) of the level graph and the natural-order sync-free
+ * solve: they form a row's products in parallel and add them in stored
+ * order.  Every schedule and lane count keeps the reference's per-row
+ * operation order, so the level-set and sync-free solves are bit-identical
+ * to the reference's. */
 typedef struct sable_spts_s sable_spts_t;
 
 /* Level-set analysis: checks the structure (SABLE_EMALFORMED) and the
@@ -631,14 +634,43 @@ SABLE_API int sable_spts_get_schedule(sable_ctx_t* ctx, const sable_spts_t* a, i
                                       int32_t* d_permutation, int32_t* d_order, int32_t* d_row_ptr, int32_t* d_col,
                                       void* d_val);
 
-/* Level-set solve with b and x in natural order: one launch per level,
- * captured once in a hipGraph and replayed.  Replaces spts_level_csr
- * (par/src/sparse_opt_32.wat:11759-11984 with spts_csr :11679-11757) and
- * the level + sync-free combined solve (:12846-13353). */
+/* Schedules of the level-set and sync-free solves of an analysis:
+ *  SABLE_SPTS_BLOCKS (default): one launch.  Rows are cut into blocks of
+ *    consecutive rows, claimed by workgroups in increasing order; inside a
+ *    block the rows run in level order, one level per step, with the
+ *    block's solution in LDS and a barrier per step, and a dependency on an
+ *    earlier block is waited for on its published value.  One thread per row, entries in stored order, so the
+ *    level solve's bits equal the reference level solve's and the sync-free
+ *    solve's equal the reference sync-free solve's.  block_rows: a power of
+ *    two in [64, 16384] (0 keeps the current one; the analysis starts with
+ *    about N/256 rows, in [1024, 8192]).
+ *  SABLE_SPTS_LEVEL_GRAPH: one launch per level over the level-ordered
+ *    matrix, captured once in a hipGraph and replayed (the analysis's lane
+ *    count per row); the sync-free solve: waves claim rows in natural order.
+ *  SABLE_SPTS_LEVEL_LAUNCHES: the same per-level launches without a graph. */
+enum { SABLE_SPTS_BLOCKS = 0, SABLE_SPTS_LEVEL_GRAPH = 1, SABLE_SPTS_LEVEL_LAUNCHES = 2 };
+SABLE_API int sable_spts_set_schedule(sable_spts_t* a, int schedule, int32_t block_rows);
+SABLE_API int sable_spts_schedule_info(const sable_spts_t* a, int* schedule, int32_t* block_rows,
+                                       int32_t* nblocks, int32_t* nsteps, int64_t* slots);
+/* Times the level-set solve under row blocks of 1024, 2048, ... 16384 rows
+ * (up to twice N) and under the level graph at 1, 4, 8 and 16 lanes per
+ * row, on b = L*1 in buffers of its own, and keeps the fastest schedule and
+ * lane count (the row-block schedule stays built at its fastest size either
+ * way).  The solution depends on neither, bit for bit.  Outputs may be NULL.
+ * Has no counterpart in the reference, whose solves take a fixed schedule. */
+SABLE_API int sable_spts_tune(sable_ctx_t* ctx, sable_spts_t* a, int* schedule, int32_t* block_rows, int* lanes,
+                              float* best_ms);
+
+/* Level-set solve with b and x in natural order (distinct arrays), on the
+ * analysis's schedule.  Replaces spts_level_csr
+ * (par/src/sparse_opt_32.wat:11759-11984 with spts_csr :11679-11757), whose
+ * workers run the levels of one call behind an in-call barrier, and the
+ * level + sync-free combined solve (:12846-13353). */
 SABLE_API int sable_spts_solve_level(sable_ctx_t* ctx, sable_spts_t* a, const void* b, void* x);
 
-/* Sync-free solve, natural order, one launch.  Replaces
- * spts_csr_sync_free(_wrapper) (par/src/spts_opt_32.wat:8-260). */
+/* Sync-free solve, natural order (b and x distinct), one launch, on the
+ * analysis's schedule.  Replaces spts_csr_sync_free(_wrapper)
+ * (par/src/spts_opt_32.wat:8-260). */
 SABLE_API int sable_spts_solve_syncfree(sable_ctx_t* ctx, sable_spts_t* a, const void* b, void* x);
 
 /* Level + sync-free combined solve, b and x in natural order: the
@@ -671,9 +703,8 @@ SABLE_API int sable_spts_csc_analyze(sable_ctx_t* ctx, sable_dtype_t dtype, int3
                                      const int32_t* row, const void* val, int lanes, sable_spts_t** out);
 /* The column sweep y = L^-1 b, inner_max times (y overwritten each time):
  * per row the subtractions happen in ascending column order, then the
- * division, so the sync-free solve at lanes = 1 on the transposed CSR
- * performs the same operations in the same order and the result is
- * bit-identical.  Replaces spts_csc / spts_csc_wrapper
+ * division, so the sync-free solve on the transposed CSR performs the
+ * same operations in the same order and the result is bit-identical.  Replaces spts_csc / spts_csc_wrapper
  * (seq/src/sparse_opt_32.wat:11-173).  Allocates and synchronises. */
 SABLE_API int sable_spts_csc(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* col_ptr,
                              const int32_t* row, const void* val, const void* b, void* y, int inner_max);
@@ -733,7 +764,9 @@ typedef struct {
 } sable_batch_spts_item_t;
 
 /* Sync-free solve of every item in one launch (synchronises; SABLE_ETIMEOUT
- * when a wait exceeded its bound). */
+ * when a wait exceeded its bound).  lanes per row (1, 4, 8, 16, 64) form a
+ * row's products in parallel and subtract them in stored order: the
+ * reference sync-free solve's bits at every lane count. */
 SABLE_API int sable_batch_spts(sable_ctx_t* ctx, sable_dtype_t dtype, int lanes, const sable_batch_spts_item_t* d_items,
                                int nitems, int64_t total_rows);
 

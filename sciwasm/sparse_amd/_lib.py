@@ -83,6 +83,11 @@ SIGNATURES = {
     "sable_memcpy_h2d_staged": [c_vp, c_vp, c_vp, c_sz],
     "sable_memcpy_d2h_staged": [c_vp, c_vp, c_vp, c_sz],
     "sable_max_i32": [c_vp, c_vp, c_i64, ctypes.POINTER(c_i32)],
+    "sable_spts_set_schedule": [c_vp, c_int, c_i32],
+    "sable_spts_schedule_info": [c_vp, ctypes.POINTER(c_int), ctypes.POINTER(c_i32), ctypes.POINTER(c_i32),
+                                 ctypes.POINTER(c_i32), ctypes.POINTER(c_i64)],
+    "sable_spts_tune": [c_vp, c_vp, ctypes.POINTER(c_int), ctypes.POINTER(c_i32), ctypes.POINTER(c_int),
+                        ctypes.POINTER(ctypes.c_float)],
     "sable_csr_tiles_create_rect": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_vpp],
     "sable_csr_auto_create": [c_vp, c_int, c_i32, c_i32, c_vp, c_vp, c_vp, c_int, c_vpp],
     "sable_csr_auto_destroy": [c_vp],

@@ -448,8 +448,11 @@ __global__ __launch_bounds__(BB) void k_batch_syncfree(const sable_batch_spts_it
         const int32_t* col = nullptr;
         const T* val = nullptr;
         T* x = nullptr;
-        int32_t i = 0, j = 0, e = 0;
-        T s = T(0), bi = T(0);
+        // j0: the group's next chunk of G entries; p: this lane's product of
+        // it once its x has been published
+        int32_t i = 0, j0 = 0, e = 0;
+        T s = T(0), p = T(0);
+        bool have = false;
         if (!published) {
             int lo = 0, hi = nitems - 1;
             while (lo < hi) {
@@ -462,32 +465,48 @@ __global__ __launch_bounds__(BB) void k_batch_syncfree(const sable_batch_spts_it
             val = (const T*)items[lo].val;
             x = (T*)items[lo].x;
             i = (int32_t)(g - items[lo].row_begin);
-            j = rp[i] + l;
+            j0 = rp[i];
             e = rp[i + 1] - 1;
-            bi = ((const T*)items[lo].b)[i];
-            if (G == 1) s = bi;
+            s = ((const T*)items[lo].b)[i];
         }
         long long spins = 0;
         while (__ballot(!published) != 0ull) {
             bool progressed = false;
             if (!published) {
-                while (j < e) {
-                    const T v = __hip_atomic_load(&x[col[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (is_bsent(v)) break;
-                    if (G == 1) s = sub_rn(s, mul_rn(val[j], v));
-                    else s = add_rn(s, mul_rn(val[j], v));
-                    j += G;
-                    progressed = true;
+                if (G == 1) {
+                    while (j0 < e) {
+                        const T v = __hip_atomic_load(&x[col[j0]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (is_bsent(v)) break;
+                        s = sub_rn(s, mul_rn(val[j0], v));
+                        j0++;
+                        progressed = true;
+                    }
+                } else {
+                    while (j0 < e) {
+                        const int32_t j = j0 + l;
+                        if (!have && j < e) {
+                            const T v = __hip_atomic_load(&x[col[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            if (!is_bsent(v)) {
+                                p = mul_rn(val[j], v);
+                                have = true;
+                            }
+                        }
+                        // group-uniform: the whole group leaves or goes on
+                        if ((__ballot(have || j >= e) & gmask) != gmask) break;
+                        const int n = min(G, e - j0);
+                        s = group_ordered_sum<true, G>(s, p, n);
+                        j0 += G;
+                        have = false;
+                        progressed = true;
+                    }
                 }
             }
-            const bool ldone = published || j >= e;
+            const bool ldone = published || j0 >= e;
             const uint64_t m = __ballot(ldone);
             const bool gdone = (m & gmask) == gmask;
-            T tot = s;
-            if (G > 1) tot = group_sum(s, G);
             if (!published && gdone) {
                 if (l == 0) {
-                    const T yv = (G == 1) ? div_rn(tot, val[e]) : div_rn(sub_rn(bi, tot), val[e]);
+                    const T yv = div_rn(s, val[e]);
                     __hip_atomic_store(&x[i], yv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 published = true;
@@ -576,9 +595,10 @@ static unsigned gsz(int64_t n) {
 }
 
 // One level of a batched level-set solve, one workgroup per chunk of the
-// level's rows: G lanes per row; t = sum of val*x over the row's
-// off-diagonal entries, x[i] = (b[i] - t) / diag.  With G = 1 the per-row
-// order is the level-set solve's t = 0 + v0*x0 + ...
+// level's rows: G lanes per row; t = 0 + v0*x0 + v1*x1 + ... over the row's
+// off-diagonal entries in stored order (the lanes form G products at once,
+// then every lane of the group adds them in order), x[i] = (b[i] - t) /
+// diag: the level-set solve's bits at every G.
 template <typename T, int G>
 __global__ __launch_bounds__(BB) void k_batch_level(const sable_batch_spts_item_t* __restrict__ items,
                                                     const RowChunk* __restrict__ chunks) {
@@ -588,17 +608,26 @@ __global__ __launch_bounds__(BB) void k_batch_level(const sable_batch_spts_item_
     const bool valid = g < c.count;
     const sable_batch_spts_item_t& it = items[c.item];
     T s = T(0);
-    int32_t i = 0, e = 0;
+    int32_t i = 0, a = 0, e = 0;
     if (valid) {
         i = c.rows ? c.rows[g] : c.first + g;
-        const int32_t* __restrict__ rp = it.row_ptr;
-        const int32_t* __restrict__ col = it.col;
-        const T* __restrict__ val = (const T*)it.val;
-        const T* __restrict__ x = (const T*)it.x;
-        e = rp[i + 1] - 1;
-        for (int32_t j = rp[i] + l; j < e; j += G) s = add_rn(s, mul_rn(val[j], x[col[j]]));
+        a = it.row_ptr[i];
+        e = it.row_ptr[i + 1] - 1;
     }
-    if (G > 1) s = group_sum(s, G);
+    const int32_t* __restrict__ col = it.col;
+    const T* __restrict__ val = (const T*)it.val;
+    const T* __restrict__ x = (const T*)it.x;
+    // a and e are the same on the lanes of a group, so is every trip count
+    for (int32_t j0 = a; j0 < e; j0 += G) {
+        const int32_t j = j0 + l;
+        const T p = j < e ? mul_rn(val[j], x[col[j]]) : T(0);
+        if (G == 1) {
+            s = add_rn(s, p);
+        } else {
+            const int n = min(G, e - j0);
+            s = group_ordered_sum<false, G>(s, p, n);
+        }
+    }
     if (valid && l == 0) {
         const T d = ((const T*)it.val)[e];
         ((T*)it.x)[i] = div_rn(sub_rn(((const T*)it.b)[i], s), d);

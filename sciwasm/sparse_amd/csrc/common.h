@@ -126,6 +126,25 @@ __device__ __forceinline__ T group_sum(T v, int L) {
     return v;
 }
 
+// s += p (SUB: s -= p) for the values p of lanes 0 .. n-1 of an aligned
+// group of G lanes, in lane order: the serial sum's bits.  The values are
+// read across the group eight at a time before they are added, so the reads
+// overlap; every lane of the group gets the same s.
+template <bool SUB, int G, typename T>
+__device__ __forceinline__ T group_ordered_sum(T s, T p, int n) {
+    constexpr int W = G < 8 ? G : 8;
+#pragma unroll
+    for (int q0 = 0; q0 < G; q0 += W) {
+        T v[W];
+#pragma unroll
+        for (int q = 0; q < W; q++) v[q] = __shfl(p, q0 + q, G);
+#pragma unroll
+        for (int q = 0; q < W; q++)
+            if (q0 + q < n) s = SUB ? sub_rn(s, v[q]) : add_rn(s, v[q]);
+    }
+    return s;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

@@ -64,6 +64,24 @@ struct sable_spts_s {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     // CSR copy owned by the analysis (built from a CSC input), else null
     void* owned[3] = {nullptr, nullptr, nullptr};
+    // Row-block schedule (k_blocked): rows cut into blocks of bl_rows; in a
+    // block, rows ordered by level and cut into steps of at most 256
+    // rows; each step's off-diagonals stored step-major (entry j of the
+    // step's row t at base + j*rows + t, padding column -1).
+    int schedule = SABLE_SPTS_BLOCKS;
+    int32_t bl_rows = 0;   // 0: not built
+    int32_t bl_nblk = 0, bl_nsteps = 0, bl_max_steps = 0, bl_width = 0;
+    int64_t bl_slots = 0;
+    int32_t* d_bl_blkstep = nullptr;  // nblk + 1 step ranges
+    int4* d_bl_steps = nullptr;       // {first schedule row, rows, width, entry base}
+    int32_t* d_bl_row = nullptr;      // N: row of each schedule position
+    void* d_bl_diag = nullptr;        // N diagonal values in schedule order (dtype)
+    int32_t* d_bl_col = nullptr;      // slots
+    void* d_bl_val = nullptr;         // slots (dtype)
+    float* d_bl_diag32 = nullptr;     // FP32 copies for the refined solve
+    float* d_bl_val32 = nullptr;
+    // host structure and levels kept to rebuild the block schedule
+    std::vector<int32_t> h_rp, h_col, h_level;
 };
 
 namespace sable {
@@ -95,7 +113,11 @@ __device__ __forceinline__ void store_agent(T* p, T v) {
 }
 
 // ------------------------------------------------------------- level-set
-// Rows [r0, r0+nrows) of a level; G lanes per row.
+// Rows [r0, r0+nrows) of a level; G lanes per row.  The lanes load and
+// multiply G entries of the row at once (entries j0 + l), and the sum then
+// takes those products in stored order: every lane of the group adds all of
+// them, read across the group, so the sum, and y, are the one-lane result
+// bit for bit whatever G.
 template <typename T, int G>
 __global__ __launch_bounds__(TB) void k_level(int32_t r0, int32_t nrows, const int32_t* __restrict__ rp,
                                               const int32_t* __restrict__ col, const T* __restrict__ val,
@@ -106,13 +128,22 @@ __global__ __launch_bounds__(TB) void k_level(int32_t r0, int32_t nrows, const i
     const bool valid = g < nrows;
     const int32_t i = r0 + (int32_t)(valid ? g : 0);
     T s = T(0);
-    int32_t e = 0;
+    int32_t a = 0, e = 0;
     if (valid) {
-        const int32_t a = rp[i];
+        a = rp[i];
         e = rp[i + 1] - 1;
-        for (int32_t j = a + l; j < e; j += G) s = add_rn(s, mul_rn(val[j], yl[col[j]]));
     }
-    if (G > 1) s = group_sum(s, G);
+    // a and e are the same on the lanes of a group, so is every trip count
+    for (int32_t j0 = a; j0 < e; j0 += G) {
+        const int32_t j = j0 + l;
+        const T p = j < e ? mul_rn(val[j], yl[col[j]]) : T(0);
+        if (G == 1) {
+            s = add_rn(s, p);
+        } else {
+            const int n = min(G, e - j0);
+            s = group_ordered_sum<false, G>(s, p, n);
+        }
+    }
     if (valid && l == 0) yl[i] = div_rn(sub_rn(bl[i], s), val[e]);
 }
 
@@ -189,9 +220,12 @@ __global__ void k_sentinel(int32_t N, T* __restrict__ x, unsigned* __restrict__ 
 }
 
 // LVL: the level solve's arithmetic (t = 0, t += val * x over the
-// off-diagonals, x = (b - t) / diag) also for G = 1, where the plain
-// sync-free solve follows the reference's t = b, t -= val * x.  With LVL
-// every lane count gives the level solve's result bit for bit.
+// off-diagonals, x = (b - t) / diag); else the reference sync-free solve's
+// t = b, t -= val * x, x = t / diag.  G lanes per row take the row's entries
+// G at a time: each lane waits for its own x and forms its product, and once
+// the group holds all G products they are added to t in stored order (by
+// every lane of the group, read across it), so every lane count gives the
+// one-lane result bit for bit.
 template <typename T, int G, bool LVL = false>
 __global__ __launch_bounds__(TB) void k_syncfree(int32_t N, const int32_t* __restrict__ rp,
                                                  const int32_t* __restrict__ col, const T* __restrict__ val,
@@ -209,38 +243,57 @@ __global__ __launch_bounds__(TB) void k_syncfree(int32_t N, const int32_t* __res
         if (base >= (unsigned)N) break;
         const int64_t i = (int64_t)base + grp;
         bool published = i >= N;
-        int32_t j = 0, e = 0;
+        // j0: the group's next chunk of G entries (the same on its lanes)
+        int32_t j0 = 0, e = 0;
         T s = T(0);
         T bi = T(0);
+        // this lane's product of the chunk, once its x has been published
+        T p = T(0);
+        bool have = false;
         if (!published) {
-            j = rp[i] + l;
+            j0 = rp[i];
             e = rp[i + 1] - 1;
             bi = b[i];
-            if (G == 1 && !LVL) s = bi;
+            if (!LVL) s = bi;
         }
         long long spins = 0;
         while (__ballot(!published) != 0ull) {
             bool progressed = false;
             if (!published) {
-                while (j < e) {
-                    const T v = load_agent(&x[col[j]]);
-                    if (Sent<T>::is(v)) break;
-                    if (G == 1 && !LVL)
-                        s = sub_rn(s, mul_rn(val[j], v));
-                    else
-                        s = add_rn(s, mul_rn(val[j], v));
-                    j += G;
-                    progressed = true;
+                if (G == 1) {
+                    while (j0 < e) {
+                        const T v = load_agent(&x[col[j0]]);
+                        if (Sent<T>::is(v)) break;
+                        s = LVL ? add_rn(s, mul_rn(val[j0], v)) : sub_rn(s, mul_rn(val[j0], v));
+                        j0++;
+                        progressed = true;
+                    }
+                } else {
+                    while (j0 < e) {
+                        const int32_t j = j0 + l;
+                        if (!have && j < e) {
+                            const T v = load_agent(&x[col[j]]);
+                            if (!Sent<T>::is(v)) {
+                                p = mul_rn(val[j], v);
+                                have = true;
+                            }
+                        }
+                        // group-uniform: the whole group leaves or goes on
+                        if ((__ballot(have || j >= e) & gmask) != gmask) break;
+                        const int n = min(G, e - j0);
+                        s = group_ordered_sum<!LVL, G>(s, p, n);
+                        j0 += G;
+                        have = false;
+                        progressed = true;
+                    }
                 }
             }
-            const bool ldone = published || j >= e;
+            const bool ldone = published || j0 >= e;
             const uint64_t m = __ballot(ldone);
             const bool gdone = (m & gmask) == gmask;
-            T tot = s;
-            if (G > 1) tot = group_sum(s, G);
             if (!published && gdone) {
                 if (l == 0) {
-                    const T y = (G == 1 && !LVL) ? div_rn(tot, val[e]) : div_rn(sub_rn(bi, tot), val[e]);
+                    const T y = LVL ? div_rn(sub_rn(bi, s), val[e]) : div_rn(s, val[e]);
                     store_agent(&x[i], y);
                 }
                 published = true;
@@ -256,6 +309,181 @@ __global__ __launch_bounds__(TB) void k_syncfree(int32_t N, const int32_t* __res
                 __builtin_amdgcn_s_sleep(1);
             }
         }
+    }
+}
+
+// ------------------------------------------------------- row blocks
+// One workgroup per block of consecutive rows (claimed in increasing order
+// from a ticket, so every block a workgroup waits on is held by a running
+// workgroup).  Inside a block the rows run step by step in level order (one
+// level per step, at most SB rows), with the block's solution in LDS and a
+// barrier between steps; a dependency on an earlier block is read from x,
+// whose entries hold
+// a sentinel until they are published (one agent-scope store per row).  The
+// next step's rows and entries are loaded while the current step runs.  One
+// thread per row, entries in stored order: SF = false is the level solve's
+// arithmetic t = 0 + v0*y0 + ..., y = (b - t)/d; SF = true the sync-free
+// one, t = b - v0*y0 - ..., y = t/d.
+constexpr int SB = 256;  // rows per step, threads per workgroup
+// P (a template parameter): entries per row loaded ahead of their step, the
+// schedule's widest row rounded up to 4, 8 or 16; wider rows take the rest P
+// at a time in their step.
+
+template <typename T, int P>
+struct BlRow {
+    int32_t rid;
+    int32_t c[P];
+    T v[P];
+    T d;
+};
+
+template <typename T, int P>
+__device__ __forceinline__ void bl_load(BlRow<T, P>& r, const int4 st, const int32_t* __restrict__ srow,
+                                        const T* __restrict__ diag, const int32_t* __restrict__ scol,
+                                        const T* __restrict__ sval) {
+    const int t = threadIdx.x;
+    r.rid = -1;
+    if (t < st.y) {
+        const int32_t k = st.x + t;
+        r.rid = srow[k];
+        r.d = diag[k];
+#pragma unroll
+        for (int j = 0; j < P; j++) {
+            const bool in = j < st.z;
+            const int64_t e = (int64_t)st.w + (int64_t)j * st.y + t;
+            r.c[j] = in ? scol[e] : -1;
+            r.v[j] = in ? sval[e] : T(0);
+        }
+    }
+}
+
+// The x values of one group of P entries: local columns from LDS; columns
+// of earlier blocks with plain loads first, all issued together (a value
+// already published is final wherever it is cached, so only the sentinel
+// can be stale), then agent-scope polls, again all together, for the ones
+// that read the sentinel (a spin limit ends the wait with NaN and raises
+// err).  Padding (column -1) reads 0.
+template <typename T>
+__device__ __forceinline__ T load_cached(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <typename T, int P>
+__device__ __forceinline__ void bl_fetch(const int32_t (&c)[P], T (&xv)[P], const T* xl, int32_t r0, T* x,
+                                         int* err, long long spin) {
+    uint32_t pend = 0;
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+        if (c[j] >= r0) {
+            xv[j] = xl[c[j] - r0];
+        } else if (c[j] >= 0) {
+            xv[j] = load_cached(&x[c[j]]);
+            pend |= 1u << j;
+        } else {
+            xv[j] = T(0);
+        }
+    }
+    long long n = 0;
+    while (true) {
+#pragma unroll
+        for (int j = 0; j < P; j++)
+            if (((pend >> j) & 1u) && !Sent<T>::is(xv[j])) pend &= ~(1u << j);
+        if (!pend) break;
+        if (++n > spin) {
+            atomicOr(err, 4);
+#pragma unroll
+            for (int j = 0; j < P; j++)
+                if ((pend >> j) & 1u) xv[j] = T(__builtin_nan(""));
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int j = 0; j < P; j++)
+            if ((pend >> j) & 1u) xv[j] = load_agent(&x[c[j]]);
+    }
+}
+
+template <typename T, bool SF>
+__device__ __forceinline__ T bl_acc(T t, T v, T y) {
+    return SF ? sub_rn(t, mul_rn(v, y)) : add_rn(t, mul_rn(v, y));
+}
+
+template <typename T, bool SF, int P>
+__global__ __launch_bounds__(SB) void k_blocked(int32_t nblk, int32_t B, const int32_t* __restrict__ blkstep,
+                                                const int4* __restrict__ steps, const int32_t* __restrict__ srow,
+                                                const T* __restrict__ diag, const int32_t* __restrict__ scol,
+                                                const T* __restrict__ sval, const T* __restrict__ b, T* x,
+                                                unsigned* __restrict__ ticket, int* __restrict__ err,
+                                                long long spin) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* xl = reinterpret_cast<T*>(smem_raw);
+    __shared__ unsigned s_blk;
+    while (true) {
+        __syncthreads();  // the previous block's xl and s_blk are no longer read
+        if (threadIdx.x == 0) s_blk = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const unsigned blk = s_blk;
+        if (blk >= (unsigned)nblk) break;
+        const int32_t r0 = (int32_t)blk * B;
+        const int32_t s0 = blkstep[blk], s1 = blkstep[blk + 1];
+        BlRow<T, P> cur, nxt;
+        int4 st = s0 < s1 ? steps[s0] : make_int4(0, 0, 0, 0);
+        if (s0 < s1) bl_load(cur, st, srow, diag, scol, sval);
+        for (int32_t s = s0; s < s1; s++) {
+            const int4 stn = s + 1 < s1 ? steps[s + 1] : make_int4(0, 0, 0, 0);
+            if (s + 1 < s1) bl_load(nxt, stn, srow, diag, scol, sval);
+            if (cur.rid >= 0) {
+                const T bi = b[cur.rid];
+                T xv[P];
+                bl_fetch(cur.c, xv, xl, r0, x, err, spin);
+                T t = SF ? bi : T(0);
+#pragma unroll
+                for (int j = 0; j < P; j++)
+                    if (cur.c[j] >= 0) t = bl_acc<T, SF>(t, cur.v[j], xv[j]);
+                // entries past the first P, P at a time
+                for (int32_t j0 = P; j0 < st.z; j0 += P) {
+                    int32_t c[P];
+                    T v[P];
+#pragma unroll
+                    for (int j = 0; j < P; j++) {
+                        const bool in = j0 + j < st.z;
+                        const int64_t e = (int64_t)st.w + (int64_t)(j0 + j) * st.y + threadIdx.x;
+                        c[j] = in ? scol[e] : -1;
+                        v[j] = in ? sval[e] : T(0);
+                    }
+                    bl_fetch(c, xv, xl, r0, x, err, spin);
+#pragma unroll
+                    for (int j = 0; j < P; j++)
+                        if (c[j] >= 0) t = bl_acc<T, SF>(t, v[j], xv[j]);
+                }
+                const T y = SF ? div_rn(t, cur.d) : div_rn(sub_rn(bi, t), cur.d);
+                xl[cur.rid - r0] = y;
+                store_agent(&x[cur.rid], y);
+            }
+            __syncthreads();
+            cur = nxt;
+            st = stn;
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_bl_gather(int64_t n, const int32_t* __restrict__ idx, const T* __restrict__ val,
+                            T* __restrict__ out) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t j = idx[s];
+        out[s] = j >= 0 ? val[j] : T(0);
+    }
+}
+
+// b[i] = sum_j L_ij (a right-hand side whose solution is all ones, for
+// timing solves on a matrix without the caller's vectors).
+template <typename T>
+__global__ void k_rowsum(int32_t N, const int32_t* __restrict__ rp, const T* __restrict__ val, T* __restrict__ b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        T s = T(0);
+        for (int32_t j = rp[i]; j < rp[i + 1]; j++) s += val[j];
+        b[i] = s;
     }
 }
 
@@ -389,8 +617,193 @@ static int valid_lanes(int G) { return G == 1 || G == 4 || G == 8 || G == 16 || 
 
 constexpr long long DEFAULT_SPIN = 1ll << 24;
 
+// Rows per block: about 256 blocks for the matrix, between 1024 and 8192.
+static int32_t default_block_rows(int32_t N) {
+    const int64_t want = ((int64_t)N + 255) / 256;
+    int32_t B = 1024;
+    while (B < want && B < 8192) B *= 2;
+    return B;
+}
+
+static void free_blocked(sable_spts_s* a) {
+    void* ptrs[] = {a->d_bl_blkstep, a->d_bl_steps, a->d_bl_row, a->d_bl_diag, a->d_bl_col, a->d_bl_val,
+                    a->d_bl_diag32, a->d_bl_val32};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    a->d_bl_blkstep = nullptr;
+    a->d_bl_steps = nullptr;
+    a->d_bl_row = nullptr;
+    a->d_bl_diag = nullptr;
+    a->d_bl_col = nullptr;
+    a->d_bl_val = nullptr;
+    a->d_bl_diag32 = nullptr;
+    a->d_bl_val32 = nullptr;
+    a->bl_rows = a->bl_nblk = a->bl_nsteps = a->bl_max_steps = a->bl_width = 0;
+    a->bl_slots = 0;
+}
+
+// The row-block schedule of k_blocked for blocks of B rows, from the host
+// copy of the structure; the values are gathered on the device from the
+// caller's natural-order array.
+static int build_blocked(sable_spts_s* a, int32_t B) {
+    SABLE_CHECK(B >= 64 && B <= 16384 && (B & (B - 1)) == 0, SABLE_EINVAL,
+                "block rows must be a power of two in [64, 16384]");
+    free_blocked(a);
+    const int32_t N = a->N;
+    const std::vector<int32_t>& rp = a->h_rp;
+    const std::vector<int32_t>& c = a->h_col;
+    const std::vector<int32_t>& lev = a->h_level;
+    const int32_t nblk = (int32_t)(((int64_t)N + B - 1) / B);
+    std::vector<int32_t> blkstep((size_t)nblk + 1), srow(N), didx(N), eidx, scol, ord, first;
+    std::vector<int4> steps;
+    int64_t slots = 0;
+    int32_t k = 0, max_steps = 0, max_width = 0;
+    for (int32_t blk = 0; blk < nblk; blk++) {
+        const int32_t r0 = blk * B, r1 = (int32_t)std::min<int64_t>(N, (int64_t)r0 + B);
+        // Rows of the block in level order (stable), one level per step: a
+        // row's dependencies inside the block then lie in earlier steps, and
+        // a step waits only for rows of lower levels, so every block moves
+        // through the levels in step with the others instead of waiting for
+        // the whole of an earlier block.
+        ord.resize((size_t)(r1 - r0));
+        for (int32_t i = r0; i < r1; i++) ord[i - r0] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](int32_t p, int32_t q) { return lev[p] < lev[q]; });
+        first.clear();
+        for (int32_t p = 0; p < r1 - r0; p++)
+            if (p == 0 || lev[ord[p]] != lev[ord[p - 1]]) first.push_back(p);
+        first.push_back(r1 - r0);
+        const int32_t L = (int32_t)first.size() - 1;
+        blkstep[blk] = (int32_t)steps.size();
+        for (int32_t l = 0; l < L; l++)
+            for (int32_t p = first[l]; p < first[l + 1]; p += SB) {
+                const int32_t n = std::min(SB, first[l + 1] - p);
+                int32_t width = 0;
+                for (int32_t t = 0; t < n; t++) width = std::max(width, rp[ord[p + t] + 1] - 1 - rp[ord[p + t]]);
+                SABLE_CHECK(slots + (int64_t)width * n < 0x7fffffffLL, SABLE_EINVAL,
+                            "row-block schedule: more than 2^31 entry slots");
+                steps.push_back(make_int4(k, n, width, (int)slots));
+                max_width = std::max(max_width, width);
+                eidx.resize((size_t)(slots + (int64_t)width * n));
+                scol.resize(eidx.size());
+                for (int32_t t = 0; t < n; t++) {
+                    const int32_t i = ord[p + t], a0 = rp[i], len = rp[i + 1] - 1 - a0;
+                    srow[k + t] = i;
+                    didx[k + t] = rp[i + 1] - 1;
+                    for (int32_t j = 0; j < width; j++) {
+                        const size_t sl = (size_t)(slots + (int64_t)j * n + t);
+                        eidx[sl] = j < len ? a0 + j : -1;
+                        scol[sl] = j < len ? c[a0 + j] : -1;
+                    }
+                }
+                slots += (int64_t)width * n;
+                k += n;
+            }
+        max_steps = std::max(max_steps, (int32_t)steps.size() - blkstep[blk]);
+    }
+    blkstep[nblk] = (int32_t)steps.size();
+    const size_t vs = a->dtype == SABLE_F64 ? 8 : 4;
+    const size_t ns = std::max<size_t>(steps.size(), 1), sl = std::max<size_t>((size_t)slots, 1);
+    int32_t* d_eidx = nullptr;
+    int32_t* d_didx = nullptr;
+    bool ok = hipMalloc(&a->d_bl_blkstep, 4 * ((size_t)nblk + 1)) == hipSuccess &&
+              hipMalloc(&a->d_bl_steps, sizeof(int4) * ns) == hipSuccess &&
+              hipMalloc(&a->d_bl_row, 4 * (size_t)N) == hipSuccess && hipMalloc(&a->d_bl_diag, vs * N) == hipSuccess &&
+              hipMalloc(&a->d_bl_col, 4 * sl) == hipSuccess && hipMalloc(&a->d_bl_val, vs * sl) == hipSuccess &&
+              hipMalloc(&d_eidx, 4 * sl) == hipSuccess && hipMalloc(&d_didx, 4 * (size_t)N) == hipSuccess;
+    auto done = [&](int rc) {
+        if (d_eidx) (void)hipFree(d_eidx);
+        if (d_didx) (void)hipFree(d_didx);
+        if (rc) free_blocked(a);
+        return rc;
+    };
+    if (!ok) {
+        (void)hipGetLastError();
+        return done(fail(SABLE_ENOMEM, "row-block schedule allocation failed"));
+    }
+    hipStream_t st = a->ctx->stream;
+    hipError_t e = hipMemcpyAsync(a->d_bl_blkstep, blkstep.data(), 4 * blkstep.size(), hipMemcpyHostToDevice, st);
+    if (!e && !steps.empty())
+        e = hipMemcpyAsync(a->d_bl_steps, steps.data(), sizeof(int4) * steps.size(), hipMemcpyHostToDevice, st);
+    if (!e) e = hipMemcpyAsync(a->d_bl_row, srow.data(), 4 * (size_t)N, hipMemcpyHostToDevice, st);
+    if (!e) e = hipMemcpyAsync(d_didx, didx.data(), 4 * (size_t)N, hipMemcpyHostToDevice, st);
+    if (!e && slots) e = hipMemcpyAsync(a->d_bl_col, scol.data(), 4 * (size_t)slots, hipMemcpyHostToDevice, st);
+    if (!e && slots) e = hipMemcpyAsync(d_eidx, eidx.data(), 4 * (size_t)slots, hipMemcpyHostToDevice, st);
+    if (e) return done(fail(SABLE_EHIP, "row-block schedule upload: %s", hipGetErrorString(e)));
+    if (a->dtype == SABLE_F64) {
+        hipLaunchKernelGGL(k_bl_gather<double>, dim3(gsz(N)), dim3(TB), 0, st, (int64_t)N, d_didx,
+                           (const double*)a->val, (double*)a->d_bl_diag);
+        if (slots)
+            hipLaunchKernelGGL(k_bl_gather<double>, dim3(gsz(slots)), dim3(TB), 0, st, slots, d_eidx,
+                               (const double*)a->val, (double*)a->d_bl_val);
+    } else {
+        hipLaunchKernelGGL(k_bl_gather<float>, dim3(gsz(N)), dim3(TB), 0, st, (int64_t)N, d_didx,
+                           (const float*)a->val, (float*)a->d_bl_diag);
+        if (slots)
+            hipLaunchKernelGGL(k_bl_gather<float>, dim3(gsz(slots)), dim3(TB), 0, st, slots, d_eidx,
+                               (const float*)a->val, (float*)a->d_bl_val);
+    }
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        return done(fail(SABLE_EHIP, "row-block schedule build failed"));
+    a->bl_rows = B;
+    a->bl_nblk = nblk;
+    a->bl_nsteps = (int32_t)steps.size();
+    a->bl_max_steps = max_steps;
+    a->bl_width = max_width;
+    a->bl_slots = slots;
+    return done(SABLE_OK);
+}
+
+// FP32 copies of the block schedule's values (refined solve).
+static int ensure_bl32(sable_spts_s* a) {
+    if (a->d_bl_val32) return SABLE_OK;
+    SABLE_CHECK(a->dtype == SABLE_F64 && a->bl_rows, SABLE_EINVAL, "no FP64 row-block schedule");
+    const size_t sl = std::max<size_t>((size_t)a->bl_slots, 1);
+    SABLE_HIP(hipMalloc(&a->d_bl_diag32, 4 * (size_t)a->N));
+    SABLE_HIP(hipMalloc(&a->d_bl_val32, 4 * sl));
+    hipStream_t st = a->ctx->stream;
+    hipLaunchKernelGGL(k_to_f32, dim3(gsz(a->N)), dim3(TB), 0, st, (int64_t)a->N, (const double*)a->d_bl_diag,
+                       a->d_bl_diag32);
+    if (a->bl_slots)
+        hipLaunchKernelGGL(k_to_f32, dim3(gsz(a->bl_slots)), dim3(TB), 0, st, a->bl_slots,
+                           (const double*)a->d_bl_val, a->d_bl_val32);
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+// x = L^-1 b over the row-block schedule (b and x natural order, distinct).
+template <typename T, bool SF>
+static int launch_blocked(hipStream_t st, int num_cu, sable_spts_s* a, const T* diag, const T* val, const T* b,
+                          T* x) {
+    hipLaunchKernelGGL(k_sentinel<T>, dim3(gsz(a->N)), dim3(TB), 0, st, a->N, x, a->d_ticket);
+    const size_t shm = (size_t)a->bl_rows * sizeof(T);
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(a->bl_nblk, (int64_t)num_cu * 2));
+    // a dependency is waited for up to 2^20 polls (about a second): every
+    // row a block waits on belongs to an earlier block held by a running
+    // workgroup, so a wait that long is an error, reported by status
+    constexpr long long BL_SPIN = 1ll << 20;
+#define BL_LAUNCH(PP)                                                                                          \
+    do {                                                                                                       \
+        if (shm > 65536 - 1024)                                                                                \
+            SABLE_HIP(hipFuncSetAttribute((const void*)k_blocked<T, SF, PP>,                                   \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));              \
+        hipLaunchKernelGGL((k_blocked<T, SF, PP>), dim3((unsigned)grid), dim3(SB), shm, st, a->bl_nblk,         \
+                           a->bl_rows, a->d_bl_blkstep, a->d_bl_steps, a->d_bl_row, diag, a->d_bl_col, val, b, x, \
+                           a->d_ticket, a->d_err, BL_SPIN);                                                     \
+    } while (0)
+    if (a->bl_width <= 4)
+        BL_LAUNCH(4);
+    else if (a->bl_width <= 8)
+        BL_LAUNCH(8);
+    else
+        BL_LAUNCH(16);
+#undef BL_LAUNCH
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
 static void free_analysis(sable_spts_s* a) {
     if (!a) return;
+    free_blocked(a);
     for (int k = 0; k < 2; k++) {
         if (a->exec[k]) (void)hipGraphExecDestroy(a->exec[k]);
         if (a->graph[k]) (void)hipGraphDestroy(a->graph[k]);
@@ -415,6 +828,9 @@ static int capture_levels(sable_spts_s* a, bool use32) {
         (void)hipGraphDestroy(a->graph[k]);
         a->graph[k] = nullptr;
     }
+    // The capture stream lives until the graph is instantiated, the capture
+    // is always ended (also when a launch inside it failed), and a failed
+    // capture leaves no graph behind.
     hipStream_t cs;
     SABLE_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
@@ -423,19 +839,31 @@ static int capture_levels(sable_spts_s* a, bool use32) {
         return fail(SABLE_EHIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
     }
     const T* val = use32 ? (const T*)a->d_lval32 : (const T*)a->d_val;
-    levels_on_stream<T>(cs, a->lanes, a->h_level_ptr, a->d_rp, a->d_col, val, (const T*)a->d_bl, (T*)a->d_yl);
+    const int lrc =
+        levels_on_stream<T>(cs, a->lanes, a->h_level_ptr, a->d_rp, a->d_col, val, (const T*)a->d_bl, (T*)a->d_yl);
     e = hipStreamEndCapture(cs, &a->graph[k]);
+    if (e == hipSuccess && lrc == SABLE_OK) e = hipGraphInstantiate(&a->exec[k], a->graph[k], nullptr, nullptr, 0);
     (void)hipStreamDestroy(cs);
-    if (e != hipSuccess) return fail(SABLE_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&a->exec[k], a->graph[k], nullptr, nullptr, 0);
-    if (e != hipSuccess) return fail(SABLE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    if (e != hipSuccess || lrc != SABLE_OK) {
+        if (a->exec[k]) (void)hipGraphExecDestroy(a->exec[k]);
+        if (a->graph[k]) (void)hipGraphDestroy(a->graph[k]);
+        a->exec[k] = nullptr;
+        a->graph[k] = nullptr;
+        return lrc != SABLE_OK ? lrc : fail(SABLE_EHIP, "level graph capture: %s", hipGetErrorString(e));
+    }
     a->graph_lanes[k] = a->lanes;
     return SABLE_OK;
 }
 
-// Level solve on level-ordered buffers bl -> yl (graph replay).
+// Level solve on level-ordered buffers bl -> yl: graph replay, or the
+// per-level launches themselves.
 template <typename T>
 static int solve_level_ordered(sable_spts_s* a, bool use32) {
+    if (a->schedule == SABLE_SPTS_LEVEL_LAUNCHES) {
+        const T* val = use32 ? (const T*)a->d_lval32 : (const T*)a->d_val;
+        return levels_on_stream<T>(a->ctx->stream, a->lanes, a->h_level_ptr, a->d_rp, a->d_col, val,
+                                   (const T*)a->d_bl, (T*)a->d_yl);
+    }
     int rc = capture_levels<T>(a, use32);
     if (rc) return rc;
     SABLE_HIP(hipGraphLaunch(a->exec[use32 ? 1 : 0], a->ctx->stream));
@@ -565,7 +993,138 @@ SABLE_API int sable_spts_analyze(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t 
         free_analysis(a);
         return rc ? rc : fail(SABLE_ESINGULAR, "zero on the diagonal");
     }
+    a->h_rp = std::move(rp);
+    a->h_col = std::move(c);
+    a->h_level = std::move(level);
+    rc = build_blocked(a, default_block_rows(N));
+    if (rc) {
+        free_analysis(a);
+        return rc;
+    }
     *out = a;
+    return SABLE_OK;
+}
+
+SABLE_API int sable_spts_set_schedule(sable_spts_t* a, int schedule, int32_t block_rows) {
+    SABLE_CHECK(a, SABLE_EINVAL, "NULL analysis");
+    SABLE_CHECK(schedule == SABLE_SPTS_BLOCKS || schedule == SABLE_SPTS_LEVEL_GRAPH ||
+                    schedule == SABLE_SPTS_LEVEL_LAUNCHES,
+                SABLE_EINVAL, "unknown schedule %d", schedule);
+    if (schedule == SABLE_SPTS_BLOCKS && block_rows > 0 && block_rows != a->bl_rows) {
+        (void)hipStreamSynchronize(a->ctx->stream);
+        const int rc = build_blocked(a, block_rows);
+        if (rc) return rc;
+    }
+    a->schedule = schedule;
+    return SABLE_OK;
+}
+
+SABLE_API int sable_spts_schedule_info(const sable_spts_t* a, int* schedule, int32_t* block_rows,
+                                       int32_t* nblocks, int32_t* nsteps, int64_t* slots) {
+    SABLE_CHECK(a, SABLE_EINVAL, "NULL analysis");
+    if (schedule) *schedule = a->schedule;
+    if (block_rows) *block_rows = a->bl_rows;
+    if (nblocks) *nblocks = a->bl_nblk;
+    if (nsteps) *nsteps = a->bl_nsteps;
+    if (slots) *slots = a->bl_slots;
+    return SABLE_OK;
+}
+
+// Times solve_level under each schedule (row blocks of 1024 to 16384 rows,
+// the level graph at 1, 4, 8 and 16 lanes per row) on b = L*1 and keeps the
+// fastest. Every schedule and lane count gives the same x bit for bit, so
+// the choice changes only the time.
+SABLE_API int sable_spts_tune(sable_ctx_t* ctx, sable_spts_t* a, int* schedule, int32_t* block_rows, int* lanes,
+                              float* best_ms) {
+    SABLE_CHECK(ctx && a, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(ctx == a->ctx, SABLE_EINVAL, "analysis belongs to another context");
+    hipStream_t st = ctx->stream;
+    const int32_t N = a->N;
+    const size_t vs = a->dtype == SABLE_F64 ? 8 : 4;
+    void* b = nullptr;
+    void* x = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SABLE_OK;
+    if (hipMalloc(&b, vs * (size_t)N) != hipSuccess || hipMalloc(&x, vs * (size_t)N) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = fail(SABLE_ENOMEM, "tune: allocation failed");
+    }
+    if (!rc) {
+        if (a->dtype == SABLE_F64)
+            hipLaunchKernelGGL(k_rowsum<double>, dim3(gsz(N)), dim3(TB), 0, st, N, a->rp, (const double*)a->val,
+                               (double*)b);
+        else
+            hipLaunchKernelGGL(k_rowsum<float>, dim3(gsz(N)), dim3(TB), 0, st, N, a->rp, (const float*)a->val,
+                               (float*)b);
+        if (hipGetLastError() != hipSuccess) rc = fail(SABLE_EHIP, "tune: launch failed");
+    }
+    // one untimed solve (it also captures the graph), then the best of 5
+    auto time_one = [&](float* out) -> int {
+        int r = sable_spts_solve_level(ctx, a, b, x);
+        float best = 1e30f;
+        for (int it = 0; it < 5 && !r; it++) {
+            if (hipEventRecord(e0, st) != hipSuccess) return fail(SABLE_EHIP, "tune: event record failed");
+            r = sable_spts_solve_level(ctx, a, b, x);
+            if (r) break;
+            float ms = 0.f;
+            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                return fail(SABLE_EHIP, "tune: event timing failed");
+            best = std::min(best, ms);
+        }
+        if (!r) r = sable_spts_status(ctx, a);
+        *out = best;
+        return r;
+    };
+    int best_s = SABLE_SPTS_BLOCKS;
+    int32_t best_b = 0;
+    float best = 1e30f;
+    const int saved = a->schedule, saved_lanes = a->lanes;
+    int graph_lanes = saved_lanes;
+    for (int32_t B = 1024; B <= 16384 && !rc; B *= 2) {
+        if (B > 1024 && B >= 2 * (int64_t)N) break;
+        rc = sable_spts_set_schedule(a, SABLE_SPTS_BLOCKS, B);
+        float ms = 0.f;
+        if (!rc) rc = time_one(&ms);
+        if (!rc && ms < best) {
+            best = ms;
+            best_b = B;
+            best_s = SABLE_SPTS_BLOCKS;
+        }
+    }
+    for (int G : {1, 4, 8, 16}) {
+        if (rc) break;
+        a->schedule = SABLE_SPTS_LEVEL_GRAPH;
+        a->lanes = G;
+        float ms = 0.f;
+        rc = time_one(&ms);
+        if (!rc && ms < best) {
+            best = ms;
+            best_s = SABLE_SPTS_LEVEL_GRAPH;
+            graph_lanes = G;
+        }
+    }
+    if (!rc) {
+        // the row-block schedule stays built at its best size, also when the
+        // graph wins (the sync-free solve runs on it)
+        rc = sable_spts_set_schedule(a, SABLE_SPTS_BLOCKS, best_b);
+        if (!rc) a->schedule = best_s;
+        a->lanes = best_s == SABLE_SPTS_LEVEL_GRAPH ? graph_lanes : saved_lanes;
+    } else {
+        a->schedule = saved;
+        a->lanes = saved_lanes;
+    }
+    (void)hipStreamSynchronize(st);
+    if (b) (void)hipFree(b);
+    if (x) (void)hipFree(x);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (schedule) *schedule = a->schedule;
+    if (block_rows) *block_rows = a->bl_rows;
+    if (lanes) *lanes = a->lanes;
+    if (best_ms) *best_ms = best;
     return SABLE_OK;
 }
 
@@ -609,6 +1168,13 @@ SABLE_API int sable_spts_solve_level(sable_ctx_t* ctx, sable_spts_t* a, const vo
     SABLE_CHECK(ctx == a->ctx, SABLE_EINVAL, "analysis belongs to another context");
     hipStream_t st = ctx->stream;
     const int32_t N = a->N;
+    if (a->schedule == SABLE_SPTS_BLOCKS) {
+        if (a->dtype == SABLE_F64)
+            return launch_blocked<double, false>(st, ctx->num_cu, a, (const double*)a->d_bl_diag,
+                                                 (const double*)a->d_bl_val, (const double*)b, (double*)x);
+        return launch_blocked<float, false>(st, ctx->num_cu, a, (const float*)a->d_bl_diag, (const float*)a->d_bl_val,
+                                            (const float*)b, (float*)x);
+    }
     if (a->dtype == SABLE_F64) {
         hipLaunchKernelGGL(k_gather_perm<double>, dim3(gsz(N)), dim3(TB), 0, st, N, a->d_perm, (const double*)b,
                            (double*)a->d_bl);
@@ -631,6 +1197,13 @@ SABLE_API int sable_spts_solve_level(sable_ctx_t* ctx, sable_spts_t* a, const vo
 SABLE_API int sable_spts_solve_syncfree(sable_ctx_t* ctx, sable_spts_t* a, const void* b, void* x) {
     SABLE_CHECK(ctx && a && b && x, SABLE_EINVAL, "NULL argument");
     SABLE_CHECK(ctx == a->ctx, SABLE_EINVAL, "analysis belongs to another context");
+    if (a->schedule == SABLE_SPTS_BLOCKS) {
+        if (a->dtype == SABLE_F64)
+            return launch_blocked<double, true>(ctx->stream, ctx->num_cu, a, (const double*)a->d_bl_diag,
+                                                (const double*)a->d_bl_val, (const double*)b, (double*)x);
+        return launch_blocked<float, true>(ctx->stream, ctx->num_cu, a, (const float*)a->d_bl_diag,
+                                           (const float*)a->d_bl_val, (const float*)b, (float*)x);
+    }
     if (a->dtype == SABLE_F64)
         launch_syncfree<double>(ctx->stream, a->lanes, ctx->num_cu, a->N, a->rp, a->col, (const double*)a->val,
                                 (const double*)b, (double*)x, a->d_ticket, a->d_err, DEFAULT_SPIN);
@@ -708,7 +1281,14 @@ SABLE_API int sable_spts_solve_refined(sable_ctx_t* ctx, sable_spts_t* a, const 
             }
         }
         // L32 d = r (FP32)
-        if (inner == 1) {
+        if (a->schedule == SABLE_SPTS_BLOCKS) {
+            rc = ensure_bl32(a);
+            if (rc) return rc;
+            rc = inner == 1 ? launch_blocked<float, true>(st, ctx->num_cu, a, a->d_bl_diag32, a->d_bl_val32, r32, d32)
+                            : launch_blocked<float, false>(st, ctx->num_cu, a, a->d_bl_diag32, a->d_bl_val32, r32,
+                                                           d32);
+            if (rc) return rc;
+        } else if (inner == 1) {
             launch_syncfree<float>(st, G, ctx->num_cu, N, a->rp, a->col, a->d_val32, r32, d32, a->d_ticket, a->d_err,
                                    DEFAULT_SPIN);
         } else {

@@ -43,6 +43,35 @@ class SptsAnalysis:
         call("sable_spts_set_lanes", self.handle, int(lanes))
         self.lanes = int(lanes)
 
+    SCHEDULES = {"blocks": 0, "graph": 1, "launches": 2}
+
+    def set_schedule(self, schedule="blocks", block_rows=0):
+        """'blocks' (row blocks stepping through the levels, one launch; block_rows a
+        power of two, 0 keeps the current), 'graph' (one launch per level in
+        a hipGraph) or 'launches' (the per-level launches without a graph)."""
+        call("sable_spts_set_schedule", self.handle, self.SCHEDULES[schedule], int(block_rows))
+
+    def tune(self):
+        """Times the solve under each schedule and lane count and keeps the
+        fastest (the solution is the same bit for bit); returns
+        schedule_info() plus the lanes kept and the best time in ms."""
+        s, b, ln, ms = ctypes.c_int(), c_i32(), ctypes.c_int(), ctypes.c_float()
+        call("sable_spts_tune", self.ctx.handle, self.handle, ctypes.byref(s), ctypes.byref(b), ctypes.byref(ln),
+             ctypes.byref(ms))
+        self.lanes = ln.value
+        info = self.schedule_info()
+        info["lanes"] = self.lanes
+        info["ms"] = ms.value
+        return info
+
+    def schedule_info(self):
+        s, b, nb, ns, sl = ctypes.c_int(), c_i32(), c_i32(), c_i32(), ctypes.c_int64()
+        call("sable_spts_schedule_info", self.handle, ctypes.byref(s), ctypes.byref(b), ctypes.byref(nb),
+             ctypes.byref(ns), ctypes.byref(sl))
+        names = {v: k for k, v in self.SCHEDULES.items()}
+        return {"schedule": names[s.value], "block_rows": b.value, "nblocks": nb.value, "nsteps": ns.value,
+                "slots": sl.value}
+
     def schedule(self):
         """(level_ptr, permutation, order, row_ptr, col, val) as numpy arrays."""
         L = self.L

@@ -1,14 +1,15 @@
 """GPU lower triangular solve against the oracle.
 
 * Level schedule (level_ptr, permutation, order, reordered CSR): bit-exact.
-* lanes = 1 keeps the reference's per-row operation order, so the level-set
-  solve is bit-identical to spts_csr/spts_level_csr and the sync-free solve
-  to spts_csr_sync_free (both restated in the oracle).
-* lanes > 1 reassociates each row's dot product: the result must satisfy
-  the acceptance bound of the FP64 path, relative residual
-  ||b - L x||_inf / (||L||_inf ||x||_inf + ||b||_inf) <= 1e-12 against the
-  long-double oracle (FP32 solves: <= 1e-5).
-* FP32 solve + FP64 refinement must reach the same 1e-12 bound."""
+* Every schedule and lane count keeps the reference's per-row operation
+  order (lanes form the products of a row in parallel and add them in
+  stored order), so the level-set solve is bit-identical to
+  spts_csr/spts_level_csr and the sync-free solve to spts_csr_sync_free
+  (both restated in the oracle).
+* FP32 solve + FP64 refinement must reach the acceptance bound of the FP64
+  path, relative residual ||b - L x||_inf / (||L||_inf ||x||_inf +
+  ||b||_inf) <= 1e-12 against the long-double oracle (FP32 solves:
+  <= 1e-5)."""
 import numpy as np
 import pytest
 
@@ -70,19 +71,33 @@ def test_lanes1_bit_exact(ctx, dtype, name):
 
 
 @pytest.mark.parametrize("lanes", [4, 8, 16, 64])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("name", list(MATS))
-def test_multilane_residual(ctx, lanes, name):
+def test_multilane_bit_exact(ctx, lanes, dtype, name):
+    """More lanes per row in the level graph and the natural-order sync-free
+    solve: the products of a row are formed in parallel and added in stored
+    order, so the results are the reference solves' bit for bit."""
     rp, c, v = MATS[name]()
+    v = v.astype(dtype)
     N = len(rp) - 1
-    b = np.random.default_rng(2).uniform(-1, 1, N)
-    an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v), lanes=lanes)
+    b = np.random.default_rng(2).uniform(-1, 1, N).astype(dtype)
+    L = sp.csr_from_host(N, rp, c, v)
+    an = sp.SptsAnalysis(L, lanes=lanes)
+    an.set_schedule("graph")
     bv = sp.vec_from_host(b)
-    for method in ("level", "syncfree"):
-        x = sp.create_vec(N, 0.0, np.float64)
-        getattr(an, "solve_" + method)(bv, x)
+    want_l = host_level_solution(N, rp, c, v, b)
+    want_s = oracle.spts_syncfree(N, rp, c, v, b)
+    for _ in range(2):
+        xl = sp.create_vec(N, 0.0, dtype)
+        an.solve_level(bv, xl)
+        xs = sp.create_vec(N, 0.0, dtype)
+        an.solve_syncfree(bv, xs)
         an.status()
-        r = oracle.rel_residual(N, rp, c, v, x.array(), b)
-        assert r <= 1e-12, (method, r)
+        assert np.array_equal(xl.array().view(np.uint8), want_l.view(np.uint8))
+        assert np.array_equal(xs.array().view(np.uint8), want_s.view(np.uint8))
+    xr = sp.create_vec(N, 0.0, dtype)
+    sp.spts_syncfree(L, bv, xr, lanes=lanes)
+    assert np.array_equal(xr.array().view(np.uint8), want_s.view(np.uint8))
 
 
 @pytest.mark.parametrize("lanes", [1, 4, 16, 64])
@@ -90,8 +105,8 @@ def test_multilane_residual(ctx, lanes, name):
 @pytest.mark.parametrize("name", list(MATS))
 def test_combined_equals_level(ctx, lanes, dtype, name):
     """The combined solve (sync-free over the level-ordered matrix) uses the
-    level solve's arithmetic: bit-identical to it at every lane count, and
-    at lanes = 1 to the reference-order level solve of the oracle."""
+    level solve's arithmetic: bit-identical to it and to the reference-order
+    level solve of the oracle at every lane count."""
     rp, c, v = MATS[name]()
     v = v.astype(dtype)
     N = len(rp) - 1
@@ -99,20 +114,21 @@ def test_combined_equals_level(ctx, lanes, dtype, name):
     an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v), lanes=lanes)
     bv = sp.vec_from_host(b)
     xl = sp.create_vec(N, 0.0, dtype)
+    an.set_schedule("graph")  # the per-level launches, with the analysis's lanes
     an.solve_level(bv, xl)
     for _ in range(2):
         xc = sp.create_vec(N, 0.0, dtype)
         an.solve_combined(bv, xc)
         an.status()
         assert np.array_equal(xc.array(), xl.array())
-    if lanes == 1:
-        assert np.array_equal(xc.array(), host_level_solution(N, rp, c, v, b))
+    assert np.array_equal(xc.array(), host_level_solution(N, rp, c, v, b))
 
 
 def test_graph_replays_match_eager(ctx):
     rp, c, v = gen.random_lower(30000, 16, seed=5)
     N = len(rp) - 1
     an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v), lanes=1)
+    an.set_schedule("graph")
     rng = np.random.default_rng(6)
     for k in range(4):
         b = rng.uniform(-1, 1, N)
@@ -241,3 +257,76 @@ def test_spts_csc_rejects_diagonal_not_first(ctx):
     with pytest.raises(SableError) as e:
         sp.spts_csc(A, sp.vec_from_host(np.ones(2)), sp.create_vec(2, 0.0, np.float64))
     assert e.value.name == "SABLE_EMALFORMED"
+
+
+BLOCK_MATS = dict(MATS, lap3d_32=lambda: gen.laplacian_3d_lower(32),
+                  random_lower_200k=lambda: gen.random_lower(200000, 32, seed=8),
+                  random_lower_w8=lambda: gen.random_lower(30000, 8, seed=9),
+                  single_row=lambda: (np.array([0, 1], np.int32), np.array([0], np.int32), np.array([2.0])))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("name", list(BLOCK_MATS))
+def test_row_blocks_bit_exact(ctx, dtype, name):
+    """The row-block schedule (one launch: a block's rows step through the
+    levels with its solution in LDS, published values across blocks) keeps
+    one thread per row and the stored entry order: its level solve equals
+    the reference level solve and its sync-free solve the reference
+    sync-free solve, bit for bit, for every block size, for schedules whose
+    widest row takes 4, 8 and 16 entries ahead (and more than 16, in a
+    second pass), and on a second solve with the same analysis."""
+    rp, c, v = BLOCK_MATS[name]()
+    v = v.astype(dtype)
+    N = len(rp) - 1
+    b = np.random.default_rng(11).uniform(-1, 1, N).astype(dtype)
+    an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v), lanes=4)
+    assert an.schedule_info()["schedule"] == "blocks"
+    want_l = host_level_solution(N, rp, c, v, b)
+    want_s = oracle.spts_syncfree(N, rp, c, v, b)
+    bv = sp.vec_from_host(b)
+    for B in (64, 1024, 16384, 0):
+        an.set_schedule("blocks", B)
+        info = an.schedule_info()
+        print(name, dtype.__name__, info, flush=True)
+        for _ in range(2):
+            xl = sp.create_vec(N, 0.0, dtype)
+            an.solve_level(bv, xl)
+            xs = sp.create_vec(N, 0.0, dtype)
+            an.solve_syncfree(bv, xs)
+            an.status()
+            assert np.array_equal(xl.array().view(np.uint8), want_l.view(np.uint8)), (name, B)
+            assert np.array_equal(xs.array().view(np.uint8), want_s.view(np.uint8)), (name, B)
+
+
+def test_row_blocks_schedule_shape(ctx):
+    """The 3D Laplacian in blocks of whole planes: a 16^3 grid in 256-row
+    blocks has 16 blocks, each holding 31 levels (i + j + k for its k) of at
+    most 16 rows: one step per level."""
+    rp, c, v = gen.laplacian_3d_lower(16)
+    an = sp.SptsAnalysis(sp.csr_from_host(len(rp) - 1, rp, c, v))
+    an.set_schedule("blocks", 256)
+    info = an.schedule_info()
+    assert info["nblocks"] == 16 and info["nsteps"] == 16 * 31
+    # every step holds its rows' off-diagonals, padded to the step's widest row
+    assert info["slots"] >= len(c) - (len(rp) - 1)
+    with pytest.raises(SableError):
+        an.set_schedule("blocks", 3000)
+
+
+@pytest.mark.parametrize("name", ["lap3d_32", "random_lower_20k"])
+def test_tune_keeps_bits(ctx, name):
+    """The tuner times the row-block sizes and the level graph at several
+    lane counts and keeps the fastest; whichever it keeps, the level solve
+    stays bit for bit the reference level solve."""
+    rp, c, v = BLOCK_MATS[name]()
+    N = len(rp) - 1
+    b = np.random.default_rng(12).uniform(-1, 1, N)
+    an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v))
+    info = an.tune()
+    print(name, info, flush=True)
+    assert info["schedule"] in ("blocks", "graph") and info["ms"] > 0
+    assert 1024 <= info["block_rows"] <= 16384 and info["lanes"] in (1, 4, 8, 16, 64)
+    x = sp.create_vec(N, 0.0, np.float64)
+    an.solve_level(sp.vec_from_host(b), x)
+    an.status()
+    assert np.array_equal(x.array(), host_level_solution(N, rp, c, v, b))
