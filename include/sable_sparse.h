@@ -141,6 +141,21 @@ SABLE_API int sable_spmv_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, c
                              const int32_t* col, const void* val, const void* x, void* y,
                              double beta, int inner_max);
 
+/* sable_spmv_csr without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (work under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * device-built row-block plan goes to a fixed work space in the library's
+ * code object, one copy per GPU (SABLE_NOALLOC_WORK_BYTES, held until the
+ * process ends), so N is at most SABLE_NOALLOC_MAX_ROWS (larger N:
+ * SABLE_EINVAL, nothing launched).  Same kernels, plan and bits as
+ * sable_spmv_csr.  Every call on one GPU shares that GPU's work space:
+ * calls that overlap on one GPU (other streams or contexts) race. */
+#define SABLE_NOALLOC_MAX_ROWS (1 << 23)
+#define SABLE_NOALLOC_WORK_BYTES (256 + 4 * ((SABLE_NOALLOC_MAX_ROWS / 2) + 1024 + 1))
+SABLE_API int sable_spmv_csr_noalloc(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                                     const int32_t* col, const void* val, const void* x, void* y, double beta,
+                                     int inner_max);
+
 /* Column-panel blocked CSR (no reference counterpart; a schedule of the same
  * CSR product for matrices whose x does not fit one XCD's L2): the columns
  * are cut into panels of panel_cols columns (0 = 2 MiB of x) and the matrix

@@ -113,6 +113,7 @@ SIGNATURES = {
     "sable_spmv_csr_plan": [c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_csr_kernel_select": [c_int],
     "sable_spmv_csr": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
+    "sable_spmv_csr_noalloc": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_int],
     "sable_csr_panels_create": [c_vp, c_int, c_i32, c_vp, c_vp, c_vp, c_i32, ctypes.POINTER(CsrTune), c_vpp],
     "sable_csr_panels_destroy": [c_vp],
     "sable_csr_panels_info": [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32), ctypes.POINTER(c_i64)],

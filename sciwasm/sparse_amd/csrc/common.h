@@ -52,6 +52,8 @@ struct sable_ctx_s {
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     bool stage_used[2] = {false, false};
     int stage_next = 0;
+    // device address of the allocation-free work space on this device
+    void* noalloc = nullptr;
     std::mutex mu;
 };
 
@@ -72,6 +74,9 @@ int syncfree_launch(hipStream_t st, int num_cu, sable_dtype_t dt, int lanes, int
                     const int32_t* col, const void* val, const void* b, void* x, unsigned* ticket, int* err,
                     long long spin);
 int ctx_scratch(sable_ctx_t* ctx, size_t bytes, void** out);
+// Address on `device` (the current device) of the allocation-free CSR
+// product's work space, looked up once per device.
+int spmv_noalloc_setup(int device, void** out);
 // Temporary device memory ordered on ctx->stream: usable by work enqueued on
 // that stream after the call, released by ctx_pool_free once the work that
 // uses it has been enqueued.  Without a memory pool it is plain hipMalloc.

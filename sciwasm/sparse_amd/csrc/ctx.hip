@@ -174,6 +174,13 @@ SABLE_API int sable_ctx_create(int device, void* hip_stream, sable_ctx_t** out) 
         delete ctx;
         return fail(SABLE_ENOMEM, "pinned host buffer allocation failed");
     }
+    const int rc = spmv_noalloc_setup(device, &ctx->noalloc);
+    if (rc) {
+        (void)hipHostFree(ctx->pinned);
+        if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return rc;
+    }
     *out = ctx;
     return SABLE_OK;
 }

@@ -785,21 +785,24 @@ static int coo_atomic_launch(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, i
     return SABLE_OK;
 }
 
-// Scratch of a device-built plan: the shape {C, T, cap}, the error flag and
-// cap + 1 block boundaries.
-size_t devplan_bytes(int64_t cap) { return 256 + sizeof(int32_t) * (size_t)(cap + 1); }
+// Work space of a device-built plan: the shape {C, T, cap}, the error flag
+// and cap + 1 block boundaries.
+constexpr size_t devplan_bytes(int64_t cap) { return 256 + sizeof(int32_t) * (size_t)(cap + 1); }
+// Block capacity from N alone: matrices whose default-weight plan would
+// need more blocks get proportionally heavier blocks.
+constexpr int64_t devplan_cap(int32_t N) { return (int64_t)N / 2 + 1024; }
 
 // y = A.x + beta*y, inner_max times, with the block plan built on the
-// device into `scratch` (devplan_bytes(cap) bytes): no host synchronisation
+// device into `work` (devplan_bytes(cap) bytes): no host synchronisation
 // and no allocation.  A malformed row_ptr leaves y unchanged.
 int devplan_spmv(hipStream_t st, int num_cu, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
                  const int32_t* col, const void* val, const void* x, void* y, double beta, int inner_max,
-                 void* scratch, int64_t cap) {
+                 void* work, int64_t cap) {
     int T, C;
     tune_defaults(nullptr, &T, &C);
-    int64_t* shape = (int64_t*)scratch;
-    int* err = (int*)((char*)scratch + 64);
-    int32_t* blk = (int32_t*)((char*)scratch + 256);
+    int64_t* shape = (int64_t*)work;
+    int* err = (int*)((char*)work + 64);
+    int32_t* blk = (int32_t*)((char*)work + 256);
     // shape and flag are written by a kernel (no memset node in captured work)
     hipLaunchKernelGGL(k_devplan_init, dim3(1), dim3(64), 0, st, shape, (int64_t)C, (int64_t)T, cap, err);
     hipLaunchKernelGGL(k_plan_fill_dev, dim3(grid_for((int64_t)N + 1, BS)), dim3(BS), 0, st, N, row_ptr, (int64_t)C,
@@ -820,6 +823,38 @@ int devplan_spmv(hipStream_t st, int num_cu, sable_dtype_t dtype, int32_t N, con
                                lds_cap, N, (const int64_t*)shape, (const int*)err);
         SABLE_LAUNCH_CHECK();
     }
+    return SABLE_OK;
+}
+
+// The allocation-free product's work space: the device-built plan of a
+// matrix of up to SABLE_NOALLOC_MAX_ROWS rows, one copy per GPU for the
+// life of the process.  Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in for the symbol): it uses the
+// device address that hipGetSymbolAddress returns on each device.
+static_assert(devplan_bytes(devplan_cap(SABLE_NOALLOC_MAX_ROWS)) <= SABLE_NOALLOC_WORK_BYTES,
+              "allocation-free work space too small for its row bound");
+static __device__ __align__(256) unsigned char scratch[(SABLE_NOALLOC_WORK_BYTES + 255) / 256 * 256];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int spmv_noalloc_setup(int device, void** out) {
+    *out = nullptr;
+    SABLE_CHECK(device >= 0 && device < NOALLOC_MAX_DEVICES, SABLE_EINVAL, "device %d: at most %d devices", device,
+                NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            (void)hipGetLastError();
+            return fail(SABLE_EHIP, "hipGetSymbolAddress(work space) on device %d: %s", device,
+                        hipGetErrorString(e));
+        }
+        g_noalloc_ptr[device] = p;
+    }
+    *out = g_noalloc_ptr[device];
     return SABLE_OK;
 }
 
@@ -942,13 +977,26 @@ SABLE_API int sable_spmv_csr(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, c
     SABLE_CHECK(N >= 0 && inner_max >= 0, SABLE_EINVAL, "negative size");
     SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
     if (N == 0 || inner_max == 0) return SABLE_OK;
-    // Block capacity from N alone: matrices whose default-weight plan would
-    // need more blocks get proportionally heavier blocks.
-    const int64_t cap = (int64_t)N / 2 + 1024;
+    const int64_t cap = devplan_cap(N);
     void* s;
     int rc = ctx_scratch(ctx, devplan_bytes(cap), &s);
     if (rc) return rc;
     return devplan_spmv(ctx->stream, ctx->num_cu, dtype, N, row_ptr, col, val, x, y, beta, inner_max, s, cap);
+}
+
+SABLE_API int sable_spmv_csr_noalloc(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, const int32_t* row_ptr,
+                                     const int32_t* col, const void* val, const void* x, void* y, double beta,
+                                     int inner_max) {
+    SABLE_CHECK(ctx && row_ptr, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(N >= 0 && inner_max >= 0, SABLE_EINVAL, "negative size");
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    SABLE_CHECK(N <= SABLE_NOALLOC_MAX_ROWS, SABLE_EINVAL,
+                "sable_spmv_csr_noalloc: N = %d rows exceeds the %d its fixed work space holds", N,
+                SABLE_NOALLOC_MAX_ROWS);
+    SABLE_CHECK(ctx->noalloc, SABLE_EINVAL, "sable_spmv_csr_noalloc: no work space on device %d", ctx->device);
+    if (N == 0 || inner_max == 0) return SABLE_OK;
+    return devplan_spmv(ctx->stream, ctx->num_cu, dtype, N, row_ptr, col, val, x, y, beta, inner_max, ctx->noalloc,
+                        devplan_cap(N));
 }
 
 SABLE_API int sable_spmv_coo(sable_ctx_t* ctx, sable_dtype_t dtype, int32_t N, int64_t nnz, const int32_t* row,
