@@ -30,8 +30,10 @@ Also measured and reported under "extra" (each section bounded to seconds):
   reorder sort_rows_by_nnz of the headline matrix, reorder_NN of a
           scattered 2D Laplacian
   spts    lower solves, 3D 7-point Laplacian 128^3 and a random 1,048,576-row
-          lower matrix: analysis, level-set (hipGraph), sync-free, FP32 +
-          FP64 refinement, residuals
+          lower matrix: analysis, level-set solve over row blocks (one
+          launch) and over levels (hipGraph, or per-level launches with
+          --no-spts-graph), the tuned choice, sync-free, combined, FP32 +
+          FP64 refinement; residuals and bit-exactness against the oracle
   batch   2,048 matrices in one persistent launch vs one launch per matrix
   batch_tiles  (not in the default list) only the FP64 batched tile product
           of the same pool, three launches, for hardware-counter runs
@@ -58,15 +60,36 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# sable_csr_tiles_select modes: (variant name, kernel)
+TILE_MODES = {0: ("perpanel", "k_tiles"), 1: ("fused", "k_tiles_fused")}
 # Fabric/HBM bytes per product measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (one counter per run, scaled by the readings of a 1 GiB copy:
 # FETCH x2, WRITE x1) for the headline matrix (seed 20261203) and schedule;
-# profiles/csr_spmv_fp64_zipf4M_*.md hold the runs.
+# profiles/csr_spmv_fp64_zipf4M_*.md hold the runs.  Each entry names the
+# source file of its kernel and the SHA-256 of that file when measured: a
+# changed kernel source reports traffic null until it is measured again.
 MEASURED_TRAFFIC = {
-    "panels_524288_bw1024_k0": {"per_product": 8.303e9, "per_launch": 1.038e9},
-    "panels_524288_bw1024_k3": {"per_product": 5.854e9, "per_launch": 0.7318e9},
-    "tiles_16384x8192_fused": {"per_product": 2.956e9, "per_launch": 2.956e9},
+    "tiles_16384x8192_fused": {
+        "per_product": 2.962e9, "per_launch": 2.962e9,
+        "source": "sciwasm/sparse_amd/csrc/spmv_tiles.hip",
+        "sha256": "5cbd39fb33abfc7657c55485cc49dc4dbe8669b074e21fd79fe4ee5096f8d674",
+        "limiter": "x gathers: every CU touches ~78% of x's 64-byte sectors (805k entries over 524k sectors), "
+                   "~52M L1 misses to the L2 per product beside ~39M streaming requests; fetched traffic "
+                   "1.16x the algorithmic bytes"},
 }
+
+
+def measured_traffic(variant, N):
+    """The measured traffic of a schedule on the headline matrix, or None
+    when it was not measured or its kernel source has changed since."""
+    import hashlib
+    m = MEASURED_TRAFFIC.get(variant)
+    if m is None or N != 4194304 or m["sha256"] is None:
+        return None
+    with open(os.path.join(ROOT, m["source"]), "rb") as f:
+        if hashlib.sha256(f.read()).hexdigest() != m["sha256"]:
+            return None
+    return m
 
 
 def log(*a):
@@ -157,11 +180,11 @@ def headline(args, sp, ctx, T):
         for mode in [int(v) for v in args.tiles_mode.split(",") if v.strip()]:
             L.call("sable_csr_tiles_select", mode)
             ms = T(lambda: tl.run(xv.vec, yv.vec, beta=0.0), args.steps, args.warmup)
-            key = "tiles_%dx%d_%s" % (tr, pc, ["perpanel", "fused"][mode])
+            key = "tiles_%dx%d_%s" % (tr, pc, TILE_MODES[mode][0])
             variants[key] = {"ms": ms, "gbs": bytes_step / (ms * 1e-3) / 1e9,
                              "launches_per_step": 1 if mode else info["npanels"],
                              "npanels": info["npanels"], "row_tiles": info["row_tiles"], "tile_rows": tr,
-                             "panel_cols": pc, "kernel": "k_tiles_fused" if mode else "k_tiles",
+                             "panel_cols": pc, "kernel": TILE_MODES[mode][1],
                              "plan_build_s": round(build_s, 3)}
             results[key] = yv.array()
             log(key, info, "built in %.2fs, %.4f ms  %.1f GB/s" % (build_s, ms, variants[key]["gbs"]))
@@ -718,6 +741,12 @@ def sec_spts(args, sp, ctx, T):
             res["level_tuned"]["choice"] = tuned
             an.set_lanes(lanes0)
             an.set_schedule("blocks")
+        else:
+            # the level kernels without a graph: one launch per level
+            an.set_schedule("launches")
+            timed("level_launches", lambda: an.solve_level(bv, xv))
+            res["level_launches"]["bit_exact_vs_oracle_level"] = bool(np.array_equal(xv.array(), want))
+            an.set_schedule("blocks")
         t0 = time.perf_counter()
         oracle.spts_syncfree(N, rp, col, val, b)
         res["cpu_serial_ms"] = r4((time.perf_counter() - t0) * 1e3)
@@ -855,21 +884,37 @@ def sec_batch_spts(args, sp, ctx, T, lows):
         bs.append(sp.DeviceArray.from_host(np.full(n, 0.5), ctx=ctx))
         xs.append(sp.DeviceArray(n, np.float64, ctx))
     rows = sum(lo[0] for lo in lows)
+    nnz = sum(len(lo[2]) for lo in lows)
     ms_b = T(lambda: sp.batch_spts(Ls, bs, xs, lanes=4), 5, 1)
-    ans = [sp.SptsAnalysis(Lm, lanes=4) for Lm in Ls[:64]]
-    bvs = [sp.sparse_vec_t(b, Lm.N) for b, Lm in zip(bs[:64], Ls[:64])]
-    xvs = [sp.sparse_vec_t(x, Lm.N) for x, Lm in zip(xs[:64], Ls[:64])]
+    ok_b = all(np.array_equal(x.numpy(), oracle_sf(lo, 0.5)) for x, lo in zip(xs[:16], lows[:16]))
+    t0 = time.time()
+    ans = [sp.SptsAnalysis(Lm, lanes=4) for Lm in Ls]
+    t_an = time.time() - t0
+    bvs = [sp.sparse_vec_t(b, Lm.N) for b, Lm in zip(bs, Ls)]
+    xvs = [sp.sparse_vec_t(x, Lm.N) for x, Lm in zip(xs, Ls)]
 
     def per():
         for a, b, x in zip(ans, bvs, xvs):
             a.solve_syncfree(b, x)
-    ms_p64 = T(per, 3, 1)
-    out = {"matrices": len(lows), "rows": rows, "batched_syncfree_ms": r4(ms_b),
-           "one_launch_per_matrix_ms_first64": r4(ms_p64),
-           "one_launch_per_matrix_ms_est_all": r4(ms_p64 * len(lows) / 64)}
-    out["speedup_est"] = r4(out["one_launch_per_matrix_ms_est_all"] / ms_b)
+    out = {"matrices": len(lows), "rows": rows, "nnz": nnz, "batched_syncfree_ms": r4(ms_b),
+           "batched_bit_exact_vs_oracle_first16": ok_b, "analysis_s_all": r4(t_an)}
+    # the per-matrix baseline over every matrix: each analysis's own sync-free
+    # solve (row blocks: a sentinel fill and one solve launch per matrix) and
+    # the natural-order sync-free kernel (the reference's per-matrix unit)
+    for name, sched in (("per_matrix_blocks_ms", "blocks"), ("per_matrix_natural_ms", "graph")):
+        for a in ans:
+            a.set_schedule(sched)
+        out[name] = r4(T(per, 3, 1))
+    best = min(out["per_matrix_blocks_ms"], out["per_matrix_natural_ms"])
+    out["speedup_vs_best_per_matrix"] = r4(best / ms_b)
     log("batch spts:", out)
     return out
+
+
+def oracle_sf(lo, bval):
+    import oracle
+    n, lrp, lc, lv = lo
+    return oracle.spts_syncfree(n, lrp, lc, lv, np.full(n, bval))
 
 
 # ------------------------------------------------------------------ serve
@@ -1016,7 +1061,7 @@ def main():
         pool = gen.batch_csr_concat(args.batch_count, seed=args.seed + 4)
         log("pool generated: %d matrices, %d rows, %d nnz in %.1fs" % (len(pool[0]), pool[1][-1], pool[2][-1],
                                                                       time.time() - t0))
-        lows = make_lower_pool(pool, min(512, args.batch_count)) if "batch_tiles" not in sections else None
+        lows = make_lower_pool(pool, args.batch_count) if "batch_tiles" not in sections else None
     if "batch_tiles" in sections:
         run("batch_tiles", lambda: sec_batch_tiles(args, sp, ctx, pool))
     if "batch" in sections:
@@ -1027,6 +1072,7 @@ def main():
         run("serve", lambda: sec_serve(args, sp, ctx, pool, lows, big))
 
     gbs, ms = head["gbs"], head["ms"]
+    traffic = measured_traffic(head["best"], head["N"])
     out = {
         "metric": "CSR SpMV GB/s (% HBM peak); served solves/s + p50/p99 latency vs plain",
         "value": round(gbs, 2),
@@ -1046,13 +1092,10 @@ def main():
                    "schedule": head["best"], "variants": head["variants"]},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": (MEASURED_TRAFFIC[head["best"]]["per_product"]
-                                 if head["best"] in MEASURED_TRAFFIC and head["N"] == 4194304 else None),
-                     "traffic_per_launch": (MEASURED_TRAFFIC[head["best"]]["per_launch"]
-                                            if head["best"] in MEASURED_TRAFFIC and head["N"] == 4194304 else None),
+                     "traffic": (traffic or {}).get("per_product"),
+                     "traffic_per_launch": (traffic or {}).get("per_launch"),
                      "kernel": head["variants"][head["best"]].get("kernel", "k_csr_block"),
-                     "limiter": "L2 request rate (TCC busy ~94%): streaming loads plus x lines missing the L1, "
-                                "not HBM bytes (fetched traffic ~1.13x the algorithmic bytes)",
+                     "limiter": (traffic or {}).get("limiter"),
                      "launches_per_step": head["variants"][head["best"]]["launches_per_step"]},
         "cpu_baseline": head["cpu"],
         "check": head["check"],
