@@ -827,6 +827,11 @@ typedef struct {
                              the mode's precision, one batched tile launch per width
                              (faster; rows summed in LDS-atomic order, so results agree
                              within the CSR bound) */
+    int spts_sweep;       /* modes 1 and 2, the batch's solves: 0 = level by level, one
+                             launch per level of the deepest matrix for all solves at
+                             once (mode 1: the reference level solve's bits); 1 = one
+                             sync-free launch for all of them (mode 1: the reference
+                             sync-free solve's bits) */
 } sable_server_config_t;
 
 SABLE_API int sable_server_create(sable_ctx_t* ctx, const sable_server_config_t* cfg, sable_server_t** out);

@@ -37,7 +37,7 @@ class SableError(RuntimeError):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", c_i64), ("mode", c_int), ("max_batch", c_int), ("slots", c_int),
                 ("arena_bytes", c_i64), ("refine_iters", c_int), ("spts_lanes", c_int), ("group_rhs", c_int),
-                ("batch_tiles", c_int)]
+                ("batch_tiles", c_int), ("spts_sweep", c_int)]
 
 
 class LoadSpec(ctypes.Structure):

@@ -600,9 +600,7 @@ static unsigned gsz(int64_t n) {
 // then every lane of the group adds them in order), x[i] = (b[i] - t) /
 // diag: the level-set solve's bits at every G.
 template <typename T, int G>
-__global__ __launch_bounds__(BB) void k_batch_level(const sable_batch_spts_item_t* __restrict__ items,
-                                                    const RowChunk* __restrict__ chunks) {
-    const RowChunk c = chunks[blockIdx.x];
+__device__ __forceinline__ void level_chunk(const sable_batch_spts_item_t* __restrict__ items, const RowChunk& c) {
     const int g = threadIdx.x / G;
     const int l = threadIdx.x & (G - 1);
     const bool valid = g < c.count;
@@ -632,6 +630,12 @@ __global__ __launch_bounds__(BB) void k_batch_level(const sable_batch_spts_item_
         const T d = ((const T*)it.val)[e];
         ((T*)it.x)[i] = div_rn(sub_rn(((const T*)it.b)[i], s), d);
     }
+}
+
+template <typename T, int G>
+__global__ __launch_bounds__(BB) void k_batch_level(const sable_batch_spts_item_t* __restrict__ items,
+                                                    const RowChunk* __restrict__ chunks) {
+    level_chunk<T, G>(items, chunks[blockIdx.x]);
 }
 
 int batch_level_launch(hipStream_t st, sable_dtype_t dtype, int lanes, const sable_batch_spts_item_t* items,
@@ -664,9 +668,8 @@ int batch_level_launch(hipStream_t st, sable_dtype_t dtype, int lanes, const sab
 // Chunked residual r32 = b64 - L64 x64 (first pass: r32 = b64) and update
 // x64 += d32 (first pass: x64 = d32), one workgroup per chunk of rows.
 template <int G>
-__global__ __launch_bounds__(BB) void k_chunk_residual(const sable_batch_refine_item_t* __restrict__ items,
-                                                       const RowChunk* __restrict__ chunks, int first) {
-    const RowChunk c = chunks[blockIdx.x];
+__device__ __forceinline__ void residual_chunk(const sable_batch_refine_item_t* __restrict__ items, const RowChunk& c,
+                                               int first) {
     const int g = threadIdx.x / G;
     const int l = threadIdx.x & (G - 1);
     const bool valid = g < c.count;
@@ -679,13 +682,23 @@ __global__ __launch_bounds__(BB) void k_chunk_residual(const sable_batch_refine_
     if (valid && l == 0) it.r32[i] = (float)(it.b64[i] - s);
 }
 
-__global__ __launch_bounds__(BB) void k_chunk_update(const sable_batch_refine_item_t* __restrict__ items,
-                                                     const RowChunk* __restrict__ chunks, int first) {
-    const RowChunk c = chunks[blockIdx.x];
+__device__ __forceinline__ void update_chunk(const sable_batch_refine_item_t* __restrict__ items, const RowChunk& c,
+                                             int first) {
     if ((int)threadIdx.x >= c.count) return;
     const sable_batch_refine_item_t& it = items[c.item];
     const int32_t i = c.first + threadIdx.x;
     it.x64[i] = first ? (double)it.d32[i] : it.x64[i] + (double)it.d32[i];
+}
+
+template <int G>
+__global__ __launch_bounds__(BB) void k_chunk_residual(const sable_batch_refine_item_t* __restrict__ items,
+                                                       const RowChunk* __restrict__ chunks, int first) {
+    residual_chunk<G>(items, chunks[blockIdx.x], first);
+}
+
+__global__ __launch_bounds__(BB) void k_chunk_update(const sable_batch_refine_item_t* __restrict__ items,
+                                                     const RowChunk* __restrict__ chunks, int first) {
+    update_chunk(items, chunks[blockIdx.x], first);
 }
 
 int batch_spts_launch(hipStream_t st, int num_cu, sable_dtype_t dtype, int lanes, const sable_batch_spts_item_t* items,

@@ -18,10 +18,11 @@
 //   mode 2 (mixed):   batched, SpMV with FP32 values/vectors and FP32
 //                     accumulation; solves in FP32 with FP64 residual and
 //                     update for `refine_iters` refinement steps.
-// Batched solves (modes 1 and 2) run level by level: every lower matrix
-// carries its level schedule from registration, and launch l of a batch
-// solves level l of all of the batch's solves at once (a batch needs as
-// many launches as its deepest matrix has levels).
+// Batched solves (modes 1 and 2) run level by level (spts_sweep 0): every
+// lower matrix carries its level schedule from registration, and launch l
+// of a batch solves level l of all of the batch's solves at once (a batch
+// needs as many launches as its deepest matrix has levels); or (spts_sweep
+// 1) in one sync-free launch over all of the batch's rows.
 // A completion thread waits on each slot's event in order, copies results
 // out and records latencies.  Several slots in flight let a new batch be
 // staged and launched while earlier ones run on the GPU.
@@ -550,7 +551,7 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
             fch.push_back(FillChunk{(int32_t)f, a, std::min(FILL_CHUNK, fills[f].n - a), 0});
     LevelPlan plan;
     std::vector<RowChunk> lch, rch;
-    if (cfg.mode != 0) {
+    if (cfg.mode != 0 && cfg.spts_sweep != 1) {
         std::vector<const SRequest*> solves;
         for (auto* r : batch)
             if (r->op == 1) solves.push_back(r);
@@ -621,13 +622,24 @@ int sable_server_s::launch_batch(Slot& s, std::vector<SRequest*>& batch) {
                 rc = batch_tiles_launch(st, ctx->num_cu, mixed ? SABLE_F32 : SABLE_F64, widths[wi],
                                         (const sable_batch_tiles_item_t*)(s.d_items + tofs[wi]), (int)bt[wi].size(),
                                         unit_total[wi], bt_rows[wi], s.d_counter, s.d_err, 0);
-        if (!rc && !tv.empty())
-            rc = batch_level_launch(st, SABLE_F64, lanes, (const sable_batch_spts_item_t*)(s.d_items + spts_off),
-                                    d_lch, plan);
-        if (!rc && !rv.empty())
-            rc = batch_refine_level_launch(st, lanes, (const sable_batch_refine_item_t*)(s.d_items + spts_off),
-                                           (sable_batch_spts_item_t*)(s.d_items + view_off), (int)rv.size(),
-                                           cfg.refine_iters, d_lch, plan, d_rch, (int)rch.size());
+        const auto* titems = (const sable_batch_spts_item_t*)(s.d_items + spts_off);
+        const auto* ritems = (const sable_batch_refine_item_t*)(s.d_items + spts_off);
+        auto* view = (sable_batch_spts_item_t*)(s.d_items + view_off);
+        if (!rc && !tv.empty()) {
+            if (cfg.spts_sweep == 1)
+                rc = batch_spts_launch(st, ctx->num_cu, SABLE_F64, lanes, titems, (int)tv.size(), row_total,
+                                       s.d_ticket, s.d_err, spin);
+            else
+                rc = batch_level_launch(st, SABLE_F64, lanes, titems, d_lch, plan);
+        }
+        if (!rc && !rv.empty()) {
+            if (cfg.spts_sweep == 1)
+                rc = batch_refine_launch(st, ctx->num_cu, lanes, ritems, view, (int)rv.size(), row_total,
+                                         cfg.refine_iters, s.d_ticket, s.d_err, spin);
+            else
+                rc = batch_refine_level_launch(st, lanes, ritems, view, (int)rv.size(), cfg.refine_iters, d_lch,
+                                               plan, d_rch, (int)rch.size());
+        }
     }
     // results go back to the host only for requests that brought a host
     // output buffer (requests generated on the device keep theirs in HBM)
@@ -726,6 +738,7 @@ extern "C" {
 SABLE_API int sable_server_create(sable_ctx_t* ctx, const sable_server_config_t* cfg, sable_server_t** out) {
     SABLE_CHECK(ctx && cfg && out, SABLE_EINVAL, "NULL argument");
     SABLE_CHECK(cfg->mode >= 0 && cfg->mode <= 2, SABLE_EINVAL, "mode must be 0, 1 or 2");
+    SABLE_CHECK(cfg->spts_sweep == 0 || cfg->spts_sweep == 1, SABLE_EINVAL, "spts_sweep must be 0 or 1");
     *out = nullptr;
     auto* s = new sable_server_s();
     s->ctx = ctx;

@@ -162,11 +162,14 @@ MODES = {"plain": 0, "batched": 1, "mixed": 2}
 class Server:
     """The native request server (sable_server_*)."""
 
+    SWEEPS = {"levels": 0, "syncfree": 1}
+
     def __init__(self, mode="mixed", hbm_budget=0, max_batch=256, slots=3, arena_bytes=256 << 20, refine_iters=2,
-                 spts_lanes=4, group_rhs=8, batch_tiles=False, ctx=None):
+                 spts_lanes=4, group_rhs=8, batch_tiles=False, spts_sweep="levels", ctx=None):
         self.ctx = ctx if ctx is not None else Context.default()
         cfg = _lib.ServerConfig(int(hbm_budget), MODES[mode], int(max_batch), int(slots), int(arena_bytes),
-                                int(refine_iters), int(spts_lanes), int(group_rhs), int(bool(batch_tiles)))
+                                int(refine_iters), int(spts_lanes), int(group_rhs), int(bool(batch_tiles)),
+                                self.SWEEPS[spts_sweep])
         h = c_vp()
         call("sable_server_create", self.ctx.handle, ctypes.byref(cfg), ctypes.byref(h))
         self.handle = h

@@ -1,8 +1,8 @@
 """The request server: results alone and in a concurrent mix, accuracy
 against the oracle, HBM-budget admission, and a short open-loop run.
 
-Alone vs in the mix: every mode computes each request with an arithmetic
-that does not depend on its batch (the batched kernels keep each item's
+Alone vs in the mix: every mode and solve sweep computes each request with
+an arithmetic that does not depend on its batch (the batched kernels keep each item's
 per-row order; the solves' per-row order is fixed by the lane layout), so
 the two results must be identical; the stated tolerance against the oracle
 is the CSR SpMV bound for SpMV (FP32 unit roundoff in mixed mode) and
@@ -41,10 +41,24 @@ def spmv_bound(rp, col, val, x, dtype):
     return (2 * k + 4) * u / (1 - (k + 2) * u) * s + 1e-300
 
 
+def level_solution(n, rp, c, v, b):
+    nl, lp, perm, order, _ = oracle.level_sets(n, rp, c)
+    nrp, nc_, nv = oracle.level_reorder(n, rp, c, v, perm, order)
+    yl = oracle.spts_level(n, nrp, nc_, nv, b[perm])
+    x = np.empty_like(yl)
+    x[perm] = yl
+    return x
+
+
 @pytest.mark.parametrize("mode,kw", [("plain", {}), ("batched", {}), ("batched", {"group_rhs": 1}), ("mixed", {}),
-                                     ("mixed", {"refine_iters": 1})],
-                         ids=["plain", "batched", "batched_ungrouped", "mixed", "mixed_refine1"])
+                                     ("mixed", {"refine_iters": 1}), ("batched", {"spts_sweep": "syncfree"}),
+                                     ("mixed", {"spts_sweep": "syncfree"})],
+                         ids=["plain", "batched", "batched_ungrouped", "mixed", "mixed_refine1", "batched_sf",
+                              "mixed_sf"])
 def test_alone_equals_mix_and_accuracy(ctx, pool, mode, kw):
+    """Also: the FP64 solves are a reference solve's bits, the sync-free
+    solve's (plain; batched one-launch sweep) or the level solve's (batched
+    level sweep)."""
     mats, lows = pool
     S, gids, lids = make_server(mode, pool, **kw)
     rng = np.random.default_rng(5)
@@ -82,6 +96,10 @@ def test_alone_equals_mix_and_accuracy(ctx, pool, mode, kw):
             assert np.all(np.abs(out - yref) <= spmv_bound(rp, c, v, vec, dt)), (mode, p)
         else:
             assert oracle.rel_residual(n, rp, c, v, out, vec) <= 1e-12, (mode, p)
+            if mode == "plain" or (mode == "batched" and kw.get("spts_sweep") == "syncfree"):
+                assert np.array_equal(out, oracle.spts_syncfree(n, rp, c, v, vec)), (mode, p)
+            elif mode == "batched":
+                assert np.array_equal(out, level_solution(n, rp, c, v, vec)), (mode, p)
     for i in ids:
         S.wait(i)
     S.close()
@@ -158,9 +176,9 @@ def test_malformed_lower_rejected(ctx):
     S.close()
 
 
-@pytest.mark.parametrize("mode", ["plain", "mixed"])
-def test_open_loop_load(ctx, pool, mode):
-    S, gids, lids = make_server(mode, pool)
+@pytest.mark.parametrize("mode,sweep", [("plain", "levels"), ("mixed", "levels"), ("mixed", "syncfree")])
+def test_open_loop_load(ctx, pool, mode, sweep):
+    S, gids, lids = make_server(mode, pool, spts_sweep=sweep)
     st = S.run_load(rate=2000, duration_s=0.5, spts_fraction=0.3, seed=7)
     assert st["failed"] == 0 and st["rejected"] == 0
     assert st["completed"] == st["submitted"] > 0
