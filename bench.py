@@ -782,6 +782,7 @@ def sec_batch(args, sp, ctx, T, pool):
             ys.append(sp.DeviceArray(int(ns[k]) * 8, dtype, ctx))
         for xx in xs:
             xx.fill(0.5)
+        rowptrs = [(rp[rs[k]:rs[k + 1] + 1] - rp[rs[k]]) for k in range(count)]
         log("batch %s: %d matrices uploaded in %.1fs" % (np.dtype(dtype).name, count, time.time() - t0))
         plans = {}
         for nrhs in (1, 2, 4, 8):
@@ -809,10 +810,39 @@ def sec_batch(args, sp, ctx, T, pool):
                 plans.clear()
                 plans[tr] = (sp.batch_tile_plans(mats, nrhs=nrhs), time.time() - t0)
             res["tiles"] = batch_tiles(sp, ctx, T, mats, xs, ys, nrhs, plans[tr], bytes_, ms_b)
+            res["autotuned"] = batch_autotuned(sp, ctx, T, mats, xs, ys, nrhs, bytes_, res, rowptrs)
             out["spmv_%s_nrhs%d" % (np.dtype(dtype).name, nrhs)] = res
             log("batch spmv %s nrhs=%d" % (np.dtype(dtype).name, nrhs), res)
         del plans, mats, xs, ys
     return out
+
+
+def batch_autotuned(sp, ctx, T, mats, xs, ys, nrhs, bytes_, res, rowptrs):
+    """The same batch under the autotuned batch plan (per row-length class,
+    the fastest of row-block and tile schedules measured on 16 of the
+    class's matrices), checked against the row-block batch's y on the first
+    32 matrices (within the CSR bound: tile classes sum rows in LDS-atomic
+    order)."""
+    ref = [y.numpy()[:A.N * nrhs].astype(np.float64) for A, y in zip(mats[:32], ys[:32])]
+    t0 = time.time()
+    bp = sp.BatchPlan(mats, nrhs=nrhs, reps=3, sample=16, row_ptrs=rowptrs)
+    tune_s = time.time() - t0
+    run = bp.bind(xs, ys)
+    ms = T(run.run, 10, 2)
+    err = 0.0
+    for r, A, y in zip(ref, mats[:32], ys[:32]):
+        err = max(err, float(np.max(np.abs(y.numpy()[:A.N * nrhs] - r)) / max(float(np.max(np.abs(r))), 1e-300)))
+    info = bp.info()
+    fmts = {}
+    for v in info.values():
+        f = v["format"] if v["format"] == "rowblock" else "tiles%d" % v["tile_rows"]
+        fmts[f] = fmts.get(f, 0) + v["matrices"]
+    best_fixed = min(res["batched_ms"], res["tiles"]["ms"])
+    ib = bp.info_batch()
+    return {"ms": r4(ms), "gbs": r4(bytes_ / (ms * 1e-3) / 1e9), "classes": len(info),
+            "assignment": ib["assignment"], "assignments_ms": {t["assignment"]: r4(t["ms"]) for t in ib["table"]},
+            "per_class_matrices_by_schedule": fmts, "tune_s": r4(tune_s),
+            "speedup_vs_fixed_schedules": r4(best_fixed / ms), "max_rel_diff_vs_rowblock_first32": err}
 
 
 def batch_tiles(sp, ctx, T, mats, xs, ys, nrhs, plans_built, bytes_, ms_rowblock):

@@ -158,3 +158,177 @@ class _Run:
         if not self.vectors:
             return self.target.run(x, y, beta=beta)
         return self.target(sparse_vec_t(x, self.N), sparse_vec_t(y, self.N), beta)
+
+
+class BatchPlan:
+    """Autotuned schedule for a batch of many CSR matrices with nrhs
+    right-hand sides each, run as at most two persistent launches.
+
+    The matrices are grouped by row-length class (row_stats_key, with the
+    dtype and nrhs); for each class the candidates are measured on (up to
+    `sample` of) the class's own matrices in batched launches, and the
+    fastest is kept for every matrix of the class (and cached by key):
+
+    * row-block items of one sable_batch_spmv launch, with the block weight
+      and row weight of their plans (rows per wavefront: a block packs rows
+      until nnz + row_weight * rows reaches the block weight, so a class of
+      short rows gets many rows per wave and long rows whole workgroups);
+    * column-sorted tile items of one sable_batch_spmv_tiles launch, with
+      row tiles of 1024 to 8192 rows over a single column panel (each row
+      tile gathers x in ascending column order).
+
+    The per-class assignment is then measured on the whole batch beside the
+    uniform schedules (every matrix on row blocks; every matrix on tiles of
+    each height), since two launches of parts of a batch do not overlap and
+    a class timed alone sees a smaller launch than the batch; the fastest
+    whole-batch assignment is kept (`assignment` in info_batch()).
+
+    Row-block items sum each row in a fixed order; tile items in LDS-atomic
+    order, so with allow_tiles the results meet the CSR bound rather than a
+    fixed bit pattern."""
+
+    ROWBLOCK = ((256, 1), (512, 1), (1024, 1), (2048, 1), (1024, 2), (2048, 2))
+    TILE_ROWS = (1024, 2048, 4096, 8192)
+
+    def __init__(self, mats, nrhs=1, reps=5, allow_tiles=True, sample=32, cache=None, row_ptrs=None):
+        self.mats, self.nrhs = list(mats), int(nrhs)
+        self.ctx = self.mats[0].row.ctx
+        self.reps, self.allow_tiles, self.sample = reps, allow_tiles, sample
+        self.cache = cache if cache is not None else {}
+        rps = row_ptrs if row_ptrs is not None else [A.row.numpy() for A in self.mats]
+        self.classes = {}
+        for k, (A, rp) in enumerate(zip(self.mats, rps)):
+            if A.N == 0:
+                continue
+            key = "%s_%s_r%d" % (row_stats_key(rp), np.dtype(A.dtype).name, self.nrhs)
+            self.classes.setdefault(key, []).append(k)
+        for key, idx in self.classes.items():
+            if key not in self.cache:
+                self.cache[key] = self._tune_class([self.mats[k] for k in idx[:sample]])
+        per_class = {}
+        for key, idx in self.classes.items():
+            for k in idx:
+                per_class[k] = self._build(self.mats[k], self.cache[key])
+        # whole-batch assignments: per class, and the uniform ones
+        uniform = [("all_rowblock", {"format": "rowblock", "block_weight": 0, "row_weight": 0})]
+        if allow_tiles:
+            uniform += [("all_tiles%d" % tr, {"format": "tiles", "tile_rows": tr})
+                        for tr in self.TILE_ROWS if tr * self.nrhs <= 20480]
+        xs = [DeviceArray(max(A.N, 1) * self.nrhs, A.dtype, self.ctx) for A in self.mats]
+        ys = [DeviceArray(max(A.N, 1) * self.nrhs, A.dtype, self.ctx) for A in self.mats]
+        for x in xs:
+            x.fill(0.5)
+        self.batch_table = []
+        best = ("per_class", self._time_batch(self.mats, per_class, xs, ys), per_class)
+        self.batch_table.append({"assignment": best[0], "ms": best[1]})
+        for name, cand in uniform:
+            plans = {k: self._build(self.mats[k], cand) for idx in self.classes.values() for k in idx}
+            ms = self._time_batch(self.mats, plans, xs, ys)
+            self.batch_table.append({"assignment": name, "ms": ms})
+            if ms < best[1]:
+                best = (name, ms, plans)
+            del plans
+        self.assignment, self.batch_ms, self.plans = best
+
+    def _time_batch(self, mats, plans, xs, ys):
+        runner = _BatchRunner(mats, plans, xs, ys, self.nrhs, 0.0)
+        runner.run()
+        self.ctx.sync()
+        e0, e1 = self.ctx.event(), self.ctx.event()
+        self.ctx.record(e0)
+        for _ in range(self.reps):
+            runner.run()
+        self.ctx.record(e1)
+        self.ctx.sync()
+        return self.ctx.elapsed_ms(e0, e1) / self.reps
+
+    def info_batch(self):
+        """The whole-batch assignment kept, its time, and every time measured."""
+        return {"assignment": self.assignment, "ms": self.batch_ms, "table": self.batch_table}
+
+    def _candidates(self):
+        out = [{"format": "rowblock", "block_weight": bw, "row_weight": rw} for bw, rw in self.ROWBLOCK]
+        if self.allow_tiles:
+            out += [{"format": "tiles", "tile_rows": tr} for tr in self.TILE_ROWS if tr * self.nrhs <= 20480]
+        return out
+
+    @staticmethod
+    def _build(A, cand):
+        if cand["format"] == "tiles":
+            tr = min(int(cand["tile_rows"]), (A.N + 63) // 64 * 64)
+            pc = 64
+            while pc < A.N:
+                pc *= 2
+            return CsrTiles(A, tile_rows=tr, panel_cols=pc)
+        if not cand["block_weight"]:
+            return A.plan()
+        return CsrPlan(A, {"block_weight": cand["block_weight"], "row_weight": cand["row_weight"]})
+
+    def _tune_class(self, mats):
+        xs = [DeviceArray(A.N * self.nrhs, A.dtype, self.ctx) for A in mats]
+        ys = [DeviceArray(A.N * self.nrhs, A.dtype, self.ctx) for A in mats]
+        for x in xs:
+            x.fill(0.5)
+        best, best_ms, table = None, float("inf"), []
+        for cand in self._candidates():
+            plans = {k: self._build(A, cand) for k, A in enumerate(mats)}
+            ms = self._time_batch(mats, plans, xs, ys)
+            table.append(dict(cand, ms=ms))
+            if ms < best_ms:
+                best, best_ms = cand, ms
+            del plans
+        return dict(best, ms=best_ms, matrices_timed=len(mats), table=table)
+
+    def info(self):
+        """Per class: matrices, the chosen schedule and its time on the
+        class's timed sample."""
+        return {key: {"matrices": len(idx), **{k: v for k, v in self.cache[key].items() if k != "table"}}
+                for key, idx in self.classes.items()}
+
+    def bind(self, xs, ys, beta=0.0):
+        """A runner for ys[k] = A_k.xs[k] (+ beta*ys[k]): its item lists
+        are on the device, so run() only launches."""
+        return _BatchRunner(self.mats, self.plans, xs, ys, self.nrhs, beta)
+
+    def run(self, xs, ys, beta=0.0):
+        self.bind(xs, ys, beta).run()
+
+
+class _BatchRunner:
+    """The two launches of a batch: row-block items (sable_batch_spmv) and
+    tile items (sable_batch_spmv_tiles), item lists uploaded once."""
+
+    def __init__(self, mats, plans, xs, ys, nrhs, beta):
+        from ._lib import call, c_vp, c_i64
+        from .core import dtype_code
+        from .serve import BatchSpmvItem, _upload_items, _check_vectors, batch_tiles_items
+        import ctypes
+        self.ctx, self.nrhs, self._call = mats[0].row.ctx, int(nrhs), call
+        rb, tl, txs, tys, total = [], [], [], [], 0
+        for k, A in enumerate(mats):
+            p = plans.get(k)
+            if p is None:
+                continue
+            if isinstance(p, CsrTiles):
+                tl.append(p)
+                txs.append(xs[k])
+                tys.append(ys[k])
+                continue
+            _check_vectors(A, xs[k], ys[k], nrhs, "BatchPlan")
+            blk, nb = c_vp(), c_i64()
+            call("sable_csr_plan_blocks", p.handle, ctypes.byref(blk), ctypes.byref(nb))
+            rb.append(BatchSpmvItem(A.row.ptr, A.col.ptr, A.val.ptr, blk.value, xs[k].ptr, ys[k].ptr, total,
+                                    float(beta)))
+            total += nb.value
+        self.code = dtype_code(mats[0].dtype)
+        self.rb = (_upload_items(rb, self.ctx), len(rb), total) if rb else None
+        self.tl = batch_tiles_items(tl, txs, tys, beta, nrhs) if tl else None
+        self.keep = (plans, xs, ys)
+
+    def run(self):
+        if self.rb is not None:
+            d, n, total = self.rb
+            self._call("sable_batch_spmv", self.ctx.handle, self.code, self.nrhs, d.ptr, n, total, 0)
+        if self.tl is not None:
+            d, n, total, maxr, dt, _ = self.tl
+            self._call("sable_batch_spmv_tiles", self.ctx.handle, self.code, self.nrhs, d.ptr, n, total, maxr, 0)

@@ -247,6 +247,71 @@ def test_batch_spmv_tiles_many_panels(ctx, dtype):
     assert e.value.name == "SABLE_EINVAL"
 
 
+@pytest.mark.parametrize("allow_tiles", [True, False], ids=["tiles", "rowblock_only"])
+@pytest.mark.parametrize("nrhs", [1, 4])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_autotuned_batch(ctx, dtype, nrhs, allow_tiles):
+    """The autotuned batch (row-length classes, each with the fastest of
+    row-block plans of several block/row weights and tile plans of several
+    tile heights, measured on the class's own matrices in batched launches;
+    then the fastest whole-batch assignment of that and the uniform ones)
+    over matrices of 64 to 20 000 rows: every product within the CSR bound
+    of the oracle, twice in a row (the second with beta = 1 on the first's
+    y), and without tiles the row-block batch's bits (each row in a fixed
+    order) on a repeat."""
+    rng = np.random.default_rng(9)
+    mm = gen.batch_csr(30, rows_lo=64, rows_hi=20000, seed=12) + [(0, np.zeros(1, np.int32),
+                                                                 np.zeros(0, np.int32), np.zeros(0))]
+    As, xs, ys, X = [], [], [], []
+    for n, rp, c, v in mm:
+        As.append(sp.csr_from_host(n, rp, c, v.astype(dtype)))
+        x = rng.uniform(-1, 1, (n, nrhs)).astype(dtype)
+        X.append(x)
+        xs.append(DeviceArray.from_host(x.ravel()) if n else DeviceArray.zeros(1, dtype))
+        ys.append(DeviceArray.zeros(max(n, 1) * nrhs, dtype))
+    bp = sp.BatchPlan(As, nrhs=nrhs, reps=2, allow_tiles=allow_tiles)
+    info = bp.info()
+    print(dtype.__name__, nrhs, allow_tiles, {k: (v["format"], v["matrices"]) for k, v in info.items()}, flush=True)
+    assert sum(v["matrices"] for v in info.values()) == len(mm) - 1
+    assert allow_tiles or all(v["format"] == "rowblock" for v in info.values())
+    ib = bp.info_batch()
+    names = {t["assignment"] for t in ib["table"]}
+    assert ib["assignment"] in names and "per_class" in names and "all_rowblock" in names
+    assert ib["ms"] == min(t["ms"] for t in ib["table"])
+    assert allow_tiles or names == {"per_class", "all_rowblock"}
+    bp.run(xs, ys)
+    ctx.sync()
+    first = [y.numpy().copy() for y in ys]
+    for k, (n, rp, c, v) in enumerate(mm):
+        if n == 0:
+            continue
+        Y = first[k][:n * nrhs].reshape(n, nrhs)
+        for r in range(nrhs):
+            yref = oracle.spmv_csr(n, rp, c, v.astype(dtype), X[k][:, r].copy(), np.zeros(n, dtype))
+            d = np.abs(Y[:, r].astype(np.float64) - yref)
+            assert np.all(d <= bound(rp, c, v.astype(dtype), X[k][:, r], dtype)), (k, r)
+    run = bp.bind(xs, ys, beta=1.0)
+    run.run()
+    ctx.sync()
+    u = np.finfo(dtype).eps / 2
+    for k, (n, rp, c, v) in enumerate(mm):
+        if n == 0:
+            continue
+        Y = ys[k].numpy()[:n * nrhs].reshape(n, nrhs)
+        Y0 = first[k][:n * nrhs].reshape(n, nrhs).astype(np.float64)
+        for r in range(nrhs):
+            yref = oracle.spmv_csr(n, rp, c, v.astype(dtype), X[k][:, r].copy(), np.zeros(n, dtype))
+            b = bound(rp, c, v.astype(dtype), X[k][:, r], dtype)
+            assert np.all(np.abs(Y[:, r] - (Y0[:, r] + yref)) <= 2 * b + 2 * u * (np.abs(Y0[:, r]) + np.abs(yref)))
+    if not allow_tiles:
+        for y in ys:
+            y.fill(0.0)
+        bp.run(xs, ys)
+        ctx.sync()
+        for k, (n, rp, c, v) in enumerate(mm):
+            assert np.array_equal(ys[k].numpy(), first[k]), k
+
+
 def lower_set(mats, dtype=np.float64):
     out = []
     for n, rp, c, v in mats:
