@@ -399,7 +399,8 @@ def sec_zipf16(args, sp, ctx, T):
     sab[lens == 0] = 0
     u = np.finfo(np.float64).eps / 2
     bnd = 2 * (lens + 1) * u / (1 - (lens + 1) * u) * sab + 1e-300
-    for tr, pc in ((16384, 16384), (16384, 8192)):
+    shapes = [tuple(int(t) for t in v.split("x")) for v in args.zipf16_tiles.split(",") if v.strip()]
+    for tr, pc in shapes:
         tl = sp.CsrTiles(A, tr, pc)
         ms = T(lambda: tl.run(xv.vec, yv.vec, beta=0.0), args.steps, args.warmup)
         y = yv.array()[:R]
@@ -1026,6 +1027,8 @@ def main():
                     help="comma list of column-sorted tile shapes ROWSxCOLS to time")
     ap.add_argument("--tiles-mode", type=str, default="1",
                     help="tile schedules to time: 0 one launch per panel, 1 one launch over all panels")
+    ap.add_argument("--zipf16-tiles", type=str, default="16384x16384,16384x8192",
+                    help="tile shapes ROWSxCOLS timed on the zipf(s=1.73) matrix")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy (a plain copy kernel of known size, used to "
                          "scale hardware byte counters)")
