@@ -185,7 +185,9 @@ def headline(args, sp, ctx, T):
                              "launches_per_step": 1 if mode else info["npanels"],
                              "npanels": info["npanels"], "row_tiles": info["row_tiles"], "tile_rows": tr,
                              "panel_cols": pc, "kernel": TILE_MODES[mode][1],
-                             "plan_build_s": round(build_s, 3)}
+                             "plan_build_s": round(build_s, 3),
+                             # the plan's own copy of the matrix, resident beside the caller's CSR
+                             "plan_extra_hbm_bytes": info["bytes"]}
             results[key] = yv.array()
             log(key, info, "built in %.2fs, %.4f ms  %.1f GB/s" % (build_s, ms, variants[key]["gbs"]))
         L.call("sable_csr_tiles_select", 1)
@@ -949,6 +951,36 @@ def oracle_sf(lo, bval):
 
 
 # ------------------------------------------------------------------ serve
+def serve_accuracy(S, pool, lows, nsmall, count=16):
+    """Served results against the FP64 oracle, on `count` SpMV and `count`
+    solve requests submitted together: the largest SpMV error relative to
+    the row's sum_j |a_ij x_j| (the scale of the CSR bound), and the largest
+    solve relative residual ||b - L x|| / (||L|| ||x|| + ||b||)."""
+    import oracle
+    ns, rs, rp, col, val = pool
+    rng = np.random.default_rng(77)
+    ids = []
+    for k in range(count):
+        a, b = rs[k], rs[k + 1]
+        n = int(ns[k])
+        x = rng.uniform(-1, 1, n)
+        ids.append(("spmv", k, x, S.submit(k, "spmv", x)))
+        bb = rng.uniform(-1, 1, lows[k][0])
+        ids.append(("spts", k, bb, S.submit(nsmall + k, "spts", bb)))
+    err, res = 0.0, 0.0
+    for op, k, v, rid in ids:
+        out, _ = S.wait(rid)
+        if op == "spmv":
+            a, b = rs[k], rs[k + 1]
+            lrp = (rp[a:b + 1] - rp[a]).astype(np.int32)
+            y64, ab = oracle.spmv_csr_fp64(int(ns[k]), lrp, col[rp[a]:rp[b]], val[rp[a]:rp[b]], v)
+            err = max(err, float(np.max(np.abs(out - y64) / np.maximum(ab, 1e-300))))
+        else:
+            n, lrp, lc, lv = lows[k]
+            res = max(res, oracle.rel_residual(n, lrp, lc, lv, out, v))
+    return {"requests": 2 * count, "spmv_max_err_over_abs_row_sum": err, "spts_max_rel_residual": res}
+
+
 def sec_serve(args, sp, ctx, pool, lows, big):
     out = {}
     rates = [float(r) for r in args.serve_rates.split(",")]
@@ -979,6 +1011,8 @@ def sec_serve(args, sp, ctx, pool, lows, big):
                        "large": args.serve_large if big is not None else 0,
                        "hbm_used_gb": r4(S.info()["hbm_used"] / 1e9)}
         log("serve %s: pool registered in %.1fs" % (mode, time.time() - t0), out["pool"])
+        out[mode + "_accuracy"] = serve_accuracy(S, pool, lows, nsmall)
+        log("serve %s accuracy:" % mode, out[mode + "_accuracy"])
         res = []
         saturated = 0
         for rate in rates:
