@@ -723,6 +723,8 @@ def sec_spts(args, sp, ctx, T):
         want_sf = oracle.spts_syncfree(N, rp, col, val, b)
         timed("syncfree_blocks", lambda: an.solve_syncfree(bv, xv))
         res["syncfree_blocks"]["bit_exact_vs_oracle_syncfree"] = bool(np.array_equal(xv.array(), want_sf))
+        timed("syncfree_indegree", lambda: an.solve_indegree(bv, xv))
+        res["syncfree_indegree"]["bit_exact_vs_oracle_syncfree"] = bool(np.array_equal(xv.array(), want_sf))
         timed("combined", lambda: an.solve_combined(bv, xv))
         res["combined"]["bit_exact_vs_oracle_level"] = bool(np.array_equal(xv.array(), want))
         timed("fp32_refine1", lambda: an.solve_refined(bv, xv, iters=1, inner="level"))

@@ -688,6 +688,15 @@ SABLE_API int sable_spts_solve_level(sable_ctx_t* ctx, sable_spts_t* a, const vo
  * (par/src/spts_opt_32.wat:8-260). */
 SABLE_API int sable_spts_solve_syncfree(sable_ctx_t* ctx, sable_spts_t* a, const void* b, void* x);
 
+/* Sync-free solve with per-row in-degree counters, natural order (b and x
+ * distinct), one launch: row i waits until the count of its solved
+ * dependencies reaches its in-degree, then forms its sum in stored order
+ * and increments the count of every row that depends on it (the analysis
+ * keeps those lists, built on first use).  The same bits as
+ * sable_spts_solve_syncfree (the reference's spts_csr_sync_free,
+ * par/src/spts_opt_32.wat:8-260), a different way of waiting. */
+SABLE_API int sable_spts_solve_indegree(sable_ctx_t* ctx, sable_spts_t* a, const void* b, void* x);
+
 /* Level + sync-free combined solve, b and x in natural order: the
  * sync-free kernel over the analysis's level-ordered matrix, with the
  * level solve's arithmetic, so that the result equals sable_spts_solve_level

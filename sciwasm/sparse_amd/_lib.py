@@ -172,6 +172,7 @@ SIGNATURES = {
     "sable_spts_get_schedule": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "sable_spts_solve_level": [c_vp, c_vp, c_vp, c_vp],
     "sable_spts_solve_syncfree": [c_vp, c_vp, c_vp, c_vp],
+    "sable_spts_solve_indegree": [c_vp, c_vp, c_vp, c_vp],
     "sable_spts_solve_combined": [c_vp, c_vp, c_vp, c_vp],
     "sable_spts_status": [c_vp, c_vp],
     "sable_spts_solve_refined": [c_vp, c_vp, c_vp, c_vp, c_int, c_int],

@@ -80,6 +80,13 @@ struct sable_spts_s {
     void* d_bl_val = nullptr;         // slots (dtype)
     float* d_bl_diag32 = nullptr;     // FP32 copies for the refined solve
     float* d_bl_val32 = nullptr;
+    // in-degree sync-free solve (built on first use): the rows that depend
+    // on each row (strictly lower part by column), per-row in-degree, and
+    // per-row counters of published dependencies
+    int32_t* d_dep_ptr = nullptr;  // N + 1
+    int32_t* d_dep_row = nullptr;  // nnz - N
+    int32_t* d_indeg = nullptr;    // N
+    int32_t* d_cnt = nullptr;      // N
     // host structure and levels kept to rebuild the block schedule
     std::vector<int32_t> h_rp, h_col, h_level;
 };
@@ -305,6 +312,82 @@ __global__ __launch_bounds__(TB) void k_syncfree(int32_t N, const int32_t* __res
                     // give up: publish NaN so that waiting rows finish too
                     if (!published && l == 0) store_agent(&x[i], T(__builtin_nan("")));
                     published = true;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------- in-degree sync-free
+// The sync-free solve with per-row in-degree counters: row i waits until
+// cnt[i], the number of its off-diagonal dependencies already solved,
+// reaches indeg[i]; it then forms t = b - v0*x0 - v1*x1 - ... in stored
+// order (G lanes form G products at once, added in order), x = t / d, and
+// pushes: one increment of cnt[r] for every row r that depends on it.  The
+// push follows the completed store of x (the wave waits for it), and x and
+// the counters move through agent-scope atomics, so a row that sees its
+// count complete reads published values.  Rows are claimed in increasing
+// order; a wave serves every row of its claim that is ready in each pass,
+// so a row waiting on another row of the same wave does not stall it.
+__global__ void k_indeg_reset(int32_t N, int32_t* __restrict__ cnt, unsigned* __restrict__ ticket) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        cnt[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ticket = 0u;
+}
+
+template <typename T, int G>
+__global__ __launch_bounds__(TB) void k_syncfree_indeg(int32_t N, const int32_t* __restrict__ rp,
+                                                       const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                       const int32_t* __restrict__ dep_ptr,
+                                                       const int32_t* __restrict__ dep_row,
+                                                       const int32_t* __restrict__ indeg, int32_t* cnt,
+                                                       const T* __restrict__ b, T* x, unsigned* __restrict__ ticket,
+                                                       int* __restrict__ err, long long spin_limit) {
+    constexpr int R = WAVE / G;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int l = lane & (G - 1);
+    const int grp = lane / G;
+    while (true) {
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(ticket, (unsigned)R);
+        base = __shfl(base, 0, WAVE);
+        if (base >= (unsigned)N) break;
+        const int64_t i = (int64_t)base + grp;
+        bool done = i >= N;
+        const int32_t need = done ? 0 : indeg[i];
+        long long spins = 0;
+        while (__ballot(!done) != 0ull) {
+            // the group's lanes read the same counter: `ready` is group-uniform
+            const bool ready = !done && load_agent(&cnt[i]) >= need;
+            if (ready) {
+                const int32_t a0 = rp[i], e = rp[i + 1] - 1;
+                T s = b[i];
+                for (int32_t j0 = a0; j0 < e; j0 += G) {
+                    const int32_t j = j0 + l;
+                    const T p = j < e ? mul_rn(val[j], load_agent(&x[col[j]])) : T(0);
+                    if (G == 1) s = sub_rn(s, p);
+                    else s = group_ordered_sum<true, G>(s, p, min(G, e - j0));
+                }
+                if (l == 0) store_agent(&x[i], div_rn(s, val[e]));
+                // the store has completed before any dependent is told
+                __builtin_amdgcn_s_waitcnt(0);
+                for (int32_t k = dep_ptr[i] + l; k < dep_ptr[i + 1]; k += G)
+                    __hip_atomic_fetch_add(&cnt[dep_row[k]], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                done = true;
+            }
+            if (__ballot(ready) == 0ull) {
+                if (++spins > spin_limit) {
+                    if (lane == 0) atomicOr(err, 4);
+                    // give up: publish NaN and push, so that waiting rows finish too
+                    if (!done) {
+                        if (l == 0) store_agent(&x[i], T(__builtin_nan("")));
+                        __builtin_amdgcn_s_waitcnt(0);
+                        for (int32_t k = dep_ptr[i] + l; k < dep_ptr[i + 1]; k += G)
+                            __hip_atomic_fetch_add(&cnt[dep_row[k]], 1, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    done = true;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
@@ -801,9 +884,76 @@ static int launch_blocked(hipStream_t st, int num_cu, sable_spts_s* a, const T* 
     return SABLE_OK;
 }
 
+// The dependents of every row (the strictly lower part by column, rows
+// ascending) and the in-degrees, from the host copy of the structure.
+static int ensure_indeg(sable_spts_s* a) {
+    if (a->d_dep_ptr) return SABLE_OK;
+    const int32_t N = a->N;
+    const std::vector<int32_t>& rp = a->h_rp;
+    const std::vector<int32_t>& c = a->h_col;
+    const int64_t m = a->nnz - N;
+    std::vector<int32_t> dptr((size_t)N + 1, 0), drow((size_t)std::max<int64_t>(m, 1)), indeg(N);
+    for (int32_t i = 0; i < N; i++) {
+        indeg[i] = rp[i + 1] - 1 - rp[i];
+        for (int32_t j = rp[i]; j < rp[i + 1] - 1; j++) dptr[c[j] + 1]++;
+    }
+    for (int32_t j = 0; j < N; j++) dptr[j + 1] += dptr[j];
+    std::vector<int32_t> pos(dptr.begin(), dptr.end() - 1);
+    for (int32_t i = 0; i < N; i++)
+        for (int32_t j = rp[i]; j < rp[i + 1] - 1; j++) drow[pos[c[j]]++] = i;
+    hipStream_t st = a->ctx->stream;
+    bool ok = hipMalloc(&a->d_dep_ptr, 4 * ((size_t)N + 1)) == hipSuccess &&
+              hipMalloc(&a->d_dep_row, 4 * drow.size()) == hipSuccess &&
+              hipMalloc(&a->d_indeg, 4 * (size_t)N) == hipSuccess && hipMalloc(&a->d_cnt, 4 * (size_t)N) == hipSuccess;
+    if (ok)
+        ok = hipMemcpyAsync(a->d_dep_ptr, dptr.data(), 4 * dptr.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+             hipMemcpyAsync(a->d_dep_row, drow.data(), 4 * drow.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+             hipMemcpyAsync(a->d_indeg, indeg.data(), 4 * (size_t)N, hipMemcpyHostToDevice, st) == hipSuccess &&
+             hipStreamSynchronize(st) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        int32_t** ptrs[] = {&a->d_dep_ptr, &a->d_dep_row, &a->d_indeg, &a->d_cnt};
+        for (int32_t** p : ptrs) {
+            if (*p) (void)hipFree(*p);
+            *p = nullptr;
+        }
+        return fail(SABLE_ENOMEM, "in-degree tables: allocation or upload failed");
+    }
+    return SABLE_OK;
+}
+
+template <typename T>
+static void launch_indeg(hipStream_t st, int num_cu, sable_spts_s* a, const T* b, T* x) {
+    const int32_t N = a->N;
+    const int G = a->lanes;
+    hipLaunchKernelGGL(k_indeg_reset, dim3(gsz(N)), dim3(TB), 0, st, N, a->d_cnt, a->d_ticket);
+    const int64_t waves_needed = ((int64_t)N * G + WAVE - 1) / WAVE;
+    int64_t blocks = (waves_needed + (TB / WAVE) - 1) / (TB / WAVE);
+    const int64_t cap = (int64_t)num_cu * 8;
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+#define ID_LAUNCH(GG)                                                                                             \
+    hipLaunchKernelGGL((k_syncfree_indeg<T, GG>), dim3((unsigned)blocks), dim3(TB), 0, st, N, a->rp, a->col,     \
+                       (const T*)a->val, a->d_dep_ptr, a->d_dep_row, a->d_indeg, a->d_cnt, b, x, a->d_ticket,      \
+                       a->d_err, DEFAULT_SPIN)
+    switch (G) {
+        case 1: ID_LAUNCH(1); break;
+        case 4: ID_LAUNCH(4); break;
+        case 8: ID_LAUNCH(8); break;
+        case 16: ID_LAUNCH(16); break;
+        default: ID_LAUNCH(64); break;
+    }
+#undef ID_LAUNCH
+}
+
 static void free_analysis(sable_spts_s* a) {
     if (!a) return;
     free_blocked(a);
+    {
+        void* ptrs[] = {a->d_dep_ptr, a->d_dep_row, a->d_indeg, a->d_cnt};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+    }
     for (int k = 0; k < 2; k++) {
         if (a->exec[k]) (void)hipGraphExecDestroy(a->exec[k]);
         if (a->graph[k]) (void)hipGraphDestroy(a->graph[k]);
@@ -1210,6 +1360,20 @@ SABLE_API int sable_spts_solve_syncfree(sable_ctx_t* ctx, sable_spts_t* a, const
     else
         launch_syncfree<float>(ctx->stream, a->lanes, ctx->num_cu, a->N, a->rp, a->col, (const float*)a->val,
                                (const float*)b, (float*)x, a->d_ticket, a->d_err, DEFAULT_SPIN);
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+SABLE_API int sable_spts_solve_indegree(sable_ctx_t* ctx, sable_spts_t* a, const void* b, void* x) {
+    SABLE_CHECK(ctx && a && b && x, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(ctx == a->ctx, SABLE_EINVAL, "analysis belongs to another context");
+    SABLE_CHECK(b != x, SABLE_EINVAL, "b and x must be distinct");
+    const int rc = ensure_indeg(a);
+    if (rc) return rc;
+    if (a->dtype == SABLE_F64)
+        launch_indeg<double>(ctx->stream, ctx->num_cu, a, (const double*)b, (double*)x);
+    else
+        launch_indeg<float>(ctx->stream, ctx->num_cu, a, (const float*)b, (float*)x);
     SABLE_LAUNCH_CHECK();
     return SABLE_OK;
 }

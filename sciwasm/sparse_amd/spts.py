@@ -91,6 +91,11 @@ class SptsAnalysis:
     def solve_syncfree(self, b, x):
         call("sable_spts_solve_syncfree", self.ctx.handle, self.handle, b.vec.ptr, x.vec.ptr)
 
+    def solve_indegree(self, b, x):
+        """Sync-free with per-row in-degree counters (same bits as
+        solve_syncfree)."""
+        call("sable_spts_solve_indegree", self.ctx.handle, self.handle, b.vec.ptr, x.vec.ptr)
+
     def solve_combined(self, b, x):
         """Level + sync-free combined: the sync-free kernel over the
         level-ordered matrix; equal to solve_level bit for bit."""

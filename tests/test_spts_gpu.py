@@ -330,3 +330,24 @@ def test_tune_keeps_bits(ctx, name):
     an.solve_level(sp.vec_from_host(b), x)
     an.status()
     assert np.array_equal(x.array(), host_level_solution(N, rp, c, v, b))
+
+
+@pytest.mark.parametrize("lanes", [1, 4, 16])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("name", list(BLOCK_MATS))
+def test_indegree_syncfree_bit_exact(ctx, lanes, dtype, name):
+    """The sync-free solve with per-row in-degree counters: the reference
+    sync-free solve's bits at every lane count, on two solves in a row (the
+    counters are reset by each launch)."""
+    rp, c, v = BLOCK_MATS[name]()
+    v = v.astype(dtype)
+    N = len(rp) - 1
+    b = np.random.default_rng(13).uniform(-1, 1, N).astype(dtype)
+    want = oracle.spts_syncfree(N, rp, c, v, b)
+    an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v), lanes=lanes)
+    bv = sp.vec_from_host(b)
+    for _ in range(2):
+        x = sp.create_vec(N, 0.0, dtype)
+        an.solve_indegree(bv, x)
+        an.status()
+        assert np.array_equal(x.array().view(np.uint8), want.view(np.uint8)), name
