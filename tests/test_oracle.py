@@ -3,6 +3,8 @@ forms and order-sensitive known answers.  The reference ships no golden
 vectors for this path (its only checksums are for SuiteSparse matrices that
 are not in the repository), so these checks are what pin each restatement.
 CPU only."""
+import os
+
 import numpy as np
 import pytest
 
@@ -533,3 +535,21 @@ def test_rowmajor_ell_stops_at_negative_index_and_gs_order():
     ref = (cd.reshape(N, nc) * xv[ci.reshape(N, nc)]).sum(1)
     for gs in (False, True):
         assert np.array_equal(oracle.spmv_ell_row(N, nc, ci, cd, xv, np.zeros(N), gs=gs), ref)
+
+
+def test_threaded_baseline_under_tsan():
+    """The threaded CPU baseline (persistent workers over static_nnz row
+    ranges, barriers around each product) built with ThreadSanitizer: no
+    race reported, and every worker count gives the serial restatement's
+    bits."""
+    import shutil
+    import subprocess
+    if shutil.which("make") is None or shutil.which("gcc") is None:
+        pytest.skip("make or gcc missing")
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    r = subprocess.run(["make", "-s", "-C", here, "tsan"], capture_output=True, text=True, timeout=300)
+    if "unsupported" in r.stderr.lower() or "cannot find -ltsan" in r.stderr:
+        pytest.skip("no ThreadSanitizer runtime: " + r.stderr[-200:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ThreadSanitizer" not in r.stdout + r.stderr
+    assert "equal to the serial restatement" in r.stdout
