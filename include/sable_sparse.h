@@ -808,6 +808,24 @@ typedef struct {
     int32_t pad;
 } sable_batch_refine_item_t;
 
+/* Level-set solve of every item of a batch, level by level: one launch per
+ * level of the deepest matrix, each solving that level of every item at
+ * once (the batched form of the per-level launches of
+ * sable_spts_solve_level; the reference level solve's bits for each item at
+ * every lane count).  The plan reads only row_ptr, col and N of the host
+ * item array given at creation (device pointers; checks the structure:
+ * SABLE_EMALFORMED), computes every item's levels and keeps the row lists
+ * and chunk tables; a solve takes a device item array of the same matrices
+ * in the same order, with any b and x, and only launches.  No reference
+ * counterpart (the reference solves one matrix per call). */
+typedef struct sable_batch_level_plan_s sable_batch_level_plan_t;
+SABLE_API int sable_batch_level_plan_create(sable_ctx_t* ctx, int lanes, const sable_batch_spts_item_t* h_items,
+                                            int nitems, sable_batch_level_plan_t** out);
+SABLE_API void sable_batch_level_plan_destroy(sable_batch_level_plan_t* p);
+SABLE_API int sable_batch_level_plan_info(const sable_batch_level_plan_t* p, int32_t* levels, int64_t* chunks);
+SABLE_API int sable_batch_level_plan_solve(sable_ctx_t* ctx, const sable_batch_level_plan_t* p, sable_dtype_t dtype,
+                                           const sable_batch_spts_item_t* d_items, int nitems);
+
 /* FP32 sync-free solve + `iters` FP64 refinement steps for every item
  * (synchronises). */
 SABLE_API int sable_batch_spts_refined(sable_ctx_t* ctx, int lanes, const sable_batch_refine_item_t* d_items,

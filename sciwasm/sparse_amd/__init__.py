@@ -15,7 +15,7 @@ from .convert import coo_csr, coo_csc, num_cols, csr_ell, num_diags, csr_dia, tr
 from .spts import (SptsAnalysis, CSR_create_level_sets, CSR_create_level_sets_with_reorder,  # noqa: F401
                    spts_level_csr, spts_syncfree, spts_csc)
 from .tune import Autotuner, BatchPlan, row_stats_key  # noqa: F401
-from .serve import Server, batch_spmv, batch_spmv_tiles, batch_tile_plans, batch_spts, batch_spts_refined, lower_dominant  # noqa: F401
+from .serve import Server, BatchLevelPlan, batch_spmv, batch_spmv_tiles, batch_tile_plans, batch_spts, batch_spts_refined, lower_dominant  # noqa: F401
 from .mm import MMInfo, read_matrix_MM_files, create_COO_from_MM, create_LCOO_from_MM, mmread  # noqa: F401
 from .ewise import (expm1, log1p, sin, tan, pow, deg2rad, rad2deg, sign, abs, neg, sqrt, ceil, floor,  # noqa: F401
                     trunc, nearest, eliminate_zeros, count_nonzeros)
@@ -26,7 +26,7 @@ __all__ = [
     "sparse_COO_t", "sparse_CSC_t", "sparse_CSR_t", "sparse_DIA_t", "sparse_ELL_t", "sparse_vec_t",
     "coo_from_host", "csc_from_host", "csr_from_host", "dia_from_host", "ell_from_host",
     "create_vec", "vec_from_host", "clear_vec", "init_x", "fletcher_sum", "fletcher_sum_y",
-    "spmv", "CsrPlan", "CsrPanels", "CsrTiles", "CooPlan", "CsrAuto", "Autotuner", "BatchPlan", "row_stats_key",
+    "spmv", "CsrPlan", "CsrPanels", "CsrTiles", "CooPlan", "CsrAuto", "Autotuner", "BatchPlan", "BatchLevelPlan", "row_stats_key",
     "coo_csr", "coo_csc", "num_cols", "csr_ell", "num_diags", "csr_dia", "transpose",
     "SptsAnalysis", "CSR_create_level_sets", "CSR_create_level_sets_with_reorder", "spts_level_csr",
     "spts_syncfree", "spts_csc", "Server", "batch_spmv", "batch_spmv_tiles", "batch_tile_plans", "batch_spts", "batch_spts_refined", "lower_dominant",

@@ -173,6 +173,10 @@ SIGNATURES = {
     "sable_spts_solve_level": [c_vp, c_vp, c_vp, c_vp],
     "sable_spts_solve_syncfree": [c_vp, c_vp, c_vp, c_vp],
     "sable_spts_solve_indegree": [c_vp, c_vp, c_vp, c_vp],
+    "sable_batch_level_plan_create": [c_vp, c_int, c_vp, c_int, c_vpp],
+    "sable_batch_level_plan_destroy": [c_vp],
+    "sable_batch_level_plan_info": [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i64)],
+    "sable_batch_level_plan_solve": [c_vp, c_vp, c_int, c_vp, c_int],
     "sable_spts_solve_combined": [c_vp, c_vp, c_vp, c_vp],
     "sable_spts_status": [c_vp, c_vp],
     "sable_spts_solve_refined": [c_vp, c_vp, c_vp, c_vp, c_int, c_int],
@@ -206,6 +210,7 @@ _RESTYPES = {
     "sable_spts_destroy": None,
     "sable_csr_panels_destroy": None,
     "sable_server_destroy": None,
+    "sable_batch_level_plan_destroy": None,
     "sable_ctx_stream": c_vp,
     "sable_last_error": ctypes.c_char_p,
     "sable_version": ctypes.c_char_p,

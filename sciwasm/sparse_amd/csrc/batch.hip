@@ -19,6 +19,17 @@
 #include "common.h"
 #include "batch.h"
 
+struct sable_batch_level_plan_s {
+    sable_ctx_t* ctx = nullptr;
+    int lanes = 4;
+    int nitems = 0;
+    int64_t rows = 0;
+    int64_t nchunks = 0;
+    int32_t* d_rows = nullptr;           // every item's rows listed by level
+    sable::RowChunk* d_chunks = nullptr;  // the chunks of every level, level after level
+    sable::LevelPlan plan;               // level l: chunks [chunk_lo[l], chunk_lo[l + 1])
+};
+
 namespace sable {
 
 constexpr int BB = 256;
@@ -878,6 +889,129 @@ SABLE_API int sable_batch_spts_refined(sable_ctx_t* ctx, int lanes, const sable_
     SABLE_HIP(hipStreamSynchronize(ctx->stream));
     SABLE_CHECK(!*(int*)ctx->pinned, SABLE_ETIMEOUT, "batched sync-free solve exceeded its spin limit");
     return SABLE_OK;
+}
+
+// ------------------------------------------------ batched level schedule
+SABLE_API int sable_batch_level_plan_create(sable_ctx_t* ctx, int lanes, const sable_batch_spts_item_t* h_items,
+                                            int nitems, sable_batch_level_plan_t** out) {
+    SABLE_CHECK(ctx && out && (h_items || nitems == 0) && nitems >= 0, SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(lanes == 1 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 64, SABLE_EINVAL,
+                "lanes must be 1, 4, 8, 16 or 64");
+    *out = nullptr;
+    hipStream_t st = ctx->stream;
+    const int32_t CH = BB / lanes;
+    // levels of every item (par/src/sparse32.js:2952-2974: 0 for a row with
+    // only its diagonal, else 1 + the largest level among its columns), rows
+    // listed by level, ascending within a level
+    std::vector<std::vector<int32_t>> lps((size_t)nitems);
+    std::vector<int32_t> rows_all;
+    std::vector<int64_t> row_off((size_t)nitems + 1, 0);
+    std::vector<int32_t> rp, col, level, next;
+    for (int k = 0; k < nitems; k++) {
+        const sable_batch_spts_item_t& it = h_items[k];
+        const int32_t N = it.N;
+        SABLE_CHECK(N >= 0 && (N == 0 || (it.row_ptr && it.col)), SABLE_EINVAL, "item %d: bad matrix", k);
+        row_off[(size_t)k + 1] = row_off[k] + N;
+        if (N == 0) continue;
+        rp.resize((size_t)N + 1);
+        SABLE_HIP(hipMemcpyAsync(rp.data(), it.row_ptr, 4 * ((size_t)N + 1), hipMemcpyDeviceToHost, st));
+        SABLE_HIP(hipStreamSynchronize(st));
+        SABLE_CHECK(rp[0] == 0 && rp[N] >= N, SABLE_EMALFORMED, "item %d: row_ptr must start at 0", k);
+        col.resize((size_t)rp[N]);
+        SABLE_HIP(hipMemcpyAsync(col.data(), it.col, 4 * (size_t)rp[N], hipMemcpyDeviceToHost, st));
+        SABLE_HIP(hipStreamSynchronize(st));
+        level.resize(N);
+        int32_t top = 0;
+        for (int32_t i = 0; i < N; i++) {
+            SABLE_CHECK(rp[i + 1] > rp[i] && col[(size_t)rp[i + 1] - 1] == i, SABLE_EMALFORMED,
+                        "item %d, row %d: the last entry is not the diagonal", k, i);
+            int32_t m = -1;
+            for (int32_t j = rp[i]; j < rp[i + 1] - 1; j++) {
+                SABLE_CHECK(col[j] >= 0 && col[j] < i, SABLE_EMALFORMED, "item %d, row %d: column %d is not below "
+                            "the diagonal", k, i, col[j]);
+                m = std::max(m, level[col[j]]);
+            }
+            level[i] = m + 1;
+            top = std::max(top, level[i]);
+        }
+        std::vector<int32_t>& lp = lps[k];
+        lp.assign((size_t)top + 2, 0);
+        for (int32_t i = 0; i < N; i++) lp[(size_t)level[i] + 1]++;
+        for (int32_t l = 0; l <= top; l++) lp[(size_t)l + 1] += lp[l];
+        next.assign(lp.begin(), lp.end() - 1);
+        const size_t base = rows_all.size();
+        rows_all.resize(base + N);
+        for (int32_t i = 0; i < N; i++) rows_all[base + next[level[i]]++] = i;
+    }
+    auto* p = new sable_batch_level_plan_s();
+    p->ctx = ctx;
+    p->lanes = lanes;
+    p->nitems = nitems;
+    p->rows = (int64_t)rows_all.size();
+    if (!rows_all.empty() && hipMalloc(&p->d_rows, 4 * rows_all.size()) != hipSuccess) {
+        (void)hipGetLastError();
+        delete p;
+        return fail(SABLE_ENOMEM, "batched level plan: row lists");
+    }
+    // chunks, level by level: level l lists, item by item, the rows of level
+    // l of every item that has one, CH rows (one workgroup) per chunk
+    int top = 0;
+    for (auto& lp : lps) top = std::max(top, (int)lp.size() - 1);
+    std::vector<RowChunk> ch;
+    p->plan.chunk_lo.assign((size_t)top + 1, 0);
+    for (int lv = 0; lv < top; lv++) {
+        p->plan.chunk_lo[lv] = (int32_t)ch.size();
+        for (int k = 0; k < nitems; k++) {
+            const std::vector<int32_t>& lp = lps[k];
+            if (lv + 1 >= (int)lp.size()) continue;
+            for (int32_t a = lp[lv]; a < lp[(size_t)lv + 1]; a += CH)
+                ch.push_back(RowChunk{p->d_rows + row_off[k] + a, k, 0, std::min(CH, lp[(size_t)lv + 1] - a), 0});
+        }
+    }
+    p->plan.chunk_lo[top] = (int32_t)ch.size();
+    p->nchunks = (int64_t)ch.size();
+    if (!ch.empty() && hipMalloc(&p->d_chunks, sizeof(RowChunk) * ch.size()) != hipSuccess) {
+        (void)hipGetLastError();
+        sable_batch_level_plan_destroy(p);
+        return fail(SABLE_ENOMEM, "batched level plan: chunk table");
+    }
+    hipError_t e = hipSuccess;
+    if (!rows_all.empty())
+        e = hipMemcpyAsync(p->d_rows, rows_all.data(), 4 * rows_all.size(), hipMemcpyHostToDevice, st);
+    if (!e && !ch.empty()) e = hipMemcpyAsync(p->d_chunks, ch.data(), sizeof(RowChunk) * ch.size(),
+                                              hipMemcpyHostToDevice, st);
+    if (!e) e = hipStreamSynchronize(st);
+    if (e) {
+        sable_batch_level_plan_destroy(p);
+        return fail(SABLE_EHIP, "batched level plan upload: %s", hipGetErrorString(e));
+    }
+    *out = p;
+    return SABLE_OK;
+}
+
+SABLE_API void sable_batch_level_plan_destroy(sable_batch_level_plan_t* p) {
+    if (!p) return;
+    (void)hipStreamSynchronize(p->ctx->stream);
+    if (p->d_rows) (void)hipFree(p->d_rows);
+    if (p->d_chunks) (void)hipFree(p->d_chunks);
+    delete p;
+}
+
+SABLE_API int sable_batch_level_plan_info(const sable_batch_level_plan_t* p, int32_t* levels, int64_t* chunks) {
+    SABLE_CHECK(p, SABLE_EINVAL, "NULL plan");
+    if (levels) *levels = p->plan.levels();
+    if (chunks) *chunks = p->nchunks;
+    return SABLE_OK;
+}
+
+SABLE_API int sable_batch_level_plan_solve(sable_ctx_t* ctx, const sable_batch_level_plan_t* p, sable_dtype_t dtype,
+                                           const sable_batch_spts_item_t* d_items, int nitems) {
+    SABLE_CHECK(ctx && p && (d_items || nitems == 0), SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(ctx == p->ctx, SABLE_EINVAL, "plan belongs to another context");
+    SABLE_CHECK(nitems == p->nitems, SABLE_EINVAL, "%d items for a plan of %d", nitems, p->nitems);
+    SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
+    if (p->nchunks == 0) return SABLE_OK;
+    return batch_level_launch(ctx->stream, dtype, p->lanes, d_items, p->d_chunks, p->plan);
 }
 
 }  // extern "C"

@@ -140,6 +140,55 @@ def batch_spts(mats, bs, xs, lanes=4):
     call("sable_batch_spts", ctx.handle, dtype_code(mats[0].dtype), int(lanes), d.ptr, len(items), total)
 
 
+class BatchLevelPlan:
+    """The level schedule of many lower matrices (sable_batch_level_plan_*):
+    bind(bs, xs).run() solves every L_k x_k = b_k level by level, one launch
+    per level of the deepest matrix, each item with the reference level
+    solve's bits."""
+
+    def __init__(self, mats, lanes=4):
+        self.mats = list(mats)
+        self.ctx = self.mats[0].row.ctx
+        items = (BatchSptsItem * len(self.mats))()
+        for k, L in enumerate(self.mats):
+            items[k] = BatchSptsItem(L.row.ptr, L.col.ptr, L.val.ptr, None, None, 0, L.N, 0)
+        h = c_vp()
+        call("sable_batch_level_plan_create", self.ctx.handle, int(lanes), ctypes.cast(items, c_vp), len(self.mats),
+             ctypes.byref(h))
+        self.handle = h
+
+    def info(self):
+        lv, ch = c_i32(), c_i64()
+        call("sable_batch_level_plan_info", self.handle, ctypes.byref(lv), ctypes.byref(ch))
+        return {"levels": lv.value, "chunks": ch.value}
+
+    def bind(self, bs, xs):
+        items, total = [], 0
+        for L, b, x in zip(self.mats, bs, xs):
+            items.append(BatchSptsItem(L.row.ptr, L.col.ptr, L.val.ptr, b.ptr, x.ptr, total, L.N, 0))
+            total += L.N
+        d = _upload_items(items, self.ctx)
+        code = dtype_code(self.mats[0].dtype)
+
+        class _Run:
+            def run(_self):
+                call("sable_batch_level_plan_solve", self.ctx.handle, self.handle, code, d.ptr, len(items))
+        r = _Run()
+        r.keep = d
+        return r
+
+    def solve(self, bs, xs):
+        self.bind(bs, xs).run()
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.load().sable_batch_level_plan_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
 def batch_spts_refined(mats, bs, xs, lanes=4, iters=2):
     """FP64 matrices; FP32 solves with FP64 refinement, all in one batch."""
     ctx = mats[0].row.ctx

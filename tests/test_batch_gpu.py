@@ -337,6 +337,47 @@ def test_batch_spts(ctx, mats, lanes):
         assert np.array_equal(x, oracle.spts_syncfree(n, rp, c, v, bh[k])), k
 
 
+def level_solution(n, rp, c, v, b):
+    nl, lp, perm, order, _ = oracle.level_sets(n, rp, c)
+    nrp, nc_, nv = oracle.level_reorder(n, rp, c, v, perm, order)
+    yl = oracle.spts_level(n, nrp, nc_, nv, b[perm])
+    x = np.empty_like(yl)
+    x[perm] = yl
+    return x
+
+
+@pytest.mark.parametrize("lanes", [1, 4, 16])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batch_level_plan(ctx, mats, lanes, dtype):
+    """The batched level schedule: every item's solve is the reference level
+    solve's bits, twice with other right-hand sides; one matrix with only
+    its diagonal in the batch."""
+    Ls = lower_set(mats, dtype) + [(3, np.arange(4, dtype=np.int32), np.arange(3, dtype=np.int32),
+                                    np.array([2.0, 4.0, 8.0], dtype))]
+    dL = [sp.csr_from_host(n, rp, c, v) for n, rp, c, v in Ls]
+    plan = sp.BatchLevelPlan(dL, lanes=lanes)
+    info = plan.info()
+    assert info["levels"] >= 1 and info["chunks"] >= len(Ls)
+    rng = np.random.default_rng(21)
+    for rep in range(2):
+        bh = [rng.uniform(-1, 1, n).astype(dtype) for n, _, _, _ in Ls]
+        bs = [DeviceArray.from_host(b) for b in bh]
+        xs = [DeviceArray.zeros(n, dtype) for n, _, _, _ in Ls]
+        plan.solve(bs, xs)
+        ctx.sync()
+        for k, (n, rp, c, v) in enumerate(Ls):
+            assert np.array_equal(xs[k].numpy(), level_solution(n, rp, c, v, bh[k])), (rep, k)
+
+
+def test_batch_level_plan_rejects_malformed(ctx):
+    rp = np.array([0, 1, 3], np.int32)
+    c = np.array([0, 1, 0], np.int32)  # row 1 ends with column 0: the diagonal is not last
+    A = sp.csr_from_host(2, rp, c, np.ones(3))
+    with pytest.raises(SableError) as e:
+        sp.BatchLevelPlan([A])
+    assert e.value.name == "SABLE_EMALFORMED"
+
+
 def test_batch_spts_refined(ctx, mats):
     Ls = lower_set(mats)
     rng = np.random.default_rng(4)
