@@ -1024,7 +1024,7 @@ def sec_serve(args, sp, ctx, pool, lows, big):
         base, *opts = mode.split("+")
         sweep = "syncfree" if "sf" in opts else "levels"
         S = sp.Server(mode=base, arena_bytes=args.serve_arena_mb << 20, max_batch=args.serve_max_batch, slots=3,
-                      refine_iters=args.refine_iters, spts_lanes=4, group_rhs=args.group_rhs,
+                      refine_iters=args.refine_iters, spts_lanes=args.serve_lanes, group_rhs=args.group_rhs,
                       batch_tiles="tiles" in opts, spts_sweep=sweep, ctx=ctx)
         t0 = time.time()
         for k in range(nsmall):
@@ -1069,7 +1069,7 @@ def sec_serve(args, sp, ctx, pool, lows, big):
     if "mixed+tiles" in best and "plain" in best:
         out["speedup_mixed_tiles_vs_plain"] = r4(best["mixed+tiles"] / max(best["plain"], 1e-9))
     out["config"] = {"max_batch": args.serve_max_batch, "arena_mb_per_slot": args.serve_arena_mb, "slots": 3,
-                     "refine_iters": args.refine_iters, "group_rhs": args.group_rhs}
+                     "refine_iters": args.refine_iters, "group_rhs": args.group_rhs, "spts_lanes": args.serve_lanes}
     return out
 
 
@@ -1112,6 +1112,7 @@ def main():
     ap.add_argument("--serve-large", type=int, default=8)
     ap.add_argument("--serve-modes", type=str, default="plain,batched,mixed,mixed+tiles")
     ap.add_argument("--serve-max-batch", type=int, default=4096)
+    ap.add_argument("--serve-lanes", type=int, default=4, help="lanes per row of the served solves")
     ap.add_argument("--serve-arena-mb", type=int, default=2048)
     ap.add_argument("--refine-iters", type=int, default=1)
     ap.add_argument("--spts-blocks", type=str, default="2048,4096,8192,16384",
