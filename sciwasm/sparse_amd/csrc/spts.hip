@@ -154,6 +154,51 @@ __global__ __launch_bounds__(TB) void k_level(int32_t r0, int32_t nrows, const i
     if (valid && l == 0) yl[i] = div_rn(sub_rn(bl[i], s), val[e]);
 }
 
+// A run of consecutive narrow levels in one launch of one workgroup: the
+// levels one after another with a barrier between them, each in passes of
+// 1024 / G rows, with k_level's arithmetic (the same bits).  The run's level
+// bounds travel in the kernel arguments.
+constexpr int RUN_MAX = 64;
+constexpr int RUN_TB = 1024;
+struct LevelRun {
+    int32_t n;                 // levels in the run
+    int32_t lp[RUN_MAX + 1];   // their bounds in the level-ordered rows
+};
+
+template <typename T, int G>
+__global__ __launch_bounds__(RUN_TB) void k_level_run(LevelRun run, const int32_t* __restrict__ rp,
+                                                      const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                      const T* __restrict__ bl, T* yl) {
+    constexpr int RPP = RUN_TB / G;
+    const int g = threadIdx.x / G;
+    const int l = threadIdx.x & (G - 1);
+    for (int k = 0; k < run.n; k++) {
+        const int32_t r1 = run.lp[k + 1];
+        for (int32_t base = run.lp[k]; base < r1; base += RPP) {
+            const int32_t i = base + g;
+            const bool valid = i < r1;
+            T s = T(0);
+            int32_t a = 0, e = 0;
+            if (valid) {
+                a = rp[i];
+                e = rp[i + 1] - 1;
+            }
+            for (int32_t j0 = a; j0 < e; j0 += G) {
+                const int32_t j = j0 + l;
+                const T p = j < e ? mul_rn(val[j], yl[col[j]]) : T(0);
+                if (G == 1) {
+                    s = add_rn(s, p);
+                } else {
+                    const int n = min(G, e - j0);
+                    s = group_ordered_sum<false, G>(s, p, n);
+                }
+            }
+            if (valid && l == 0) yl[i] = div_rn(sub_rn(bl[i], s), val[e]);
+        }
+        __syncthreads();
+    }
+}
+
 template <typename T>
 __global__ void k_gather_perm(int32_t N, const int32_t* __restrict__ perm, const T* __restrict__ src,
                               T* __restrict__ dst) {
@@ -630,14 +675,48 @@ static void launch_level(hipStream_t st, int G, int32_t r0, int32_t nrows, const
 }
 
 template <typename T>
+static void launch_run(hipStream_t st, int G, const LevelRun& run, const int32_t* rp, const int32_t* col, const T* val,
+                       const T* bl, T* yl) {
+#define RUN_LAUNCH(GG) \
+    hipLaunchKernelGGL((k_level_run<T, GG>), dim3(1), dim3(RUN_TB), 0, st, run, rp, col, val, bl, yl)
+    switch (G) {
+        case 1: RUN_LAUNCH(1); break;
+        case 4: RUN_LAUNCH(4); break;
+        case 8: RUN_LAUNCH(8); break;
+        case 16: RUN_LAUNCH(16); break;
+        default: RUN_LAUNCH(64); break;
+    }
+#undef RUN_LAUNCH
+}
+
+// The levels in order: a level of at most two passes of one 1024-thread
+// workgroup (2048 / G rows) joins a run of such levels solved in one
+// launch (up to RUN_MAX levels), every other level takes its own launch.
+template <typename T>
 static int levels_on_stream(hipStream_t st, int G, const std::vector<int32_t>& lp, const int32_t* rp,
                             const int32_t* col, const T* val, const T* bl, T* yl) {
     const int nl = (int)lp.size() - 1;
+    const int32_t narrow = 2 * (RUN_TB / G);
+    LevelRun run;
+    run.n = 0;
+    auto flush = [&]() {
+        if (run.n == 1) launch_level<T>(st, G, run.lp[0], run.lp[1] - run.lp[0], rp, col, val, bl, yl);
+        else if (run.n > 1) launch_run<T>(st, G, run, rp, col, val, bl, yl);
+        run.n = 0;
+    };
     for (int l = 0; l < nl; l++) {
         const int32_t r0 = lp[l], nr = lp[l + 1] - lp[l];
         if (nr <= 0) continue;
-        launch_level<T>(st, G, r0, nr, rp, col, val, bl, yl);
+        if (nr <= narrow) {
+            if (run.n == 0) run.lp[0] = r0;
+            run.lp[++run.n] = r0 + nr;
+            if (run.n == RUN_MAX) flush();
+        } else {
+            flush();
+            launch_level<T>(st, G, r0, nr, rp, col, val, bl, yl);
+        }
     }
+    flush();
     SABLE_LAUNCH_CHECK();
     return SABLE_OK;
 }
