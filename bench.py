@@ -70,12 +70,13 @@ TILE_MODES = {0: ("perpanel", "k_tiles"), 1: ("fused", "k_tiles_fused")}
 # changed kernel source reports traffic null until it is measured again.
 MEASURED_TRAFFIC = {
     "tiles_16384x8192_fused": {
-        "per_product": 2.962e9, "per_launch": 2.962e9,
+        "per_product": 2.858e9, "per_launch": 2.858e9,
         "source": "sciwasm/sparse_amd/csrc/spmv_tiles.hip",
-        "sha256": "5cbd39fb33abfc7657c55485cc49dc4dbe8669b074e21fd79fe4ee5096f8d674",
-        "limiter": "x gathers: every CU touches ~78% of x's 64-byte sectors (805k entries over 524k sectors), "
-                   "~52M L1 misses to the L2 per product beside ~39M streaming requests; fetched traffic "
-                   "1.16x the algorithmic bytes"},
+        "sha256": "4ff0891a1cd5dcb1da0a76a17d5129b803968cf4a21ed5f716ec961321da4714",
+        "limiter": "L2 request rate: TCC busy 97% of the kernel's cycles at 0.47 requests per cycle per "
+                   "channel; ~52M x-line requests (every CU touches most of x's lines: 805k entries over "
+                   "262k 128-byte lines) beside ~39M streaming requests and 2.1M x prefetches; fetched "
+                   "traffic 1.12x the algorithmic bytes"},
 }
 
 

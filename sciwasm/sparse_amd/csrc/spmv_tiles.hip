@@ -11,13 +11,15 @@
 //  * within each (panel, row tile) the entries are sorted by column (stable:
 //    equal columns keep ascending rows) and stored as one 32-bit word,
 //    column-in-panel << row_bits | row-in-tile, plus the value;
-//  * one workgroup per row tile streams its entries panel by panel, gathers
-//    x (consecutive lanes hit the same or adjacent x lines, so the gathers
-//    of a wave coalesce into fewer L2 requests, and a panel narrow enough
-//    keeps the lines in flight within the CU's L1), and accumulates the
-//    products into the tile's rows held in LDS with LDS atomic adds; y is
-//    written once at the end (or, in the per-panel schedule, one launch per
-//    panel adds the panel's sums to y).
+//  * one workgroup per row tile streams its entries panel by panel (two
+//    panels per step in the one-launch schedule), gathers x (consecutive
+//    lanes hit the same or adjacent x lines, so the gathers of a wave
+//    coalesce into fewer L2 requests), and accumulates the products into
+//    the tile's rows held in LDS with LDS atomic adds; y is written once at
+//    the end (or, in the per-panel schedule, one launch per panel adds the
+//    panel's sums to y);
+//  * in the one-launch schedule the workgroups of each XCD also load the x
+//    lines of the next step's panels into their L2 ahead of the gathers.
 // Each row's products are summed in an order that depends on the LDS atomic
 // order, so the result is not bit-reproducible from run to run; it meets the
 // same per-row bound as every other CSR schedule.
@@ -164,9 +166,29 @@ __global__ __launch_bounds__(TILE_BS) void k_tiles(int32_t N, int32_t tile_rows,
 }
 
 // All panels in one launch: the workgroup of a row tile walks the panels in
-// order with its row sums kept in LDS throughout, and writes y once.  The
-// workgroups start together and proceed through the panels at about the
-// same pace, so the x panels in use at any time stay few and L2-resident.
+// order with its row sums kept in LDS throughout, and writes y once.
+//  * A step takes TILE_G panels: each thread loads up to TILE_GU entries of
+//    each (the loads of both panels in flight together), gathers their x,
+//    and adds the products; entries past TILE_GU * TILE_BS in a panel are
+//    taken TILE_GU at a time after that.  There is no barrier between
+//    steps: each wave walks on its own.
+//  * The workgroups start together and go through the panels at about the
+//    same pace, since every step costs one load -> gather -> add chain, so
+//    the x panels in use at any time stay few and L2-resident.
+//  * x prefetch: workgroup w runs on XCD w % 8, so the w / 8-th workgroup
+//    of each XCD, with its 16 waves, is one of that XCD's (ntr + 7) / 8
+//    slots.  Each wave loads one 128-byte x line of each panel of the next
+//    step (line = slot * 16 + wave), so that the lines of those panels are
+//    in the XCD's L2 before its workgroups gather from them; otherwise the
+//    first gather of a line misses and the gathers of the other workgroups
+//    on that XCD wait behind its fill.  The loaded values are kept in
+//    registers only until the step ends (an empty asm statement consumes
+//    them); a panel wider than the XCD's waves can cover is prefetched in
+//    part.  The mapping only decides which L2 a line warms: results do not
+//    depend on it.
+constexpr int TILE_G = 2;
+constexpr int TILE_GU = 2;
+
 template <typename T>
 __global__ __launch_bounds__(TILE_BS) void k_tiles_fused(int32_t N, int32_t tile_rows, int row_bits, int32_t ntr,
                                                          int32_t npanels, int32_t panel_cols,
@@ -177,31 +199,72 @@ __global__ __launch_bounds__(TILE_BS) void k_tiles_fused(int32_t N, int32_t tile
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* acc = reinterpret_cast<double*>(smem_raw);
     const int tr = blockIdx.x;
+    constexpr int XL = 128 / (int)sizeof(T);  // x values per 128-byte line
+    const int pf_line = (int)(blockIdx.x >> 3) * (TILE_BS / 64) + (int)(threadIdx.x >> 6);
+    const bool pf_on = (int64_t)pf_line * XL < panel_cols;
+    T pfs = T(0);
     for (int r = threadIdx.x; r < tile_rows; r += TILE_BS) acc[r] = 0.0;
     __syncthreads();
     const uint32_t rmask = (1u << row_bits) - 1u;
-    for (int32_t panel = 0; panel < npanels; panel++) {
-        const int64_t t = (int64_t)panel * ntr + tr;
-        const int64_t k0 = tile_ptr[t], k1 = tile_ptr[t + 1];
-        const T* __restrict__ xp = x + (int64_t)panel * panel_cols;
-        for (int64_t k = k0 + threadIdx.x; k < k1; k += (int64_t)TILE_U * TILE_BS) {
-            uint32_t pk[TILE_U];
-            T v[TILE_U];
+    // entry positions fit 32 unsigned bits: nnz < 2^31 (int32 row_ptr), and
+    // a position runs at most TILE_GU * TILE_BS past its range's end
+    for (int32_t p = 0; p < npanels; p += TILE_G) {
+        uint32_t k0[TILE_G], k1[TILE_G];
 #pragma unroll
-            for (int u = 0; u < TILE_U; u++) {
-                const int64_t kk = k + (int64_t)u * TILE_BS;
-                pk[u] = kk < k1 ? ld_nt(packed + kk) : 0u;
-                v[u] = kk < k1 ? ld_nt(val + kk) : T(0);
+        for (int g = 0; g < TILE_G; g++) {
+            if (p + g < npanels) {
+                const int64_t t = (int64_t)(p + g) * ntr + tr;
+                k0[g] = (uint32_t)tile_ptr[t];
+                k1[g] = (uint32_t)tile_ptr[t + 1];
+            } else {
+                k0[g] = k1[g] = 0;
             }
-            T xv[TILE_U];
-#pragma unroll
-            for (int u = 0; u < TILE_U; u++)
-                xv[u] = (k + (int64_t)u * TILE_BS < k1) ? xp[pk[u] >> row_bits] : T(0);
-#pragma unroll
-            for (int u = 0; u < TILE_U; u++)
-                if (k + (int64_t)u * TILE_BS < k1) atomicAdd(&acc[pk[u] & rmask], (double)(v[u] * xv[u]));
         }
+        uint32_t pk[TILE_G][TILE_GU];
+        T v[TILE_G][TILE_GU];
+#pragma unroll
+        for (int g = 0; g < TILE_G; g++)
+#pragma unroll
+            for (int u = 0; u < TILE_GU; u++) {
+                const uint32_t kk = k0[g] + threadIdx.x + (uint32_t)(u * TILE_BS);
+                pk[g][u] = kk < k1[g] ? ld_nt(packed + kk) : 0xffffffffu;
+                v[g][u] = kk < k1[g] ? ld_nt(val + kk) : T(0);
+            }
+        // 0xffffffff marks an empty slot: a packed word never has it, since
+        // the column-in-panel is below 2^col_bits and col_bits + row_bits < 32
+        T pf = T(0);
+        if (pf_on) {
+#pragma unroll
+            for (int g = 0; g < TILE_G; g++) {
+                const int64_t i = (int64_t)(p + TILE_G + g) * panel_cols + (int64_t)pf_line * XL;
+                if (p + TILE_G + g < npanels && i < N) pf += x[i];
+            }
+        }
+        T xv[TILE_G][TILE_GU];
+#pragma unroll
+        for (int g = 0; g < TILE_G; g++)
+#pragma unroll
+            for (int u = 0; u < TILE_GU; u++)
+                xv[g][u] = pk[g][u] != 0xffffffffu ? x[(int64_t)(p + g) * panel_cols + (pk[g][u] >> row_bits)] : T(0);
+#pragma unroll
+        for (int g = 0; g < TILE_G; g++)
+#pragma unroll
+            for (int u = 0; u < TILE_GU; u++)
+                if (pk[g][u] != 0xffffffffu) atomicAdd(&acc[pk[g][u] & rmask], (double)(v[g][u] * xv[g][u]));
+        // the rest of a panel holding more than TILE_GU * TILE_BS entries
+#pragma unroll
+        for (int g = 0; g < TILE_G; g++) {
+            const T* __restrict__ xp = x + (int64_t)(p + g) * panel_cols;
+            for (uint32_t k = k0[g] + threadIdx.x + (uint32_t)(TILE_GU * TILE_BS); k < k1[g]; k += TILE_BS) {
+                const uint32_t q = packed[k];
+                atomicAdd(&acc[q & rmask], (double)(val[k] * xp[q >> row_bits]));
+            }
+        }
+        pfs += pf;
     }
+    // the prefetched values are consumed once, after the walk, so that the
+    // loads are kept without tying the walk's instruction order to them
+    asm volatile("" ::"v"(pfs));
     __syncthreads();
     const int64_t r0 = (int64_t)tr * tile_rows;
     for (int r = threadIdx.x; r < tile_rows; r += TILE_BS) {
@@ -424,7 +487,8 @@ template <typename T>
 static int tiles_run_t(hipStream_t st, const sable_csr_tiles_s* p, const void* x, void* y, double beta) {
     const size_t shm = (size_t)p->tile_rows * sizeof(double);
     if (g_tiles_mode == 1) {
-        hipLaunchKernelGGL(k_tiles_fused<T>, dim3((unsigned)p->ntr), dim3(TILE_BS), shm, st, p->N,
+        auto kern = k_tiles_fused<T>;
+        hipLaunchKernelGGL(kern, dim3((unsigned)p->ntr), dim3(TILE_BS), shm, st, p->N,
                            p->tile_rows, p->row_bits, p->ntr, p->npanels, p->panel_cols, p->tile_ptr, p->packed,
                            (const T*)p->val, (const T*)x, (T*)y, (T)beta);
         SABLE_LAUNCH_CHECK();
