@@ -17,7 +17,7 @@ Also measured and reported under "extra" (each section bounded to seconds):
   addon   the reference's call surface: spmv_csr_wrapper through the
           Node-API addon (node sciwasm/sparse_amd/node/bench_node.js) on the
           c1 matrix (inner_max 1 and 500) and on the headline matrix
-          (inner_max 1 and 10), x and y across PCIe on every call
+          (inner_max 1 and 100), x and y across PCIe on every call
   band    FP64 7-diagonal band, 16,777,216 rows (offsets 0, +-1, +-64,
           +-4096): csr_dia / csr_ell / coo_csr / transpose on the GPU and
           DIA/ELL/CSR SpMV GB/s
@@ -322,7 +322,7 @@ def sec_addon(args, sp, ctx, host):
     """The reference's call surface, timed: spmv_csr_wrapper through the
     Node-API addon (bench_node.js), matrix resident after the first call,
     x and y across PCIe on every call, on C1 (FP32 Laplacian 256^2, inner_max
-    1 and 500) and on the headline matrix (FP64, inner_max 1 and 10)."""
+    1 and 500) and on the headline matrix (FP64, inner_max 1 and 100)."""
     import tempfile
     import shutil
     import oracle
@@ -358,20 +358,23 @@ def sec_addon(args, sp, ctx, host):
             d = os.path.join(base, "c2a")
             os.makedirs(d)
             lin = int((4 * (N + 1) + 12 * nnz + 16 * N) / (1 << 20)) + 256
-            r, y = _addon_run(d, 64, rp, col, val, x, 2, 10, (1, 10), lin)
+            im = 100
+            r, y = _addon_run(d, 64, rp, col, val, x, 2, 10, (1, im), lin)
             b = 4 * (N + 1) + 12 * nnz + 16 * N
             pcie = 24 * N  # x and y up, y down per call
-            c1m, c10 = r["runs"]["1"]["median_ms"], r["runs"]["10"]["median_ms"]
+            c1m, cim = r["runs"]["1"]["median_ms"], r["runs"][str(im)]["median_ms"]
             # per-call cost = transfers + products: the two call shapes give
-            # the transfer part and the product part apart
-            prod = (c10 - c1m) / 9
+            # the transfer part and the product part apart (a long call keeps
+            # the GPU busy, so its products run at the clocks of a busy GPU)
+            prod = (cim - c1m) / (im - 1)
             out["c2a"] = {"N": N, "nnz": nnz, "schedule": names.get(r["schedule"], r["schedule"]),
                           "load_s": r4(r["load_s"]), "first_call_ms": r4(r["first_call_ms"]),
-                          "call_ms_inner1": r4(c1m), "call_ms_inner10": r4(c10),
+                          "call_ms_inner1": r4(c1m), "call_ms_inner%d" % im: r4(cim),
                           "product_ms_from_calls": r4(prod), "transfer_ms_per_call": r4(c1m - prod),
                           "pcie_gbs": r4(pcie / ((c1m - prod) * 1e-3) / 1e9) if c1m > prod else None,
-                          "gbs_inner1": r4(b / (c1m * 1e-3) / 1e9), "gbs_inner10": r4(10 * b / (c10 * 1e-3) / 1e9),
-                          "uploads_in_timed_calls": r["runs"]["1"]["uploads"] + r["runs"]["10"]["uploads"],
+                          "gbs_inner1": r4(b / (c1m * 1e-3) / 1e9),
+                          "gbs_inner%d" % im: r4(im * b / (cim * 1e-3) / 1e9),
+                          "uploads_in_timed_calls": r["runs"]["1"]["uploads"] + r["runs"][str(im)]["uploads"],
                           "rows_checked": N // 4, "within_csr_bound": _row_check(rp, col, val, x, y, N // 4)}
             log("addon c2a:", out["c2a"])
     finally:
