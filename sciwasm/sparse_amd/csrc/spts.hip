@@ -1050,6 +1050,9 @@ static int capture_levels(sable_spts_s* a, bool use32) {
     const int k = use32 ? 1 : 0;
     if (a->exec[k] && a->graph_lanes[k] == a->lanes) return SABLE_OK;
     if (a->exec[k]) {
+        // a replay of the old graph may still be queued on the stream (the
+        // lanes were changed without a wait): let it finish first
+        (void)hipStreamSynchronize(a->ctx->stream);
         (void)hipGraphExecDestroy(a->exec[k]);
         a->exec[k] = nullptr;
     }

@@ -351,3 +351,25 @@ def test_indegree_syncfree_bit_exact(ctx, lanes, dtype, name):
         an.solve_indegree(bv, x)
         an.status()
         assert np.array_equal(x.array().view(np.uint8), want.view(np.uint8)), name
+
+
+def test_graph_lane_changes_without_wait(ctx):
+    """Lanes changed between queued graph solves, with no host wait between
+    them: each change re-captures the level graph, and the old graph's
+    queued replay still finishes first; every solve gives the reference
+    level solve's bits."""
+    rp, c, v = gen.random_lower(40000, 16, seed=11)
+    N = len(rp) - 1
+    b = np.random.default_rng(12).uniform(-1, 1, N)
+    an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v), lanes=1)
+    an.set_schedule("graph")
+    bv = sp.vec_from_host(b)
+    seq = [1, 4, 8, 16, 4, 1]
+    xs = [sp.create_vec(N, 0.0, np.float64) for _ in seq]
+    ctx.sync()
+    for lanes, x in zip(seq, xs):
+        an.set_lanes(lanes)
+        an.solve_level(bv, x)
+    want = host_level_solution(N, rp, c, v, b)
+    for lanes, x in zip(seq, xs):
+        assert np.array_equal(x.array(), want), lanes
