@@ -74,8 +74,8 @@ MEASURED_TRAFFIC = {
         "source": "sciwasm/sparse_amd/csrc/spmv_tiles.hip",
         "sha256": "4ff0891a1cd5dcb1da0a76a17d5129b803968cf4a21ed5f716ec961321da4714",
         "limiter": "L2 request rate: TCC busy 97% of the kernel's cycles at 0.47 requests per cycle per "
-                   "channel; ~52M x-line requests (every CU touches most of x's lines: 805k entries over "
-                   "262k 128-byte lines) beside ~39M streaming requests and 2.1M x prefetches; fetched "
+                   "channel; ~71M x-line requests (every CU touches most of x's lines: 805k entries over "
+                   "262k 128-byte lines) beside ~19M streaming requests and 2.1M x prefetches; fetched "
                    "traffic 1.12x the algorithmic bytes"},
 }
 
