@@ -734,6 +734,8 @@ def sec_spts(args, sp, ctx, T):
         timed("fp32_refine1", lambda: an.solve_refined(bv, xv, iters=1, inner="level"))
         timed("fp32_refine2", lambda: an.solve_refined(bv, xv, iters=2, inner="level"))
         timed("fp32_refine1_syncfree", lambda: an.solve_refined(bv, xv, iters=1, inner="syncfree"))
+        # the FP32 solve and its refinement step in one sweep of the row blocks
+        timed("fp32_refine1_fused", lambda: an.solve_refined(bv, xv, iters=1, inner="fused"))
         if not args.no_spts_graph:
             an.set_schedule("graph")
             timed("level_graph", lambda: an.solve_level(bv, xv))
@@ -748,6 +750,7 @@ def sec_spts(args, sp, ctx, T):
             timed("level_tuned", lambda: an.solve_level(bv, xv))
             res["level_tuned"]["bit_exact_vs_oracle_level"] = bool(np.array_equal(xv.array(), want))
             res["level_tuned"]["choice"] = tuned
+            timed("fp32_refine1_fused_tuned", lambda: an.solve_refined(bv, xv, iters=1, inner="fused"))
             an.set_lanes(lanes0)
             an.set_schedule("blocks")
         else:
@@ -755,6 +758,7 @@ def sec_spts(args, sp, ctx, T):
             an.set_schedule("launches")
             timed("level_launches", lambda: an.solve_level(bv, xv))
             res["level_launches"]["bit_exact_vs_oracle_level"] = bool(np.array_equal(xv.array(), want))
+            timed("fp32_refine1_fused_launches", lambda: an.solve_refined(bv, xv, iters=1, inner="fused"))
             an.set_schedule("blocks")
         t0 = time.perf_counter()
         oracle.spts_syncfree(N, rp, col, val, b)

@@ -712,8 +712,13 @@ SABLE_API int sable_spts_status(sable_ctx_t* ctx, sable_spts_t* a);
 
 /* FP32 solve plus `iters` steps of FP64 iterative refinement on an FP64
  * analysis: r = b - L.x in FP64, L32.d = r in FP32 (inner 0 = level-set,
- * 1 = sync-free), x += d in FP64.  No reference counterpart (the
- * reference has no mixed-precision solve). */
+ * 1 = sync-free), x += d in FP64.  inner 2 (iters must be 1): the FP32
+ * level solve and its one refinement step fused into one sweep of the
+ * analysis's schedule (row blocks, level graph or per-level launches), each
+ * row forming x0 = L32 solve of (float)b, r = (float)(b - L.x0) with the
+ * FP64 products chained by fma in stored order (diagonal last), d = L32
+ * solve of r, and x = x0 + d: the same x on every schedule.  No reference
+ * counterpart (the reference has no mixed-precision solve). */
 SABLE_API int sable_spts_solve_refined(sable_ctx_t* ctx, sable_spts_t* a, const double* b, double* x, int iters,
                                        int inner);
 

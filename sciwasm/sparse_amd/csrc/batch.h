@@ -35,7 +35,9 @@ struct LevelPlan {
 int batch_level_launch(hipStream_t st, sable_dtype_t dtype, int lanes, const sable_batch_spts_item_t* items,
                        const RowChunk* d_level_chunks, const LevelPlan& plan);
 // FP32 level-set solves with `iters` FP64 residual + update steps; the
-// residual and update run over d_row_chunks (natural row order).
+// residual and update run over d_row_chunks (natural row order).  One step
+// (iters = 1) is fused into one sweep of the level chunks: every row forms
+// its FP32 x0, FP64 residual, FP32 correction and x in the same launch.
 int batch_refine_level_launch(hipStream_t st, int lanes, const sable_batch_refine_item_t* items,
                               sable_batch_spts_item_t* spts_view, int nitems, int iters,
                               const RowChunk* d_level_chunks, const LevelPlan& plan, const RowChunk* d_row_chunks,

@@ -18,6 +18,7 @@
 // residual and update kernels.
 #include "common.h"
 #include "batch.h"
+#include "refine.h"
 
 struct sable_batch_level_plan_s {
     sable_ctx_t* ctx = nullptr;
@@ -805,11 +806,44 @@ int batch_refine_launch(hipStream_t st, int num_cu, int lanes, const sable_batch
     return SABLE_OK;
 }
 
+// One level of the batched FP32 solve with one FP64 refinement step fused
+// into the same sweep (refine.h), one workgroup per chunk of the level's
+// rows: the (x0, d) pair of row i is kept at float offset 2i of the item's
+// work space from r32 (r32 and d32 must then be one region of 2N floats, as
+// the server lays them out), x goes to x64.
+template <int G>
+__global__ __launch_bounds__(BB) void k_batch_level_ref(const sable_batch_refine_item_t* __restrict__ items,
+                                                        const RowChunk* __restrict__ chunks) {
+    const RowChunk c = chunks[blockIdx.x];
+    const int g = threadIdx.x / G;
+    const bool valid = g < c.count;
+    const sable_batch_refine_item_t& it = items[c.item];
+    const int32_t i = valid ? (c.rows ? c.rows[g] : c.first + g) : 0;
+    ref_row<G>(i, valid, it.row_ptr, it.col, it.val64, it.val32, it.b64, (float2*)it.r32, it.x64);
+}
+
 int batch_refine_level_launch(hipStream_t st, int lanes, const sable_batch_refine_item_t* items,
                               sable_batch_spts_item_t* spts_view, int nitems, int iters,
                               const RowChunk* d_level_chunks, const LevelPlan& plan, const RowChunk* d_row_chunks,
                               int nrow_chunks) {
     if (nitems <= 0 || nrow_chunks <= 0) return SABLE_OK;
+    if (iters == 1) {
+        // one sweep: each row's FP32 solve, residual, correction and update
+        for (int lv = 0; lv < plan.levels(); lv++) {
+            const int nch = plan.chunk_lo[lv + 1] - plan.chunk_lo[lv];
+            if (nch <= 0) continue;
+            const RowChunk* ch = d_level_chunks + plan.chunk_lo[lv];
+            switch (lanes) {
+                case 1: hipLaunchKernelGGL((k_batch_level_ref<1>), dim3((unsigned)nch), dim3(BB), 0, st, items, ch); break;
+                case 4: hipLaunchKernelGGL((k_batch_level_ref<4>), dim3((unsigned)nch), dim3(BB), 0, st, items, ch); break;
+                case 8: hipLaunchKernelGGL((k_batch_level_ref<8>), dim3((unsigned)nch), dim3(BB), 0, st, items, ch); break;
+                case 16: hipLaunchKernelGGL((k_batch_level_ref<16>), dim3((unsigned)nch), dim3(BB), 0, st, items, ch); break;
+                default: hipLaunchKernelGGL((k_batch_level_ref<64>), dim3((unsigned)nch), dim3(BB), 0, st, items, ch); break;
+            }
+        }
+        SABLE_LAUNCH_CHECK();
+        return SABLE_OK;
+    }
     hipLaunchKernelGGL(k_refine_to_spts, dim3(1), dim3(BB), 0, st, items, spts_view, nitems);
     for (int it = 0; it <= iters; it++) {
         const int first = it == 0;

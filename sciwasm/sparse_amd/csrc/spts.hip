@@ -25,6 +25,7 @@
 // * FP32 solve + FP64 iterative refinement: r = b - L.x in FP64 from the
 //   FP64 values, correction solved in FP32, x += d in FP64.
 #include "common.h"
+#include "refine.h"
 #include "prims.h"
 
 #include <algorithm>
@@ -58,10 +59,11 @@ struct sable_spts_s {
     void* d_work = nullptr;      // r32, d32 for refinement
     unsigned* d_ticket = nullptr;
     int* d_err = nullptr;
-    // captured level sweeps: [0] analysis dtype, [1] FP32 copy (refinement)
-    int graph_lanes[2] = {0, 0};
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    // captured level sweeps: [0] analysis dtype, [1] FP32 copy (refinement),
+    // [2] the refined solve fused into one sweep
+    int graph_lanes[3] = {0, 0, 0};
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
     // CSR copy owned by the analysis (built from a CSC input), else null
     void* owned[3] = {nullptr, nullptr, nullptr};
     // Row-block schedule (k_blocked): rows cut into blocks of bl_rows; in a
@@ -647,6 +649,214 @@ __global__ void k_add_f32(int32_t N, const float* __restrict__ d, double* __rest
         x[i] += (double)d[i];
 }
 
+// ------------------------------------------------- fused refinement
+// The FP32 solve and one FP64 refinement step in one sweep.  Row i needs
+// only rows of earlier levels, for all three of its passes:
+//   x0_i = ((float)b_i - (0 + w_0 x0_c0 + w_1 x0_c1 + ...)) / w_d      FP32, w = (float)L
+//   s    = fma(L_d, x0_i, fma(... fma(L_0, x0_c0, 0)))                  FP64, stored order
+//   r_i  = (float)(b_i - s)
+//   d_i  = (r_i - (0 + w_0 d_c0 + ...)) / w_d                           FP32
+//   x_i  = (double)x0_i + (double)d_i
+// so the sweep that forms x0_i can form r_i and d_i right after it: the
+// refined solve costs one latency-bound sweep instead of two sweeps, a
+// residual and an update.  x0 is the FP32 level solve's x bit for bit; every
+// fused schedule gives the same x.
+
+template <int G>
+__global__ __launch_bounds__(TB) void k_level_ref(int32_t r0, int32_t nrows, const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ col, const double* __restrict__ v64,
+                                                  const float* __restrict__ v32, const double* __restrict__ b,
+                                                  float2* XD, double* __restrict__ xo) {
+    const int64_t g = ((int64_t)blockIdx.x * TB + threadIdx.x) / G;
+    const bool valid = g < nrows;
+    ref_row<G>(r0 + (int32_t)(valid ? g : 0), valid, rp, col, v64, v32, b, XD, xo);
+}
+
+template <int G>
+__global__ __launch_bounds__(RUN_TB) void k_level_ref_run(LevelRun run, const int32_t* __restrict__ rp,
+                                                          const int32_t* __restrict__ col,
+                                                          const double* __restrict__ v64,
+                                                          const float* __restrict__ v32, const double* __restrict__ b,
+                                                          float2* XD, double* __restrict__ xo) {
+    constexpr int RPP = RUN_TB / G;
+    const int g = threadIdx.x / G;
+    for (int k = 0; k < run.n; k++) {
+        const int32_t r1 = run.lp[k + 1];
+        for (int32_t base = run.lp[k]; base < r1; base += RPP) {
+            const int32_t i = base + g;
+            ref_row<G>(i < r1 ? i : base, i < r1, rp, col, v64, v32, b, XD, xo);
+        }
+        __syncthreads();
+    }
+}
+
+// Row blocks (k_blocked's schedule and waits) for the fused refined solve:
+// a row publishes x0 and d together, as one 64-bit word (x0 in the low
+// half), in W, which holds the word sentinel until then; the block's words
+// sit in LDS.  xo is the FP64 result.
+constexpr uint64_t WSENT = ((uint64_t)SENT32 << 32) | (uint64_t)SENT32;
+
+__device__ __forceinline__ uint64_t pack_x0_d(float x0, float d) {
+    return (uint64_t)__float_as_uint(x0) | ((uint64_t)__float_as_uint(d) << 32);
+}
+__device__ __forceinline__ float word_x0(uint64_t w) { return __uint_as_float((uint32_t)w); }
+__device__ __forceinline__ float word_d(uint64_t w) { return __uint_as_float((uint32_t)(w >> 32)); }
+
+template <int P>
+struct RfRow {
+    int32_t rid;
+    int32_t c[P];
+    float w[P];
+    double v[P];
+    float w_d;
+    double v_d;
+};
+
+template <int P>
+__device__ __forceinline__ void rf_load(RfRow<P>& r, const int4 st, const int32_t* __restrict__ srow,
+                                        const double* __restrict__ diag64, const float* __restrict__ diag32,
+                                        const int32_t* __restrict__ scol, const double* __restrict__ sval64,
+                                        const float* __restrict__ sval32) {
+    const int t = threadIdx.x;
+    r.rid = -1;
+    if (t < st.y) {
+        const int32_t k = st.x + t;
+        r.rid = srow[k];
+        r.v_d = diag64[k];
+        r.w_d = diag32[k];
+#pragma unroll
+        for (int j = 0; j < P; j++) {
+            const bool in = j < st.z;
+            const int64_t e = (int64_t)st.w + (int64_t)j * st.y + t;
+            r.c[j] = in ? scol[e] : -1;
+            r.w[j] = in ? sval32[e] : 0.f;
+            r.v[j] = in ? sval64[e] : 0.0;
+        }
+    }
+}
+
+// The words of one group of P entries, as bl_fetch reads x values.
+template <int P>
+__device__ __forceinline__ void rf_fetch(const int32_t (&c)[P], uint64_t (&wv)[P], const uint64_t* wl, int32_t r0,
+                                         uint64_t* W, int* err, long long spin) {
+    uint32_t pend = 0;
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+        if (c[j] >= r0) {
+            wv[j] = wl[c[j] - r0];
+        } else if (c[j] >= 0) {
+            wv[j] = load_cached(&W[c[j]]);
+            pend |= 1u << j;
+        } else {
+            wv[j] = 0;
+        }
+    }
+    long long n = 0;
+    while (true) {
+#pragma unroll
+        for (int j = 0; j < P; j++)
+            if (((pend >> j) & 1u) && wv[j] != WSENT) pend &= ~(1u << j);
+        if (!pend) break;
+        if (++n > spin) {
+            atomicOr(err, 4);
+            const uint32_t q = __float_as_uint(__builtin_nanf(""));
+#pragma unroll
+            for (int j = 0; j < P; j++)
+                if ((pend >> j) & 1u) wv[j] = ((uint64_t)q << 32) | q;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int j = 0; j < P; j++)
+            if ((pend >> j) & 1u) wv[j] = load_agent(&W[c[j]]);
+    }
+}
+
+template <int P>
+__device__ __forceinline__ void rf_acc(const int32_t (&c)[P], const float (&w)[P], const double (&v)[P],
+                                       const uint64_t (&wv)[P], float& s0, float& s2, double& s) {
+#pragma unroll
+    for (int j = 0; j < P; j++)
+        if (c[j] >= 0) {
+            const float x0 = word_x0(wv[j]);
+            s0 = add_rn(s0, mul_rn(w[j], x0));
+            s2 = add_rn(s2, mul_rn(w[j], word_d(wv[j])));
+            s = fma(v[j], (double)x0, s);
+        }
+}
+
+template <int P>
+__global__ __launch_bounds__(SB) void k_blocked_ref(int32_t nblk, int32_t B, const int32_t* __restrict__ blkstep,
+                                                    const int4* __restrict__ steps, const int32_t* __restrict__ srow,
+                                                    const double* __restrict__ diag64,
+                                                    const float* __restrict__ diag32,
+                                                    const int32_t* __restrict__ scol,
+                                                    const double* __restrict__ sval64,
+                                                    const float* __restrict__ sval32, const double* __restrict__ b,
+                                                    uint64_t* W, double* __restrict__ xo,
+                                                    unsigned* __restrict__ ticket, int* __restrict__ err,
+                                                    long long spin) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    uint64_t* wl = reinterpret_cast<uint64_t*>(smem_raw);
+    __shared__ unsigned s_blk;
+    while (true) {
+        __syncthreads();  // the previous block's wl and s_blk are no longer read
+        if (threadIdx.x == 0) s_blk = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const unsigned blk = s_blk;
+        if (blk >= (unsigned)nblk) break;
+        const int32_t r0 = (int32_t)blk * B;
+        const int32_t s0i = blkstep[blk], s1i = blkstep[blk + 1];
+        RfRow<P> cur, nxt;
+        int4 st = s0i < s1i ? steps[s0i] : make_int4(0, 0, 0, 0);
+        if (s0i < s1i) rf_load(cur, st, srow, diag64, diag32, scol, sval64, sval32);
+        for (int32_t q = s0i; q < s1i; q++) {
+            const int4 stn = q + 1 < s1i ? steps[q + 1] : make_int4(0, 0, 0, 0);
+            if (q + 1 < s1i) rf_load(nxt, stn, srow, diag64, diag32, scol, sval64, sval32);
+            if (cur.rid >= 0) {
+                const double bi = b[cur.rid];
+                uint64_t wv[P];
+                rf_fetch(cur.c, wv, wl, r0, W, err, spin);
+                float s0 = 0.f, s2 = 0.f;
+                double s = 0.0;
+                rf_acc(cur.c, cur.w, cur.v, wv, s0, s2, s);
+                // entries past the first P, P at a time
+                for (int32_t j0 = P; j0 < st.z; j0 += P) {
+                    int32_t c[P];
+                    float w[P];
+                    double v[P];
+#pragma unroll
+                    for (int j = 0; j < P; j++) {
+                        const bool in = j0 + j < st.z;
+                        const int64_t e = (int64_t)st.w + (int64_t)(j0 + j) * st.y + threadIdx.x;
+                        c[j] = in ? scol[e] : -1;
+                        w[j] = in ? sval32[e] : 0.f;
+                        v[j] = in ? sval64[e] : 0.0;
+                    }
+                    rf_fetch(c, wv, wl, r0, W, err, spin);
+                    rf_acc(c, w, v, wv, s0, s2, s);
+                }
+                const float x0 = div_rn(sub_rn((float)bi, s0), cur.w_d);
+                s = fma(cur.v_d, (double)x0, s);
+                const float d = div_rn(sub_rn((float)(bi - s), s2), cur.w_d);
+                const uint64_t word = pack_x0_d(x0, d);
+                wl[cur.rid - r0] = word;
+                store_agent(&W[cur.rid], word);
+                xo[cur.rid] = (double)x0 + (double)d;
+            }
+            __syncthreads();
+            cur = nxt;
+            st = stn;
+        }
+    }
+}
+
+__global__ void k_fill_words(int32_t N, uint64_t* __restrict__ W, unsigned* __restrict__ ticket) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        W[i] = WSENT;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ticket = 0u;
+}
+
 // ------------------------------------------------------------------ host
 static unsigned gsz(int64_t n) {
     unsigned g = grid_for(n, TB);
@@ -714,6 +924,57 @@ static int levels_on_stream(hipStream_t st, int G, const std::vector<int32_t>& l
         } else {
             flush();
             launch_level<T>(st, G, r0, nr, rp, col, val, bl, yl);
+        }
+    }
+    flush();
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
+// The fused refined solve over level-ordered buffers, one launch per wide
+// level and one workgroup per run of narrow levels (as levels_on_stream).
+template <int G>
+static void launch_level_ref(hipStream_t st, int32_t r0, int32_t nr, const LevelRun* run, const int32_t* rp,
+                             const int32_t* col, const double* v64, const float* v32, const double* bl, float2* XD,
+                             double* yl) {
+    if (run)
+        hipLaunchKernelGGL((k_level_ref_run<G>), dim3(1), dim3(RUN_TB), 0, st, *run, rp, col, v64, v32, bl, XD, yl);
+    else
+        hipLaunchKernelGGL((k_level_ref<G>), dim3((unsigned)(((int64_t)nr * G + TB - 1) / TB)), dim3(TB), 0, st, r0,
+                           nr, rp, col, v64, v32, bl, XD, yl);
+}
+
+static int levels_ref_on_stream(hipStream_t st, int G, const std::vector<int32_t>& lp, const int32_t* rp,
+                                const int32_t* col, const double* v64, const float* v32, const double* bl, float2* XD,
+                                double* yl) {
+    const int nl = (int)lp.size() - 1;
+    const int32_t narrow = 2 * (RUN_TB / G);
+    LevelRun run;
+    run.n = 0;
+    auto go = [&](int32_t r0, int32_t nr, const LevelRun* rr) {
+        switch (G) {
+            case 1: launch_level_ref<1>(st, r0, nr, rr, rp, col, v64, v32, bl, XD, yl); break;
+            case 4: launch_level_ref<4>(st, r0, nr, rr, rp, col, v64, v32, bl, XD, yl); break;
+            case 8: launch_level_ref<8>(st, r0, nr, rr, rp, col, v64, v32, bl, XD, yl); break;
+            case 16: launch_level_ref<16>(st, r0, nr, rr, rp, col, v64, v32, bl, XD, yl); break;
+            default: launch_level_ref<64>(st, r0, nr, rr, rp, col, v64, v32, bl, XD, yl); break;
+        }
+    };
+    auto flush = [&]() {
+        if (run.n == 1) go(run.lp[0], run.lp[1] - run.lp[0], nullptr);
+        else if (run.n > 1) go(0, 0, &run);
+        run.n = 0;
+    };
+    for (int l = 0; l < nl; l++) {
+        const int32_t r0 = lp[l], nr = lp[l + 1] - lp[l];
+        if (nr <= 0) continue;
+        if (nr <= narrow) {
+            if (run.n == 0) run.lp[0] = r0;
+            run.lp[++run.n] = r0 + nr;
+            if (run.n == RUN_MAX) flush();
+        } else {
+            flush();
+            go(r0, nr, nullptr);
         }
     }
     flush();
@@ -963,6 +1224,35 @@ static int launch_blocked(hipStream_t st, int num_cu, sable_spts_s* a, const T* 
     return SABLE_OK;
 }
 
+// The fused refined solve over the row-block schedule: b and x natural
+// order (FP64), the row words in the analysis's work space.
+static int launch_blocked_ref(hipStream_t st, int num_cu, sable_spts_s* a, const double* b, double* x) {
+    uint64_t* W = (uint64_t*)a->d_work;
+    hipLaunchKernelGGL(k_fill_words, dim3(gsz(a->N)), dim3(TB), 0, st, a->N, W, a->d_ticket);
+    const size_t shm = (size_t)a->bl_rows * sizeof(uint64_t);
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(a->bl_nblk, (int64_t)num_cu * 2));
+    constexpr long long BL_SPIN = 1ll << 20;
+#define BLR_LAUNCH(PP)                                                                                          \
+    do {                                                                                                        \
+        if (shm > 65536 - 1024)                                                                                 \
+            SABLE_HIP(hipFuncSetAttribute((const void*)k_blocked_ref<PP>,                                       \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));               \
+        hipLaunchKernelGGL((k_blocked_ref<PP>), dim3((unsigned)grid), dim3(SB), shm, st, a->bl_nblk, a->bl_rows,  \
+                           a->d_bl_blkstep, a->d_bl_steps, a->d_bl_row, (const double*)a->d_bl_diag,             \
+                           a->d_bl_diag32, a->d_bl_col, (const double*)a->d_bl_val, a->d_bl_val32, b, W, x,      \
+                           a->d_ticket, a->d_err, BL_SPIN);                                                      \
+    } while (0)
+    if (a->bl_width <= 4)
+        BLR_LAUNCH(4);
+    else if (a->bl_width <= 8)
+        BLR_LAUNCH(8);
+    else
+        BLR_LAUNCH(16);
+#undef BLR_LAUNCH
+    SABLE_LAUNCH_CHECK();
+    return SABLE_OK;
+}
+
 // The dependents of every row (the strictly lower part by column, rows
 // ascending) and the in-degrees, from the host copy of the structure.
 static int ensure_indeg(sable_spts_s* a) {
@@ -1033,7 +1323,7 @@ static void free_analysis(sable_spts_s* a) {
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
     }
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < 3; k++) {
         if (a->exec[k]) (void)hipGraphExecDestroy(a->exec[k]);
         if (a->graph[k]) (void)hipGraphDestroy(a->graph[k]);
     }
@@ -1045,9 +1335,12 @@ static void free_analysis(sable_spts_s* a) {
     delete a;
 }
 
-template <typename T>
-static int capture_levels(sable_spts_s* a, bool use32) {
-    const int k = use32 ? 1 : 0;
+// Captures body(stream) into graph slot k of the analysis, once per lane
+// count.  The capture stream lives until the graph is instantiated, the
+// capture is always ended (also when a launch inside it failed), and a
+// failed capture leaves no graph behind.
+template <typename F>
+static int capture_graph(sable_spts_s* a, int k, F&& body) {
     if (a->exec[k] && a->graph_lanes[k] == a->lanes) return SABLE_OK;
     if (a->exec[k]) {
         // a replay of the old graph may still be queued on the stream (the
@@ -1060,9 +1353,6 @@ static int capture_levels(sable_spts_s* a, bool use32) {
         (void)hipGraphDestroy(a->graph[k]);
         a->graph[k] = nullptr;
     }
-    // The capture stream lives until the graph is instantiated, the capture
-    // is always ended (also when a launch inside it failed), and a failed
-    // capture leaves no graph behind.
     hipStream_t cs;
     SABLE_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
@@ -1070,9 +1360,7 @@ static int capture_levels(sable_spts_s* a, bool use32) {
         (void)hipStreamDestroy(cs);
         return fail(SABLE_EHIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
     }
-    const T* val = use32 ? (const T*)a->d_lval32 : (const T*)a->d_val;
-    const int lrc =
-        levels_on_stream<T>(cs, a->lanes, a->h_level_ptr, a->d_rp, a->d_col, val, (const T*)a->d_bl, (T*)a->d_yl);
+    const int lrc = body(cs);
     e = hipStreamEndCapture(cs, &a->graph[k]);
     if (e == hipSuccess && lrc == SABLE_OK) e = hipGraphInstantiate(&a->exec[k], a->graph[k], nullptr, nullptr, 0);
     (void)hipStreamDestroy(cs);
@@ -1085,6 +1373,15 @@ static int capture_levels(sable_spts_s* a, bool use32) {
     }
     a->graph_lanes[k] = a->lanes;
     return SABLE_OK;
+}
+
+template <typename T>
+static int capture_levels(sable_spts_s* a, bool use32) {
+    const T* val = use32 ? (const T*)a->d_lval32 : (const T*)a->d_val;
+    return capture_graph(a, use32 ? 1 : 0, [&](hipStream_t cs) {
+        return levels_on_stream<T>(cs, a->lanes, a->h_level_ptr, a->d_rp, a->d_col, val, (const T*)a->d_bl,
+                                   (T*)a->d_yl);
+    });
 }
 
 // Level solve on level-ordered buffers bl -> yl: graph replay, or the
@@ -1504,11 +1801,38 @@ SABLE_API int sable_spts_status(sable_ctx_t* ctx, sable_spts_t* a) {
 SABLE_API int sable_spts_solve_refined(sable_ctx_t* ctx, sable_spts_t* a, const double* b, double* x, int iters,
                                        int inner) {
     SABLE_CHECK(ctx && a && b && x, SABLE_EINVAL, "NULL argument");
-    SABLE_CHECK(iters >= 0 && (inner == 0 || inner == 1), SABLE_EINVAL, "bad iters/inner");
+    SABLE_CHECK(iters >= 0 && (inner == 0 || inner == 1 || inner == 2), SABLE_EINVAL, "bad iters/inner");
+    SABLE_CHECK(inner != 2 || iters == 1, SABLE_EINVAL, "the fused refined solve takes one refinement step");
     int rc = ensure_f32(a);
     if (rc) return rc;
     hipStream_t st = ctx->stream;
     const int32_t N = a->N;
+    if (inner == 2) {
+        // FP32 solve, FP64 residual, FP32 correction and FP64 update in one
+        // sweep of the analysis's schedule
+        if (a->schedule == SABLE_SPTS_BLOCKS) {
+            rc = ensure_bl32(a);
+            if (rc) return rc;
+            return launch_blocked_ref(st, ctx->num_cu, a, b, x);
+        }
+        float2* XD = (float2*)a->d_work;  // N (x0, d) pairs in the 2N floats of work space
+        hipLaunchKernelGGL(k_gather_perm<double>, dim3(gsz(N)), dim3(TB), 0, st, N, a->d_perm, b, (double*)a->d_bl);
+        auto body = [&](hipStream_t s2) {
+            return levels_ref_on_stream(s2, a->lanes, a->h_level_ptr, a->d_rp, a->d_col, (const double*)a->d_val,
+                                        a->d_lval32, (const double*)a->d_bl, XD, (double*)a->d_yl);
+        };
+        if (a->schedule == SABLE_SPTS_LEVEL_LAUNCHES) {
+            rc = body(st);
+        } else {
+            rc = capture_graph(a, 2, body);
+            if (!rc) SABLE_HIP(hipGraphLaunch(a->exec[2], st));
+        }
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_scatter_perm<double>, dim3(gsz(N)), dim3(TB), 0, st, N, a->d_perm,
+                           (const double*)a->d_yl, x);
+        SABLE_LAUNCH_CHECK();
+        return SABLE_OK;
+    }
     float* r32 = (float*)a->d_work;
     float* d32 = r32 + ((N + 63) / 64) * 64;
     const int G = a->lanes;

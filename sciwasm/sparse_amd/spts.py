@@ -101,9 +101,14 @@ class SptsAnalysis:
         level-ordered matrix; equal to solve_level bit for bit."""
         call("sable_spts_solve_combined", self.ctx.handle, self.handle, b.vec.ptr, x.vec.ptr)
 
+    REFINE_INNER = {"level": 0, "syncfree": 1, "fused": 2}
+
     def solve_refined(self, b, x, iters=2, inner="syncfree"):
+        """FP32 solve + `iters` FP64 refinement steps; inner "level" or
+        "syncfree" (separate sweeps), or "fused" (iters = 1: the solve and
+        its refinement step in one sweep of the schedule)."""
         call("sable_spts_solve_refined", self.ctx.handle, self.handle, b.vec.ptr, x.vec.ptr, int(iters),
-             1 if inner == "syncfree" else 0)
+             self.REFINE_INNER[inner])
 
     def status(self):
         call("sable_spts_status", self.ctx.handle, self.handle)

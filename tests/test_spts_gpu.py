@@ -373,3 +373,68 @@ def test_graph_lane_changes_without_wait(ctx):
     want = host_level_solution(N, rp, c, v, b)
     for lanes, x in zip(seq, xs):
         assert np.array_equal(x.array(), want), lanes
+
+
+def host_fused_refine(N, rp, c, v, b):
+    """The fused refined solve computed exactly on the host: x0 the FP32
+    reference level solve of (float)b; r = (float)(b - s) with s the FP64
+    products of the row chained by fused multiply-add in stored order
+    (diagonal last; each fma rounded once, computed with exact rationals);
+    d the FP32 level solve of r; x = x0 + d in FP64."""
+    from fractions import Fraction
+    v32 = v.astype(np.float32)
+    x0 = host_level_solution(N, rp, c, v32, b.astype(np.float32))
+    r = np.empty(N, np.float32)
+    for i in range(N):
+        s = 0.0
+        for j in range(rp[i], rp[i + 1]):
+            s = float(Fraction(float(v[j])) * Fraction(float(x0[c[j]])) + Fraction(s))
+        r[i] = np.float32(b[i] - s)
+    d = host_level_solution(N, rp, c, v32, r)
+    return x0.astype(np.float64) + d.astype(np.float64)
+
+
+@pytest.mark.parametrize("name", ["lap3d_16", "random_lower_sparse", "random_lower_20k"])
+def test_fused_refinement_every_schedule(ctx, name):
+    """FP32 solve + one FP64 refinement step in one sweep: the same x on the
+    row blocks (two block sizes), the level graph and the per-level launches
+    at 1, 4 and 16 lanes, equal to the exact host computation of the same
+    steps, within 1e-12 of the FP64 oracle, and no slower to converge than
+    the unfused refinement."""
+    rp, c, v = MATS[name]()
+    N = len(rp) - 1
+    b = np.random.default_rng(21).uniform(-1, 1, N)
+    an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v))
+    bv = sp.vec_from_host(b)
+    want = host_fused_refine(N, rp, c, v, b)
+    runs = []
+    for sched, arg in (("blocks", 1024), ("blocks", 4096), ("graph", 1), ("graph", 4), ("graph", 16),
+                       ("launches", 4)):
+        if sched == "blocks":
+            an.set_schedule("blocks", arg)
+        else:
+            an.set_lanes(arg)
+            an.set_schedule(sched)
+        for _ in range(2):
+            x = sp.create_vec(N, 0.0, np.float64)
+            an.solve_refined(bv, x, iters=1, inner="fused")
+            an.status()
+            runs.append(((sched, arg), x.array()))
+    for key, xa in runs:
+        assert np.array_equal(xa, want), key
+    assert oracle.rel_residual(N, rp, c, v, want, b) <= 1e-12
+    x = sp.create_vec(N, 0.0, np.float64)
+    an.solve_refined(bv, x, iters=1, inner="level")
+    an.status()
+    assert oracle.rel_residual(N, rp, c, v, x.array(), b) <= 1e-12
+
+
+def test_fused_refinement_needs_one_step(ctx):
+    rp, c, v = MATS["lap3d_16"]()
+    N = len(rp) - 1
+    an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v))
+    bv = sp.vec_from_host(np.ones(N))
+    for iters in (0, 2):
+        with pytest.raises(SableError) as e:
+            an.solve_refined(bv, sp.create_vec(N, 0.0, np.float64), iters=iters, inner="fused")
+        assert e.value.name == "SABLE_EINVAL"
