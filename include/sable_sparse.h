@@ -849,7 +849,8 @@ typedef struct {
     int max_batch;        /* requests per batch (0 = 256) */
     int slots;            /* batches in flight (0 = 3) */
     int64_t arena_bytes;  /* device vector arena per slot (0 = 256 MiB) */
-    int refine_iters;     /* refinement steps in mode 2 */
+    int refine_iters;     /* refinement steps in mode 2 (< 0: 1); one step runs fused
+                             into the level sweep of the solve */
     int spts_lanes;       /* lanes per row of the solves (0 = 4) */
     int group_rhs;        /* modes 1 and 2: SpMV requests on one matrix in a batch share
                              one multi-RHS product, up to this many (0 = 8, 1 = off) */

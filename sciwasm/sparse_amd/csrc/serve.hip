@@ -17,7 +17,8 @@
 //                     matrix is read once for all of them;
 //   mode 2 (mixed):   batched, SpMV with FP32 values/vectors and FP32
 //                     accumulation; solves in FP32 with FP64 residual and
-//                     update for `refine_iters` refinement steps.
+//                     update for `refine_iters` refinement steps (one
+//                     step, the default, is fused into the solve's sweep).
 // Batched solves (modes 1 and 2) run level by level (spts_sweep 0): every
 // lower matrix carries its level schedule from registration, and launch l
 // of a batch solves level l of all of the batch's solves at once (a batch
@@ -746,7 +747,7 @@ SABLE_API int sable_server_create(sable_ctx_t* ctx, const sable_server_config_t*
     if (s->cfg.max_batch <= 0) s->cfg.max_batch = 256;
     if (s->cfg.slots <= 0) s->cfg.slots = 3;
     if (s->cfg.arena_bytes <= 0) s->cfg.arena_bytes = 256ll << 20;
-    if (s->cfg.refine_iters < 0) s->cfg.refine_iters = 2;
+    if (s->cfg.refine_iters < 0) s->cfg.refine_iters = 1;
     if (s->cfg.max_batch > 4096) s->cfg.max_batch = 4096;
     if (s->cfg.group_rhs <= 0) s->cfg.group_rhs = 8;
     if (s->cfg.group_rhs > 8) s->cfg.group_rhs = 8;

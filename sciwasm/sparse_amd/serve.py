@@ -213,7 +213,7 @@ class Server:
 
     SWEEPS = {"levels": 0, "syncfree": 1}
 
-    def __init__(self, mode="mixed", hbm_budget=0, max_batch=256, slots=3, arena_bytes=256 << 20, refine_iters=2,
+    def __init__(self, mode="mixed", hbm_budget=0, max_batch=256, slots=3, arena_bytes=256 << 20, refine_iters=1,
                  spts_lanes=4, group_rhs=8, batch_tiles=False, spts_sweep="levels", ctx=None):
         self.ctx = ctx if ctx is not None else Context.default()
         cfg = _lib.ServerConfig(int(hbm_budget), MODES[mode], int(max_batch), int(slots), int(arena_bytes),

@@ -14,6 +14,8 @@ import oracle
 import sciwasm.sparse_amd as sp
 from sciwasm.sparse_amd import gen, SableError
 
+from test_spts_gpu import host_fused_refine
+
 pytestmark = pytest.mark.gpu
 
 
@@ -51,14 +53,16 @@ def level_solution(n, rp, c, v, b):
 
 
 @pytest.mark.parametrize("mode,kw", [("plain", {}), ("batched", {}), ("batched", {"group_rhs": 1}), ("mixed", {}),
-                                     ("mixed", {"refine_iters": 1}), ("batched", {"spts_sweep": "syncfree"}),
+                                     ("mixed", {"refine_iters": 2}), ("batched", {"spts_sweep": "syncfree"}),
                                      ("mixed", {"spts_sweep": "syncfree"})],
-                         ids=["plain", "batched", "batched_ungrouped", "mixed", "mixed_refine1", "batched_sf",
+                         ids=["plain", "batched", "batched_ungrouped", "mixed", "mixed_refine2", "batched_sf",
                               "mixed_sf"])
 def test_alone_equals_mix_and_accuracy(ctx, pool, mode, kw):
     """Also: the FP64 solves are a reference solve's bits, the sync-free
     solve's (plain; batched one-launch sweep) or the level solve's (batched
-    level sweep)."""
+    level sweep); the mixed solves with one refinement step (the default,
+    fused into the level sweep) are the exact host computation of those
+    steps."""
     mats, lows = pool
     S, gids, lids = make_server(mode, pool, **kw)
     rng = np.random.default_rng(5)
@@ -100,6 +104,8 @@ def test_alone_equals_mix_and_accuracy(ctx, pool, mode, kw):
                 assert np.array_equal(out, oracle.spts_syncfree(n, rp, c, v, vec)), (mode, p)
             elif mode == "batched":
                 assert np.array_equal(out, level_solution(n, rp, c, v, vec)), (mode, p)
+            elif not kw:
+                assert np.array_equal(out, host_fused_refine(n, rp, c, v, vec)), (mode, p)
     for i in ids:
         S.wait(i)
     S.close()
