@@ -921,6 +921,7 @@ def make_lower_pool(pool, count):
 
 
 def sec_batch_spts(args, sp, ctx, T, lows):
+    import oracle
     Ls, bs, xs = [], [], []
     for n, lrp, lc, lv in lows:
         Ls.append(sp.csr_from_host(n, lrp, lc, lv, ctx))
@@ -955,6 +956,23 @@ def sec_batch_spts(args, sp, ctx, T, lows):
             out["batched_level_bit_exact_vs_oracle_first16"] = all(
                 np.array_equal(x.numpy(), oracle_level(lo, 0.5)) for x, lo in zip(xs[:16], lows[:16]))
         del plan, run
+    # the same level schedule in FP32 (values rounded), and FP32 with one FP64
+    # refinement step fused into its sweep, at the fastest lane count
+    best_lanes = min((1, 4, 8), key=lambda g: out["level_plan_lanes%d" % g]["ms"])
+    plan = sp.BatchLevelPlan(Ls, lanes=best_lanes)
+    L32 = [sp.csr_from_host(n, lrp, lc, lv.astype(np.float32), ctx) for n, lrp, lc, lv in lows]
+    plan32 = sp.BatchLevelPlan(L32, lanes=best_lanes)
+    b32 = [sp.DeviceArray.from_host(np.full(n, 0.5, np.float32), ctx=ctx) for n, _, _, _ in lows]
+    x32 = [sp.DeviceArray(n, np.float32, ctx) for n, _, _, _ in lows]
+    run32 = plan32.bind(b32, x32)
+    out["batched_level_fp32_ms"] = r4(T(run32.run, 5, 1))
+    runr = plan.bind_refined(bs, xs, vals32=[m.val for m in L32])
+    out["batched_level_fp32_refine1_fused_ms"] = r4(T(runr.run, 5, 1))
+    out["batched_level_fp32_refine1_fused_max_rel_residual_first16"] = max(
+        oracle.rel_residual(n, lrp, lc, lv, x.numpy(), np.full(n, 0.5)) for x, (n, lrp, lc, lv) in
+        zip(xs[:16], lows[:16]))
+    out["batched_level_precision_lanes"] = best_lanes
+    del plan, plan32, run32, runr, L32, b32, x32
     # the per-matrix baseline over every matrix: each analysis's own sync-free
     # solve (row blocks: a sentinel fill and one solve launch per matrix) and
     # the natural-order sync-free kernel (the reference's per-matrix unit)

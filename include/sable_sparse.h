@@ -830,6 +830,16 @@ SABLE_API void sable_batch_level_plan_destroy(sable_batch_level_plan_t* p);
 SABLE_API int sable_batch_level_plan_info(const sable_batch_level_plan_t* p, int32_t* levels, int64_t* chunks);
 SABLE_API int sable_batch_level_plan_solve(sable_ctx_t* ctx, const sable_batch_level_plan_t* p, sable_dtype_t dtype,
                                            const sable_batch_spts_item_t* d_items, int nitems);
+/* The FP32 solve of every item with one FP64 refinement step fused into
+ * the same level sweep (one launch per level of the deepest matrix): per
+ * row x0 = L32 solve of (float)b, r = (float)(b - L.x0) (FP64 products
+ * chained by fma in stored order, diagonal last), d = L32 solve of r, x64 =
+ * x0 + d, as sable_spts_solve_refined does with inner 2.  Items (device
+ * memory, the plan's matrices in its order) give val64 and val32 = (float)
+ * val64; r32 must hold 2N floats of work space (d32 is not used).  No
+ * reference counterpart. */
+SABLE_API int sable_batch_level_plan_solve_refined(sable_ctx_t* ctx, const sable_batch_level_plan_t* p,
+                                                   const sable_batch_refine_item_t* d_items, int nitems);
 
 /* FP32 sync-free solve + `iters` FP64 refinement steps for every item
  * (synchronises). */

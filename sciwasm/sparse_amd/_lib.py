@@ -177,6 +177,7 @@ SIGNATURES = {
     "sable_batch_level_plan_destroy": [c_vp],
     "sable_batch_level_plan_info": [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i64)],
     "sable_batch_level_plan_solve": [c_vp, c_vp, c_int, c_vp, c_int],
+    "sable_batch_level_plan_solve_refined": [c_vp, c_vp, c_vp, c_int],
     "sable_spts_solve_combined": [c_vp, c_vp, c_vp, c_vp],
     "sable_spts_status": [c_vp, c_vp],
     "sable_spts_solve_refined": [c_vp, c_vp, c_vp, c_vp, c_int, c_int],

@@ -826,7 +826,7 @@ int batch_refine_level_launch(hipStream_t st, int lanes, const sable_batch_refin
                               sable_batch_spts_item_t* spts_view, int nitems, int iters,
                               const RowChunk* d_level_chunks, const LevelPlan& plan, const RowChunk* d_row_chunks,
                               int nrow_chunks) {
-    if (nitems <= 0 || nrow_chunks <= 0) return SABLE_OK;
+    if (nitems <= 0) return SABLE_OK;
     if (iters == 1) {
         // one sweep: each row's FP32 solve, residual, correction and update
         for (int lv = 0; lv < plan.levels(); lv++) {
@@ -844,6 +844,7 @@ int batch_refine_level_launch(hipStream_t st, int lanes, const sable_batch_refin
         SABLE_LAUNCH_CHECK();
         return SABLE_OK;
     }
+    if (nrow_chunks <= 0) return SABLE_OK;
     hipLaunchKernelGGL(k_refine_to_spts, dim3(1), dim3(BB), 0, st, items, spts_view, nitems);
     for (int it = 0; it <= iters; it++) {
         const int first = it == 0;
@@ -1046,6 +1047,18 @@ SABLE_API int sable_batch_level_plan_solve(sable_ctx_t* ctx, const sable_batch_l
     SABLE_CHECK(dtype == SABLE_F32 || dtype == SABLE_F64, SABLE_EINVAL, "bad dtype");
     if (p->nchunks == 0) return SABLE_OK;
     return batch_level_launch(ctx->stream, dtype, p->lanes, d_items, p->d_chunks, p->plan);
+}
+
+SABLE_API int sable_batch_level_plan_solve_refined(sable_ctx_t* ctx, const sable_batch_level_plan_t* p,
+                                                   const sable_batch_refine_item_t* d_items, int nitems) {
+    SABLE_CHECK(ctx && p && (d_items || nitems == 0), SABLE_EINVAL, "NULL argument");
+    SABLE_CHECK(ctx == p->ctx, SABLE_EINVAL, "plan belongs to another context");
+    SABLE_CHECK(nitems == p->nitems, SABLE_EINVAL, "%d items for a plan of %d", nitems, p->nitems);
+    if (p->nchunks == 0) return SABLE_OK;
+    // one refinement step: the fused sweep (no residual or update launches,
+    // so no spts view or row chunks are needed)
+    return batch_refine_level_launch(ctx->stream, p->lanes, d_items, nullptr, nitems, 1, p->d_chunks, p->plan,
+                                     nullptr, 0);
 }
 
 }  // extern "C"

@@ -180,6 +180,29 @@ class BatchLevelPlan:
     def solve(self, bs, xs):
         self.bind(bs, xs).run()
 
+    def bind_refined(self, bs, xs, vals32=None):
+        """The FP32 solve with one FP64 refinement step fused into the same
+        level sweep, for FP64 matrices: x_k (FP64) from b_k (FP64).  vals32[k]
+        (optional) is matrix k's values rounded to FP32; the work space is
+        allocated here and kept by the returned runner."""
+        items, keep, total = [], [], 0
+        for k, (L, b, x) in enumerate(zip(self.mats, bs, xs)):
+            v32 = vals32[k] if vals32 is not None else DeviceArray.from_host(
+                L.val.numpy().astype(np.float32), ctx=self.ctx)
+            w = DeviceArray(2 * L.N, np.float32, self.ctx)
+            keep += [v32, w]
+            items.append(BatchRefineItem(L.row.ptr, L.col.ptr, L.val.ptr, v32.ptr, b.ptr, x.ptr, w.ptr, None, total,
+                                         L.N, 0))
+            total += L.N
+        d = _upload_items(items, self.ctx)
+
+        class _Run:
+            def run(_self):
+                call("sable_batch_level_plan_solve_refined", self.ctx.handle, self.handle, d.ptr, len(items))
+        r = _Run()
+        r.keep = (d, keep)
+        return r
+
     def __del__(self):
         try:
             if self.handle:

@@ -369,6 +369,32 @@ def test_batch_level_plan(ctx, mats, lanes, dtype):
             assert np.array_equal(xs[k].numpy(), level_solution(n, rp, c, v, bh[k])), (rep, k)
 
 
+@pytest.mark.parametrize("lanes", [1, 4, 16])
+def test_batch_level_plan_refined(ctx, mats, lanes):
+    """The batched level schedule with the FP32 solve and one FP64
+    refinement step fused into its sweep: each item's x is the exact host
+    computation of those steps (the single-matrix fused solve's x; checked on
+    a sample of items) and every item's is within 1e-12 of the FP64 oracle,
+    twice with other right-hand sides."""
+    from test_spts_gpu import host_fused_refine
+    Ls = lower_set(mats) + [(3, np.arange(4, dtype=np.int32), np.arange(3, dtype=np.int32),
+                             np.array([2.0, 4.0, 8.0]))]
+    dL = [sp.csr_from_host(n, rp, c, v) for n, rp, c, v in Ls]
+    plan = sp.BatchLevelPlan(dL, lanes=lanes)
+    rng = np.random.default_rng(23)
+    for rep in range(2):
+        bh = [rng.uniform(-1, 1, n) for n, _, _, _ in Ls]
+        bs = [DeviceArray.from_host(b) for b in bh]
+        xs = [DeviceArray.zeros(n, np.float64) for n, _, _, _ in Ls]
+        plan.bind_refined(bs, xs).run()
+        ctx.sync()
+        for k, (n, rp, c, v) in enumerate(Ls):
+            x = xs[k].numpy()
+            if k < 6 or k == len(Ls) - 1:  # the exact host steps are slow: a sample
+                assert np.array_equal(x, host_fused_refine(n, rp, c, v, bh[k])), (rep, k)
+            assert oracle.rel_residual(n, rp, c, v, x, bh[k]) <= 1e-12, (rep, k)
+
+
 def test_batch_level_plan_rejects_malformed(ctx):
     rp = np.array([0, 1, 3], np.int32)
     c = np.array([0, 1, 0], np.int32)  # row 1 ends with column 0: the diagonal is not last
