@@ -668,7 +668,7 @@ SABLE_API int sable_spts_set_schedule(sable_spts_t* a, int schedule, int32_t blo
 SABLE_API int sable_spts_schedule_info(const sable_spts_t* a, int* schedule, int32_t* block_rows,
                                        int32_t* nblocks, int32_t* nsteps, int64_t* slots);
 /* Times the level-set solve under row blocks of 1024, 2048, ... 16384 rows
- * (up to twice N) and under the level graph at 1, 4, 8 and 16 lanes per
+ * (up to twice N) and under the level graph at 1, 4, 8, 16 and 64 lanes per
  * row, on b = L*1 in buffers of its own, and keeps the fastest schedule and
  * lane count (the row-block schedule stays built at its fastest size either
  * way).  The solution depends on neither, bit for bit.  Outputs may be NULL.
@@ -834,10 +834,10 @@ SABLE_API int sable_batch_level_plan_solve(sable_ctx_t* ctx, const sable_batch_l
  * the same level sweep (one launch per level of the deepest matrix): per
  * row x0 = L32 solve of (float)b, r = (float)(b - L.x0) (FP64 products
  * chained by fma in stored order, diagonal last), d = L32 solve of r, x64 =
- * x0 + d, as sable_spts_solve_refined does with inner 2.  Items (device
- * memory, the plan's matrices in its order) give val64 and val32 = (float)
- * val64; r32 must hold 2N floats of work space (d32 is not used).  No
- * reference counterpart. */
+ * x0 + d, as sable_spts_solve_refined does with inner 2 (the FP32 values
+ * are the FP64 values rounded as they are read).  Items (device memory, the
+ * plan's matrices in its order) give val64; r32 must hold 2N floats of work
+ * space; val32 and d32 are not read.  No reference counterpart. */
 SABLE_API int sable_batch_level_plan_solve_refined(sable_ctx_t* ctx, const sable_batch_level_plan_t* p,
                                                    const sable_batch_refine_item_t* d_items, int nitems);
 

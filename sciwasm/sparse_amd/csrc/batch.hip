@@ -612,7 +612,8 @@ static unsigned gsz(int64_t n) {
 // then every lane of the group adds them in order), x[i] = (b[i] - t) /
 // diag: the level-set solve's bits at every G.
 template <typename T, int G>
-__device__ __forceinline__ void level_chunk(const sable_batch_spts_item_t* __restrict__ items, const RowChunk& c) {
+__device__ __forceinline__ void level_chunk(const sable_batch_spts_item_t* __restrict__ items, const RowChunk& c,
+                                            T* stage) {
     const int g = threadIdx.x / G;
     const int l = threadIdx.x & (G - 1);
     const bool valid = g < c.count;
@@ -635,7 +636,7 @@ __device__ __forceinline__ void level_chunk(const sable_batch_spts_item_t* __res
             s = add_rn(s, p);
         } else {
             const int n = min(G, e - j0);
-            s = group_ordered_sum<false, G>(s, p, n);
+            s = group_ordered_sum_lds<false, G>(s, p, n, stage);
         }
     }
     if (valid && l == 0) {
@@ -647,7 +648,8 @@ __device__ __forceinline__ void level_chunk(const sable_batch_spts_item_t* __res
 template <typename T, int G>
 __global__ __launch_bounds__(BB) void k_batch_level(const sable_batch_spts_item_t* __restrict__ items,
                                                     const RowChunk* __restrict__ chunks) {
-    level_chunk<T, G>(items, chunks[blockIdx.x]);
+    __shared__ T stage[BB];
+    level_chunk<T, G>(items, chunks[blockIdx.x], stage);
 }
 
 int batch_level_launch(hipStream_t st, sable_dtype_t dtype, int lanes, const sable_batch_spts_item_t* items,
@@ -814,12 +816,13 @@ int batch_refine_launch(hipStream_t st, int num_cu, int lanes, const sable_batch
 template <int G>
 __global__ __launch_bounds__(BB) void k_batch_level_ref(const sable_batch_refine_item_t* __restrict__ items,
                                                         const RowChunk* __restrict__ chunks) {
+    __shared__ RefEntry stage[BB];
     const RowChunk c = chunks[blockIdx.x];
     const int g = threadIdx.x / G;
     const bool valid = g < c.count;
     const sable_batch_refine_item_t& it = items[c.item];
     const int32_t i = valid ? (c.rows ? c.rows[g] : c.first + g) : 0;
-    ref_row<G>(i, valid, it.row_ptr, it.col, it.val64, it.val32, it.b64, (float2*)it.r32, it.x64);
+    ref_row<G>(i, valid, it.row_ptr, it.col, it.val64, it.b64, (float2*)it.r32, it.x64, stage);
 }
 
 int batch_refine_level_launch(hipStream_t st, int lanes, const sable_batch_refine_item_t* items,

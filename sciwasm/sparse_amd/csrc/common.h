@@ -150,6 +150,27 @@ __device__ __forceinline__ T group_ordered_sum(T s, T p, int n) {
     return s;
 }
 
+// The same ordered sum for groups of 16 or more lanes through LDS: each
+// lane stages its value in stage[threadIdx.x] (one T per thread of the
+// workgroup), then every lane of the group reads the group's n values in
+// lane order (the group's lanes read one address at a time, a broadcast)
+// and adds them itself: the serial sum's bits, with one LDS read per value
+// instead of one cross-lane read per value and 32 bits.  Smaller groups
+// take the cross-lane reads, which measured faster there.
+template <bool SUB, int G, typename T>
+__device__ __forceinline__ T group_ordered_sum_lds(T s, T p, int n, T* stage) {
+    if constexpr (G <= 8) return group_ordered_sum<SUB, G>(s, p, n);
+    T* grp = stage + (threadIdx.x - (threadIdx.x & (G - 1)));
+    stage[threadIdx.x] = p;
+    __builtin_amdgcn_wave_barrier();
+    for (int q = 0; q < n; q++) {
+        const T v = grp[q];
+        s = SUB ? sub_rn(s, v) : add_rn(s, v);
+    }
+    __builtin_amdgcn_wave_barrier();
+    return s;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

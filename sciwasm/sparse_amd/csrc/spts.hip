@@ -131,6 +131,7 @@ template <typename T, int G>
 __global__ __launch_bounds__(TB) void k_level(int32_t r0, int32_t nrows, const int32_t* __restrict__ rp,
                                               const int32_t* __restrict__ col, const T* __restrict__ val,
                                               const T* __restrict__ bl, T* __restrict__ yl) {
+    __shared__ T stage[TB];
     const int64_t gt = (int64_t)blockIdx.x * TB + threadIdx.x;
     const int64_t g = gt / G;
     const int l = threadIdx.x & (G - 1);
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(TB) void k_level(int32_t r0, int32_t nrows, const i
             s = add_rn(s, p);
         } else {
             const int n = min(G, e - j0);
-            s = group_ordered_sum<false, G>(s, p, n);
+            s = group_ordered_sum_lds<false, G>(s, p, n, stage);
         }
     }
     if (valid && l == 0) yl[i] = div_rn(sub_rn(bl[i], s), val[e]);
@@ -171,6 +172,7 @@ template <typename T, int G>
 __global__ __launch_bounds__(RUN_TB) void k_level_run(LevelRun run, const int32_t* __restrict__ rp,
                                                       const int32_t* __restrict__ col, const T* __restrict__ val,
                                                       const T* __restrict__ bl, T* yl) {
+    __shared__ T stage[RUN_TB];
     constexpr int RPP = RUN_TB / G;
     const int g = threadIdx.x / G;
     const int l = threadIdx.x & (G - 1);
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(RUN_TB) void k_level_run(LevelRun run, const int32_
                     s = add_rn(s, p);
                 } else {
                     const int n = min(G, e - j0);
-                    s = group_ordered_sum<false, G>(s, p, n);
+                    s = group_ordered_sum_lds<false, G>(s, p, n, stage);
                 }
             }
             if (valid && l == 0) yl[i] = div_rn(sub_rn(bl[i], s), val[e]);
@@ -665,26 +667,28 @@ __global__ void k_add_f32(int32_t N, const float* __restrict__ d, double* __rest
 template <int G>
 __global__ __launch_bounds__(TB) void k_level_ref(int32_t r0, int32_t nrows, const int32_t* __restrict__ rp,
                                                   const int32_t* __restrict__ col, const double* __restrict__ v64,
-                                                  const float* __restrict__ v32, const double* __restrict__ b,
-                                                  float2* XD, double* __restrict__ xo) {
+                                                  const double* __restrict__ b, float2* XD,
+                                                  double* __restrict__ xo) {
+    __shared__ RefEntry stage[TB];
     const int64_t g = ((int64_t)blockIdx.x * TB + threadIdx.x) / G;
     const bool valid = g < nrows;
-    ref_row<G>(r0 + (int32_t)(valid ? g : 0), valid, rp, col, v64, v32, b, XD, xo);
+    ref_row<G>(r0 + (int32_t)(valid ? g : 0), valid, rp, col, v64, b, XD, xo, stage);
 }
 
 template <int G>
 __global__ __launch_bounds__(RUN_TB) void k_level_ref_run(LevelRun run, const int32_t* __restrict__ rp,
                                                           const int32_t* __restrict__ col,
                                                           const double* __restrict__ v64,
-                                                          const float* __restrict__ v32, const double* __restrict__ b,
-                                                          float2* XD, double* __restrict__ xo) {
+                                                          const double* __restrict__ b, float2* XD,
+                                                          double* __restrict__ xo) {
+    __shared__ RefEntry stage[RUN_TB];
     constexpr int RPP = RUN_TB / G;
     const int g = threadIdx.x / G;
     for (int k = 0; k < run.n; k++) {
         const int32_t r1 = run.lp[k + 1];
         for (int32_t base = run.lp[k]; base < r1; base += RPP) {
             const int32_t i = base + g;
-            ref_row<G>(i < r1 ? i : base, i < r1, rp, col, v64, v32, b, XD, xo);
+            ref_row<G>(i < r1 ? i : base, i < r1, rp, col, v64, b, XD, xo, stage);
         }
         __syncthreads();
     }
@@ -706,30 +710,25 @@ template <int P>
 struct RfRow {
     int32_t rid;
     int32_t c[P];
-    float w[P];
     double v[P];
-    float w_d;
     double v_d;
 };
 
 template <int P>
 __device__ __forceinline__ void rf_load(RfRow<P>& r, const int4 st, const int32_t* __restrict__ srow,
-                                        const double* __restrict__ diag64, const float* __restrict__ diag32,
-                                        const int32_t* __restrict__ scol, const double* __restrict__ sval64,
-                                        const float* __restrict__ sval32) {
+                                        const double* __restrict__ diag64, const int32_t* __restrict__ scol,
+                                        const double* __restrict__ sval64) {
     const int t = threadIdx.x;
     r.rid = -1;
     if (t < st.y) {
         const int32_t k = st.x + t;
         r.rid = srow[k];
         r.v_d = diag64[k];
-        r.w_d = diag32[k];
 #pragma unroll
         for (int j = 0; j < P; j++) {
             const bool in = j < st.z;
             const int64_t e = (int64_t)st.w + (int64_t)j * st.y + t;
             r.c[j] = in ? scol[e] : -1;
-            r.w[j] = in ? sval32[e] : 0.f;
             r.v[j] = in ? sval64[e] : 0.0;
         }
     }
@@ -772,15 +771,17 @@ __device__ __forceinline__ void rf_fetch(const int32_t (&c)[P], uint64_t (&wv)[P
     }
 }
 
+// The FP32 value of an entry is its FP64 value rounded.
 template <int P>
-__device__ __forceinline__ void rf_acc(const int32_t (&c)[P], const float (&w)[P], const double (&v)[P],
-                                       const uint64_t (&wv)[P], float& s0, float& s2, double& s) {
+__device__ __forceinline__ void rf_acc(const int32_t (&c)[P], const double (&v)[P], const uint64_t (&wv)[P],
+                                       float& s0, float& s2, double& s) {
 #pragma unroll
     for (int j = 0; j < P; j++)
         if (c[j] >= 0) {
+            const float w = (float)v[j];
             const float x0 = word_x0(wv[j]);
-            s0 = add_rn(s0, mul_rn(w[j], x0));
-            s2 = add_rn(s2, mul_rn(w[j], word_d(wv[j])));
+            s0 = add_rn(s0, mul_rn(w, x0));
+            s2 = add_rn(s2, mul_rn(w, word_d(wv[j])));
             s = fma(v[j], (double)x0, s);
         }
 }
@@ -789,10 +790,8 @@ template <int P>
 __global__ __launch_bounds__(SB) void k_blocked_ref(int32_t nblk, int32_t B, const int32_t* __restrict__ blkstep,
                                                     const int4* __restrict__ steps, const int32_t* __restrict__ srow,
                                                     const double* __restrict__ diag64,
-                                                    const float* __restrict__ diag32,
                                                     const int32_t* __restrict__ scol,
-                                                    const double* __restrict__ sval64,
-                                                    const float* __restrict__ sval32, const double* __restrict__ b,
+                                                    const double* __restrict__ sval64, const double* __restrict__ b,
                                                     uint64_t* W, double* __restrict__ xo,
                                                     unsigned* __restrict__ ticket, int* __restrict__ err,
                                                     long long spin) {
@@ -809,36 +808,35 @@ __global__ __launch_bounds__(SB) void k_blocked_ref(int32_t nblk, int32_t B, con
         const int32_t s0i = blkstep[blk], s1i = blkstep[blk + 1];
         RfRow<P> cur, nxt;
         int4 st = s0i < s1i ? steps[s0i] : make_int4(0, 0, 0, 0);
-        if (s0i < s1i) rf_load(cur, st, srow, diag64, diag32, scol, sval64, sval32);
+        if (s0i < s1i) rf_load(cur, st, srow, diag64, scol, sval64);
         for (int32_t q = s0i; q < s1i; q++) {
             const int4 stn = q + 1 < s1i ? steps[q + 1] : make_int4(0, 0, 0, 0);
-            if (q + 1 < s1i) rf_load(nxt, stn, srow, diag64, diag32, scol, sval64, sval32);
+            if (q + 1 < s1i) rf_load(nxt, stn, srow, diag64, scol, sval64);
             if (cur.rid >= 0) {
                 const double bi = b[cur.rid];
                 uint64_t wv[P];
                 rf_fetch(cur.c, wv, wl, r0, W, err, spin);
                 float s0 = 0.f, s2 = 0.f;
                 double s = 0.0;
-                rf_acc(cur.c, cur.w, cur.v, wv, s0, s2, s);
+                rf_acc(cur.c, cur.v, wv, s0, s2, s);
                 // entries past the first P, P at a time
                 for (int32_t j0 = P; j0 < st.z; j0 += P) {
                     int32_t c[P];
-                    float w[P];
                     double v[P];
 #pragma unroll
                     for (int j = 0; j < P; j++) {
                         const bool in = j0 + j < st.z;
                         const int64_t e = (int64_t)st.w + (int64_t)(j0 + j) * st.y + threadIdx.x;
                         c[j] = in ? scol[e] : -1;
-                        w[j] = in ? sval32[e] : 0.f;
                         v[j] = in ? sval64[e] : 0.0;
                     }
                     rf_fetch(c, wv, wl, r0, W, err, spin);
-                    rf_acc(c, w, v, wv, s0, s2, s);
+                    rf_acc(c, v, wv, s0, s2, s);
                 }
-                const float x0 = div_rn(sub_rn((float)bi, s0), cur.w_d);
+                const float wd = (float)cur.v_d;
+                const float x0 = div_rn(sub_rn((float)bi, s0), wd);
                 s = fma(cur.v_d, (double)x0, s);
-                const float d = div_rn(sub_rn((float)(bi - s), s2), cur.w_d);
+                const float d = div_rn(sub_rn((float)(bi - s), s2), wd);
                 const uint64_t word = pack_x0_d(x0, d);
                 wl[cur.rid - r0] = word;
                 store_agent(&W[cur.rid], word);
@@ -935,29 +933,27 @@ static int levels_on_stream(hipStream_t st, int G, const std::vector<int32_t>& l
 // level and one workgroup per run of narrow levels (as levels_on_stream).
 template <int G>
 static void launch_level_ref(hipStream_t st, int32_t r0, int32_t nr, const LevelRun* run, const int32_t* rp,
-                             const int32_t* col, const double* v64, const float* v32, const double* bl, float2* XD,
-                             double* yl) {
+                             const int32_t* col, const double* v64, const double* bl, float2* XD, double* yl) {
     if (run)
-        hipLaunchKernelGGL((k_level_ref_run<G>), dim3(1), dim3(RUN_TB), 0, st, *run, rp, col, v64, v32, bl, XD, yl);
+        hipLaunchKernelGGL((k_level_ref_run<G>), dim3(1), dim3(RUN_TB), 0, st, *run, rp, col, v64, bl, XD, yl);
     else
         hipLaunchKernelGGL((k_level_ref<G>), dim3((unsigned)(((int64_t)nr * G + TB - 1) / TB)), dim3(TB), 0, st, r0,
-                           nr, rp, col, v64, v32, bl, XD, yl);
+                           nr, rp, col, v64, bl, XD, yl);
 }
 
 static int levels_ref_on_stream(hipStream_t st, int G, const std::vector<int32_t>& lp, const int32_t* rp,
-                                const int32_t* col, const double* v64, const float* v32, const double* bl, float2* XD,
-                                double* yl) {
+                                const int32_t* col, const double* v64, const double* bl, float2* XD, double* yl) {
     const int nl = (int)lp.size() - 1;
     const int32_t narrow = 2 * (RUN_TB / G);
     LevelRun run;
     run.n = 0;
     auto go = [&](int32_t r0, int32_t nr, const LevelRun* rr) {
         switch (G) {
-            case 1: launch_level_ref<1>(st, r0, nr, rr, rp, col, v64, v32, bl, XD, yl); break;
-            case 4: launch_level_ref<4>(st, r0, nr, rr, rp, col, v64, v32, bl, XD, yl); break;
-            case 8: launch_level_ref<8>(st, r0, nr, rr, rp, col, v64, v32, bl, XD, yl); break;
-            case 16: launch_level_ref<16>(st, r0, nr, rr, rp, col, v64, v32, bl, XD, yl); break;
-            default: launch_level_ref<64>(st, r0, nr, rr, rp, col, v64, v32, bl, XD, yl); break;
+            case 1: launch_level_ref<1>(st, r0, nr, rr, rp, col, v64, bl, XD, yl); break;
+            case 4: launch_level_ref<4>(st, r0, nr, rr, rp, col, v64, bl, XD, yl); break;
+            case 8: launch_level_ref<8>(st, r0, nr, rr, rp, col, v64, bl, XD, yl); break;
+            case 16: launch_level_ref<16>(st, r0, nr, rr, rp, col, v64, bl, XD, yl); break;
+            default: launch_level_ref<64>(st, r0, nr, rr, rp, col, v64, bl, XD, yl); break;
         }
     };
     auto flush = [&]() {
@@ -1238,9 +1234,8 @@ static int launch_blocked_ref(hipStream_t st, int num_cu, sable_spts_s* a, const
             SABLE_HIP(hipFuncSetAttribute((const void*)k_blocked_ref<PP>,                                       \
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));               \
         hipLaunchKernelGGL((k_blocked_ref<PP>), dim3((unsigned)grid), dim3(SB), shm, st, a->bl_nblk, a->bl_rows,  \
-                           a->d_bl_blkstep, a->d_bl_steps, a->d_bl_row, (const double*)a->d_bl_diag,             \
-                           a->d_bl_diag32, a->d_bl_col, (const double*)a->d_bl_val, a->d_bl_val32, b, W, x,      \
-                           a->d_ticket, a->d_err, BL_SPIN);                                                      \
+                           a->d_bl_blkstep, a->d_bl_steps, a->d_bl_row, (const double*)a->d_bl_diag, a->d_bl_col, \
+                           (const double*)a->d_bl_val, b, W, x, a->d_ticket, a->d_err, BL_SPIN);                  \
     } while (0)
     if (a->bl_width <= 4)
         BLR_LAUNCH(4);
@@ -1560,7 +1555,7 @@ SABLE_API int sable_spts_schedule_info(const sable_spts_t* a, int* schedule, int
 }
 
 // Times solve_level under each schedule (row blocks of 1024 to 16384 rows,
-// the level graph at 1, 4, 8 and 16 lanes per row) on b = L*1 and keeps the
+// the level graph at 1, 4, 8, 16 and 64 lanes per row) on b = L*1 and keeps the
 // fastest. Every schedule and lane count gives the same x bit for bit, so
 // the choice changes only the time.
 SABLE_API int sable_spts_tune(sable_ctx_t* ctx, sable_spts_t* a, int* schedule, int32_t* block_rows, int* lanes,
@@ -1622,7 +1617,7 @@ SABLE_API int sable_spts_tune(sable_ctx_t* ctx, sable_spts_t* a, int* schedule, 
             best_s = SABLE_SPTS_BLOCKS;
         }
     }
-    for (int G : {1, 4, 8, 16}) {
+    for (int G : {1, 4, 8, 16, 64}) {
         if (rc) break;
         a->schedule = SABLE_SPTS_LEVEL_GRAPH;
         a->lanes = G;
@@ -1810,16 +1805,12 @@ SABLE_API int sable_spts_solve_refined(sable_ctx_t* ctx, sable_spts_t* a, const 
     if (inner == 2) {
         // FP32 solve, FP64 residual, FP32 correction and FP64 update in one
         // sweep of the analysis's schedule
-        if (a->schedule == SABLE_SPTS_BLOCKS) {
-            rc = ensure_bl32(a);
-            if (rc) return rc;
-            return launch_blocked_ref(st, ctx->num_cu, a, b, x);
-        }
+        if (a->schedule == SABLE_SPTS_BLOCKS) return launch_blocked_ref(st, ctx->num_cu, a, b, x);
         float2* XD = (float2*)a->d_work;  // N (x0, d) pairs in the 2N floats of work space
         hipLaunchKernelGGL(k_gather_perm<double>, dim3(gsz(N)), dim3(TB), 0, st, N, a->d_perm, b, (double*)a->d_bl);
         auto body = [&](hipStream_t s2) {
             return levels_ref_on_stream(s2, a->lanes, a->h_level_ptr, a->d_rp, a->d_col, (const double*)a->d_val,
-                                        a->d_lval32, (const double*)a->d_bl, XD, (double*)a->d_yl);
+                                        (const double*)a->d_bl, XD, (double*)a->d_yl);
         };
         if (a->schedule == SABLE_SPTS_LEVEL_LAUNCHES) {
             rc = body(st);
