@@ -13,7 +13,9 @@ to about twenty.
 * The lower solves of the 3D Laplacian 128^3 and of the random 1,048,576-row
   lower matrix: the analysis's default and tuned level solve bit-exact
   against the oracle's level solve, the sync-free solve against the
-  oracle's sync-free solve, and the relative residual of both <= 1e-12.
+  oracle's sync-free solve, and the relative residual of both <= 1e-12; the
+  fused FP32 + FP64-refinement solve the same on every schedule, exact on
+  the leading rows and within 1e-12.
 * The 2,048-matrix batch: the batched tile product of every matrix within
   its CSR bound of the oracle, and the batched sync-free solves of all
   2,048 diagonally dominant lower triangles bit-exact against the oracle.
@@ -133,6 +135,39 @@ def test_spts_fullsize_bit_exact(ctx, name):
         assert np.array_equal(x.array(), want_s), (name, label)
     assert oracle.rel_residual(N, rp, c, v, want_l, b) <= 1e-12
     assert oracle.rel_residual(N, rp, c, v, want_s, b) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["lap3d_128", "random_lower_1M"])
+def test_spts_fullsize_fused_refinement(ctx, name):
+    """The FP32 solve with one FP64 refinement step fused into its sweep, at
+    full size: the same x on the row blocks, the level graph at 8 and 64
+    lanes and the tuned schedule, the leading 20,000 rows (a lower matrix's
+    leading block depends on nothing after it) equal to the exact host
+    computation of the steps, and a relative residual <= 1e-12."""
+    from test_spts_gpu import host_fused_refine
+    rp, c, v = gen.laplacian_3d_lower(128) if name == "lap3d_128" else gen.random_lower(1 << 20, 32, seed=7)
+    N = len(rp) - 1
+    b = np.random.default_rng(9).uniform(-1, 1, N)
+    an = sp.SptsAnalysis(sp.csr_from_host(N, rp, c, v))
+    bv = sp.vec_from_host(b)
+    xs = {}
+    for label in ("blocks", "graph8", "graph64", "tuned"):
+        if label.startswith("graph"):
+            an.set_lanes(int(label[5:]))
+            an.set_schedule("graph")
+        elif label == "tuned":
+            an.tune()
+        x = sp.create_vec(N, 0.0, np.float64)
+        an.solve_refined(bv, x, iters=1, inner="fused")
+        an.status()
+        xs[label] = x.array()
+    for label, xa in xs.items():
+        assert np.array_equal(xa, xs["blocks"]), (name, label)
+    K = 20000
+    nk = int(rp[K])
+    want = host_fused_refine(K, rp[:K + 1], c[:nk], v[:nk], b[:K])
+    assert np.array_equal(xs["blocks"][:K], want), name
+    assert oracle.rel_residual(N, rp, c, v, xs["blocks"], b) <= 1e-12
 
 
 def test_batch_fullsize_tiles_and_solves(ctx):
