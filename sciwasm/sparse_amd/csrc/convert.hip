@@ -13,6 +13,7 @@
 //    reference's sequential loop).
 //  * transpose_ell: nonzero slots enumerated row-major (row j, then slot),
 //    compacted with a scan, then stably sorted by column.
+#include <type_traits>
 #include "common.h"
 #include "prims.h"
 
@@ -43,13 +44,37 @@ __global__ void k_heads_from_keys(int64_t n, int32_t N, int shift, const uint64_
     for (int64_t r = lo; r <= hi; r++) rp[r] = (int32_t)k;
 }
 
-template <typename T>
-__global__ void k_gather_coo(int64_t nnz, uint64_t mask, const uint64_t* __restrict__ keys,
-                             const uint32_t* __restrict__ idx, const T* __restrict__ val, int32_t* __restrict__ ocol,
+// The value's bits ride through the sort as its payload (P: an unsigned
+// integer of the value's size), so no gather by input position is needed.
+template <typename T, typename P>
+__global__ void k_coo_keys_val(int64_t nnz, int32_t N, int colbits, const int32_t* __restrict__ row,
+                               const int32_t* __restrict__ col, const T* __restrict__ val,
+                               uint64_t* __restrict__ keys, P* __restrict__ pay, int* __restrict__ err) {
+    static_assert(sizeof(T) == sizeof(P), "payload of the value's size");
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
+        const int r = row[k], c = col[k];
+        if (r < 0 || r >= N || c < 0 || c >= N) atomicOr(err, 1);
+        keys[k] = ((uint64_t)(uint32_t)r << colbits) | (uint32_t)c;
+        pay[k] = __builtin_bit_cast(P, val[k]);
+    }
+}
+
+// Entry k of the sorted keys: its column and value, and the heads of the
+// rows that start at k (thread nnz writes the heads of the trailing empty
+// rows and row_ptr[N]).
+template <typename T, typename P>
+__global__ void k_unpack_coo(int64_t nnz, int32_t N, int colbits, const uint64_t* __restrict__ keys,
+                             const P* __restrict__ pay, int32_t* __restrict__ rp, int32_t* __restrict__ ocol,
                              T* __restrict__ oval) {
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
-        ocol[p] = (int32_t)(keys[p] & mask);
-        oval[p] = val[idx[p]];
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nnz) return;
+    const uint64_t key = k < nnz ? keys[k] : 0;
+    const int64_t lo = k == 0 ? 0 : (int64_t)(keys[k - 1] >> colbits) + 1;
+    const int64_t hi = k == nnz ? N : (int64_t)(key >> colbits);
+    for (int64_t r = lo; r <= hi; r++) rp[r] = (int32_t)k;
+    if (k < nnz) {
+        ocol[k] = (int32_t)(key & (((uint64_t)1 << colbits) - 1));
+        oval[k] = __builtin_bit_cast(T, pay[k]);
     }
 }
 
@@ -321,6 +346,7 @@ static int read_i32(sable_ctx_t* ctx, const int32_t* dev, int count, int32_t* ou
 template <typename T>
 static int coo_csr_t(sable_ctx_t* ctx, int32_t N, int64_t nnz, const int32_t* row, const int32_t* col, const void* val,
                      int32_t* orp, int32_t* ocol, void* oval) {
+    using P = typename std::conditional<sizeof(T) == 8, uint64_t, uint32_t>::type;
     hipStream_t st = ctx->stream;
     if (nnz == 0) {
         hipLaunchKernelGGL(k_fill_t<int32_t>, dim3(gs(N + 1)), dim3(CB), 0, st, orp, (int64_t)N + 1);
@@ -330,28 +356,31 @@ static int coo_csr_t(sable_ctx_t* ctx, int32_t N, int64_t nnz, const int32_t* ro
     SABLE_CHECK(nnz < 0xffffffffLL, SABLE_EINVAL, "nnz too large");
     const int colbits = bits_for(N > 0 ? N - 1 : 0);
     const int rowbits = bits_for(N > 0 ? N - 1 : 0);
-    size_t need = Carver::round(8 * nnz) + Carver::round(4 * nnz) + 256 + sort_tmp_bytes(nnz);
+    size_t need = Carver::round(8 * nnz) + Carver::round(sizeof(P) * nnz) + 256 + sort_tmp_bytes(nnz);
     void* s;
     int rc = ctx_scratch(ctx, need, &s);
     if (rc) return rc;
     Carver cv(s);
     uint64_t* keys = cv.take<uint64_t>(nnz);
-    uint32_t* idx = cv.take<uint32_t>(nnz);
+    P* pay = cv.take<P>(nnz);
     int* err = cv.take<int>(1);
     void* stmp = cv.p + cv.used;
     SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_coo_keys, dim3(gs(nnz)), dim3(CB), 0, st, nnz, N, colbits, row, col, keys, idx, err);
+    hipLaunchKernelGGL((k_coo_keys_val<T, P>), dim3(gs(nnz)), dim3(CB), 0, st, nnz, N, colbits, row, col,
+                       (const T*)val, keys, pay, err);
     SABLE_LAUNCH_CHECK();
     int e = 0;
     rc = read_err(ctx, err, &e);
     if (rc) return rc;
     SABLE_CHECK(!e, SABLE_EMALFORMED, "COO index outside [0, N)");
-    rc = sort_pairs_u64(st, keys, idx, nnz, colbits + rowbits, stmp);
+    // stable: entries of one (row, col) keep their input order
+    if constexpr (sizeof(P) == 8)
+        rc = sort_pairs_u64_u64(st, keys, pay, nnz, colbits + rowbits, stmp);
+    else
+        rc = sort_pairs_u64(st, keys, pay, nnz, colbits + rowbits, stmp);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_heads_from_keys, dim3(grid_for(nnz + 1, CB)), dim3(CB), 0, st, nnz, N, colbits, keys, orp);
-    SABLE_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gather_coo<T>, dim3(gs(nnz)), dim3(CB), 0, st, nnz, ((uint64_t)1 << colbits) - 1, keys, idx,
-                       (const T*)val, ocol, (T*)oval);
+    hipLaunchKernelGGL((k_unpack_coo<T, P>), dim3(grid_for(nnz + 1, CB)), dim3(CB), 0, st, nnz, N, colbits, keys, pay,
+                       orp, ocol, (T*)oval);
     SABLE_LAUNCH_CHECK();
     return SABLE_OK;
 }

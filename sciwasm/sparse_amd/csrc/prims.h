@@ -13,6 +13,8 @@ int scan_exclusive_i32(hipStream_t st, const int32_t* in, int32_t* out, int64_t 
 size_t sort_tmp_bytes(int64_t n);
 // Stable LSD radix sort of (keys, vals) by key bits [0, end_bit), in place.
 int sort_pairs_u64(hipStream_t st, uint64_t* keys, uint32_t* vals, int64_t n, int end_bit, void* tmp);
+// The same with 64-bit values (the same work space size).
+int sort_pairs_u64_u64(hipStream_t st, uint64_t* keys, uint64_t* vals, int64_t n, int end_bit, void* tmp);
 
 // Carves aligned sub-buffers out of one scratch allocation.
 struct Carver {

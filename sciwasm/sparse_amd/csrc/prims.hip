@@ -5,8 +5,9 @@
 // adds the tile offset.  In place is allowed.
 //
 // Radix sort: 8-bit digits, tiles of 4096 keys (4 waves x 1024 contiguous
-// keys).  Per pass: digit histogram per tile (digit-major layout), exclusive
-// scan of the histogram, then a scatter that ranks the tile's keys by digit
+// keys), 32- or 64-bit values.  Per pass: digit histogram per tile
+// (tile-major), the start of every (tile, digit) run from a chunked scan down
+// the digit columns, then a scatter that ranks the tile's keys by digit
 // with wave ballots, reorders them in LDS and writes each digit's run
 // contiguously.  Equal digits keep their input order, so the sort is stable.
 #include "common.h"
@@ -120,9 +121,9 @@ constexpr int RS_WKEYS = WAVE * RS_ROUNDS; // keys per wave (contiguous)
 constexpr int RS_TILE = RS_BS * RS_ROUNDS; // 4096 keys per workgroup
 constexpr int RADIX = 256;
 
-// Per-tile digit histogram, digit-major: counts[d * ntiles + tile].  Each
-// wave counts into its own LDS row so that skewed digits do not serialise
-// on one counter.
+// Per-tile digit histogram, tile-major: counts[tile * RADIX + d], so each
+// workgroup writes one contiguous 1 KiB row.  Each wave counts into its own
+// LDS row so that skewed digits do not serialise on one counter.
 __global__ __launch_bounds__(RS_BS) void k_rs_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
                                                    int32_t* __restrict__ counts, int64_t ntiles) {
     __shared__ int hist[RS_WAVES][RADIX];
@@ -131,16 +132,70 @@ __global__ __launch_bounds__(RS_BS) void k_rs_hist(const uint64_t* __restrict__ 
     for (int k = 0; k < RS_WAVES; k++) hist[k][tid] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-#pragma unroll 4
+    // every load of the tile in flight before the first count
+    uint64_t k[RS_ROUNDS];
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++) {
         const int64_t i = base + r * RS_BS + tid;
-        if (i < n) atomicAdd(&hist[w][(int)((keys[i] >> shift) & (RADIX - 1))], 1);
+        k[r] = i < n ? keys[i] : 0;
     }
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++)
+        if (base + r * RS_BS + tid < n) atomicAdd(&hist[w][(int)((k[r] >> shift) & (RADIX - 1))], 1);
     __syncthreads();
     int t = 0;
 #pragma unroll
     for (int k = 0; k < RS_WAVES; k++) t += hist[k][tid];
-    counts[(int64_t)tid * ntiles + blockIdx.x] = t;
+    counts[(int64_t)blockIdx.x * RADIX + tid] = t;
+}
+
+// The scatter needs, per tile and digit, the global start of the tile's run
+// of that digit: all keys of smaller digits plus the digit's keys in earlier
+// tiles.  Over the tile-major counts that is a scan down each column, done
+// in chunks of RS_CHUNK tiles: column sums per chunk (written digit-major),
+// a flat exclusive scan of those (digit-major order gives exactly the
+// smaller-digit totals plus the earlier chunks), then a walk down each
+// chunk's column that replaces the counts by the starts.
+constexpr int RS_CHUNK = 64;
+
+__global__ __launch_bounds__(RADIX) void k_rs_chunk_sums(const int32_t* __restrict__ counts, int64_t ntiles,
+                                                         int64_t nchunks, int32_t* __restrict__ csum) {
+    const int d = threadIdx.x;
+    const int64_t t0 = (int64_t)blockIdx.x * RS_CHUNK;
+    const int m = (int)((ntiles - t0) < RS_CHUNK ? (ntiles - t0) : RS_CHUNK);
+    int s = 0;
+#pragma unroll 16
+    for (int j = 0; j < m; j++) s += counts[(t0 + j) * RADIX + d];
+    csum[(int64_t)d * nchunks + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RADIX) void k_rs_chunk_starts(int32_t* __restrict__ counts, int64_t ntiles,
+                                                           int64_t nchunks, const int32_t* __restrict__ cstart) {
+    const int d = threadIdx.x;
+    const int64_t t0 = (int64_t)blockIdx.x * RS_CHUNK;
+    const int m = (int)((ntiles - t0) < RS_CHUNK ? (ntiles - t0) : RS_CHUNK);
+    int run = cstart[(int64_t)d * nchunks + blockIdx.x];
+    for (int j0 = 0; j0 < m; j0 += 16) {
+        int c[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) c[j] = j0 + j < m ? counts[(t0 + j0 + j) * RADIX + d] : 0;
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if (j0 + j < m) {
+                counts[(t0 + j0 + j) * RADIX + d] = run;
+                run += c[j];
+            }
+    }
+}
+
+// Workgroup b runs on XCD b % 8.  Each XCD is given a contiguous range of
+// tiles (in launch order), so tiles t and t + 1, whose runs of one digit
+// meet inside a 128-byte line, write that line through the same L2 at
+// about the same time; with tile = blockIdx.x the two halves of most lines
+// came from different XCDs and reached HBM as partial writes.
+__device__ __forceinline__ int64_t rs_tile_of_block(int64_t b, int64_t ntiles) {
+    const int64_t q = ntiles >> 3, rem = ntiles & 7, x = b & 7;
+    return x * q + (x < rem ? x : rem) + (b >> 3);
 }
 
 // One stable scatter pass over a 4096-key tile.
@@ -150,16 +205,20 @@ __global__ __launch_bounds__(RS_BS) void k_rs_hist(const uint64_t* __restrict__ 
 //     that digit (kept in LDS by the group's lowest lane).
 //  2. Tile-local start of each digit (scan over digits) and each wave's
 //     offset inside it give every key its position in the tile sorted by
-//     digit; keys and values are placed there in LDS.
+//     digit; the keys are placed there in LDS.
 //  3. The tile is written out in sorted order: consecutive threads write
 //     consecutive positions of one bucket, so the stores coalesce into runs
-//     instead of one scattered 8-byte store per key.
-__global__ __launch_bounds__(RS_BS) void k_rs_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                      uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                      int64_t n, int shift, const int32_t* __restrict__ offsets,
-                                                      int64_t ntiles) {
-    __shared__ uint64_t sk[RS_TILE];
-    __shared__ uint32_t sv[RS_TILE];
+//     instead of one scattered 8-byte store per key.  The values (32 or
+//     64 bits) then take the same LDS buffer and are written the same way,
+//     so one 32 KiB buffer serves both and four workgroups fit in a CU's
+//     LDS.
+template <typename V>
+__global__ __launch_bounds__(RS_BS, sizeof(V) > 4 ? 3 : 4) void k_rs_scatter(const uint64_t* __restrict__ kin, const V* __restrict__ vin,
+                                                         uint64_t* __restrict__ kout, V* __restrict__ vout,
+                                                         int64_t n, int shift, const int32_t* __restrict__ offsets,
+                                                         int64_t ntiles) {
+    static_assert(sizeof(V) <= sizeof(uint64_t), "values share the key buffer");
+    __shared__ uint64_t sk[RS_TILE];        // keys, then values
     __shared__ int wcnt[RS_WAVES][RADIX];  // running, then exclusive-over-waves counts
     __shared__ int dstart[RADIX];           // tile-local start of each digit
     __shared__ int gbase[RADIX];            // global start of this tile's run of each digit
@@ -167,20 +226,20 @@ __global__ __launch_bounds__(RS_BS) void k_rs_scatter(const uint64_t* __restrict
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
 #pragma unroll
     for (int k = 0; k < RS_WAVES; k++) wcnt[k][tid] = 0;
-    gbase[tid] = offsets[(int64_t)tid * ntiles + blockIdx.x];
+    const int64_t tile = rs_tile_of_block(blockIdx.x, ntiles);
+    gbase[tid] = offsets[tile * RADIX + tid];
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    const int64_t tbase = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t tbase = tile * RS_TILE;
     const int64_t wbase = tbase + (int64_t)w * RS_WKEYS;
     uint64_t key[RS_ROUNDS];
-    uint32_t val[RS_ROUNDS];
-    int rank[RS_ROUNDS];
+    V val[RS_ROUNDS];
+    int pos[RS_ROUNDS];  // rank in the wave's run of the digit, then position in the tile
 #pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++) {
         const int64_t i = wbase + r * WAVE + lane;
         const bool valid = i < n;
         key[r] = valid ? kin[i] : 0;
-        val[r] = valid ? vin[i] : 0;
     }
 #pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++) {
@@ -197,7 +256,7 @@ __global__ __launch_bounds__(RS_BS) void k_rs_scatter(const uint64_t* __restrict
         const int below = __popcll(peers & lt);
         int before = 0;
         if (valid) before = wcnt[w][d];
-        rank[r] = valid ? before + below : -1;
+        pos[r] = valid ? before + below : -1;
         // all lanes have read the running count before the leader bumps it
         __builtin_amdgcn_wave_barrier();
         if (valid && below == 0) wcnt[w][d] = before + __popcll(peers);
@@ -219,67 +278,107 @@ __global__ __launch_bounds__(RS_BS) void k_rs_scatter(const uint64_t* __restrict
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++) {
-        if (rank[r] >= 0) {
+        if (pos[r] >= 0) {
             const int d = (int)((key[r] >> shift) & (RADIX - 1));
-            const int pos = dstart[d] + wcnt[w][d] + rank[r];
-            sk[pos] = key[r];
-            sv[pos] = val[r];
+            pos[r] += dstart[d] + wcnt[w][d];
+            sk[pos[r]] = key[r];
+        }
+    }
+    // the values are loaded only once the keys are placed (fewer live
+    // registers); they arrive while the keys are written
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        const int64_t i = wbase + r * WAVE + lane;
+        val[r] = i < n ? vin[i] : V(0);
+    }
+    __syncthreads();
+    const int tn = (n - tbase) < RS_TILE ? (int)(n - tbase) : RS_TILE;
+    int dst[RS_ROUNDS];  // global position - tile-local position of slot tid + j * RS_BS
+#pragma unroll
+    for (int j = 0; j < RS_ROUNDS; j++) {
+        const int i = tid + j * RS_BS;
+        if (i < tn) {
+            const uint64_t k = sk[i];
+            dst[j] = gbase[(int)((k >> shift) & (RADIX - 1))];
+            kout[(int64_t)dst[j] + i] = k;
         }
     }
     __syncthreads();
-    const int64_t tn = (n - tbase) < RS_TILE ? (n - tbase) : RS_TILE;
-#pragma unroll 4
-    for (int i = tid; i < tn; i += RS_BS) {
-        const uint64_t k = sk[i];
-        const int d = (int)((k >> shift) & (RADIX - 1));
-        const int64_t g = (int64_t)gbase[d] + i;
-        kout[g] = k;
-        vout[g] = sv[i];
+    V* sv = (V*)sk;
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++)
+        if (pos[r] >= 0) sv[pos[r]] = val[r];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ROUNDS; j++) {
+        const int i = tid + j * RS_BS;
+        if (i < tn) vout[(int64_t)dst[j] + i] = sv[i];
     }
 }
 
 size_t sort_tmp_bytes(int64_t n) {
     int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+    int64_t nchunks = (ntiles + RS_CHUNK - 1) / RS_CHUNK;
     size_t cnt = ((size_t)ntiles * RADIX * 4 + 255) & ~(size_t)255;
-    size_t kv = (((size_t)n * 8 + 255) & ~(size_t)255) + (((size_t)n * 4 + 255) & ~(size_t)255);
-    return cnt + kv + scan_tmp_bytes(ntiles * RADIX) + 256;
+    size_t cs = ((size_t)nchunks * RADIX * 4 + 255) & ~(size_t)255;
+    // the second key and value buffers (values of up to 8 bytes)
+    size_t kv = (((size_t)n * 8 + 255) & ~(size_t)255) + (((size_t)n * 8 + 255) & ~(size_t)255);
+    return cnt + cs + kv + scan_tmp_bytes(nchunks * RADIX) + 256;
 }
 
-// Sorts (keys, vals) in place by bits [0, end_bit) of the keys, stably.
-int sort_pairs_u64(hipStream_t st, uint64_t* keys, uint32_t* vals, int64_t n, int end_bit, void* tmp) {
+template <typename V>
+static int sort_pairs_t(hipStream_t st, uint64_t* keys, V* vals, int64_t n, int end_bit, void* tmp) {
     if (n <= 1 || end_bit <= 0) return SABLE_OK;
     int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
     SABLE_CHECK(ntiles * RADIX < 0x7fffffffLL, SABLE_EINVAL, "sort too large");
+    const int64_t nchunks = (ntiles + RS_CHUNK - 1) / RS_CHUNK;
     char* p = (char*)tmp;
     int32_t* counts = (int32_t*)p;
     p += ((size_t)ntiles * RADIX * 4 + 255) & ~(size_t)255;
+    int32_t* csum = (int32_t*)p;
+    p += ((size_t)nchunks * RADIX * 4 + 255) & ~(size_t)255;
     uint64_t* k2 = (uint64_t*)p;
     p += ((size_t)n * 8 + 255) & ~(size_t)255;
-    uint32_t* v2 = (uint32_t*)p;
-    p += ((size_t)n * 4 + 255) & ~(size_t)255;
+    V* v2 = (V*)p;
+    p += ((size_t)n * 8 + 255) & ~(size_t)255;
     void* stmp = p;
     uint64_t *ka = keys, *kb = k2;
-    uint32_t *va = vals, *vb = v2;
+    V *va = vals, *vb = v2;
     for (int shift = 0; shift < end_bit; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_BS), 0, st, ka, n, shift, counts, ntiles);
         SABLE_LAUNCH_CHECK();
-        int rc = scan_exclusive_i32(st, counts, counts, ntiles * RADIX, nullptr, stmp);
+        hipLaunchKernelGGL(k_rs_chunk_sums, dim3((unsigned)nchunks), dim3(RADIX), 0, st, counts, ntiles, nchunks,
+                           csum);
+        SABLE_LAUNCH_CHECK();
+        int rc = scan_exclusive_i32(st, csum, csum, nchunks * RADIX, nullptr, stmp);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)ntiles), dim3(RS_BS), 0, st, ka, va, kb, vb, n, shift,
+        hipLaunchKernelGGL(k_rs_chunk_starts, dim3((unsigned)nchunks), dim3(RADIX), 0, st, counts, ntiles, nchunks,
+                           csum);
+        SABLE_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_rs_scatter<V>, dim3((unsigned)ntiles), dim3(RS_BS), 0, st, ka, va, kb, vb, n, shift,
                            counts, ntiles);
         SABLE_LAUNCH_CHECK();
         uint64_t* tk = ka;
         ka = kb;
         kb = tk;
-        uint32_t* tv = va;
+        V* tv = va;
         va = vb;
         vb = tv;
     }
     if (ka != keys) {
         SABLE_HIP(hipMemcpyAsync(keys, ka, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
-        SABLE_HIP(hipMemcpyAsync(vals, va, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+        SABLE_HIP(hipMemcpyAsync(vals, va, (size_t)n * sizeof(V), hipMemcpyDeviceToDevice, st));
     }
     return SABLE_OK;
+}
+
+// Sorts (keys, vals) in place by bits [0, end_bit) of the keys, stably.
+int sort_pairs_u64(hipStream_t st, uint64_t* keys, uint32_t* vals, int64_t n, int end_bit, void* tmp) {
+    return sort_pairs_t<uint32_t>(st, keys, vals, n, end_bit, tmp);
+}
+
+int sort_pairs_u64_u64(hipStream_t st, uint64_t* keys, uint64_t* vals, int64_t n, int end_bit, void* tmp) {
+    return sort_pairs_t<uint64_t>(st, keys, vals, n, end_bit, tmp);
 }
 
 }  // namespace sable
