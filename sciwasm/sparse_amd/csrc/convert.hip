@@ -356,7 +356,7 @@ static int coo_csr_t(sable_ctx_t* ctx, int32_t N, int64_t nnz, const int32_t* ro
     SABLE_CHECK(nnz < 0xffffffffLL, SABLE_EINVAL, "nnz too large");
     const int colbits = bits_for(N > 0 ? N - 1 : 0);
     const int rowbits = bits_for(N > 0 ? N - 1 : 0);
-    size_t need = Carver::round(8 * nnz) + Carver::round(sizeof(P) * nnz) + 256 + sort_tmp_bytes(nnz);
+    size_t need = Carver::round(8 * nnz) + Carver::round(sizeof(P) * nnz) + 256 + sort_tmp_bytes(nnz, (int)sizeof(P));
     void* s;
     int rc = ctx_scratch(ctx, need, &s);
     if (rc) return rc;

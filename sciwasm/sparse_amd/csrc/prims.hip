@@ -316,13 +316,13 @@ __global__ __launch_bounds__(RS_BS, sizeof(V) > 4 ? 3 : 4) void k_rs_scatter(con
     }
 }
 
-size_t sort_tmp_bytes(int64_t n) {
+size_t sort_tmp_bytes(int64_t n, int value_bytes) {
     int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
     int64_t nchunks = (ntiles + RS_CHUNK - 1) / RS_CHUNK;
     size_t cnt = ((size_t)ntiles * RADIX * 4 + 255) & ~(size_t)255;
     size_t cs = ((size_t)nchunks * RADIX * 4 + 255) & ~(size_t)255;
-    // the second key and value buffers (values of up to 8 bytes)
-    size_t kv = (((size_t)n * 8 + 255) & ~(size_t)255) + (((size_t)n * 8 + 255) & ~(size_t)255);
+    // the second key and value buffers
+    size_t kv = (((size_t)n * 8 + 255) & ~(size_t)255) + (((size_t)n * value_bytes + 255) & ~(size_t)255);
     return cnt + cs + kv + scan_tmp_bytes(nchunks * RADIX) + 256;
 }
 
@@ -340,7 +340,7 @@ static int sort_pairs_t(hipStream_t st, uint64_t* keys, V* vals, int64_t n, int 
     uint64_t* k2 = (uint64_t*)p;
     p += ((size_t)n * 8 + 255) & ~(size_t)255;
     V* v2 = (V*)p;
-    p += ((size_t)n * 8 + 255) & ~(size_t)255;
+    p += ((size_t)n * sizeof(V) + 255) & ~(size_t)255;
     void* stmp = p;
     uint64_t *ka = keys, *kb = k2;
     V *va = vals, *vb = v2;
