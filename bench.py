@@ -72,7 +72,7 @@ MEASURED_TRAFFIC = {
     "tiles_16384x8192_fused": {
         "per_product": 2.858e9, "per_launch": 2.858e9,
         "source": "sciwasm/sparse_amd/csrc/spmv_tiles.hip",
-        "sha256": "4ff0891a1cd5dcb1da0a76a17d5129b803968cf4a21ed5f716ec961321da4714",
+        "sha256": "4860447ddc8b9ea8673fab2724c8901a6d783cc793bb54c04ef3da5088313357",
         "limiter": "L2 request rate: TCC busy 97% of the kernel's cycles at 0.47 requests per cycle per "
                    "channel; ~71M x-line requests (every CU touches most of x's lines: 805k entries over "
                    "262k 128-byte lines) beside ~19M streaming requests and 2.1M x prefetches; fetched "

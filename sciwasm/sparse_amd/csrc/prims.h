@@ -12,7 +12,8 @@ int scan_exclusive_i32(hipStream_t st, const int32_t* in, int32_t* out, int64_t 
 
 // Work space of a sort of n keys with values of value_bytes (4 or 8) bytes.
 size_t sort_tmp_bytes(int64_t n, int value_bytes = 4);
-// Stable LSD radix sort of (keys, vals) by key bits [0, end_bit), in place.
+// Stable LSD radix sort of (keys, vals) by key bits [0, end_bit), in place;
+// key bits from end_bit up are carried with their keys and not sorted.
 int sort_pairs_u64(hipStream_t st, uint64_t* keys, uint32_t* vals, int64_t n, int end_bit, void* tmp);
 // The same with 64-bit values (work space sort_tmp_bytes(n, 8)).
 int sort_pairs_u64_u64(hipStream_t st, uint64_t* keys, uint64_t* vals, int64_t n, int end_bit, void* tmp);

@@ -125,7 +125,7 @@ constexpr int RADIX = 256;
 // workgroup writes one contiguous 1 KiB row.  Each wave counts into its own
 // LDS row so that skewed digits do not serialise on one counter.
 __global__ __launch_bounds__(RS_BS) void k_rs_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
-                                                   int32_t* __restrict__ counts, int64_t ntiles) {
+                                                   uint32_t dmask, int32_t* __restrict__ counts, int64_t ntiles) {
     __shared__ int hist[RS_WAVES][RADIX];
     const int tid = threadIdx.x, w = tid / WAVE;
 #pragma unroll
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(RS_BS) void k_rs_hist(const uint64_t* __restrict__ 
     }
 #pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++)
-        if (base + r * RS_BS + tid < n) atomicAdd(&hist[w][(int)((k[r] >> shift) & (RADIX - 1))], 1);
+        if (base + r * RS_BS + tid < n) atomicAdd(&hist[w][(int)((k[r] >> shift) & dmask)], 1);
     __syncthreads();
     int t = 0;
 #pragma unroll
@@ -215,8 +215,8 @@ __device__ __forceinline__ int64_t rs_tile_of_block(int64_t b, int64_t ntiles) {
 template <typename V>
 __global__ __launch_bounds__(RS_BS, sizeof(V) > 4 ? 3 : 4) void k_rs_scatter(const uint64_t* __restrict__ kin, const V* __restrict__ vin,
                                                          uint64_t* __restrict__ kout, V* __restrict__ vout,
-                                                         int64_t n, int shift, const int32_t* __restrict__ offsets,
-                                                         int64_t ntiles) {
+                                                         int64_t n, int shift, uint32_t dmask,
+                                                         const int32_t* __restrict__ offsets, int64_t ntiles) {
     static_assert(sizeof(V) <= sizeof(uint64_t), "values share the key buffer");
     __shared__ uint64_t sk[RS_TILE];        // keys, then values
     __shared__ int wcnt[RS_WAVES][RADIX];  // running, then exclusive-over-waves counts
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(RS_BS, sizeof(V) > 4 ? 3 : 4) void k_rs_scatter(con
     for (int r = 0; r < RS_ROUNDS; r++) {
         const int64_t i = wbase + r * WAVE + lane;
         const bool valid = i < n;
-        const int d = (int)((key[r] >> shift) & (RADIX - 1));
+        const int d = (int)((key[r] >> shift) & dmask);
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(RS_BS, sizeof(V) > 4 ? 3 : 4) void k_rs_scatter(con
 #pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++) {
         if (pos[r] >= 0) {
-            const int d = (int)((key[r] >> shift) & (RADIX - 1));
+            const int d = (int)((key[r] >> shift) & dmask);
             pos[r] += dstart[d] + wcnt[w][d];
             sk[pos[r]] = key[r];
         }
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(RS_BS, sizeof(V) > 4 ? 3 : 4) void k_rs_scatter(con
         const int i = tid + j * RS_BS;
         if (i < tn) {
             const uint64_t k = sk[i];
-            dst[j] = gbase[(int)((k >> shift) & (RADIX - 1))];
+            dst[j] = gbase[(int)((k >> shift) & dmask)];
             kout[(int64_t)dst[j] + i] = k;
         }
     }
@@ -345,7 +345,10 @@ static int sort_pairs_t(hipStream_t st, uint64_t* keys, V* vals, int64_t n, int 
     uint64_t *ka = keys, *kb = k2;
     V *va = vals, *vb = v2;
     for (int shift = 0; shift < end_bit; shift += 8) {
-        hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_BS), 0, st, ka, n, shift, counts, ntiles);
+        // the last digit stops at end_bit: key bits above it are carried, not sorted
+        const uint32_t dmask = end_bit - shift >= 8 ? RADIX - 1 : (1u << (end_bit - shift)) - 1u;
+        hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_BS), 0, st, ka, n, shift, dmask, counts,
+                           ntiles);
         SABLE_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_rs_chunk_sums, dim3((unsigned)nchunks), dim3(RADIX), 0, st, counts, ntiles, nchunks,
                            csum);
@@ -356,7 +359,7 @@ static int sort_pairs_t(hipStream_t st, uint64_t* keys, V* vals, int64_t n, int 
                            csum);
         SABLE_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_rs_scatter<V>, dim3((unsigned)ntiles), dim3(RS_BS), 0, st, ka, va, kb, vb, n, shift,
-                           counts, ntiles);
+                           dmask, counts, ntiles);
         SABLE_LAUNCH_CHECK();
         uint64_t* tk = ka;
         ka = kb;

@@ -66,9 +66,17 @@ __global__ void k_tile_rows(int32_t N, const int32_t* __restrict__ rp, int32_t* 
 // entries are contiguous, row tile after row tile, each tile by column, so
 // that the workgroups working on one panel at the same time read one
 // compact region of memory
-__global__ void k_tile_keys(int64_t nnz, int32_t ncols, const int32_t* __restrict__ rows, const int32_t* __restrict__ col,
-                            int32_t tile_rows, int col_bits, int32_t ntr, uint64_t* __restrict__ keys,
-                            uint32_t* __restrict__ idx, int* __restrict__ err) {
+// Sort key of entry k: (panel * ntr + row tile) << col_bits | column-in-
+// panel in the low sort_bits bits (the sorted part), the row-in-tile from
+// bit rin_shift = sort_bits on (carried through the sort untouched); the value's bits are the
+// sort's value (P: an unsigned integer of the value's size), so the packing
+// pass reads everything in sorted order and gathers nothing.
+template <typename T, typename P>
+__global__ void k_tile_keys(int64_t nnz, int32_t ncols, const int32_t* __restrict__ rows,
+                            const int32_t* __restrict__ col, const T* __restrict__ val, int32_t tile_rows,
+                            int col_bits, int rin_shift, int32_t ntr, uint64_t* __restrict__ keys,
+                            P* __restrict__ pay, int* __restrict__ err) {
+    static_assert(sizeof(T) == sizeof(P), "value bits of the value's size");
     const uint32_t cmask = (1u << col_bits) - 1u;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
         const int32_t c = col[k];
@@ -76,36 +84,34 @@ __global__ void k_tile_keys(int64_t nnz, int32_t ncols, const int32_t* __restric
             atomicOr(err, 1);
             keys[k] = 0;
         } else {
-            const int64_t t = (int64_t)(c >> col_bits) * ntr + (rows[k] / tile_rows);
-            keys[k] = ((uint64_t)t << col_bits) | ((uint32_t)c & cmask);
+            const int32_t r = rows[k];
+            const int64_t t = (int64_t)(c >> col_bits) * ntr + (r / tile_rows);
+            keys[k] = ((uint64_t)(uint32_t)(r % tile_rows) << rin_shift) | ((uint64_t)t << col_bits) |
+                      ((uint32_t)c & cmask);
         }
-        idx[k] = (uint32_t)k;
+        pay[k] = __builtin_bit_cast(P, val[k]);
     }
 }
-
-template <typename T>
-__global__ void k_tile_pack(int64_t nnz, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ idx,
-                            const int32_t* __restrict__ rows, const T* __restrict__ val, int32_t tile_rows,
-                            int row_bits, int col_bits, uint32_t* __restrict__ packed, T* __restrict__ vout) {
+template <typename T, typename P>
+__global__ void k_tile_pack(int64_t nnz, const uint64_t* __restrict__ keys, const P* __restrict__ pay,
+                            int row_bits, int col_bits, int rin_shift, uint32_t* __restrict__ packed,
+                            T* __restrict__ vout) {
     const uint32_t cmask = (1u << col_bits) - 1u;
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nnz; s += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t k = idx[s];
-        const uint32_t rin = (uint32_t)(rows[k] % tile_rows);
-        packed[s] = (((uint32_t)keys[s] & cmask) << row_bits) | rin;
-        vout[s] = val[k];
+        const uint64_t key = keys[s];
+        packed[s] = (((uint32_t)key & cmask) << row_bits) | (uint32_t)(key >> rin_shift);
+        vout[s] = __builtin_bit_cast(T, pay[s]);
     }
 }
-
-// tile_ptr[t] = first sorted position whose tile id is >= t (t in [0, ntiles])
-__global__ void k_tile_heads(int64_t nnz, int64_t ntiles, int col_bits, const uint64_t* __restrict__ keys,
-                             int32_t* __restrict__ tile_ptr) {
+__global__ void k_tile_heads(int64_t nnz, int64_t ntiles, int col_bits, int sort_bits,
+                             const uint64_t* __restrict__ keys, int32_t* __restrict__ tile_ptr) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s > nnz) return;
-    const int64_t prev = s > 0 ? (int64_t)(keys[s - 1] >> col_bits) : -1;
-    const int64_t cur = s < nnz ? (int64_t)(keys[s] >> col_bits) : ntiles;
+    const uint64_t m = ((uint64_t)1 << sort_bits) - 1;
+    const int64_t prev = s > 0 ? (int64_t)((keys[s - 1] & m) >> col_bits) : -1;
+    const int64_t cur = s < nnz ? (int64_t)((keys[s] & m) >> col_bits) : ntiles;
     for (int64_t t = prev + 1; t <= cur; t++) tile_ptr[t] = (int32_t)s;
 }
-
 template <typename V>
 __device__ __forceinline__ V ld_nt(const V* p) {
     return __builtin_nontemporal_load(p);
@@ -544,6 +550,13 @@ SABLE_API int sable_csr_tiles_create_rect(sable_ctx_t* ctx, sable_dtype_t dtype,
     SABLE_HIP(hipStreamSynchronize(st));
     SABLE_CHECK(ends[0] == 0 && ends[1] >= 0, SABLE_EMALFORMED, "row_ptr must start at 0 and end at nnz >= 0");
     const int64_t nnz = ends[1];
+    const int32_t ntr = (int32_t)(((int64_t)N + tile_rows - 1) / tile_rows);
+    const int32_t npanels = (int32_t)(((int64_t)ncols + panel_cols - 1) / panel_cols);
+    const int64_t ntiles = (int64_t)ntr * npanels;
+    SABLE_CHECK(ntiles < (int64_t)1 << 30, SABLE_EINVAL, "too many tiles");
+    const int sort_bits = bits_for(ntiles) + cb;
+    const int rin_shift = sort_bits;
+    SABLE_CHECK(rin_shift + rb <= 64, SABLE_EINVAL, "tile plan keys exceed 64 bits");
     auto* p = new sable_csr_tiles_s();
     p->ctx = ctx;
     p->dtype = dtype;
@@ -553,14 +566,12 @@ SABLE_API int sable_csr_tiles_create_rect(sable_ctx_t* ctx, sable_dtype_t dtype,
     p->panel_cols = panel_cols;
     p->row_bits = rb;
     p->col_bits = cb;
-    p->ntr = (int32_t)(((int64_t)N + tile_rows - 1) / tile_rows);
-    p->npanels = (int32_t)(((int64_t)ncols + panel_cols - 1) / panel_cols);
-    const int64_t ntiles = (int64_t)p->ntr * p->npanels;
-    SABLE_CHECK(ntiles < (int64_t)1 << 30, SABLE_EINVAL, "too many tiles");
-    // build buffers: rows, keys, idx and the sort's work space
-    const size_t tmp_sort = sort_tmp_bytes(nnz);
+    p->ntr = ntr;
+    p->npanels = npanels;
+    // build buffers: rows, keys, the value bits and the sort's work space
+    const size_t tmp_sort = sort_tmp_bytes(nnz, (int)vs);
     const size_t need = Carver::round(4 * (size_t)nnz) + Carver::round(8 * (size_t)nnz) +
-                        Carver::round(4 * (size_t)nnz) + Carver::round(tmp_sort) + 256;
+                        Carver::round(vs * (size_t)nnz) + Carver::round(tmp_sort) + 256;
     void* work = nullptr;
     bool ok = hipMalloc(&work, need) == hipSuccess &&
               hipMalloc(&p->tile_ptr, 4 * (size_t)(ntiles + 1)) == hipSuccess &&
@@ -580,31 +591,40 @@ SABLE_API int sable_csr_tiles_create_rect(sable_ctx_t* ctx, sable_dtype_t dtype,
     Carver cv(work);
     int32_t* rows = cv.take<int32_t>((size_t)nnz);
     uint64_t* keys = cv.take<uint64_t>((size_t)nnz);
-    uint32_t* idx = cv.take<uint32_t>((size_t)nnz);
+    void* pay = cv.take<char>(vs * (size_t)nnz);
     void* stmp = cv.take<char>(tmp_sort);
     int* err = cv.take<int>(64);
     SABLE_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
     hipLaunchKernelGGL(k_tile_rows, dim3(gs(N)), dim3(256), 0, st, N, row_ptr, rows);
-    if (nnz > 0)
-        hipLaunchKernelGGL(k_tile_keys, dim3(gs(nnz)), dim3(256), 0, st, nnz, ncols, rows, col, tile_rows, cb, p->ntr,
-                           keys, idx, err);
+    if (nnz > 0) {
+        if (dtype == SABLE_F64)
+            hipLaunchKernelGGL((k_tile_keys<double, uint64_t>), dim3(gs(nnz)), dim3(256), 0, st, nnz, ncols, rows, col,
+                               (const double*)val, tile_rows, cb, rin_shift, p->ntr, keys, (uint64_t*)pay, err);
+        else
+            hipLaunchKernelGGL((k_tile_keys<float, uint32_t>), dim3(gs(nnz)), dim3(256), 0, st, nnz, ncols, rows, col,
+                               (const float*)val, tile_rows, cb, rin_shift, p->ntr, keys, (uint32_t*)pay, err);
+    }
     if (hipGetLastError() != hipSuccess) return cleanup(fail(SABLE_EHIP, "tile key launch failed"));
     if (hipMemcpyAsync(ctx->pinned, err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return cleanup(fail(SABLE_EHIP, "tile key check failed"));
     if (*(int*)ctx->pinned) return cleanup(fail(SABLE_EMALFORMED, "column index outside [0, ncols)"));
     if (nnz > 0) {
-        const int rc = sort_pairs_u64(st, keys, idx, nnz, bits_for(ntiles) + cb, stmp);
-        if (rc) return cleanup(rc);
-        if (dtype == SABLE_F64)
-            hipLaunchKernelGGL(k_tile_pack<double>, dim3(gs(nnz)), dim3(256), 0, st, nnz, keys, idx, rows,
-                               (const double*)val, tile_rows, rb, cb, p->packed, (double*)p->val);
-        else
-            hipLaunchKernelGGL(k_tile_pack<float>, dim3(gs(nnz)), dim3(256), 0, st, nnz, keys, idx, rows,
-                               (const float*)val, tile_rows, rb, cb, p->packed, (float*)p->val);
+        // stable on the sorted bits: equal (tile, column) keep ascending rows
+        if (dtype == SABLE_F64) {
+            const int rc = sort_pairs_u64_u64(st, keys, (uint64_t*)pay, nnz, sort_bits, stmp);
+            if (rc) return cleanup(rc);
+            hipLaunchKernelGGL((k_tile_pack<double, uint64_t>), dim3(gs(nnz)), dim3(256), 0, st, nnz, keys,
+                               (const uint64_t*)pay, rb, cb, rin_shift, p->packed, (double*)p->val);
+        } else {
+            const int rc = sort_pairs_u64(st, keys, (uint32_t*)pay, nnz, sort_bits, stmp);
+            if (rc) return cleanup(rc);
+            hipLaunchKernelGGL((k_tile_pack<float, uint32_t>), dim3(gs(nnz)), dim3(256), 0, st, nnz, keys,
+                               (const uint32_t*)pay, rb, cb, rin_shift, p->packed, (float*)p->val);
+        }
     }
-    hipLaunchKernelGGL(k_tile_heads, dim3((unsigned)grid_for(nnz + 1, 256)), dim3(256), 0, st, nnz, ntiles, cb, keys,
-                       p->tile_ptr);
+    hipLaunchKernelGGL(k_tile_heads, dim3((unsigned)grid_for(nnz + 1, 256)), dim3(256), 0, st, nnz, ntiles, cb,
+                       sort_bits, keys, p->tile_ptr);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
         return cleanup(fail(SABLE_EHIP, "tile plan build failed"));
     // LDS above the default dynamic limit must be requested per kernel

@@ -49,6 +49,19 @@ def test_sort_pairs_stable(ctx, n, bits):
     assert np.array_equal(dv.numpy(), vals[order])
 
 
+@pytest.mark.parametrize("n,bits", [(5000, 12), (70_001, 27), (9000, 8)])
+def test_sort_pairs_carries_bits_above_end_bit(ctx, n, bits):
+    """Only key bits [0, bits) order the keys; the bits above ride along."""
+    rng = np.random.default_rng(bits)
+    keys = rng.integers(0, 1 << 40, n).astype(np.uint64)
+    vals = np.arange(n, dtype=np.uint32)
+    dk, dv = DeviceArray.from_host(keys), DeviceArray.from_host(vals)
+    _lib.call("sable_sort_pairs_u64", ctx.handle, dk.ptr, dv.ptr, n, bits)
+    order = np.argsort(keys & np.uint64((1 << bits) - 1), kind="stable")
+    assert np.array_equal(dk.numpy(), keys[order])
+    assert np.array_equal(dv.numpy(), vals[order])
+
+
 # ----------------------------------------------------------------- coo_csr
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("N,nnz", [(1, 1), (37, 400), (1000, 30000)])
