@@ -2,8 +2,10 @@
 // Node-API addon: a CSR matrix in the linear memory (arrays at _malloc
 // offsets, filled through typed-array views, as seq/src/sparse32.js does),
 // then spmv_csr_wrapper(rowptr, col, val, x, y, N, inner_max) calls, each a
-// full y += A.x inner_max times with x and y crossing PCIe
+// full y += A.x inner_max times, y brought back to the linear memory
 // (seq/tests/main32.js:122-134 times the same call after its warm-up).
+// `uploads` counts matrix-array uploads and `vector_uploads` uploads of x
+// or y (both stay on the device while the caller leaves them unchanged).
 //
 // usage: node bench_node.js <dir> <32|64> <warmup> <calls> <inner_max>[,<inner_max>...]
 // <dir> holds spec.json ({N, nnz}) and rp.bin, col.bin, val.bin, x.bin
@@ -62,7 +64,7 @@ out.schedule = inst.stats().schedule;
 out.schedule_ms = inst.stats().schedule_ms;
 for (const im of inner) {
   for (let i = 0; i < warmup; i++) E.spmv_csr_wrapper(rp, col, val, x, y, N, im);
-  const up0 = inst.stats().uploads;
+  const up0 = inst.stats().uploads, vu0 = inst.stats().vector_uploads;
   const t = [];
   for (let i = 0; i < calls; i++) {
     a = process.hrtime.bigint();
@@ -71,7 +73,8 @@ for (const im of inner) {
   }
   t.sort((p, q) => p - q);
   out.runs[im] = { calls: calls, median_ms: t[Math.floor(calls / 2)], min_ms: t[0],
-                   mean_ms: t.reduce((p, q) => p + q, 0) / calls, uploads: inst.stats().uploads - up0 };
+                   mean_ms: t.reduce((p, q) => p + q, 0) / calls, uploads: inst.stats().uploads - up0,
+                   vector_uploads: inst.stats().vector_uploads - vu0 };
 }
 yv.fill(0);
 E.spmv_csr_wrapper(rp, col, val, x, y, N, 1);

@@ -168,10 +168,10 @@ void upload(void* d, size_t off, size_t bytes) {
         check(sable_memcpy_h2d_staged(g_ctx, d, g_mem.base + off, bytes), "upload");
 }
 
-void download_raw(size_t off, const void* d, size_t bytes) {
+void download_raw(size_t off, const void* d, size_t bytes, bool count = true) {
     check_range(off, bytes);
     if (!bytes) return;
-    unprotect_range(off, bytes);
+    unprotect_range(off, bytes, count);
     if (g_mem.pinned)
         check(sable_memcpy_d2h(g_ctx, g_mem.base + off, d, bytes), "download");
     else
@@ -209,15 +209,20 @@ bool mirror_valid(Mirror& m) {
     return true;
 }
 
-uint64_t g_uploads = 0, g_upload_bytes = 0;  // mirror uploads (statistics)
+uint64_t g_uploads = 0, g_upload_bytes = 0;  // mirror uploads of matrix arrays (statistics)
+uint64_t g_vec_uploads = 0;                   // ... of product vectors x and y
 int g_last_sched = -1;                         // SABLE_SCHED_* of the last plan built
 double g_last_sched_ms = 0;
 
-void fill_mirror(Mirror& m) {
+void fill_mirror(Mirror& m, bool vector = false) {
     // order: counts first, then protection, then the copy, so that a write
     // racing with the copy is always seen by the next check
-    g_uploads++;
-    g_upload_bytes += m.bytes;
+    if (vector) {
+        g_vec_uploads++;
+    } else {
+        g_uploads++;
+        g_upload_bytes += m.bytes;
+    }
     m.version = ++g_version;
     m.seen = sable_track::faults();
     if (m.tracked) {
@@ -228,7 +233,7 @@ void fill_mirror(Mirror& m) {
     upload(m.d, m.off, m.bytes);
 }
 
-Mirror& mirror_rec(size_t off, size_t bytes) {
+Mirror& mirror_rec(size_t off, size_t bytes, bool vector = false) {
     check_range(off, bytes);
     auto& slot = g_cache[{off, bytes}];
     if (slot && slot->d && mirror_valid(*slot)) return *slot;
@@ -241,11 +246,23 @@ Mirror& mirror_rec(size_t off, size_t bytes) {
         if (slot->tracked) full_pages(off, bytes, &slot->p0, &slot->p1);
         check(sable_device_alloc(g_ctx, bytes ? bytes : 64, &slot->d), "device alloc");
     }
-    fill_mirror(*slot);
+    fill_mirror(*slot, vector);
     return *slot;
 }
 
 void* mirror(size_t off, size_t bytes) { return mirror_rec(off, bytes).d; }
+
+// The mirror's device bytes were just made equal to the host's (written on
+// both sides by the export): it is valid again, as a new version.
+void revalidate(Mirror& m) {
+    m.version = ++g_version;
+    m.seen = sable_track::faults();
+    if (m.tracked) {
+        m.sum = wsum(m.p0, m.p1);
+        protect_pages(m.p0, m.p1);
+    }
+    m.hash = m.tracked ? edge_hash(m) : hash_bytes(g_mem.base + m.off, m.bytes);
+}
 
 // After an export wrote `bytes` at `off` from device memory d (the bytes
 // are now the same on both sides), d becomes that region's mirror, so
@@ -260,13 +277,7 @@ void adopt(size_t off, size_t bytes, void* d) {
     m.d = d;
     m.tracked = sable_track::g_on && bytes >= 4 * sable_track::g_page;
     if (m.tracked) full_pages(off, bytes, &m.p0, &m.p1);
-    m.version = ++g_version;
-    m.seen = sable_track::faults();
-    if (m.tracked) {
-        m.sum = wsum(m.p0, m.p1);
-        protect_pages(m.p0, m.p1);
-    }
-    m.hash = m.tracked ? edge_hash(m) : hash_bytes(g_mem.base + off, bytes);
+    revalidate(m);
 }
 
 // Fresh device buffers for one call's outputs and per-call inputs.
@@ -322,6 +333,89 @@ void* vec_in(int k, size_t off, size_t bytes) {
     void* d = vec(k, bytes);
     upload(d, off, bytes);
     return d;
+}
+
+// ------------------------------------------------------------ product vectors
+// x and y of the whole-matrix products.  A vector the host left unchanged
+// since the last call need not cross PCIe again, so x and y are kept as
+// mirrors (write-tracked like the matrix arrays); y is updated on the
+// device in place, downloaded, and stays the valid mirror of its region.
+// A vector found written by the host since the call that mirrored it goes
+// to the pool vectors for the next kStreamCalls calls (uploaded on every
+// call, its pages left writable), so that a caller that rewrites x or y
+// before each call pays one round of page faults, not one per call.  When y
+// overlaps x or one of the matrix arrays, both take the pool vectors.
+// SABLE_RESIDENT_VECTORS=0 keeps x and y out of the mirrors altogether
+// (their pages are then never write-protected, for callers that fill them
+// with system calls such as fs.readSync).
+constexpr int kStreamCalls = 64;
+
+bool resident_vectors() {
+    static const bool on = [] {
+        const char* e = getenv("SABLE_RESIDENT_VECTORS");
+        return !(e && atoi(e) == 0);
+    }();
+    return on;
+}
+std::map<std::pair<size_t, size_t>, int> g_stream;  // calls left in pool mode
+
+bool overlaps(size_t a, size_t an, size_t b, size_t bn) { return an && bn && a < b + bn && b < a + an; }
+
+Mirror* product_mirror(size_t off, size_t bytes) {
+    check_range(off, bytes);
+    const auto key = std::make_pair(off, bytes);
+    auto st = g_stream.find(key);
+    if (st != g_stream.end()) {
+        if (--st->second > 0) return nullptr;
+        g_stream.erase(st);
+    }
+    auto it = g_cache.find(key);
+    if (it != g_cache.end() && it->second->d && !mirror_valid(*it->second)) {
+        g_stream[key] = kStreamCalls;
+        return nullptr;
+    }
+    return &mirror_rec(off, bytes, true);
+}
+
+struct ProductXY {
+    void* dx = nullptr;
+    void* dy = nullptr;
+    Mirror* my = nullptr;  // y's mirror (null: pool vector)
+    size_t y = 0, yb = 0;
+};
+
+ProductXY xy_in(size_t x, size_t xb, size_t y, size_t yb, std::initializer_list<std::pair<size_t, size_t>> mats) {
+    bool alias = !resident_vectors() || overlaps(x, xb, y, yb);
+    for (const auto& m : mats) alias = alias || overlaps(m.first, m.second, y, yb);
+    ProductXY r;
+    r.y = y;
+    r.yb = yb;
+    Mirror* mx = alias ? nullptr : product_mirror(x, xb);
+    r.dx = mx ? mx->d : vec_in(0, x, xb);
+    r.my = alias ? nullptr : product_mirror(y, yb);
+    r.dy = r.my ? r.my->d : vec_in(1, y, yb);
+    return r;
+}
+
+// Runs the product `f` (which writes y on the device) and brings y back.
+template <typename F>
+void xy_run(ProductXY& r, F&& f) {
+    try {
+        f();
+    } catch (...) {
+        // y's device copy may be part-written: no longer a mirror of the host
+        if (r.my) r.my->hash = ~r.my->hash;
+        throw;
+    }
+    if (!r.my) {
+        download(r.y, r.dy, r.yb);
+        return;
+    }
+    bool shared = false;  // another mirror covers some of y's pages
+    for (const auto& e : g_cache)
+        if (e.second.get() != r.my && overlaps(e.first.first, e.first.second, r.y, r.yb)) shared = true;
+    download_raw(r.y, r.dy, r.yb, shared);
+    revalidate(*r.my);
 }
 
 const int32_t* host_i32(size_t off, size_t count) {
@@ -562,12 +656,12 @@ ResidentCsr& coo_resident(Prec* P, size_t row, size_t col, size_t val, int64_t n
 }
 
 // One product call: x (ncols values) and y (N values) in, y += A.x
-// inner_max times on the resident schedule, y out.
-void run_resident(Prec* P, ResidentCsr& R, size_t x, size_t y, int im) {
-    void* dx = vec_in(0, x, P->vs * (size_t)R.ncols);
-    void* dy = vec_in(1, y, P->vs * (size_t)R.N);
-    check(sable_spmv_csr_auto(g_ctx, R.plan, dx, dy, 1.0, im), "spmv");
-    download(y, dy, P->vs * (size_t)R.N);
+// inner_max times on the resident schedule, y out.  `mats`: the matrix's
+// regions of the linear memory.
+void run_resident(Prec* P, ResidentCsr& R, size_t x, size_t y, int im,
+                  std::initializer_list<std::pair<size_t, size_t>> mats) {
+    ProductXY r = xy_in(x, P->vs * (size_t)R.ncols, y, P->vs * (size_t)R.N, mats);
+    xy_run(r, [&] { check(sable_spmv_csr_auto(g_ctx, R.plan, r.dx, r.dy, 1.0, im), "spmv"); });
 }
 
 // spmv_coo(row, col, val, x, y, len[, inner_max])
@@ -580,7 +674,8 @@ napi_value spmv_coo_impl(napi_env env, napi_callback_info info, bool wrapper) {
         const int64_t nnz = a.i(5);
         const int im = wrapper ? (int)a.i(6) : 1;
         if (nnz <= 0 || im <= 0) return undef(env);
-        run_resident(P, coo_resident(P, row, col, val, nnz), x, y, im);
+        run_resident(P, coo_resident(P, row, col, val, nnz), x, y, im,
+                     {{row, 4 * (size_t)nnz}, {col, 4 * (size_t)nnz}, {val, P->vs * (size_t)nnz}});
         return undef(env);
     });
 }
@@ -595,7 +690,9 @@ napi_value spmv_csr_impl(napi_env env, napi_callback_info info, bool wrapper) {
         const int32_t N = (int32_t)a.i(5);
         const int im = wrapper ? (int)a.i(6) : 1;
         if (N <= 0 || im <= 0) return undef(env);
-        run_resident(P, csr_resident(P, rp, col, val, N), x, y, im);
+        ResidentCsr& R = csr_resident(P, rp, col, val, N);
+        const size_t j1 = (size_t)host_i32(rp, (size_t)N + 1)[N];
+        run_resident(P, R, x, y, im, {{rp, 4 * ((size_t)N + 1)}, {col, 4 * j1}, {val, P->vs * j1}});
         return undef(env);
     });
 }
@@ -622,7 +719,12 @@ napi_value spmv_csr_short_impl(napi_env env, napi_callback_info info) {
         const int32_t* h = host_i32(rp, (size_t)N + 1);
         if (h[n1] - h[0] != n1 || h[n1 + n2] - h[n1] != 2 * n2 || h[n1 + n2 + n3] - h[n1 + n2] != 3 * n3)
             throw Err{"short-row group sizes do not match rowptr"};
-        run_resident(P, csr_resident(P, rp, col, val, (int32_t)N), x, y, im);
+        // the workers' calls share y: the pool vectors, as for the other slices
+        ResidentCsr& R = csr_resident(P, rp, col, val, (int32_t)N);
+        void* dx = vec_in(0, x, P->vs * (size_t)R.ncols);
+        void* dy = vec_in(1, y, P->vs * (size_t)R.N);
+        check(sable_spmv_csr_auto(g_ctx, R.plan, dx, dy, 1.0, im), "spmv");
+        download(y, dy, P->vs * (size_t)R.N);
         return undef(env);
     });
 }
@@ -639,10 +741,11 @@ napi_value spmv_dia_impl(napi_env env, napi_callback_info info, bool wrapper) {
         if (nd <= 0 || N <= 0 || im <= 0) return undef(env);
         void* doff = mirror(off, 4 * (size_t)nd);
         void* dd = mirror(data, P->vs * (size_t)nd * stride);
-        void* dx = vec_in(0, x, P->vs * N);
-        void* dy = vec_in(1, y, P->vs * N);
-        check(sable_spmv_dia(g_ctx, P->dt, N, nd, stride, (const int32_t*)doff, dd, dx, dy, 1.0, im), "spmv_dia");
-        download(y, dy, P->vs * N);
+        ProductXY r = xy_in(x, P->vs * N, y, P->vs * N, {{off, 4 * (size_t)nd}, {data, P->vs * (size_t)nd * stride}});
+        xy_run(r, [&] {
+            check(sable_spmv_dia(g_ctx, P->dt, N, nd, stride, (const int32_t*)doff, dd, r.dx, r.dy, 1.0, im),
+                  "spmv_dia");
+        });
         return undef(env);
     });
 }
@@ -673,10 +776,10 @@ napi_value spmv_ell_impl(napi_env env, napi_callback_info info, bool wrapper) {
         Mirror& mi = mirror_rec(idx, 4 * slots);
         void* dd = mirror(data, P->vs * slots);
         const int32_t ncol = max_index(mi, (int64_t)slots) + 1;
-        void* dx = vec_in(0, x, P->vs * (ncol > N ? ncol : N));
-        void* dy = vec_in(1, y, P->vs * N);
-        check(sable_spmv_ell(g_ctx, P->dt, N, nc, (const int32_t*)mi.d, dd, dx, dy, 1.0, im), "spmv_ell");
-        download(y, dy, P->vs * N);
+        ProductXY r = xy_in(x, P->vs * (ncol > N ? ncol : N), y, P->vs * N, {{idx, 4 * slots}, {data, P->vs * slots}});
+        xy_run(r, [&] {
+            check(sable_spmv_ell(g_ctx, P->dt, N, nc, (const int32_t*)mi.d, dd, r.dx, r.dy, 1.0, im), "spmv_ell");
+        });
         return undef(env);
     });
 }
@@ -1718,6 +1821,7 @@ napi_value stats_fn(napi_env env, napi_callback_info) {
     napi_set_named_property(env, o, "writes_caught", num(env, (double)sable_track::faults()));
     napi_set_named_property(env, o, "uploads", num(env, (double)g_uploads));
     napi_set_named_property(env, o, "upload_bytes", num(env, (double)g_upload_bytes));
+    napi_set_named_property(env, o, "vector_uploads", num(env, (double)g_vec_uploads));
     napi_set_named_property(env, o, "resident", num(env, (double)g_resident.size()));
     napi_set_named_property(env, o, "schedule", num(env, g_last_sched));
     napi_set_named_property(env, o, "schedule_ms", num(env, g_last_sched_ms));

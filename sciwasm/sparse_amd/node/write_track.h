@@ -107,16 +107,20 @@ inline void protect_pages(size_t p0, size_t p1) {
 }
 
 // Makes every page touching [off, off + bytes) writable before the owner
-// writes there itself, counting a write on each page that was protected.
-inline void unprotect_range(size_t off, size_t bytes) {
+// writes there itself, counting a write on each page that was protected
+// (count = false: the owner knows that no other mirror covers these pages
+// and revalidates its own mirror of them itself).
+inline void unprotect_range(size_t off, size_t bytes, bool count = true) {
     if (!g_on || !bytes) return;
     const size_t p0 = off >> g_page_shift, p1 = (off + bytes + g_page - 1) >> g_page_shift;
     size_t run = p1;  // start of the current run of protected pages
     for (size_t p = p0; p <= p1; p++) {
         const bool prot = p < p1 && __atomic_exchange_n(&g_prot[p], (uint8_t)0, __ATOMIC_ACQ_REL);
         if (prot) {
-            __atomic_add_fetch(&g_wcount[p], 1u, __ATOMIC_ACQ_REL);
-            __atomic_add_fetch(&g_faults, 1ull, __ATOMIC_ACQ_REL);
+            if (count) {
+                __atomic_add_fetch(&g_wcount[p], 1u, __ATOMIC_ACQ_REL);
+                __atomic_add_fetch(&g_faults, 1ull, __ATOMIC_ACQ_REL);
+            }
             if (run == p1) run = p;
         } else if (run != p1) {
             mprotect(g_base + (run << g_page_shift), (p - run) << g_page_shift, PROT_READ | PROT_WRITE);

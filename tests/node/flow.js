@@ -179,6 +179,43 @@ outputs.write_tracking = inst.stats().write_tracking;
   v[nnz - 1] = keep[1];
 }
 
+// x and y stay on the device between products while the host leaves them
+// alone (write-tracked like the matrix); host writes to either are seen.
+// A 3-diagonal DIA of 20000 rows, so that x and y span whole pages.
+{
+  step('resident_xy');
+  const N2 = 20000, nd2 = 3;
+  const off2 = alloc(nd2, Int32Array);
+  new Int32Array(memory.buffer, off2, nd2).set([-1, 0, 1]);
+  const d2 = alloc(nd2 * N2, FArray);
+  const dv2 = new FArray(memory.buffer, d2, nd2 * N2);
+  for (let i = 0; i < nd2 * N2; i++) dv2[i] = ((i * 7) % 11 - 5) / 4;
+  const x2 = alloc(N2, FArray);
+  const xv2 = new FArray(memory.buffer, x2, N2);
+  for (let i = 0; i < N2; i++) xv2[i] = ((i * 5) % 13 - 6) / 8;
+  save('rx_off', off2, nd2, Int32Array);
+  save('rx_data', d2, nd2 * N2, FArray);
+  save('rx_x', x2, N2, FArray);
+  const y2 = alloc(N2, FArray);
+  const call = () => E.spmv_dia_wrapper(off2, d2, N2, nd2, N2, x2, y2, 1);
+  call();
+  const v0 = inst.stats().vector_uploads;
+  call();  // x and y unchanged: neither crosses to the device again
+  outputs.rx_vec_uploads_repeat = inst.stats().vector_uploads - v0;
+  save('rx_y2', y2, N2, FArray);
+  new FArray(memory.buffer, y2, N2).fill(0.5);
+  call();
+  save('rx_yw', y2, N2, FArray);
+  xv2[3] = 2.5;
+  xv2[N2 - 1] = -1.25;
+  call();
+  save('rx_x_mod', x2, N2, FArray);
+  save('rx_yx', y2, N2, FArray);
+  for (let k = 0; k < 3; k++) call();
+  save('rx_y3', y2, N2, FArray);
+  outputs.rx_n = N2;
+}
+
 // ---------------------------------------------------------------- transposes
 step('transpose_csr');
 const tcsr = { row: alloc(N + 1, Int32Array), col: alloc(nnz, Int32Array), val: alloc(nnz, FArray) };

@@ -176,14 +176,15 @@ def read_out(d, name, dtype):
 
 @needs_node
 @pytest.mark.gpu
+@pytest.mark.parametrize("resident_vectors", ["1", "0"], ids=["resident_xy", "pool_xy"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
-def test_node_flow_matches_oracle(tmp_path, dtype):
+def test_node_flow_matches_oracle(tmp_path, dtype, resident_vectors):
     d = str(tmp_path)
     spec, a = write_inputs(d, dtype)
     bits = "32" if dtype == np.float32 else "64"
-    print("node flow", bits, flush=True)
+    print("node flow", bits, "resident vectors", resident_vectors, flush=True)
     r = subprocess.run([NODE, FLOW, d, bits], capture_output=True, text=True, timeout=300, cwd=ROOT,
-                       env=dict(os.environ, SABLE_LINEAR_MEMORY_MB="256"))
+                       env=dict(os.environ, SABLE_LINEAR_MEMORY_MB="256", SABLE_RESIDENT_VECTORS=resident_vectors))
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     out = json.load(open(os.path.join(d, "outputs.json")))
     N, nnz, im, W = spec["N"], spec["nnz"], spec["inner_max"], spec["workers"]
@@ -242,6 +243,20 @@ def test_node_flow_matches_oracle(tmp_path, dtype):
     ymod = oracle.spmv_csr(N, rp, col, vmod, x, z, 1)
     err = np.abs(read_out(d, "y_csr_mod", dtype).astype(np.float64) - ymod)
     assert np.all(err <= row_bound(rp, col, vmod, x, z, dtype))
+
+    # x and y resident between products: a repeated call uploads neither,
+    # host writes to y and then to x are seen (DIA: bit-exact)
+    n2 = out["rx_n"]
+    roff, rdata = read_out(d, "rx_off", i32), read_out(d, "rx_data", dtype)
+    rx, rxm = read_out(d, "rx_x", dtype), read_out(d, "rx_x_mod", dtype)
+    assert out["rx_vec_uploads_repeat"] == 0
+    assert not np.array_equal(rx, rxm)
+    y2 = oracle.spmv_dia(n2, n2, roff, rdata, rx, np.zeros(n2, dtype), 2)
+    yw = oracle.spmv_dia(n2, n2, roff, rdata, rx, np.full(n2, 0.5, dtype), 1)
+    yx = oracle.spmv_dia(n2, n2, roff, rdata, rxm, yw, 1)
+    y3 = oracle.spmv_dia(n2, n2, roff, rdata, rxm, yx, 3)
+    for name, yref in (("rx_y2", y2), ("rx_yw", yw), ("rx_yx", yx), ("rx_y3", y3)):
+        assert np.array_equal(read_out(d, name, dtype).view(np.uint8), yref.view(np.uint8)), name
 
     # parallel-tree SIMD variants: CSR/COO within the bound, column-major
     # DIA/ELL bit for bit (each row keeps its stored order)
