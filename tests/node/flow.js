@@ -88,8 +88,8 @@ const P = my_instance.exports;
 // ---------------------------------------------------------------- COO -> CSR
 // The reference's conversion shape (seq/src/sparse32.js:1674, sparse64.js:1255,
 // par/src/sparse32.js:2096): coo_csr(A_coo) allocates A_csr with allocate_CSR
-// and returns it; with the addon its body calls the coo_csr export (the
-// edited function INTEGRATION.md shows).  Callers use the return value.
+// and returns it; with the addon its body calls the coo_csr export instead
+// of the reference's JS loops.  Callers use the return value.
 function allocate_CSR(N_, nnz_) {
   return { row_index: alloc(N_ + 1, Int32Array), col_index: alloc(nnz_, Int32Array), val_index: alloc(nnz_, FArray),
            N: N_, nnz: nnz_ };
