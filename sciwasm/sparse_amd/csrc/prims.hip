@@ -330,7 +330,8 @@ template <typename V>
 static int sort_pairs_t(hipStream_t st, uint64_t* keys, V* vals, int64_t n, int end_bit, void* tmp) {
     if (n <= 1 || end_bit <= 0) return SABLE_OK;
     int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
-    SABLE_CHECK(ntiles * RADIX < 0x7fffffffLL, SABLE_EINVAL, "sort too large");
+    // positions and digit counts are 32-bit
+    SABLE_CHECK(n <= 0x7fffffffLL && ntiles * RADIX < 0x7fffffffLL, SABLE_EINVAL, "sort too large");
     const int64_t nchunks = (ntiles + RS_CHUNK - 1) / RS_CHUNK;
     char* p = (char*)tmp;
     int32_t* counts = (int32_t*)p;
