@@ -1778,6 +1778,8 @@ napi_value free_fn(napi_env env, napi_callback_info info) {
             drop_resident_in(off, it->second);
             drop_mirrors_in(off, it->second);
             unprotect_range(off, it->second);
+            for (auto st = g_stream.begin(); st != g_stream.end();)
+                st = overlaps(st->first.first, st->first.second, off, it->second) ? g_stream.erase(st) : std::next(st);
         }
         lm_free(off);
         return undef(env);
