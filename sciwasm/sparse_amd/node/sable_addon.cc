@@ -962,8 +962,9 @@ napi_value coo_csc(napi_env env, napi_callback_info info) {
 }
 
 // A lower-triangular matrix with its solve analysis (level schedule, the
-// level-ordered copy), kept while its arrays are unchanged.  One lane per
-// row, so every solve keeps the reference's per-row order.
+// level-ordered copy), kept while its arrays are unchanged.  Every schedule
+// (row blocks, lanes per row) adds a row's products in the reference's
+// per-row order, so the solves keep the reference's bits.
 struct ResidentSpts : ResidentBase {
     sable_spts_t* a = nullptr;
     uint64_t v[3] = {0, 0, 0};
@@ -972,7 +973,11 @@ struct ResidentSpts : ResidentBase {
     }
 };
 
-enum { KIND_SPTS_CSR = 32, KIND_SPTS_CSC = 48 };
+// KIND_SPTS_LEVEL: the level-set exports' analysis, whose schedule is tuned
+// once (sable_spts_tune: the fastest of the row-block and level-graph
+// schedules, all of which give the level solve's bits); the sync-free
+// export keeps its own analysis on the default schedule.
+enum { KIND_SPTS_CSR = 32, KIND_SPTS_CSC = 48, KIND_SPTS_LEVEL = 64 };
 
 ResidentSpts& spts_resident(Prec* P, int kind, size_t p, size_t i, size_t v, int32_t N) {
     const int32_t* h = host_i32(p, (size_t)N + 1);
@@ -991,6 +996,7 @@ ResidentSpts& spts_resident(Prec* P, int kind, size_t p, size_t i, size_t v, int
         else
             check(sable_spts_analyze(g_ctx, P->dt, N, (const int32_t*)mp.d, (const int32_t*)mi.d, mv.d, 1, &R.a),
                   "spts analysis");
+        if (kind == KIND_SPTS_LEVEL) check(sable_spts_tune(g_ctx, R.a, nullptr, nullptr, nullptr, nullptr), "spts tune");
         R.v[0] = mp.version;
         R.v[1] = mi.version;
         R.v[2] = mv.version;
@@ -1231,7 +1237,7 @@ napi_value level_solve(napi_env env, Prec* P, const Args& a, size_t lp_at, size_
     if (a.id() != 0 || nl <= 0 || im <= 0 || N <= 0) return undef(env);
     ensure_ctx();
     (void)host_i32(a.i(lp_at), (size_t)nl + 1);
-    ResidentSpts& R = spts_resident(P, KIND_SPTS_CSR, a.i(row_at), a.i(row_at + 1), a.i(row_at + 2), N);
+    ResidentSpts& R = spts_resident(P, KIND_SPTS_LEVEL, a.i(row_at), a.i(row_at + 1), a.i(row_at + 2), N);
     void* db = vec_in(0, a.i(b_at), P->vs * N);
     void* dy = vec(1, P->vs * N);
     for (int it = 0; it < im; it++) check(sable_spts_solve_level(g_ctx, R.a, db, dy), "spts_level_csr");
