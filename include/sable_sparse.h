@@ -916,7 +916,8 @@ SABLE_API int sable_server_info(sable_server_t* s, int64_t* hbm_used, int32_t* n
  * built from, exported for callers that assemble their own formats. */
 /* out[0..n) = exclusive prefix sum of in[0..n); out[n] = total (out has n+1). */
 SABLE_API int sable_scan_exclusive_i32(sable_ctx_t* ctx, const int32_t* in, int32_t* out, int64_t n);
-/* Stable sort of (keys, vals) by key bits [0, end_bit), in place. */
+/* Stable sort of (keys, vals) by key bits [0, end_bit), in place; key bits
+ * from end_bit up travel with their keys unsorted.  At most 2^31 - 1 keys. */
 SABLE_API int sable_sort_pairs_u64(sable_ctx_t* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int end_bit);
 
 #ifdef __cplusplus
