@@ -55,6 +55,15 @@ int main() {
     base[7 * P] = 9;
     CHECK(faults() == f1 + 3);
 
+    // unprotect without counting (the owner revalidates its own mirror):
+    // the pages become writable, no count moves, later writes land unseen
+    protect_pages(10, 13);
+    const uint64_t f3 = faults(), s3 = wsum(10, 13);
+    unprotect_range(10 * P, 3 * P, false);
+    CHECK(faults() == f3 && wsum(10, 13) == s3 && g_prot[10] == 0 && g_prot[12] == 0);
+    base[11 * P + 5] = 77;
+    CHECK(faults() == f3 && base[11 * P + 5] == 77);
+
     // many threads writing every page at once: each page counted exactly once
     protect_pages(0, NP);
     const uint64_t f2 = faults(), s2 = wsum(0, NP);
