@@ -423,6 +423,17 @@ def sec_zipf16(args, sp, ctx, T):
                                               "rows_checked": R,
                                               "within_csr_bound": bool(np.all(np.abs(y - yref) <= bnd))}
         del tl
+    # the schedule the reference's spmv_csr gets through the addon
+    sp.CsrAuto.clear_cache()
+    auto = sp.CsrAuto(A)
+    ai = auto.info()
+    ms = T(lambda: auto.run(xv.vec, yv.vec, beta=0.0), args.steps, args.warmup)
+    y = yv.array()[:R]
+    gbs = bytes_step / (ms * 1e-3) / 1e9
+    out["autotuned"] = {"schedule": ai["schedule"], "param0": ai["param0"], "param1": ai["param1"], "ms": r4(ms),
+                        "gbs": r4(gbs), "pct_hbm": r4(100 * gbs / HBM_PEAK_GBS), "rows_checked": R,
+                        "within_csr_bound": bool(np.all(np.abs(y - yref) <= bnd))}
+    del auto
     log("zipf16:", out)
     return out
 
