@@ -16,8 +16,9 @@ so later matrices of the same class reuse the choice without measuring:
   decides which rows take the whole-workgroup long-row path;
 * column panels for matrices whose x does not fit one XCD's L2;
 * column-sorted tiles (allow_tiles): 16384-row tiles over 8192-column
-  panels when x does not fit one XCD's L2, and single-panel tiles of up to
-  4096 rows otherwise.  Tiles sum each row in LDS-atomic order, so a matrix
+  panels when x does not fit one XCD's L2 (and over 16384-column panels
+  when a tile holds fewer than 1024 entries per 8192-column panel), and
+  single-panel tiles of up to 4096 rows otherwise.  Tiles sum each row in LDS-atomic order, so a matrix
   tuned to them gives results within the CSR bound but not bit-identical
   from run to run; allow_tiles=False keeps every candidate deterministic.
 
@@ -74,6 +75,11 @@ class Autotuner:
             out.append({"format": "panels", "block_weight": 1024, "row_weight": 2})
             if self.allow_tiles:
                 out.append({"format": "tiles", "tile_rows": 16384, "panel_cols": 8192})
+                # few entries per (tile, panel): twice-wider panels as well
+                # (the same rule as the C autotuner, csrc/auto.hip)
+                nnz = int(row_ptr[-1]) if N else 0
+                if nnz / math.ceil(N / 16384) * (8192 / max(N, 1)) < 1024:
+                    out.append({"format": "tiles", "tile_rows": 16384, "panel_cols": 16384})
         elif self.allow_tiles and N > 0:
             pc = 64
             while pc < N:
