@@ -201,21 +201,21 @@ def headline(args, sp, ctx, T):
         import oracle
         R = N // 4
         nnz_s = int(rp[R])
-        t, ycpu = oracle.time_spmv_csr_f64(R, rp, col, val, x, warm=10, reps=30)
+        t, ycpu = oracle.time_spmv_csr_f64(R, rp, col, val, x, warm=10, reps=60)
         bytes_s = 4 * (R + 1) + 12 * nnz_s + 8 * N + 8 * R
         serial = {"value": round(bytes_s / t / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
                   "sample": "serial C restatement of seq/src/sparse_opt_64.wat spmv_csr on rows [0, N/4) "
-                            "(%d rows, %d nnz) of the same matrix, 10 warm-up + 30 timed calls, %.4f s/call"
+                            "(%d rows, %d nnz) of the same matrix, 10 warm-up + 60 timed calls, %.4f s/call"
                             % (R, nnz_s, t)}
         try:
             W = len(os.sched_getaffinity(0))
         except AttributeError:
             W = os.cpu_count() or 1
         W = max(1, min(W, 64))
-        tw, yw = oracle.time_spmv_csr_f64_mt(N, rp, col, val, x, W, warm=3, reps=10)
+        tw, yw = oracle.time_spmv_csr_f64_mt(N, rp, col, val, x, W, warm=3, reps=60)
         cpu = {"value": round(bytes_step / tw / 1e9, 3), "unit": "GB/s", "cores": W, "kind": "port",
                "sample": "the same C restatement on %d persistent threads over static_nnz row ranges "
-                         "(par/src/sparse32.js:2769-2821), whole matrix (%d rows, %d nnz), 3 warm-up + 10 "
+                         "(par/src/sparse32.js:2769-2821), whole matrix (%d rows, %d nnz), 3 warm-up + 60 "
                          "timed calls, %.4f s/call; results identical to the serial restatement: %s"
                          % (W, N, nnz, tw, bool(np.array_equal(yw[:R], ycpu))),
                "serial": serial}
