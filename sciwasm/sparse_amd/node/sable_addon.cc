@@ -377,6 +377,14 @@ Mirror* product_mirror(size_t off, size_t bytes) {
     return &mirror_rec(off, bytes, true);
 }
 
+// A solve's right-hand side: mirrored like x of the products (uploaded
+// only when the host wrote it since the last call), unless it overlaps the
+// region the solution is written to.
+void* rhs_in(size_t b, size_t bb, size_t y, size_t yb) {
+    Mirror* m = (!resident_vectors() || overlaps(b, bb, y, yb)) ? nullptr : product_mirror(b, bb);
+    return m ? m->d : vec_in(0, b, bb);
+}
+
 struct ProductXY {
     void* dx = nullptr;
     void* dy = nullptr;
@@ -1015,7 +1023,7 @@ napi_value spts_csc_impl(napi_env env, napi_callback_info info, bool wrapper) {
         const int im = wrapper ? (int)a.i(6) : 1;
         if (N <= 0 || im <= 0) return undef(env);
         ResidentSpts& R = spts_resident(P, KIND_SPTS_CSC, a.i(0), a.i(1), a.i(2), N);
-        void* dx = vec_in(0, a.i(3), P->vs * N);
+        void* dx = rhs_in(a.i(3), P->vs * N, a.i(4), P->vs * N);
         void* dy = vec(1, P->vs * N);
         for (int it = 0; it < im; it++) check(sable_spts_solve_syncfree(g_ctx, R.a, dx, dy), "spts_csc");
         check(sable_spts_status(g_ctx, R.a), "spts_csc");
@@ -1238,7 +1246,7 @@ napi_value level_solve(napi_env env, Prec* P, const Args& a, size_t lp_at, size_
     ensure_ctx();
     (void)host_i32(a.i(lp_at), (size_t)nl + 1);
     ResidentSpts& R = spts_resident(P, KIND_SPTS_LEVEL, a.i(row_at), a.i(row_at + 1), a.i(row_at + 2), N);
-    void* db = vec_in(0, a.i(b_at), P->vs * N);
+    void* db = rhs_in(a.i(b_at), P->vs * N, a.i(y_at), P->vs * N);
     void* dy = vec(1, P->vs * N);
     for (int it = 0; it < im; it++) check(sable_spts_solve_level(g_ctx, R.a, db, dy), "spts_level_csr");
     download(a.i(y_at), dy, P->vs * N);
@@ -1300,7 +1308,7 @@ napi_value spts_csr_sync_free_wrapper(napi_env env, napi_callback_info info) {
         const int im = (int)a.i(10);
         if (im <= 0 || N <= 0) return undef(env);
         ResidentSpts& R = spts_resident(P, KIND_SPTS_CSR, a.i(0), a.i(1), a.i(2), N);
-        void* db = vec_in(0, a.i(3), P->vs * N);
+        void* db = rhs_in(a.i(3), P->vs * N, a.i(4), P->vs * N);
         void* dy = vec(1, P->vs * N);
         for (int it = 0; it < im; it++) check(sable_spts_solve_syncfree(g_ctx, R.a, db, dy), "spts_csr_sync_free");
         check(sable_spts_status(g_ctx, R.a), "spts_csr_sync_free");

@@ -382,6 +382,18 @@ if (spec.spts) {
   for (let id = 0; id < W; id++)
     P.spts_csr_sync_free_wrapper(id, L.row.off, L.col.off, L.val.off, b.off, y1, S.N, barrier, flag, aflag, W, 1);
   save('spts_syncfree', y1, S.N, FArray);
+  // the right-hand side stays on the device between solves; a host write
+  // to it is seen by the next solve
+  {
+    const bv = new FArray(memory.buffer, b.off, S.N);
+    const keep = bv[5];
+    bv[5] = 0.75;
+    const y1b = alloc(S.N, FArray);
+    for (let id = 0; id < W; id++)
+      P.spts_csr_sync_free_wrapper(id, L.row.off, L.col.off, L.val.off, b.off, y1b, S.N, barrier, flag, aflag, W, 1);
+    save('spts_syncfree_bmod', y1b, S.N, FArray);
+    bv[5] = keep;
+  }
 
   const Lr = { lp: load_array('Lr_level_ptr', Int32Array), perm: load_array('Lr_perm', Int32Array),
                row: load_array('Lr_row', Int32Array), col: load_array('Lr_col', Int32Array),

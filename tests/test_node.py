@@ -305,6 +305,10 @@ def test_node_flow_matches_oracle(tmp_path, dtype, resident_vectors):
     n = spec["spts"]["N"]
     ys = oracle.spts_syncfree(n, a["L_row"], a["L_col"], a["L_val"], a["L_b"])
     assert np.array_equal(read_out(d, "spts_syncfree", dtype), ys)
+    bmod = a["L_b"].copy()
+    bmod[5] = 0.75
+    ysm = oracle.spts_syncfree(n, a["L_row"], a["L_col"], a["L_val"], bmod)
+    assert np.array_equal(read_out(d, "spts_syncfree_bmod", dtype), ysm)
     yl = oracle.spts_level(n, a["Lr_row"], a["Lr_col"], a["Lr_val"], a["Lr_b"])
     assert np.array_equal(read_out(d, "spts_level", dtype), yl)
     assert np.array_equal(read_out(d, "spts_combined", dtype), yl)
