@@ -324,9 +324,11 @@ def sec_addon(args, sp, ctx, host):
     x and y resident while the caller leaves them unchanged (as in the
     reference's own timing loop) and y brought back after every call, on C1
     (FP32 Laplacian 256^2, inner_max 1 and 500) and on the headline matrix
-    (FP64, inner_max 1 and 100)."""
+    (FP64, inner_max 1 and 100); and spts_csr_sync_free_wrapper on the two
+    C3 lower matrices (FP64, b resident, the solution brought back)."""
     import tempfile
     import shutil
+    import subprocess
     import oracle
     from sciwasm.sparse_amd import gen
     names = {-1: "none", 0: "row blocks", 1: "row vector", 2: "column-sorted tiles"}
@@ -385,6 +387,37 @@ def sec_addon(args, sp, ctx, host):
                                                             r["runs"][str(im)]["vector_uploads"]),
                           "rows_checked": N // 4, "within_csr_bound": _row_check(rp, col, val, x, y, N // 4)}
             log("addon c2a:", out["c2a"])
+        # the reference's sync-free lower solve (spts_csr_sync_free_wrapper)
+        # on the two C3 matrices, FP64, bit-exact against the oracle's
+        # sync-free order; CPU: the serial restatement of the same solve
+        for name, make in (("c3a_lap3d_128", lambda: gen.laplacian_3d_lower(128)),
+                           ("c3b_random_lower_1M", lambda: gen.random_lower(1 << 20, 32, seed=7))):
+            Lrp, Lc, Lv = make()
+            n = len(Lrp) - 1
+            b = np.random.default_rng(3).uniform(-1, 1, n)
+            d = os.path.join(base, name)
+            os.makedirs(d)
+            for fname, arr in (("rp", Lrp), ("col", Lc), ("val", Lv), ("b", b)):
+                np.ascontiguousarray(arr).tofile(os.path.join(d, fname + ".bin"))
+            with open(os.path.join(d, "spec.json"), "w") as f:
+                json.dump({"N": n, "nnz": len(Lc)}, f)
+            lin = int((4 * (n + 1) + 12 * len(Lc) + 24 * n) / (1 << 20)) + 128
+            r = subprocess.run([shutil.which("node"), os.path.join(ROOT, "sciwasm", "sparse_amd", "node",
+                                                                   "bench_node_spts.js"), d, "64", "3", "20"],
+                               capture_output=True, text=True, timeout=600,
+                               env=dict(os.environ, SABLE_LINEAR_MEMORY_MB=str(lin)))
+            if r.returncode != 0:
+                raise RuntimeError("bench_node_spts.js failed: " + (r.stdout[-2000:] + r.stderr[-2000:]))
+            rs = json.loads(r.stdout.strip().splitlines()[-1])
+            y = np.fromfile(os.path.join(d, "y.bin"), dtype=np.float64)
+            t0 = time.perf_counter()
+            ys = oracle.spts_syncfree(n, Lrp, Lc, Lv, b)
+            cpu_ms = (time.perf_counter() - t0) * 1e3
+            out[name] = {"N": n, "nnz": len(Lc), "first_call_ms": r4(rs["first_call_ms"]),
+                         "call_ms": r4(rs["median_ms"]), "uploads_in_timed_calls": rs["uploads"],
+                         "vector_uploads_in_timed_calls": rs["vector_uploads"],
+                         "cpu_serial_ms": r4(cpu_ms), "bit_exact_vs_oracle_syncfree": bool(np.array_equal(y, ys))}
+            log("addon %s:" % name, out[name])
     finally:
         shutil.rmtree(base, ignore_errors=True)
     return out
