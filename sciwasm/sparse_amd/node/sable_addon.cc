@@ -976,6 +976,7 @@ napi_value coo_csc(napi_env env, napi_callback_info info) {
 struct ResidentSpts : ResidentBase {
     sable_spts_t* a = nullptr;
     uint64_t v[3] = {0, 0, 0};
+    bool sf_tuned = false;  // the sync-free schedule has been timed
     ~ResidentSpts() override {
         if (a) sable_spts_destroy(a);
     }
@@ -984,7 +985,7 @@ struct ResidentSpts : ResidentBase {
 // KIND_SPTS_LEVEL: the level-set exports' analysis, whose schedule is tuned
 // once (sable_spts_tune: the fastest of the row-block and level-graph
 // schedules, all of which give the level solve's bits); the sync-free
-// export keeps its own analysis on the default schedule.
+// export keeps its own analysis, its schedule timed once (tune_syncfree).
 enum { KIND_SPTS_CSR = 32, KIND_SPTS_CSC = 48, KIND_SPTS_LEVEL = 64 };
 
 ResidentSpts& spts_resident(Prec* P, int kind, size_t p, size_t i, size_t v, int32_t N) {
@@ -998,6 +999,7 @@ ResidentSpts& spts_resident(Prec* P, int kind, size_t p, size_t i, size_t v, int
     if (!R.a || R.v[0] != mp.version || R.v[1] != mi.version || R.v[2] != mv.version) {
         if (R.a) sable_spts_destroy(R.a);
         R.a = nullptr;
+        R.sf_tuned = false;
         if (kind == KIND_SPTS_CSC)
             check(sable_spts_csc_analyze(g_ctx, P->dt, N, (const int32_t*)mp.d, (const int32_t*)mi.d, mv.d, 1, &R.a),
                   "spts_csc");
@@ -1296,6 +1298,55 @@ napi_value metadata(napi_env env, napi_callback_info info) {
     });
 }
 
+// The sync-free export's schedule, timed once per resident matrix on its
+// first call: row blocks of 2048, 8192 and 16384 rows (at most twice N;
+// each size costs a rebuild of the block layout, so the set is kept short)
+// and the natural-order sweep; every one gives the reference sync-free
+// solve's bits.
+void tune_syncfree(ResidentSpts& R, const void* db, void* dy, int32_t N) {
+    struct C {
+        int s;
+        int32_t br;
+    };
+    std::vector<C> cands;
+    for (int64_t br : {2048, 8192, 16384})
+        if (br <= 2 * (int64_t)N || br == 2048) cands.push_back({SABLE_SPTS_BLOCKS, (int32_t)br});
+    cands.push_back({SABLE_SPTS_LEVEL_GRAPH, 0});
+    void *e0 = nullptr, *e1 = nullptr;
+    check(sable_event_create(&e0), "event");
+    if (sable_event_create(&e1) != SABLE_OK) {
+        sable_event_destroy(e0);
+        throw Err{"event"};
+    }
+    C pick = cands[0];
+    float best = 0.f;
+    try {
+        for (const C& c : cands) {
+            check(sable_spts_set_schedule(R.a, c.s, c.br), "spts schedule");
+            check(sable_spts_solve_syncfree(g_ctx, R.a, db, dy), "spts schedule timing");
+            check(sable_event_record(g_ctx, e0), "event");
+            for (int r = 0; r < 3; r++) check(sable_spts_solve_syncfree(g_ctx, R.a, db, dy), "spts schedule timing");
+            check(sable_event_record(g_ctx, e1), "event");
+            check(sable_ctx_sync(g_ctx), "spts schedule timing");
+            float ms = 0.f;
+            check(sable_event_elapsed_ms(e0, e1, &ms), "event");
+            if (&c == &cands[0] || ms < best) {
+                best = ms;
+                pick = c;
+            }
+        }
+        check(sable_spts_status(g_ctx, R.a), "spts schedule timing");
+        check(sable_spts_set_schedule(R.a, pick.s, pick.br), "spts schedule");
+    } catch (...) {
+        sable_event_destroy(e0);
+        sable_event_destroy(e1);
+        throw;
+    }
+    sable_event_destroy(e0);
+    sable_event_destroy(e1);
+    R.sf_tuned = true;
+}
+
 // spts_csr_sync_free_wrapper(id, rowptr, col, val, b, y, N, barrier,
 //                            global_flag, flag, nthreads, inner_max)
 napi_value spts_csr_sync_free_wrapper(napi_env env, napi_callback_info info) {
@@ -1310,6 +1361,7 @@ napi_value spts_csr_sync_free_wrapper(napi_env env, napi_callback_info info) {
         ResidentSpts& R = spts_resident(P, KIND_SPTS_CSR, a.i(0), a.i(1), a.i(2), N);
         void* db = rhs_in(a.i(3), P->vs * N, a.i(4), P->vs * N);
         void* dy = vec(1, P->vs * N);
+        if (!R.sf_tuned) tune_syncfree(R, db, dy, N);
         for (int it = 0; it < im; it++) check(sable_spts_solve_syncfree(g_ctx, R.a, db, dy), "spts_csr_sync_free");
         check(sable_spts_status(g_ctx, R.a), "spts_csr_sync_free");
         download(a.i(4), dy, P->vs * N);
