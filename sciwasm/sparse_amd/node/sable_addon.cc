@@ -1014,6 +1014,55 @@ ResidentSpts& spts_resident(Prec* P, int kind, size_t p, size_t i, size_t v, int
     return R;
 }
 
+// The sync-free exports' schedule (spts_csr_sync_free_wrapper, spts_csc),
+// timed once per resident matrix on its first call: row blocks of 2048,
+// 8192 and 16384 rows (at most twice N; each size costs a rebuild of the
+// block layout, so the set is kept short) and the natural-order sweep;
+// every one gives the reference sync-free solve's bits.
+void tune_syncfree(ResidentSpts& R, const void* db, void* dy, int32_t N) {
+    struct C {
+        int s;
+        int32_t br;
+    };
+    std::vector<C> cands;
+    for (int64_t br : {2048, 8192, 16384})
+        if (br <= 2 * (int64_t)N || br == 2048) cands.push_back({SABLE_SPTS_BLOCKS, (int32_t)br});
+    cands.push_back({SABLE_SPTS_LEVEL_GRAPH, 0});
+    void *e0 = nullptr, *e1 = nullptr;
+    check(sable_event_create(&e0), "event");
+    if (sable_event_create(&e1) != SABLE_OK) {
+        sable_event_destroy(e0);
+        throw Err{"event"};
+    }
+    C pick = cands[0];
+    float best = 0.f;
+    try {
+        for (const C& c : cands) {
+            check(sable_spts_set_schedule(R.a, c.s, c.br), "spts schedule");
+            check(sable_spts_solve_syncfree(g_ctx, R.a, db, dy), "spts schedule timing");
+            check(sable_event_record(g_ctx, e0), "event");
+            for (int r = 0; r < 3; r++) check(sable_spts_solve_syncfree(g_ctx, R.a, db, dy), "spts schedule timing");
+            check(sable_event_record(g_ctx, e1), "event");
+            check(sable_ctx_sync(g_ctx), "spts schedule timing");
+            float ms = 0.f;
+            check(sable_event_elapsed_ms(e0, e1, &ms), "event");
+            if (&c == &cands[0] || ms < best) {
+                best = ms;
+                pick = c;
+            }
+        }
+        check(sable_spts_status(g_ctx, R.a), "spts schedule timing");
+        check(sable_spts_set_schedule(R.a, pick.s, pick.br), "spts schedule");
+    } catch (...) {
+        sable_event_destroy(e0);
+        sable_event_destroy(e1);
+        throw;
+    }
+    sable_event_destroy(e0);
+    sable_event_destroy(e1);
+    R.sf_tuned = true;
+}
+
 // spts_csc(col_ptr, row, val, x, y, N[, inner_max]): y = L^-1 x, L lower in
 // CSC with the diagonal first in each column
 napi_value spts_csc_impl(napi_env env, napi_callback_info info, bool wrapper) {
@@ -1027,6 +1076,7 @@ napi_value spts_csc_impl(napi_env env, napi_callback_info info, bool wrapper) {
         ResidentSpts& R = spts_resident(P, KIND_SPTS_CSC, a.i(0), a.i(1), a.i(2), N);
         void* dx = rhs_in(a.i(3), P->vs * N, a.i(4), P->vs * N);
         void* dy = vec(1, P->vs * N);
+        if (!R.sf_tuned) tune_syncfree(R, dx, dy, N);
         for (int it = 0; it < im; it++) check(sable_spts_solve_syncfree(g_ctx, R.a, dx, dy), "spts_csc");
         check(sable_spts_status(g_ctx, R.a), "spts_csc");
         download(a.i(4), dy, P->vs * N);
@@ -1296,55 +1346,6 @@ napi_value metadata(napi_env env, napi_callback_info info) {
         (void)get_args(env, info, 6);
         return undef(env);
     });
-}
-
-// The sync-free export's schedule, timed once per resident matrix on its
-// first call: row blocks of 2048, 8192 and 16384 rows (at most twice N;
-// each size costs a rebuild of the block layout, so the set is kept short)
-// and the natural-order sweep; every one gives the reference sync-free
-// solve's bits.
-void tune_syncfree(ResidentSpts& R, const void* db, void* dy, int32_t N) {
-    struct C {
-        int s;
-        int32_t br;
-    };
-    std::vector<C> cands;
-    for (int64_t br : {2048, 8192, 16384})
-        if (br <= 2 * (int64_t)N || br == 2048) cands.push_back({SABLE_SPTS_BLOCKS, (int32_t)br});
-    cands.push_back({SABLE_SPTS_LEVEL_GRAPH, 0});
-    void *e0 = nullptr, *e1 = nullptr;
-    check(sable_event_create(&e0), "event");
-    if (sable_event_create(&e1) != SABLE_OK) {
-        sable_event_destroy(e0);
-        throw Err{"event"};
-    }
-    C pick = cands[0];
-    float best = 0.f;
-    try {
-        for (const C& c : cands) {
-            check(sable_spts_set_schedule(R.a, c.s, c.br), "spts schedule");
-            check(sable_spts_solve_syncfree(g_ctx, R.a, db, dy), "spts schedule timing");
-            check(sable_event_record(g_ctx, e0), "event");
-            for (int r = 0; r < 3; r++) check(sable_spts_solve_syncfree(g_ctx, R.a, db, dy), "spts schedule timing");
-            check(sable_event_record(g_ctx, e1), "event");
-            check(sable_ctx_sync(g_ctx), "spts schedule timing");
-            float ms = 0.f;
-            check(sable_event_elapsed_ms(e0, e1, &ms), "event");
-            if (&c == &cands[0] || ms < best) {
-                best = ms;
-                pick = c;
-            }
-        }
-        check(sable_spts_status(g_ctx, R.a), "spts schedule timing");
-        check(sable_spts_set_schedule(R.a, pick.s, pick.br), "spts schedule");
-    } catch (...) {
-        sable_event_destroy(e0);
-        sable_event_destroy(e1);
-        throw;
-    }
-    sable_event_destroy(e0);
-    sable_event_destroy(e1);
-    R.sf_tuned = true;
 }
 
 // spts_csr_sync_free_wrapper(id, rowptr, col, val, b, y, N, barrier,
